@@ -1,0 +1,174 @@
+"""Seeded synthetic ice geometries for the velocity solve.
+
+Every field is generated here from a seed; nothing is read from disk.  A
+:class:`Problem` holds the raw model state that ``glissade_velo_higher_solve``
+receives (thickness, bed, flow factor, basal friction, velocity guess, in the
+solver's yr-based units, glissade_velo_higher.F90:3922-3978) plus the options
+that select the BP path.  Cell (i, j) (1-based, halos included) is centred
+at ``((i - 2.5) dx, (j - 2.5) dy)``; vertex (i, j) is its NE corner.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .grid import BC_NO_ICE, BC_PERIODIC, Grid, glide_sigma
+
+# which_ho_babc values (glide_types.F90:253-268)
+HO_BABC_BETA_CONSTANT = 0
+HO_BABC_BETA_LARGE = 4
+HO_BABC_BETA_EXTERNAL = 5
+HO_BABC_NO_SLIP = 6
+
+
+@dataclass
+class Problem:
+    grid: Grid
+    dx: float
+    dy: float
+    sigma: np.ndarray            # (nz,)
+    thck: np.ndarray             # (nx, ny) m
+    topg: np.ndarray             # (nx, ny) m
+    flwa: np.ndarray             # (nz-1, nx, ny) Pa^-n yr^-1
+    beta: np.ndarray             # (nx-1, ny-1) Pa yr/m
+    uvel: np.ndarray             # (nz, nx-1, ny-1) m/yr
+    vvel: np.ndarray
+    eus: float = 0.0
+    thklim: float = 100.0        # glide_types.F90:2426
+    max_slope: float = 1.0       # glide_types.F90:2569
+    gradient_margin: int = 1     # which_ho_gradient_margin default (glide_types.F90:1021)
+    thck_gradient_ramp: float = 0.0
+    whichbabc: int = HO_BABC_BETA_EXTERNAL
+    beta_const: float = 0.0
+    n_glen: float = 3.0
+    effstrain_min: float = 1.0e-8   # glide_types.F90:2567
+    whichefvs: int = 2
+    efvs_constant: float = 0.0
+    meta: dict = field(default_factory=dict)
+
+    def cell_centres(self):
+        g = self.grid
+        xc = (np.arange(1, g.nx + 1) - 2.5) * self.dx
+        yc = (np.arange(1, g.ny + 1) - 2.5) * self.dy
+        return xc, yc
+
+    def vertex_coords(self):
+        g = self.grid
+        x0 = (np.arange(1, g.nx) - 2.0) * self.dx
+        y0 = (np.arange(1, g.ny) - 2.0) * self.dy
+        xV = np.asfortranarray(np.repeat(x0[:, None], g.ny - 1, axis=1))
+        yV = np.asfortranarray(np.repeat(y0[None, :], g.nx - 1, axis=0))
+        return xV, yV
+
+
+def _smooth_field(rng, shape, corr_cells: float) -> np.ndarray:
+    """Unit-variance Gaussian random field with a Gaussian spectrum (FFT filter)."""
+    white = rng.standard_normal(shape)
+    kx = np.fft.fftfreq(shape[0])[:, None]
+    ky = np.fft.rfftfreq(shape[1])[None, :]
+    filt = np.exp(-0.5 * (2 * np.pi * corr_cells) ** 2 * (kx ** 2 + ky ** 2))
+    f = np.fft.irfft2(np.fft.rfft2(white) * filt, s=shape)
+    f -= f.mean()
+    s = f.std()
+    return f / s if s > 0 else f
+
+
+def dome(n_cells: int = 31, nz: int = 10, dx: float = 2000.0, radius: float = 30.0e3,
+         h0: float = 2000.0, flwa: float = 1.0e-16) -> Problem:
+    """Parabolic dome, H(r) = h0 sqrt(1 - (r/R)^2) on a flat bed, no-slip base."""
+    g = Grid.from_owned(n_cells, n_cells, nz, global_bc=BC_NO_ICE)
+    xc = (np.arange(1, g.nx + 1) - 2.5) * dx
+    yc = (np.arange(1, g.ny + 1) - 2.5) * dx
+    xm = 0.5 * n_cells * dx
+    r2 = ((xc[:, None] - xm) ** 2 + (yc[None, :] - xm) ** 2) / radius ** 2
+    thck = np.where(r2 < 1.0, h0 * np.sqrt(np.clip(1.0 - r2, 0.0, None)), 0.0)
+    p = Problem(
+        grid=g, dx=dx, dy=dx, sigma=glide_sigma(nz),
+        thck=np.asfortranarray(thck), topg=np.zeros((g.nx, g.ny), order="F"),
+        flwa=np.full((nz - 1, g.nx, g.ny), flwa, order="F"),
+        beta=np.zeros((g.nx - 1, g.ny - 1), order="F"),
+        uvel=g.zeros_vec(), vvel=g.zeros_vec(),
+        whichbabc=HO_BABC_NO_SLIP,
+        meta=dict(workload="dome", n_cells=n_cells, nz=nz, dx=dx, radius=radius, h0=h0, flwa=flwa),
+    )
+    return p
+
+
+def ice_stream(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0,
+               flwa: float = 1.0e-16, h_mean: float = 1000.0, h_amp: float = 200.0,
+               wavelength: float = 20.0e3, bed_slope: float = 0.002, bump_sigma: float = 50.0,
+               bump_corr: float = 5.0e3, beta_median: float = 1.0e3, beta_sigma_log: float = 1.0,
+               beta_corr: float = 5.0e3, u_surf: float = 150.0) -> Problem:
+    """Synthetic ice stream on an ``n_vert x n_vert`` owned vertex grid.
+
+    Thickness ``h_mean + h_amp sin(2 pi x / wavelength)``; bed sloping down in
+    +x at ``bed_slope`` plus seeded Gaussian bumps (std ``bump_sigma``,
+    correlation length ``bump_corr``) on land; log-normal beta (median
+    ``beta_median``, sigma_log ``beta_sigma_log``); the outermost owned cell
+    ring and the halos are ice-free so that every halo value is zero.  The
+    velocity guess is a plug-plus-shear profile that sets the viscosity of
+    the single assembled system.
+    """
+    rng = np.random.default_rng(seed)
+    g = Grid.from_owned(n_vert, n_vert, nz, global_bc=BC_NO_ICE)
+    nx, ny = g.nx, g.ny
+    xc = (np.arange(1, nx + 1) - 2.5) * dx
+    yc = (np.arange(1, ny + 1) - 2.5) * dx
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    thck = h_mean + h_amp * np.sin(2 * np.pi * X / wavelength)
+    ring = np.zeros((nx, ny), bool)
+    ring[:3, :] = ring[-3:, :] = True
+    ring[:, :3] = ring[:, -3:] = True
+    thck[ring] = 0.0
+    bumps = bump_sigma * _smooth_field(rng, (nx, ny), bump_corr / dx)
+    topg = 2000.0 - bed_slope * X + bumps
+    lb = _smooth_field(rng, (nx - 1, ny - 1), beta_corr / dx)
+    beta = beta_median * np.exp(beta_sigma_log * lb)
+    # velocity guess: basal sliding + internal shear, varying across the stream
+    sig = glide_sigma(nz)
+    xv = (np.arange(1, nx) - 2.0) * dx
+    yv = (np.arange(1, ny) - 2.0) * dx
+    XV, YV = np.meshgrid(xv, yv, indexing="ij")
+    Ly = max(yv[-1] - yv[0], dx)
+    ub = 0.5 * u_surf * (1.0 + 0.5 * np.sin(np.pi * (YV - yv[0]) / Ly)) * (1.0 + 0.1 * np.cos(2 * np.pi * XV / wavelength))
+    us = 0.5 * u_surf * (1.0 + 0.3 * np.cos(2 * np.pi * XV / (1.7 * wavelength)))
+    shape = 1.0 - sig[:, None, None] ** 4          # 1 at the surface, 0 at the bed
+    uvel = ub[None] + us[None] * shape
+    vvel = 0.1 * uvel * np.sin(2 * np.pi * (YV[None] + 0.3 * XV[None]) / (2.3 * wavelength))
+    p = Problem(
+        grid=g, dx=dx, dy=dx, sigma=sig,
+        thck=np.asfortranarray(thck), topg=np.asfortranarray(topg),
+        flwa=np.full((nz - 1, nx, ny), flwa, order="F"),
+        beta=np.asfortranarray(beta),
+        uvel=np.asfortranarray(uvel), vvel=np.asfortranarray(vvel),
+        whichbabc=HO_BABC_BETA_EXTERNAL,
+        meta=dict(workload="ice_stream", n_vert=n_vert, nz=nz, dx=dx, seed=seed, flwa=flwa,
+                  h_mean=h_mean, h_amp=h_amp, wavelength=wavelength, bed_slope=bed_slope,
+                  bump_sigma=bump_sigma, bump_corr=bump_corr, beta_median=beta_median,
+                  beta_sigma_log=beta_sigma_log, beta_corr=beta_corr),
+    )
+    return p
+
+
+def slab(n_cells: int = 8, nz: int = 21, dx: float = 1000.0, thck: float = 1000.0,
+         theta_deg: float = 0.5, beta: float = 2000.0, flwa: float = 1.0e-16) -> Problem:
+    """Doubly periodic inclined slab (tests/slab): uniform thickness on a bed
+    tilted by ``theta`` in x, linear sliding with uniform beta.  Periodic BCs
+    wrap the vertex grid, so the surface is shifted by the slab drop across
+    one period through the cell elevations only (the solver sees the slope
+    through the surface gradient and the node z coordinates)."""
+    g = Grid.from_owned(n_cells, n_cells, nz, global_bc=BC_PERIODIC)
+    xc = (np.arange(1, g.nx + 1) - 2.5) * dx
+    slope = np.tan(np.deg2rad(theta_deg))
+    topg = np.asfortranarray(np.repeat((-slope * xc)[:, None], g.ny, axis=1) + 5000.0)
+    p = Problem(
+        grid=g, dx=dx, dy=dx, sigma=np.linspace(0.0, 1.0, nz),
+        thck=np.full((g.nx, g.ny), thck, order="F"), topg=topg,
+        flwa=np.full((nz - 1, g.nx, g.ny), flwa, order="F"),
+        beta=np.full((g.nx - 1, g.ny - 1), beta, order="F"),
+        uvel=g.zeros_vec(), vvel=g.zeros_vec(), whichbabc=HO_BABC_BETA_EXTERNAL,
+        gradient_margin=0,
+        meta=dict(workload="slab", theta_deg=theta_deg, thck=thck, beta=beta, flwa=flwa),
+    )
+    return p

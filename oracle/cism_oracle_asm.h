@@ -1,0 +1,58 @@
+/*
+ * cism_oracle_asm.h -- assembly-side entry points of the CPU restatement
+ * (oracle_assembly.c, oracle_picard.c).  TEST INFRASTRUCTURE ONLY.
+ * Layouts as documented in cism_oracle.h.
+ */
+#ifndef CISM_ORACLE_ASM_H
+#define CISM_ORACLE_ASM_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void orc_fe_tables(double* phi, double* dxr, double* dyr, double* dzr);
+void orc_calclsrf(int nx, int ny, const double* thck, const double* topg, double eus, double* lsrf);
+void orc_get_masks(int nx, int ny, const double* thck, const double* topg, double eus, double thklim,
+                   int32_t* ice_mask, int32_t* floating_mask, int32_t* ocean_mask, int32_t* land_mask);
+void orc_stagger(int nx, int ny, const double* var, double* stagvar);
+void orc_surface_elevation_gradient(int nx, int ny, double dx, double dy,
+                                    const int32_t* ice_mask, const int32_t* land_mask,
+                                    const double* usrf, const double* thck,
+                                    double thklim, double thck_gradient_ramp, int margin,
+                                    double max_slope, double* ds_dx, double* ds_dy);
+void orc_slope_angle(int nx, int ny, double dx, double dy, const double* zsfc, double* theta);
+void orc_vertex_masks(int nx, int ny, int nhalo, const int32_t* ice_mask,
+                      int32_t* active_cell, int32_t* active_vertex);
+void orc_flwafact(int nx, int ny, int nz, int nhalo, double n_glen, const double* flwa, double* flwafact);
+void orc_load_vector_gravity_local(int nx, int ny, int nz, const double* stagwbndsigma,
+                                   double dx, double dy, const int32_t* active_vertex,
+                                   const double* stagthck, const double* dusrf_dx, const double* dusrf_dy,
+                                   double* loadu, double* loadv);
+int orc_element_matrix(const double* x, const double* y, const double* z, const double* u, const double* v,
+                       double flwafact, int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                       double Kuu[8][8], double Kuv[8][8], double Kvu[8][8], double Kvv[8][8], double* efvs_mean);
+int orc_assemble_stiffness_matrix_3d(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                     const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                     const double* uvel, const double* vvel,
+                                     const double* stagusrf, const double* stagthck, const double* flwafact,
+                                     int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                                     double* efvs, double* Auu, double* Auv, double* Avu, double* Avv);
+void orc_basal_sliding_bc_local(int nx, int ny, int nz, double dx, double dy,
+                                const int32_t* active_vertex, const double* beta,
+                                const double* theta_basal_slope, double* Auu, double* Avv);
+void orc_dirichlet_bc_3d(int nx, int ny, int nz, int nhalo, const int32_t* active_vertex,
+                         const int32_t* umask, const int32_t* vmask, const double* uvel, const double* vvel,
+                         double* Auu, double* Auv, double* Avu, double* Avv, double* bu, double* bv);
+int64_t orc_check_symmetry_assembled_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi,
+                                        const int32_t* active_vertex,
+                                        double* Auu, double* Auv, double* Avu, double* Avv);
+int64_t orc_count_nonzeros_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi,
+                              const int32_t* active_vertex, const double* Auu, const double* Auv,
+                              const double* Avu, const double* Avv);
+double orc_residual_velocity_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int whichresid,
+                                const double* usav, const double* vsav, const double* uvel, const double* vvel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

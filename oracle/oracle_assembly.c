@@ -1,0 +1,583 @@
+/*
+ * oracle_assembly.c -- CPU restatement of the 3-D Blatter-Pattyn assembly
+ * path of libglissade/glissade_velo_higher.F90 and of the 2-D input
+ * operators it calls (glissade_grid_operators.F90, glissade_masks.F90,
+ * glide_thck.F90).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP product).  Layouts as
+ * in cism_oracle.h; each function cites the reference lines it follows.
+ */
+#include "cism_oracle.h"
+#include "cism_oracle_asm.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NVX (nx - 1)
+#define NVY (ny - 1)
+#define C2(i, j) ((size_t)((i) - 1) + (size_t)nx * (size_t)((j) - 1))
+#define V2(i, j) ((size_t)((i) - 1) + (size_t)NVX * (size_t)((j) - 1))
+#define V3(k, i, j) ((size_t)((k) - 1) + (size_t)nz * ((size_t)((i) - 1) + (size_t)NVX * (size_t)((j) - 1)))
+#define A4(m, k, i, j) ((size_t)((m) - 1) + 27 * V3(k, i, j))
+#define E3(k, i, j) ((size_t)((k) - 1) + (size_t)(nz - 1) * ((size_t)((i) - 1) + (size_t)nx * (size_t)((j) - 1)))
+#define IDX3(iA, jA, kA) (1 + ((iA) + 1) + 3 * ((jA) + 1) + 9 * ((kA) + 1))
+
+static const double RHOI = 910.0, RHOO = 1028.0, GRAV = 9.81, VOL0 = 1.0e9;
+static const double EPS11 = 1.0e-11;
+
+/* ------------------------------------------------------------------ */
+/* glissade_velo_higher_init (glissade_velo_higher.F90:278-658)        */
+/* Tables are stored [n][p] for Fortran (n,p).                          */
+/* ------------------------------------------------------------------ */
+static double phi3[8][8], dxr3[8][8], dyr3[8][8], dzr3[8][8];
+static int node_di[8], node_dj[8], node_dk[8];   /* ishift/jshift/kshift(7,n) */
+static int tables_ready = 0;
+
+static void init_tables(void)
+{
+    if (tables_ready) return;
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[8] = {-r, r, r, -r, -r, r, r, -r};
+    const double yq[8] = {-r, -r, r, r, -r, -r, r, r};
+    const double zq[8] = {-r, -r, -r, -r, r, r, r, r};
+    /* node signs of the trilinear basis (:352-388) */
+    const double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    const double sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const double sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int p = 0; p < 8; ++p)
+        for (int n = 0; n < 8; ++n) {
+            phi3[n][p] = (1.0 + sx[n] * xq[p]) * (1.0 + sy[n] * yq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+            dxr3[n][p] = sx[n] * (1.0 + sy[n] * yq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+            dyr3[n][p] = sy[n] * (1.0 + sx[n] * xq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+            dzr3[n][p] = sz[n] * (1.0 + sx[n] * xq[p]) * (1.0 + sy[n] * yq[p]) / 8.0;
+        }
+    /* offsets of node n from node 7 = (k,i,j) of the element (:461-494) */
+    const int di[8] = {-1, 0, 0, -1, -1, 0, 0, -1};
+    const int dj[8] = {-1, -1, 0, 0, -1, -1, 0, 0};
+    const int dk[8] = {1, 1, 1, 1, 0, 0, 0, 0};
+    for (int n = 0; n < 8; ++n) { node_di[n] = di[n]; node_dj[n] = dj[n]; node_dk[n] = dk[n]; }
+    tables_ready = 1;
+}
+
+void orc_fe_tables(double* phi, double* dxr, double* dyr, double* dzr)
+{
+    init_tables();
+    for (int n = 0; n < 8; ++n)
+        for (int p = 0; p < 8; ++p) {
+            phi[n + 8 * p] = phi3[n][p]; dxr[n + 8 * p] = dxr3[n][p];
+            dyr[n + 8 * p] = dyr3[n][p]; dzr[n + 8 * p] = dzr3[n][p];
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-D input operators                                                  */
+/* ------------------------------------------------------------------ */
+
+/* glide_calclsrf (glide_thck.F90:1053-1076) */
+void orc_calclsrf(int nx, int ny, const double* thck, const double* topg, double eus, double* lsrf)
+{
+    for (size_t p = 0; p < (size_t)nx * ny; ++p)
+        lsrf[p] = (topg[p] - eus < (-RHOI / RHOO) * thck[p]) ? (-RHOI / RHOO) * thck[p] : topg[p];
+}
+
+/* glissade_get_masks (glissade_masks.F90:70-210), the four masks used here */
+void orc_get_masks(int nx, int ny, const double* thck, const double* topg, double eus, double thklim,
+                   int32_t* ice_mask, int32_t* floating_mask, int32_t* ocean_mask, int32_t* land_mask)
+{
+    for (size_t p = 0; p < (size_t)nx * ny; ++p) {
+        ice_mask[p] = thck[p] > thklim ? 1 : 0;
+        if (ocean_mask) ocean_mask[p] = (topg[p] < eus && ice_mask[p] == 0) ? 1 : 0;
+        if (floating_mask) floating_mask[p] = (topg[p] - eus < (-RHOI / RHOO) * thck[p] && ice_mask[p] == 1) ? 1 : 0;
+        if (land_mask) land_mask[p] = topg[p] >= eus ? 1 : 0;
+    }
+}
+
+/* glissade_stagger, stagger_margin = 0 (glissade_grid_operators.F90:125-133) */
+void orc_stagger(int nx, int ny, const double* var, double* stagvar)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i)
+            stagvar[V2(i, j)] = (var[C2(i, j + 1)] + var[C2(i + 1, j + 1)] + var[C2(i, j)] + var[C2(i + 1, j)]) / 4.0;
+}
+
+/* glissade_surface_elevation_gradient (glissade_grid_operators.F90:834-1260)
+ * for ho_gradient = centered, all three margin options, max_slope clip. */
+void orc_surface_elevation_gradient(int nx, int ny, double dx, double dy,
+                                    const int32_t* ice_mask, const int32_t* land_mask,
+                                    const double* usrf, const double* thck,
+                                    double thklim, double thck_gradient_ramp, int margin,
+                                    double max_slope, double* ds_dx, double* ds_dy)
+{
+    double* ex = (double*)calloc((size_t)nx * ny, sizeof(double));
+    double* ey = (double*)calloc((size_t)nx * ny, sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (margin == 0) {
+                ex[C2(i, j)] = (usrf[C2(i + 1, j)] - usrf[C2(i, j)]) / dx;
+            } else if (margin == 2) {
+                if (ice_mask[C2(i, j)] == 1 && ice_mask[C2(i + 1, j)] == 1)
+                    ex[C2(i, j)] = (usrf[C2(i + 1, j)] - usrf[C2(i, j)]) / dx;
+            } else {
+                int iu, il; double sgn;
+                if (usrf[C2(i, j)] > usrf[C2(i + 1, j)]) { iu = i; il = i + 1; sgn = -1.0; }
+                else { iu = i + 1; il = i; sgn = 1.0; }
+                double ef = 1.0;
+                if (land_mask[C2(iu, j)] == 1 && thck_gradient_ramp > 0.0) {
+                    ef = fmin(1.0, (thck[C2(iu, j)] - thklim) / thck_gradient_ramp);
+                    ef = fmax(ef, 0.0);
+                }
+                if ((ice_mask[C2(iu, j)] == 1 && ice_mask[C2(il, j)] == 1) ||
+                    (ice_mask[C2(iu, j)] == 1 && land_mask[C2(il, j)] == 1))
+                    ex[C2(i, j)] = ef * sgn * (usrf[C2(iu, j)] - usrf[C2(il, j)]) / dx;
+            }
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            if (margin == 0) {
+                ey[C2(i, j)] = (usrf[C2(i, j + 1)] - usrf[C2(i, j)]) / dy;
+            } else if (margin == 2) {
+                if (ice_mask[C2(i, j)] == 1 && ice_mask[C2(i, j + 1)] == 1)
+                    ey[C2(i, j)] = (usrf[C2(i, j + 1)] - usrf[C2(i, j)]) / dy;
+            } else {
+                int ju, jl; double sgn;
+                if (usrf[C2(i, j)] > usrf[C2(i, j + 1)]) { ju = j; jl = j + 1; sgn = -1.0; }
+                else { ju = j + 1; jl = j; sgn = 1.0; }
+                double ef = 1.0;
+                if (land_mask[C2(i, ju)] == 1 && thck_gradient_ramp > 0.0) {
+                    ef = fmin(1.0, (thck[C2(i, ju)] - thklim) / thck_gradient_ramp);
+                    ef = fmax(ef, 0.0);
+                }
+                if ((ice_mask[C2(i, ju)] == 1 && ice_mask[C2(i, jl)] == 1) ||
+                    (ice_mask[C2(i, ju)] == 1 && land_mask[C2(i, jl)] == 1))
+                    ey[C2(i, j)] = ef * sgn * (usrf[C2(i, ju)] - usrf[C2(i, jl)]) / dy;
+            }
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            ds_dx[V2(i, j)] = 0.5 * (ex[C2(i, j)] + ex[C2(i, j + 1)]);
+            ds_dy[V2(i, j)] = 0.5 * (ey[C2(i, j)] + ey[C2(i + 1, j)]);
+        }
+    if (max_slope > 0.0)
+        for (size_t p = 0; p < (size_t)(nx - 1) * (ny - 1); ++p) {
+            ds_dx[p] = ds_dx[p] > 0.0 ? fmin(ds_dx[p], max_slope) : fmax(ds_dx[p], -max_slope);
+            ds_dy[p] = ds_dy[p] > 0.0 ? fmin(ds_dy[p], max_slope) : fmax(ds_dy[p], -max_slope);
+        }
+    free(ex); free(ey);
+}
+
+/* glissade_gradient with gradient_margin = 1 and a full mask
+ * (glissade_grid_operators.F90:309-468), as called by glissade_slope_angle:
+ * with slope_mask = 1 everywhere the edge gradients are plain differences. */
+static void gradient_full(int nx, int ny, double dx, double dy, const double* f, double* gx, double* gy)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const double n_ = (f[C2(i + 1, j + 1)] - f[C2(i, j + 1)]) / dx;
+            const double s_ = (f[C2(i + 1, j)] - f[C2(i, j)]) / dx;
+            gx[V2(i, j)] = (n_ + s_) / 2.0;
+            const double e_ = (f[C2(i + 1, j + 1)] - f[C2(i + 1, j)]) / dy;
+            const double w_ = (f[C2(i, j + 1)] - f[C2(i, j)]) / dy;
+            gy[V2(i, j)] = (e_ + w_) / 2.0;
+        }
+}
+
+/* glissade_slope_angle (glissade_grid_operators.F90:1264-1395): gradient at
+ * vertices, unstagger to centres (:226-305, margin 0), theta = atan|grad|.
+ * Called with the vertex grid as its "cell" grid by basal_sliding_bc_3d. */
+void orc_slope_angle(int nx, int ny, double dx, double dy, const double* zsfc, double* theta)
+{
+    double* gx = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    double* gy = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    gradient_full(nx, ny, dx, dy, zsfc, gx, gy);
+    for (size_t p = 0; p < (size_t)nx * ny; ++p) theta[p] = 0.0;
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            double dzx = (gx[V2(i, j)] + gx[V2(i - 1, j)] + gx[V2(i, j - 1)] + gx[V2(i - 1, j - 1)]) / 4.0;
+            double dzy = (gy[V2(i, j)] + gy[V2(i - 1, j)] + gy[V2(i, j - 1)] + gy[V2(i - 1, j - 1)]) / 4.0;
+            double s = sqrt(dzx * dzx + dzy * dzy);
+            theta[C2(i, j)] = (s >= 0.0) ? atan(s) : 0.0;
+        }
+    free(gx); free(gy);
+}
+
+/* get_vertex_geometry (glissade_velo_higher.F90:4052-4224), active masks */
+void orc_vertex_masks(int nx, int ny, int nhalo, const int32_t* ice_mask,
+                      int32_t* active_cell, int32_t* active_vertex)
+{
+    memset(active_cell, 0, (size_t)nx * ny * sizeof(int32_t));
+    memset(active_vertex, 0, (size_t)(nx - 1) * (ny - 1) * sizeof(int32_t));
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i)
+            if (ice_mask[C2(i, j)] == 1) active_cell[C2(i, j)] = 1;
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i)
+            if (active_cell[C2(i, j)]) {
+                active_vertex[V2(i - 1, j - 1)] = 1; active_vertex[V2(i, j - 1)] = 1;
+                active_vertex[V2(i - 1, j)] = 1; active_vertex[V2(i, j)] = 1;
+            }
+}
+
+/* flwafact = 0.5 * flwa^(-1/n) (glissade_velo_higher.F90:1838-1847);
+ * cells outside the loop range keep 0. */
+void orc_flwafact(int nx, int ny, int nz, int nhalo, double n_glen, const double* flwa, double* flwafact)
+{
+    memset(flwafact, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    for (int j = 1 + nhalo; j <= ny - nhalo + 1; ++j)
+        for (int i = 1 + nhalo; i <= nx - nhalo + 1; ++i)
+            for (int k = 1; k <= nz - 1; ++k)
+                if (flwa[E3(k, i, j)] > 0.0) flwafact[E3(k, i, j)] = 0.5 * pow(flwa[E3(k, i, j)], -1.0 / n_glen);
+}
+
+/* ------------------------------------------------------------------ */
+/* load_vector_gravity, local option (glissade_velo_higher.F90:4311-4342) */
+/* ------------------------------------------------------------------ */
+void orc_load_vector_gravity_local(int nx, int ny, int nz, const double* stagwbndsigma /* 0:nz */,
+                                   double dx, double dy, const int32_t* active_vertex,
+                                   const double* stagthck, const double* dusrf_dx, const double* dusrf_dy,
+                                   double* loadu, double* loadv)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            for (int k = 1; k <= nz; ++k) {
+                double dz = stagthck[V2(i, j)] * (stagwbndsigma[k] - stagwbndsigma[k - 1]);
+                loadu[V3(k, i, j)] = loadu[V3(k, i, j)] - RHOI * GRAV * dx * dy * dz / VOL0 * dusrf_dx[V2(i, j)];
+                loadv[V3(k, i, j)] = loadv[V3(k, i, j)] - RHOI * GRAV * dx * dy * dz / VOL0 * dusrf_dy[V2(i, j)];
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Element kernels                                                      */
+/* ------------------------------------------------------------------ */
+
+/* get_basis_function_derivatives_3d (glissade_velo_higher.F90:6508-6732) */
+static int basis_derivs(const double* x, const double* y, const double* z, int p,
+                        double* dx, double* dy, double* dz, double* detJ)
+{
+    double J[3][3] = {{0}};
+    for (int n = 0; n < 8; ++n) {
+        J[0][0] = J[0][0] + dxr3[n][p] * x[n];
+        J[0][1] = J[0][1] + dxr3[n][p] * y[n];
+        J[0][2] = J[0][2] + dxr3[n][p] * z[n];
+        J[1][0] = J[1][0] + dyr3[n][p] * x[n];
+        J[1][1] = J[1][1] + dyr3[n][p] * y[n];
+        J[1][2] = J[1][2] + dyr3[n][p] * z[n];
+        J[2][0] = J[2][0] + dzr3[n][p] * x[n];
+        J[2][1] = J[2][1] + dzr3[n][p] * y[n];
+        J[2][2] = J[2][2] + dzr3[n][p] * z[n];
+    }
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    c[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    double d = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+    *detJ = d;
+    if (!(fabs(d) > 0.0)) return ORC_ERR_SINGULAR;
+    double Ji[3][3];
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) Ji[row][col] = c[col][row] / d;
+    for (int n = 0; n < 8; ++n) {
+        dx[n] = Ji[0][0] * dxr3[n][p] + Ji[0][1] * dyr3[n][p] + Ji[0][2] * dzr3[n][p];
+        dy[n] = Ji[1][0] * dxr3[n][p] + Ji[1][1] * dyr3[n][p] + Ji[1][2] * dzr3[n][p];
+        dz[n] = Ji[2][0] * dxr3[n][p] + Ji[2][1] * dyr3[n][p] + Ji[2][2] * dzr3[n][p];
+    }
+    return ORC_OK;
+}
+
+/* compute_effective_viscosity, BP branch (glissade_velo_higher.F90:7285-7470) */
+static double effective_viscosity(int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                                  const double* dx, const double* dy, const double* dz,
+                                  const double* u, const double* v, double flwafact)
+{
+    const double p_effstr = (1.0 - n_glen) / n_glen;
+    if (whichefvs == 0) return efvs_constant;
+    if (whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;   /* velo_scale/len_scale*scyr */
+        return flwafact * pow(effstrain, p_effstr);
+    }
+    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+    for (int n = 0; n < 8; ++n) {
+        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n]; uz = uz + dz[n] * u[n];
+        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n]; vz = vz + dz[n] * v[n];
+    }
+    double t1 = ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
+    double t2 = 0.25 * (uz * uz + vz * vz);
+    double esq = effstrain_min * effstrain_min + t1 + t2;
+    return flwafact * pow(esq, p_effstr / 2.0);
+}
+
+/* One element's stiffness blocks: quadrature loop of
+ * assemble_stiffness_matrix_3d (:5562-5602) with compute_element_matrix
+ * (:8001-8123, BP branch).  K are [nr][nc]. */
+int orc_element_matrix(const double* x, const double* y, const double* z, const double* u, const double* v,
+                       double flwafact, int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                       double Kuu[8][8], double Kuv[8][8], double Kvu[8][8], double Kvv[8][8], double* efvs_mean)
+{
+    init_tables();
+    double dx[8], dy[8], dz[8], detJ, efvs_qp[8];
+    memset(Kuu, 0, 64 * sizeof(double)); memset(Kuv, 0, 64 * sizeof(double));
+    memset(Kvu, 0, 64 * sizeof(double)); memset(Kvv, 0, 64 * sizeof(double));
+    for (int p = 0; p < 8; ++p) {
+        int st = basis_derivs(x, y, z, p, dx, dy, dz, &detJ);
+        if (st) return st;
+        efvs_qp[p] = effective_viscosity(whichefvs, efvs_constant, effstrain_min, n_glen, dx, dy, dz, u, v, flwafact);
+        const double f = efvs_qp[p] * 1.0 * detJ / VOL0;
+        for (int nc = 0; nc < 8; ++nc)
+            for (int nr = 0; nr < 8; ++nr) {
+                Kuu[nr][nc] = Kuu[nr][nc] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc] + dz[nr] * dz[nc]);
+                Kuv[nr][nc] = Kuv[nr][nc] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                Kvu[nr][nc] = Kvu[nr][nc] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                Kvv[nr][nc] = Kvv[nr][nc] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc] + dz[nr] * dz[nc]);
+            }
+    }
+    double s = 0.0;
+    for (int p = 0; p < 8; ++p) s = s + efvs_qp[p];
+    *efvs_mean = s / 8.0;
+    /* check_symmetry_element_matrix (:10583-10644) */
+    for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b) {
+            if (fabs(Kuu[a][b] - Kuu[b][a]) > EPS11) return ORC_ERR_ASYM;
+            if (fabs(Kvv[a][b] - Kvv[b][a]) > EPS11) return ORC_ERR_ASYM;
+            if (fabs(Kuv[a][b] - Kvu[b][a]) > EPS11) return ORC_ERR_ASYM;
+        }
+    return ORC_OK;
+}
+
+/* assemble_stiffness_matrix_3d (glissade_velo_higher.F90:5382-5643) with
+ * element_to_global_matrix_3d (:8200-8268). */
+int orc_assemble_stiffness_matrix_3d(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                     const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                     const double* uvel, const double* vvel,
+                                     const double* stagusrf, const double* stagthck, const double* flwafact,
+                                     int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                                     double* efvs, double* Auu, double* Auv, double* Avu, double* Avv)
+{
+    init_tables();
+    const size_t nA = (size_t)27 * nz * (nx - 1) * (ny - 1);
+    memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    memset(Auu, 0, nA * sizeof(double)); memset(Auv, 0, nA * sizeof(double));
+    memset(Avu, 0, nA * sizeof(double)); memset(Avv, 0, nA * sizeof(double));
+    double Kuu[8][8], Kuv[8][8], Kvu[8][8], Kvv[8][8];
+    double x[8], y[8], z[8], u[8], v[8];
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                for (int n = 0; n < 8; ++n) {
+                    const int in = i + node_di[n], jn = j + node_dj[n], kn = k + node_dk[n];
+                    x[n] = xVertex[V2(in, jn)];
+                    y[n] = yVertex[V2(in, jn)];
+                    z[n] = stagusrf[V2(in, jn)] - sigma[kn - 1] * stagthck[V2(in, jn)];
+                    u[n] = uvel[V3(kn, in, jn)];
+                    v[n] = vvel[V3(kn, in, jn)];
+                }
+                double em;
+                int st = orc_element_matrix(x, y, z, u, v, flwafact[E3(k, i, j)], whichefvs, efvs_constant,
+                                            effstrain_min, n_glen, Kuu, Kuv, Kvu, Kvv, &em);
+                if (st) return st;
+                efvs[E3(k, i, j)] = em;
+                for (int nr = 0; nr < 8; ++nr) {
+                    const int ir = i + node_di[nr], jr = j + node_dj[nr], kr = k + node_dk[nr];
+                    for (int nc = 0; nc < 8; ++nc) {
+                        const int m = IDX3(node_di[nc] - node_di[nr], node_dj[nc] - node_dj[nr], node_dk[nc] - node_dk[nr]);
+                        const size_t a = A4(m, kr, ir, jr);
+                        Auu[a] = Auu[a] + Kuu[nr][nc];
+                        Auv[a] = Auv[a] + Kuv[nr][nc];
+                        Avu[a] = Avu[a] + Kvu[nr][nc];
+                        Avv[a] = Avv[a] + Kvv[nr][nc];
+                    }
+                }
+            }
+        }
+    return ORC_OK;
+}
+
+/* basal_sliding_bc_3d, local option (glissade_velo_higher.F90:8921-8953).
+ * theta_basal_slope is glissade_slope_angle of staglsrf on the vertex grid
+ * (orc_slope_angle with nx-1, ny-1); its parallel_halo call is a no-op for
+ * vertex-sized arrays (parallel_mpi.F90:7769). */
+void orc_basal_sliding_bc_local(int nx, int ny, int nz, double dx, double dy,
+                                const int32_t* active_vertex, const double* beta,
+                                const double* theta_basal_slope, double* Auu, double* Avv)
+{
+    const int m = IDX3(0, 0, 0);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            const size_t a = A4(m, nz, i, j);
+            const double c = cos(theta_basal_slope[V2(i, j)]);
+            Auu[a] = Auu[a] + dx * dy / VOL0 * beta[V2(i, j)] / c;
+            Avv[a] = Avv[a] + dx * dy / VOL0 * beta[V2(i, j)] / c;
+        }
+}
+
+/* dirichlet_boundary_conditions_3d (glissade_velo_higher.F90:9305-9498) */
+void orc_dirichlet_bc_3d(int nx, int ny, int nz, int nhalo, const int32_t* active_vertex,
+                         const int32_t* umask, const int32_t* vmask, const double* uvel, const double* vvel,
+                         double* Auu, double* Auv, double* Avu, double* Avv, double* bu, double* bv)
+{
+    const int md = IDX3(0, 0, 0);
+    for (int j = nhalo; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo; i <= nx - nhalo + 1; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            for (int k = 1; k <= nz; ++k) {
+                if (umask[V3(k, i, j)] == 1) {
+                    bu[V3(k, i, j)] = uvel[V3(k, i, j)];
+                    for (int kA = -1; kA <= 1; ++kA)
+                        for (int jA = -1; jA <= 1; ++jA)
+                            for (int iA = -1; iA <= 1; ++iA) {
+                                if (!(k + kA >= 1 && k + kA <= nz && i + iA >= 1 && i + iA <= nx - 1 &&
+                                      j + jA >= 1 && j + jA <= ny - 1)) continue;
+                                if (iA == 0 && jA == 0 && kA == 0) {
+                                    Auu[A4(md, k, i, j)] = 1.0; Auv[A4(md, k, i, j)] = 0.0; Avu[A4(md, k, i, j)] = 0.0;
+                                } else {
+                                    int m = IDX3(iA, jA, kA);
+                                    Auu[A4(m, k, i, j)] = 0.0; Auv[A4(m, k, i, j)] = 0.0;
+                                    m = IDX3(-iA, -jA, -kA);
+                                    const size_t nb = V3(k + kA, i + iA, j + jA);
+                                    if (umask[nb] != 1) {
+                                        bu[nb] = bu[nb] - Auu[m - 1 + 27 * nb] * uvel[V3(k, i, j)];
+                                        Auu[m - 1 + 27 * nb] = 0.0;
+                                    }
+                                    if (vmask[nb] != 1) {
+                                        bv[nb] = bv[nb] - Avu[m - 1 + 27 * nb] * uvel[V3(k, i, j)];
+                                        Avu[m - 1 + 27 * nb] = 0.0;
+                                    }
+                                }
+                            }
+                }
+                if (vmask[V3(k, i, j)] == 1) {
+                    bv[V3(k, i, j)] = vvel[V3(k, i, j)];
+                    for (int kA = -1; kA <= 1; ++kA)
+                        for (int jA = -1; jA <= 1; ++jA)
+                            for (int iA = -1; iA <= 1; ++iA) {
+                                if (!(k + kA >= 1 && k + kA <= nz && i + iA >= 1 && i + iA <= nx - 1 &&
+                                      j + jA >= 1 && j + jA <= ny - 1)) continue;
+                                if (iA == 0 && jA == 0 && kA == 0) {
+                                    Auv[A4(md, k, i, j)] = 0.0; Avu[A4(md, k, i, j)] = 0.0; Avv[A4(md, k, i, j)] = 1.0;
+                                } else {
+                                    int m = IDX3(iA, jA, kA);
+                                    Avu[A4(m, k, i, j)] = 0.0; Avv[A4(m, k, i, j)] = 0.0;
+                                    m = IDX3(-iA, -jA, -kA);
+                                    const size_t nb = V3(k + kA, i + iA, j + jA);
+                                    if (umask[nb] != 1) {
+                                        bu[nb] = bu[nb] - Auv[m - 1 + 27 * nb] * vvel[V3(k, i, j)];
+                                        Auv[m - 1 + 27 * nb] = 0.0;
+                                    }
+                                    if (vmask[nb] != 1) {
+                                        bv[nb] = bv[nb] - Avv[m - 1 + 27 * nb] * vvel[V3(k, i, j)];
+                                        Avv[m - 1 + 27 * nb] = 0.0;
+                                    }
+                                }
+                            }
+                }
+            }
+        }
+}
+
+/* check_symmetry_assembled_matrix_3d (glissade_velo_higher.F90:10854-11117):
+ * averages partner entries that differ by less than eps11*|diag|.
+ * Returns the number of pairs left asymmetric (the reference warns). */
+int64_t orc_check_symmetry_assembled_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi,
+                                        const int32_t* active_vertex,
+                                        double* Auu, double* Auv, double* Avu, double* Avv)
+{
+    int64_t nwarn = 0;
+    const int md = IDX3(0, 0, 0);
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            for (int k = 1; k <= nz; ++k) {
+                double diag = Auu[A4(md, k, i, j)];
+                for (int jA = -1; jA <= 1; ++jA)
+                    for (int iA = -1; iA <= 1; ++iA)
+                        for (int kA = -1; kA <= 1; ++kA) {
+                            if (!(k + kA >= 1 && k + kA <= nz)) continue;
+                            const size_t a1 = A4(IDX3(iA, jA, kA), k, i, j);
+                            const size_t a2 = A4(IDX3(-iA, -jA, -kA), k + kA, i + iA, j + jA);
+                            double v1 = Auu[a1], v2 = Auu[a2];
+                            if (v2 != v1) {
+                                if (fabs(v2 - v1) < EPS11 * fabs(diag)) { double av = 0.5 * (v1 + v2); Auu[a1] = av; Auu[a2] = av; }
+                                else nwarn++;
+                            }
+                            v1 = Auv[a1]; v2 = Avu[a2];
+                            if (v2 != v1) {
+                                if (fabs(v2 - v1) < EPS11 * fabs(diag)) { double av = 0.5 * (v1 + v2); Auv[a1] = av; Avu[a2] = av; }
+                                else nwarn++;
+                            }
+                        }
+                diag = Avv[A4(md, k, i, j)];
+                for (int jA = -1; jA <= 1; ++jA)
+                    for (int iA = -1; iA <= 1; ++iA)
+                        for (int kA = -1; kA <= 1; ++kA) {
+                            if (!(k + kA >= 1 && k + kA <= nz)) continue;
+                            const size_t a1 = A4(IDX3(iA, jA, kA), k, i, j);
+                            const size_t a2 = A4(IDX3(-iA, -jA, -kA), k + kA, i + iA, j + jA);
+                            double v1 = Avv[a1], v2 = Avv[a2];
+                            if (v2 != v1) {
+                                if (fabs(v2 - v1) < EPS11 * fabs(diag)) { double av = 0.5 * (v1 + v2); Avv[a1] = av; Avv[a2] = av; }
+                                else nwarn++;
+                            }
+                            v1 = Avu[a1]; v2 = Auv[a2];
+                            if (v2 != v1) {
+                                if (fabs(v2 - v1) < EPS11 * fabs(diag)) { double av = 0.5 * (v1 + v2); Avu[a1] = av; Auv[a2] = av; }
+                                else nwarn++;
+                            }
+                        }
+            }
+        }
+    return nwarn;
+}
+
+/* count_nonzeros_3d (glissade_velo_higher.F90:10513-10579): nonzero
+ * entries of the four blocks over owned active rows. */
+int64_t orc_count_nonzeros_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi,
+                              const int32_t* active_vertex, const double* Auu, const double* Auv,
+                              const double* Avu, const double* Avv)
+{
+    int64_t c = 0;
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            for (int k = 1; k <= nz; ++k)
+                for (int m = 1; m <= 27; ++m) {
+                    const size_t a = A4(m, k, i, j);
+                    if (Auu[a] != 0.0) c++;
+                    if (Auv[a] != 0.0) c++;
+                    if (Avu[a] != 0.0) c++;
+                    if (Avv[a] != 0.0) c++;
+                }
+        }
+    return c;
+}
+
+/* compute_residual_velocity_3d (glissade_velo_higher.F90:10314-10446) */
+double orc_residual_velocity_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int whichresid,
+                                const double* usav, const double* vsav, const double* uvel, const double* vvel)
+{
+    double r = 0.0;
+    int64_t count = 0;
+    const int ktop = (whichresid == 1 || whichresid == 2) ? nz - 1 : nz;
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i)
+            for (int k = 1; k <= ktop; ++k) {
+                const size_t p = V3(k, i, j);
+                double speed = sqrt(uvel[p] * uvel[p] + vvel[p] * vvel[p]);
+                if (speed != 0.0) {
+                    double old = sqrt(usav[p] * usav[p] + vsav[p] * vsav[p]);
+                    double d = fabs((old - speed) / speed);
+                    if (whichresid == 2) { count++; r = r + d; }
+                    else if (d > r) r = d;
+                }
+            }
+    if (whichresid == 2 && count > 0) r = r / (double)count;
+    return r;
+}

@@ -1,0 +1,291 @@
+"""ctypes bindings of the CPU restatement (oracle/libcism_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product.  Only tests/,
+``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg import this.
+Arrays use the reference layouts (Fortran order), see cism_amd/grid.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libcism_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    from cism_amd.grid import CGrid
+    G = ctypes.POINTER(CGrid)
+    solver_args = [G, _I, _D, _D, _D, _D, _D, _D, _D, _D, ctypes.c_int32, ctypes.c_int32,
+                   ctypes.c_double, ctypes.c_int32, _D, _I]
+    for name in ("orc_pcg_solver_standard_3d", "orc_pcg_solver_chrongear_3d"):
+        f = getattr(L, name)
+        f.argtypes = solver_args
+        f.restype = ctypes.c_int
+    L.orc_pcg_solver_chrongear_3d_omp.argtypes = solver_args[:-2] + [ctypes.c_int32, _D, _I]
+    L.orc_pcg_solver_chrongear_3d_omp.restype = ctypes.c_int
+    L.orc_matvec_3d.argtypes = [G, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_halo3d.argtypes = [G, ctypes.c_int, _D]
+    L.orc_global_sum.argtypes = [G, _D, _D]
+    L.orc_global_sum.restype = ctypes.c_double
+    L.orc_ddpdd_sum.argtypes = [ctypes.c_int64, _D, _D, _D]
+    L.orc_tridiag_solver.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D]
+    L.orc_set_trace.argtypes = [_D, ctypes.c_int32]
+    L.orc_trace_len.restype = ctypes.c_int32
+    L.orc_fe_tables.argtypes = [_D, _D, _D, _D]
+    i = ctypes.c_int
+    d = ctypes.c_double
+    L.orc_calclsrf.argtypes = [i, i, _D, _D, d, _D]
+    L.orc_get_masks.argtypes = [i, i, _D, _D, d, d, _I, _I, _I, _I]
+    L.orc_stagger.argtypes = [i, i, _D, _D]
+    L.orc_surface_elevation_gradient.argtypes = [i, i, d, d, _I, _I, _D, _D, d, d, i, d, _D, _D]
+    L.orc_slope_angle.argtypes = [i, i, d, d, _D, _D]
+    L.orc_vertex_masks.argtypes = [i, i, i, _I, _I, _I]
+    L.orc_flwafact.argtypes = [i, i, i, i, d, _D, _D]
+    L.orc_load_vector_gravity_local.argtypes = [i, i, i, _D, d, d, _I, _D, _D, _D, _D, _D]
+    L.orc_element_matrix.argtypes = [_D, _D, _D, _D, _D, d, i, d, d, d, _D, _D, _D, _D, _D]
+    L.orc_element_matrix.restype = ctypes.c_int
+    L.orc_assemble_stiffness_matrix_3d.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, _D, _D,
+                                                   i, d, d, d, _D, _D, _D, _D, _D]
+    L.orc_assemble_stiffness_matrix_3d.restype = ctypes.c_int
+    L.orc_basal_sliding_bc_local.argtypes = [i, i, i, d, d, _I, _D, _D, _D, _D]
+    L.orc_dirichlet_bc_3d.argtypes = [i, i, i, i, _I, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_check_symmetry_assembled_3d.argtypes = [i, i, i, i, i, i, i, _I, _D, _D, _D, _D]
+    L.orc_check_symmetry_assembled_3d.restype = ctypes.c_int64
+    L.orc_count_nonzeros_3d.argtypes = [i, i, i, i, i, i, i, _I, _D, _D, _D, _D]
+    L.orc_count_nonzeros_3d.restype = ctypes.c_int64
+    L.orc_residual_velocity_3d.argtypes = [i, i, i, i, i, i, i, i, _D, _D, _D, _D]
+    L.orc_residual_velocity_3d.restype = ctypes.c_double
+
+
+def dp(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous), a.dtype
+    return a.ctypes.data_as(_D)
+
+
+def ip(a: np.ndarray):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_I)
+
+
+def as_mask(m) -> np.ndarray:
+    return np.asfortranarray(np.asarray(m, dtype=np.int32))
+
+
+# ---------------------------------------------------------------- solver path
+def pcg_solve(grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, *, algo="chrongear",
+              precond=2, ncheck=5, tolerance=1e-8, maxiters=200, trace=False, nthreads=None):
+    """Run the restated solver in place on xu, xv.  Returns (status, err, niters[, trace])."""
+    L = lib()
+    g = grid.cstruct()
+    err = ctypes.c_double(float("nan"))
+    nit = ctypes.c_int32(0)
+    act = as_mask(active_vertex)
+    tbuf = None
+    if trace:
+        tbuf = np.zeros(3 * (maxiters + 1))
+        L.orc_set_trace(dp(tbuf), maxiters + 1)
+    else:
+        L.orc_set_trace(None, 0)
+    args = [ctypes.byref(g), ip(act), dp(Auu), dp(Auv), dp(Avu), dp(Avv), dp(bu), dp(bv), dp(xu), dp(xv),
+            int(precond), int(ncheck), float(tolerance), int(maxiters)]
+    if nthreads is not None:
+        st = L.orc_pcg_solver_chrongear_3d_omp(*args, int(nthreads), ctypes.byref(err), ctypes.byref(nit))
+    elif algo == "chrongear":
+        st = L.orc_pcg_solver_chrongear_3d(*args, ctypes.byref(err), ctypes.byref(nit))
+    else:
+        st = L.orc_pcg_solver_standard_3d(*args, ctypes.byref(err), ctypes.byref(nit))
+    out = (st, err.value, nit.value)
+    if trace:
+        n = L.orc_trace_len()
+        out = out + (tbuf[: 3 * n].reshape(n, 3).copy(),)
+        L.orc_set_trace(None, 0)
+    return out
+
+
+def matvec(grid, active_vertex, Auu, Auv, Avu, Avv, xu, xv):
+    L = lib()
+    g = grid.cstruct()
+    yu = grid.zeros_vec()
+    yv = grid.zeros_vec()
+    L.orc_matvec_3d(ctypes.byref(g), ip(as_mask(active_vertex)), dp(Auu), dp(Auv), dp(Avu), dp(Avv),
+                    dp(xu), dp(xv), dp(yu), dp(yv))
+    return yu, yv
+
+
+def halo(grid, a: np.ndarray):
+    L = lib()
+    g = grid.cstruct()
+    nlev = a.shape[0] if a.ndim == 3 else 1
+    L.orc_halo3d(ctypes.byref(g), nlev, dp(a))
+
+
+def global_sum(grid, a1, a2=None):
+    L = lib()
+    g = grid.cstruct()
+    return L.orc_global_sum(ctypes.byref(g), dp(a1), dp(a2) if a2 is not None else None)
+
+
+def ddpdd_sum(arr: np.ndarray):
+    L = lib()
+    hi = ctypes.c_double()
+    lo = ctypes.c_double()
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    L.orc_ddpdd_sum(a.size, dp(a), ctypes.byref(hi), ctypes.byref(lo))
+    return hi.value, lo.value
+
+
+def tridiag(sb, d, sp, rhs):
+    L = lib()
+    n = len(d)
+    out = np.zeros(n)
+    L.orc_tridiag_solver(n, dp(np.ascontiguousarray(sb, float)), dp(np.ascontiguousarray(d, float)),
+                         dp(np.ascontiguousarray(sp, float)), dp(np.ascontiguousarray(rhs, float)), dp(out))
+    return out
+
+
+# ------------------------------------------------------------ assembly path
+def fe_tables():
+    L = lib()
+    t = [np.zeros((8, 8), order="F") for _ in range(4)]
+    L.orc_fe_tables(*[dp(a) for a in t])
+    return t   # phi, dphi_dxr, dphi_dyr, dphi_dzr as (n, p)
+
+
+def element_matrix(x, y, z, u, v, flwafact, whichefvs=2, efvs_constant=0.0, effstrain_min=1e-8, n_glen=3.0):
+    L = lib()
+    K = [np.zeros((8, 8)) for _ in range(4)]   # C order [nr][nc]
+    em = ctypes.c_double()
+    f = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64))
+    st = L.orc_element_matrix(f(x), f(y), f(z), f(u), f(v), flwafact, whichefvs, efvs_constant,
+                              effstrain_min, n_glen, *[dp(k) for k in K], ctypes.byref(em))
+    return st, K, em.value
+
+
+def geometry(problem):
+    """Setup part of glissade_velo_higher_solve for the 3-D BP path
+    (glissade_velo_higher.F90:1247-1847): lsrf/usrf, masks, staggered fields,
+    surface gradient with its halo, active masks, flwafact."""
+    L = lib()
+    p = problem
+    g = p.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    lsrf = np.zeros((nx, ny), order="F")
+    L.orc_calclsrf(nx, ny, dp(p.thck), dp(p.topg), p.eus, dp(lsrf))
+    usrf = np.asfortranarray(np.maximum(0.0, p.thck + lsrf))
+    ice = np.zeros((nx, ny), np.int32, order="F")
+    flt = np.zeros_like(ice); ocn = np.zeros_like(ice); land = np.zeros_like(ice)
+    L.orc_get_masks(nx, ny, dp(p.thck), dp(p.topg), p.eus, p.thklim, ip(ice), ip(flt), ip(ocn), ip(land))
+    stagthck = np.zeros((nx - 1, ny - 1), order="F")
+    stagusrf = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_stagger(nx, ny, dp(p.thck), dp(stagthck))
+    L.orc_stagger(nx, ny, dp(usrf), dp(stagusrf))
+    dsdx = np.zeros((nx - 1, ny - 1), order="F")
+    dsdy = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_surface_elevation_gradient(nx, ny, p.dx, p.dy, ip(ice), ip(land), dp(usrf), dp(p.thck),
+                                     p.thklim, p.thck_gradient_ramp, p.gradient_margin, p.max_slope,
+                                     dp(dsdx), dp(dsdy))
+    halo(g, dsdx)
+    halo(g, dsdy)
+    acell = np.zeros((nx, ny), np.int32, order="F")
+    avert = np.zeros((nx - 1, ny - 1), np.int32, order="F")
+    L.orc_vertex_masks(nx, ny, 2, ip(ice), ip(acell), ip(avert))
+    flwafact = np.zeros((nz - 1, nx, ny), order="F")
+    L.orc_flwafact(nx, ny, nz, 2, p.n_glen, dp(np.asfortranarray(p.flwa)), dp(flwafact))
+    staglsrf = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_stagger(nx, ny, dp(lsrf), dp(staglsrf))
+    theta = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_slope_angle(nx - 1, ny - 1, p.dx, p.dy, dp(staglsrf), dp(theta))
+    return dict(lsrf=lsrf, usrf=usrf, ice_mask=ice, floating_mask=flt, ocean_mask=ocn, land_mask=land,
+                stagthck=stagthck, stagusrf=stagusrf, dusrf_dx=dsdx, dusrf_dy=dsdy,
+                active_cell=acell, active_vertex=avert, flwafact=flwafact, theta_basal=theta)
+
+
+def beta_internal(problem, geo):
+    """glissade_calcbeta for the constant/large/external cases with f_ground = 1
+    on grounded ice (glissade_basal_traction.F90:202, 304, 399, 650-709)."""
+    p = problem
+    g = p.grid
+    if p.whichbabc == 0:
+        b = np.full((g.nx - 1, g.ny - 1), p.beta_const, order="F")
+    elif p.whichbabc == 4:
+        b = np.full((g.nx - 1, g.ny - 1), 1.0e10, order="F")
+    else:
+        b = np.asfortranarray(p.beta.copy())
+    return b
+
+
+def assemble(problem, uvel=None, vvel=None, geo=None):
+    """One assembly of the linear BP system in the reference's order
+    (glissade_velo_higher.F90:2815-2931): stiffness, basal BC, b = load,
+    Dirichlet rows, halo of A and b, symmetry fix."""
+    L = lib()
+    p = problem
+    g = p.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    geo = geo or geometry(p)
+    uvel = np.asfortranarray(p.uvel.copy() if uvel is None else uvel)
+    vvel = np.asfortranarray(p.vvel.copy() if vvel is None else vvel)
+    av = geo["active_vertex"]
+    own = g.owned_slice()
+    inact = (av[own[1], own[2]] == 0)
+    uo = uvel[own]; vo = vvel[own]
+    uo[:, inact] = 0.0; vo[:, inact] = 0.0
+    uvel[own] = uo; vvel[own] = vo
+    xV, yV = p.vertex_coords()
+    loadu = g.zeros_vec(); loadv = g.zeros_vec()
+    from cism_amd.grid import stagwbndsigma
+    swb = stagwbndsigma(p.sigma)
+    L.orc_load_vector_gravity_local(nx, ny, nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
+                                    dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+    A = [g.zeros_mat() for _ in range(4)]
+    efvs = np.zeros((nz - 1, nx, ny), order="F")
+    sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
+    st = L.orc_assemble_stiffness_matrix_3d(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                            dp(uvel), dp(vvel), dp(geo["stagusrf"]), dp(geo["stagthck"]),
+                                            dp(geo["flwafact"]), p.whichefvs, p.efvs_constant,
+                                            p.effstrain_min, p.n_glen, dp(efvs), *[dp(a) for a in A])
+    if st:
+        raise RuntimeError(f"oracle assembly failed with status {st}")
+    binternal = beta_internal(p, geo)
+    if p.whichbabc != 6:
+        L.orc_basal_sliding_bc_local(nx, ny, nz, p.dx, p.dy, ip(av), dp(binternal), dp(geo["theta_basal"]),
+                                     dp(A[0]), dp(A[3]))
+    bu = loadu.copy(order="F"); bv = loadv.copy(order="F")
+    umask = np.zeros(g.vec_shape, np.int32, order="F")
+    if p.whichbabc == 6:
+        umask[nz - 1, :, :] = 1
+    vmask = umask.copy(order="F")
+    L.orc_dirichlet_bc_3d(nx, ny, nz, 2, ip(av), ip(umask), ip(vmask), dp(uvel), dp(vvel),
+                          *[dp(a) for a in A], dp(bu), dp(bv))
+    for a in A:
+        L.orc_halo3d(ctypes.byref(g.cstruct()), 27 * nz, dp(a))
+    halo(g, bu)
+    halo(g, bv)
+    nwarn = L.orc_check_symmetry_assembled_3d(nx, ny, nz, g.ilo, g.ihi, g.jlo, g.jhi, ip(av), *[dp(a) for a in A])
+    nnz = L.orc_count_nonzeros_3d(nx, ny, nz, g.ilo, g.ihi, g.jlo, g.jhi, ip(av), *[dp(a) for a in A])
+    return dict(Auu=A[0], Auv=A[1], Avu=A[2], Avv=A[3], bu=bu, bv=bv, efvs=efvs, uvel=uvel, vvel=vvel,
+                active_vertex=av, loadu=loadu, loadv=loadv, umask=umask, vmask=vmask,
+                beta_internal=binternal, nwarn_asym=nwarn, nnz=nnz, geo=geo)
