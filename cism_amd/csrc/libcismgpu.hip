@@ -1,0 +1,936 @@
+// libcismgpu.hip -- host drivers and C ABI of the structured 3-D velocity
+// solver (glissade_velo_higher_pcg.F90) on MI355X.
+//
+// One translation unit: the kernels are included so that every template
+// instance the drivers launch is generated here.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "cismgpu_internal.h"
+#include "kernels.h"
+#include "kernels.hip"
+
+using namespace cg;
+
+static thread_local char g_err[1024] = "";
+extern "C" void cismgpu_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+extern "C" const char* cismgpu_last_error(void) { return g_err; }
+
+extern "C" int32_t cismgpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+namespace {
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct TriPC {               // horizontal tridiagonal preconditioner (precond 3/4), 2-D arrays
+    double* a = nullptr;     // 13 fields of nvx*nvy doubles
+    double *Adu, *Asu, *deu, *omu, *Adv, *Asv, *dev_, *omv, *rsu, *rsv, *zsu, *zsv, *gam;
+    float* a32 = nullptr;
+    float *Adu32, *Asu32, *deu32, *omu32, *Adv32, *Asv32, *dev32, *omv32, *rsu32, *rsv32, *zsu32, *zsv32, *gam32;
+    int mode = 0;            // 3 or 4 when set up
+};
+
+}  // namespace
+
+struct cismgpu_system {
+    Dims d{};
+    hipStream_t s = nullptr;
+    int8_t* active = nullptr;      // nvx*nvy
+    double* A64 = nullptr;
+    float* A32 = nullptr;
+    __hip_bfloat16* A16 = nullptr;
+    double* stage = nullptr;       // 2 raw blocks (27*nz*nvx*nvy each), kept for precond setup
+    double* M64 = nullptr;         // 6 vectors (SIA) + 2 (diag)
+    float* M32 = nullptr;
+    double* v64 = nullptr;         // vectors: 0 xu 1 xv 2 bu 3 bv 4 ru 5 rv 6 du 7 dv 8 qu 9 qv 10 zu 11 zv 12 Azu 13 Azv
+    float* v32 = nullptr;          // inner: 0 eu 1 ev 2 r0u 3 r0v 4 ru 5 rv 6 du 7 dv 8 qu 9 qv 10 zu 11 zv 12 Azu 13 Azv
+    double* partA = nullptr;
+    double* partB = nullptr;
+    double* scal = nullptr;
+    double* scal_h = nullptr;      // pinned mirror
+    TriPC tri;
+    bool have_matrix = false;
+    bool m32_ready = false, a32_ready = false, a16_ready = false;
+    std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
+    int ev_used = 0;
+    bool timing = false;
+    int spmv_kind_timed = 0;
+
+    double* V(int q) { return v64 + (size_t)q * d.vlen; }
+    float* W(int q) { return v32 + (size_t)q * d.vlen; }
+};
+
+namespace {
+
+int check_grid(const cismgpu_grid* g) {
+    if (!g) { cismgpu_set_error("null grid"); return CISMGPU_ERR_ARG; }
+    if (g->nx < 6 || g->ny < 6 || g->nz < 2 || g->nz > 64) {
+        cismgpu_set_error("bad grid sizes nx=%d ny=%d nz=%d (nz must be 2..64)", g->nx, g->ny, g->nz);
+        return CISMGPU_ERR_ARG;
+    }
+    if (g->ilo < 2 || g->ihi > g->nx - 2 || g->jlo < 2 || g->jhi > g->ny - 2 || g->ilo > g->ihi ||
+        g->jlo > g->jhi) {
+        cismgpu_set_error("owned bounds (%d:%d, %d:%d) outside the halo rules", g->ilo, g->ihi, g->jlo, g->jhi);
+        return CISMGPU_ERR_ARG;
+    }
+    if (g->global_bc < 0 || g->global_bc > 3) { cismgpu_set_error("bad global_bc %d", g->global_bc); return CISMGPU_ERR_ARG; }
+    if ((g->global_bc == 0 || g->global_bc == 2) && (g->ilo != 3 || g->jlo != 3 || g->ihi != g->nx - 2 || g->jhi != g->ny - 2)) {
+        cismgpu_set_error("periodic halo rules need the one-task bounds 3..n-2");
+        return CISMGPU_ERR_ARG;
+    }
+    return CISMGPU_OK;
+}
+
+Dims make_dims(const cismgpu_grid* g) {
+    Dims d{};
+    d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
+    d.nvx = g->nx - 1; d.nvy = g->ny - 1;
+    d.ilo = g->ilo; d.ihi = g->ihi; d.jlo = g->jlo; d.jhi = g->jhi;
+    d.nxo = d.ihi - d.ilo + 1; d.nyo = d.jhi - d.jlo + 1;
+    d.nxp = (d.nxo + WAVE - 1) / WAVE * WAVE;
+    d.px = (d.nvx + 7) / 8 * 8;
+    d.bc = g->global_bc;
+    d.plane = (long long)d.nvy * d.px;
+    d.vlen = (long long)(d.nz + 2) * d.plane;
+    d.nrows = (long long)d.nz * d.nyo * d.nxp;
+    return d;
+}
+
+inline int spmv_blocks(const Dims& d) { return (d.nxp / WAVE) * ((d.nyo + 3) / 4); }
+inline int col_blocks(const Dims& d) { return (int)nblocks((long long)d.nvx * d.nvy, 256); }
+
+template <typename TV>
+void halo(cismgpu_system* S, TV* a, TV* b, int nlev) {
+    const Dims& d = S->d;
+    if (d.bc == 0 || d.bc == 2) {
+        const int nj = d.jhi - d.jlo + 1;
+        k_halo_ew<TV><<<nblocks((long long)nlev * nj, 256), 256, 0, S->s>>>(d, a, b, nlev);
+        k_halo_ns<TV><<<nblocks((long long)nlev * d.nvx, 256), 256, 0, S->s>>>(d, a, b, nlev);
+    } else if (d.bc == 1) {
+        k_halo_zero<TV><<<col_blocks(d), 256, 0, S->s>>>(d, a, b, nlev, 1, 1);
+    } else if (d.bc == 3) {
+        k_halo_zero<TV><<<col_blocks(d), 256, 0, S->s>>>(d, a, b, nlev, 2, 2);
+    }
+}
+
+template <typename TA, typename TV, int MODE>
+void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV* yv, const TV* bu, const TV* bv,
+          double* part) {
+    const bool t = S->timing && S->ev_used + 2 <= (int)S->ev.size();
+    if (t) hipEventRecord(S->ev[S->ev_used], S->s);
+    k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    if (t) { hipEventRecord(S->ev[S->ev_used + 1], S->s); S->ev_used += 2; }
+}
+
+template <typename TV, int PRE, int STD>
+void launch_update(cismgpu_system* S, TV* const* v, const TV* M) {
+    // v: xu xv ru rv du dv qu qv zu zv Azu Azv
+    const Dims& d = S->d;
+#define UPD(NZM) k_update<TV, NZM, PRE, STD><<<col_blocks(d), 256, 0, S->s>>>(d, S->scal, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11], M, S->active, S->partA)
+    if (d.nz <= 16) UPD(16);
+    else if (d.nz <= 32) UPD(32);
+    else UPD(64);
+#undef UPD
+}
+
+template <typename TV>
+void launch_precond(cismgpu_system* S, int kind, const TV* ru, const TV* rv, TV* zu, TV* zv, const TV* M,
+                    const TV* Dg, const TV* zsu, const TV* zsv) {
+    const Dims& d = S->d;
+#define PRC(NZM) k_precond<TV, NZM><<<col_blocks(d), 256, 0, S->s>>>(d, kind, S->scal, ru, rv, zu, zv, M, Dg, S->active, zsu, zsv, S->partA)
+    if (d.nz <= 16) PRC(16);
+    else if (d.nz <= 32) PRC(32);
+    else PRC(64);
+#undef PRC
+}
+
+void scalars(cismgpu_system* S, int stage, const double* pa, int na, int sa, const double* pb, int nb, int sb,
+             double tol = 0.0) {
+    k_scalars<<<1, RED_THREADS, 0, S->s>>>(stage, pa, na, sa, pb, nb, sb, S->scal, tol);
+}
+
+// dot over owned vertices of (a1,b1)+(a2,b2), compensated, into partB (stride 2)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_dot(Dims d, const TV* __restrict__ a1, const TV* __restrict__ b1,
+                                             const TV* __restrict__ a2, const TV* __restrict__ b2,
+                                             double* __restrict__ partials) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    dd acc = {0.0, 0.0};
+    if (t < (long long)d.nxo * d.nyo) {
+        const int i = d.ilo + (int)(t % d.nxo), j = d.jlo + (int)(t / d.nxo);
+        for (int k = 0; k < d.nz; ++k) {
+            const long long c = vidx(d, k, i, j);
+            acc = dd_add_d(acc, (double)a1[c] * (double)b1[c]);
+            acc = dd_add_d(acc, (double)a2[c] * (double)b2[c]);
+        }
+    }
+    dd v[1] = {acc};
+    block_reduce_dd<1, 256>(v);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = v[0].hi; partials[2 * blockIdx.x + 1] = v[0].lo; }
+}
+
+template <typename TV>
+int dot_blocks(const Dims& d) { return (int)nblocks((long long)d.nxo * d.nyo, 256); }
+
+int sync_scal(cismgpu_system* S) {
+    CG_CHECK(hipMemcpyAsync(S->scal_h, S->scal, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+int set_scal(cismgpu_system* S, int slot, double v) {
+    CG_CHECK(hipMemcpyAsync(S->scal + slot, &v, sizeof(double), hipMemcpyHostToDevice, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+int reset_scal(cismgpu_system* S) {
+    CG_CHECK(hipMemsetAsync(S->scal, 0, S_NSLOTS * sizeof(double), S->s));
+    return CISMGPU_OK;
+}
+
+// ---------------------------------------------------------------- tridiagonal PC setup
+// Vertical sums of the 2-D stencil entries used by the line solves
+// (pcg.F90:3815-3840 local; :3996-4021 global) from the raw reference blocks.
+__global__ void k_tri_vsum(Dims d, int mode, const double* __restrict__ Auu, const double* __restrict__ Avv,
+                           const int8_t* __restrict__ active, double* __restrict__ su_s, double* __restrict__ su_d,
+                           double* __restrict__ su_p, double* __restrict__ sv_s, double* __restrict__ sv_d,
+                           double* __restrict__ sv_p) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    // 2-D indices: (-1,0)->4, (0,0)->5, (1,0)->6, (0,-1)->2, (0,1)->8 (1-based)
+    const int mu[3] = {3, 4, 5}, mv[3] = {1, 4, 7};
+    double u[3], v[3];
+    for (int q = 0; q < 3; ++q) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < d.nz; ++k) {
+            const double* pu = Auu + 27 * ((long long)k + (long long)d.nz * t);
+            const double* pv = Avv + 27 * ((long long)k + (long long)d.nz * t);
+            a = a + pu[mu[q]] + pu[mu[q] + 9] + pu[mu[q] + 18];
+            b = b + pv[mv[q]] + pv[mv[q] + 9] + pv[mv[q] + 18];
+        }
+        u[q] = a; v[q] = b;
+    }
+    if (!active[t]) {
+        if (mode == 3) { u[0] = u[2] = v[0] = v[2] = 0.0; }
+        u[1] = 1.0; v[1] = 1.0;
+    }
+    su_s[t] = u[0]; su_d[t] = u[1]; su_p[t] = u[2];
+    sv_s[t] = v[0]; sv_d[t] = v[1]; sv_p[t] = v[2];
+}
+
+// Forward factors of the owned-block line systems (pcg.F90:4313-4390 local;
+// :4446-4517 global, which also forms the reverse sweep whose zero pivots
+// are fatal).  One thread per line.  In: sub, diag, sup (2-D).  Out: Ad =
+// 1/diag, de = 1/denom, om; *flag = 1 on a zero pivot.
+__global__ void k_tri_factor(Dims d, int dir, int mode, double* __restrict__ sub, double* __restrict__ diag,
+                             double* __restrict__ sup, double* __restrict__ Ad, double* __restrict__ de,
+                             double* __restrict__ om, double* __restrict__ flagp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = d.nvx;
+    int lo, hi, fixed;
+    if (dir == 0) { fixed = d.jlo + t; if (fixed > d.jhi) return; lo = d.ilo; hi = d.ihi; }
+    else { fixed = d.ilo + t; if (fixed > d.ihi) return; lo = d.jlo; hi = d.jhi; }
+    auto id = [&](int s) -> long long {
+        return dir == 0 ? (long long)(fixed - 1) * nv + (s - 1) : (long long)(s - 1) * nv + (fixed - 1);
+    };
+    const int a0 = (mode == 3) ? 1 : lo, a1 = (mode == 3) ? (dir == 0 ? d.nvx : d.nvy) : hi;
+    for (int s = a0; s <= a1; ++s) {
+        if (diag[id(s)] == 0.0) { diag[id(s)] = 1.0; sub[id(s)] = 0.0; sup[id(s)] = 0.0; }
+        om[id(s)] = 0.0; de[id(s)] = 0.0;
+    }
+    om[id(lo)] = sup[id(lo)] / diag[id(lo)];
+    for (int s = lo + 1; s <= hi; ++s) {
+        const double den = diag[id(s)] - sub[id(s)] * om[id(s - 1)];
+        if (den == 0.0) { *flagp = 1.0; return; }
+        de[id(s)] = 1.0 / den;
+        om[id(s)] = sup[id(s)] * de[id(s)];
+    }
+    if (mode == 4) {
+        double omuh = sub[id(hi)] / diag[id(hi)];
+        for (int s = hi - 1; s >= lo; --s) {
+            const double duh = diag[id(s)] - sup[id(s)] * omuh;
+            if (duh == 0.0) { *flagp = 1.0; return; }
+            omuh = sub[id(s)] / duh;
+        }
+    }
+    for (int s = a0; s <= a1; ++s) Ad[id(s)] = 1.0 / diag[id(s)];
+}
+
+int setup_tri(cismgpu_system* S, int mode) {
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    TriPC& T = S->tri;
+    if (!T.a) {
+        CG_CHECK(hipMalloc(&T.a, 13 * n2 * sizeof(double)));
+        CG_CHECK(hipMalloc(&T.a32, 13 * n2 * sizeof(float)));
+        double** f[13] = {&T.Adu, &T.Asu, &T.deu, &T.omu, &T.Adv, &T.Asv, &T.dev_, &T.omv, &T.rsu, &T.rsv, &T.zsu, &T.zsv, &T.gam};
+        float** g[13] = {&T.Adu32, &T.Asu32, &T.deu32, &T.omu32, &T.Adv32, &T.Asv32, &T.dev32, &T.omv32, &T.rsu32, &T.rsv32, &T.zsu32, &T.zsv32, &T.gam32};
+        for (int q = 0; q < 13; ++q) { *f[q] = T.a + q * n2; *g[q] = T.a32 + q * n2; }
+    }
+    CG_CHECK(hipMemsetAsync(T.a, 0, 13 * n2 * sizeof(double), S->s));
+    // scratch for sums: use rsu.. as sub/diag/sup holders
+    double *su_s = T.rsu, *su_d = T.rsv, *su_p = T.zsu, *sv_s = T.zsv, *sv_d = T.gam, *sv_p = T.Asv;
+    // stage[0] holds Auu, stage[1] Avv (raw layout)
+    const long long nraw = 27LL * d.nz * d.nvx * d.nvy;
+    k_tri_vsum<<<nblocks(n2, 256), 256, 0, S->s>>>(d, mode, S->stage, S->stage + nraw, S->active, su_s, su_d, su_p,
+                                                  sv_s, sv_d, sv_p);
+    if (mode == 4) {
+        k_halo2d<double><<<1, 256, 0, S->s>>>(d, su_s, su_d);
+        k_halo2d<double><<<1, 256, 0, S->s>>>(d, su_p, sv_s);
+        k_halo2d<double><<<1, 256, 0, S->s>>>(d, sv_d, sv_p);
+    }
+    // u lines: As_u = sub (copy), factor into Adu, deu, omu
+    CG_CHECK(hipMemcpyAsync(T.Asu, su_s, n2 * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+    CG_CHECK(hipMemsetAsync(S->scal + S_DOT2, 0, sizeof(double), S->s));
+    k_tri_factor<<<nblocks(d.nyo, 64), 64, 0, S->s>>>(d, 0, mode, T.Asu, su_d, su_p, T.Adu, T.deu, T.omu, S->scal + S_DOT2);
+    // v lines (sv_p lives in T.Asv: move it out first)
+    double* tmp_p = T.rsu;   // su_s no longer needed
+    CG_CHECK(hipMemcpyAsync(tmp_p, sv_p, n2 * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+    CG_CHECK(hipMemcpyAsync(T.Asv, sv_s, n2 * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+    k_tri_factor<<<nblocks(d.nxo, 64), 64, 0, S->s>>>(d, 1, mode, T.Asv, sv_d, tmp_p, T.Adv, T.dev_, T.omv, S->scal + S_DOT2);
+    {
+        const int st0 = sync_scal(S);
+        if (st0) return st0;
+    }
+    if (S->scal_h[S_DOT2] != 0.0) { cismgpu_set_error("zero pivot in tridiagonal preconditioner setup"); return CISMGPU_ERR_DIVZERO; }
+    // FP32 copies of the factors
+    k_cast<double, float><<<nblocks(8 * n2, 256), 256, 0, S->s>>>(8 * n2, T.a, T.a32);
+    CG_CHECK(hipMemsetAsync(T.rsu, 0, 5 * n2 * sizeof(double), S->s));
+    T.mode = mode;
+    return CISMGPU_OK;
+}
+
+// precond 3/4 horizontal part: vsum(r) -> line solves -> 2-D halo -> zs
+template <typename TV>
+void tri_horizontal(cismgpu_system* S, const TV* ru, const TV* rv, TV** zsu_out, TV** zsv_out) {
+    const Dims& d = S->d;
+    TriPC& T = S->tri;
+    TV *Adu, *Asu, *deu, *omu, *Adv, *Asv, *dev_, *omv, *rsu, *rsv, *zsu, *zsv, *gam;
+    if constexpr (std::is_same<TV, double>::value) {
+        Adu = T.Adu; Asu = T.Asu; deu = T.deu; omu = T.omu; Adv = T.Adv; Asv = T.Asv; dev_ = T.dev_; omv = T.omv;
+        rsu = T.rsu; rsv = T.rsv; zsu = T.zsu; zsv = T.zsv; gam = T.gam;
+    } else {
+        Adu = T.Adu32; Asu = T.Asu32; deu = T.deu32; omu = T.omu32; Adv = T.Adv32; Asv = T.Asv32; dev_ = T.dev32;
+        omv = T.omv32; rsu = T.rsu32; rsv = T.rsv32; zsu = T.zsu32; zsv = T.zsv32; gam = T.gam32;
+    }
+    k_vsum<TV><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, rsu, rsv);
+    k_line_solve<TV><<<nblocks(d.nyo, 64), 64, 0, S->s>>>(d, 0, 1, Adu, Asu, deu, omu, rsu, zsu, gam);
+    k_line_solve<TV><<<nblocks(d.nxo, 64), 64, 0, S->s>>>(d, 1, 1, Adv, Asv, dev_, omv, rsv, zsv, gam);
+    k_halo2d<TV><<<1, 256, 0, S->s>>>(d, zsu, zsv);
+    *zsu_out = zsu;
+    *zsv_out = zsv;
+}
+
+// z = PC(r) with partials (r,z), (r,r) in partA
+template <typename TV>
+void apply_precond(cismgpu_system* S, int precond, const TV* ru, const TV* rv, TV* zu, TV* zv) {
+    const Dims& d = S->d;
+    const TV* M;
+    const TV* Dg;
+    if constexpr (std::is_same<TV, double>::value) { M = S->M64; Dg = S->M64 + 6 * d.vlen; }
+    else { M = S->M32; Dg = S->M32 + 6 * d.vlen; }
+    TV *zsu = nullptr, *zsv = nullptr;
+    if (precond >= 3) tri_horizontal<TV>(S, ru, rv, &zsu, &zsv);
+    launch_precond<TV>(S, precond, ru, rv, zu, zv, M, Dg, zsu, zsv);
+}
+
+int finish_timing(cismgpu_system* S, cismgpu_solve_info* info) {
+    if (!S->timing) return CISMGPU_OK;
+    CG_CHECK(hipStreamSynchronize(S->s));
+    double tot = 0.0;
+    for (int e = 0; e + 1 < S->ev_used; e += 2) {
+        float ms = 0.f;
+        CG_CHECK(hipEventElapsedTime(&ms, S->ev[e], S->ev[e + 1]));
+        tot += ms;
+    }
+    if (info) {
+        info->spmv_ms += tot;
+        info->spmv_count += S->ev_used / 2;
+    }
+    S->ev_used = 0;
+    return CISMGPU_OK;
+}
+
+// Chronopoulos-Gear solve, reference semantics (pcg.F90:2632-3594), on
+// vectors of type TV with matrix storage TA.  x, r etc. point into the
+// system's vector pools.  With `given_r` the initial residual is supplied in
+// r (x = 0), as in the inner solve of iterative refinement, and the
+// convergence test uses the recurrence residual relative to ||r0||.
+template <typename TA, typename TV>
+int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const TV* bv, int precond, int ncheck,
+             double tol, int maxiters, bool given_r, double* err, int* niters, bool fuse_ok) {
+    const Dims& d = S->d;
+    TV *xu = v[0], *xv = v[1], *ru = v[2], *rv = v[3], *du = v[4], *dv = v[5], *qu = v[6], *qv = v[7];
+    TV *zu = v[8], *zv = v[9], *Azu = v[10], *Azv = v[11];
+    const TV* M = std::is_same<TV, double>::value ? (const TV*)S->M64 : (const TV*)S->M32;
+    const int nsp = spmv_blocks(d), ncb = col_blocks(d);
+    const long long nv = d.vlen * sizeof(TV);
+    int st = reset_scal(S);
+    if (st) return st;
+    for (TV* p : {du, dv, zu, zv, qu, qv, Azu, Azv}) CG_CHECK(hipMemsetAsync(p, 0, nv, S->s));
+    *niters = maxiters;
+    // L2 norm of the rhs (or of r0 for the inner solve)
+    k_dot<TV><<<dot_blocks<TV>(d), 256, 0, S->s>>>(d, given_r ? ru : bu, given_r ? ru : bu, given_r ? rv : bv,
+                                                   given_r ? rv : bv, S->partB);
+    scalars(S, ST_BB, S->partB, dot_blocks<TV>(d), 2, nullptr, 0, 0);
+    if (!given_r) {
+        CG_CHECK(hipMemsetAsync(ru, 0, nv, S->s));
+        CG_CHECK(hipMemsetAsync(rv, 0, nv, S->s));
+        halo<TV>(S, xu, xv, d.nz);
+        spmv<TA, TV, 2>(S, A, xu, xv, ru, rv, bu, bv, S->partB);
+        k_ring<TV><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, bu, bv);
+    }
+    halo<TV>(S, ru, rv, d.nz);
+    apply_precond<TV>(S, precond, ru, rv, zu, zv);
+    k_axpy_ops<TV><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, 1, S->scal, zu, zv, du, dv, nullptr, nullptr);
+    spmv<TA, TV, 1>(S, A, du, dv, qu, qv, nullptr, nullptr, S->partB);
+    scalars(S, ST_CG_FIRST, S->partA, ncb, 4, S->partB, nsp, 2);
+    halo<TV>(S, qu, qv, d.nz);
+    k_first_update<TV><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, S->scal, xu, xv, ru, rv, du, dv, qu, qv);
+    TV* vv[12] = {xu, xv, ru, rv, du, dv, qu, qv, zu, zv, Azu, Azv};
+    bool z_ready = false;
+    for (int iter = 2; iter <= maxiters; ++iter) {
+        if (!z_ready) apply_precond<TV>(S, precond, ru, rv, zu, zv);
+        spmv<TA, TV, 1>(S, A, zu, zv, Azu, Azv, nullptr, nullptr, S->partB);
+        scalars(S, ST_CG_ITER, S->partA, ncb, 4, S->partB, nsp, 2);
+        halo<TV>(S, Azu, Azv, d.nz);
+        const bool check = (iter % ncheck == 0) || iter == 5;
+        const bool fuse = fuse_ok && precond == 2 && !(check && !given_r);
+        if (fuse) launch_update<TV, 1, 0>(S, vv, M);
+        else launch_update<TV, 0, 0>(S, vv, M);
+        z_ready = fuse;
+        if (check) {
+            if (given_r) {
+                // recurrence residual (r,r) from the fused update's partials
+                if (!fuse) {
+                    apply_precond<TV>(S, precond, ru, rv, zu, zv);
+                    z_ready = true;
+                }
+                scalars(S, ST_RR, S->partA + 2, ncb, 4, nullptr, 0, 0, tol);
+            } else {
+                spmv<TA, TV, 2>(S, A, xu, xv, ru, rv, bu, bv, S->partB);
+                k_ring<TV><<<ncb, 256, 0, S->s>>>(d, ru, rv, bu, bv);
+                scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+            }
+            st = sync_scal(S);
+            if (st) return st;
+            if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Chron-Gear PCG solver has failed, alpha = NaN"); return CISMGPU_ERR_NAN; }
+            *err = S->scal_h[S_ERR];
+            if (S->scal_h[S_CONV] != 0.0) { *niters = iter; break; }
+            if (!given_r) halo<TV>(S, ru, rv, d.nz);
+        }
+    }
+    st = sync_scal(S);
+    if (st) return st;
+    if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Chron-Gear PCG solver has failed, alpha = NaN"); return CISMGPU_ERR_NAN; }
+    return CISMGPU_OK;
+}
+
+// Standard PCG (pcg.F90:827-1537), FP64.
+int std_solve(cismgpu_system* S, int precond, int ncheck, double tol, int maxiters, double* err, int* niters) {
+    const Dims& d = S->d;
+    double *xu = S->V(0), *xv = S->V(1), *bu = S->V(2), *bv = S->V(3), *ru = S->V(4), *rv = S->V(5);
+    double *du = S->V(6), *dv = S->V(7), *qu = S->V(8), *qv = S->V(9), *zu = S->V(10), *zv = S->V(11);
+    const int nsp = spmv_blocks(d), ncb = col_blocks(d);
+    const long long nv = d.vlen * sizeof(double);
+    int st = reset_scal(S);
+    if (st) return st;
+    for (double* p : {ru, rv, du, dv, zu, zv, qu, qv}) CG_CHECK(hipMemsetAsync(p, 0, nv, S->s));
+    halo<double>(S, xu, xv, d.nz);
+    spmv<double, double, 2>(S, S->A64, xu, xv, ru, rv, bu, bv, S->partB);
+    k_ring<double><<<ncb, 256, 0, S->s>>>(d, ru, rv, bu, bv);
+    *niters = maxiters;
+    st = set_scal(S, S_ETA0, 1.0);
+    if (st) return st;
+    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
+    scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    double* vv[12] = {xu, xv, ru, rv, du, dv, qu, qv, zu, zv, nullptr, nullptr};
+    bool z_ready = false;
+    for (int iter = 1; iter <= maxiters; ++iter) {
+        if (!z_ready) apply_precond<double>(S, precond, ru, rv, zu, zv);
+        scalars(S, ST_STD_ETA1, S->partA, ncb, 4, nullptr, 0, 0);
+        k_axpy_ops<double><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, 0, S->scal, zu, zv, du, dv, nullptr, nullptr);
+        halo<double>(S, du, dv, d.nz);
+        spmv<double, double, 1>(S, S->A64, du, dv, qu, qv, nullptr, nullptr, S->partB);
+        k_ring<double><<<ncb, 256, 0, S->s>>>(d, qu, qv, nullptr, nullptr);
+        scalars(S, ST_STD_ETA2, S->partB, nsp, 2, nullptr, 0, 0);
+        const bool check = (iter % ncheck == 0) || iter == 5;
+        const bool fuse = precond == 2 && !check;
+        if (fuse) launch_update<double, 1, 1>(S, vv, S->M64);
+        else launch_update<double, 0, 1>(S, vv, S->M64);
+        z_ready = fuse;
+        if (check) {
+            halo<double>(S, xu, xv, d.nz);
+            spmv<double, double, 2>(S, S->A64, xu, xv, ru, rv, bu, bv, S->partB);
+            k_ring<double><<<ncb, 256, 0, S->s>>>(d, ru, rv, bu, bv);
+            scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+            st = sync_scal(S);
+            if (st) return st;
+            if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("PCG solver has failed, alpha = NaN"); return CISMGPU_ERR_NAN; }
+            *err = S->scal_h[S_ERR];
+            if (S->scal_h[S_CONV] != 0.0) { *niters = iter; break; }
+        }
+    }
+    st = sync_scal(S);
+    if (st) return st;
+    if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("PCG solver has failed, alpha = NaN"); return CISMGPU_ERR_NAN; }
+    return CISMGPU_OK;
+}
+
+int ensure_lowp(cismgpu_system* S, int precision) {
+    const Dims& d = S->d;
+    const long long na = d.nrows * NCOEF;
+    if (!S->m32_ready) {
+        k_cast<double, float><<<nblocks(8 * d.vlen, 256), 256, 0, S->s>>>(8 * d.vlen, S->M64, S->M32);
+        S->m32_ready = true;
+    }
+    if (precision == CISMGPU_MIXED_FP32 && !S->a32_ready) {
+        if (!S->A32) CG_CHECK(hipMalloc(&S->A32, na * sizeof(float)));
+        k_convert_matrix<float><<<nblocks(na, 256), 256, 0, S->s>>>(na, S->A64, S->A32);
+        S->a32_ready = true;
+    }
+    if (precision == CISMGPU_MIXED_BF16 && !S->a16_ready) {
+        if (!S->A16) CG_CHECK(hipMalloc(&S->A16, na * sizeof(__hip_bfloat16)));
+        k_convert_matrix<__hip_bfloat16><<<nblocks(na, 256), 256, 0, S->s>>>(na, S->A64, S->A16);
+        S->a16_ready = true;
+    }
+    if (S->tri.mode && !S->tri.a32) return CISMGPU_ERR_ARG;
+    return CISMGPU_OK;
+}
+
+// Mixed precision: FP64 iterative refinement around an FP32 (or BF16-matrix)
+// Chronopoulos-Gear inner solve.  The outer residual r = b - A x uses the
+// FP64 matrix and compensated FP64 sums; the correction e solves A e = r in
+// FP32 to inner_tol relative (recurrence residual), then x += e.
+int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_info* info) {
+    const Dims& d = S->d;
+    int st = ensure_lowp(S, o->precision);
+    if (st) return st;
+    double *xu = S->V(0), *xv = S->V(1), *bu = S->V(2), *bv = S->V(3), *ru = S->V(4), *rv = S->V(5);
+    float* w[12] = {S->W(0), S->W(1), S->W(4), S->W(5), S->W(6), S->W(7), S->W(8), S->W(9), S->W(10), S->W(11), S->W(12), S->W(13)};
+    const int nsp = spmv_blocks(d);
+    const long long nvf = d.vlen * sizeof(float);
+    // ||b|| once
+    st = reset_scal(S);
+    if (st) return st;
+    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
+    scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    st = sync_scal(S);
+    if (st) return st;
+    const double l2rhs = S->scal_h[S_L2RHS];
+    int inner_total = 0, outer = 0;
+    double err = INFINITY;
+    const double inner_tol = o->inner_tol > 0 ? o->inner_tol : 1e-4;
+    for (outer = 0; outer <= std::max(o->max_outer, 1); ++outer) {
+        CG_CHECK(hipMemsetAsync(ru, 0, d.vlen * sizeof(double), S->s));
+        CG_CHECK(hipMemsetAsync(rv, 0, d.vlen * sizeof(double), S->s));
+        halo<double>(S, xu, xv, d.nz);
+        const int tspmv = S->timing;
+        S->timing = false;   // time the inner (dominant) SpMV only
+        spmv<double, double, 2>(S, S->A64, xu, xv, ru, rv, bu, bv, S->partB);
+        S->timing = tspmv;
+        k_ring<double><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, bu, bv);
+        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, nsp, 2, nullptr, 0, 0, S->scal, 0.0);
+        st = sync_scal(S);
+        if (st) return st;
+        err = std::sqrt(S->scal_h[S_DOT]) / l2rhs;
+        if (!(err == err)) { cismgpu_set_error("refinement residual is NaN"); return CISMGPU_ERR_NAN; }
+        if (err < o->tolerance || outer == std::max(o->max_outer, 1)) break;
+        // inner solve A e = r in FP32
+        k_cast<double, float><<<nblocks(d.vlen, 256), 256, 0, S->s>>>(d.vlen, ru, w[2]);
+        k_cast<double, float><<<nblocks(d.vlen, 256), 256, 0, S->s>>>(d.vlen, rv, w[3]);
+        CG_CHECK(hipMemsetAsync(w[0], 0, nvf, S->s));
+        CG_CHECK(hipMemsetAsync(w[1], 0, nvf, S->s));
+        double ierr = 0.0;
+        int nit = 0;
+        if (o->precision == CISMGPU_MIXED_BF16)
+            st = cg_solve<__hip_bfloat16, float>(S, S->A16, w, nullptr, nullptr, o->precond, o->ncheck, inner_tol,
+                                                 o->maxiters, true, &ierr, &nit, true);
+        else
+            st = cg_solve<float, float>(S, S->A32, w, nullptr, nullptr, o->precond, o->ncheck, inner_tol,
+                                        o->maxiters, true, &ierr, &nit, true);
+        if (st) return st;
+        inner_total += nit;
+        k_ir_update<<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, xu, xv, w[0], w[1]);
+    }
+    info->err = err;
+    info->niters = inner_total;
+    info->outer_iters = outer;
+    info->spmv_kind = o->precision == CISMGPU_MIXED_BF16 ? 2 : 1;
+    if (!(err < o->tolerance)) {
+        cismgpu_set_error("iterative refinement stopped at relative residual %.3e > %.3e", err, o->tolerance);
+        return CISMGPU_ERR_NOT_CONVERGED;
+    }
+    return CISMGPU_OK;
+}
+
+int check_opts(const cismgpu_solve_opts* o) {
+    if (!o || o->ncheck < 1 || o->maxiters < 1 || o->precond < 0 || o->precond > 4 ||
+        (o->algo != CISMGPU_ALGO_STANDARD && o->algo != CISMGPU_ALGO_CHRONGEAR) || o->precision < 0 ||
+        o->precision > 2) {
+        cismgpu_set_error("bad solve options");
+        return CISMGPU_ERR_ARG;
+    }
+    if (o->precision != CISMGPU_FP64 && (o->algo != CISMGPU_ALGO_CHRONGEAR)) {
+        cismgpu_set_error("mixed precision uses the Chronopoulos-Gear inner solver");
+        return CISMGPU_ERR_ARG;
+    }
+    return CISMGPU_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out) {
+    if (!out) return CISMGPU_ERR_ARG;
+    *out = nullptr;
+    int st = check_grid(grid);
+    if (st) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        cismgpu_set_error("no HIP device available");
+        return CISMGPU_ERR_HIP;
+    }
+    cismgpu_system* S = new cismgpu_system();
+    S->d = make_dims(grid);
+    const Dims& d = S->d;
+    auto fail = [&](hipError_t e, const char* what) {
+        cismgpu_set_error("%s: %s", what, hipGetErrorString(e));
+        cismgpu_system_destroy(S);
+        return CISMGPU_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    const long long n2 = (long long)d.nvx * d.nvy;
+    const long long na = d.nrows * NCOEF;
+    const int nparts = std::max(spmv_blocks(d), std::max(col_blocks(d), dot_blocks<double>(d))) + 8;
+    if ((e = hipMalloc(&S->active, n2)) != hipSuccess) return fail(e, "active");
+    if ((e = hipMalloc(&S->A64, na * sizeof(double))) != hipSuccess) return fail(e, "A64");
+    if ((e = hipMalloc(&S->M64, 8 * d.vlen * sizeof(double))) != hipSuccess) return fail(e, "M64");
+    if ((e = hipMalloc(&S->M32, 8 * d.vlen * sizeof(float))) != hipSuccess) return fail(e, "M32");
+    if ((e = hipMalloc(&S->v64, 14 * d.vlen * sizeof(double))) != hipSuccess) return fail(e, "v64");
+    if ((e = hipMalloc(&S->v32, 14 * d.vlen * sizeof(float))) != hipSuccess) return fail(e, "v32");
+    if ((e = hipMalloc(&S->partA, 4 * nparts * sizeof(double))) != hipSuccess) return fail(e, "partA");
+    if ((e = hipMalloc(&S->partB, 4 * nparts * sizeof(double))) != hipSuccess) return fail(e, "partB");
+    if ((e = hipMalloc(&S->scal, S_NSLOTS * sizeof(double))) != hipSuccess) return fail(e, "scal");
+    if ((e = hipHostMalloc(&S->scal_h, S_NSLOTS * sizeof(double), hipHostMallocDefault)) != hipSuccess) return fail(e, "scal_h");
+    hipMemsetAsync(S->v64, 0, 14 * d.vlen * sizeof(double), S->s);
+    hipMemsetAsync(S->v32, 0, 14 * d.vlen * sizeof(float), S->s);
+    hipMemsetAsync(S->M64, 0, 8 * d.vlen * sizeof(double), S->s);
+    hipMemsetAsync(S->A64, 0, na * sizeof(double), S->s);
+    hipMemsetAsync(S->active, 0, n2, S->s);
+    if ((e = hipStreamSynchronize(S->s)) != hipSuccess) return fail(e, "init");
+    *out = S;
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_destroy(cismgpu_system* S) {
+    if (!S) return CISMGPU_OK;
+    if (S->s) hipStreamSynchronize(S->s);
+    for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
+                    (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
+                    (void*)S->tri.a, (void*)S->tri.a32})
+        if (p) hipFree(p);
+    if (S->scal_h) hipHostFree(S->scal_h);
+    for (auto& x : S->ev) hipEventDestroy(x);
+    if (S->s) hipStreamDestroy(S->s);
+    delete S;
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_set_matrix(cismgpu_system* S, const int32_t* active_vertex, const double* Auu,
+                                  const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                  const double* bv, int32_t where) {
+    if (!S || !active_vertex || !Auu || !Auv || !Avu || !Avv || !bu || !bv) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    const long long nraw = 27LL * d.nz * n2;
+    const long long nvec = (long long)d.nz * n2;
+    const hipMemcpyKind kind = where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
+    // active mask (int32 -> int8)
+    std::vector<int8_t> act(n2);
+    if (where) {
+        std::vector<int32_t> tmp(n2);
+        CG_CHECK(hipMemcpy(tmp.data(), active_vertex, n2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n2; ++i) act[i] = tmp[i] ? 1 : 0;
+    } else {
+        for (long long i = 0; i < n2; ++i) act[i] = active_vertex[i] ? 1 : 0;
+    }
+    CG_CHECK(hipMemcpyAsync(S->active, act.data(), n2, hipMemcpyHostToDevice, S->s));
+    double* s0 = S->stage;
+    double* s1 = S->stage + nraw;
+    const unsigned gp = nblocks(d.nrows * 27, 256);
+    // Auv, Avu first through the staging pair, then Auu/Avv which stay staged
+    CG_CHECK(hipMemcpyAsync(s0, Auv, nraw * sizeof(double), kind, S->s));
+    CG_CHECK(hipMemcpyAsync(s1, Avu, nraw * sizeof(double), kind, S->s));
+    k_pack_matrix<double><<<gp, 256, 0, S->s>>>(d, 1, s0, S->active, S->A64);
+    k_pack_matrix<double><<<gp, 256, 0, S->s>>>(d, 2, s1, S->active, S->A64);
+    CG_CHECK(hipMemcpyAsync(s0, Auu, nraw * sizeof(double), kind, S->s));
+    CG_CHECK(hipMemcpyAsync(s1, Avv, nraw * sizeof(double), kind, S->s));
+    k_pack_matrix<double><<<gp, 256, 0, S->s>>>(d, 0, s0, S->active, S->A64);
+    k_pack_matrix<double><<<gp, 256, 0, S->s>>>(d, 3, s1, S->active, S->A64);
+    k_setup_precond<double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, s0, s1, S->M64, S->M64 + 6 * d.vlen);
+    // rhs
+    double* tmp = S->V(4);   // r vectors as staging for b
+    CG_CHECK(hipMemcpyAsync(tmp, bu, nvec * sizeof(double), kind, S->s));
+    k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(2));
+    CG_CHECK(hipMemcpyAsync(tmp, bv, nvec * sizeof(double), kind, S->s));
+    k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(3));
+    CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->have_matrix = true;
+    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->tri.mode = 0;
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_get_matrix(cismgpu_system* S, double* Auu, double* Auv, double* Avu, double* Avv,
+                                  double* bu, double* bv) {
+    if (!S || !S->have_matrix) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    const long long nraw = 27LL * d.nz * n2, nvec = (long long)d.nz * n2;
+    double* tmp = nullptr;
+    CG_CHECK(hipMalloc(&tmp, nraw * sizeof(double)));
+    double* outs[4] = {Auu, Auv, Avu, Avv};
+    int st = CISMGPU_OK;
+    for (int b = 0; b < 4 && !st; ++b) {
+        if (!outs[b]) continue;
+        k_unpack_matrix<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, b, S->A64, tmp);
+        if (hipMemcpyAsync(outs[b], tmp, nraw * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+            hipStreamSynchronize(S->s) != hipSuccess)
+            st = CISMGPU_ERR_HIP;
+    }
+    double* vo[2] = {bu, bv};
+    for (int c = 0; c < 2 && !st; ++c) {
+        if (!vo[c]) continue;
+        k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, S->V(2 + c), tmp);
+        if (hipMemcpyAsync(vo[c], tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+            hipStreamSynchronize(S->s) != hipSuccess)
+            st = CISMGPU_ERR_HIP;
+    }
+    hipFree(tmp);
+    if (st) cismgpu_set_error("get_matrix: device copy failed");
+    return st;
+}
+
+int32_t cismgpu_system_set_x(cismgpu_system* S, const double* xu, const double* xv) {
+    if (!S || !xu || !xv) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    const long long nvec = (long long)d.nz * d.nvx * d.nvy;
+    double* tmp = S->V(12);
+    CG_CHECK(hipMemsetAsync(S->V(0), 0, 2 * d.vlen * sizeof(double), S->s));
+    CG_CHECK(hipMemcpyAsync(tmp, xu, nvec * sizeof(double), hipMemcpyHostToDevice, S->s));
+    k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(0));
+    CG_CHECK(hipMemcpyAsync(tmp, xv, nvec * sizeof(double), hipMemcpyHostToDevice, S->s));
+    k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(1));
+    CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_get_x(cismgpu_system* S, double* xu, double* xv) {
+    if (!S || !xu || !xv) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    const long long nvec = (long long)d.nz * d.nvx * d.nvy;
+    double* tmp = S->V(12);
+    k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, S->V(0), tmp);
+    CG_CHECK(hipMemcpyAsync(xu, tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, S->V(1), tmp);
+    CG_CHECK(hipMemcpyAsync(xv, tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_info* info) {
+    if (!S || !S->have_matrix || !info) { cismgpu_set_error("system has no matrix"); return CISMGPU_ERR_ARG; }
+    int st = check_opts(o);
+    if (st) return st;
+    memset(info, 0, sizeof(*info));
+    info->err = NAN;
+    if ((o->precond == 3 || o->precond == 4) && S->tri.mode != o->precond) {
+        st = setup_tri(S, o->precond);
+        if (st) return st;
+    }
+    S->timing = o->time_spmv != 0;
+    if (S->timing && S->ev.empty()) {
+        S->ev.resize(8192);
+        for (auto& x : S->ev) hipEventCreate(&x);
+    }
+    S->ev_used = 0;
+    hipEvent_t e0, e1;
+    CG_CHECK(hipEventCreate(&e0));
+    CG_CHECK(hipEventCreate(&e1));
+    CG_CHECK(hipEventRecord(e0, S->s));
+    if (o->precision == CISMGPU_FP64) {
+        double err = NAN;
+        int nit = 0;
+        if (o->algo == CISMGPU_ALGO_CHRONGEAR) {
+            double* v[12] = {S->V(0), S->V(1), S->V(4), S->V(5), S->V(6), S->V(7), S->V(8), S->V(9), S->V(10), S->V(11), S->V(12), S->V(13)};
+            st = cg_solve<double, double>(S, S->A64, v, S->V(2), S->V(3), o->precond, o->ncheck, o->tolerance,
+                                          o->maxiters, false, &err, &nit, true);
+        } else {
+            st = std_solve(S, o->precond, o->ncheck, o->tolerance, o->maxiters, &err, &nit);
+        }
+        info->err = err;
+        info->niters = nit;
+        info->spmv_kind = 0;
+    } else {
+        st = mixed_solve(S, o, info);
+    }
+    hipEventRecord(e1, S->s);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    info->solve_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    int st2 = finish_timing(S, info);
+    S->timing = false;
+    return st ? st : st2;
+}
+
+int32_t cismgpu_system_matvec(cismgpu_system* S, double* yu, double* yv) {
+    if (!S || !S->have_matrix || !yu || !yv) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    double *y0 = S->V(12), *y1 = S->V(13);
+    spmv<double, double, 0>(S, S->A64, S->V(0), S->V(1), y0, y1, nullptr, nullptr, nullptr);
+    k_ring<double><<<col_blocks(d), 256, 0, S->s>>>(d, y0, y1, nullptr, nullptr);
+    const long long nvec = (long long)d.nz * d.nvx * d.nvy;
+    double* tmp = S->V(10);
+    k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, y0, tmp);
+    CG_CHECK(hipMemcpyAsync(yu, tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, y1, tmp);
+    CG_CHECK(hipMemcpyAsync(yv, tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep, double* avg_ms,
+                                  double* bytes_per_launch) {
+    if (!S || !S->have_matrix || nrep < 1 || kind < 0 || kind > 2) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    const long long nodes = (long long)d.nz * d.nxo * d.nyo;
+    int st = CISMGPU_OK;
+    if (kind > 0) {
+        st = ensure_lowp(S, kind);
+        if (st) return st;
+        k_cast<double, float><<<nblocks(2 * d.vlen, 256), 256, 0, S->s>>>(2 * d.vlen, S->V(0), S->W(10));
+    }
+    hipEvent_t e0, e1;
+    CG_CHECK(hipEventCreate(&e0));
+    CG_CHECK(hipEventCreate(&e1));
+    auto launch = [&]() {
+        if (kind == 0)
+            k_spmv<double, double, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, nullptr);
+        else if (kind == 1)
+            k_spmv<float, float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+        else
+            k_spmv<__hip_bfloat16, float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+    };
+    launch();   // warm-up
+    CG_CHECK(hipEventRecord(e0, S->s));
+    for (int r = 0; r < nrep; ++r) launch();
+    CG_CHECK(hipEventRecord(e1, S->s));
+    CG_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (avg_ms) *avg_ms = ms / nrep;
+    const double esz = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 2.0);
+    const double vsz = kind == 0 ? 8.0 : 4.0;
+    if (bytes_per_launch) *bytes_per_launch = (double)nodes * (NCOEF * esz + 4.0 * vsz);
+    return CISMGPU_OK;
+}
+
+// ---------------------------------------------------------------- drop-in entry points
+static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, const double* Auu, const double* Auv,
+                  const double* Avu, const double* Avv, const double* bu, const double* bv, double* xu, double* xv,
+                  int32_t precond, int32_t ncheck, double tol, int32_t maxiters, double* err, int32_t* niters) {
+    if (!active || !Auu || !Auv || !Avu || !Avv || !bu || !bv || !xu || !xv || !err || !niters) {
+        cismgpu_set_error("null argument");
+        return CISMGPU_ERR_ARG;
+    }
+    cismgpu_system* S = nullptr;
+    int st = cismgpu_system_create(grid, &S);
+    if (st) return st;
+    st = cismgpu_system_set_matrix(S, active, Auu, Auv, Avu, Avv, bu, bv, 0);
+    if (!st) st = cismgpu_system_set_x(S, xu, xv);
+    cismgpu_solve_info info{};
+    if (!st) {
+        cismgpu_solve_opts o{};
+        o.algo = algo; o.precond = precond; o.ncheck = ncheck; o.maxiters = maxiters; o.tolerance = tol;
+        o.precision = CISMGPU_FP64;
+        st = cismgpu_system_solve(S, &o, &info);
+    }
+    if (!st || st == CISMGPU_ERR_NAN) {
+        int st2 = cismgpu_system_get_x(S, xu, xv);
+        if (!st) st = st2;
+        if (info.err == info.err) *err = info.err;   // untouched when no check ran
+        *niters = info.niters;
+    }
+    cismgpu_system_destroy(S);
+    return st;
+}
+
+int32_t cismgpu_pcg_solver_standard_3d(const cismgpu_grid* grid, const int32_t* active_vertex, const double* Auu,
+                                       const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                       const double* bv, double* xu, double* xv, int32_t precond,
+                                       int32_t linear_solve_ncheck, double tolerance, int32_t maxiters, double* err,
+                                       int32_t* niters) {
+    return dropin(CISMGPU_ALGO_STANDARD, grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond,
+                  linear_solve_ncheck, tolerance, maxiters, err, niters);
+}
+
+int32_t cismgpu_pcg_solver_chrongear_3d(const cismgpu_grid* grid, const int32_t* active_vertex, const double* Auu,
+                                        const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                        const double* bv, double* xu, double* xv, int32_t precond,
+                                        int32_t linear_solve_ncheck, double tolerance, int32_t maxiters, double* err,
+                                        int32_t* niters) {
+    return dropin(CISMGPU_ALGO_CHRONGEAR, grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond,
+                  linear_solve_ncheck, tolerance, maxiters, err, niters);
+}
+
+int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                              const double* Auu, const double* Auv, const double* Avu,
+                                              const double* Avv, const double* xu, const double* xv, double* yu,
+                                              double* yv) {
+    cismgpu_system* S = nullptr;
+    int st = cismgpu_system_create(grid, &S);
+    if (st) return st;
+    const Dims& d = S->d;
+    std::vector<double> zero((size_t)d.nz * d.nvx * d.nvy, 0.0);
+    st = cismgpu_system_set_matrix(S, active_vertex, Auu, Auv, Avu, Avv, zero.data(), zero.data(), 0);
+    if (!st) st = cismgpu_system_set_x(S, xu, xv);
+    if (!st) st = cismgpu_system_matvec(S, yu, yv);
+    cismgpu_system_destroy(S);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+/*
+ * cismgpu.h -- C ABI of libcismgpu, the MI355X (gfx950) implementation of
+ * CISM Glissade's higher-order (Blatter-Pattyn) velocity hot path.
+ *
+ * Plain C types only.  Host arrays use the reference's Fortran layouts:
+ *   vectors   (nz, nx-1, ny-1)        k fastest
+ *   matrices  (27, nz, nx-1, ny-1)    neighbour index m fastest,
+ *             m = 1 + (iA+1) + 3(jA+1) + 9(kA+1)
+ *   vertex masks (nx-1, ny-1) as int32 (0/1) in place of Fortran LOGICAL
+ * so that a Fortran caller passes its arrays with c_loc and no copies.
+ *
+ * Every entry point returns a status: 0 ok, otherwise one of CISMGPU_ERR_*.
+ * cismgpu_last_error() gives a message for the last failure of this thread.
+ */
+#ifndef CISMGPU_H
+#define CISMGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CISMGPU_OK                0
+#define CISMGPU_ERR_NAN           1   /* alpha or eta1 is NaN: the reference's GM_FATAL (pcg.F90:1331,1391,3263,3434) */
+#define CISMGPU_ERR_DIVZERO       2   /* zero pivot in tridiagonal preconditioner setup (pcg.F90:4338-4342, 4474-4493) */
+#define CISMGPU_ERR_ARG           3   /* bad argument (sizes, bounds, options) */
+#define CISMGPU_ERR_SINGULAR      4   /* singular element Jacobian (glissade_velo_higher.F90:6611-6626) */
+#define CISMGPU_ERR_ASYM          5   /* element matrix not symmetric (glissade_velo_higher.F90:10583-10644) */
+#define CISMGPU_ERR_HIP          10   /* HIP runtime failure (message in cismgpu_last_error) */
+#define CISMGPU_ERR_UNSUPPORTED  11   /* e.g. more than one task per row/column */
+#define CISMGPU_ERR_NOT_CONVERGED 12  /* mixed-precision refinement stalled before its tolerance */
+
+/* which_ho_sparse values (glide_types.F90:327-332) */
+#define CISMGPU_ALGO_STANDARD  2
+#define CISMGPU_ALGO_CHRONGEAR 3
+
+/* precisions of the Krylov iterations */
+#define CISMGPU_FP64        0   /* plain FP64: the drop-in */
+#define CISMGPU_MIXED_FP32  1   /* FP32 matrix + FP32 inner C-G inside FP64 iterative refinement */
+#define CISMGPU_MIXED_BF16  2   /* BF16 matrix storage, FP32 vectors/accumulation, FP64 refinement */
+
+/* Fields of CISM's parallel_type read on the solver path
+ * (glissade_velo_higher_pcg.F90:2888-2894; parallel_mpi.F90:2633-2653),
+ * for a single task.  Bounds are 1-based like parallel%staggered_*. */
+typedef struct cismgpu_grid {
+    int32_t nx, ny, nz;          /* cells incl. 2-cell halos; sigma levels */
+    int32_t ilo, ihi, jlo, jhi;  /* staggered_ilo/ihi/jlo/jhi */
+    int32_t global_bc;           /* 0 periodic, 1 outflow, 2 no_penetration, 3 no_ice */
+    int32_t reprosum;            /* accepted for interface parity; device sums are always compensated */
+} cismgpu_grid;
+
+const char* cismgpu_last_error(void);
+int32_t cismgpu_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* Drop-in solver entry points (host arrays in, host solution out).    */
+/* ------------------------------------------------------------------ */
+
+/* Replaces pcg_solver_standard_3d (glissade_velo_higher_pcg.F90:827-838);
+ * called from glissade_velo_higher.F90:3304-3316.  xu, xv: initial guess in,
+ * solution out.  err is written only when a convergence check ran (the
+ * reference leaves it unassigned otherwise); niters = maxiters when the
+ * tolerance is not reached (non-convergence is not an error). */
+int32_t cismgpu_pcg_solver_standard_3d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                       const double* Auu, const double* Auv,
+                                       const double* Avu, const double* Avv,
+                                       const double* bu, const double* bv,
+                                       double* xu, double* xv,
+                                       int32_t precond, int32_t linear_solve_ncheck,
+                                       double tolerance, int32_t maxiters,
+                                       double* err, int32_t* niters);
+
+/* Replaces pcg_solver_chrongear_3d (glissade_velo_higher_pcg.F90:2632-2643);
+ * called from glissade_velo_higher.F90:3288-3300.  Same contract as above. */
+int32_t cismgpu_pcg_solver_chrongear_3d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                        const double* Auu, const double* Auv,
+                                        const double* Avu, const double* Avv,
+                                        const double* bu, const double* bv,
+                                        double* xu, double* xv,
+                                        int32_t precond, int32_t linear_solve_ncheck,
+                                        double tolerance, int32_t maxiters,
+                                        double* err, int32_t* niters);
+
+/* Replaces matvec_multiply_structured_3d (glissade_velo_higher_pcg.F90:5528-5652):
+ * y = A x on owned active vertices, 0 elsewhere. */
+int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                              const double* Auu, const double* Auv,
+                                              const double* Avu, const double* Avv,
+                                              const double* xu, const double* xv,
+                                              double* yu, double* yv);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident system: the matrix, rhs and iterate stay in HBM     */
+/* across calls (Picard iterations, benchmarks).  No reference         */
+/* counterpart: the reference keeps everything in host arrays.         */
+/* ------------------------------------------------------------------ */
+typedef struct cismgpu_system cismgpu_system;
+
+typedef struct cismgpu_solve_opts {
+    int32_t algo;            /* CISMGPU_ALGO_STANDARD or CISMGPU_ALGO_CHRONGEAR */
+    int32_t precond;         /* 0 none, 1 diag, 2 SIA column, 3 tridiag local, 4 tridiag global (one task) */
+    int32_t ncheck;          /* linear_solve_ncheck */
+    int32_t maxiters;        /* linear_maxiters (per inner solve in mixed mode) */
+    double  tolerance;       /* relative residual ||b-Ax||/||b|| target */
+    int32_t precision;       /* CISMGPU_FP64 / CISMGPU_MIXED_FP32 / CISMGPU_MIXED_BF16 */
+    double  inner_tol;       /* mixed: inner relative residual target per refinement step */
+    int32_t max_outer;       /* mixed: maximum refinement steps */
+    int32_t time_spmv;       /* 1: time every SpMV launch with HIP events (info.spmv_*) */
+} cismgpu_solve_opts;
+
+typedef struct cismgpu_solve_info {
+    double  err;             /* final relative residual (FP64) */
+    int32_t niters;          /* FP64: iterations; mixed: total inner iterations */
+    int32_t outer_iters;     /* mixed: refinement steps (0 for FP64) */
+    double  solve_ms;        /* device time of the solve (HIP events on the system stream) */
+    double  spmv_ms;         /* summed duration of the timed SpMV launches */
+    int32_t spmv_count;      /* number of timed SpMV launches */
+    int32_t spmv_kind;       /* storage of the dominant SpMV: 0 fp64, 1 fp32, 2 bf16 */
+} cismgpu_solve_info;
+
+int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out);
+int32_t cismgpu_system_destroy(cismgpu_system* sys);
+
+/* Load A (reference layout) and b.  where = 0: host pointers; 1: device pointers. */
+int32_t cismgpu_system_set_matrix(cismgpu_system* sys, const int32_t* active_vertex,
+                                  const double* Auu, const double* Auv,
+                                  const double* Avu, const double* Avv,
+                                  const double* bu, const double* bv, int32_t where);
+/* Copy A and b back to reference layout (host pointers).  Rows that are not
+ * owned are returned as zero. */
+int32_t cismgpu_system_get_matrix(cismgpu_system* sys, double* Auu, double* Auv,
+                                  double* Avu, double* Avv, double* bu, double* bv);
+int32_t cismgpu_system_set_x(cismgpu_system* sys, const double* xu, const double* xv);
+int32_t cismgpu_system_get_x(cismgpu_system* sys, double* xu, double* xv);
+
+/* Solve with the device-resident x as initial guess; x is overwritten. */
+int32_t cismgpu_system_solve(cismgpu_system* sys, const cismgpu_solve_opts* opts,
+                             cismgpu_solve_info* info);
+
+/* y = A x with the resident matrix (FP64) on the resident x; host result. */
+int32_t cismgpu_system_matvec(cismgpu_system* sys, double* yu, double* yv);
+
+/* Average device time of `nrep` back-to-back SpMV launches with storage
+ * `kind` (0 fp64, 1 fp32, 2 bf16), measured with HIP events on the system
+ * stream.  Also returns the algorithmic bytes of one launch. */
+int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nrep,
+                                  double* avg_ms, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
