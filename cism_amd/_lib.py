@@ -30,7 +30,8 @@ EXPORTS = (
     "cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d", "cismgpu_matvec_multiply_structured_3d",
     "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix", "cismgpu_system_get_matrix",
     "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve", "cismgpu_system_matvec",
-    "cismgpu_system_bench_spmv",
+    "cismgpu_system_bench_spmv", "cismgpu_system_halo_x", "cismgpu_system_assemble_3d", "cismgpu_system_get_efvs",
+    "cismgpu_system_residual",
 )
 
 

@@ -1,0 +1,493 @@
+// assembly.hip -- device assembly of the 3-D Blatter-Pattyn system
+// (glissade_velo_higher.F90): element stiffness with Glen's-law viscosity
+// (:5382-5643, :6508-6732, :7285-7470, :8001-8123), gravity load (:4311-4342),
+// local basal friction (:8921-8953), Dirichlet rows/columns (:9305-9498),
+// the symmetry fix (:10854-11117), nonzero count (:10513-10579) and the
+// residual (:9693-9894).
+//
+// Owner-computes gather: one thread per owned node builds that node's whole
+// matrix row from the (up to) 8 elements that contain it, visiting them in
+// the reference's element order (cells j-major then i, layer above before
+// layer below) so every coefficient is summed in the same order as
+// element_to_global_matrix_3d.  No two threads write the same row, so the
+// scatter needs neither atomics nor colouring passes, every coefficient is
+// written exactly once, and the result is deterministic.
+#include "cismgpu_internal.h"
+
+namespace cg {
+
+// Reference-element tables [n][p] (glissade_velo_higher.F90:352-388)
+__constant__ double c_dxr[8][8], c_dyr[8][8], c_dzr[8][8];
+
+// node offsets from node 7 of an element (ishift/jshift/kshift(7,:), :461-494)
+__host__ __device__ constexpr int nd_i(int n) { return (n == 0 || n == 3 || n == 4 || n == 7) ? -1 : 0; }
+__host__ __device__ constexpr int nd_j(int n) { return (n == 0 || n == 1 || n == 4 || n == 5) ? -1 : 0; }
+__host__ __device__ constexpr int nd_k(int n) { return n < 4 ? 1 : 0; }
+
+struct AsmDev {
+    const double *xV, *yV, *susrf, *sthck, *sigma, *swbs, *flwafact, *dsdx, *dsdy, *beta, *theta;
+    const int32_t *acell, *avert, *umask, *vmask;
+    double* efvs;
+    double dx, dy, efvs_constant, effstrain_min, n_glen;
+    int whichefvs, nhalo;
+};
+
+struct Elem { double x[8], y[8], z[8], u[8], v[8]; };
+
+__device__ __forceinline__ long long c2(const Dims& d, int i, int j) { return (long long)(i - 1) + (long long)d.nx * (j - 1); }
+__device__ __forceinline__ long long v2(const Dims& d, int i, int j) { return (long long)(i - 1) + (long long)d.nvx * (j - 1); }
+
+// Nodes of element (cell ci,cj; layer kk, 1-based) from vertex fields and
+// the velocity in device layout.
+__device__ __forceinline__ void load_elem(const Dims& d, const AsmDev& in, const double* __restrict__ uv,
+                                          const double* __restrict__ vv, int ci, int cj, int kk, Elem& e) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int in_ = ci + nd_i(n), jn = cj + nd_j(n), kn = kk + nd_k(n);
+        const long long p = v2(d, in_, jn);
+        e.x[n] = in.xV[p];
+        e.y[n] = in.yV[p];
+        e.z[n] = in.susrf[p] - in.sigma[kn - 1] * in.sthck[p];
+        const long long c = vidx(d, kn - 1, in_, jn);
+        e.u[n] = uv[c];
+        e.v[n] = vv[c];
+    }
+}
+
+// get_basis_function_derivatives_3d at quadrature point p; false if singular
+__device__ __forceinline__ bool basis_derivs(const Elem& e, int p, double* dx, double* dy, double* dz, double& detJ) {
+    double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double a = c_dxr[n][p], b = c_dyr[n][p], c = c_dzr[n][p];
+        J00 = J00 + a * e.x[n]; J01 = J01 + a * e.y[n]; J02 = J02 + a * e.z[n];
+        J10 = J10 + b * e.x[n]; J11 = J11 + b * e.y[n]; J12 = J12 + b * e.z[n];
+        J20 = J20 + c * e.x[n]; J21 = J21 + c * e.y[n]; J22 = J22 + c * e.z[n];
+    }
+    const double c00 = J11 * J22 - J12 * J21;
+    const double c01 = -(J10 * J22 - J12 * J20);
+    const double c02 = J10 * J21 - J11 * J20;
+    const double c10 = -(J01 * J22 - J02 * J21);
+    const double c11 = J00 * J22 - J02 * J20;
+    const double c12 = -(J00 * J21 - J01 * J20);
+    const double c20 = J01 * J12 - J02 * J11;
+    const double c21 = -(J00 * J12 - J02 * J10);
+    const double c22 = J00 * J11 - J01 * J10;
+    detJ = J00 * c00 + J01 * c01 + J02 * c02;
+    if (!(fabs(detJ) > 0.0)) return false;
+    // Jinv(row,col) = cofactor(col,row) / detJ
+    const double i00 = c00 / detJ, i01 = c10 / detJ, i02 = c20 / detJ;
+    const double i10 = c01 / detJ, i11 = c11 / detJ, i12 = c21 / detJ;
+    const double i20 = c02 / detJ, i21 = c12 / detJ, i22 = c22 / detJ;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double a = c_dxr[n][p], b = c_dyr[n][p], c = c_dzr[n][p];
+        dx[n] = i00 * a + i01 * b + i02 * c;
+        dy[n] = i10 * a + i11 * b + i12 * c;
+        dz[n] = i20 * a + i21 * b + i22 * c;
+    }
+    return true;
+}
+
+// compute_effective_viscosity, BP branch
+__device__ __forceinline__ double efvs_at(const AsmDev& in, const Elem& e, const double* dx, const double* dy,
+                                          const double* dz, double flwafact) {
+    const double p_effstr = (1.0 - in.n_glen) / in.n_glen;
+    if (in.whichefvs == 0) return in.efvs_constant;
+    if (in.whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
+        return flwafact * pow(effstrain, p_effstr);
+    }
+    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        ux = ux + dx[n] * e.u[n]; uy = uy + dy[n] * e.u[n]; uz = uz + dz[n] * e.u[n];
+        vx = vx + dx[n] * e.v[n]; vy = vy + dy[n] * e.v[n]; vz = vz + dz[n] * e.v[n];
+    }
+    const double t1 = ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
+    const double t2 = 0.25 * (uz * uz + vz * vz);
+    const double esq = in.effstrain_min * in.effstrain_min + t1 + t2;
+    return flwafact * pow(esq, p_effstr / 2.0);
+}
+
+__device__ __forceinline__ bool cell_in_range(const Dims& d, int nhalo, int ci, int cj) {
+    return ci >= nhalo + 1 && ci <= d.nx - nhalo + 1 && cj >= nhalo + 1 && cj <= d.ny - nhalo + 1;
+}
+
+// Row NR of the element matrix of (ci,cj,kk), summed over the 8 quadrature
+// points exactly as compute_element_matrix accumulates K, then added into
+// the node row accumulators acc[blk][m].
+template <int NR>
+__device__ __forceinline__ bool add_element_row(const Dims& d, const AsmDev& in, const double* __restrict__ uv,
+                                                const double* __restrict__ vv, int ci, int cj, int kk,
+                                                double (&acc)[4][27]) {
+    Elem e;
+    load_elem(d, in, uv, vv, ci, cj, kk, e);
+    const double fl = in.flwafact[(long long)(kk - 1) + (long long)(d.nz - 1) * c2(d, ci, cj)];
+    double K[4][8];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) K[b][c] = 0.0;
+    bool ok = true;
+    for (int p = 0; p < 8; ++p) {
+        double dx[8], dy[8], dz[8], detJ;
+        if (!basis_derivs(e, p, dx, dy, dz, detJ)) { ok = false; break; }
+        const double ef = efvs_at(in, e, dx, dy, dz, fl);
+        const double f = ef * 1.0 * detJ / 1.0e9;
+        const double ar = dx[NR], br = dy[NR], cr = dz[NR];
+#pragma unroll
+        for (int nc = 0; nc < 8; ++nc) {
+            K[0][nc] = K[0][nc] + f * (4.0 * ar * dx[nc] + br * dy[nc] + cr * dz[nc]);
+            K[1][nc] = K[1][nc] + f * (2.0 * ar * dy[nc] + br * dx[nc]);
+            K[2][nc] = K[2][nc] + f * (2.0 * br * dx[nc] + ar * dy[nc]);
+            K[3][nc] = K[3][nc] + f * (4.0 * br * dy[nc] + ar * dx[nc] + cr * dz[nc]);
+        }
+    }
+#pragma unroll
+    for (int nc = 0; nc < 8; ++nc) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int m = (nd_i(nc) - nd_i(NR) + 1) + 3 * (nd_j(nc) - nd_j(NR) + 1) + 9 * (nd_k(nc) - nd_k(NR) + 1);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b][m] = acc[b][m] + K[b][nc];
+    }
+    return ok;
+}
+
+// One thread per owned node (io fastest).  Writes the full row (108
+// coefficients) into the row-blocked matrix.  flag[0] = 1 on a singular
+// Jacobian.
+__global__ __launch_bounds__(64) void k_assemble(Dims d, AsmDev in, const double* __restrict__ uv,
+                                                 const double* __restrict__ vv, double* __restrict__ A,
+                                                 double* __restrict__ flag) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k = (int)(kj / d.nyo);   // k 0-based level
+    double acc[4][27];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int m = 0; m < 27; ++m) acc[b][m] = 0.0;
+    if (io < d.nxo) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+        const int kf = k + 1;   // 1-based level of this node
+        if (in.avert[v2(d, i, j)]) {
+            bool ok = true;
+            // cells in reference order: (i,j), (i+1,j), (i,j+1), (i+1,j+1); in each,
+            // layer kf-1 (node on the lower face) then layer kf (node on the upper face)
+#define CELL(DI, DJ, NL, NU)                                                                       \
+    {                                                                                              \
+        const int ci = i + DI, cj = j + DJ;                                                        \
+        if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]) {                       \
+            if (kf - 1 >= 1) ok = add_element_row<NL>(d, in, uv, vv, ci, cj, kf - 1, acc) && ok;   \
+            if (kf <= d.nz - 1) ok = add_element_row<NU>(d, in, uv, vv, ci, cj, kf, acc) && ok;    \
+        }                                                                                          \
+    }
+            CELL(0, 0, 2, 6)
+            CELL(1, 0, 3, 7)
+            CELL(0, 1, 1, 5)
+            CELL(1, 1, 0, 4)
+#undef CELL
+            if (!ok) flag[0] = 1.0;
+        }
+    }
+    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int m = 0; m < 27; ++m) a[(b * 27 + m) * WAVE] = acc[b][m];
+}
+
+// efvs(k,i,j) = mean over the quadrature points (:5604-5610), one thread
+// per element in the assembly range.
+__global__ void k_efvs(Dims d, AsmDev in, const double* __restrict__ uv, const double* __restrict__ vv,
+                       double* __restrict__ efvs) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n = (long long)(d.nz - 1) * d.nx * d.ny;
+    if (t >= n) return;
+    const int kk = 1 + (int)(t % (d.nz - 1));
+    const long long cell = t / (d.nz - 1);
+    const int ci = 1 + (int)(cell % d.nx), cj = 1 + (int)(cell / d.nx);
+    double out = 0.0;
+    if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]) {
+        Elem e;
+        load_elem(d, in, uv, vv, ci, cj, kk, e);
+        const double fl = in.flwafact[t];
+        double s = 0.0;
+        for (int p = 0; p < 8; ++p) {
+            double dx[8], dy[8], dz[8], detJ;
+            if (!basis_derivs(e, p, dx, dy, dz, detJ)) { s = NAN; break; }
+            s = s + efvs_at(in, e, dx, dy, dz, fl);
+        }
+        out = s / 8.0;
+    }
+    efvs[t] = out;
+}
+
+// b = load_vector_gravity, local option, over every active vertex
+// (:4311-4342); b starts from zero.
+__global__ void k_load_gravity(Dims d, AsmDev in, double* __restrict__ bu, double* __restrict__ bv) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const double rhoi = 910.0, grav = 9.81, vol0 = 1.0e9;
+    const bool act = in.avert[t] != 0;
+    for (int k = 1; k <= d.nz; ++k) {
+        const long long c = vidx(d, k - 1, i, j);
+        double lu = 0.0, lv = 0.0;
+        if (act) {
+            const double dz = in.sthck[t] * (in.swbs[k] - in.swbs[k - 1]);
+            lu = lu - rhoi * grav * in.dx * in.dy * dz / vol0 * in.dsdx[t];
+            lv = lv - rhoi * grav * in.dx * in.dy * dz / vol0 * in.dsdy[t];
+        }
+        bu[c] = lu;
+        bv[c] = lv;
+    }
+}
+
+// basal_sliding_bc_3d, local option: diagonal of the basal level
+// (:8921-8953), owned rows.
+__global__ void k_basal_local(Dims d, AsmDev in, double* __restrict__ A) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nxo * d.nyo) return;
+    const int io = (int)(t % d.nxo), jo = (int)(t / d.nxo);
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const long long p = v2(d, i, j);
+    if (!in.avert[p]) return;
+    const long long r = ((long long)(d.nz - 1) * d.nyo + jo) * d.nxp + io;
+    double* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+    const double add = in.dx * in.dy / 1.0e9 * in.beta[p] / cos(in.theta[p]);
+    a[(0 * 27 + 13) * WAVE] = a[(0 * 27 + 13) * WAVE] + add;
+    a[(3 * 27 + 13) * WAVE] = a[(3 * 27 + 13) * WAVE] + add;
+}
+
+// dirichlet_boundary_conditions_3d restated per owned row P: the row of a
+// prescribed node becomes the identity; entries of P that couple to a
+// prescribed active node Q in the loop range are moved to the rhs, in the
+// reference's (j, i, k; u then v) order of Q.
+__global__ void k_dirichlet(Dims d, AsmDev in, const double* __restrict__ uv, const double* __restrict__ vv,
+                            double* __restrict__ A, double* __restrict__ bu, double* __restrict__ bv) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+    if (io >= d.nxo) return;
+    const int i = d.ilo + io, j = d.jlo + jo, k = k0 + 1;
+    const int nh = in.nhalo;
+    auto vec3 = [&](int kk, int ii, int jj) { return (long long)(kk - 1) + (long long)d.nz * v2(d, ii, jj); };
+    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    const long long pc = vidx(d, k0, i, j);
+    const bool inrange = i >= nh && i <= d.nx - nh + 1 && j >= nh && j <= d.ny - nh + 1;
+    const bool pact = in.avert[v2(d, i, j)] != 0;
+    const int um = in.umask[vec3(k, i, j)], vm = in.vmask[vec3(k, i, j)];
+    double bU = bu[pc], bV = bv[pc];
+    // column elimination from prescribed neighbours Q, in Q's loop order
+    for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di)
+            for (int dk = -1; dk <= 1; ++dk) {
+                // Q = (k+dk, i+di, j+dj); loop order over Q is j, then i, then k
+                const int qi = i + di, qj = j + dj, qk = k + dk;
+                if (di == 0 && dj == 0 && dk == 0) continue;
+                if (qk < 1 || qk > d.nz) continue;
+                if (!(qi >= nh && qi <= d.nx - nh + 1 && qj >= nh && qj <= d.ny - nh + 1)) continue;
+                if (!in.avert[v2(d, qi, qj)]) continue;
+                const long long q = vec3(qk, qi, qj);
+                const int m = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);   // entry of row P coupling to Q
+                const long long qc = vidx(d, qk - 1, qi, qj);
+                if (in.umask[q] == 1) {
+                    if (um != 1) { bU = bU - a[(0 * 27 + m) * WAVE] * uv[qc]; a[(0 * 27 + m) * WAVE] = 0.0; }
+                    if (vm != 1) { bV = bV - a[(2 * 27 + m) * WAVE] * uv[qc]; a[(2 * 27 + m) * WAVE] = 0.0; }
+                }
+                if (in.vmask[q] == 1) {
+                    if (um != 1) { bU = bU - a[(1 * 27 + m) * WAVE] * vv[qc]; a[(1 * 27 + m) * WAVE] = 0.0; }
+                    if (vm != 1) { bV = bV - a[(3 * 27 + m) * WAVE] * vv[qc]; a[(3 * 27 + m) * WAVE] = 0.0; }
+                }
+            }
+    // own row (the reference does this when it visits P; entries of row P are
+    // touched by no other node's visit, so the order is immaterial)
+    if (inrange && pact) {
+        if (um == 1) {
+            bU = uv[pc];
+            for (int m = 0; m < 27; ++m) {
+                const int kA = m / 9 - 1;
+                if (k + kA < 1 || k + kA > d.nz) continue;
+                a[(0 * 27 + m) * WAVE] = (m == 13) ? 1.0 : 0.0;
+                a[(1 * 27 + m) * WAVE] = 0.0;
+                if (m == 13) a[(2 * 27 + m) * WAVE] = 0.0;
+            }
+        }
+        if (vm == 1) {
+            bV = vv[pc];
+            for (int m = 0; m < 27; ++m) {
+                const int kA = m / 9 - 1;
+                if (k + kA < 1 || k + kA > d.nz) continue;
+                a[(2 * 27 + m) * WAVE] = 0.0;
+                a[(3 * 27 + m) * WAVE] = (m == 13) ? 1.0 : 0.0;
+                if (m == 13) a[(1 * 27 + m) * WAVE] = 0.0;
+            }
+        }
+    }
+    bu[pc] = bU;
+    bv[pc] = bV;
+}
+
+// Halo of the assembled matrix for the no_ice rule: rows of the last owned
+// vertex row/column are zeroed (parallel_mpi.F90:10912-10921 applied by
+// staggered_parallel_halo_real8_4d to A, glissade_velo_higher.F90:2893-2898).
+__global__ void k_matrix_noice_edge(Dims d, double* __restrict__ A) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo);
+    if (io >= d.nxo) return;
+    const int i = d.ilo + io, j = d.jlo + jo;
+    if (i < d.nvx - 1 && j < d.nvy - 1) return;
+    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    for (int c = 0; c < NCOEF; ++c) a[c * WAVE] = 0.0;
+}
+
+// Row of a vertex after the halo update: owned -> itself; periodic halo ->
+// its owned image; other rules -> none (zeroed halo rows).  Returns false
+// when the row is zero.
+__device__ __forceinline__ bool row_image(const Dims& d, int& i, int& j) {
+    const bool periodic = (d.bc == 0 || d.bc == 2);
+    const int n_i = d.ihi - d.ilo + 1, n_j = d.jhi - d.jlo + 1;
+    if (i < d.ilo || i > d.ihi) {
+        if (!periodic) return false;
+        i = d.ilo + ((i - d.ilo) % n_i + n_i) % n_i;
+    }
+    if (j < d.jlo || j > d.jhi) {
+        if (!periodic) return false;
+        j = d.jlo + ((j - d.jlo) % n_j + n_j) % n_j;
+    }
+    return true;
+}
+
+__device__ __forceinline__ const double* row_ptr(const Dims& d, const double* A, int k0, int i, int j) {
+    const long long r = ((long long)k0 * d.nyo + (j - d.jlo)) * d.nxp + (i - d.ilo);
+    return A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+}
+
+// check_symmetry_assembled_matrix_3d restated per entry: an entry and its
+// transpose partner are replaced by their average when they differ by less
+// than eps11 times the diagonal of either row that checks the pair (the row
+// visited first checks with its own diagonal, the partner row -- when it is
+// an owned active row -- with its diagonal).  Each thread writes only its
+// own row, into a copy (Aout), so the update is race-free.  count[0] counts
+// pairs left asymmetric (the reference warns for those).
+__global__ void k_symmetry(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
+                           double* __restrict__ Aout, unsigned long long* __restrict__ nwarn) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+    const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    double* o = Aout + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    for (int c = 0; c < NCOEF; ++c) o[c * WAVE] = a[c * WAVE];
+    if (io >= d.nxo) return;
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const long long pv = (long long)(i - 1) + (long long)d.nvx * (j - 1);
+    if (!avert[pv]) return;
+    const double eps11 = 1.0e-11;
+    const double diagu = a[13 * WAVE], diagv = a[(3 * 27 + 13) * WAVE];
+    unsigned long long warns = 0;
+    for (int m = 0; m < 27; ++m) {
+        const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+        if (k0 + kA < 0 || k0 + kA >= d.nz) continue;
+        const int mm = 26 - m;
+        int qi = i + iA, qj = j + jA;
+        const bool owned_q = qi >= d.ilo && qi <= d.ihi && qj >= d.jlo && qj <= d.jhi &&
+                             avert[(long long)(qi - 1) + (long long)d.nvx * (qj - 1)];
+        double qdu = 0.0, qdv = 0.0;
+        const double* q = nullptr;
+        if (row_image(d, qi, qj)) {
+            q = row_ptr(d, A, k0 + kA, qi, qj);
+            qdu = q[13 * WAVE];
+            qdv = q[(3 * 27 + 13) * WAVE];
+        }
+        // (uu), (uv<->vu), (vv), (vu<->uv) pairs: own block, partner block, own diag, partner diag
+        const int ob[4] = {0, 1, 3, 2}, pb[4] = {0, 2, 3, 1};
+        for (int s = 0; s < 4; ++s) {
+            const double v1 = a[(ob[s] * 27 + m) * WAVE];
+            const double v2_ = q ? q[(pb[s] * 27 + mm) * WAVE] : 0.0;
+            if (v2_ == v1) continue;
+            const double dmine = (s == 0 || s == 1) ? diagu : diagv;
+            const double dpart = (s == 0 || s == 3) ? qdu : qdv;
+            const double diff = fabs(v2_ - v1);
+            if (diff < eps11 * fabs(dmine) || (owned_q && diff < eps11 * fabs(dpart)))
+                o[(ob[s] * 27 + m) * WAVE] = 0.5 * (v1 + v2_);
+            else
+                ++warns;
+        }
+    }
+    if (warns) atomicAdd(nwarn, warns);
+}
+
+// count_nonzeros_3d over owned active rows
+__global__ void k_count_nnz(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
+                            unsigned long long* __restrict__ count) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long c = 0;
+    if (t < d.nrows) {
+        const int io = (int)(t % d.nxp);
+        const long long kj = t / d.nxp;
+        const int jo = (int)(kj % d.nyo);
+        if (io < d.nxo) {
+            const int i = d.ilo + io, j = d.jlo + jo;
+            if (avert[(long long)(i - 1) + (long long)d.nvx * (j - 1)]) {
+                const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+                for (int q = 0; q < NCOEF; ++q) c += (a[q * WAVE] != 0.0) ? 1ull : 0ull;
+            }
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) c += __shfl_down(c, off, WAVE);
+    if (threadIdx.x % WAVE == 0 && c) atomicAdd(count, c);
+}
+
+// SIA/diag preconditioner coefficients of the owned rows of the row-blocked
+// matrix; the ring is filled afterwards with the halo rule.
+__global__ void k_precond_from_rows(Dims d, const double* __restrict__ A, double* __restrict__ M,
+                                    double* __restrict__ Dg) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+    if (io >= d.nxo) return;
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    const long long c = vidx(d, k0, i, j);
+    M[0 * d.vlen + c] = a[4 * WAVE];
+    M[1 * d.vlen + c] = a[13 * WAVE];
+    M[2 * d.vlen + c] = a[22 * WAVE];
+    M[3 * d.vlen + c] = a[(81 + 4) * WAVE];
+    M[4 * d.vlen + c] = a[(81 + 13) * WAVE];
+    M[5 * d.vlen + c] = a[(81 + 22) * WAVE];
+    Dg[c] = a[13 * WAVE];
+    Dg[d.vlen + c] = a[(81 + 13) * WAVE];
+}
+
+// Raw Auu/Avv staging for the tridiagonal preconditioners (reference layout
+// of the vertical-sum inputs) from the owned rows plus the halo rule.
+__global__ void k_rows_to_raw(Dims d, int blk, const double* __restrict__ A, double* __restrict__ raw) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long total = 27LL * d.nz * d.nvx * d.nvy;
+    if (t >= total) return;
+    const int m = (int)(t % 27);
+    long long rest = t / 27;
+    const int k0 = (int)(rest % d.nz);
+    rest /= d.nz;
+    int i = 1 + (int)(rest % d.nvx), j = 1 + (int)(rest / d.nvx);
+    double v = 0.0;
+    if (row_image(d, i, j)) v = row_ptr(d, A, k0, i, j)[(blk * 27 + m) * WAVE];
+    raw[t] = v;
+}
+
+}  // namespace cg

@@ -14,6 +14,7 @@
 #include "cismgpu_internal.h"
 #include "kernels.h"
 #include "kernels.hip"
+#include "assembly.hip"
 
 using namespace cg;
 
@@ -63,6 +64,11 @@ struct cismgpu_system {
     double* scal = nullptr;
     double* scal_h = nullptr;      // pinned mirror
     TriPC tri;
+    // assembly inputs on the device
+    void* asm_buf = nullptr;
+    size_t asm_bytes = 0;
+    double* efvs = nullptr;
+    double* Atmp = nullptr;      // second matrix buffer for the symmetry fix
     bool have_matrix = false;
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
@@ -648,7 +654,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -866,6 +872,209 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     const double esz = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 2.0);
     const double vsz = kind == 0 ? 8.0 : 4.0;
     if (bytes_per_launch) *bytes_per_launch = (double)nodes * (NCOEF * esz + 4.0 * vsz);
+    return CISMGPU_OK;
+}
+
+// ---------------------------------------------------------------- assembly
+static bool g_tables_ready = false;
+
+static int upload_fe_tables() {
+    if (g_tables_ready) return CISMGPU_OK;
+    const double r = 1.0 / std::sqrt(3.0);
+    const double xq[8] = {-r, r, r, -r, -r, r, r, -r};
+    const double yq[8] = {-r, -r, r, r, -r, -r, r, r};
+    const double zq[8] = {-r, -r, -r, -r, r, r, r, r};
+    const double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    const double sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const double sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    double dxr[8][8], dyr[8][8], dzr[8][8];
+    for (int p = 0; p < 8; ++p)
+        for (int n = 0; n < 8; ++n) {
+            dxr[n][p] = sx[n] * (1.0 + sy[n] * yq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+            dyr[n][p] = sy[n] * (1.0 + sx[n] * xq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+            dzr[n][p] = sz[n] * (1.0 + sx[n] * xq[p]) * (1.0 + sy[n] * yq[p]) / 8.0;
+        }
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dxr), dxr, sizeof(dxr)));
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dyr), dyr, sizeof(dyr)));
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dzr), dzr, sizeof(dzr)));
+    g_tables_ready = true;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_assembly_in* in,
+                                              cismgpu_assembly_info* info) {
+    if (!S || !in || !in->sigma || !in->stagwbndsigma || !in->active_cell || !in->active_vertex || !in->xVertex ||
+        !in->yVertex || !in->stagusrf || !in->stagthck || !in->dusrf_dx || !in->dusrf_dy || !in->flwafact) {
+        cismgpu_set_error("assemble_3d: missing input");
+        return CISMGPU_ERR_ARG;
+    }
+    int st = upload_fe_tables();
+    if (st) return st;
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
+    const long long nvec = (long long)d.nz * n2, nfl = (long long)(d.nz - 1) * nc;
+    // device input block: doubles first, then int32s
+    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl), ni = (size_t)(nc + n2 + 2 * nvec);
+    const size_t bytes = nd * sizeof(double) + ni * sizeof(int32_t);
+    if (S->asm_bytes < bytes) {
+        if (S->asm_buf) hipFree(S->asm_buf);
+        S->asm_buf = nullptr;
+        CG_CHECK(hipMalloc(&S->asm_buf, bytes));
+        S->asm_bytes = bytes;
+    }
+    if (!S->efvs) CG_CHECK(hipMalloc(&S->efvs, nfl * sizeof(double)));
+    double* db = (double*)S->asm_buf;
+    int32_t* ib = (int32_t*)(db + nd);
+    AsmDev a{};
+    double* p = db;
+    auto putd = [&](const double* h, long long n, bool zero_if_null) -> const double* {
+        double* dst = p;
+        p += n;
+        if (h) hipMemcpyAsync(dst, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
+        else if (zero_if_null) hipMemsetAsync(dst, 0, n * sizeof(double), S->s);
+        return dst;
+    };
+    a.sigma = putd(in->sigma, d.nz, true);
+    a.swbs = putd(in->stagwbndsigma, d.nz + 1, true);
+    a.xV = putd(in->xVertex, n2, true);
+    a.yV = putd(in->yVertex, n2, true);
+    a.susrf = putd(in->stagusrf, n2, true);
+    a.sthck = putd(in->stagthck, n2, true);
+    a.dsdx = putd(in->dusrf_dx, n2, true);
+    a.dsdy = putd(in->dusrf_dy, n2, true);
+    a.beta = putd(in->beta_internal, n2, true);
+    a.theta = putd(in->theta_basal, n2, true);
+    a.flwafact = putd(in->flwafact, nfl, true);
+    int32_t* q = ib;
+    auto puti = [&](const int32_t* h, long long n) -> const int32_t* {
+        int32_t* dst = q;
+        q += n;
+        if (h) hipMemcpyAsync(dst, h, n * sizeof(int32_t), hipMemcpyHostToDevice, S->s);
+        else hipMemsetAsync(dst, 0, n * sizeof(int32_t), S->s);
+        return dst;
+    };
+    a.acell = puti(in->active_cell, nc);
+    a.avert = puti(in->active_vertex, n2);
+    a.umask = puti(in->umask_dirichlet, nvec);
+    a.vmask = puti(in->vmask_dirichlet, nvec);
+    a.efvs = S->efvs;
+    a.dx = in->dx; a.dy = in->dy; a.efvs_constant = in->efvs_constant; a.effstrain_min = in->effstrain_min;
+    a.n_glen = in->n_glen; a.whichefvs = in->whichefvs; a.nhalo = 2;
+    // active mask (int8) for the solver
+    {
+        std::vector<int8_t> act(n2);
+        for (long long t = 0; t < n2; ++t) act[t] = in->active_vertex[t] ? 1 : 0;
+        CG_CHECK(hipMemcpyAsync(S->active, act.data(), n2, hipMemcpyHostToDevice, S->s));
+        CG_CHECK(hipStreamSynchronize(S->s));
+    }
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2);
+    { const int st_ = reset_scal(S); if (st_) return st_; }
+    const double* uv = S->V(0);
+    const double* vv = S->V(1);
+    hipEventRecord(e0, S->s);
+    k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
+    hipEventRecord(e1, S->s);
+    k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
+    if (in->beta_internal) k_basal_local<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+    k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->V(2), S->V(3));
+    k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
+    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
+    halo<double>(S, S->V(2), S->V(3), d.nz);
+    unsigned long long* cnt = nullptr;
+    CG_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
+    CG_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), S->s));
+    if (in->check_symmetry) {
+        if (!S->Atmp) CG_CHECK(hipMalloc(&S->Atmp, d.nrows * NCOEF * sizeof(double)));
+        k_symmetry<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a.avert, S->A64, S->Atmp, cnt + 1);
+        std::swap(S->A64, S->Atmp);
+    }
+    k_count_nnz<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt);
+    // preconditioner coefficients: owned rows, then the halo rule on the ring
+    CG_CHECK(hipMemsetAsync(S->M64, 0, 8 * d.vlen * sizeof(double), S->s));
+    k_precond_from_rows<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, S->M64, S->M64 + 6 * d.vlen);
+    for (int f = 0; f < 8; f += 2) halo<double>(S, S->M64 + f * d.vlen, S->M64 + (f + 1) * d.vlen, d.nz);
+    // raw Auu/Avv for the horizontal tridiagonal preconditioners
+    const long long nraw = 27LL * d.nz * n2;
+    if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
+    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
+    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw);
+    hipEventRecord(e2, S->s);
+    unsigned long long hc[2] = {0, 0};
+    CG_CHECK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(sync_scal(S) ? hipErrorUnknown : hipSuccess);
+    hipFree(cnt);
+    float ms1 = 0.f, ms2 = 0.f;
+    hipEventElapsedTime(&ms1, e0, e1);
+    hipEventElapsedTime(&ms2, e0, e2);
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+    if (info) {
+        info->nnz = (int64_t)hc[0];
+        info->asym_pairs = (int64_t)hc[1];
+        info->assemble_ms = ms1;
+        info->total_ms = ms2;
+    }
+    S->have_matrix = true;
+    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->tri.mode = 0;
+    if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_get_efvs(cismgpu_system* S, double* efvs) {
+    if (!S || !S->efvs || !efvs) return CISMGPU_ERR_ARG;
+    const long long nfl = (long long)(S->d.nz - 1) * S->d.nx * S->d.ny;
+    CG_CHECK(hipMemcpyAsync(efvs, S->efvs, nfl * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_residual(cismgpu_system* S, double* ru, double* rv, double* L2, double* L2rel) {
+    if (!S || !S->have_matrix) return CISMGPU_ERR_ARG;
+    const Dims& d = S->d;
+    double *r0 = S->V(12), *r1 = S->V(13);
+    CG_CHECK(hipMemsetAsync(r0, 0, 2 * d.vlen * sizeof(double), S->s));
+    { const int st_ = reset_scal(S); if (st_) return st_; }
+    // r = b - A x over owned rows with partial (r,r); resid = -r on owned active rows
+    k_spmv<double, double, 2><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), r0, r1, S->V(2), S->V(3), S->partB);
+    k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, spmv_blocks(d), 2, nullptr, 0, 0, S->scal, 0.0);
+    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, S->V(2), S->V(2), S->V(3), S->V(3), S->partA);
+    k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->partA, dot_blocks<double>(d), 2, nullptr, 0, 0, S->scal, 0.0);
+    int st = sync_scal(S);
+    if (st) return st;
+    if (L2) *L2 = std::sqrt(S->scal_h[S_DOT]);
+    if (L2rel) *L2rel = std::sqrt(S->scal_h[S_DOT]) / S->scal_h[S_L2RHS];
+    if (ru && rv) {
+        const long long nvec = (long long)d.nz * d.nvx * d.nvy;
+        std::vector<double> h(nvec);
+        double* outs[2] = {ru, rv};
+        double* srcs[2] = {r0, r1};
+        for (int c = 0; c < 2; ++c) {
+            k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, srcs[c], S->V(10));
+            CG_CHECK(hipMemcpyAsync(outs[c], S->V(10), nvec * sizeof(double), hipMemcpyDeviceToHost, S->s));
+            CG_CHECK(hipStreamSynchronize(S->s));
+        }
+        // resid = A x - b on owned active vertices, zero elsewhere
+        for (long long t = 0; t < nvec; ++t) {
+            const long long col = t / d.nz;
+            const int i = 1 + (int)(col % d.nvx), j = 1 + (int)(col / d.nvx);
+            const bool own = i >= d.ilo && i <= d.ihi && j >= d.jlo && j <= d.jhi;
+            ru[t] = own ? -ru[t] : 0.0;
+            rv[t] = own ? -rv[t] : 0.0;
+        }
+        std::vector<int8_t> act(d.nvx * (size_t)d.nvy);
+        CG_CHECK(hipMemcpy(act.data(), S->active, act.size(), hipMemcpyDeviceToHost));
+        for (long long t = 0; t < nvec; ++t)
+            if (!act[t / d.nz]) { ru[t] = 0.0; rv[t] = 0.0; }
+        CG_CHECK(hipMemsetAsync(S->V(10), 0, d.vlen * sizeof(double), S->s));
+    }
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_halo_x(cismgpu_system* S) {
+    if (!S) return CISMGPU_ERR_ARG;
+    halo<double>(S, S->V(0), S->V(1), S->d.nz);
+    CG_CHECK(hipStreamSynchronize(S->s));
     return CISMGPU_OK;
 }
 

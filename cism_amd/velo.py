@@ -1,0 +1,221 @@
+"""3-D Blatter-Pattyn velocity solve on the GPU.
+
+Mirrors the 3-D BP branch of ``glissade_velo_higher_solve``
+(glissade_velo_higher.F90:662-3918): host-side 2-D setup (masks, staggered
+geometry, surface gradient, flow factor, beta), device assembly of the
+linear system (stiffness, basal friction, gravity load, Dirichlet rows,
+symmetry fix), device Krylov solve and the outer Picard loop on the
+effective viscosity (:2163-3663) with the residual measures of
+compute_residual_velocity_3d (:10314-10446) and compute_residual_vector_3d
+(:9693-9894).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, geometry as ge
+from .grid import stagwbndsigma
+from .pcg import ALGO_CHRONGEAR, FP64, HO_PRECOND_SIA, System
+
+
+class AssemblyIn(ctypes.Structure):
+    _P = ctypes.c_void_p
+    _fields_ = [("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("sigma", _P), ("stagwbndsigma", _P), ("active_cell", _P), ("active_vertex", _P),
+                ("xVertex", _P), ("yVertex", _P), ("stagusrf", _P), ("stagthck", _P),
+                ("dusrf_dx", _P), ("dusrf_dy", _P), ("flwafact", _P), ("beta_internal", _P),
+                ("theta_basal", _P), ("umask_dirichlet", _P), ("vmask_dirichlet", _P),
+                ("whichefvs", ctypes.c_int32), ("efvs_constant", ctypes.c_double),
+                ("effstrain_min", ctypes.c_double), ("n_glen", ctypes.c_double),
+                ("check_symmetry", ctypes.c_int32)]
+
+
+class AssemblyInfo(ctypes.Structure):
+    _fields_ = [("nnz", ctypes.c_int64), ("asym_pairs", ctypes.c_int64), ("assemble_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+
+def _declare(L):
+    if getattr(L, "_velo_declared", False):
+        return
+    L.cismgpu_system_assemble_3d.argtypes = [ctypes.c_void_p, ctypes.POINTER(AssemblyIn), ctypes.POINTER(AssemblyInfo)]
+    L.cismgpu_system_assemble_3d.restype = ctypes.c_int32
+    D = ctypes.POINTER(ctypes.c_double)
+    L.cismgpu_system_get_efvs.argtypes = [ctypes.c_void_p, D]
+    L.cismgpu_system_get_efvs.restype = ctypes.c_int32
+    L.cismgpu_system_residual.argtypes = [ctypes.c_void_p, D, D, D, D]
+    L.cismgpu_system_residual.restype = ctypes.c_int32
+    L.cismgpu_system_halo_x.argtypes = [ctypes.c_void_p]
+    L.cismgpu_system_halo_x.restype = ctypes.c_int32
+    L._velo_declared = True
+
+
+@dataclass
+class PicardResult:
+    uvel: np.ndarray
+    vvel: np.ndarray
+    efvs: np.ndarray
+    iterations: int
+    converged: bool
+    resid_velo: list = field(default_factory=list)
+    L2_norm: list = field(default_factory=list)
+    linear_iters: list = field(default_factory=list)
+    linear_err: list = field(default_factory=list)
+    assemble_ms: list = field(default_factory=list)
+    solve_ms: list = field(default_factory=list)
+
+
+class BPSolver:
+    """Device-resident BP velocity solver for one :class:`~cism_amd.synthetic.Problem`."""
+
+    def __init__(self, problem, system: System | None = None):
+        self.p = problem
+        self.grid = problem.grid
+        self.sys = system or System(self.grid)
+        self._L = _lib.load()
+        _declare(self._L)
+        self._setup()
+
+    # ---------------------------------------------------------- host setup
+    def _setup(self):
+        p, g = self.p, self.grid
+        self.lsrf = ge.calclsrf(p.thck, p.topg, p.eus)
+        self.usrf = np.asfortranarray(np.maximum(0.0, p.thck + self.lsrf))
+        self.ice_mask, self.floating_mask, self.ocean_mask, self.land_mask = ge.get_masks(
+            p.thck, p.topg, p.eus, p.thklim)
+        self.stagthck = ge.stagger(p.thck)
+        self.stagusrf = ge.stagger(self.usrf)
+        self.dusrf_dx, self.dusrf_dy = ge.surface_elevation_gradient(
+            p.dx, p.dy, self.ice_mask, self.land_mask, self.usrf, p.thck, p.thklim, p.thck_gradient_ramp,
+            p.gradient_margin, p.max_slope)
+        ge.staggered_halo_2d(g, self.dusrf_dx)
+        ge.staggered_halo_2d(g, self.dusrf_dy)
+        self.active_cell, self.active_vertex = ge.vertex_masks(self.ice_mask)
+        self.flwafact = ge.flwafact(np.asfortranarray(p.flwa), p.n_glen)
+        self.theta_basal = ge.slope_angle(p.dx, p.dy, ge.stagger(self.lsrf))
+        self.beta_internal = ge.beta_internal(p, self.floating_mask)
+        self.xV, self.yV = p.vertex_coords()
+        self.sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
+        self.swbs = np.ascontiguousarray(stagwbndsigma(p.sigma))
+        no_slip = p.whichbabc == 6
+        self.umask = np.zeros(g.vec_shape, np.int32, order="F")
+        if no_slip:
+            self.umask[g.nz - 1] = 1
+        self.vmask = self.umask.copy(order="F")
+        self._ain = AssemblyIn(
+            p.dx, p.dy, self.sigma.ctypes.data, self.swbs.ctypes.data, self.active_cell.ctypes.data,
+            self.active_vertex.ctypes.data, self.xV.ctypes.data, self.yV.ctypes.data, self.stagusrf.ctypes.data,
+            self.stagthck.ctypes.data, self.dusrf_dx.ctypes.data, self.dusrf_dy.ctypes.data,
+            self.flwafact.ctypes.data, None if no_slip else self.beta_internal.ctypes.data,
+            self.theta_basal.ctypes.data, self.umask.ctypes.data if no_slip else None,
+            self.vmask.ctypes.data if no_slip else None, p.whichefvs, p.efvs_constant, p.effstrain_min,
+            p.n_glen, 1)
+
+    def zero_inactive(self, uvel, vvel):
+        """Velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""
+        g = self.grid
+        own = g.owned_slice()
+        inact = self.active_vertex[own[1], own[2]] == 0
+        for a in (uvel, vvel):
+            o = a[own]
+            o[:, inact] = 0.0
+            a[own] = o
+
+    # ---------------------------------------------------------- device steps
+    def assemble(self, uvel=None, vvel=None):
+        """Assemble A, b for the viscosity of (uvel, vvel); None keeps the resident x."""
+        if uvel is not None:
+            u = np.asfortranarray(np.array(uvel, dtype=np.float64))
+            v = np.asfortranarray(np.array(vvel, dtype=np.float64))
+            self.zero_inactive(u, v)
+            self.sys.set_x(u, v)
+        info = AssemblyInfo()
+        _lib.check(self._L.cismgpu_system_assemble_3d(self.sys.handle, ctypes.byref(self._ain), ctypes.byref(info)),
+                   "assemble_3d")
+        return dict(nnz=info.nnz, asym_pairs=info.asym_pairs, assemble_ms=info.assemble_ms, total_ms=info.total_ms)
+
+    def efvs(self):
+        g = self.grid
+        out = np.zeros((g.nz - 1, g.nx, g.ny), order="F")
+        _lib.check(self._L.cismgpu_system_get_efvs(self.sys.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                   "get_efvs")
+        return out
+
+    def residual(self, want_vectors=False):
+        g = self.grid
+        D = ctypes.POINTER(ctypes.c_double)
+        L2 = ctypes.c_double()
+        L2r = ctypes.c_double()
+        ru = rv = None
+        if want_vectors:
+            ru, rv = g.zeros_vec(), g.zeros_vec()
+        _lib.check(self._L.cismgpu_system_residual(self.sys.handle, ru.ctypes.data_as(D) if ru is not None else None,
+                                                   rv.ctypes.data_as(D) if rv is not None else None,
+                                                   ctypes.byref(L2), ctypes.byref(L2r)), "residual")
+        return (L2.value, L2r.value, ru, rv) if want_vectors else (L2.value, L2r.value)
+
+    # ---------------------------------------------------------- Picard loop
+    def picard(self, *, maxiter=100, resid_target=1e-4, whichresid=0, algo=ALGO_CHRONGEAR,
+               precond=HO_PRECOND_SIA, ncheck=5, linear_tol=1e-8, linear_maxiters=200, precision=FP64,
+               inner_tol=1e-3, max_outer=20, verbose=False) -> PicardResult:
+        """Outer Picard iteration on the viscosity (glissade_velo_higher.F90:2163-3663),
+        which_ho_nonlinear = 0.  Convergence: resid_velo < resid_target for
+        which_ho_resid 0/1/2, or L2 < resid_target for 3 (:3652-3661)."""
+        g = self.grid
+        u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
+        v = np.asfortranarray(np.array(self.p.vvel, dtype=np.float64))
+        self.zero_inactive(u, v)
+        self.sys.set_x(u, v)
+        res = PicardResult(u, v, None, 0, False)
+        criterion = np.inf
+        counter = 0
+        while criterion >= resid_target and counter < maxiter:
+            counter += 1
+            usav, vsav = self.sys.get_x()
+            ai = self.assemble()
+            res.assemble_ms.append(ai["total_ms"])
+            if ai["nnz"] == 0:          # nothing to solve (:2937-3032)
+                z = g.zeros_vec()
+                res.uvel, res.vvel = z, g.zeros_vec()
+                res.converged = True
+                break
+            L2, L2r = self.residual()
+            res.L2_norm.append(L2)
+            si = self.sys.solve(algo=algo, precond=precond, ncheck=ncheck, maxiters=linear_maxiters,
+                                tolerance=linear_tol, precision=precision, inner_tol=inner_tol, max_outer=max_outer,
+                                raise_on_error=precision == FP64)
+            res.linear_iters.append(si["niters"])
+            res.linear_err.append(si["err"])
+            res.solve_ms.append(si["solve_ms"])
+            _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+            uu, vv = self.sys.get_x()
+            rv = residual_velocity(g, whichresid, usav, vsav, uu, vv)
+            res.resid_velo.append(rv)
+            criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
+            if verbose:
+                print(f"picard {counter}: L2={L2:.4e} resid_velo={rv:.4e} linear iters={si['niters']} "
+                      f"err={si['err']:.2e}", flush=True)
+        res.iterations = counter
+        res.converged = criterion < resid_target
+        res.uvel, res.vvel = self.sys.get_x()
+        res.efvs = self.efvs()
+        return res
+
+
+def residual_velocity(grid, whichresid, usav, vsav, uvel, vvel):
+    """compute_residual_velocity_3d (glissade_velo_higher.F90:10314-10446)."""
+    own = grid.owned_slice()
+    u, v, u0, v0 = uvel[own], vvel[own], usav[own], vsav[own]
+    if whichresid in (1, 2):
+        u, v, u0, v0 = u[:-1], v[:-1], u0[:-1], v0[:-1]
+    speed = np.sqrt(u * u + v * v)
+    old = np.sqrt(u0 * u0 + v0 * v0)
+    nz_ = speed != 0.0
+    if not nz_.any():
+        return 0.0
+    d = np.abs((old[nz_] - speed[nz_]) / speed[nz_])
+    return float(d.mean()) if whichresid == 2 else float(d.max())

@@ -146,6 +146,60 @@ int32_t cismgpu_system_matvec(cismgpu_system* sys, double* yu, double* yv);
 int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nrep,
                                   double* avg_ms, double* bytes_per_launch);
 
+
+/* staggered_parallel_halo of the resident x (glissade_velo_higher.F90:3598-3600) */
+int32_t cismgpu_system_halo_x(cismgpu_system* sys);
+
+/* ------------------------------------------------------------------ */
+/* Device assembly of the linear BP system (3-D branch of              */
+/* glissade_velo_higher_solve, glissade_velo_higher.F90:2815-2931):    */
+/* assemble_stiffness_matrix_3d, basal_sliding_bc_3d (local), b = load  */
+/* (load_vector_gravity, local), dirichlet_boundary_conditions_3d, the  */
+/* halo of A and b, check_symmetry_assembled_matrix_3d.  The result is  */
+/* left in the system (owned rows) for cismgpu_system_solve.  The       */
+/* velocity that sets the viscosity is the system's resident x          */
+/* (cismgpu_system_set_x).  Host arrays, reference layouts.             */
+/* ------------------------------------------------------------------ */
+typedef struct cismgpu_assembly_in {
+    double dx, dy;
+    const double*  sigma;            /* (nz) */
+    const double*  stagwbndsigma;    /* (0:nz) */
+    const int32_t* active_cell;      /* (nx, ny) */
+    const int32_t* active_vertex;    /* (nx-1, ny-1) */
+    const double*  xVertex;          /* (nx-1, ny-1) */
+    const double*  yVertex;
+    const double*  stagusrf;         /* (nx-1, ny-1) */
+    const double*  stagthck;
+    const double*  dusrf_dx;         /* (nx-1, ny-1), halo already applied */
+    const double*  dusrf_dy;
+    const double*  flwafact;         /* (nz-1, nx, ny): 0.5 * A^(-1/n) */
+    const double*  beta_internal;    /* (nx-1, ny-1); NULL: no basal friction term (no-slip) */
+    const double*  theta_basal;      /* (nx-1, ny-1) basal slope angle; NULL: flat */
+    const int32_t* umask_dirichlet;  /* (nz, nx-1, ny-1); NULL: none */
+    const int32_t* vmask_dirichlet;
+    int32_t whichefvs;               /* 0 constant, 1 flow factor, 2 nonlinear (glide_types.F90:240-242) */
+    double  efvs_constant;
+    double  effstrain_min;
+    double  n_glen;
+    int32_t check_symmetry;          /* 1: check_symmetry_assembled_matrix_3d */
+} cismgpu_assembly_in;
+
+typedef struct cismgpu_assembly_info {
+    int64_t nnz;                     /* count_nonzeros_3d over owned active rows */
+    int64_t asym_pairs;              /* entry pairs left asymmetric (the reference warns) */
+    double  assemble_ms;             /* device time of the element kernel */
+    double  total_ms;                /* device time of the whole assembly */
+} cismgpu_assembly_info;
+
+int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_in* in,
+                                   cismgpu_assembly_info* info);
+/* efvs(nz-1, nx, ny) of the last assembly (mean over quadrature points) */
+int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
+/* resid = A x - b on owned active vertices, L2 norm and norm relative to
+ * |b| (compute_residual_vector_3d, glissade_velo_higher.F90:9693-9894). */
+int32_t cismgpu_system_residual(cismgpu_system* sys, double* resid_u, double* resid_v,
+                                double* L2_norm, double* L2_norm_relative);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,3 +289,58 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     return dict(Auu=A[0], Auv=A[1], Avu=A[2], Avv=A[3], bu=bu, bv=bv, efvs=efvs, uvel=uvel, vvel=vvel,
                 active_vertex=av, loadu=loadu, loadv=loadv, umask=umask, vmask=vmask,
                 beta_internal=binternal, nwarn_asym=nwarn, nnz=nnz, geo=geo)
+
+
+def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
+    """compute_residual_vector_3d (glissade_velo_higher.F90:9693-9894): L2 of A u - b
+    over owned active vertices, and relative to |b|."""
+    yu, yv = matvec(grid, active_vertex, *A, uvel, vvel)
+    own = grid.owned_slice()
+    act = np.asarray(active_vertex, bool)[own[1], own[2]][None]
+    ru = np.where(act, yu[own] - bu[own], 0.0)
+    rv = np.where(act, yv[own] - bv[own], 0.0)
+    wu = np.zeros(grid.vec_shape, order="F"); wv = np.zeros(grid.vec_shape, order="F")
+    wu[own] = ru * ru; wv[own] = rv * rv
+    L2 = np.sqrt(global_sum(grid, wu, wv))
+    bu2 = np.zeros(grid.vec_shape, order="F"); bv2 = np.zeros(grid.vec_shape, order="F")
+    bu2[own] = np.where(act, bu[own] * bu[own], 0.0); bv2[own] = np.where(act, bv[own] * bv[own], 0.0)
+    return L2, L2 / np.sqrt(global_sum(grid, bu2, bv2))
+
+
+def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chrongear", precond=2, ncheck=5,
+           linear_tol=1e-8, linear_maxiters=200):
+    """Outer Picard loop of glissade_velo_higher_solve for the BP path
+    (glissade_velo_higher.F90:2163-3663, which_ho_nonlinear = 0)."""
+    L = lib()
+    p = problem
+    g = p.grid
+    geo = geometry(p)
+    u = np.asfortranarray(p.uvel.copy())
+    v = np.asfortranarray(p.vvel.copy())
+    own = g.owned_slice()
+    inact = geo["active_vertex"][own[1], own[2]] == 0
+    for a in (u, v):
+        o = a[own]; o[:, inact] = 0.0; a[own] = o
+    hist = dict(L2=[], resid_velo=[], linear_iters=[], linear_err=[])
+    criterion = np.inf
+    counter = 0
+    s = None
+    while criterion >= resid_target and counter < maxiter:
+        counter += 1
+        usav, vsav = u.copy(order="F"), v.copy(order="F")
+        s = assemble(p, u, v, geo)
+        A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+        L2, L2r = residual_L2(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
+        hist["L2"].append(L2)
+        st, err, nit = pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], u, v, algo=algo, precond=precond,
+                                 ncheck=ncheck, tolerance=linear_tol, maxiters=linear_maxiters)
+        if st:
+            raise RuntimeError(f"oracle solver failed with status {st}")
+        hist["linear_iters"].append(nit); hist["linear_err"].append(err)
+        halo(g, u); halo(g, v)
+        rv = L.orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                        dp(usav), dp(vsav), dp(u), dp(v))
+        hist["resid_velo"].append(rv)
+        criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
+    return dict(uvel=u, vvel=v, efvs=s["efvs"] if s else None, iterations=counter,
+                converged=criterion < resid_target, **hist)
