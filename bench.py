@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: mixed-precision vs plain FP64 velocity solve on one MI355X.
+
+Workload: the synthetic ice-stream system at 1024x1024 vertices x 11 levels
+(23,068,672 unknowns), assembled on the GPU from a plug-plus-shear velocity
+guess.  A "step" is one complete linear solve of that system from x0 = 0 to
+a relative residual of 1e-10 with the Chronopoulos-Gear method and the SIA
+column-tridiagonal preconditioner:
+
+* mixed: FP32 matrix and FP32 inner Krylov iterations inside FP64 iterative
+  refinement (FP64 residual on the FP64 matrix, compensated FP64 sums);
+* fp64:  the plain FP64 drop-in solver (reference semantics).
+
+value = mixed solves per second; the FP64 rate, the speedup, the error of
+each solution against a tight (1e-12) FP64 solve, the SpMV bandwidth and a
+CPU-restatement baseline are reported beside it.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E, 8 TB/s
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024, help="owned vertices per horizontal direction")
+    ap.add_argument("--nz", type=int, default=11)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--cpu-iters", type=int, default=10, help="C-G iterations of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the single-GPU share of the path); --gpus {args.gpus} is not "
+              f"supported: the multi-GPU halo/all-reduce design is described but not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from cism_amd import pcg, synthetic
+    from cism_amd.velo import BPSolver
+
+    t0 = time.time()
+    log(f"generating synthetic ice stream {args.n}x{args.n}x{args.nz} seed={args.seed}")
+    prob = synthetic.ice_stream(args.n, nz=args.nz, seed=args.seed)
+    g = prob.grid
+    nodes = g.n_owned_nodes
+    log(f"grid nx={g.nx} ny={g.ny} nz={g.nz}: {nodes} owned nodes, {2 * nodes} unknowns")
+    bp = BPSolver(prob)
+    ainfo = bp.assemble(prob.uvel, prob.vvel)
+    s = bp.sys
+    log(f"GPU assembly: element kernel {ainfo['assemble_ms']:.1f} ms, total {ainfo['total_ms']:.1f} ms, "
+        f"nnz {ainfo['nnz']}")
+    lowp = pcg.MIXED_FP32 if args.precision == "fp32" else pcg.MIXED_BF16
+
+    # tight FP64 reference solution (untimed)
+    s.zero_x()
+    rref = s.solve(precond=2, tolerance=1e-12, maxiters=50000)
+    xref = np.stack(s.get_x())
+    log(f"reference FP64 solve to 1e-12: {rref['niters']} iterations, {rref['solve_ms']:.1f} ms")
+
+    def rel_err():
+        x = np.stack(s.get_x())
+        return float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+
+    def fp64_step(time_spmv=False):
+        s.zero_x()
+        return s.solve(precond=2, tolerance=args.tol, maxiters=50000, time_spmv=time_spmv)
+
+    def mixed_step(time_spmv=False):
+        s.zero_x()
+        return s.solve(precond=2, tolerance=args.tol, maxiters=50000, precision=lowp, inner_tol=args.inner_tol,
+                       max_outer=40, time_spmv=time_spmv)
+
+    for w in range(args.warmup):
+        r = fp64_step()
+        log(f"warmup fp64 {w}: {r['niters']} it {r['solve_ms']:.1f} ms")
+        r = mixed_step()
+        log(f"warmup mixed {w}: {r['niters']} inner it, {r['outer_iters']} outer, {r['solve_ms']:.1f} ms")
+
+    # ---- timed FP64 solves
+    t64, it64, sp64 = [], [], []
+    for k in range(args.steps):
+        tw = time.perf_counter()
+        r = fp64_step(time_spmv=True)
+        t64.append(time.perf_counter() - tw)
+        it64.append(r["niters"])
+        sp64.append((r["spmv_ms"], r["spmv_count"]))
+        log(f"fp64 step {k}: {r['niters']} it, device {r['solve_ms']:.1f} ms, wall {t64[-1] * 1e3:.1f} ms")
+    err64 = rel_err()
+
+    # ---- timed mixed solves (the metric)
+    tm, itm, outm, spm = [], [], [], []
+    for k in range(args.steps):
+        tw = time.perf_counter()
+        r = mixed_step(time_spmv=True)
+        tm.append(time.perf_counter() - tw)
+        itm.append(r["niters"])
+        outm.append(r["outer_iters"])
+        spm.append((r["spmv_ms"], r["spmv_count"]))
+        log(f"mixed step {k}: {r['niters']} inner it, {r['outer_iters']} outer, device {r['solve_ms']:.1f} ms, "
+            f"wall {tm[-1] * 1e3:.1f} ms, err {r['err']:.2e}")
+    errm = rel_err()
+
+    ms_mixed = 1e3 * sum(tm) / len(tm)
+    ms_fp64 = 1e3 * sum(t64) / len(t64)
+    # dominant kernel of the mixed solve: the low-precision SpMV, timed live
+    sp_ms = sum(a for a, _ in spm)
+    sp_n = sum(b for _, b in spm)
+    esz = 4 if lowp == pcg.MIXED_FP32 else 2
+    bytes_launch = nodes * (108 * esz + 4 * 4)
+    spmv_avg_ms = sp_ms / max(sp_n, 1)
+    achieved = bytes_launch / (spmv_avg_ms * 1e-3) / 1e9
+    sp64_ms = sum(a for a, _ in sp64) / max(sum(b for _, b in sp64), 1)
+    ach64 = nodes * (108 * 8 + 4 * 8) / (sp64_ms * 1e-3) / 1e9
+
+    # isolated SpMV rates (back-to-back launches, HIP events)
+    spmv_iso = {}
+    for kind, name in ((0, "fp64"), (1, "fp32"), (2, "bf16")):
+        ms, by = s.bench_spmv(kind, 20)
+        spmv_iso[name] = dict(avg_ms=round(ms, 4), algorithmic_bytes=int(by), gbps=round(by / (ms * 1e-3) / 1e9, 1),
+                              frac_of_peak=round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3))
+    log(f"isolated SpMV: {spmv_iso}")
+
+    # ---- CPU baseline: the C restatement of pcg_solver_chrongear_3d on the same
+    # system, a bounded sample of iterations, extrapolated to the FP64 iteration count
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import pyoracle as po
+            log("exporting the assembled system for the CPU baseline")
+            (Auu, Auv, Avu, Avv), (bu, bv) = s.get_matrix()
+            xu, xv = g.zeros_vec(), g.zeros_vec()
+            nthreads = args.cpu_threads
+            tc = time.perf_counter()
+            st, err, nit = po.pcg_solve(g, bp.active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond=2,
+                                        tolerance=0.0, maxiters=args.cpu_iters,
+                                        nthreads=None if nthreads == 1 else nthreads)
+            tcpu = time.perf_counter() - tc
+            per_iter = tcpu / args.cpu_iters
+            full_iters = int(round(sum(it64) / len(it64)))
+            cpu_solve_s = per_iter * full_iters
+            cpu = dict(value=round(1.0 / cpu_solve_s, 6), unit="solves/s", cores=nthreads, kind="port",
+                       sample=(f"{args.cpu_iters} Chronopoulos-Gear iterations (SIA preconditioner, checks at "
+                               f"iter 5 and 10) of the same {args.n}x{args.n}x{args.nz} assembled FP64 system by "
+                               f"the C restatement in oracle/, {tcpu:.1f} s; solve time = {per_iter * 1e3:.0f} "
+                               f"ms/iteration x {full_iters} iterations (the GPU FP64 count to {args.tol:g})"),
+                       seconds_per_solve=round(cpu_solve_s, 2))
+            log(f"CPU baseline: {per_iter * 1e3:.0f} ms/iteration -> {cpu_solve_s:.1f} s per solve")
+            del Auu, Auv, Avu, Avv
+        except Exception as e:   # the CPU leg must not hide the GPU result
+            log(f"CPU baseline failed: {e!r}")
+
+    out = {
+        "metric": "velocity solves/sec mixed vs FP64 at 1e-6 rel err; SpMV HBM GB/s vs peak",
+        "value": round(1e3 / ms_mixed, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if lowp == pcg.MIXED_FP32 else "bf16+f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"synthetic ice stream {args.n}x{args.n}x{args.nz} (mixed-precision C-G solve, FP64 refinement)",
+            "n": args.n, "nz": args.nz, "owned_nodes": nodes, "unknowns": 2 * nodes, "seed": args.seed,
+            "precond": "SIA column tridiagonal (which_ho_precond=2)", "tolerance": args.tol,
+            "inner_tol": args.inner_tol, "matrix_storage": args.precision, "parallelism": "single GPU",
+        },
+        "fp64": {"solves_per_s": round(1e3 / ms_fp64, 4), "ms_per_solve": round(ms_fp64, 2),
+                 "iterations": it64, "rel_err_vs_tight_fp64": err64,
+                 "spmv_avg_ms": round(sp64_ms, 4), "spmv_gbps": round(ach64, 1)},
+        "mixed": {"inner_iterations": itm, "outer_iterations": outm, "rel_err_vs_tight_fp64": errm,
+                  "meets_1e-6": bool(errm <= 1e-6)},
+        "speedup_mixed_vs_fp64": round(ms_fp64 / ms_mixed, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": f"k_spmv<{args.precision},float,MODE1> (inner SpMV + dot)",
+                     "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": round(spmv_avg_ms, 4),
+                     "launches_timed": sp_n},
+        "spmv_isolated": spmv_iso,
+        "assembly": {"element_kernel_ms": round(ainfo["assemble_ms"], 2), "total_ms": round(ainfo["total_ms"], 2)},
+        "cpu_baseline": cpu,
+        "wall_s": round(time.time() - t0, 1),
+    }
+    line = json.dumps(out)
+    print(line, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -31,7 +31,7 @@ EXPORTS = (
     "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix", "cismgpu_system_get_matrix",
     "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve", "cismgpu_system_matvec",
     "cismgpu_system_bench_spmv", "cismgpu_system_halo_x", "cismgpu_system_assemble_3d", "cismgpu_system_get_efvs",
-    "cismgpu_system_residual",
+    "cismgpu_system_residual", "cismgpu_system_zero_x",
 )
 
 
@@ -87,7 +87,8 @@ def _declare(L):
     L.cismgpu_system_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
     L.cismgpu_system_matvec.argtypes = [vp, _D, _D]
     L.cismgpu_system_bench_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _D, _D]
-    for n in ("cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix",
+    L.cismgpu_system_zero_x.argtypes = [vp]
+    for n in ("cismgpu_system_zero_x", "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix",
               "cismgpu_system_get_matrix", "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve",
               "cismgpu_system_matvec", "cismgpu_system_bench_spmv"):
         getattr(L, n).restype = ctypes.c_int32

@@ -1078,6 +1078,13 @@ extern "C" int32_t cismgpu_system_halo_x(cismgpu_system* S) {
     return CISMGPU_OK;
 }
 
+extern "C" int32_t cismgpu_system_zero_x(cismgpu_system* S) {
+    if (!S) return CISMGPU_ERR_ARG;
+    CG_CHECK(hipMemsetAsync(S->V(0), 0, 2 * S->d.vlen * sizeof(double), S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
 // ---------------------------------------------------------------- drop-in entry points
 static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, const double* Auu, const double* Auv,
                   const double* Avu, const double* Avv, const double* bu, const double* bv, double* xu, double* xv,

@@ -150,6 +150,9 @@ class System:
         _lib.check(self._L.cismgpu_system_get_x(self._h, _d(xu), _d(xv)), "system_get_x")
         return xu, xv
 
+    def zero_x(self):
+        _lib.check(self._L.cismgpu_system_zero_x(self._h), "system_zero_x")
+
     def matvec(self):
         yu = self.grid.zeros_vec()
         yv = self.grid.zeros_vec()

@@ -147,6 +147,9 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nre
                                   double* avg_ms, double* bytes_per_launch);
 
 
+/* x = 0 on the device (initial guess for benchmarks) */
+int32_t cismgpu_system_zero_x(cismgpu_system* sys);
+
 /* staggered_parallel_halo of the resident x (glissade_velo_higher.F90:3598-3600) */
 int32_t cismgpu_system_halo_x(cismgpu_system* sys);
 
