@@ -1,0 +1,56 @@
+"""The C-ABI library builds, loads without a GPU and exports every entry
+point that include/cismgpu.h declares; without a device it fails loudly."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from cism_amd import _lib
+from cism_amd.grid import Grid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "cismgpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cismgpu_\w+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d",
+              "cismgpu_matvec_multiply_structured_3d", "cismgpu_system_assemble_3d"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r"\sT\s(cismgpu_\w+)", out.stdout))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    for n in declared_functions():
+        assert hasattr(L, n)
+    assert set(_lib.EXPORTS) <= exported
+
+
+def test_no_device_fails_loudly():
+    L = _lib.load()
+    if L.cismgpu_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    g = Grid.from_owned(8, 8, 5).cstruct()
+    h = ctypes.c_void_p()
+    st = L.cismgpu_system_create(ctypes.byref(g), ctypes.byref(h))
+    assert st == 10 and not h.value
+    assert "device" in _lib.last_error().lower()
+
+
+def test_bad_grid_is_rejected():
+    L = _lib.load()
+    g = Grid.from_owned(8, 8, 5)
+    g.ilo = 1   # owned bounds inside the halo
+    h = ctypes.c_void_p()
+    assert L.cismgpu_system_create(ctypes.byref(g.cstruct()), ctypes.byref(h)) == 3

@@ -1,0 +1,47 @@
+"""Host-side 2-D setup of the product (cism_amd/geometry.py) equals the
+restated reference operators bit for bit."""
+import numpy as np
+import pytest
+
+from cism_amd import geometry as ge, synthetic
+from oracle import pyoracle as po
+
+
+@pytest.mark.parametrize("make", [lambda: synthetic.dome(), lambda: synthetic.ice_stream(40, seed=3),
+                                  lambda: synthetic.slab(n_cells=6, nz=5)])
+def test_geometry_bitwise(make):
+    p = make()
+    geo = po.geometry(p)
+    g = p.grid
+    lsrf = ge.calclsrf(p.thck, p.topg, p.eus)
+    usrf = np.asfortranarray(np.maximum(0, p.thck + lsrf))
+    ice, flt, ocn, land = ge.get_masks(p.thck, p.topg, p.eus, p.thklim)
+    dsdx, dsdy = ge.surface_elevation_gradient(p.dx, p.dy, ice, land, usrf, p.thck, p.thklim,
+                                               p.thck_gradient_ramp, p.gradient_margin, p.max_slope)
+    ge.staggered_halo_2d(g, dsdx)
+    ge.staggered_halo_2d(g, dsdy)
+    acell, avert = ge.vertex_masks(ice)
+    pairs = [(lsrf, geo["lsrf"]), (usrf, geo["usrf"]), (ice, geo["ice_mask"]), (flt, geo["floating_mask"]),
+             (ocn, geo["ocean_mask"]), (land, geo["land_mask"]), (ge.stagger(p.thck), geo["stagthck"]),
+             (ge.stagger(usrf), geo["stagusrf"]), (dsdx, geo["dusrf_dx"]), (dsdy, geo["dusrf_dy"]),
+             (acell, geo["active_cell"]), (avert, geo["active_vertex"]),
+             (ge.flwafact(p.flwa, p.n_glen), geo["flwafact"]),
+             (ge.slope_angle(p.dx, p.dy, ge.stagger(lsrf)), geo["theta_basal"])]
+    for a, b in pairs:
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("margin", [0, 1, 2])
+def test_gradient_margin_options(margin):
+    p = synthetic.dome()
+    p.gradient_margin = margin
+    p.thck_gradient_ramp = 50.0 if margin == 1 else 0.0
+    geo = po.geometry(p)
+    lsrf = ge.calclsrf(p.thck, p.topg, p.eus)
+    usrf = np.asfortranarray(np.maximum(0, p.thck + lsrf))
+    ice, flt, ocn, land = ge.get_masks(p.thck, p.topg, p.eus, p.thklim)
+    dsdx, dsdy = ge.surface_elevation_gradient(p.dx, p.dy, ice, land, usrf, p.thck, p.thklim,
+                                               p.thck_gradient_ramp, margin, p.max_slope)
+    ge.staggered_halo_2d(p.grid, dsdx)
+    ge.staggered_halo_2d(p.grid, dsdy)
+    assert np.array_equal(dsdx, geo["dusrf_dx"]) and np.array_equal(dsdy, geo["dusrf_dy"])

@@ -1,0 +1,468 @@
+"""Pins of the CPU restatement (oracle/) against mathematics the reference
+path must satisfy -- the reference ships no golden vectors for this path.
+
+Each check is independent of the restated formula it pins: dense or sparse
+linear algebra, closed-form integrals, exact rational sums, textbook PCG,
+and the closed-form first-order slab solution of tests/slab/plotSlab.py.
+"""
+from fractions import Fraction
+from functools import reduce
+from math import cos, pi, sin, tan
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from cism_amd import synthetic
+from cism_amd.grid import BC_NO_ICE, BC_OUTFLOW, BC_PERIODIC, glide_sigma, indxA_3d
+from oracle import pyoracle as po
+from systems import dome_system, random_spd_system, stream_system
+
+
+# ----------------------------------------------------------------- helpers
+def owned_operator(g, act, A):
+    """Sparse operator on owned-vertex unknowns implied by the structured
+    matvec and the one-task halo rule (periodic: halo = owned image; other
+    rules: halo = 0).  Unknown order: (comp, k, i, j) over owned vertices."""
+    ni, nj, nz = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1, g.nz
+    periodic = g.global_bc in (0, 2)
+
+    def uid(c, k, i, j):
+        return (((c * nz + k) * nj) + (j - g.jlo)) * ni + (i - g.ilo)
+
+    rows, cols, vals = [], [], []
+    for j in range(g.jlo, g.jhi + 1):
+        for i in range(g.ilo, g.ihi + 1):
+            if not act[i - 1, j - 1]:
+                continue
+            for k in range(1, nz + 1):
+                for kA in (-1, 0, 1):
+                    if not 1 <= k + kA <= nz:
+                        continue
+                    for jA in (-1, 0, 1):
+                        for iA in (-1, 0, 1):
+                            qi, qj = i + iA, j + jA
+                            if not (g.ilo <= qi <= g.ihi and g.jlo <= qj <= g.jhi):
+                                if not periodic:
+                                    continue
+                                qi = g.ilo + (qi - g.ilo) % ni
+                                qj = g.jlo + (qj - g.jlo) % nj
+                            m = indxA_3d(iA, jA, kA) - 1
+                            for cr, blkpair in ((0, (0, 1)), (1, (2, 3))):
+                                for cc, blk in enumerate(blkpair):
+                                    v = A[blk][m, k - 1, i - 1, j - 1]
+                                    if v != 0.0:
+                                        rows.append(uid(cr, k - 1, i, j))
+                                        cols.append(uid(cc, k + kA - 1, qi, qj))
+                                        vals.append(v)
+    n = 2 * nz * ni * nj
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    return M, uid
+
+
+def to_owned(g, u, v):
+    o = g.owned_slice()
+    return np.concatenate([np.transpose(u[o], (0, 2, 1)).ravel(), np.transpose(v[o], (0, 2, 1)).ravel()])
+
+
+SPD = {bc: random_spd_system(bc=bc) for bc in (BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW)}
+
+
+# ----------------------------------------------------------------- FE tables
+def test_fe_tables_identities():
+    """Partition of unity and zero-sum gradients at every quadrature point
+    (glissade_velo_higher.F90:398-401); the 2x2x2 rule integrates each
+    basis function to 1/8 of the reference cube volume 8."""
+    phi, dxr, dyr, dzr = po.fe_tables()
+    assert np.allclose(phi.sum(axis=0), 1.0, atol=1e-15)
+    for d in (dxr, dyr, dzr):
+        assert np.allclose(d.sum(axis=0), 0.0, atol=1e-15)
+    assert np.allclose(phi.sum(axis=1), 1.0, atol=1e-15)
+
+
+# ----------------------------------------------------------------- tridiagonal
+def test_thomas_matches_dense():
+    rng = np.random.default_rng(0)
+    for n in (2, 5, 11, 15):
+        sb = rng.standard_normal(n)
+        sp_ = rng.standard_normal(n)
+        d = 4.0 + np.abs(sb) + np.abs(sp_)
+        rhs = rng.standard_normal(n)
+        T = np.diag(d) + np.diag(sb[1:], -1) + np.diag(sp_[:-1], 1)
+        x = po.tridiag(sb, d, sp_, rhs)
+        assert np.allclose(x, np.linalg.solve(T, rhs), rtol=1e-13, atol=1e-14)
+
+
+# ----------------------------------------------------------------- matvec
+@pytest.mark.parametrize("bc", list(SPD))
+def test_matvec_matches_sparse_operator(bc):
+    g, act, A, bu, bv = SPD[bc]
+    M, _ = owned_operator(g, act, A)
+    rng = np.random.default_rng(2)
+    xu = np.asfortranarray(rng.standard_normal(g.vec_shape))
+    xv = np.asfortranarray(rng.standard_normal(g.vec_shape))
+    po.halo(g, xu)
+    po.halo(g, xv)
+    yu, yv = po.matvec(g, act, *A, xu, xv)
+    y = M @ to_owned(g, xu, xv)
+    assert np.allclose(to_owned(g, yu, yv), y, rtol=1e-12, atol=1e-12 * np.abs(y).max())
+
+
+def test_matvec_dome_matches_sparse_operator():
+    p, s = dome_system()
+    g = p.grid
+    A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+    M, _ = owned_operator(g, s["active_vertex"], A)
+    rng = np.random.default_rng(3)
+    xu = np.asfortranarray(rng.standard_normal(g.vec_shape))
+    xv = np.asfortranarray(rng.standard_normal(g.vec_shape))
+    po.halo(g, xu)
+    po.halo(g, xv)
+    yu, yv = po.matvec(g, s["active_vertex"], *A, xu, xv)
+    y = M @ to_owned(g, xu, xv)
+    assert np.allclose(to_owned(g, yu, yv), y, rtol=1e-12, atol=1e-12 * np.abs(y).max())
+
+
+# ----------------------------------------------------------------- solvers
+def sia_block_inverse(g, act, A):
+    """Dense-free application of the SIA preconditioner: per active column
+    and component, the vertical tridiagonal part of Auu / Avv."""
+    def apply(r):
+        ni, nj, nz = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1, g.nz
+        z = np.zeros_like(r)
+        R = r.reshape(2, nz, nj, ni)
+        Z = z.reshape(2, nz, nj, ni)
+        for c, blk in ((0, 0), (1, 3)):
+            for jo in range(nj):
+                for io in range(ni):
+                    i, j = g.ilo + io, g.jlo + jo
+                    if not act[i - 1, j - 1]:
+                        continue
+                    col = A[blk][:, :, i - 1, j - 1]
+                    T = np.diag(col[13]) + np.diag(col[4, 1:], -1) + np.diag(col[22, :-1], 1)
+                    Z[c, :, jo, io] = np.linalg.solve(T, R[c, :, jo, io])
+        return z
+    return apply
+
+
+def textbook_pcg(M, b, minv, iters):
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = minv(r)
+    p = z.copy()
+    rz = r @ z
+    out = []
+    for _ in range(iters):
+        q = M @ p
+        a = rz / (p @ q)
+        x = x + a * p
+        r = r - a * q
+        z = minv(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        out.append(x.copy())
+    return out
+
+
+@pytest.mark.parametrize("algo", ["standard", "chrongear"])
+@pytest.mark.parametrize("bc", [BC_PERIODIC, BC_NO_ICE])
+def test_iterates_equal_textbook_pcg(bc, algo):
+    """Chronopoulos-Gear is algebraically standard PCG: the restated solvers'
+    k-th iterates equal a textbook PCG with the same (SIA) preconditioner on
+    the same operator.  Iterations 5 and 10 replace r by b - Ax, which is
+    exact in exact arithmetic."""
+    g, act, A, bu, bv = SPD[bc]
+    M, _ = owned_operator(g, act, A)
+    b = to_owned(g, bu, bv)
+    ref = textbook_pcg(M, b, sia_block_inverse(g, act, A), 12)
+    for k in (1, 2, 3, 5, 8, 12):
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        st, err, nit = po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo=algo, precond=2, tolerance=0.0, maxiters=k)
+        assert st == 0 and nit == k
+        x = to_owned(g, xu, xv)
+        assert np.linalg.norm(x - ref[k - 1]) <= 1e-10 * np.linalg.norm(ref[k - 1]), k
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("algo", ["standard", "chrongear"])
+def test_converged_solution_matches_sparse_solve(algo, precond):
+    for bc in (BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW):
+        g, act, A, bu, bv = SPD[bc]
+        M, _ = owned_operator(g, act, A)
+        b = to_owned(g, bu, bv)
+        keep = np.asarray(M.getnnz(axis=1) > 0)
+        xs = np.zeros_like(b)
+        xs[keep] = spla.spsolve(M[keep][:, keep].tocsc(), b[keep])
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        st, err, nit = po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo=algo, precond=precond, tolerance=1e-12,
+                                    maxiters=3000)
+        assert st == 0 and err < 1e-12
+        x = to_owned(g, xu, xv)
+        assert np.linalg.norm(x - xs) <= 1e-10 * np.linalg.norm(xs), (bc, precond)
+
+
+def test_stream_solution_matches_sparse_solve():
+    p, s = stream_system()
+    g = p.grid
+    A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+    M, _ = owned_operator(g, s["active_vertex"], A)
+    b = to_owned(g, s["bu"], s["bv"])
+    keep = np.asarray(M.getnnz(axis=1) > 0)
+    xs = np.zeros_like(b)
+    xs[keep] = spla.spsolve(M[keep][:, keep].tocsc(), b[keep])
+    xu, xv = g.zeros_vec(), g.zeros_vec()
+    st, err, nit = po.pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], xu, xv, precond=2, tolerance=1e-12,
+                                maxiters=5000)
+    assert st == 0
+    assert np.linalg.norm(to_owned(g, xu, xv) - xs) <= 1e-9 * np.linalg.norm(xs)
+
+
+def test_tridiagonal_preconditioner_first_step():
+    """precond 3: z = SIA^-1 r + (column-broadcast of the line solves of the
+    vertically summed system).  The first standard-PCG iterate is
+    x1 = alpha z0 with alpha = (b,z0)/(z0,A z0), z0 = PC(b)."""
+    g, act, A, bu, bv = SPD[BC_NO_ICE]
+    M, uid = owned_operator(g, act, A)
+    b = to_owned(g, bu, bv)
+    ni, nj, nz = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1, g.nz
+    z = sia_block_inverse(g, act, A)(b)
+    B = b.reshape(2, nz, nj, ni).sum(axis=1)
+    Z = z.reshape(2, nz, nj, ni)
+    for c, blk, m_sub, m_sup in ((0, 0, 4, 6), (1, 3, 2, 8)):
+        vs = [A[blk][[m - 1, m + 8, m + 17]].sum(axis=(0, 1)) for m in (m_sub, 5, m_sup)]
+        sub, dia, sup = [v[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi] for v in vs]
+        a = act[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi].astype(bool)
+        sub = np.where(a, sub, 0.0); sup = np.where(a, sup, 0.0); dia = np.where(a, dia, 1.0)
+        bad = dia == 0.0
+        dia = np.where(bad, 1.0, dia); sub = np.where(bad, 0.0, sub); sup = np.where(bad, 0.0, sup)
+        if c == 0:   # u: lines along i for each j
+            for jo in range(nj):
+                T = np.diag(dia[:, jo]) + np.diag(sub[1:, jo], -1) + np.diag(sup[:-1, jo], 1)
+                Z[c, :, jo, :] += np.linalg.solve(T, B[c, jo, :])[None, :]
+        else:        # v: lines along j for each i
+            for io in range(ni):
+                T = np.diag(dia[io, :]) + np.diag(sub[io, 1:], -1) + np.diag(sup[io, :-1], 1)
+                Z[c, :, :, io] += np.linalg.solve(T, B[c, :, io])[None, :]
+    z = Z.ravel()
+    alpha = (b @ z) / (z @ (M @ z))
+    for precond in (3, 4):
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo="standard", precond=precond, tolerance=0.0, maxiters=1)
+        assert np.linalg.norm(to_owned(g, xu, xv) - alpha * z) <= 1e-11 * np.linalg.norm(alpha * z)
+
+
+def test_nan_and_nonconvergence_semantics():
+    g, act, A, bu, bv = SPD[BC_PERIODIC]
+    Z = [np.zeros_like(a) for a in A]
+    for algo in ("standard", "chrongear"):
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        st, err, nit = po.pcg_solve(g, act, *Z, bu, bv, xu, xv, algo=algo, precond=0, tolerance=1e-8, maxiters=10)
+        assert st == 1      # alpha = NaN is fatal in the reference
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        st, err, nit = po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo=algo, precond=2, tolerance=1e-30, maxiters=7)
+        assert st == 0 and nit == 7 and err > 0   # non-convergence is silent
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        st, err, nit = po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo=algo, precond=2, tolerance=1e-30, maxiters=3)
+        assert st == 0 and nit == 3 and np.isnan(err)   # no check ran: err unassigned
+
+
+# ----------------------------------------------------------------- parallel layer
+def test_global_sums_exact_and_ordered():
+    rng = np.random.default_rng(5)
+    g, act, A, bu, bv = SPD[BC_NO_ICE]
+    a1 = np.asfortranarray(rng.standard_normal(g.vec_shape) * 10.0 ** rng.integers(-8, 8, g.vec_shape))
+    a2 = np.asfortranarray(rng.standard_normal(g.vec_shape) * 10.0 ** rng.integers(-8, 8, g.vec_shape))
+    own = g.owned_slice()
+    seq = []
+    for j in range(g.jlo - 1, g.jhi):
+        for i in range(g.ilo - 1, g.ihi):
+            for k in range(g.nz):
+                seq.append((a1[k, i, j], a2[k, i, j]))
+    # non-reproducible order: local = (local + a1) + a2, loops j, i, k
+    expect = reduce(lambda s, t: (s + t[0]) + t[1], seq, 0.0)
+    assert po.global_sum(g, a1, a2) == expect
+    # reproducible (DDPDD) sum of the packed a1 + a2 is within 1 ulp of the exact sum
+    exact = sum(Fraction(x + y) for x, y in seq)
+    g.reprosum = 1
+    try:
+        r = po.global_sum(g, a1, a2)
+    finally:
+        g.reprosum = 0
+    assert abs(Fraction(r) - exact) <= abs(np.spacing(float(exact)))
+
+
+def test_halo_rules():
+    for bc in (BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW):
+        g = SPD[bc][0]
+        a = np.asfortranarray(np.arange(np.prod(g.vec_shape), dtype=float).reshape(g.vec_shape, order="F") + 1.0)
+        b = a.copy(order="F")
+        po.halo(g, b)
+        nv, nw = g.nvx, g.nvy
+        if bc == BC_PERIODIC:
+            # one-task periodic: vertex 1,2 <- nv-2,nv-1; vertex nv <- 3 (E/W on owned rows, then N/S)
+            e = a.copy()
+            e[:, 0:2, g.jlo - 1:g.jhi] = a[:, nv - 3:nv - 1, g.jlo - 1:g.jhi]
+            e[:, nv - 1, g.jlo - 1:g.jhi] = a[:, 2, g.jlo - 1:g.jhi]
+            e2 = e.copy()
+            e2[:, :, 0:2] = e[:, :, nw - 3:nw - 1]
+            e2[:, :, nw - 1] = e[:, :, 2]
+            assert np.array_equal(b, e2)
+        elif bc == BC_NO_ICE:
+            e = a.copy()
+            e[:, 0:2, :] = 0; e[:, nv - 2:, :] = 0; e[:, :, 0:2] = 0; e[:, :, nw - 2:] = 0
+            assert np.array_equal(b, e)
+            assert np.all(b[:, g.ihi - 1, :] == 0)   # the last owned column is zeroed too
+        else:
+            e = a.copy()
+            e[:, 0, :] = 0; e[:, nv - 1, :] = 0; e[:, :, 0] = 0; e[:, :, nw - 1] = 0
+            assert np.array_equal(b, e)
+
+
+# ----------------------------------------------------------------- element matrix
+def _brick_integrals(a, b, c):
+    """Closed-form integrals of trilinear basis products on [0,a]x[0,b]x[0,c]
+    from 1-D linear-element integrals."""
+    sx = [-1, 1, 1, -1, -1, 1, 1, -1]
+    sy = [-1, -1, 1, 1, -1, -1, 1, 1]
+    sz = [-1, -1, -1, -1, 1, 1, 1, 1]
+
+    def dd(L, s, t):   # int N_s' N_t'
+        return (1.0 if s == t else -1.0) / L
+
+    def mm(L, s, t):   # int N_s N_t
+        return L * (1.0 / 3.0 if s == t else 1.0 / 6.0)
+
+    def dm(s):         # int N_s' N_t (any t)
+        return 0.5 * s
+
+    I = {}
+    for key in ("xx", "yy", "zz", "xy", "yx"):
+        I[key] = np.zeros((8, 8))
+    for r in range(8):
+        for q in range(8):
+            X = (sx[r], sx[q]); Y = (sy[r], sy[q]); Zs = (sz[r], sz[q])
+            I["xx"][r, q] = dd(a, *X) * mm(b, *Y) * mm(c, *Zs)
+            I["yy"][r, q] = mm(a, *X) * dd(b, *Y) * mm(c, *Zs)
+            I["zz"][r, q] = mm(a, *X) * mm(b, *Y) * dd(c, *Zs)
+            I["xy"][r, q] = dm(sx[r]) * dm(sy[q]) * mm(c, *Zs)   # int phi_r,x phi_q,y
+            I["yx"][r, q] = dm(sy[r]) * dm(sx[q]) * mm(c, *Zs)
+    return I
+
+
+def test_element_matrix_brick_closed_form():
+    """Constant viscosity on an axis-aligned brick: the BP element blocks
+    are eta/vol0 times combinations of closed-form basis integrals."""
+    a, b, c = 1000.0, 700.0, 90.0
+    sx = np.array([-1, 1, 1, -1, -1, 1, 1, -1])
+    sy = np.array([-1, -1, 1, 1, -1, -1, 1, 1])
+    sz = np.array([-1, -1, -1, -1, 1, 1, 1, 1])
+    x = (sx + 1) / 2 * a
+    y = (sy + 1) / 2 * b
+    z = (sz + 1) / 2 * c
+    eta = 3.7e6
+    st, K, em = po.element_matrix(x, y, z, np.zeros(8), np.zeros(8), 0.0, whichefvs=0, efvs_constant=eta)
+    assert st == 0 and em == eta
+    I = _brick_integrals(a, b, c)
+    f = eta / 1e9
+    assert np.allclose(K[0], f * (4 * I["xx"] + I["yy"] + I["zz"]), rtol=1e-12, atol=0)
+    assert np.allclose(K[3], f * (4 * I["yy"] + I["xx"] + I["zz"]), rtol=1e-12, atol=0)
+    assert np.allclose(K[1], f * (2 * I["xy"] + I["yx"]), rtol=1e-12, atol=1e-20)
+    assert np.allclose(K[2], f * (2 * I["yx"] + I["xy"]), rtol=1e-12, atol=1e-20)
+
+
+def test_element_matrix_invariants_distorted():
+    """Symmetry (Kuu, Kvv symmetric; Kuv = Kvu^T), rigid translation in the
+    null space, positive semi-definiteness, and the viscosity of a linear
+    field (trilinear elements reproduce it exactly at every point)."""
+    rng = np.random.default_rng(7)
+    sx = np.array([-1, 1, 1, -1, -1, 1, 1, -1])
+    sy = np.array([-1, -1, 1, 1, -1, -1, 1, 1])
+    sz = np.array([-1, -1, -1, -1, 1, 1, 1, 1])
+    x = (sx + 1) / 2 * 1000 + rng.uniform(-80, 80, 8)
+    y = (sy + 1) / 2 * 1000 + rng.uniform(-80, 80, 8)
+    z = (sz + 1) / 2 * 120 + rng.uniform(-20, 20, 8) + 0.01 * x
+    u = rng.uniform(0, 100, 8)
+    v = rng.uniform(-20, 20, 8)
+    st, K, em = po.element_matrix(x, y, z, u, v, 1.2e5)
+    assert st == 0
+    Kuu, Kuv, Kvu, Kvv = K
+    assert np.array_equal(Kuu, Kuu.T) and np.array_equal(Kvv, Kvv.T)
+    assert np.allclose(Kuv, Kvu.T, rtol=1e-14, atol=0)
+    big = np.block([[Kuu, Kuv], [Kvu, Kvv]])
+    sc = np.abs(big).max()
+    assert np.abs(big @ np.r_[np.ones(8), np.zeros(8)]).max() < 1e-12 * sc
+    assert np.abs(big @ np.r_[np.zeros(8), np.ones(8)]).max() < 1e-12 * sc
+    assert np.linalg.eigvalsh(big).min() > -1e-12 * sc
+    # linear velocity u = s x: eps_xx = s, so efvs = flwafact * (emin^2 + s^2)^(-1/3)
+    s = 3e-3
+    st, K, em = po.element_matrix(x, y, z, s * x, np.zeros(8), 1.2e5)
+    assert np.isclose(em, 1.2e5 * (1e-16 + s * s) ** (-1.0 / 3.0), rtol=1e-12)
+
+
+def test_gravity_load_column_sum():
+    """The local gravity load of a column sums to -rho g dx dy H ds/dx / vol0."""
+    p, s = stream_system()
+    g = p.grid
+    geo = s["geo"]
+    av = s["active_vertex"].astype(bool)
+    tot = s["loadu"].sum(axis=0)
+    expect = -910.0 * 9.81 * p.dx * p.dy * geo["stagthck"] / 1e9 * geo["dusrf_dx"]
+    assert np.allclose(tot[av], expect[av], rtol=1e-12, atol=0)
+
+
+def test_dirichlet_rows_and_columns():
+    """No-slip: basal rows become identity rows, their columns are eliminated
+    symmetrically (glissade_velo_higher.F90:9305-9498)."""
+    p, s = dome_system()
+    g = p.grid
+    av = s["active_vertex"].astype(bool)
+    own = np.zeros_like(av)
+    own[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi] = True
+    sel = av & own
+    k = g.nz - 1
+    for blk, diag in ((0, 1.0), (1, 0.0), (2, 0.0), (3, 1.0)):
+        rows = s[("Auu", "Auv", "Avu", "Avv")[blk]][:, k][:, sel]
+        expect = np.zeros(27)
+        expect[13] = diag
+        # couplings to level nz+1 do not exist; the rest of the row is zero
+        assert np.all(rows[:18] == expect[:18, None])
+    # no row couples to a basal (prescribed) node except the basal rows themselves
+    A = s["Auu"]
+    for m in range(18, 27):
+        assert np.all(A[m, k - 1][sel] == 0.0)
+
+
+# ----------------------------------------------------------------- full BP: analytic slab
+def _slab_error(nz):
+    th = 5.0 * pi / 180
+    gn, rhoi, grav, beta, Hp = 3.0, 910.0, 9.81, 2000.0, 1000.0
+    mu_n_pwrn = 2.0 ** ((3.0 - gn) / 2.0) / (gn + 1.0) * sin(th) ** (gn - 1.0) * (rhoi * grav * Hp) ** (gn - 1.0) * 1e6
+    flwa = 1.0 / (2.0 ** ((1.0 + gn) / 2.0) * mu_n_pwrn)
+    p = synthetic.slab(n_cells=6, nz=nz, dx=50.0, thck=Hp / cos(th), theta_deg=5.0, beta=beta, flwa=flwa)
+    r = po.picard(p, algo="chrongear", precond=2, linear_tol=1e-11, linear_maxiters=3000, resid_target=1e-9,
+                  maxiter=300)
+    assert r["converged"]
+    g = p.grid
+    u = r["uvel"][:, g.ilo + 1, g.jlo + 1]
+    H = p.thck[0, 0] * cos(th)
+    mu_n = 1.0 / (2.0 ** ((1.0 + gn) / (2.0 * gn)) * flwa ** (1.0 / gn))
+    eta = (beta * H / mu_n ** gn) * (rhoi * grav * H) ** (gn - 1)
+    velscale = (rhoi * grav * H / mu_n) ** gn * H
+    zp = (1.0 - p.sigma) * p.thck[0, 0] / H * cos(th)
+    ufo = tan(th) / eta - 2 ** ((1.0 - gn) / 2.0) * tan(th) ** gn / (
+        (gn + 1) * (1.0 + 3.0 * sin(th) ** 2) ** ((gn + 1.0) / 2.0)) * ((1.0 - zp) ** (gn + 1) - 1.0)
+    return float(np.abs(u / velscale - ufo).max() / np.abs(ufo).max()), abs(u[-1] / velscale / ufo[-1] - 1.0)
+
+
+def test_slab_matches_first_order_analytic_solution():
+    """Closed-form first-order (BP) velocity of a sliding, shearing slab
+    (tests/slab/plotSlab.py:242-250, Dukowicz 2012): the restated assembly,
+    basal friction and solver converge to it at second order in the level
+    spacing, and the basal sliding velocity tan(theta)/eta is reproduced."""
+    e1, b1 = _slab_error(11)
+    e2, b2 = _slab_error(21)
+    assert e2 < 1e-3
+    assert 3.5 < e1 / e2 < 4.5
+    assert b1 < 1e-7 and b2 < 1e-7
