@@ -1,0 +1,69 @@
+! Reads a system written by tests/test_fortran_shim.py, calls the drop-in
+! pcg_solver_{standard,chrongear}_3d exactly as glissade_velo_higher.F90
+! does (:3288-3316), and writes the solution, err and niters back.
+program shim_driver
+  use glimmer_global, only: dp
+  use cism_parallel, only: parallel_type
+  use glissade_velo_higher_pcg_gpu, only: pcg_solver_standard_3d, pcg_solver_chrongear_3d
+  implicit none
+  integer :: hdr(12), nx, ny, nz, bc, algo, precond, ncheck, maxiters, niters, i, j, k, m, u
+  real(dp) :: tol, err
+  type(parallel_type) :: parallel
+  integer :: indxA_2d(-1:1,-1:1), indxA_3d(-1:1,-1:1,-1:1)
+  integer, allocatable :: act(:,:)
+  logical, allocatable :: active_vertex(:,:)
+  real(dp), allocatable :: Auu(:,:,:,:), Auv(:,:,:,:), Avu(:,:,:,:), Avv(:,:,:,:)
+  real(dp), allocatable :: bu(:,:,:), bv(:,:,:), xu(:,:,:), xv(:,:,:)
+  character(len=512) :: fin, fout
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) hdr
+  read(u) tol
+  nx = hdr(1); ny = hdr(2); nz = hdr(3)
+  parallel%staggered_ilo = hdr(4); parallel%staggered_ihi = hdr(5)
+  parallel%staggered_jlo = hdr(6); parallel%staggered_jhi = hdr(7)
+  bc = hdr(8); algo = hdr(9); precond = hdr(10); ncheck = hdr(11); maxiters = hdr(12)
+  parallel%periodic_bc = (bc == 0 .or. bc == 2)
+  parallel%outflow_bc = (bc == 1)
+  parallel%no_ice_bc = (bc == 3)
+  allocate(act(nx-1,ny-1), active_vertex(nx-1,ny-1))
+  allocate(Auu(27,nz,nx-1,ny-1), Auv(27,nz,nx-1,ny-1), Avu(27,nz,nx-1,ny-1), Avv(27,nz,nx-1,ny-1))
+  allocate(bu(nz,nx-1,ny-1), bv(nz,nx-1,ny-1), xu(nz,nx-1,ny-1), xv(nz,nx-1,ny-1))
+  read(u) act
+  read(u) Auu, Auv, Avu, Avv, bu, bv, xu, xv
+  close(u)
+  active_vertex = act /= 0
+  ! neighbour numbering of glissade_velo_higher_init (glissade_velo_higher.F90:619-640)
+  m = 0
+  do k = -1, 1
+     do j = -1, 1
+        do i = -1, 1
+           m = m + 1
+           indxA_3d(i,j,k) = m
+        enddo
+     enddo
+  enddo
+  m = 0
+  do j = -1, 1
+     do i = -1, 1
+        m = m + 1
+        indxA_2d(i,j) = m
+     enddo
+  enddo
+  if (algo == 3) then
+     call pcg_solver_chrongear_3d(nx, ny, nz, parallel, indxA_2d, indxA_3d, active_vertex, &
+                                  Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tol, maxiters, &
+                                  err, niters, 1, 1, 0)
+  else
+     call pcg_solver_standard_3d(nx, ny, nz, parallel, indxA_2d, indxA_3d, active_vertex, &
+                                 Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tol, maxiters, &
+                                 err, niters, 1, 1, 0)
+  endif
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) niters
+  write(u) err
+  write(u) xu, xv
+  close(u)
+end program shim_driver
