@@ -32,6 +32,8 @@ EXPORTS = (
     "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve", "cismgpu_system_matvec",
     "cismgpu_system_bench_spmv", "cismgpu_system_halo_x", "cismgpu_system_assemble_3d", "cismgpu_system_get_efvs",
     "cismgpu_system_residual", "cismgpu_system_zero_x",
+    "cismgpu_ensemble_create", "cismgpu_ensemble_destroy", "cismgpu_ensemble_autotune", "cismgpu_ensemble_solve",
+    "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info",
 )
 
 

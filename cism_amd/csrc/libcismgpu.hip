@@ -15,6 +15,7 @@
 #include "kernels.h"
 #include "kernels.hip"
 #include "assembly.hip"
+#include "ensemble.hip"
 
 using namespace cg;
 
@@ -1082,6 +1083,333 @@ extern "C" int32_t cismgpu_system_zero_x(cismgpu_system* S) {
     if (!S) return CISMGPU_ERR_ARG;
     CG_CHECK(hipMemsetAsync(S->V(0), 0, 2 * S->d.vlen * sizeof(double), S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+// ---------------------------------------------------------------- ensemble
+struct cismgpu_ensemble {
+    int nmem = 0;
+    hipStream_t s = nullptr;
+    std::vector<EnsMember> mem;      // host copy
+    std::vector<int> side;           // owned vertices per side (bucketing)
+    EnsMember* d_mem = nullptr;
+    int *d_tile_mem_mat = nullptr, *d_tile_mem_col = nullptr;
+    int ntile_mat = 0, ntile_col = 0;
+    double* A64 = nullptr;
+    float* A32 = nullptr;
+    __hip_bfloat16* A16 = nullptr;
+    float* M32 = nullptr;
+    int8_t* active = nullptr;
+    double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
+    float* f32 = nullptr;            // r, d, q, z, Az, e
+    double *part_mat = nullptr, *part_col = nullptr;
+    EnsState* st = nullptr;
+    unsigned* sync = nullptr;        // bar[2], queue[3], err
+    long long nrows_total = 0, vtotal = 0, ctotal = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    EnsParams gparams{};
+    int ggrid = 0;
+    int gbf16 = -1;
+};
+
+__global__ void k_ens_init(long long nvec, double* __restrict__ x, int nmem, EnsState* __restrict__ st,
+                           unsigned* __restrict__ sync) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long q = t; q < nvec; q += (long long)gridDim.x * blockDim.x) x[q] = 0.0;
+    if (t < 2 * nmem) {
+        EnsState z{};
+        z.phase = M_OUTER;
+        st[t] = z;
+    }
+    if (t < 6) sync[t] = 0u;
+}
+
+static void ens_tiles(cismgpu_ensemble* E) {
+    std::vector<int> tm, tc;
+    int tmat = 0, tcol = 0;
+    for (int m = 0; m < E->nmem; ++m) {
+        EnsMember& M = E->mem[m];
+        const int cpw = WAVE / M.split;
+        M.ntile_mat = (M.ncol + cpw - 1) / cpw;
+        M.ntile_col = (int)(((long long)M.d.nvx * M.d.nvy + ENS_BS - 1) / ENS_BS);
+        M.t0_mat = tmat;
+        M.t0_col = tcol;
+        for (int t = 0; t < M.ntile_mat; ++t) tm.push_back(m);
+        for (int t = 0; t < M.ntile_col; ++t) tc.push_back(m);
+        tmat += M.ntile_mat;
+        tcol += M.ntile_col;
+    }
+    E->ntile_mat = tmat;
+    E->ntile_col = tcol;
+    if (E->d_tile_mem_mat) hipFree(E->d_tile_mem_mat);
+    if (E->d_tile_mem_col) hipFree(E->d_tile_mem_col);
+    if (E->part_mat) hipFree(E->part_mat);
+    if (E->part_col) hipFree(E->part_col);
+    hipMalloc(&E->d_tile_mem_mat, tm.size() * sizeof(int));
+    hipMalloc(&E->d_tile_mem_col, tc.size() * sizeof(int));
+    hipMemcpy(E->d_tile_mem_mat, tm.data(), tm.size() * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(E->d_tile_mem_col, tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice);
+    hipMalloc(&E->part_mat, 2 * (size_t)tmat * sizeof(double));
+    hipMalloc(&E->part_col, 2 * 4 * (size_t)tcol * sizeof(double));
+    hipMemcpy(E->d_mem, E->mem.data(), E->nmem * sizeof(EnsMember), hipMemcpyHostToDevice);
+    if (E->gexec) { hipGraphExecDestroy(E->gexec); E->gexec = nullptr; }
+    if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+}
+
+extern "C" int32_t cismgpu_ensemble_destroy(cismgpu_ensemble* E) {
+    if (!E) return CISMGPU_OK;
+    if (E->s) hipStreamSynchronize(E->s);
+    if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    for (void* p : {(void*)E->d_mem, (void*)E->d_tile_mem_mat, (void*)E->d_tile_mem_col, (void*)E->A64, (void*)E->A32,
+                    (void*)E->A16, (void*)E->M32, (void*)E->active, (void*)E->b64, (void*)E->x64, (void*)E->r64,
+                    (void*)E->f32, (void*)E->part_mat, (void*)E->part_col, (void*)E->st, (void*)E->sync})
+        if (p) hipFree(p);
+    if (E->s) hipStreamDestroy(E->s);
+    delete E;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* members, cismgpu_ensemble** out) {
+    if (!members || n < 1 || !out) return CISMGPU_ERR_ARG;
+    *out = nullptr;
+    cismgpu_ensemble* E = new cismgpu_ensemble();
+    E->nmem = n;
+    auto fail = [&](int code, const char* msg) { cismgpu_set_error("ensemble: %s", msg); cismgpu_ensemble_destroy(E); return code; };
+    if (hipStreamCreateWithFlags(&E->s, hipStreamNonBlocking) != hipSuccess) return fail(CISMGPU_ERR_HIP, "stream");
+    long long rows = 0, voff = 0, coff = 0;
+    for (int m = 0; m < n; ++m) {
+        cismgpu_system* S = members[m];
+        if (!S || !S->have_matrix) return fail(CISMGPU_ERR_ARG, "member without an assembled matrix");
+        const Dims& d = S->d;
+        if (d.nz > 16) return fail(CISMGPU_ERR_ARG, "members need nz <= 16");
+        if (d.bc != 1 && d.bc != 3) return fail(CISMGPU_ERR_ARG, "members need the no_ice or outflow halo rule");
+        // active vertices must avoid every row/column the halo rule writes
+        std::vector<int8_t> act((size_t)d.nvx * d.nvy);
+        if (hipMemcpy(act.data(), S->active, act.size(), hipMemcpyDeviceToHost) != hipSuccess) return fail(CISMGPU_ERR_HIP, "copy");
+        const int lw = d.bc == 3 ? 2 : 1, uw = d.bc == 3 ? 2 : 1;
+        for (int j = 1; j <= d.nvy; ++j)
+            for (int i = 1; i <= d.nvx; ++i)
+                if (act[(size_t)(j - 1) * d.nvx + (i - 1)] && (i <= lw || i >= d.nvx - uw + 1 || j <= lw || j >= d.nvy - uw + 1))
+                    return fail(CISMGPU_ERR_ARG, "member has active vertices on halo-rule rows (needs an ice-free outer ring)");
+        EnsMember M{};
+        M.d = d;
+        M.ncol = d.nxo * d.nyo;
+        M.ncolp = (M.ncol + WAVE - 1) / WAVE * WAVE;
+        M.split = 1;
+        M.a_off = rows;
+        M.v_off = voff;
+        M.c_off = coff;
+        rows += (long long)d.nz * M.ncolp;
+        voff += 2 * d.vlen;
+        coff += (long long)d.nvx * d.nvy;
+        // ||b||
+        int st = reset_scal(S);
+        if (st) return fail(st, "scal");
+        k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, S->V(2), S->V(2), S->V(3), S->V(3), S->partB);
+        scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+        st = sync_scal(S);
+        if (st) return fail(st, "norm");
+        M.bnorm = S->scal_h[S_L2RHS];
+        E->mem.push_back(M);
+        E->side.push_back(std::max(d.nxo, d.nyo));
+    }
+    E->nrows_total = rows;
+    E->vtotal = voff;   // doubles per vector pool (both components)
+    E->ctotal = coff;
+    const size_t na = (size_t)rows * NCOEF;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&E->d_mem, n * sizeof(EnsMember))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc");
+    if ((e = hipMalloc(&E->A64, na * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A64");
+    if ((e = hipMalloc(&E->A32, na * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A32");
+    if ((e = hipMalloc(&E->M32, 3 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc M");
+    if ((e = hipMalloc(&E->active, coff)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc active");
+    if ((e = hipMalloc(&E->b64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc b");
+    if ((e = hipMalloc(&E->x64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc x");
+    if ((e = hipMalloc(&E->r64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc r");
+    if ((e = hipMalloc(&E->f32, 6 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc f32");
+    if ((e = hipMalloc(&E->st, 2 * n * sizeof(EnsState))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc st");
+    if ((e = hipMalloc(&E->sync, 8 * sizeof(unsigned))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc sync");
+    hipMemsetAsync(E->f32, 0, 6 * (size_t)voff * sizeof(float), E->s);
+    hipMemsetAsync(E->r64, 0, voff * sizeof(double), E->s);
+    hipMemsetAsync(E->x64, 0, voff * sizeof(double), E->s);
+    for (int m = 0; m < n; ++m) {
+        cismgpu_system* S = members[m];
+        const EnsMember& M = E->mem[m];
+        const Dims& d = M.d;
+        const long long nel = (long long)d.nz * M.ncolp * NCOEF;
+        hipStreamSynchronize(S->s);
+        k_ens_pack<double><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A64);
+        k_ens_pack<float><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A32);
+        k_cast<double, float><<<nblocks(6 * d.vlen, 256), 256, 0, E->s>>>(6 * d.vlen, S->M64, E->M32 + 3 * M.v_off);
+        hipMemcpyAsync(E->b64 + M.v_off, S->V(2), 2 * d.vlen * sizeof(double), hipMemcpyDeviceToDevice, E->s);
+        hipMemcpyAsync(E->active + M.c_off, S->active, (size_t)d.nvx * d.nvy, hipMemcpyDeviceToDevice, E->s);
+    }
+    if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "build");
+    ens_tiles(E);
+    *out = E;
+    return CISMGPU_OK;
+}
+
+static int ens_bucket(int side) { return side <= 80 ? 0 : (side <= 136 ? 1 : 2); }
+
+extern "C" int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* E, int32_t* split_out, double* ms_out) {
+    if (!E) return CISMGPU_ERR_ARG;
+    const int splits[3] = {1, 2, 4};
+    int best[3] = {1, 1, 1};
+    double tbest[3] = {1e30, 1e30, 1e30};
+    float* z = E->f32 + 3 * E->vtotal;
+    float* Az = E->f32 + 4 * E->vtotal;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (int si = 0; si < 3; ++si) {
+        for (auto& M : E->mem) M.split = splits[si];
+        ens_tiles(E);
+        for (int b = 0; b < 3; ++b) {
+            std::vector<int> tl;
+            for (int m = 0; m < E->nmem; ++m)
+                if (ens_bucket(E->side[m]) == b)
+                    for (int t = 0; t < E->mem[m].ntile_mat; ++t) tl.push_back(E->mem[m].t0_mat + t);
+            double ms = 0.0;
+            if (!tl.empty()) {
+                int* dtl = nullptr;
+                CG_CHECK(hipMalloc(&dtl, tl.size() * sizeof(int)));
+                CG_CHECK(hipMemcpy(dtl, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice));
+                const unsigned g = nblocks((long long)tl.size(), ENS_WPB);
+                k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, z, Az, E->part_mat);
+                hipEventRecord(e0, E->s);
+                for (int r = 0; r < 5; ++r)
+                    k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, z, Az, E->part_mat);
+                hipEventRecord(e1, E->s);
+                hipEventSynchronize(e1);
+                float f = 0.f;
+                hipEventElapsedTime(&f, e0, e1);
+                ms = f / 5.0;
+                hipFree(dtl);
+            }
+            if (ms_out) ms_out[b * 3 + si] = ms;
+            if (ms < tbest[b]) { tbest[b] = ms; best[b] = splits[si]; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (int m = 0; m < E->nmem; ++m) E->mem[m].split = best[ens_bucket(E->side[m])];
+    ens_tiles(E);
+    CG_CHECK(hipMemsetAsync(Az, 0, E->vtotal * sizeof(float), E->s));
+    CG_CHECK(hipStreamSynchronize(E->s));
+    if (split_out) for (int b = 0; b < 3; ++b) split_out[b] = best[b];
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ensemble_opts* o,
+                                          cismgpu_ensemble_info* info) {
+    if (!E || !o || !info) return CISMGPU_ERR_ARG;
+    if (o->ncheck < 1 || o->inner_maxiters < 1 || o->max_outer < 1) return CISMGPU_ERR_ARG;
+    memset(info, 0, sizeof(*info));
+    if (o->bf16 && !E->A16) {
+        const size_t na = (size_t)E->nrows_total * NCOEF;
+        CG_CHECK(hipMalloc(&E->A16, na * sizeof(__hip_bfloat16)));
+        k_convert_matrix<__hip_bfloat16><<<nblocks((long long)na, 256), 256, 0, E->s>>>((long long)na, E->A64, E->A16);
+    }
+    int dev = 0, ncu = 0, per = 0;
+    CG_CHECK(hipGetDevice(&dev));
+    CG_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    CG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ensemble<16>, ENS_BS, 0));
+    int bpc = o->blocks_per_cu > 0 ? std::min(o->blocks_per_cu, per) : per;
+    if (bpc < 1) { cismgpu_set_error("ensemble kernel cannot be resident"); return CISMGPU_ERR_HIP; }
+    const int grid = bpc * ncu;
+    EnsParams P{};
+    P.nmem = E->nmem; P.ntile_mat = E->ntile_mat; P.ntile_col = E->ntile_col;
+    P.inner_maxiters = o->inner_maxiters; P.max_outer = o->max_outer; P.ncheck = o->ncheck;
+    P.tol = o->tolerance; P.inner_tol = o->inner_tol;
+    P.maxsteps = (o->inner_maxiters + 3) * (o->max_outer + 1) + 8;
+    P.spin_limit = 50000000LL;
+    EnsPtrs Q{};
+    Q.mem = E->d_mem; Q.tile_mem_mat = E->d_tile_mem_mat; Q.tile_mem_col = E->d_tile_mem_col;
+    Q.A64 = E->A64; Q.A32 = E->A32; Q.A16 = E->A16; Q.M32 = E->M32; Q.active = E->active;
+    Q.b64 = E->b64; Q.x64 = E->x64; Q.r64 = E->r64;
+    const long long V = E->vtotal;
+    Q.r32 = E->f32; Q.d32 = E->f32 + V; Q.q32 = E->f32 + 2 * V; Q.z32 = E->f32 + 3 * V; Q.Az32 = E->f32 + 4 * V;
+    Q.e32 = E->f32 + 5 * V;
+    Q.part_mat = E->part_mat; Q.part_col = E->part_col; Q.st = E->st;
+    Q.bar = E->sync; Q.queue = E->sync + 2; Q.err_flag = (int*)(E->sync + 5);
+    Q.use_bf16 = o->bf16 ? 1 : 0;
+    // (re)capture the graph: init kernel + persistent solve
+    const bool same = E->gexec && E->ggrid == grid && E->gbf16 == Q.use_bf16 && !memcmp(&E->gparams, &P, sizeof(P));
+    if (!same) {
+        if (E->gexec) { hipGraphExecDestroy(E->gexec); E->gexec = nullptr; }
+        if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+        CG_CHECK(hipStreamBeginCapture(E->s, hipStreamCaptureModeThreadLocal));
+        k_ens_init<<<1024, 256, 0, E->s>>>(V, E->x64, E->nmem, E->st, E->sync);
+        k_ensemble<16><<<grid, ENS_BS, 0, E->s>>>(P, Q);
+        CG_CHECK(hipStreamEndCapture(E->s, &E->graph));
+        CG_CHECK(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+        E->gparams = P;
+        E->ggrid = grid;
+        E->gbf16 = Q.use_bf16;
+    }
+    hipEvent_t e0, e1;
+    CG_CHECK(hipEventCreate(&e0));
+    CG_CHECK(hipEventCreate(&e1));
+    CG_CHECK(hipEventRecord(e0, E->s));
+    CG_CHECK(hipGraphLaunch(E->gexec, E->s));
+    CG_CHECK(hipEventRecord(e1, E->s));
+    CG_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    info->solve_ms = ms;
+    info->grid_blocks = grid;
+    unsigned sy[8];
+    CG_CHECK(hipMemcpy(sy, E->sync, sizeof(sy), hipMemcpyDeviceToHost));
+    info->barrier_error = ((int*)sy)[5] ? 1 : 0;
+    std::vector<EnsState> st(2 * E->nmem);
+    CG_CHECK(hipMemcpy(st.data(), E->st, st.size() * sizeof(EnsState), hipMemcpyDeviceToHost));
+    // the final state is in the buffer of the parity after the last step: take, per member, the
+    // buffer whose phase is DONE (both agree once a member is done)
+    for (int m = 0; m < E->nmem; ++m) {
+        const EnsState& a = st[m].phase == M_DONE ? st[m] : st[E->nmem + m];
+        if (a.flags & 1) info->converged++;
+        info->max_outer = std::max(info->max_outer, a.outer_it);
+        info->max_inner = std::max(info->max_inner, a.total_inner);
+        info->total_inner += a.total_inner;
+        info->max_err = std::max(info->max_err, a.err);
+    }
+    if (info->barrier_error) { cismgpu_set_error("ensemble grid barrier timed out"); return CISMGPU_ERR_HIP; }
+    return info->converged == E->nmem ? CISMGPU_OK : CISMGPU_ERR_NOT_CONVERGED;
+}
+
+extern "C" int32_t cismgpu_ensemble_get_x(cismgpu_ensemble* E, int32_t m, double* xu, double* xv) {
+    if (!E || m < 0 || m >= E->nmem || !xu || !xv) return CISMGPU_ERR_ARG;
+    const EnsMember& M = E->mem[m];
+    const Dims& d = M.d;
+    const long long nvec = (long long)d.nz * d.nvx * d.nvy;
+    double* tmp = nullptr;
+    CG_CHECK(hipMalloc(&tmp, nvec * sizeof(double)));
+    double* outs[2] = {xu, xv};
+    for (int c = 0; c < 2; ++c) {
+        k_vec_out<double, double><<<nblocks(nvec, 256), 256, 0, E->s>>>(d, E->x64 + M.v_off + c * d.vlen, tmp);
+        hipMemcpyAsync(outs[c], tmp, nvec * sizeof(double), hipMemcpyDeviceToHost, E->s);
+        hipStreamSynchronize(E->s);
+    }
+    hipFree(tmp);
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_ensemble_member_info(cismgpu_ensemble* E, int32_t m, double* err, int32_t* outer,
+                                                int32_t* total_inner, int32_t* flags) {
+    if (!E || m < 0 || m >= E->nmem) return CISMGPU_ERR_ARG;
+    EnsState a[2];
+    CG_CHECK(hipMemcpy(&a[0], E->st + m, sizeof(EnsState), hipMemcpyDeviceToHost));
+    CG_CHECK(hipMemcpy(&a[1], E->st + E->nmem + m, sizeof(EnsState), hipMemcpyDeviceToHost));
+    const EnsState& s = a[0].phase == M_DONE ? a[0] : a[1];
+    if (err) *err = s.err;
+    if (outer) *outer = s.outer_it;
+    if (total_inner) *total_inner = s.total_inner;
+    if (flags) *flags = s.flags;
     return CISMGPU_OK;
 }
 

@@ -99,7 +99,8 @@ def ice_stream(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0,
                flwa: float = 1.0e-16, h_mean: float = 1000.0, h_amp: float = 200.0,
                wavelength: float = 20.0e3, bed_slope: float = 0.002, bump_sigma: float = 50.0,
                bump_corr: float = 5.0e3, beta_median: float = 1.0e3, beta_sigma_log: float = 1.0,
-               beta_corr: float = 5.0e3, u_surf: float = 150.0) -> Problem:
+               beta_corr: float = 5.0e3, u_surf: float = 150.0, bed_amp: float = 0.0,
+               bed_wavelength: float = 15.0e3) -> Problem:
     """Synthetic ice stream on an ``n_vert x n_vert`` owned vertex grid.
 
     Thickness ``h_mean + h_amp sin(2 pi x / wavelength)``; bed sloping down in
@@ -122,7 +123,7 @@ def ice_stream(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0,
     ring[:, :3] = ring[:, -3:] = True
     thck[ring] = 0.0
     bumps = bump_sigma * _smooth_field(rng, (nx, ny), bump_corr / dx)
-    topg = 2000.0 - bed_slope * X + bumps
+    topg = 2000.0 - bed_slope * X + bumps + bed_amp * np.sin(2 * np.pi * (X + 0.5 * Y) / bed_wavelength)
     lb = _smooth_field(rng, (nx - 1, ny - 1), beta_corr / dx)
     beta = beta_median * np.exp(beta_sigma_log * lb)
     # velocity guess: basal sliding + internal shear, varying across the stream
@@ -172,3 +173,23 @@ def slab(n_cells: int = 8, nz: int = 21, dx: float = 1000.0, thck: float = 1000.
         meta=dict(workload="slab", theta_deg=theta_deg, thck=thck, beta=beta, flwa=flwa),
     )
     return p
+
+
+def ensemble(n_members: int = 256, seed: int = 2024, nmin: int = 32, nmax: int = 192, nz_choices=(5, 7, 11, 15),
+             dx: float = 1000.0):
+    """Ensemble of independent ice-stream members with uneven sizes: side
+    n ~ U{nmin..nmax} owned vertices, nz from ``nz_choices``, Glen A
+    log-uniform in [1e-17, 1e-15] Pa^-3 yr^-1, log-normal beta and a
+    sinusoid-plus-noise bed, every member seeded from ``seed``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n_members):
+        n = int(rng.integers(nmin, nmax + 1))
+        nz = int(rng.choice(nz_choices))
+        A = float(10.0 ** rng.uniform(-17.0, -15.0))
+        p = ice_stream(n, nz=nz, dx=dx, seed=int(rng.integers(0, 2**31 - 1)), flwa=A,
+                       bed_amp=float(rng.uniform(20.0, 80.0)),
+                       beta_median=float(10.0 ** rng.uniform(2.5, 3.5)))
+        p.meta.update(member=m, glen_A=A)
+        out.append(p)
+    return out

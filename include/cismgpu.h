@@ -203,6 +203,52 @@ int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
 int32_t cismgpu_system_residual(cismgpu_system* sys, double* resid_u, double* resid_v,
                                 double* L2_norm, double* L2_norm_relative);
 
+/* ------------------------------------------------------------------ */
+/* Batched ensemble of independent systems (no reference counterpart:   */
+/* the reference solves one system per call).  Members are assembled    */
+/* systems; the ensemble copies their FP64 matrices, rhs and SIA         */
+/* preconditioners into batched storage and solves all of them with FP64 */
+/* iterative refinement around FP32 (or BF16-matrix) Chronopoulos-Gear   */
+/* in ONE persistent launch, captured in a hipGraph.  Members must use   */
+/* the no_ice or outflow halo rule with an ice-free outer ring.          */
+/* ------------------------------------------------------------------ */
+typedef struct cismgpu_ensemble cismgpu_ensemble;
+
+typedef struct cismgpu_ensemble_opts {
+    double  tolerance;        /* outer relative residual per member */
+    double  inner_tol;        /* inner relative residual per refinement step */
+    int32_t inner_maxiters;
+    int32_t max_outer;
+    int32_t ncheck;           /* inner convergence checked every ncheck iterations */
+    int32_t bf16;             /* 1: BF16 matrix storage in the inner solve */
+    int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu x CUs (0: from occupancy) */
+} cismgpu_ensemble_opts;
+
+typedef struct cismgpu_ensemble_info {
+    double  solve_ms;         /* device time of the graph launch */
+    int32_t steps;            /* lock-step steps executed */
+    int32_t converged;        /* members that reached tolerance */
+    int32_t max_outer;        /* largest number of refinement steps */
+    int32_t max_inner;        /* largest total inner iterations of a member */
+    int64_t total_inner;      /* inner iterations summed over members */
+    double  max_err;          /* largest final outer relative residual */
+    int32_t barrier_error;    /* 1 if a grid barrier timed out (result invalid) */
+    int32_t grid_blocks;      /* persistent workgroups launched */
+} cismgpu_ensemble_info;
+
+int32_t cismgpu_ensemble_create(int32_t nmembers, cismgpu_system* const* members, cismgpu_ensemble** out);
+int32_t cismgpu_ensemble_destroy(cismgpu_ensemble* ens);
+/* time the FP32 SpMV per size bucket for each level split and keep the
+ * fastest; buckets: owned vertices per side <= 80, <= 136, larger.
+ * split_out[3] receives the choice (1, 2 or 4 lanes per column). */
+int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* ens, int32_t* split_out, double* ms_out /* [3][3] */);
+int32_t cismgpu_ensemble_solve(cismgpu_ensemble* ens, const cismgpu_ensemble_opts* opts,
+                               cismgpu_ensemble_info* info);
+/* member solution in reference layout; per-member statistics */
+int32_t cismgpu_ensemble_get_x(cismgpu_ensemble* ens, int32_t member, double* xu, double* xv);
+int32_t cismgpu_ensemble_member_info(cismgpu_ensemble* ens, int32_t member, double* err, int32_t* outer,
+                                     int32_t* total_inner, int32_t* flags);
+
 #ifdef __cplusplus
 }
 #endif
