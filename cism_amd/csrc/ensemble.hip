@@ -3,17 +3,21 @@
 //
 // Each member is an assembled FP64 system (owned rows of A, b, the SIA
 // column-tridiagonal preconditioner).  The solve is FP64 iterative
-// refinement around an FP32 Chronopoulos-Gear inner solve (the same
-// recurrence as pcg_solver_chrongear_3d, glissade_velo_higher_pcg.F90:
-// 3296-3581, with beta = 0 and sigma = delta on the first inner step).
-// All members advance in lock step through a sequence of "steps"; a step is
-// a matrix phase (FP64 residual or FP32 SpMV, per member state) and a column
-// phase (refinement bookkeeping or the fused C-G update + SIA solve), each
-// ended by a grid-wide barrier.  Work units of a phase are taken from an
-// atomic queue so that the members still iterating are spread over every
-// workgroup.  Every per-member decision is a deterministic function of
-// per-tile compensated partial sums reduced in a fixed order, so all
-// workgroups reach the same decision without extra communication.
+// refinement around an FP32 Chronopoulos-Gear inner solve (the recurrence of
+// pcg_solver_chrongear_3d, glissade_velo_higher_pcg.F90:3296-3581, with
+// beta = 0 and sigma = delta on the first inner step).
+//
+// All members advance in lock step through "steps".  A step is a matrix
+// phase (per member: the FP64 residual b - A x, or the FP32 product A z) and
+// a column phase (per member: refinement bookkeeping, or the fused C-G
+// update + SIA column solve + partial sums), each closed by a grid-wide
+// barrier.  Work is split into wavefront tiles -- 64 owned columns (a share
+// of the levels) in the matrix phase, 64 vertex columns in the column phase --
+// dealt round-robin over every wavefront of the grid, so members that are
+// still iterating are spread over all CUs while finished members cost one
+// state read per tile.  Every per-member decision is a deterministic
+// function of per-tile compensated partial sums reduced in a fixed order,
+// so all wavefronts reach the same decision without extra communication.
 //
 // Members must have an ice-free outer ring (no active vertex on a row that
 // the halo rule writes), so halo updates are identities and are omitted;
@@ -22,7 +26,7 @@
 
 namespace cg {
 
-constexpr int ENS_BS = 256;          // threads per workgroup
+constexpr int ENS_BS = 1024;         // threads per workgroup (16 wavefronts, one workgroup per CU)
 constexpr int ENS_WPB = ENS_BS / WAVE;
 
 struct EnsMember {
@@ -31,9 +35,10 @@ struct EnsMember {
     long long v_off;                 // offset (elements) of the member's vectors
     long long c_off;                 // offset of the member's vertex-column arrays (nvx*nvy)
     int ncol, ncolp;                 // owned columns nxo*nyo and padded to 64
-    int split;                       // levels split across `split` lanes (1, 2, 4)
-    int ntile_mat;                   // matrix-phase tiles (waves) of this member
-    int ntile_col;                   // column-phase tiles (workgroups) of this member
+    int ncg;                         // groups of 64 owned columns
+    int split;                       // level groups per column group (1, 2, 4): the tuned parameter
+    int ntile_mat;                   // matrix-phase tiles = ncg * split
+    int ntile_col;                   // column-phase tiles (64 vertex columns each)
     int t0_mat, t0_col;              // first tile indices in the global lists
     double bnorm;                    // ||b|| (FP64, compensated), set at build
 };
@@ -41,14 +46,14 @@ struct EnsMember {
 // per-member state, double buffered by step parity
 enum { M_OUTER = 0, M_INNER = 1, M_FOLD = 2, M_DONE = 3 };
 struct EnsState {
-    int phase;                       // M_*
+    int phase;                       // M_OUTER, M_INNER or M_DONE (M_FOLD is decided at the start of a step)
     int inner_it;                    // inner iterations of the current refinement step
     int outer_it;                    // refinement steps done
     int total_inner;
     double rho, alpha;               // C-G scalars (rho_old, alpha_old)
     double r0;                       // ||r|| at the start of the inner solve
     double err;                      // last outer relative residual
-    int flags;                       // 1 converged, 2 NaN, 4 inner limit
+    int flags;                       // 1 converged, 2 NaN
 };
 
 struct EnsParams {
@@ -72,44 +77,44 @@ struct EnsPtrs {
     double* x64;
     double* r64;
     float *r32, *d32, *q32, *z32, *Az32, *e32;
-    double* part_mat;                // [2][ntile_mat] dd
+    double* part_mat;                // [ntile_mat] dd
     double* part_col;                // [2][ntile_col][2] dd
     EnsState* st;                    // [2][nmem]
     unsigned* bar;                   // barrier: count, generation
-    unsigned* queue;                 // 3 work counters
-    int* err_flag;
+    int* err_flag;                   // set when a barrier wait exceeded its bound
+    int* steps_out;                  // steps taken (written by workgroup 0)
     int use_bf16;
 };
 
 // ---------------------------------------------------------------- barrier
 // Sense-reversing barrier for a grid whose workgroups are all resident (the
-// launch is sized from the occupancy query).  Every wait is bounded: after
-// `limit` polls the workgroup records an error and proceeds, so the grid
-// always drains.
+// launch is sized from the occupancy query).  Release before arriving, acquire
+// after the generation flips.  Every wait is bounded: after `limit` polls the
+// workgroup records an error and proceeds, so the grid always drains.
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks, long long limit, int* err) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived = atomicAdd(&bar[0], 1u) + 1u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         if (arrived == nblocks) {
             __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long long polls = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++polls > limit) { atomicExch(err, 1); break; }
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++polls > limit) { __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
-// deterministic reduction of a member's tile partials (stride = dd per tile)
+// Deterministic reduction of a member's tile partials (stride = doubles per
+// tile) by one wavefront; every lane returns the sum.
 __device__ double member_sum(const double* part, int t0, int n, int stride, int which) {
-    // one wave reduces; result broadcast through LDS by the caller
     dd acc = {0.0, 0.0};
     const int lane = threadIdx.x % WAVE;
     for (int t = lane; t < n; t += WAVE) {
@@ -120,368 +125,299 @@ __device__ double member_sum(const double* part, int t0, int n, int stride, int 
     return __shfl(acc.hi + acc.lo, 0, WAVE);
 }
 
+__device__ __forceinline__ dd wave_reduce(dd acc) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) acc = dd_add(acc, dd_shfl_down(acc, off));
+    return acc;
+}
+
+// Phase a member runs in this step.  An inner solve turns into a fold once its
+// residual reached inner_tol (tested every ncheck iterations) or at the
+// iteration limit; the test reads the (r,r) partials of the previous step, so
+// every wavefront evaluates it identically.
+__device__ __forceinline__ int step_phase(const EnsState& st, const EnsMember& M, const double* pc_cur,
+                                          const EnsParams& P) {
+    if (st.phase != M_INNER || st.inner_it == 0) return st.phase;
+    if (st.inner_it % P.ncheck != 0 && st.inner_it < P.inner_maxiters) return M_INNER;
+    const double rr = member_sum(pc_cur, M.t0_col, M.ntile_col, 4, 1);
+    const double ie = sqrt(rr) / st.r0;
+    return (ie < P.inner_tol || st.inner_it >= P.inner_maxiters || !(ie == ie)) ? M_FOLD : M_INNER;
+}
+
 // ---------------------------------------------------------------- matrix phase work
-// One wave processes one matrix tile: 64/split flat owned columns, each lane
-// a column and a contiguous share of the levels.
-template <typename TA>
-__device__ __forceinline__ void spmv_tile_f32(const EnsMember& M, int tile, const TA* __restrict__ A,
-                                              const float* __restrict__ xu, const float* __restrict__ xv,
-                                              float* __restrict__ yu, float* __restrict__ yv, dd& acc) {
+// One wavefront: 64 consecutive owned columns, the levels of level group lg.
+// Row (k, c) of a member sits at compact row k*ncolp + c; the 27 neighbour
+// values of one plane are fetched together ahead of their FMAs.
+template <typename TA, typename TV, bool RESID>
+__device__ __forceinline__ dd matrix_tile(const EnsMember& M, int lt, const TA* __restrict__ A,
+                                          const TV* __restrict__ xu, const TV* __restrict__ xv,
+                                          const double* __restrict__ bu, const double* __restrict__ bv,
+                                          TV* __restrict__ yu, TV* __restrict__ yv) {
     const Dims& d = M.d;
     const int lane = threadIdx.x % WAVE;
-    const int cpw = WAVE / M.split;                      // columns per wave
-    const int c = tile * cpw + lane % cpw;               // flat owned column
-    const int part = lane / cpw;                         // level group
-    if (c >= M.ncol) return;
+    const int cgi = lt % M.ncg, lg = lt / M.ncg;
+    const int c = cgi * WAVE + lane;
+    dd acc = {0.0, 0.0};
+    if (c >= M.ncol) return acc;
     const int io = c % d.nxo, jo = c / d.nxo;
     const int i = d.ilo + io, j = d.jlo + jo;
     const int kper = (d.nz + M.split - 1) / M.split;
-    const int k0 = part * kper, k1 = min(d.nz, k0 + kper);
+    const int k0 = lg * kper, k1 = min(d.nz, k0 + kper);
     const long long pl = d.plane;
     for (int k = k0; k < k1; ++k) {
         const long long r = (long long)k * M.ncolp + c + M.a_off;
         const TA* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
         const long long c0 = vidx(d, k, i, j);
-        float su = 0.f, sv = 0.f;
+        TV su = 0, sv = 0;
 #pragma unroll
-        for (int m = 0; m < 27; ++m) {
-            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-            const long long n = c0 + kA * pl + jA * d.px + iA;
-            const float xun = xu[n], xvn = xv[n];
-            su = su + ld_coef(a + (0 * 27 + m) * WAVE) * xun + ld_coef(a + (1 * 27 + m) * WAVE) * xvn;
-            sv = sv + ld_coef(a + (2 * 27 + m) * WAVE) * xun + ld_coef(a + (3 * 27 + m) * WAVE) * xvn;
-        }
-        yu[c0] = su;
-        yv[c0] = sv;
-        acc = dd_add_d(acc, (double)su * (double)xu[c0]);
-        acc = dd_add_d(acc, (double)sv * (double)xv[c0]);
-    }
-}
-
-__device__ __forceinline__ void resid_tile_f64(const EnsMember& M, int tile, const double* __restrict__ A,
-                                               const double* __restrict__ xu, const double* __restrict__ xv,
-                                               const double* __restrict__ bu, const double* __restrict__ bv,
-                                               double* __restrict__ ru, double* __restrict__ rv, dd& acc) {
-    const Dims& d = M.d;
-    const int lane = threadIdx.x % WAVE;
-    const int cpw = WAVE / M.split;
-    const int c = tile * cpw + lane % cpw;
-    const int part = lane / cpw;
-    if (c >= M.ncol) return;
-    const int io = c % d.nxo, jo = c / d.nxo;
-    const int i = d.ilo + io, j = d.jlo + jo;
-    const int kper = (d.nz + M.split - 1) / M.split;
-    const int k0 = part * kper, k1 = min(d.nz, k0 + kper);
-    const long long pl = d.plane;
-    for (int k = k0; k < k1; ++k) {
-        const long long r = (long long)k * M.ncolp + c + M.a_off;
-        const double* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
-        const long long c0 = vidx(d, k, i, j);
-        double su = 0.0, sv = 0.0;
+        for (int kA = 0; kA < 3; ++kA) {
+            TV xa[9], xb[9], cuu[9], cuv[9], cvu[9], cvv[9];
 #pragma unroll
-        for (int m = 0; m < 27; ++m) {
-            const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-            const long long n = c0 + kA * pl + jA * d.px + iA;
-            const double xun = xu[n], xvn = xv[n];
-            su = su + a[(0 * 27 + m) * WAVE] * xun + a[(1 * 27 + m) * WAVE] * xvn;
-            sv = sv + a[(2 * 27 + m) * WAVE] * xun + a[(3 * 27 + m) * WAVE] * xvn;
+            for (int q = 0; q < 9; ++q) {
+                const long long n = c0 + (kA - 1) * pl + (q / 3 - 1) * d.px + (q % 3 - 1);
+                xa[q] = xu[n];
+                xb[q] = xv[n];
+                const int m = kA * 9 + q;
+                cuu[q] = ld_coef(a + (0 * 27 + m) * WAVE);
+                cuv[q] = ld_coef(a + (1 * 27 + m) * WAVE);
+                cvu[q] = ld_coef(a + (2 * 27 + m) * WAVE);
+                cvv[q] = ld_coef(a + (3 * 27 + m) * WAVE);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                su = su + cuu[q] * xa[q] + cuv[q] * xb[q];
+                sv = sv + cvu[q] * xa[q] + cvv[q] * xb[q];
+            }
         }
-        const double tu = bu[c0] - su, tv = bv[c0] - sv;
-        ru[c0] = tu;
-        rv[c0] = tv;
-        acc = dd_add_d(acc, tu * tu);
-        acc = dd_add_d(acc, tv * tv);
+        if (RESID) {
+            const TV tu = (TV)bu[c0] - su, tv = (TV)bv[c0] - sv;
+            yu[c0] = tu;
+            yv[c0] = tv;
+            acc = dd_add_d(acc, (double)tu * (double)tu);
+            acc = dd_add_d(acc, (double)tv * (double)tv);
+        } else {
+            yu[c0] = su;
+            yv[c0] = sv;
+            acc = dd_add_d(acc, (double)su * (double)xu[c0]);
+            acc = dd_add_d(acc, (double)sv * (double)xv[c0]);
+        }
     }
+    return acc;
 }
 
 // ---------------------------------------------------------------- column phase work
-// Thomas solve of the SIA column (easy_sia_solver) in FP32.
-template <int NZMAX>
-__device__ __forceinline__ void sia_col(const Dims& d, const float* __restrict__ Ms, const float* __restrict__ Md,
-                                        const float* __restrict__ Mp, long long c0, const float (&rhs)[NZMAX],
-                                        float (&z)[NZMAX]) {
+// One vertex column, one velocity component: the C-G vector update (START:
+// load r from the FP64 residual and clear e, d, q) followed by the SIA
+// Thomas solve z = M^-1 r (easy_sia_solver), streamed level by level; the
+// forward sweep leaves the partial solution in z, the back substitution
+// finishes it.  Accumulates (r,z) and (r,r) over owned columns.
+template <int NZMAX, bool START>
+__device__ __forceinline__ void column_component(const Dims& d, const EnsPtrs& Q, long long vb,
+                                                 const float* __restrict__ Mm, long long c0, bool owned, bool act,
+                                                 float fa, float fb, dd& a_rz, dd& a_rr) {
+    const long long pl = d.plane;
+    const float *Ms = Mm, *Md = Mm + d.vlen, *Mp = Mm + 2 * d.vlen;
     float gam[NZMAX];
-    float beta = Md[c0];
-    z[0] = rhs[0] / beta;
+    float bet = 1.f, zprev = 0.f;
 #pragma unroll
-    for (int k = 1; k < NZMAX; ++k)
+    for (int k = 0; k < NZMAX; ++k)
         if (k < d.nz) {
-            const long long c = c0 + (long long)k * d.plane;
-            gam[k] = Mp[c - d.plane] / beta;
-            const float sb = Ms[c];
-            beta = Md[c] - sb * gam[k];
-            z[k] = (rhs[k] - sb * z[k - 1]) / beta;
+            const long long cl = c0 + (long long)k * pl;
+            const long long c = vb + cl;
+            float rk;
+            if (START) {
+                rk = owned ? (float)Q.r64[c] : 0.f;
+                Q.e32[c] = 0.f;
+                Q.d32[c] = 0.f;
+                Q.q32[c] = 0.f;
+            } else {
+                const float dk = Q.z32[c] + fb * Q.d32[c];
+                const float qk = Q.Az32[c] + fb * Q.q32[c];
+                Q.d32[c] = dk;
+                Q.q32[c] = qk;
+                Q.e32[c] = Q.e32[c] + fa * dk;
+                rk = Q.r32[c] - fa * qk;
+            }
+            Q.r32[c] = rk;
+            if (act) {
+                float zt;
+                if (k == 0) {
+                    bet = Md[cl];
+                    zt = rk / bet;
+                } else {
+                    gam[k] = Mp[cl - pl] / bet;
+                    const float sb = Ms[cl];
+                    bet = Md[cl] - sb * gam[k];
+                    zt = (rk - sb * zprev) / bet;
+                }
+                Q.z32[c] = zt;
+                zprev = zt;
+            } else if (START) {
+                Q.z32[c] = 0.f;
+            }
+            if (owned) a_rr = dd_add_d(a_rr, (double)rk * (double)rk);
         }
+    if (act) {
+        float znext = zprev;
+        if (owned) a_rz = dd_add_d(a_rz, (double)Q.r32[vb + c0 + (long long)(d.nz - 1) * pl] * (double)znext);
 #pragma unroll
-    for (int k = NZMAX - 2; k >= 0; --k)
-        if (k < d.nz - 1) z[k] = z[k] - gam[k + 1] * z[k + 1];
-}
-
-__device__ __forceinline__ void col_of_tile(const EnsMember& M, int tile, int& i, int& j, long long& t2) {
-    const int t = tile * ENS_BS + threadIdx.x;
-    t2 = t;
-    i = 1 + t % M.d.nvx;
-    j = 1 + t / M.d.nvx;
+        for (int k = NZMAX - 2; k >= 0; --k)
+            if (k < d.nz - 1) {
+                const long long c = vb + c0 + (long long)k * pl;
+                const float zk = Q.z32[c] - gam[k + 1] * znext;
+                Q.z32[c] = zk;
+                znext = zk;
+                if (owned) a_rz = dd_add_d(a_rz, (double)Q.r32[c] * (double)zk);
+            }
+    }
 }
 
 template <int NZMAX>
 __global__ __launch_bounds__(ENS_BS) void k_ensemble(EnsParams P, EnsPtrs Q) {
-    __shared__ int s_unit;
-    __shared__ double s_val[4];
     const unsigned G = gridDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int gw = blockIdx.x * ENS_WPB + wave;          // global wavefront id
+    const int W = G * ENS_WPB;
+    const int lane = threadIdx.x % WAVE;
+    __shared__ int s_any;
     int step = 0;
     for (; step < P.maxsteps; ++step) {
         const int cur = step & 1, nxt = cur ^ 1;
-        EnsState* S0 = Q.st + (size_t)cur * P.nmem;
+        const EnsState* S0 = Q.st + (size_t)cur * P.nmem;
         EnsState* S1 = Q.st + (size_t)nxt * P.nmem;
-        // all members done?  (every workgroup reads the same state)
-        {
-            __shared__ int s_any;
-            if (threadIdx.x == 0) s_any = 0;
-            __syncthreads();
-            for (int m = threadIdx.x; m < P.nmem; m += ENS_BS)
-                if (S0[m].phase != M_DONE) s_any = 1;
-            __syncthreads();
-            if (!s_any) break;
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) Q.queue[(step * 2 + 2) % 3] = 0u;   // counter of two phases ahead
+        const double* pc_cur = Q.part_col + (size_t)cur * P.ntile_col * 4;
+        double* pc_nxt = Q.part_col + (size_t)nxt * P.ntile_col * 4;
+        // all members done?  (every workgroup reads the same states)
+        if (threadIdx.x == 0) s_any = 0;
+        __syncthreads();
+        for (int m = threadIdx.x; m < P.nmem; m += ENS_BS)
+            if (S0[m].phase != M_DONE) s_any = 1;
+        __syncthreads();
+        if (!s_any) break;
         // ------------------------------------------------ matrix phase
-        {
-            unsigned* qc = Q.queue + (step * 2) % 3;
-            const int nunits = (P.ntile_mat + ENS_WPB - 1) / ENS_WPB;
-            while (true) {
-                if (threadIdx.x == 0) s_unit = (int)atomicAdd(qc, 1u);
-                __syncthreads();
-                const int u = s_unit;
-                __syncthreads();
-                if (u >= nunits) break;
-                const int tile = u * ENS_WPB + threadIdx.x / WAVE;
-                if (tile < P.ntile_mat) {
-                    const int m = Q.tile_mem_mat[tile];
-                    const EnsMember& M = Q.mem[m];
-                    const int ph = S0[m].phase;
-                    dd acc = {0.0, 0.0};
-                    const int lt = tile - M.t0_mat;
-                    bool wrote = false;
-                    if (ph == M_OUTER) {
-                        const long long vo = M.v_off, vl = M.d.vlen;
-                        resid_tile_f64(M, lt, Q.A64, Q.x64 + vo, Q.x64 + vo + vl, Q.b64 + vo, Q.b64 + vo + vl,
-                                       Q.r64 + vo, Q.r64 + vo + vl, acc);
-                        wrote = true;
-                    } else if (ph == M_INNER) {
-                        const long long vo = M.v_off, vl = M.d.vlen;
-                        if (Q.use_bf16)
-                            spmv_tile_f32<__hip_bfloat16>(M, lt, Q.A16, Q.z32 + vo, Q.z32 + vo + vl, Q.Az32 + vo,
-                                                          Q.Az32 + vo + vl, acc);
-                        else
-                            spmv_tile_f32<float>(M, lt, Q.A32, Q.z32 + vo, Q.z32 + vo + vl, Q.Az32 + vo,
-                                                 Q.Az32 + vo + vl, acc);
-                        wrote = true;
-                    }
-                    if (wrote) {
-                        for (int off = WAVE / 2; off > 0; off >>= 1) acc = dd_add(acc, dd_shfl_down(acc, off));
-                        if (threadIdx.x % WAVE == 0) {
-                            double* p = Q.part_mat + (size_t)tile * 2;
-                            p[0] = acc.hi;
-                            p[1] = acc.lo;
-                        }
-                    }
-                }
+        for (int tile = gw; tile < P.ntile_mat; tile += W) {
+            const int m = Q.tile_mem_mat[tile];
+            const EnsMember& M = Q.mem[m];
+            const EnsState st = S0[m];
+            if (st.phase == M_DONE) continue;
+            const int ph = step_phase(st, M, pc_cur, P);
+            if (ph != M_OUTER && ph != M_INNER) continue;
+            const long long vo = M.v_off, vl = M.d.vlen;
+            const int lt = tile - M.t0_mat;
+            dd acc;
+            if (ph == M_OUTER)
+                acc = matrix_tile<double, double, true>(M, lt, Q.A64, Q.x64 + vo, Q.x64 + vo + vl, Q.b64 + vo,
+                                                        Q.b64 + vo + vl, Q.r64 + vo, Q.r64 + vo + vl);
+            else if (Q.use_bf16)
+                acc = matrix_tile<__hip_bfloat16, float, false>(M, lt, Q.A16, Q.z32 + vo, Q.z32 + vo + vl, nullptr,
+                                                                nullptr, Q.Az32 + vo, Q.Az32 + vo + vl);
+            else
+                acc = matrix_tile<float, float, false>(M, lt, Q.A32, Q.z32 + vo, Q.z32 + vo + vl, nullptr, nullptr,
+                                                       Q.Az32 + vo, Q.Az32 + vo + vl);
+            acc = wave_reduce(acc);
+            if (lane == 0) {
+                Q.part_mat[2 * (size_t)tile] = acc.hi;
+                Q.part_mat[2 * (size_t)tile + 1] = acc.lo;
             }
         }
         grid_barrier(Q.bar, G, P.spin_limit, Q.err_flag);
         // ------------------------------------------------ column phase
-        {
-            unsigned* qc = Q.queue + (step * 2 + 1) % 3;
-            double* pc_old = Q.part_col + (size_t)cur * P.ntile_col * 4;
-            double* pc_new = Q.part_col + (size_t)nxt * P.ntile_col * 4;
-            while (true) {
-                if (threadIdx.x == 0) s_unit = (int)atomicAdd(qc, 1u);
-                __syncthreads();
-                const int tile = s_unit;
-                __syncthreads();
-                if (tile >= P.ntile_col) break;
-                const int m = Q.tile_mem_col[tile];
-                const EnsMember& M = Q.mem[m];
-                const Dims& d = M.d;
-                const EnsState st = S0[m];
-                const int lt = tile - M.t0_col;
-                const bool lead = (lt == 0);
-                // member-wide scalars from the previous phases (fixed-order reductions, wave 0)
-                if (threadIdx.x < WAVE) {
-                    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-                    if (st.phase == M_OUTER) {
-                        v0 = member_sum(Q.part_mat, M.t0_mat, M.ntile_mat, 2, 0);    // (r,r) of b - Ax
-                    } else if (st.phase == M_INNER) {
-                        v0 = member_sum(pc_old, M.t0_col, M.ntile_col, 4, 0);        // (r,z)
-                        v1 = member_sum(Q.part_mat, M.t0_mat, M.ntile_mat, 2, 0);    // (Az,z)
-                        v2 = member_sum(pc_old, M.t0_col, M.ntile_col, 4, 1);        // (r,r)
-                    }
-                    if (threadIdx.x == 0) { s_val[0] = v0; s_val[1] = v1; s_val[2] = v2; }
-                }
-                __syncthreads();
-                int i, j;
-                long long t2;
-                col_of_tile(M, lt, i, j, t2);
-                const bool incol = t2 < (long long)d.nvx * d.nvy;
-                const bool owned = incol && i >= d.ilo && i <= d.ihi && j >= d.jlo && j <= d.jhi;
-                const long long vo = M.v_off, vl = d.vlen;
-                const long long c0 = incol ? vidx(d, 0, i, j) : 0;
-                const bool act = incol && Q.active[M.c_off + t2] != 0;
-                dd a_rz = {0.0, 0.0}, a_rr = {0.0, 0.0};
-                EnsState ns = st;
-                if (st.phase == M_OUTER) {
-                    const double err = sqrt(s_val[0]) / M.bnorm;
-                    ns.err = err;
-                    if (!(err == err)) { ns.phase = M_DONE; ns.flags |= 2; }
-                    else if (err < P.tol) { ns.phase = M_DONE; ns.flags |= 1; }
-                    else if (st.outer_it >= P.max_outer) { ns.phase = M_DONE; }
-                    else {
-                        // start an inner solve: r32 = r64, e = 0, z = M^-1 r, d = q = 0
-                        ns.phase = M_INNER;
-                        ns.inner_it = 0;
-                        ns.outer_it = st.outer_it + 1;
-                        if (incol) {
-                            float rU[NZMAX], rV[NZMAX], zU[NZMAX], zV[NZMAX];
-#pragma unroll
-                            for (int k = 0; k < NZMAX; ++k)
-                                if (k < d.nz) {
-                                    const long long c = c0 + (long long)k * d.plane;
-                                    rU[k] = (float)(owned ? Q.r64[vo + c] : 0.0);
-                                    rV[k] = (float)(owned ? Q.r64[vo + vl + c] : 0.0);
-                                }
-                            if (act) {
-                                const float* Mm = Q.M32 + 3 * vo;
-                                sia_col<NZMAX>(d, Mm, Mm + vl, Mm + 2 * vl, c0, rU, zU);
-                                sia_col<NZMAX>(d, Mm + 3 * vl, Mm + 4 * vl, Mm + 5 * vl, c0, rV, zV);
-                            } else {
-#pragma unroll
-                                for (int k = 0; k < NZMAX; ++k) { zU[k] = 0.f; zV[k] = 0.f; }
-                            }
-#pragma unroll
-                            for (int k = 0; k < NZMAX; ++k)
-                                if (k < d.nz) {
-                                    const long long c = c0 + (long long)k * d.plane;
-                                    Q.r32[vo + c] = rU[k]; Q.r32[vo + vl + c] = rV[k];
-                                    Q.z32[vo + c] = zU[k]; Q.z32[vo + vl + c] = zV[k];
-                                    Q.e32[vo + c] = 0.f; Q.e32[vo + vl + c] = 0.f;
-                                    Q.d32[vo + c] = 0.f; Q.d32[vo + vl + c] = 0.f;
-                                    Q.q32[vo + c] = 0.f; Q.q32[vo + vl + c] = 0.f;
-                                    if (owned) {
-                                        a_rz = dd_add_d(a_rz, (double)rU[k] * (double)zU[k]);
-                                        a_rz = dd_add_d(a_rz, (double)rV[k] * (double)zV[k]);
-                                        a_rr = dd_add_d(a_rr, (double)rU[k] * (double)rU[k]);
-                                        a_rr = dd_add_d(a_rr, (double)rV[k] * (double)rV[k]);
-                                    }
-                                }
-                        }
-                    }
-                } else if (st.phase == M_INNER) {
-                    const double rho = s_val[0], delta = s_val[1];
-                    double beta = 0.0, sigma = delta;
-                    if (st.inner_it > 0) {
-                        beta = rho / st.rho;
-                        sigma = delta - beta * rho / st.alpha;
-                    }
-                    const double alpha = rho / sigma;
-                    if (st.inner_it == 0) ns.r0 = sqrt(s_val[2]);
-                    ns.rho = rho;
-                    ns.alpha = alpha;
-                    ns.inner_it = st.inner_it + 1;
-                    ns.total_inner = st.total_inner + 1;
-                    if (!(alpha == alpha)) { ns.phase = M_DONE; ns.flags |= 2; }
-                    if (incol && ns.phase == M_INNER) {
-                        const float fa = (float)alpha, fb = (float)beta;
-                        float rU[NZMAX], rV[NZMAX], zU[NZMAX], zV[NZMAX];
-#pragma unroll
-                        for (int k = 0; k < NZMAX; ++k)
-                            if (k < d.nz) {
-                                const long long c = vo + c0 + (long long)k * d.plane;
-                                const float dU = Q.z32[c] + fb * Q.d32[c], dV = Q.z32[c + vl] + fb * Q.d32[c + vl];
-                                const float qU = Q.Az32[c] + fb * Q.q32[c], qV = Q.Az32[c + vl] + fb * Q.q32[c + vl];
-                                Q.d32[c] = dU; Q.d32[c + vl] = dV; Q.q32[c] = qU; Q.q32[c + vl] = qV;
-                                Q.e32[c] = Q.e32[c] + fa * dU; Q.e32[c + vl] = Q.e32[c + vl] + fa * dV;
-                                rU[k] = Q.r32[c] - fa * qU; rV[k] = Q.r32[c + vl] - fa * qV;
-                                Q.r32[c] = rU[k]; Q.r32[c + vl] = rV[k];
-                            }
-                        if (act) {
-                            const float* Mm = Q.M32 + 3 * vo;
-                            sia_col<NZMAX>(d, Mm, Mm + vl, Mm + 2 * vl, c0, rU, zU);
-                            sia_col<NZMAX>(d, Mm + 3 * vl, Mm + 4 * vl, Mm + 5 * vl, c0, rV, zV);
-#pragma unroll
-                            for (int k = 0; k < NZMAX; ++k)
-                                if (k < d.nz) {
-                                    const long long c = vo + c0 + (long long)k * d.plane;
-                                    Q.z32[c] = zU[k]; Q.z32[c + vl] = zV[k];
-                                }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < NZMAX; ++k) { zU[k] = 0.f; zV[k] = 0.f; }
-                        }
-                        if (owned)
-#pragma unroll
-                            for (int k = 0; k < NZMAX; ++k)
-                                if (k < d.nz) {
-                                    a_rz = dd_add_d(a_rz, (double)rU[k] * (double)zU[k]);
-                                    a_rz = dd_add_d(a_rz, (double)rV[k] * (double)zV[k]);
-                                    a_rr = dd_add_d(a_rr, (double)rU[k] * (double)rU[k]);
-                                    a_rr = dd_add_d(a_rr, (double)rV[k] * (double)rV[k]);
-                                }
-                    }
-                    // inner convergence is decided on the next step from the new (r,r)
-                    ns.phase = (ns.phase == M_DONE) ? M_DONE : M_INNER;
-                } else if (st.phase == M_FOLD) {
-                    // x64 += e; then the next outer residual
-                    if (incol)
-                        for (int k = 0; k < d.nz; ++k) {
-                            const long long c = vo + c0 + (long long)k * d.plane;
-                            Q.x64[c] = Q.x64[c] + (double)Q.e32[c];
-                            Q.x64[c + vl] = Q.x64[c + vl] + (double)Q.e32[c + vl];
-                        }
-                    ns.phase = M_OUTER;
-                }
-                // column partials of this tile
-                if (st.phase == M_OUTER || st.phase == M_INNER) {
-                    dd v[2] = {a_rz, a_rr};
-                    block_reduce_dd<2, ENS_BS>(v);
-                    if (threadIdx.x == 0) {
-                        double* p = pc_new + (size_t)tile * 4;
-                        p[0] = v[0].hi; p[1] = v[0].lo; p[2] = v[1].hi; p[3] = v[1].lo;
-                    }
-                }
-                if (lead && threadIdx.x == 0) S1[m] = ns;
-                __syncthreads();
-            }
-        }
-        grid_barrier(Q.bar, G, P.spin_limit, Q.err_flag);
-        // ------------------------------------------------ inner convergence (same decision in every workgroup)
-        // The lead workgroup of each member turns INNER into FOLD when the new
-        // residual reached inner_tol (checked every ncheck steps) or the
-        // inner iteration limit.  Decided here, before the next matrix phase.
-        for (int m = blockIdx.x; m < P.nmem; m += G) {
-            EnsState* s = Q.st + (size_t)nxt * P.nmem + m;
-            const int ph = s->phase;
-            __syncthreads();                 // every wave has read the phase before thread 0 may change it
-            if (ph != M_INNER) continue;     // uniform across the workgroup
+        for (int tile = gw; tile < P.ntile_col; tile += W) {
+            const int m = Q.tile_mem_col[tile];
             const EnsMember& M = Q.mem[m];
-            double rr = 0.0;
-            if (threadIdx.x < WAVE) {
-                double* pc_new = Q.part_col + (size_t)nxt * P.ntile_col * 4;
-                rr = member_sum(pc_new, M.t0_col, M.ntile_col, 4, 1);
+            const Dims& d = M.d;
+            const EnsState st = S0[m];
+            const bool lead = (tile == M.t0_col);
+            if (st.phase == M_DONE) {
+                if (lead && lane == 0) S1[m] = st;
+                continue;
             }
-            if (threadIdx.x == 0) {
-                const bool chk = s->inner_it > 0 && ((s->inner_it % P.ncheck == 0) || s->inner_it >= P.inner_maxiters);
-                const double ie = sqrt(rr) / s->r0;
-                if (chk && (ie < P.inner_tol || s->inner_it >= P.inner_maxiters || !(ie == ie))) s->phase = M_FOLD;
+            const int ph = step_phase(st, M, pc_cur, P);
+            const int lt = tile - M.t0_col;
+            const long long t = (long long)lt * WAVE + lane;
+            const bool incol = t < (long long)d.nvx * d.nvy;
+            const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+            const bool owned = incol && i >= d.ilo && i <= d.ihi && j >= d.jlo && j <= d.jhi;
+            const long long vo = M.v_off, vl = d.vlen;
+            const long long c0 = incol ? vidx(d, 0, i, j) : 0;
+            const bool act = incol && Q.active[M.c_off + t] != 0;
+            const float* Mm = Q.M32 + 3 * vo;
+            dd a_rz = {0.0, 0.0}, a_rr = {0.0, 0.0};
+            EnsState ns = st;
+            bool partials = false;
+            if (ph == M_OUTER) {
+                const double err = sqrt(member_sum(Q.part_mat, M.t0_mat, M.ntile_mat, 2, 0)) / M.bnorm;
+                ns.err = err;
+                if (!(err == err)) { ns.phase = M_DONE; ns.flags |= 2; }
+                else if (err < P.tol) { ns.phase = M_DONE; ns.flags |= 1; }
+                else if (st.outer_it >= P.max_outer) { ns.phase = M_DONE; }
+                else {
+                    // start an inner solve: r32 = r64, e = d = q = 0, z = M^-1 r
+                    ns.phase = M_INNER;
+                    ns.inner_it = 0;
+                    ns.outer_it = st.outer_it + 1;
+                    partials = true;
+                    if (incol) {
+                        column_component<NZMAX, true>(d, Q, vo, Mm, c0, owned, act, 0.f, 0.f, a_rz, a_rr);
+                        column_component<NZMAX, true>(d, Q, vo + vl, Mm + 3 * vl, c0, owned, act, 0.f, 0.f, a_rz,
+                                                      a_rr);
+                    }
+                }
+            } else if (ph == M_INNER) {
+                const double rho = member_sum(pc_cur, M.t0_col, M.ntile_col, 4, 0);
+                const double delta = member_sum(Q.part_mat, M.t0_mat, M.ntile_mat, 2, 0);
+                double beta = 0.0, sigma = delta;
+                if (st.inner_it > 0) {
+                    beta = rho / st.rho;
+                    sigma = delta - beta * rho / st.alpha;
+                } else {
+                    ns.r0 = sqrt(member_sum(pc_cur, M.t0_col, M.ntile_col, 4, 1));
+                }
+                const double alpha = rho / sigma;
+                ns.rho = rho;
+                ns.alpha = alpha;
+                ns.inner_it = st.inner_it + 1;
+                ns.total_inner = st.total_inner + 1;
+                partials = true;
+                if (!(alpha == alpha)) {
+                    ns.phase = M_DONE;
+                    ns.flags |= 2;
+                } else if (incol) {
+                    const float fa = (float)alpha, fb = (float)beta;
+                    column_component<NZMAX, false>(d, Q, vo, Mm, c0, owned, act, fa, fb, a_rz, a_rr);
+                    column_component<NZMAX, false>(d, Q, vo + vl, Mm + 3 * vl, c0, owned, act, fa, fb, a_rz, a_rr);
+                }
+            } else {  // M_FOLD: x64 += e, then the next outer residual
+                if (incol)
+                    for (int k = 0; k < d.nz; ++k) {
+                        const long long c = vo + c0 + (long long)k * d.plane;
+                        Q.x64[c] = Q.x64[c] + (double)Q.e32[c];
+                        Q.x64[c + vl] = Q.x64[c + vl] + (double)Q.e32[c + vl];
+                    }
+                ns.phase = M_OUTER;
             }
+            if (partials) {
+                a_rz = wave_reduce(a_rz);
+                a_rr = wave_reduce(a_rr);
+                if (lane == 0) {
+                    double* p = pc_nxt + (size_t)tile * 4;
+                    p[0] = a_rz.hi; p[1] = a_rz.lo; p[2] = a_rr.hi; p[3] = a_rr.lo;
+                }
+            }
+            if (lead && lane == 0) S1[m] = ns;
         }
         grid_barrier(Q.bar, G, P.spin_limit, Q.err_flag);
-        if (*Q.err_flag) break;
+        if (__hip_atomic_load(Q.err_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *Q.steps_out = step;
 }
 
 // ---------------------------------------------------------------- build helpers
 // member row-blocked matrix (its own Dims) -> compact ensemble rows
+template <typename TA>
+__device__ __forceinline__ TA ens_cvt(double v) { return (TA)v; }
+template <>
+__device__ __forceinline__ __hip_bfloat16 ens_cvt<__hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+
 template <typename TA>
 __global__ void k_ens_pack(Dims d, long long a_off, int ncolp, const double* __restrict__ src, TA* __restrict__ dst) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;   // over compact rows * 108
@@ -497,43 +433,24 @@ __global__ void k_ens_pack(Dims d, long long a_off, int ncolp, const double* __r
         v = src[((r / WAVE) * NCOEF + cf) * WAVE + (r % WAVE)];
     }
     const long long R = rr + a_off;
-    dst[((R / WAVE) * NCOEF + cf) * WAVE + (R % WAVE)] = (TA)v;
+    dst[((R / WAVE) * NCOEF + cf) * WAVE + (R % WAVE)] = ens_cvt<TA>(v);
 }
 
-template <>
-__global__ void k_ens_pack<__hip_bfloat16>(Dims d, long long a_off, int ncolp, const double* __restrict__ src,
-                                           __hip_bfloat16* __restrict__ dst) {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    const long long nrow = (long long)d.nz * ncolp;
-    if (t >= nrow * NCOEF) return;
-    const int cf = (int)(t % NCOEF);
-    const long long rr = t / NCOEF;
-    const int k = (int)(rr / ncolp), c = (int)(rr % ncolp);
-    double v = 0.0;
-    if (c < d.nxo * d.nyo) {
-        const int io = c % d.nxo, jo = c / d.nxo;
-        const long long r = ((long long)k * d.nyo + jo) * d.nxp + io;
-        v = src[((r / WAVE) * NCOEF + cf) * WAVE + (r % WAVE)];
-    }
-    const long long R = rr + a_off;
-    dst[((R / WAVE) * NCOEF + cf) * WAVE + (R % WAVE)] = __float2bfloat16((float)v);
-}
-
-// Stand-alone FP32 SpMV sweep over the members of a bucket (autotuner).
+// Stand-alone FP32 products over a list of matrix tiles (autotuner).
 __global__ __launch_bounds__(ENS_BS) void k_ens_spmv_sweep(const EnsMember* __restrict__ mem,
                                                            const int* __restrict__ tiles, int ntiles,
                                                            const int* __restrict__ tile_mem, const float* __restrict__ A,
                                                            const float* __restrict__ z, float* __restrict__ Az,
                                                            double* __restrict__ part) {
-    const int u = blockIdx.x * ENS_WPB + threadIdx.x / WAVE;
+    const int u = blockIdx.x * ENS_WPB + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     if (u >= ntiles) return;
     const int tile = tiles[u];
     const EnsMember& M = mem[tile_mem[tile]];
-    dd acc = {0.0, 0.0};
     const long long vo = M.v_off, vl = M.d.vlen;
-    spmv_tile_f32<float>(M, tile - M.t0_mat, A, z + vo, z + vo + vl, Az + vo, Az + vo + vl, acc);
-    for (int off = WAVE / 2; off > 0; off >>= 1) acc = dd_add(acc, dd_shfl_down(acc, off));
-    if (threadIdx.x % WAVE == 0) { part[2 * tile] = acc.hi; part[2 * tile + 1] = acc.lo; }
+    dd acc = matrix_tile<float, float, false>(M, tile - M.t0_mat, A, z + vo, z + vo + vl, nullptr, nullptr, Az + vo,
+                                              Az + vo + vl);
+    acc = wave_reduce(acc);
+    if (threadIdx.x % WAVE == 0) { part[2 * (size_t)tile] = acc.hi; part[2 * (size_t)tile + 1] = acc.lo; }
 }
 
 }  // namespace cg

@@ -1104,7 +1104,7 @@ struct cismgpu_ensemble {
     float* f32 = nullptr;            // r, d, q, z, Az, e
     double *part_mat = nullptr, *part_col = nullptr;
     EnsState* st = nullptr;
-    unsigned* sync = nullptr;        // bar[2], queue[3], err
+    unsigned* sync = nullptr;        // [0..1] barrier, [5] barrier error, [6] steps taken
     long long nrows_total = 0, vtotal = 0, ctotal = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1122,7 +1122,7 @@ __global__ void k_ens_init(long long nvec, double* __restrict__ x, int nmem, Ens
         z.phase = M_OUTER;
         st[t] = z;
     }
-    if (t < 6) sync[t] = 0u;
+    if (t < 8) sync[t] = 0u;
 }
 
 static void ens_tiles(cismgpu_ensemble* E) {
@@ -1130,9 +1130,9 @@ static void ens_tiles(cismgpu_ensemble* E) {
     int tmat = 0, tcol = 0;
     for (int m = 0; m < E->nmem; ++m) {
         EnsMember& M = E->mem[m];
-        const int cpw = WAVE / M.split;
-        M.ntile_mat = (M.ncol + cpw - 1) / cpw;
-        M.ntile_col = (int)(((long long)M.d.nvx * M.d.nvy + ENS_BS - 1) / ENS_BS);
+        M.ncg = (M.ncol + WAVE - 1) / WAVE;
+        M.ntile_mat = M.ncg * M.split;
+        M.ntile_col = (int)(((long long)M.d.nvx * M.d.nvy + WAVE - 1) / WAVE);
         M.t0_mat = tmat;
         M.t0_col = tcol;
         for (int t = 0; t < M.ntile_mat; ++t) tm.push_back(m);
@@ -1334,7 +1334,7 @@ extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ens
     Q.r32 = E->f32; Q.d32 = E->f32 + V; Q.q32 = E->f32 + 2 * V; Q.z32 = E->f32 + 3 * V; Q.Az32 = E->f32 + 4 * V;
     Q.e32 = E->f32 + 5 * V;
     Q.part_mat = E->part_mat; Q.part_col = E->part_col; Q.st = E->st;
-    Q.bar = E->sync; Q.queue = E->sync + 2; Q.err_flag = (int*)(E->sync + 5);
+    Q.bar = E->sync; Q.err_flag = (int*)(E->sync + 5); Q.steps_out = (int*)(E->sync + 6);
     Q.use_bf16 = o->bf16 ? 1 : 0;
     // (re)capture the graph: init kernel + persistent solve
     const bool same = E->gexec && E->ggrid == grid && E->gbf16 == Q.use_bf16 && !memcmp(&E->gparams, &P, sizeof(P));
@@ -1366,6 +1366,7 @@ extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ens
     unsigned sy[8];
     CG_CHECK(hipMemcpy(sy, E->sync, sizeof(sy), hipMemcpyDeviceToHost));
     info->barrier_error = ((int*)sy)[5] ? 1 : 0;
+    info->steps = ((int*)sy)[6];
     std::vector<EnsState> st(2 * E->nmem);
     CG_CHECK(hipMemcpy(st.data(), E->st, st.size() * sizeof(EnsState), hipMemcpyDeviceToHost));
     // the final state is in the buffer of the parity after the last step: take, per member, the

@@ -240,7 +240,8 @@ int32_t cismgpu_ensemble_create(int32_t nmembers, cismgpu_system* const* members
 int32_t cismgpu_ensemble_destroy(cismgpu_ensemble* ens);
 /* time the FP32 SpMV per size bucket for each level split and keep the
  * fastest; buckets: owned vertices per side <= 80, <= 136, larger.
- * split_out[3] receives the choice (1, 2 or 4 lanes per column). */
+ * split_out[3] receives the choice (1, 2 or 4 level groups per tile of
+ * 64 owned columns). */
 int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* ens, int32_t* split_out, double* ms_out /* [3][3] */);
 int32_t cismgpu_ensemble_solve(cismgpu_ensemble* ens, const cismgpu_ensemble_opts* opts,
                                cismgpu_ensemble_info* info);
