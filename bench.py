@@ -141,6 +141,10 @@ def main():
         spmv_iso[name] = dict(avg_ms=round(ms, 4), algorithmic_bytes=int(by), gbps=round(by / (ms * 1e-3) / 1e9, 1),
                               frac_of_peak=round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3))
     log(f"isolated SpMV: {spmv_iso}")
+    # measured HBM ceilings on this box (STREAM-style copy / triad / read, 2 GiB arrays)
+    from cism_amd import _lib
+    hbm = {k: round(v, 1) for k, v in _lib.bench_stream(2 << 30, 10).items()}
+    log(f"measured HBM GB/s: {hbm}")
 
     # ---- CPU baseline: the C restatement of pcg_solver_chrongear_3d on the same
     # system, a bounded sample of iterations, extrapolated to the FP64 iteration count
@@ -202,6 +206,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": round(spmv_avg_ms, 4),
                      "launches_timed": sp_n},
         "spmv_isolated": spmv_iso,
+        "hbm_measured_gbps": hbm,
         "assembly": {"element_kernel_ms": round(ainfo["assemble_ms"], 2), "total_ms": round(ainfo["total_ms"], 2)},
         "cpu_baseline": cpu,
         "wall_s": round(time.time() - t0, 1),

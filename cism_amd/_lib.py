@@ -33,7 +33,7 @@ EXPORTS = (
     "cismgpu_system_bench_spmv", "cismgpu_system_halo_x", "cismgpu_system_assemble_3d", "cismgpu_system_get_efvs",
     "cismgpu_system_residual", "cismgpu_system_zero_x",
     "cismgpu_ensemble_create", "cismgpu_ensemble_destroy", "cismgpu_ensemble_autotune", "cismgpu_ensemble_solve",
-    "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info",
+    "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info", "cismgpu_bench_stream",
 )
 
 
@@ -90,10 +90,21 @@ def _declare(L):
     L.cismgpu_system_matvec.argtypes = [vp, _D, _D]
     L.cismgpu_system_bench_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _D, _D]
     L.cismgpu_system_zero_x.argtypes = [vp]
+    L.cismgpu_bench_stream.argtypes = [ctypes.c_int64, ctypes.c_int32, _D, _D, _D]
+    L.cismgpu_bench_stream.restype = ctypes.c_int32
     for n in ("cismgpu_system_zero_x", "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix",
               "cismgpu_system_get_matrix", "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve",
               "cismgpu_system_matvec", "cismgpu_system_bench_spmv"):
         getattr(L, n).restype = ctypes.c_int32
+
+
+def bench_stream(nbytes: int = 2 << 30, nrep: int = 10):
+    """Measured HBM bandwidth (GB/s) of STREAM-style copy, triad and read kernels."""
+    L = load()
+    c, t, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(L.cismgpu_bench_stream(int(nbytes), int(nrep), ctypes.byref(c), ctypes.byref(t), ctypes.byref(r)),
+          "bench_stream")
+    return dict(copy=c.value, triad=t.value, read=r.value)
 
 
 def last_error() -> str:

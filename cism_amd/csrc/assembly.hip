@@ -249,6 +249,132 @@ __global__ void k_load_gravity(Dims d, AsmDev in, double* __restrict__ bu, doubl
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lateral ocean-pressure BC at calving fronts (load_vector_lateral_bc /
+// lateral_shelf_bc, glissade_velo_higher.F90:4453-4957), gathered per node.
+// The reference visits active cells (j-major, loop range nhalo+1..n-nhalo+1)
+// and their ocean-facing faces (west, east, south, north), and for each
+// face layer k and 2x2 Gauss point p adds the term of every face node.  A
+// node (k, i, j) is touched by at most one u-face and one v-face per
+// neighbouring cell; visiting those cells in (j, i) order, then layer k-1
+// (node below) before layer k (node above), then p, reproduces the
+// reference's order of additions into each load entry.
+// ---------------------------------------------------------------------------
+struct LatDev {
+    const int32_t* ocean;            // (nx, ny)
+    const double *usrf, *thck;       // (nx, ny)
+    const double *susrf, *sthck;     // (nvx, nvy): staggered over ice-covered or land cells
+    int local;                       // which_ho_assemble_lateral: 1 cell values, 0 staggered
+};
+
+// Adds the terms of face `face` (0 W, 1 E, 2 S, 3 N) of cell (ic, jc) that
+// load node (k, face node hn in {0, 1}) to acc.
+__device__ void lateral_face_node(const Dims& d, const AsmDev& in, const LatDev& L, int face, int ic, int jc,
+                                  int hn, int k, double& acc) {
+#pragma clang fp contract(off)
+    const double rhoi = 910.0, rhoo = 1028.0, grav = 9.81, vol0 = 1.0e9;
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r};
+    int iN0, jN0, iN1, jN1;
+    if (face == 0) { iN0 = ic - 1; jN0 = jc; iN1 = ic - 1; jN1 = jc - 1; }
+    else if (face == 1) { iN0 = ic; jN0 = jc - 1; iN1 = ic; jN1 = jc; }
+    else if (face == 2) { iN0 = ic - 1; jN0 = jc - 1; iN1 = ic; jN1 = jc - 1; }
+    else { iN0 = ic; jN0 = jc; iN1 = ic - 1; jN1 = jc; }
+    const double* coord = face <= 1 ? in.yV : in.xV;
+    const double x0 = coord[v2(d, iN0, jN0)], x1 = coord[v2(d, iN1, jN1)];
+    const double x[4] = {x0, x1, x1, x0};
+    double s[4], h[4];
+    if (L.local) {
+        const double sc = L.usrf[c2(d, ic, jc)], hc = L.thck[c2(d, ic, jc)];
+        for (int n = 0; n < 4; ++n) { s[n] = sc; h[n] = hc; }
+    } else {
+        s[0] = L.susrf[v2(d, iN0, jN0)]; s[1] = L.susrf[v2(d, iN1, jN1)]; s[2] = s[1]; s[3] = s[0];
+        h[0] = L.sthck[v2(d, iN0, jN0)]; h[1] = L.sthck[v2(d, iN1, jN1)]; h[2] = h[1]; h[3] = h[0];
+    }
+    const double sgn = (face == 0 || face == 2) ? -1.0 : 1.0;
+    // layer k-1 (the node is a lower node n = hn), then layer k (upper node n = 3 - hn)
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ke = k - 1 + pass;                  // element layer, 1-based
+        if (ke < 1 || ke > d.nz - 1) continue;
+        const int n = pass == 0 ? hn : 3 - hn;
+        const double y[4] = {s[0] - in.sigma[ke] * h[0], s[1] - in.sigma[ke] * h[1],
+                             s[2] - in.sigma[ke - 1] * h[2], s[3] - in.sigma[ke - 1] * h[3]};
+        for (int p = 0; p < 4; ++p) {
+            double phi[4], dxr[4], dyr[4];
+            phi[0] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+            phi[1] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+            phi[2] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+            phi[3] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+            dxr[0] = -(1.0 - yq[p]) / 4.0;
+            dxr[1] = (1.0 - yq[p]) / 4.0;
+            dxr[2] = (1.0 + yq[p]) / 4.0;
+            dxr[3] = -(1.0 + yq[p]) / 4.0;
+            dyr[0] = -(1.0 - xq[p]) / 4.0;
+            dyr[1] = -(1.0 + xq[p]) / 4.0;
+            dyr[2] = (1.0 + xq[p]) / 4.0;
+            dyr[3] = (1.0 - xq[p]) / 4.0;
+            double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                J11 = J11 + dxr[q] * x[q];
+                J12 = J12 + dxr[q] * y[q];
+                J21 = J21 + dyr[q] * x[q];
+                J22 = J22 + dyr[q] * y[q];
+            }
+            const double detJ = fabs(J11 * J22 - J12 * J21);
+            double hq, sq;
+            if (L.local) {
+                hq = h[0];
+                sq = s[0];
+            } else {
+                hq = 0.0;
+                sq = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    hq = hq + phi[q] * h[q];
+                    sq = sq + phi[q] * s[q];
+                }
+            }
+            const double f = 1.0 - fmin(sq / hq, 1.0);
+            const double p_av = 0.5 * rhoi * grav * hq - 0.5 * rhoo * grav * hq * (f * f);
+            const double t = p_av * 1.0 * detJ / vol0 * phi[n];
+            acc = sgn < 0.0 ? acc - t : acc + t;
+        }
+    }
+}
+
+// One thread per vertex column; b already holds the gravity load.
+__global__ void k_lateral_bc(Dims d, AsmDev in, LatDev L, double* __restrict__ bu, double* __restrict__ bv) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const int lo = in.nhalo + 1, hix = d.nx - in.nhalo + 1, hiy = d.ny - in.nhalo + 1;
+    auto face_on = [&](int ic, int jc, int face) -> bool {
+        if (ic < lo || ic > hix || jc < lo || jc > hiy) return false;
+        if (!in.acell[c2(d, ic, jc)]) return false;
+        const int oi = ic + (face == 0 ? -1 : face == 1 ? 1 : 0), oj = jc + (face == 2 ? -1 : face == 3 ? 1 : 0);
+        return L.ocean[c2(d, oi, oj)] == 1;
+    };
+    // faces touching vertex (i, j), in the reference's cell order; (cell, face, face node)
+    const int ucell[4][4] = {{i, j, 1, 1}, {i + 1, j, 0, 0}, {i, j + 1, 1, 0}, {i + 1, j + 1, 0, 1}};
+    const int vcell[4][4] = {{i, j, 3, 0}, {i + 1, j, 3, 1}, {i, j + 1, 2, 1}, {i + 1, j + 1, 2, 0}};
+    bool uon[4], von[4], any = false;
+    for (int q = 0; q < 4; ++q) {
+        uon[q] = face_on(ucell[q][0], ucell[q][1], ucell[q][2]);
+        von[q] = face_on(vcell[q][0], vcell[q][1], vcell[q][2]);
+        any = any || uon[q] || von[q];
+    }
+    if (!any) return;
+    for (int k = 1; k <= d.nz; ++k) {
+        const long long c = vidx(d, k - 1, i, j);
+        double au = bu[c], av = bv[c];
+        for (int q = 0; q < 4; ++q)
+            if (uon[q]) lateral_face_node(d, in, L, ucell[q][2], ucell[q][0], ucell[q][1], ucell[q][3], k, au);
+        for (int q = 0; q < 4; ++q)
+            if (von[q]) lateral_face_node(d, in, L, vcell[q][2], vcell[q][0], vcell[q][1], vcell[q][3], k, av);
+        bu[c] = au;
+        bv[c] = av;
+    }
+}
+
 // basal_sliding_bc_3d, local option: diagonal of the basal level
 // (:8921-8953), owned rows.
 __global__ void k_basal_local(Dims d, AsmDev in, double* __restrict__ A) {

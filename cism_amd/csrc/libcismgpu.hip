@@ -876,6 +876,87 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     return CISMGPU_OK;
 }
 
+// ---------------------------------------------------------------- HBM probes (STREAM-style)
+// Grid-stride kernels over double2 elements, sized to a few waves per SIMD:
+// copy b = a, triad a = b + s c, read (sum of a into per-workgroup partials).
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                     long long n2) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n2; t += (long long)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+__global__ __launch_bounds__(256) void k_stream_triad(double2* __restrict__ a, const double2* __restrict__ b,
+                                                      const double2* __restrict__ c, double s, long long n2) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n2; t += (long long)gridDim.x * blockDim.x) {
+        const double2 x = b[t], y = c[t];
+        a[t] = make_double2(x.x + s * y.x, x.y + s * y.y);
+    }
+}
+__global__ __launch_bounds__(256) void k_stream_read(const double2* __restrict__ a, long long n2,
+                                                     double* __restrict__ part) {
+    double acc = 0.0;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n2; t += (long long)gridDim.x * blockDim.x) {
+        const double2 x = a[t];
+        acc += x.x + x.y;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, WAVE);
+    if (threadIdx.x % WAVE == 0) part[blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE] = acc;
+}
+
+extern "C" int32_t cismgpu_bench_stream(int64_t nbytes, int32_t nrep, double* copy_gbps, double* triad_gbps,
+                                        double* read_gbps) {
+    if (nbytes < (1 << 20) || nrep < 1) return CISMGPU_ERR_ARG;
+    const long long n2 = nbytes / 16;
+    const size_t bytes = (size_t)n2 * 16;
+    double2 *a = nullptr, *b = nullptr, *c = nullptr;
+    double* part = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int dev = 0, ncu = 0;
+    int rc = CISMGPU_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == CISMGPU_OK) {
+            cismgpu_set_error("bench_stream: %s -> %s", what, hipGetErrorString(e));
+            rc = CISMGPU_ERR_HIP;
+        }
+        return rc == CISMGPU_OK;
+    };
+    if (chk(hipGetDevice(&dev), "device") &&
+        chk(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "attr") &&
+        chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream") && chk(hipMalloc(&a, bytes), "alloc") &&
+        chk(hipMalloc(&b, bytes), "alloc") && chk(hipMalloc(&c, bytes), "alloc") &&
+        chk(hipEventCreate(&e0), "event") && chk(hipEventCreate(&e1), "event")) {
+        const unsigned grid = (unsigned)ncu * 8;
+        if (chk(hipMalloc(&part, (size_t)grid * 4 * sizeof(double)), "alloc") &&
+            chk(hipMemsetAsync(a, 0, bytes, s), "memset") && chk(hipMemsetAsync(b, 0, bytes, s), "memset") &&
+            chk(hipMemsetAsync(c, 0, bytes, s), "memset")) {
+            double* outs[3] = {copy_gbps, triad_gbps, read_gbps};
+            const double moved[3] = {2.0 * bytes, 3.0 * bytes, 1.0 * bytes};
+            for (int kind = 0; kind < 3 && rc == CISMGPU_OK; ++kind) {
+                auto launch = [&]() {
+                    if (kind == 0) k_stream_copy<<<grid, 256, 0, s>>>(a, b, n2);
+                    else if (kind == 1) k_stream_triad<<<grid, 256, 0, s>>>(a, b, c, 3.0, n2);
+                    else k_stream_read<<<grid, 256, 0, s>>>(a, n2, part);
+                };
+                launch();
+                chk(hipEventRecord(e0, s), "record");
+                for (int r = 0; r < nrep; ++r) launch();
+                chk(hipEventRecord(e1, s), "record");
+                if (!chk(hipEventSynchronize(e1), "sync")) break;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (outs[kind]) *outs[kind] = moved[kind] / ((double)ms / nrep * 1e-3) / 1e9;
+            }
+        }
+    }
+    if (s) hipStreamSynchronize(s);
+    for (void* p : {(void*)a, (void*)b, (void*)c, (void*)part})
+        if (p) hipFree(p);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    return rc;
+}
+
 // ---------------------------------------------------------------- assembly
 static bool g_tables_ready = false;
 
@@ -915,7 +996,14 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
     const long long nvec = (long long)d.nz * n2, nfl = (long long)(d.nz - 1) * nc;
     // device input block: doubles first, then int32s
-    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl), ni = (size_t)(nc + n2 + 2 * nvec);
+    const bool lateral = in->ocean_mask != nullptr;
+    if (lateral && (!in->usrf || !in->thck ||
+                    (in->which_assemble_lateral == 0 && (!in->stagusrf_lateral || !in->stagthck_lateral)))) {
+        cismgpu_set_error("assemble_3d: lateral BC needs usrf, thck and (standard option) the staggered fields");
+        return CISMGPU_ERR_ARG;
+    }
+    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? 2 * nc + 2 * n2 : 0)),
+                 ni = (size_t)(nc + n2 + 2 * nvec + (lateral ? nc : 0));
     const size_t bytes = nd * sizeof(double) + ni * sizeof(int32_t);
     if (S->asm_bytes < bytes) {
         if (S->asm_buf) hipFree(S->asm_buf);
@@ -946,6 +1034,14 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     a.beta = putd(in->beta_internal, n2, true);
     a.theta = putd(in->theta_basal, n2, true);
     a.flwafact = putd(in->flwafact, nfl, true);
+    LatDev lat{};
+    if (lateral) {
+        lat.usrf = putd(in->usrf, nc, true);
+        lat.thck = putd(in->thck, nc, true);
+        lat.susrf = putd(in->stagusrf_lateral, n2, true);
+        lat.sthck = putd(in->stagthck_lateral, n2, true);
+        lat.local = in->which_assemble_lateral == 1 ? 1 : 0;
+    }
     int32_t* q = ib;
     auto puti = [&](const int32_t* h, long long n) -> const int32_t* {
         int32_t* dst = q;
@@ -958,6 +1054,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     a.avert = puti(in->active_vertex, n2);
     a.umask = puti(in->umask_dirichlet, nvec);
     a.vmask = puti(in->vmask_dirichlet, nvec);
+    if (lateral) lat.ocean = puti(in->ocean_mask, nc);
     a.efvs = S->efvs;
     a.dx = in->dx; a.dy = in->dy; a.efvs_constant = in->efvs_constant; a.effstrain_min = in->effstrain_min;
     a.n_glen = in->n_glen; a.whichefvs = in->whichefvs; a.nhalo = 2;
@@ -979,6 +1076,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
     if (in->beta_internal) k_basal_local<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
     k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->V(2), S->V(3));
+    if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d, a, lat, S->V(2), S->V(3));
     k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
     halo<double>(S, S->V(2), S->V(3), d.nz);

@@ -150,10 +150,36 @@ def staggered_halo_2d(grid: Grid, a: np.ndarray) -> None:
         a[:, nw - 2:] = 0.0
 
 
-def beta_internal(problem, floating_mask=None):
+def grounded_fraction(ice_mask, floating_mask, land_mask):
+    """f_ground at vertices for the default options (no grounding-line
+    parameterisation, grounded-neighbour rule; glissade_grounding_line.F90:59-674,
+    glide_types.F90:1090, 1099): 1 where a neighbouring cell is land or grounded
+    ice, 0 elsewhere.  The vertex mask tests the land mask of cell (i+1, j+1)
+    twice and never that of (i+1, j), as the reference does."""
+    ice = ice_mask == 1
+    land = land_mask == 1
+    cg = (ice & (floating_mask == 0)) | land
+    vmask = (ice[:-1, 1:] | ice[1:, 1:] | ice[:-1, :-1] | ice[1:, :-1] |
+             land[:-1, 1:] | land[1:, 1:] | land[:-1, :-1])
+    anyg = cg[:-1, 1:] | cg[1:, 1:] | cg[:-1, :-1] | cg[1:, :-1]
+    return np.asfortranarray(np.where(vmask & anyg, 1.0, 0.0))
+
+
+def stagger_masked(var, mask):
+    """glissade_stagger with stagger_margin = 1: mean over the cells whose mask
+    is 1, zero where none is (glissade_grid_operators.F90:135-146)."""
+    m = mask.astype(np.int64)
+    sumvar = (m[:-1, 1:] * var[:-1, 1:] + m[1:, 1:] * var[1:, 1:]) + m[:-1, :-1] * var[:-1, :-1] + m[1:, :-1] * var[1:, :-1]
+    summask = (m[:-1, 1:] + m[1:, 1:] + m[:-1, :-1] + m[1:, :-1]).astype(np.float64)
+    out = np.zeros_like(sumvar, dtype=np.float64)
+    np.divide(sumvar, summask, out=out, where=summask > 0)
+    return np.asfortranarray(out)
+
+
+def beta_internal(problem, ice_mask, floating_mask, land_mask):
     """glissade_calcbeta for constant / large / external beta (glissade_basal_traction.F90:
-    202, 304, 399), weighted by the grounded fraction f_ground (:650-709): 1 at vertices
-    whose four cells are grounded ice or land, 0 where any neighbour floats."""
+    202, 304, 399) times f_ground (:650-672, beta_grounded_min = 0), then its
+    staggered halo."""
     g = problem.grid
     if problem.whichbabc == 0:
         b = np.full((g.nvx, g.nvy), problem.beta_const)
@@ -161,10 +187,6 @@ def beta_internal(problem, floating_mask=None):
         b = np.full((g.nvx, g.nvy), 1.0e10)
     else:
         b = np.array(problem.beta, dtype=np.float64)
-    if floating_mask is not None and floating_mask.any():
-        fl = floating_mask.astype(bool)
-        anyfl = fl[:-1, :-1] | fl[1:, :-1] | fl[:-1, 1:] | fl[1:, 1:]
-        allfl = fl[:-1, :-1] & fl[1:, :-1] & fl[:-1, 1:] & fl[1:, 1:]
-        fground = np.where(allfl, 0.0, np.where(anyfl, 0.5, 1.0))
-        b = b * fground
-    return np.asfortranarray(b)
+    b = np.asfortranarray(b * grounded_fraction(ice_mask, floating_mask, land_mask))
+    staggered_halo_2d(g, b)                    # (:697)
+    return b

@@ -15,6 +15,8 @@ import numpy as np
 
 from .grid import BC_NO_ICE, BC_PERIODIC, Grid, glide_sigma
 
+RHOI_SYN, RHOO_SYN = 910.0, 1028.0   # ice and ocean densities (glimmer_physcon.F90:67-68)
+
 # which_ho_babc values (glide_types.F90:253-268)
 HO_BABC_BETA_CONSTANT = 0
 HO_BABC_BETA_LARGE = 4
@@ -45,6 +47,7 @@ class Problem:
     effstrain_min: float = 1.0e-8   # glide_types.F90:2567
     whichefvs: int = 2
     efvs_constant: float = 0.0
+    assemble_lateral: int = 0    # which_ho_assemble_lateral (glide_types.F90:1067): 0 standard, 1 local
     meta: dict = field(default_factory=dict)
 
     def cell_centres(self):
@@ -193,3 +196,73 @@ def ensemble(n_members: int = 256, seed: int = 2024, nmin: int = 32, nmax: int =
         p.meta.update(member=m, glen_A=A)
         out.append(p)
     return out
+
+
+def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, flwa: float = 1.0e-16,
+                 h_max: float = 3000.0, h_shelf: float = 300.0, bed_front: float = -500.0,
+                 bed_interior: float = -900.0, bump_sigma: float = 30.0, bump_corr: float = 8.0e3,
+                 beta_median: float = 1.0e3, beta_sigma_log: float = 0.5, beta_corr: float = 8.0e3,
+                 gl_frac: float = 0.55, front_frac: float = 0.85, gl_amp_frac: float = 0.05,
+                 u_front: float = 800.0) -> Problem:
+    """Marine ice sheet on an ``n_vert x n_vert`` owned vertex grid.
+
+    Ice flows east: a grounded interior (thickness up to ``h_max``) on a
+    retrograde bed (``bed_interior`` in the west rising to ``bed_front`` in
+    the east, plus seeded bumps) thins past a seeded, wavy grounding line
+    (mean position ``gl_frac`` of the width) into a floating shelf that ends
+    at a calving front (``front_frac``) facing open ocean.  A land strip
+    borders the west, north and south edges, so every halo value is zero and
+    ice meets ocean only at the calving front.  beta is log-normal; under
+    floating ice it is removed by f_ground.
+    """
+    rng = np.random.default_rng(seed)
+    g = Grid.from_owned(n_vert, n_vert, nz, global_bc=BC_NO_ICE)
+    nx, ny = g.nx, g.ny
+    xc = (np.arange(1, nx + 1) - 2.5) * dx
+    yc = (np.arange(1, ny + 1) - 2.5) * dx
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    L = (n_vert - 1) * dx
+    bed = bed_interior + (bed_front - bed_interior) * np.clip(X / L, 0.0, 1.0)
+    bed = bed + bump_sigma * _smooth_field(rng, (nx, ny), bump_corr / dx)
+    # grounding line position varies along y with a seeded phase
+    phase = rng.uniform(0, 2 * np.pi)
+    x_gl = L * (gl_frac + gl_amp_frac * np.sin(2 * np.pi * Y / max(L, dx) * 1.5 + phase))
+    x_front = front_frac * L
+    rr = RHOO_SYN / RHOI_SYN
+    h_float = np.maximum(-bed * rr, 0.0)                 # flotation thickness
+    s = np.clip(X / x_gl, 0.0, 1.0)
+    h_sheet = h_float * 1.15 + (h_max - h_float * 1.15) * np.sqrt(np.clip(1.0 - s ** 2, 0.0, 1.0))
+    t = np.clip((X - x_gl) / np.maximum(x_front - x_gl, dx), 0.0, 1.0)
+    h_sh = 0.85 * h_float * (1.0 - t) + h_shelf * t
+    thck = np.where(X <= x_gl, h_sheet, np.minimum(h_sh, 0.95 * h_float))
+    thck = np.where(X > x_front, 0.0, thck)
+    # land strip (west, north, south): bed above sea level, no ice
+    strip = 4
+    land = np.zeros((nx, ny), bool)
+    land[:strip, :] = True
+    land[:, :strip] = True
+    land[:, -strip:] = True
+    bed = np.where(land, 50.0, bed)
+    thck = np.where(land, 0.0, thck)
+    thck[-3:, :] = 0.0
+    lb = _smooth_field(rng, (nx - 1, ny - 1), beta_corr / dx)
+    beta = beta_median * np.exp(beta_sigma_log * lb)
+    sig = glide_sigma(nz)
+    xv = (np.arange(1, nx) - 2.0) * dx
+    yv = (np.arange(1, ny) - 2.0) * dx
+    XV, YV = np.meshgrid(xv, yv, indexing="ij")
+    ub = u_front * np.clip(XV / L, 0.0, 1.0) ** 2
+    uvel = ub[None] * (0.8 + 0.2 * (1.0 - sig[:, None, None] ** 4))
+    vvel = 0.05 * uvel * np.sin(2 * np.pi * YV[None] / max(L, dx))
+    p = Problem(
+        grid=g, dx=dx, dy=dx, sigma=sig,
+        thck=np.asfortranarray(thck), topg=np.asfortranarray(bed),
+        flwa=np.full((nz - 1, nx, ny), flwa, order="F"),
+        beta=np.asfortranarray(beta),
+        uvel=np.asfortranarray(uvel), vvel=np.asfortranarray(vvel),
+        whichbabc=HO_BABC_BETA_EXTERNAL,
+        meta=dict(workload="marine_sheet", n_vert=n_vert, nz=nz, dx=dx, seed=seed, flwa=flwa, h_max=h_max,
+                  h_shelf=h_shelf, bed_front=bed_front, bed_interior=bed_interior, gl_frac=gl_frac,
+                  front_frac=front_frac),
+    )
+    return p

@@ -31,7 +31,9 @@ class AssemblyIn(ctypes.Structure):
                 ("theta_basal", _P), ("umask_dirichlet", _P), ("vmask_dirichlet", _P),
                 ("whichefvs", ctypes.c_int32), ("efvs_constant", ctypes.c_double),
                 ("effstrain_min", ctypes.c_double), ("n_glen", ctypes.c_double),
-                ("check_symmetry", ctypes.c_int32)]
+                ("check_symmetry", ctypes.c_int32),
+                ("ocean_mask", _P), ("usrf", _P), ("thck", _P), ("stagusrf_lateral", _P), ("stagthck_lateral", _P),
+                ("which_assemble_lateral", ctypes.c_int32)]
 
 
 class AssemblyInfo(ctypes.Structure):
@@ -97,7 +99,7 @@ class BPSolver:
         self.active_cell, self.active_vertex = ge.vertex_masks(self.ice_mask)
         self.flwafact = ge.flwafact(np.asfortranarray(p.flwa), p.n_glen)
         self.theta_basal = ge.slope_angle(p.dx, p.dy, ge.stagger(self.lsrf))
-        self.beta_internal = ge.beta_internal(p, self.floating_mask)
+        self.beta_internal = ge.beta_internal(p, self.ice_mask, self.floating_mask, self.land_mask)
         self.xV, self.yV = p.vertex_coords()
         self.sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
         self.swbs = np.ascontiguousarray(stagwbndsigma(p.sigma))
@@ -106,6 +108,13 @@ class BPSolver:
         if no_slip:
             self.umask[g.nz - 1] = 1
         self.vmask = self.umask.copy(order="F")
+        # lateral BC at calving fronts: staggered thickness/surface over ice-covered or
+        # land cells (glissade_velo_higher.F90:1510-1515, 2006-2016)
+        self.thck = np.asfortranarray(np.array(p.thck, dtype=np.float64))
+        ipl = ((self.ice_mask == 1) | (self.land_mask == 1)).astype(np.int32)
+        self.stagthck_lat = ge.stagger_masked(self.thck, ipl)
+        self.stagusrf_lat = ge.stagger_masked(self.usrf, ipl)
+        has_front = bool((self.ocean_mask == 1).any())
         self._ain = AssemblyIn(
             p.dx, p.dy, self.sigma.ctypes.data, self.swbs.ctypes.data, self.active_cell.ctypes.data,
             self.active_vertex.ctypes.data, self.xV.ctypes.data, self.yV.ctypes.data, self.stagusrf.ctypes.data,
@@ -113,7 +122,9 @@ class BPSolver:
             self.flwafact.ctypes.data, None if no_slip else self.beta_internal.ctypes.data,
             self.theta_basal.ctypes.data, self.umask.ctypes.data if no_slip else None,
             self.vmask.ctypes.data if no_slip else None, p.whichefvs, p.efvs_constant, p.effstrain_min,
-            p.n_glen, 1)
+            p.n_glen, 1,
+            self.ocean_mask.ctypes.data if has_front else None, self.usrf.ctypes.data, self.thck.ctypes.data,
+            self.stagusrf_lat.ctypes.data, self.stagthck_lat.ctypes.data, int(p.assemble_lateral))
 
     def zero_inactive(self, uvel, vvel):
         """Velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""

@@ -147,6 +147,13 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nre
                                   double* avg_ms, double* bytes_per_launch);
 
 
+/* HBM bandwidth probes (no reference counterpart; measurement only):
+ * STREAM-style copy (2 x nbytes moved), triad (3 x nbytes) and read
+ * (1 x nbytes) over arrays of nbytes each, average of nrep launches,
+ * in GB/s.  Any output pointer may be NULL. */
+int32_t cismgpu_bench_stream(int64_t nbytes, int32_t nrep, double* copy_gbps, double* triad_gbps,
+                             double* read_gbps);
+
 /* x = 0 on the device (initial guess for benchmarks) */
 int32_t cismgpu_system_zero_x(cismgpu_system* sys);
 
@@ -185,6 +192,14 @@ typedef struct cismgpu_assembly_in {
     double  effstrain_min;
     double  n_glen;
     int32_t check_symmetry;          /* 1: check_symmetry_assembled_matrix_3d */
+    /* lateral ocean-pressure BC at calving fronts (load_vector_lateral_bc,
+     * glissade_velo_higher.F90:4453-4957); ocean_mask NULL: not applied */
+    const int32_t* ocean_mask;       /* (nx, ny) */
+    const double*  usrf;             /* (nx, ny) upper surface */
+    const double*  thck;             /* (nx, ny) */
+    const double*  stagusrf_lateral; /* (nx-1, ny-1): usrf, thck averaged over ice-covered or */
+    const double*  stagthck_lateral; /*   land cells (glissade_velo_higher.F90:2006-2016)     */
+    int32_t which_assemble_lateral;  /* 0 standard (staggered values), 1 local (cell values) */
 } cismgpu_assembly_in;
 
 typedef struct cismgpu_assembly_info {

@@ -49,6 +49,13 @@ int64_t orc_check_symmetry_assembled_3d(int nx, int ny, int nz, int ilo, int ihi
 int64_t orc_count_nonzeros_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi,
                               const int32_t* active_vertex, const double* Auu, const double* Auv,
                               const double* Avu, const double* Avv);
+void orc_stagger_masked(int nx, int ny, const double* var, const int32_t* mask, double* stagvar);
+void orc_load_vector_lateral_bc(int nx, int ny, int nz, const double* sigma, int nhalo, int local,
+                                const int32_t* ocean_mask, const int32_t* active_cell,
+                                const double* xVertex, const double* yVertex, const double* usrf, const double* thck,
+                                const double* stagusrf, const double* stagthck, double* loadu, double* loadv);
+void orc_grounded_fraction_no_glp(int nx, int ny, const int32_t* ice_mask, const int32_t* floating_mask,
+                                  const int32_t* land_mask, double* f_ground);
 double orc_residual_velocity_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int whichresid,
                                 const double* usav, const double* vsav, const double* uvel, const double* vvel);
 

@@ -581,3 +581,163 @@ double orc_residual_velocity_3d(int nx, int ny, int nz, int ilo, int ihi, int jl
     if (whichresid == 2 && count > 0) r = r / (double)count;
     return r;
 }
+
+/* ------------------------------------------------------------------ */
+/* Lateral (calving-front) boundary condition                           */
+/* ------------------------------------------------------------------ */
+
+/* glissade_stagger with stagger_margin = 1: average over the cells whose
+ * mask is 1 (glissade_grid_operators.F90:135-146). */
+void orc_stagger_masked(int nx, int ny, const double* var, const int32_t* mask, double* stagvar)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const double sumvar = mask[C2(i, j + 1)] * var[C2(i, j + 1)] + mask[C2(i + 1, j + 1)] * var[C2(i + 1, j + 1)]
+                                + mask[C2(i, j)] * var[C2(i, j)] + mask[C2(i + 1, j)] * var[C2(i + 1, j)];
+            const double summask = (double)(mask[C2(i, j + 1)] + mask[C2(i + 1, j + 1)] + mask[C2(i, j)] +
+                                            mask[C2(i + 1, j)]);
+            stagvar[V2(i, j)] = summask > 0.0 ? sumvar / summask : 0.0;
+        }
+}
+
+/* lateral_shelf_bc (glissade_velo_higher.F90:4609-4957) for one face of
+ * cell (ic, jc).  face: 0 west, 1 east, 2 south, 3 north.  The face is a
+ * 2-D bilinear element per layer (nodes 1,2 at the lower level k+1, 3,4
+ * above them at level k), integrated with the 2x2 Gauss rule; the net
+ * outward pressure p_av = rho_i g h/2 - rho_o g h (1 - min(s/h,1))^2 / 2
+ * is added to loadu (west/east faces) or loadv (south/north). */
+static void lateral_face(int nx, int ny, int nz, const double* sigma, int local, int face, int ic, int jc,
+                         const double* usrf, const double* thck, const double* stagusrf, const double* stagthck,
+                         const double* xVertex, const double* yVertex, double* loadu, double* loadv)
+{
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r}, wq[4] = {1.0, 1.0, 1.0, 1.0};
+    double phi[4][4], dxr[4][4], dyr[4][4];   /* [n][p] */
+    for (int p = 0; p < 4; ++p) {
+        phi[0][p] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+        phi[1][p] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+        phi[2][p] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+        phi[3][p] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+        dxr[0][p] = -(1.0 - yq[p]) / 4.0;
+        dxr[1][p] = (1.0 - yq[p]) / 4.0;
+        dxr[2][p] = (1.0 + yq[p]) / 4.0;
+        dxr[3][p] = -(1.0 + yq[p]) / 4.0;
+        dyr[0][p] = -(1.0 - xq[p]) / 4.0;
+        dyr[1][p] = -(1.0 + xq[p]) / 4.0;
+        dyr[2][p] = (1.0 + xq[p]) / 4.0;
+        dyr[3][p] = (1.0 - xq[p]) / 4.0;
+    }
+    int iN[4], jN[4];
+    double x[4], s[4], h[4], y[4];
+    switch (face) {
+    case 0: iN[0] = ic - 1; jN[0] = jc;     iN[1] = ic - 1; jN[1] = jc - 1; break;
+    case 1: iN[0] = ic;     jN[0] = jc - 1; iN[1] = ic;     jN[1] = jc;     break;
+    case 2: iN[0] = ic - 1; jN[0] = jc - 1; iN[1] = ic;     jN[1] = jc - 1; break;
+    default: iN[0] = ic;    jN[0] = jc;     iN[1] = ic - 1; jN[1] = jc;     break;
+    }
+    const double* coord = (face <= 1) ? yVertex : xVertex;
+    x[0] = coord[V2(iN[0], jN[0])];
+    x[1] = coord[V2(iN[1], jN[1])];
+    iN[2] = iN[1]; jN[2] = jN[1]; iN[3] = iN[0]; jN[3] = jN[0];
+    x[2] = x[1];
+    x[3] = x[0];
+    if (local) {
+        for (int n = 0; n < 4; ++n) { s[n] = usrf[C2(ic, jc)]; h[n] = thck[C2(ic, jc)]; }
+    } else {
+        s[0] = stagusrf[V2(iN[0], jN[0])];
+        s[1] = stagusrf[V2(iN[1], jN[1])];
+        s[2] = s[1];
+        s[3] = s[0];
+        h[0] = stagthck[V2(iN[0], jN[0])];
+        h[1] = stagthck[V2(iN[1], jN[1])];
+        h[2] = h[1];
+        h[3] = h[0];
+    }
+    for (int k = 1; k <= nz - 1; ++k) {
+        y[0] = s[0] - sigma[k] * h[0];
+        y[1] = s[1] - sigma[k] * h[1];
+        y[2] = s[2] - sigma[k - 1] * h[2];
+        y[3] = s[3] - sigma[k - 1] * h[3];
+        const int kN[4] = {k + 1, k + 1, k, k};
+        for (int p = 0; p < 4; ++p) {
+            double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+            for (int n = 0; n < 4; ++n) {
+                J11 = J11 + dxr[n][p] * x[n];
+                J12 = J12 + dxr[n][p] * y[n];
+                J21 = J21 + dyr[n][p] * x[n];
+                J22 = J22 + dyr[n][p] * y[n];
+            }
+            const double detJ = fabs(J11 * J22 - J12 * J21);
+            double h_qp, s_qp;
+            if (local) {
+                h_qp = thck[C2(ic, jc)];
+                s_qp = usrf[C2(ic, jc)];
+            } else {
+                h_qp = 0.0;
+                s_qp = 0.0;
+                for (int n = 0; n < 4; ++n) {
+                    h_qp = h_qp + phi[n][p] * h[n];
+                    s_qp = s_qp + phi[n][p] * s[n];
+                }
+            }
+            const double f = 1.0 - fmin(s_qp / h_qp, 1.0);
+            const double p_av = 0.5 * RHOI * GRAV * h_qp - 0.5 * RHOO * GRAV * h_qp * (f * f);
+            for (int n = 0; n < 4; ++n) {
+                const double t = p_av * wq[p] * detJ / VOL0 * phi[n][p];
+                const size_t c = V3(kN[n], iN[n], jN[n]);
+                if (face == 0) loadu[c] = loadu[c] - t;
+                else if (face == 1) loadu[c] = loadu[c] + t;
+                else if (face == 2) loadv[c] = loadv[c] - t;
+                else loadv[c] = loadv[c] + t;
+            }
+        }
+    }
+}
+
+/* load_vector_lateral_bc (glissade_velo_higher.F90:4453-4605): every active
+ * cell of the loop range that shares an edge with an ice-free ocean cell
+ * gets the face term on that edge; faces in the order west, east, south,
+ * north.  local = which_ho_assemble_lateral (1: the cell's own usrf/thck;
+ * 0: staggered values at the face's vertices). */
+void orc_load_vector_lateral_bc(int nx, int ny, int nz, const double* sigma, int nhalo, int local,
+                                const int32_t* ocean_mask, const int32_t* active_cell,
+                                const double* xVertex, const double* yVertex, const double* usrf, const double* thck,
+                                const double* stagusrf, const double* stagthck, double* loadu, double* loadv)
+{
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            if (ocean_mask[C2(i - 1, j)] == 1)
+                lateral_face(nx, ny, nz, sigma, local, 0, i, j, usrf, thck, stagusrf, stagthck, xVertex, yVertex, loadu, loadv);
+            if (ocean_mask[C2(i + 1, j)] == 1)
+                lateral_face(nx, ny, nz, sigma, local, 1, i, j, usrf, thck, stagusrf, stagthck, xVertex, yVertex, loadu, loadv);
+            if (ocean_mask[C2(i, j - 1)] == 1)
+                lateral_face(nx, ny, nz, sigma, local, 2, i, j, usrf, thck, stagusrf, stagthck, xVertex, yVertex, loadu, loadv);
+            if (ocean_mask[C2(i, j + 1)] == 1)
+                lateral_face(nx, ny, nz, sigma, local, 3, i, j, usrf, thck, stagusrf, stagthck, xVertex, yVertex, loadu, loadv);
+        }
+}
+
+/* glissade_grounded_fraction with which_ho_ground = 0 (no GLP) and
+ * which_ho_fground_no_glp = 0 (grounded neighbour), the defaults
+ * (glissade_grounding_line.F90:59-674; glide_types.F90:1090, 1099):
+ * vmask(i,j) = 1 if any of the four cells has ice, or one of the land
+ * cells (i,j+1), (i+1,j+1), (i,j), (i+1,j+1) -- the reference tests
+ * (i+1,j+1) twice and never (i+1,j); f_ground = 1 at vmask vertices with
+ * a neighbour cell that is land or grounded ice, else 0. */
+void orc_grounded_fraction_no_glp(int nx, int ny, const int32_t* ice_mask, const int32_t* floating_mask,
+                                  const int32_t* land_mask, double* f_ground)
+{
+#define CG_(i, j) ((ice_mask[C2(i, j)] == 1 && floating_mask[C2(i, j)] == 0) || land_mask[C2(i, j)] == 1)
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const int vmask = ice_mask[C2(i, j + 1)] == 1 || ice_mask[C2(i + 1, j + 1)] == 1 ||
+                              ice_mask[C2(i, j)] == 1 || ice_mask[C2(i + 1, j)] == 1 ||
+                              land_mask[C2(i, j + 1)] == 1 || land_mask[C2(i + 1, j + 1)] == 1 ||
+                              land_mask[C2(i, j)] == 1 || land_mask[C2(i + 1, j + 1)] == 1;
+            double f = 0.0;
+            if (vmask && (CG_(i, j + 1) || CG_(i + 1, j + 1) || CG_(i, j) || CG_(i + 1, j))) f = 1.0;
+            f_ground[V2(i, j)] = f;
+        }
+#undef CG_
+}

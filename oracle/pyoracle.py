@@ -78,6 +78,9 @@ def _declare(L):
     L.orc_count_nonzeros_3d.restype = ctypes.c_int64
     L.orc_residual_velocity_3d.argtypes = [i, i, i, i, i, i, i, i, _D, _D, _D, _D]
     L.orc_residual_velocity_3d.restype = ctypes.c_double
+    L.orc_stagger_masked.argtypes = [i, i, _D, _I, _D]
+    L.orc_load_vector_lateral_bc.argtypes = [i, i, i, _D, i, i, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_grounded_fraction_no_glp.argtypes = [i, i, _I, _I, _I, _D]
 
 
 def dp(a: np.ndarray):
@@ -218,14 +221,25 @@ def geometry(problem):
     L.orc_stagger(nx, ny, dp(lsrf), dp(staglsrf))
     theta = np.zeros((nx - 1, ny - 1), order="F")
     L.orc_slope_angle(nx - 1, ny - 1, p.dx, p.dy, dp(staglsrf), dp(theta))
+    fground = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_grounded_fraction_no_glp(nx, ny, ip(ice), ip(flt), ip(land), dp(fground))
+    # thickness and surface at vertices for the standard lateral BC: mean over
+    # ice-covered or land cells (glissade_velo_higher.F90:1510-1515, 2006-2016)
+    ipl = np.asfortranarray(((ice == 1) | (land == 1)).astype(np.int32))
+    stagthck_lat = np.zeros((nx - 1, ny - 1), order="F")
+    stagusrf_lat = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_stagger_masked(nx, ny, dp(p.thck), ip(ipl), dp(stagthck_lat))
+    L.orc_stagger_masked(nx, ny, dp(usrf), ip(ipl), dp(stagusrf_lat))
     return dict(lsrf=lsrf, usrf=usrf, ice_mask=ice, floating_mask=flt, ocean_mask=ocn, land_mask=land,
                 stagthck=stagthck, stagusrf=stagusrf, dusrf_dx=dsdx, dusrf_dy=dsdy,
-                active_cell=acell, active_vertex=avert, flwafact=flwafact, theta_basal=theta)
+                active_cell=acell, active_vertex=avert, flwafact=flwafact, theta_basal=theta,
+                f_ground=fground, stagthck_lateral=stagthck_lat, stagusrf_lateral=stagusrf_lat)
 
 
 def beta_internal(problem, geo):
-    """glissade_calcbeta for the constant/large/external cases with f_ground = 1
-    on grounded ice (glissade_basal_traction.F90:202, 304, 399, 650-709)."""
+    """glissade_calcbeta for the constant/large/external cases
+    (glissade_basal_traction.F90:202, 304, 399), times f_ground with
+    beta_grounded_min = 0 (:650-672), then the staggered halo (:697)."""
     p = problem
     g = p.grid
     if p.whichbabc == 0:
@@ -234,6 +248,11 @@ def beta_internal(problem, geo):
         b = np.full((g.nx - 1, g.ny - 1), 1.0e10, order="F")
     else:
         b = np.asfortranarray(p.beta.copy())
+    fg = geo["f_ground"]
+    for j in range(g.ny - 1):
+        for i in range(g.nx - 1):
+            b[i, j] = b[i, j] * fg[i, j]
+    halo(g, b)
     return b
 
 
@@ -260,6 +279,10 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     swb = stagwbndsigma(p.sigma)
     L.orc_load_vector_gravity_local(nx, ny, nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
                                     dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+    sigma_ = np.ascontiguousarray(p.sigma, dtype=np.float64)
+    L.orc_load_vector_lateral_bc(nx, ny, nz, dp(sigma_), 2, int(p.assemble_lateral), ip(geo["ocean_mask"]),
+                                 ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["usrf"]), dp(p.thck),
+                                 dp(geo["stagusrf_lateral"]), dp(geo["stagthck_lateral"]), dp(loadu), dp(loadv))
     A = [g.zeros_mat() for _ in range(4)]
     efvs = np.zeros((nz - 1, nx, ny), order="F")
     sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)

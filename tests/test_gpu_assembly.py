@@ -21,7 +21,14 @@ def _problems():
         "dome": synthetic.dome(),
         "stream": synthetic.ice_stream(40, nz=11, seed=3),
         "slab_periodic": synthetic.slab(n_cells=8, nz=9),
+        "marine": synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3),
+        "marine_local_front": _local_front(synthetic.marine_sheet(36, nz=5, dx=2500.0, seed=4)),
     }
+
+
+def _local_front(p):
+    p.assemble_lateral = 1
+    return p
 
 
 PROBLEMS = _problems()
@@ -71,7 +78,7 @@ def test_assembly_matches_oracle(require_gpu, name):
     assert info["asym_pairs"] == 0 and ref["nwarn_asym"] == 0
 
 
-@pytest.mark.parametrize("name", ["dome", "stream"])
+@pytest.mark.parametrize("name", ["dome", "stream", "marine"])
 def test_assembled_solve_matches_oracle(require_gpu, name):
     p = PROBLEMS[name]
     ref = po.assemble(p)
@@ -90,8 +97,8 @@ def test_assembled_solve_matches_oracle(require_gpu, name):
 
 
 def test_dome_picard_matches_oracle(require_gpu):
-    """dome, no-slip, standard PCG with diagonal
-    preconditioning to 1e-8, Picard to 1e-4 maximum relative speed change."""
+    """Dome, no-slip base, standard PCG with diagonal preconditioning to
+    1e-8, Picard to 1e-4 maximum relative speed change."""
     p = PROBLEMS["dome"]
     ref = po.picard(p, algo="standard", precond=1, linear_tol=1e-8, linear_maxiters=200, resid_target=1e-4)
     s = BPSolver(p)

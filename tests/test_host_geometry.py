@@ -45,3 +45,15 @@ def test_gradient_margin_options(margin):
     ge.staggered_halo_2d(p.grid, dsdx)
     ge.staggered_halo_2d(p.grid, dsdy)
     assert np.array_equal(dsdx, geo["dusrf_dx"]) and np.array_equal(dsdy, geo["dusrf_dy"])
+
+
+def test_grounded_fraction_and_masked_stagger_bitwise():
+    p = synthetic.marine_sheet(40, nz=5, dx=2000.0, seed=3)
+    geo = po.geometry(p)
+    fg = ge.grounded_fraction(geo["ice_mask"], geo["floating_mask"], geo["land_mask"])
+    assert np.array_equal(fg, geo["f_ground"])
+    ipl = ((geo["ice_mask"] == 1) | (geo["land_mask"] == 1)).astype(np.int32)
+    assert np.array_equal(ge.stagger_masked(np.asfortranarray(p.thck), ipl), geo["stagthck_lateral"])
+    assert np.array_equal(ge.stagger_masked(geo["usrf"], ipl), geo["stagusrf_lateral"])
+    b = ge.beta_internal(p, geo["ice_mask"], geo["floating_mask"], geo["land_mask"])
+    assert np.array_equal(b, po.beta_internal(p, geo))

@@ -466,3 +466,81 @@ def test_slab_matches_first_order_analytic_solution():
     assert e2 < 1e-3
     assert 3.5 < e1 / e2 < 4.5
     assert b1 < 1e-7 and b2 < 1e-7
+
+
+def _front_problem(H, topg, topg_ocean=-1000.0, eus=0.0, nz=6):
+    """Uniform slab of thickness H on a bed at topg in the cells i <= 6 (all
+    j), ice-free ocean (bed topg_ocean < eus) east of it."""
+    g = synthetic.Grid.from_owned(8, 8, nz, global_bc=BC_NO_ICE)
+    thck = np.zeros((g.nx, g.ny), order="F")
+    thck[:6, :] = H
+    bed = np.full((g.nx, g.ny), float(topg_ocean), order="F")
+    bed[:6, :] = topg
+    p = synthetic.Problem(grid=g, dx=1000.0, dy=750.0, sigma=glide_sigma(nz), thck=thck,
+                          topg=bed,
+                          flwa=np.full((nz - 1, g.nx, g.ny), 1e-16, order="F"),
+                          beta=np.zeros((g.nx - 1, g.ny - 1), order="F"), uvel=g.zeros_vec(), vvel=g.zeros_vec(),
+                          eus=eus)
+    return p
+
+
+@pytest.mark.parametrize("local", [0, 1])
+@pytest.mark.parametrize("case", ["floating", "grounded_cliff"])
+def test_lateral_bc_calving_front_force(local, case):
+    """load_vector_lateral_bc (glissade_velo_higher.F90:4453-4957): the
+    depth-integrated net pressure on a calving front is
+    rho_i g H^2 / 2 - rho_o g D^2 / 2 (D = submerged depth) per unit
+    width -- rho_i g H^2 (1 - rho_i/rho_o) / 2 for a floating shelf and
+    rho_i g H^2 / 2 for a grounded cliff whose surface is above H -- and a
+    uniform pressure loads each level with half the thickness of the layers
+    on either side."""
+    rhoi, rhoo, grav, vol0 = 910.0, 1028.0, 9.81, 1.0e9
+    H = 400.0
+    if case == "floating":
+        p = _front_problem(H, topg=-1000.0)
+        F = 0.5 * rhoi * grav * H * H * (1.0 - rhoi / rhoo)
+    else:
+        # ice on a bed above sea level facing open ocean: the surface is above H
+        p = _front_problem(H, topg=5.0)
+        F = 0.5 * rhoi * grav * H * H
+    p.assemble_lateral = local
+    geo = po.geometry(p)
+    g = p.grid
+    assert geo["ocean_mask"][6, 3] == 1 and geo["active_cell"][5, 3] == 1
+    xV, yV = p.vertex_coords()
+    lu, lv = g.zeros_vec(), g.zeros_vec()
+    po.lib().orc_load_vector_lateral_bc(g.nx, g.ny, g.nz, po.dp(np.ascontiguousarray(p.sigma)), 2, local,
+                                        po.ip(geo["ocean_mask"]), po.ip(geo["active_cell"]), po.dp(xV), po.dp(yV),
+                                        po.dp(geo["usrf"]), po.dp(p.thck), po.dp(geo["stagusrf_lateral"]),
+                                        po.dp(geo["stagthck_lateral"]), po.dp(lu), po.dp(lv))
+    # east faces of the cells (6, j), j = 3 .. ny-1, load the vertex column i = 6
+    nface = g.ny - 1 - 3 + 1
+    east = lu[:, 5, :]
+    assert np.isclose(east.sum() * vol0, nface * F * p.dy, rtol=1e-12)
+    assert np.abs(lv).max() == 0.0
+    # level distribution of the uniform pressure: (dsigma_below + dsigma_above)/2
+    ds = np.diff(p.sigma)
+    share = np.zeros(g.nz)
+    share[:-1] += ds / 2
+    share[1:] += ds / 2
+    assert np.allclose(east.sum(axis=1) * vol0, nface * F * p.dy * share, rtol=1e-12)
+
+
+def test_grounded_fraction_rules():
+    """f_ground (no grounding-line parameterisation, grounded-neighbour rule):
+    1 under and next to grounded ice or land, 0 at vertices surrounded by
+    floating ice or open ocean only; beta_internal vanishes where f_ground does."""
+    p = _front_problem(400.0, topg=-1000.0)      # floating shelf
+    p.thck[:3, :] = 1200.0                      # grounded band in the west (1200 m > 1028/910 * 1000)
+    geo = po.geometry(p)
+    fg = geo["f_ground"]
+    flt, ice = geo["floating_mask"], geo["ice_mask"]
+    assert flt[4, 4] == 1 and flt[1, 4] == 0 and ice[1, 4] == 1
+    assert fg[0, 4] == 1.0 and fg[2, 4] == 1.0        # touches grounded cells
+    assert fg[3, 4] == 0.0 and fg[6, 4] == 0.0        # floating / ocean only
+    p.whichbabc = synthetic.HO_BABC_BETA_CONSTANT
+    p.beta_const = 1000.0
+    b = po.beta_internal(p, geo)
+    # (the no-ice halo rule also clears the last owned row and column)
+    own = (slice(p.grid.ilo - 1, p.grid.ihi - 1), slice(p.grid.jlo - 1, p.grid.jhi - 1))
+    assert np.array_equal(b[own] == 0.0, fg[own] == 0.0)
