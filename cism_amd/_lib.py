@@ -34,6 +34,7 @@ EXPORTS = (
     "cismgpu_system_residual", "cismgpu_system_zero_x",
     "cismgpu_ensemble_create", "cismgpu_ensemble_destroy", "cismgpu_ensemble_autotune", "cismgpu_ensemble_solve",
     "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info", "cismgpu_bench_stream",
+    "cismgpu_system_save_assembly", "cismgpu_system_restore_assembly",
 )
 
 

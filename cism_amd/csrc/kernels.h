@@ -15,6 +15,11 @@ enum { ST_BB = 0, ST_RR, ST_CG_FIRST, ST_CG_ITER, ST_STD_ETA1, ST_STD_ETA2, ST_D
 template <typename TA, typename TV, int MODE>
 __global__ void k_spmv(Dims d, const TA* A, const TV* xu, const TV* xv, TV* yu, TV* yv, const TV* bu,
                        const TV* bv, double* partials);
+template <typename TA, int MODE>
+__global__ void k_spmv_q(Dims d, const TA* A, const float* xu, const float* xv, float* yu, float* yv,
+                         const float* bu, const float* bv, double* partials);
+template <typename TA>
+__global__ void k_convert_quad(long long nrows, const double* src, TA* dst);
 template <typename TV>
 __global__ void k_ring(Dims d, TV* yu, TV* yv, const TV* bu, const TV* bv);
 template <typename TV>

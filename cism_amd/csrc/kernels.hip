@@ -93,6 +93,104 @@ __global__ __launch_bounds__(256) void k_spmv(Dims d, const TA* __restrict__ A,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Low-precision SpMV on the quad layout (FP32 or BF16 storage, FP32 vectors).
+// For owned row r and neighbour m the four block coefficients are adjacent,
+//     Aq[((r/64)*27 + m)*256 + (r%64)*4 + blk]      (blk = uu, uv, vu, vv)
+// so a lane fetches them with one 16-byte (FP32) or 8-byte (BF16) load and
+// a wavefront reads 1 KiB (512 B) contiguous per neighbour.  Each thread
+// owns one column and walks up the levels keeping the 3x3 neighbourhood of
+// levels k-1, k, k+1 in registers, so x is loaded once per level instead of
+// three times.  Same MODEs and partial sums as k_spmv.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ld_quad(const float* p, float& a, float& b, float& c, float& e) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    a = q.x; b = q.y; c = q.z; e = q.w;
+}
+__device__ __forceinline__ void ld_quad(const __hip_bfloat16* p, float& a, float& b, float& c, float& e) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    a = __uint_as_float(q.x << 16);
+    b = __uint_as_float(q.x & 0xffff0000u);
+    c = __uint_as_float(q.y << 16);
+    e = __uint_as_float(q.y & 0xffff0000u);
+}
+
+template <typename TA, int MODE>
+__global__ __launch_bounds__(256) void k_spmv_q(Dims d, const TA* __restrict__ A, const float* __restrict__ xu,
+                                                const float* __restrict__ xv, float* __restrict__ yu,
+                                                float* __restrict__ yv, const float* __restrict__ bu,
+                                                const float* __restrict__ bv, double* __restrict__ partials) {
+    const int nbx = d.nxp / WAVE;
+    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
+    const int lane = threadIdx.x % WAVE;
+    const int io = bx * WAVE + lane;
+    const int jo = by * 4 + threadIdx.x / WAVE;
+    dd acc = {0.0, 0.0};
+    if (io < d.nxo && jo < d.nyo) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+        const long long pl = d.plane;
+        const int px = d.px;
+        float wu[3][9], wv[3][9];
+        const long long cb = vidx(d, -1, i, j);
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                const long long n = cb + l * pl + (h / 3 - 1) * px + (h % 3 - 1);
+                wu[l][h] = xu[n];
+                wv[l][h] = xv[n];
+            }
+        for (int k = 0; k < d.nz; ++k) {
+            const long long cup = cb + (long long)(k + 2) * pl;
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                const long long n = cup + (h / 3 - 1) * px + (h % 3 - 1);
+                wu[2][h] = xu[n];
+                wv[2][h] = xv[n];
+            }
+            const long long r = ((long long)k * d.nyo + jo) * d.nxp + io;
+            const TA* a = A + (r / WAVE) * (27 * 4 * WAVE) + (r % WAVE) * 4;
+            float su = 0.f, sv = 0.f;
+#pragma unroll
+            for (int m = 0; m < 27; ++m) {
+                float cuu, cuv, cvu, cvv;
+                ld_quad(a + m * (4 * WAVE), cuu, cuv, cvu, cvv);
+                const float xun = wu[m / 9][m % 9], xvn = wv[m / 9][m % 9];
+                su = su + cuu * xun + cuv * xvn;
+                sv = sv + cvu * xun + cvv * xvn;
+            }
+            const long long c0 = cb + (long long)(k + 1) * pl;
+            if (MODE == 2) {
+                const float ru = bu[c0] - su, rv = bv[c0] - sv;
+                yu[c0] = ru;
+                yv[c0] = rv;
+                acc = dd_add_d(acc, (double)ru * (double)ru);
+                acc = dd_add_d(acc, (double)rv * (double)rv);
+            } else {
+                yu[c0] = su;
+                yv[c0] = sv;
+                if (MODE == 1) {
+                    acc = dd_add_d(acc, (double)su * (double)wu[1][4]);
+                    acc = dd_add_d(acc, (double)sv * (double)wv[1][4]);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                wu[0][h] = wu[1][h]; wv[0][h] = wv[1][h];
+                wu[1][h] = wu[2][h]; wv[1][h] = wv[2][h];
+            }
+        }
+    }
+    if (MODE != 0) {
+        dd v[1] = {acc};
+        block_reduce_dd<1, 256>(v);
+        if (threadIdx.x == 0) {
+            partials[2 * blockIdx.x + 0] = v[0].hi;
+            partials[2 * blockIdx.x + 1] = v[0].lo;
+        }
+    }
+}
+
 // Non-owned ring of the vertex grid: y = 0 after a matvec (the reference
 // zeroes the whole output, pcg.F90:5605-5606), or r = b - 0 after a
 // residual.  One thread per ring column.
@@ -604,6 +702,26 @@ __global__ void k_convert_matrix<__hip_bfloat16>(long long n, const double* __re
     if (t < n) dst[t] = __float2bfloat16((float)src[t]);
 }
 
+// Row-blocked FP64 -> quad-layout lower-precision copy; one thread per
+// destination element, so the writes are contiguous.
+template <typename TA>
+__device__ __forceinline__ TA to_lowp(double v) { return (TA)v; }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_lowp<__hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+
+template <typename TA>
+__global__ void k_convert_quad(long long nrows, const double* __restrict__ src, TA* __restrict__ dst) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nrows * NCOEF) return;
+    const int blk = (int)(t % 4);
+    const long long u = t / 4;
+    const int lane = (int)(u % WAVE);
+    const long long v = u / WAVE;
+    const int m = (int)(v % 27);
+    const long long rb = v / 27;
+    dst[t] = to_lowp<TA>(src[(rb * NCOEF + blk * 27 + m) * WAVE + lane]);
+}
+
 // Row-blocked -> reference layout for one block (export).
 __global__ void k_unpack_matrix(Dims d, int blk, const double* __restrict__ A, double* __restrict__ dst) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -698,5 +816,16 @@ __global__ void k_ir_update(Dims d, double* __restrict__ xu, double* __restrict_
 INST_SPMV(double, double)
 INST_SPMV(float, float)
 INST_SPMV(__hip_bfloat16, float)
+#define INST_SPMV_Q(TA)                                                                                      \
+    template __global__ void k_spmv_q<TA, 0>(Dims, const TA*, const float*, const float*, float*, float*,     \
+                                             const float*, const float*, double*);                          \
+    template __global__ void k_spmv_q<TA, 1>(Dims, const TA*, const float*, const float*, float*, float*,     \
+                                             const float*, const float*, double*);                          \
+    template __global__ void k_spmv_q<TA, 2>(Dims, const TA*, const float*, const float*, float*, float*,     \
+                                             const float*, const float*, double*);
+INST_SPMV_Q(float)
+INST_SPMV_Q(__hip_bfloat16)
+template __global__ void k_convert_quad<float>(long long, const double*, float*);
+template __global__ void k_convert_quad<__hip_bfloat16>(long long, const double*, __hip_bfloat16*);
 
 }  // namespace cg

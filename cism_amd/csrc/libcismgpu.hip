@@ -70,6 +70,9 @@ struct cismgpu_system {
     size_t asm_bytes = 0;
     double* efvs = nullptr;
     double* Atmp = nullptr;      // second matrix buffer for the symmetry fix
+    double* saved = nullptr;     // saved assembly (A, b, preconditioner, raw diagonal blocks)
+    size_t saved_n = 0;          // doubles in `saved`
+    bool have_saved = false;
     bool have_matrix = false;
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
@@ -139,7 +142,10 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
           double* part) {
     const bool t = S->timing && S->ev_used + 2 <= (int)S->ev.size();
     if (t) hipEventRecord(S->ev[S->ev_used], S->s);
-    k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    if constexpr (std::is_same<TA, double>::value)
+        k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    else   // FP32 / BF16 copies are stored in the quad layout
+        k_spmv_q<TA, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     if (t) { hipEventRecord(S->ev[S->ev_used + 1], S->s); S->ev_used += 2; }
 }
 
@@ -508,12 +514,12 @@ int ensure_lowp(cismgpu_system* S, int precision) {
     }
     if (precision == CISMGPU_MIXED_FP32 && !S->a32_ready) {
         if (!S->A32) CG_CHECK(hipMalloc(&S->A32, na * sizeof(float)));
-        k_convert_matrix<float><<<nblocks(na, 256), 256, 0, S->s>>>(na, S->A64, S->A32);
+        k_convert_quad<float><<<nblocks(na, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A32);
         S->a32_ready = true;
     }
     if (precision == CISMGPU_MIXED_BF16 && !S->a16_ready) {
         if (!S->A16) CG_CHECK(hipMalloc(&S->A16, na * sizeof(__hip_bfloat16)));
-        k_convert_matrix<__hip_bfloat16><<<nblocks(na, 256), 256, 0, S->s>>>(na, S->A64, S->A16);
+        k_convert_quad<__hip_bfloat16><<<nblocks(na, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A16);
         S->a16_ready = true;
     }
     if (S->tri.mode && !S->tri.a32) return CISMGPU_ERR_ARG;
@@ -655,7 +661,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -856,9 +862,9 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
         if (kind == 0)
             k_spmv<double, double, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, nullptr);
         else if (kind == 1)
-            k_spmv<float, float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+            k_spmv_q<float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
         else
-            k_spmv<__hip_bfloat16, float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+            k_spmv_q<__hip_bfloat16, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
     };
     launch();   // warm-up
     CG_CHECK(hipEventRecord(e0, S->s));
@@ -1117,6 +1123,68 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     S->m32_ready = S->a32_ready = S->a16_ready = false;
     S->tri.mode = 0;
     if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
+    return CISMGPU_OK;
+}
+
+// Saved-assembly snapshot for the accelerated Picard iteration
+// (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184):
+// the owned rows of A, b and the preconditioner coefficients derived from A.
+static size_t saved_doubles(const cismgpu_system* S) {
+    const Dims& d = S->d;
+    const size_t nraw = 27ull * d.nz * d.nvx * d.nvy;
+    return (size_t)d.nrows * NCOEF + 2 * (size_t)d.vlen + 8 * (size_t)d.vlen + (S->stage ? 2 * nraw : 0);
+}
+
+static void saved_parts(cismgpu_system* S, double* base, double* parts[4], size_t sizes[4]) {
+    const Dims& d = S->d;
+    const size_t nraw = 27ull * d.nz * d.nvx * d.nvy;
+    sizes[0] = (size_t)d.nrows * NCOEF;
+    sizes[1] = 2 * (size_t)d.vlen;
+    sizes[2] = 8 * (size_t)d.vlen;
+    sizes[3] = S->stage ? 2 * nraw : 0;
+    parts[0] = base;
+    parts[1] = parts[0] + sizes[0];
+    parts[2] = parts[1] + sizes[1];
+    parts[3] = parts[2] + sizes[2];
+}
+
+extern "C" int32_t cismgpu_system_save_assembly(cismgpu_system* S) {
+    if (!S || !S->have_matrix) return CISMGPU_ERR_ARG;
+    const size_t need = saved_doubles(S);
+    if (S->saved && S->saved_n < need) {
+        CG_CHECK(hipFree(S->saved));
+        S->saved = nullptr;
+    }
+    if (!S->saved) {
+        CG_CHECK(hipMalloc(&S->saved, need * sizeof(double)));
+        S->saved_n = need;
+    }
+    double* parts[4];
+    size_t sizes[4];
+    saved_parts(S, S->saved, parts, sizes);
+    double* src[4] = {S->A64, S->V(2), S->M64, S->stage};
+    for (int q = 0; q < 4; ++q)
+        if (sizes[q]) CG_CHECK(hipMemcpyAsync(parts[q], src[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->have_saved = true;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
+    if (!S || !S->have_saved || S->saved_n < saved_doubles(S)) {
+        cismgpu_set_error("restore_assembly: nothing saved");
+        return CISMGPU_ERR_ARG;
+    }
+    double* parts[4];
+    size_t sizes[4];
+    saved_parts(S, S->saved, parts, sizes);
+    double* dst[4] = {S->A64, S->V(2), S->M64, S->stage};
+    for (int q = 0; q < 4; ++q)
+        if (sizes[q]) CG_CHECK(hipMemcpyAsync(dst[q], parts[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->have_matrix = true;
+    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->tri.mode = 0;
     return CISMGPU_OK;
 }
 

@@ -203,7 +203,7 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
                  bed_interior: float = -900.0, bump_sigma: float = 30.0, bump_corr: float = 8.0e3,
                  beta_median: float = 1.0e3, beta_sigma_log: float = 0.5, beta_corr: float = 8.0e3,
                  gl_frac: float = 0.55, front_frac: float = 0.85, gl_amp_frac: float = 0.05,
-                 u_front: float = 800.0) -> Problem:
+                 margin_frac: float = 0.25, u_front: float = 800.0) -> Problem:
     """Marine ice sheet on an ``n_vert x n_vert`` owned vertex grid.
 
     Ice flows east: a grounded interior (thickness up to ``h_max``) on a
@@ -211,9 +211,10 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
     the east, plus seeded bumps) thins past a seeded, wavy grounding line
     (mean position ``gl_frac`` of the width) into a floating shelf that ends
     at a calving front (``front_frac``) facing open ocean.  A land strip
-    borders the west, north and south edges, so every halo value is zero and
-    ice meets ocean only at the calving front.  beta is log-normal; under
-    floating ice it is removed by f_ground.
+    borders the west, north and south edges; toward it the bed rises to sea
+    level and the ice thins as sqrt(distance / (margin_frac * width)), so
+    margins are grounded and every halo value is zero.  beta is log-normal;
+    under floating ice it is removed by f_ground.
     """
     rng = np.random.default_rng(seed)
     g = Grid.from_owned(n_vert, n_vert, nz, global_bc=BC_NO_ICE)
@@ -222,8 +223,14 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
     yc = (np.arange(1, ny + 1) - 2.5) * dx
     X, Y = np.meshgrid(xc, yc, indexing="ij")
     L = (n_vert - 1) * dx
+    strip = 4
+    x_w = (strip - 1.5) * dx                              # west edge of the ice
+    y_s, y_n = (strip - 1.5) * dx, (ny - strip - 1.5) * dx
+    d_land = np.maximum(np.minimum(np.minimum(X - x_w, Y - y_s), y_n - Y), 0.0)
     bed = bed_interior + (bed_front - bed_interior) * np.clip(X / L, 0.0, 1.0)
     bed = bed + bump_sigma * _smooth_field(rng, (nx, ny), bump_corr / dx)
+    wb = 0.1 * L
+    bed = bed + (20.0 - bed) * (1.0 - np.clip(d_land / wb, 0.0, 1.0)) ** 2
     # grounding line position varies along y with a seeded phase
     phase = rng.uniform(0, 2 * np.pi)
     x_gl = L * (gl_frac + gl_amp_frac * np.sin(2 * np.pi * Y / max(L, dx) * 1.5 + phase))
@@ -235,9 +242,9 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
     t = np.clip((X - x_gl) / np.maximum(x_front - x_gl, dx), 0.0, 1.0)
     h_sh = 0.85 * h_float * (1.0 - t) + h_shelf * t
     thck = np.where(X <= x_gl, h_sheet, np.minimum(h_sh, 0.95 * h_float))
+    thck = thck * np.sqrt(np.clip(d_land / (margin_frac * L), 0.0, 1.0))
     thck = np.where(X > x_front, 0.0, thck)
     # land strip (west, north, south): bed above sea level, no ice
-    strip = 4
     land = np.zeros((nx, ny), bool)
     land[:strip, :] = True
     land[:, :strip] = True
@@ -263,6 +270,6 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
         whichbabc=HO_BABC_BETA_EXTERNAL,
         meta=dict(workload="marine_sheet", n_vert=n_vert, nz=nz, dx=dx, seed=seed, flwa=flwa, h_max=h_max,
                   h_shelf=h_shelf, bed_front=bed_front, bed_interior=bed_interior, gl_frac=gl_frac,
-                  front_frac=front_frac),
+                  front_frac=front_frac, margin_frac=margin_frac),
     )
     return p

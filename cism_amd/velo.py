@@ -69,6 +69,14 @@ class PicardResult:
     linear_err: list = field(default_factory=list)
     assemble_ms: list = field(default_factory=list)
     solve_ms: list = field(default_factory=list)
+    alpha: list = field(default_factory=list)      # accelerated Picard: alpha after each evaluation
+
+
+# accelerated Picard constants (glissade_velo_higher.F90:831, 1049-1056)
+L2_NORM_LARGE = 1.0e20
+GAMMA_ACCEL = 0.40
+ALPHA_ACCEL_MAX = 3.0
+RESID_REDUCTION_THRESHOLD = 0.85
 
 
 class BPSolver:
@@ -170,32 +178,75 @@ class BPSolver:
         return (L2.value, L2r.value, ru, rv) if want_vectors else (L2.value, L2r.value)
 
     # ---------------------------------------------------------- Picard loop
+    def save_assembly(self):
+        _lib.check(self._L.cismgpu_system_save_assembly(self.sys.handle), "save_assembly")
+
+    def restore_assembly(self):
+        _lib.check(self._L.cismgpu_system_restore_assembly(self.sys.handle), "restore_assembly")
+
+    def _set_x_halo(self, u, v):
+        self.sys.set_x(u, v)
+        _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+
     def picard(self, *, maxiter=100, resid_target=1e-4, whichresid=0, algo=ALGO_CHRONGEAR,
                precond=HO_PRECOND_SIA, ncheck=5, linear_tol=1e-8, linear_maxiters=200, precision=FP64,
-               inner_tol=1e-3, max_outer=20, verbose=False) -> PicardResult:
-        """Outer Picard iteration on the viscosity (glissade_velo_higher.F90:2163-3663),
-        which_ho_nonlinear = 0.  Convergence: resid_velo < resid_target for
-        which_ho_resid 0/1/2, or L2 < resid_target for 3 (:3652-3661)."""
+               inner_tol=1e-3, max_outer=20, accelerate=False, verbose=False) -> PicardResult:
+        """Outer Picard iteration on the viscosity (glissade_velo_higher.F90:2163-3663).
+        accelerate=False: which_ho_nonlinear = 0; True: Picard acceleration
+        (Dukowicz 2019; :2186-2263, :3108-3176, evaluate_accelerated_picard_3d
+        :10052-10184): from the second iteration the velocity is extended along
+        the last Picard step, u = u_old + alpha du, and alpha grows by 0.4 (to
+        at most 3) while each extension cuts the residual |A(u) u - b| below
+        0.85 of the previous one; a rejected extension restores the best
+        velocity and its assembled system.  Convergence: resid_velo <
+        resid_target for which_ho_resid 0/1/2, L2 < resid_target for 3,
+        L2 / |b| for 4 (:3652-3661)."""
         g = self.grid
         u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
         v = np.asfortranarray(np.array(self.p.vvel, dtype=np.float64))
         self.zero_inactive(u, v)
         self.sys.set_x(u, v)
         res = PicardResult(u, v, None, 0, False)
+        if accelerate:
+            u_old, v_old = u.copy(order="F"), v.copy(order="F")
+            du, dv = g.zeros_vec(), g.zeros_vec()
+            alpha, L2_sav = 1.0, L2_NORM_LARGE
+            u_best = v_best = None
         criterion = np.inf
         counter = 0
         while criterion >= resid_target and counter < maxiter:
             counter += 1
-            usav, vsav = self.sys.get_x()
-            ai = self.assemble()
-            res.assemble_ms.append(ai["total_ms"])
-            if ai["nnz"] == 0:          # nothing to solve (:2937-3032)
-                z = g.zeros_vec()
-                res.uvel, res.vvel = z, g.zeros_vec()
-                res.converged = True
-                break
-            L2, L2r = self.residual()
-            res.L2_norm.append(L2)
+            done = False
+            while not done:
+                if accelerate and counter >= 2:
+                    self._set_x_halo(np.asfortranarray(u_old + alpha * du), np.asfortranarray(v_old + alpha * dv))
+                usav, vsav = self.sys.get_x()
+                ai = self.assemble()
+                res.assemble_ms.append(ai["total_ms"])
+                if ai["nnz"] == 0:          # nothing to solve (:2937-3032)
+                    res.uvel, res.vvel = g.zeros_vec(), g.zeros_vec()
+                    res.converged = True
+                    res.iterations = counter
+                    return res
+                L2, L2r = self.residual()
+                res.L2_norm.append(L2)
+                if accelerate and counter >= 2:
+                    if L2 < RESID_REDUCTION_THRESHOLD * L2_sav and alpha + GAMMA_ACCEL <= ALPHA_ACCEL_MAX:
+                        u_best, v_best = usav, vsav
+                        self.save_assembly()
+                        alpha = alpha + GAMMA_ACCEL
+                        L2_sav = L2
+                    elif L2 < L2_sav:
+                        u_old, v_old = usav, vsav
+                        alpha, L2_sav, done = 1.0, L2_NORM_LARGE, True
+                    else:
+                        self.sys.set_x(u_best, v_best)
+                        self.restore_assembly()
+                        u_old, v_old = u_best.copy(order="F"), v_best.copy(order="F")
+                        alpha, L2_sav, done = 1.0, L2_NORM_LARGE, True
+                    res.alpha.append(alpha)
+                else:
+                    done = True
             si = self.sys.solve(algo=algo, precond=precond, ncheck=ncheck, maxiters=linear_maxiters,
                                 tolerance=linear_tol, precision=precision, inner_tol=inner_tol, max_outer=max_outer,
                                 raise_on_error=precision == FP64)
@@ -206,6 +257,8 @@ class BPSolver:
             uu, vv = self.sys.get_x()
             rv = residual_velocity(g, whichresid, usav, vsav, uu, vv)
             res.resid_velo.append(rv)
+            if accelerate:
+                du, dv = uu - u_old, vv - v_old
             criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
             if verbose:
                 print(f"picard {counter}: L2={L2:.4e} resid_velo={rv:.4e} linear iters={si['niters']} "

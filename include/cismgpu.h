@@ -211,6 +211,11 @@ typedef struct cismgpu_assembly_info {
 
 int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_in* in,
                                    cismgpu_assembly_info* info);
+/* Snapshot of the assembled system (A, b, preconditioner) kept on the
+ * device, and its restoration: the matrix copies of the accelerated Picard
+ * iteration (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184). */
+int32_t cismgpu_system_save_assembly(cismgpu_system* sys);
+int32_t cismgpu_system_restore_assembly(cismgpu_system* sys);
 /* efvs(nz-1, nx, ny) of the last assembly (mean over quadrature points) */
 int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
 /* resid = A x - b on owned active vertices, L2 norm and norm relative to

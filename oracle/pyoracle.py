@@ -331,9 +331,11 @@ def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
 
 
 def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chrongear", precond=2, ncheck=5,
-           linear_tol=1e-8, linear_maxiters=200):
+           linear_tol=1e-8, linear_maxiters=200, accelerate=False):
     """Outer Picard loop of glissade_velo_higher_solve for the BP path
-    (glissade_velo_higher.F90:2163-3663, which_ho_nonlinear = 0)."""
+    (glissade_velo_higher.F90:2163-3663), which_ho_nonlinear = 0, or with
+    Picard acceleration (:2186-2263, :3108-3176, :3624-3630;
+    evaluate_accelerated_picard_3d :10052-10184)."""
     L = lib()
     p = problem
     g = p.grid
@@ -344,17 +346,45 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
     inact = geo["active_vertex"][own[1], own[2]] == 0
     for a in (u, v):
         o = a[own]; o[:, inact] = 0.0; a[own] = o
-    hist = dict(L2=[], resid_velo=[], linear_iters=[], linear_err=[])
+    hist = dict(L2=[], resid_velo=[], linear_iters=[], linear_err=[], alpha=[])
+    L2_large, gamma, alpha_max, thresh = 1.0e20, 0.40, 3.0, 0.85        # (:831, :1049-1056)
+    if accelerate:
+        u_old, v_old = u.copy(order="F"), v.copy(order="F")
+        du, dv = np.zeros_like(u), np.zeros_like(v)
+        alpha, L2_sav = 1.0, L2_large
+        saved = None
     criterion = np.inf
     counter = 0
     s = None
     while criterion >= resid_target and counter < maxiter:
         counter += 1
-        usav, vsav = u.copy(order="F"), v.copy(order="F")
-        s = assemble(p, u, v, geo)
-        A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
-        L2, L2r = residual_L2(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
-        hist["L2"].append(L2)
+        done = False
+        while not done:
+            if accelerate and counter >= 2:
+                u = np.asfortranarray(u_old + alpha * du)
+                v = np.asfortranarray(v_old + alpha * dv)
+                halo(g, u); halo(g, v)
+            usav, vsav = u.copy(order="F"), v.copy(order="F")
+            s = assemble(p, u, v, geo)
+            A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+            L2, L2r = residual_L2(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
+            hist["L2"].append(L2)
+            if accelerate and counter >= 2:
+                if L2 < thresh * L2_sav and alpha + gamma <= alpha_max:
+                    saved = (u.copy(order="F"), v.copy(order="F"), s)
+                    alpha = alpha + gamma
+                    L2_sav = L2
+                elif L2 < L2_sav:
+                    u_old, v_old = u.copy(order="F"), v.copy(order="F")
+                    alpha, L2_sav, done = 1.0, L2_large, True
+                else:
+                    u, v, s = saved[0].copy(order="F"), saved[1].copy(order="F"), saved[2]
+                    A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+                    u_old, v_old = u.copy(order="F"), v.copy(order="F")
+                    alpha, L2_sav, done = 1.0, L2_large, True
+                hist["alpha"].append(alpha)
+            else:
+                done = True
         st, err, nit = pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], u, v, algo=algo, precond=precond,
                                  ncheck=ncheck, tolerance=linear_tol, maxiters=linear_maxiters)
         if st:
@@ -364,6 +394,8 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
         rv = L.orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
                                         dp(usav), dp(vsav), dp(u), dp(v))
         hist["resid_velo"].append(rv)
+        if accelerate:
+            du, dv = u - u_old, v - v_old
         criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
     return dict(uvel=u, vvel=v, efvs=s["efvs"] if s else None, iterations=counter,
                 converged=criterion < resid_target, **hist)

@@ -21,8 +21,9 @@ def _problems():
         "dome": synthetic.dome(),
         "stream": synthetic.ice_stream(40, nz=11, seed=3),
         "slab_periodic": synthetic.slab(n_cells=8, nz=9),
-        "marine": synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3),
-        "marine_local_front": _local_front(synthetic.marine_sheet(36, nz=5, dx=2500.0, seed=4)),
+        "marine": synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3, h_max=1000.0, h_shelf=200.0),
+        "marine_local_front": _local_front(synthetic.marine_sheet(36, nz=5, dx=2500.0, seed=4, h_max=1000.0,
+                                                                  h_shelf=200.0)),
     }
 
 
@@ -111,3 +112,21 @@ def test_dome_picard_matches_oracle(require_gpu):
     # a relative residual landing within rounding of linear_tol can move the
     # exit by one convergence check (5 iterations)
     assert all(abs(a - b) <= 5 for a, b in zip(r.linear_iters, ref["linear_iters"]))
+
+
+@pytest.mark.parametrize("name", ["dome", "marine"])
+def test_accelerated_picard_matches_oracle(require_gpu, name):
+    """Picard acceleration (evaluate_accelerated_picard_3d): the device driver
+    takes the same accept / extend / restore decisions as the restatement
+    (same alpha sequence and iteration count) and reaches the same velocity."""
+    p = PROBLEMS[name]
+    kw = dict(linear_tol=1e-10, resid_target=1e-4, maxiter=60)
+    ref = po.picard(p, algo="chrongear", precond=2, linear_maxiters=2000, accelerate=True, **kw)
+    s = BPSolver(p)
+    r = s.picard(algo=3, precond=2, linear_maxiters=2000, accelerate=True, **kw)
+    assert r.converged and ref["converged"]
+    assert r.iterations == ref["iterations"]
+    assert r.alpha == ref["alpha"]
+    num = np.linalg.norm(np.stack([r.uvel - ref["uvel"], r.vvel - ref["vvel"]]))
+    rel = num / np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]]))
+    assert rel < 1e-7, rel

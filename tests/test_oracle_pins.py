@@ -544,3 +544,22 @@ def test_grounded_fraction_rules():
     # (the no-ice halo rule also clears the last owned row and column)
     own = (slice(p.grid.ilo - 1, p.grid.ihi - 1), slice(p.grid.jlo - 1, p.grid.jhi - 1))
     assert np.array_equal(b[own] == 0.0, fg[own] == 0.0)
+
+
+def test_accelerated_picard_same_fixed_point():
+    """Picard acceleration (evaluate_accelerated_picard_3d,
+    glissade_velo_higher.F90:10052-10184) changes the path, not the fixed
+    point: driven to a tight tolerance it reaches the plain Picard solution
+    in fewer iterations; every accepted extension has alpha in [1, 3] on the
+    0.4 ladder, and a rejected one resets alpha to 1."""
+    p = synthetic.dome(n_cells=15, nz=6, dx=2000.0, radius=14e3, h0=1500.0)
+    kw = dict(algo="standard", precond=1, linear_tol=1e-12, linear_maxiters=2000, resid_target=1e-8, maxiter=200)
+    r0 = po.picard(p, **kw)
+    r1 = po.picard(p, accelerate=True, **kw)
+    assert r0["converged"] and r1["converged"]
+    assert r1["iterations"] < r0["iterations"]
+    rel = np.linalg.norm(r1["uvel"] - r0["uvel"]) / np.linalg.norm(r0["uvel"])
+    assert rel < 1e-6, rel
+    ladder = [1.0 + 0.4 * m for m in range(6)]
+    assert all(any(abs(a - l) < 1e-12 for l in ladder) for a in r1["alpha"])
+    assert max(r1["alpha"]) <= 3.0 and 1.0 in r1["alpha"]
