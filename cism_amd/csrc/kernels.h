@@ -38,11 +38,16 @@ __global__ void k_precond(Dims d, int kind, const double* scal, const TV* ru, co
                           double* partials);
 template <typename TV>
 __global__ void k_vsum(Dims d, const TV* ru, const TV* rv, TV* su, TV* sv);
+
 template <typename TV>
-__global__ void k_line_solve(Dims d, int dir, int back_to_one, const TV* Ad, const TV* As, const TV* de,
-                             const TV* om, const TV* b, TV* x, TV* gam);
+__global__ void k_line_scan(Dims d, int dir, const TV* Ad, const TV* As, const TV* de, const TV* om, const TV* b,
+                            TV* x, TV* gam);
 template <typename TV>
-__global__ void k_halo2d(Dims d, TV* a, TV* b);
+__global__ void k_halo2d_ew(Dims d, TV* a, TV* b);
+template <typename TV>
+__global__ void k_halo2d_ns(Dims d, TV* a, TV* b);
+template <typename TV>
+__global__ void k_halo2d_zero(Dims d, TV* a, TV* b);
 template <typename TV>
 __global__ void k_axpy_ops(Dims d, int op, const double* scal, const TV* s1u, const TV* s1v, TV* tu, TV* tv,
                            const TV* s2u, const TV* s2v);

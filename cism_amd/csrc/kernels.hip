@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_spmv_q(Dims d, const TA* __restrict__ A
     const int lane = threadIdx.x % WAVE;
     const int io = bx * WAVE + lane;
     const int jo = by * 4 + threadIdx.x / WAVE;
-    dd acc = {0.0, 0.0};
+    double acc = 0.0;     // per-column sum in FP64; the cross-column reduction is compensated
     if (io < d.nxo && jo < d.nyo) {
         const int i = d.ilo + io, j = d.jlo + jo;
         const long long pl = d.plane;
@@ -164,14 +164,12 @@ __global__ __launch_bounds__(256) void k_spmv_q(Dims d, const TA* __restrict__ A
                 const float ru = bu[c0] - su, rv = bv[c0] - sv;
                 yu[c0] = ru;
                 yv[c0] = rv;
-                acc = dd_add_d(acc, (double)ru * (double)ru);
-                acc = dd_add_d(acc, (double)rv * (double)rv);
+                acc += (double)ru * (double)ru + (double)rv * (double)rv;
             } else {
                 yu[c0] = su;
                 yv[c0] = sv;
                 if (MODE == 1) {
-                    acc = dd_add_d(acc, (double)su * (double)wu[1][4]);
-                    acc = dd_add_d(acc, (double)sv * (double)wv[1][4]);
+                    acc += (double)su * (double)wu[1][4] + (double)sv * (double)wv[1][4];
                 }
             }
 #pragma unroll
@@ -182,7 +180,7 @@ __global__ __launch_bounds__(256) void k_spmv_q(Dims d, const TA* __restrict__ A
         }
     }
     if (MODE != 0) {
-        dd v[1] = {acc};
+        dd v[1] = {{acc, 0.0}};
         block_reduce_dd<1, 256>(v);
         if (threadIdx.x == 0) {
             partials[2 * blockIdx.x + 0] = v[0].hi;
@@ -481,71 +479,107 @@ __global__ void k_vsum(Dims d, const TV* __restrict__ ru, const TV* __restrict__
     sv[t] = b;
 }
 
-// Line Thomas solves of the vertically summed system over the owned block
-// (tridiag_solver_local_2d, pcg.F90:4917-5100).  One thread per line.
-// dir 0: u lines along i for each owned row j; dir 1: v lines along j.
-// Coefficient arrays are on the 2-D vertex grid (nvx*nvy, i fastest);
-// Ad holds 1/diag, de holds 1/denom, om the omega factors.  back_to_one:
-// back-substitution continues to index 1 (precond 3); otherwise it stops
-// at the owned start and the rest of the line is zero (precond 4).
+// ---------------------------------------------------------------------------
+// Line solves as parallel scans.  With the factors of k_tri_factor the
+// forward sweep is the first-order recurrence g_s = (b_s - As_s g_{s-1}) de_s
+// (g_lo = b_lo Ad_lo; g = 0 outside the owned block, where de = 0) and the
+// back substitution x_s = g_s - om_s x_{s+1} (x = 0 past the line end, where
+// g = om = 0).  Each step is an affine map y -> P + Q y; one wavefront per
+// line composes the maps of 64 consecutive positions with a shuffle scan and
+// carries the last value into the next 64, so a line of n positions takes
+// n/64 short scans instead of n dependent steps.
+// ---------------------------------------------------------------------------
 template <typename TV>
-__global__ void k_line_solve(Dims d, int dir, int back_to_one, const TV* __restrict__ Ad,
-                             const TV* __restrict__ As, const TV* __restrict__ de, const TV* __restrict__ om,
-                             const TV* __restrict__ b, TV* __restrict__ x, TV* __restrict__ gam) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void affine_scan(TV& P, TV& Q, int lane) {
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const TV Pp = __shfl_up(P, off, WAVE), Qp = __shfl_up(Q, off, WAVE);
+        if (lane >= off) {
+            P = P + Q * Pp;
+            Q = Q * Qp;
+        }
+    }
+}
+
+template <typename TV>
+__global__ __launch_bounds__(256) void k_line_scan(Dims d, int dir, const TV* __restrict__ Ad,
+                                                   const TV* __restrict__ As, const TV* __restrict__ de,
+                                                   const TV* __restrict__ om, const TV* __restrict__ b,
+                                                   TV* __restrict__ x, TV* __restrict__ gam) {
+    const int lane = threadIdx.x % WAVE;
+    const int line = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const int nv = d.nvx;
-    if (dir == 0) {
-        const int j = d.jlo + t;
-        if (j > d.jhi) return;
-        auto id = [&](int i) { return (long long)(j - 1) * nv + (i - 1); };
-        for (int i = 1; i <= d.nvx; ++i) { x[id(i)] = TV(0); gam[id(i)] = TV(0); }
-        gam[id(d.ilo)] = b[id(d.ilo)] * Ad[id(d.ilo)];
-        for (int i = d.ilo + 1; i <= d.ihi; ++i) gam[id(i)] = (b[id(i)] - As[id(i)] * gam[id(i - 1)]) * de[id(i)];
-        x[id(d.ihi)] = gam[id(d.ihi)];
-        const int iend = back_to_one ? 1 : d.ilo;
-        for (int i = d.ihi - 1; i >= iend; --i) x[id(i)] = gam[id(i)] - om[id(i)] * x[id(i + 1)];
-    } else {
-        const int i = d.ilo + t;
-        if (i > d.ihi) return;
-        auto id = [&](int j) { return (long long)(j - 1) * nv + (i - 1); };
-        for (int j = 1; j <= d.nvy; ++j) { x[id(j)] = TV(0); gam[id(j)] = TV(0); }
-        gam[id(d.jlo)] = b[id(d.jlo)] * Ad[id(d.jlo)];
-        for (int j = d.jlo + 1; j <= d.jhi; ++j) gam[id(j)] = (b[id(j)] - As[id(j)] * gam[id(j - 1)]) * de[id(j)];
-        x[id(d.jhi)] = gam[id(d.jhi)];
-        const int jend = back_to_one ? 1 : d.jlo;
-        for (int j = d.jhi - 1; j >= jend; --j) x[id(j)] = gam[id(j)] - om[id(j)] * x[id(j + 1)];
+    int fixed, n, lo;
+    if (dir == 0) { fixed = d.jlo + line; if (fixed > d.jhi) return; n = d.nvx; lo = d.ilo; }
+    else { fixed = d.ilo + line; if (fixed > d.ihi) return; n = d.nvy; lo = d.jlo; }
+    auto id = [&](int s) -> long long {
+        return dir == 0 ? (long long)(fixed - 1) * nv + (s - 1) : (long long)(s - 1) * nv + (fixed - 1);
+    };
+    // forward sweep over s = 1..n
+    TV carry = TV(0);
+    for (int s0 = 1; s0 <= n; s0 += WAVE) {
+        const int s = s0 + lane;
+        TV P = TV(0), Q = TV(1);
+        if (s <= n) {
+            const long long c = id(s);
+            if (s == lo) { P = b[c] * Ad[c]; Q = TV(0); }
+            else { const TV e = de[c]; P = b[c] * e; Q = -(As[c] * e); }
+        }
+        affine_scan<TV>(P, Q, lane);
+        const TV y = P + Q * carry;
+        if (s <= n) gam[id(s)] = y;
+        carry = __shfl(y, WAVE - 1, WAVE);
+    }
+    // back substitution over s = n..1
+    carry = TV(0);
+    for (int s0 = n; s0 >= 1; s0 -= WAVE) {
+        const int s = s0 - lane;
+        TV P = TV(0), Q = TV(1);
+        if (s >= 1) {
+            const long long c = id(s);
+            P = gam[c];
+            Q = -om[c];
+        }
+        affine_scan<TV>(P, Q, lane);
+        const TV y = P + Q * carry;
+        if (s >= 1) x[id(s)] = y;
+        carry = __shfl(y, WAVE - 1, WAVE);
     }
 }
 
 // 2-D halo of a vertex-grid field (nvx*nvy, i fastest), same rules as 3-D.
+// Periodic: east/west columns first, then north/south rows (which carry the
+// corners), as staggered_parallel_halo does.
 template <typename TV>
-__global__ void k_halo2d(Dims d, TV* __restrict__ a, TV* __restrict__ b) {
-    // single thread block handles the sequential ordering (small)
+__global__ void k_halo2d_ew(Dims d, TV* __restrict__ a, TV* __restrict__ b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nv = d.nvx;
+    const int j = d.jlo + t / 2;
+    if (j > d.jhi) return;
+    TV* v = (t % 2) ? b : a;
+    auto id = [&](int i) { return (long long)(j - 1) * nv + (i - 1); };
+    const TV w = v[id(3)], e0 = v[id(nv - 2)], e1 = v[id(nv - 1)];
+    v[id(nv)] = w; v[id(1)] = e0; v[id(2)] = e1;
+}
+template <typename TV>
+__global__ void k_halo2d_ns(Dims d, TV* __restrict__ a, TV* __restrict__ b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nv = d.nvx, nw = d.nvy;
-    auto id = [&](int i, int j) { return (long long)(j - 1) * nv + (i - 1); };
-    const bool periodic = (d.bc == 0 || d.bc == 2);
-    for (int s = 0; s < 2; ++s) {
-        TV* v = s ? b : a;
-        if (periodic) {
-            for (int j = d.jlo + threadIdx.x; j <= d.jhi; j += blockDim.x) {
-                const TV w = v[id(3, j)], e0 = v[id(nv - 2, j)], e1 = v[id(nv - 1, j)];
-                v[id(nv, j)] = w; v[id(1, j)] = e0; v[id(2, j)] = e1;
-            }
-            __syncthreads();
-            for (int i = 1 + threadIdx.x; i <= nv; i += blockDim.x) {
-                const TV so = v[id(i, 3)], n0 = v[id(i, nw - 2)], n1 = v[id(i, nw - 1)];
-                v[id(i, nw)] = so; v[id(i, 1)] = n0; v[id(i, 2)] = n1;
-            }
-            __syncthreads();
-        } else if (d.bc == 1 || d.bc == 3) {
-            const int lw = d.bc == 1 ? 1 : 2, uw = d.bc == 1 ? 1 : 2;
-            for (long long t = threadIdx.x; t < (long long)nv * nw; t += blockDim.x) {
-                const int i = 1 + (int)(t % nv), j = 1 + (int)(t / nv);
-                if (i <= lw || i >= nv - uw + 1 || j <= lw || j >= nw - uw + 1) v[t] = TV(0);
-            }
-            __syncthreads();
-        }
-    }
+    const int i = 1 + t / 2;
+    if (i > nv) return;
+    TV* v = (t % 2) ? b : a;
+    auto id = [&](int jj) { return (long long)(jj - 1) * nv + (i - 1); };
+    const TV so = v[id(3)], n0 = v[id(nw - 2)], n1 = v[id(nw - 1)];
+    v[id(nw)] = so; v[id(1)] = n0; v[id(2)] = n1;
+}
+template <typename TV>
+__global__ void k_halo2d_zero(Dims d, TV* __restrict__ a, TV* __restrict__ b) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int nv = d.nvx, nw = d.nvy;
+    if (t >= (long long)nv * nw) return;
+    const int lw = d.bc == 1 ? 1 : 2, uw = d.bc == 1 ? 1 : 2;
+    const int i = 1 + (int)(t % nv), j = 1 + (int)(t / nv);
+    if (i <= lw || i >= nv - uw + 1 || j <= lw || j >= nw - uw + 1) { a[t] = TV(0); b[t] = TV(0); }
 }
 
 // Whole-array updates used by the standard PCG and the first C-G pass.

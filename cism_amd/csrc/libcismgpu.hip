@@ -137,6 +137,18 @@ void halo(cismgpu_system* S, TV* a, TV* b, int nlev) {
     }
 }
 
+// staggered halo of two 2-D vertex fields
+template <typename TV>
+void halo2d(cismgpu_system* S, TV* a, TV* b) {
+    const Dims& d = S->d;
+    if (d.bc == 0 || d.bc == 2) {
+        k_halo2d_ew<TV><<<nblocks(2LL * (d.jhi - d.jlo + 1), 256), 256, 0, S->s>>>(d, a, b);
+        k_halo2d_ns<TV><<<nblocks(2LL * d.nvx, 256), 256, 0, S->s>>>(d, a, b);
+    } else if (d.bc == 1 || d.bc == 3) {
+        k_halo2d_zero<TV><<<nblocks((long long)d.nvx * d.nvy, 256), 256, 0, S->s>>>(d, a, b);
+    }
+}
+
 template <typename TA, typename TV, int MODE>
 void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV* yv, const TV* bu, const TV* bv,
           double* part) {
@@ -303,9 +315,9 @@ int setup_tri(cismgpu_system* S, int mode) {
     k_tri_vsum<<<nblocks(n2, 256), 256, 0, S->s>>>(d, mode, S->stage, S->stage + nraw, S->active, su_s, su_d, su_p,
                                                   sv_s, sv_d, sv_p);
     if (mode == 4) {
-        k_halo2d<double><<<1, 256, 0, S->s>>>(d, su_s, su_d);
-        k_halo2d<double><<<1, 256, 0, S->s>>>(d, su_p, sv_s);
-        k_halo2d<double><<<1, 256, 0, S->s>>>(d, sv_d, sv_p);
+        halo2d<double>(S, su_s, su_d);
+        halo2d<double>(S, su_p, sv_s);
+        halo2d<double>(S, sv_d, sv_p);
     }
     // u lines: As_u = sub (copy), factor into Adu, deu, omu
     CG_CHECK(hipMemcpyAsync(T.Asu, su_s, n2 * sizeof(double), hipMemcpyDeviceToDevice, S->s));
@@ -342,9 +354,9 @@ void tri_horizontal(cismgpu_system* S, const TV* ru, const TV* rv, TV** zsu_out,
         omv = T.omv32; rsu = T.rsu32; rsv = T.rsv32; zsu = T.zsu32; zsv = T.zsv32; gam = T.gam32;
     }
     k_vsum<TV><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, rsu, rsv);
-    k_line_solve<TV><<<nblocks(d.nyo, 64), 64, 0, S->s>>>(d, 0, 1, Adu, Asu, deu, omu, rsu, zsu, gam);
-    k_line_solve<TV><<<nblocks(d.nxo, 64), 64, 0, S->s>>>(d, 1, 1, Adv, Asv, dev_, omv, rsv, zsv, gam);
-    k_halo2d<TV><<<1, 256, 0, S->s>>>(d, zsu, zsv);
+    k_line_scan<TV><<<nblocks(d.nyo, 4), 256, 0, S->s>>>(d, 0, Adu, Asu, deu, omu, rsu, zsu, gam);
+    k_line_scan<TV><<<nblocks(d.nxo, 4), 256, 0, S->s>>>(d, 1, Adv, Asv, dev_, omv, rsv, zsv, gam);
+    halo2d<TV>(S, zsu, zsv);
     *zsu_out = zsu;
     *zsv_out = zsv;
 }
@@ -858,13 +870,14 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     hipEvent_t e0, e1;
     CG_CHECK(hipEventCreate(&e0));
     CG_CHECK(hipEventCreate(&e1));
+    // the form the solvers launch: y = A x with the fused (y, x) partial sums
     auto launch = [&]() {
         if (kind == 0)
-            k_spmv<double, double, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, nullptr);
+            k_spmv<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
         else if (kind == 1)
-            k_spmv_q<float, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+            k_spmv_q<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         else
-            k_spmv_q<__hip_bfloat16, 0><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, nullptr);
+            k_spmv_q<__hip_bfloat16, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
     };
     launch();   // warm-up
     CG_CHECK(hipEventRecord(e0, S->s));
