@@ -35,6 +35,7 @@ EXPORTS = (
     "cismgpu_ensemble_create", "cismgpu_ensemble_destroy", "cismgpu_ensemble_autotune", "cismgpu_ensemble_solve",
     "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info", "cismgpu_bench_stream",
     "cismgpu_system_save_assembly", "cismgpu_system_restore_assembly",
+    "cismgpu_system_work_vectors", "cismgpu_system_vector_op", "cismgpu_system_residual_velocity",
 )
 
 

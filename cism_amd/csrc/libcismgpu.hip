@@ -70,6 +70,8 @@ struct cismgpu_system {
     size_t asm_bytes = 0;
     double* efvs = nullptr;
     double* Atmp = nullptr;      // second matrix buffer for the symmetry fix
+    double* work = nullptr;      // Picard work vectors: slot q = (u, v) at work + 2*q*vlen
+    int nwork = 0;
     double* saved = nullptr;     // saved assembly (A, b, preconditioner, raw diagonal blocks)
     size_t saved_n = 0;          // doubles in `saved`
     bool have_saved = false;
@@ -673,7 +675,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -1255,6 +1257,134 @@ extern "C" int32_t cismgpu_system_halo_x(cismgpu_system* S) {
     if (!S) return CISMGPU_ERR_ARG;
     halo<double>(S, S->V(0), S->V(1), S->d.nz);
     CG_CHECK(hipStreamSynchronize(S->s));
+    return CISMGPU_OK;
+}
+
+// ---------------------------------------------------------------- device-resident Picard vectors
+// Elementwise operations on velocity pairs kept on the device: slot -1 is
+// the resident iterate x, slots 0..n-1 are work pairs.  Written without
+// contraction so the results equal the host formulas bit for bit.
+__global__ void k_vec_op(long long n, int op, double alpha, double* dst, const double* a, const double* b) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v;
+    if (op == 0) v = a[t];
+    else if (op == 1) v = a[t] + alpha * b[t];
+    else if (op == 2) v = a[t] - b[t];
+    else v = 0.0;
+    dst[t] = v;
+}
+
+// compute_residual_velocity_3d (glissade_velo_higher.F90:10314-10446):
+// max (whichresid 0, 1, 3, 4) or mean (2) over owned vertices of
+// |(|u_old| - |u|) / |u]| where |u| != 0; whichresid 1 and 2 skip the bed
+// level.  Per-workgroup max / compensated sum and count; the host combines.
+__global__ __launch_bounds__(256) void k_resid_velo(Dims d, int skip_bed, const double* __restrict__ u,
+                                                    const double* __restrict__ v, const double* __restrict__ u0,
+                                                    const double* __restrict__ v0, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    double mx = 0.0;
+    dd sum = {0.0, 0.0};
+    double cnt = 0.0;
+    if (t < (long long)d.nxo * d.nyo) {
+        const int i = d.ilo + (int)(t % d.nxo), j = d.jlo + (int)(t / d.nxo);
+        const int kn = skip_bed ? d.nz - 1 : d.nz;
+        for (int k = 0; k < kn; ++k) {
+            const long long c = vidx(d, k, i, j);
+            const double speed = sqrt(u[c] * u[c] + v[c] * v[c]);
+            if (speed != 0.0) {
+                const double old = sqrt(u0[c] * u0[c] + v0[c] * v0[c]);
+                const double df = fabs((old - speed) / speed);
+                mx = fmax(mx, df);
+                sum = dd_add_d(sum, df);
+                cnt += 1.0;
+            }
+        }
+    }
+    __shared__ double smx[256 / WAVE], scn[256 / WAVE];
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, off, WAVE));
+        cnt += __shfl_down(cnt, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) { smx[threadIdx.x / WAVE] = mx; scn[threadIdx.x / WAVE] = cnt; }
+    dd vv[1] = {sum};
+    block_reduce_dd<1, 256>(vv);
+    if (threadIdx.x == 0) {
+        double m = 0.0, c = 0.0;
+        for (int w = 0; w < 256 / WAVE; ++w) { m = fmax(m, smx[w]); c += scn[w]; }
+        part[4 * blockIdx.x + 0] = m;
+        part[4 * blockIdx.x + 1] = vv[0].hi;
+        part[4 * blockIdx.x + 2] = vv[0].lo;
+        part[4 * blockIdx.x + 3] = c;
+    }
+}
+
+static double* vec_slot(cismgpu_system* S, int q) {
+    if (q == -1) return S->V(0);
+    if (q < 0 || q >= S->nwork) return nullptr;
+    return S->work + 2 * (size_t)q * S->d.vlen;
+}
+
+extern "C" int32_t cismgpu_system_work_vectors(cismgpu_system* S, int32_t nslots) {
+    if (!S || nslots < 0 || nslots > 16) return CISMGPU_ERR_ARG;
+    if (nslots <= S->nwork) return CISMGPU_OK;
+    double* w = nullptr;
+    CG_CHECK(hipMalloc(&w, 2 * (size_t)nslots * S->d.vlen * sizeof(double)));
+    CG_CHECK(hipMemsetAsync(w, 0, 2 * (size_t)nslots * S->d.vlen * sizeof(double), S->s));
+    if (S->work) {
+        CG_CHECK(hipMemcpyAsync(w, S->work, 2 * (size_t)S->nwork * S->d.vlen * sizeof(double), hipMemcpyDeviceToDevice, S->s));
+        CG_CHECK(hipStreamSynchronize(S->s));
+        hipFree(S->work);
+    }
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->work = w;
+    S->nwork = nslots;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_vector_op(cismgpu_system* S, int32_t op, int32_t dst, int32_t a, int32_t b,
+                                            double alpha) {
+    if (!S || op < 0 || op > 3) return CISMGPU_ERR_ARG;
+    double* D = vec_slot(S, dst);
+    const double* A = op == 3 ? D : vec_slot(S, a);
+    const double* B = (op == 1 || op == 2) ? vec_slot(S, b) : A;
+    if (!D || !A || !B) { cismgpu_set_error("vector_op: bad slot"); return CISMGPU_ERR_ARG; }
+    const long long n = 2 * S->d.vlen;   // both components are contiguous in every slot
+    k_vec_op<<<nblocks(n, 256), 256, 0, S->s>>>(n, op, alpha, D, A, B);
+    CG_CHECK(hipGetLastError());
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_residual_velocity(cismgpu_system* S, int32_t whichresid, int32_t old_slot,
+                                                    double* resid) {
+    if (!S || !resid) return CISMGPU_ERR_ARG;
+    const double* O = vec_slot(S, old_slot);
+    if (!O || old_slot == -1) { cismgpu_set_error("residual_velocity: bad slot"); return CISMGPU_ERR_ARG; }
+    const Dims& d = S->d;
+    const int nb = (int)nblocks((long long)d.nxo * d.nyo, 256);
+    double* part = nullptr;
+    CG_CHECK(hipMalloc(&part, 4 * (size_t)nb * sizeof(double)));
+    const int skip_bed = (whichresid == 1 || whichresid == 2) ? 1 : 0;
+    k_resid_velo<<<nb, 256, 0, S->s>>>(d, skip_bed, S->V(0), S->V(1), O, O + d.vlen, part);
+    std::vector<double> h(4 * (size_t)nb);
+    CG_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    hipFree(part);
+    double mx = 0.0, c = 0.0;
+    dd sum = {0.0, 0.0};
+    for (int q = 0; q < nb; ++q) {
+        mx = std::max(mx, h[4 * q]);
+        const double a = h[4 * q + 1], b = h[4 * q + 2];
+        // host double-double add in block order
+        double s1 = sum.hi + a, bb = s1 - sum.hi, e = (sum.hi - (s1 - bb)) + (a - bb);
+        e += sum.lo + b;
+        sum.hi = s1 + e;
+        sum.lo = e - (sum.hi - s1);
+        c += h[4 * q + 3];
+    }
+    *resid = (whichresid == 2) ? (c > 0.0 ? (sum.hi + sum.lo) / c : 0.0) : mx;
     return CISMGPU_OK;
 }
 

@@ -53,6 +53,15 @@ def _declare(L):
     L.cismgpu_system_residual.restype = ctypes.c_int32
     L.cismgpu_system_halo_x.argtypes = [ctypes.c_void_p]
     L.cismgpu_system_halo_x.restype = ctypes.c_int32
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    L.cismgpu_system_save_assembly.argtypes = [vp]
+    L.cismgpu_system_restore_assembly.argtypes = [vp]
+    L.cismgpu_system_work_vectors.argtypes = [vp, i32]
+    L.cismgpu_system_vector_op.argtypes = [vp, i32, i32, i32, i32, ctypes.c_double]
+    L.cismgpu_system_residual_velocity.argtypes = [vp, i32, i32, D]
+    for n in ("cismgpu_system_save_assembly", "cismgpu_system_restore_assembly", "cismgpu_system_work_vectors",
+              "cismgpu_system_vector_op", "cismgpu_system_residual_velocity"):
+        getattr(L, n).restype = ctypes.c_int32
     L._velo_declared = True
 
 
@@ -184,43 +193,49 @@ class BPSolver:
     def restore_assembly(self):
         _lib.check(self._L.cismgpu_system_restore_assembly(self.sys.handle), "restore_assembly")
 
-    def _set_x_halo(self, u, v):
-        self.sys.set_x(u, v)
-        _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+    # device-resident Picard vectors: slot -1 is the iterate x
+    X, USAV, UOLD, DU, UBEST = -1, 0, 1, 2, 3
+
+    def _vop(self, op, dst, a=-1, b=-1, alpha=0.0):
+        _lib.check(self._L.cismgpu_system_vector_op(self.sys.handle, op, dst, a, b, float(alpha)), "vector_op")
 
     def picard(self, *, maxiter=100, resid_target=1e-4, whichresid=0, algo=ALGO_CHRONGEAR,
                precond=HO_PRECOND_SIA, ncheck=5, linear_tol=1e-8, linear_maxiters=200, precision=FP64,
                inner_tol=1e-3, max_outer=20, accelerate=False, verbose=False) -> PicardResult:
-        """Outer Picard iteration on the viscosity (glissade_velo_higher.F90:2163-3663).
-        accelerate=False: which_ho_nonlinear = 0; True: Picard acceleration
-        (Dukowicz 2019; :2186-2263, :3108-3176, evaluate_accelerated_picard_3d
-        :10052-10184): from the second iteration the velocity is extended along
-        the last Picard step, u = u_old + alpha du, and alpha grows by 0.4 (to
-        at most 3) while each extension cuts the residual |A(u) u - b| below
-        0.85 of the previous one; a rejected extension restores the best
-        velocity and its assembled system.  Convergence: resid_velo <
-        resid_target for which_ho_resid 0/1/2, L2 < resid_target for 3,
-        L2 / |b| for 4 (:3652-3661)."""
+        """Outer Picard iteration on the viscosity (glissade_velo_higher.F90:2163-3663),
+        with every velocity field kept on the device.  accelerate=False:
+        which_ho_nonlinear = 0; True: Picard acceleration (Dukowicz 2019;
+        :2186-2263, :3108-3176, evaluate_accelerated_picard_3d :10052-10184):
+        from the second iteration the velocity is extended along the last
+        Picard step, u = u_old + alpha du, and alpha grows by 0.4 (to at most
+        3) while each extension cuts the residual |A(u) u - b| below 0.85 of
+        the previous one; a rejected extension restores the best velocity and
+        its assembled system.  Convergence: resid_velo < resid_target for
+        which_ho_resid 0/1/2, L2 < resid_target for 3, L2 / |b| for 4
+        (:3652-3661)."""
         g = self.grid
         u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
         v = np.asfortranarray(np.array(self.p.vvel, dtype=np.float64))
         self.zero_inactive(u, v)
         self.sys.set_x(u, v)
+        _lib.check(self._L.cismgpu_system_work_vectors(self.sys.handle, 4), "work_vectors")
+        X, USAV, UOLD, DU, UBEST = self.X, self.USAV, self.UOLD, self.DU, self.UBEST
         res = PicardResult(u, v, None, 0, False)
         if accelerate:
-            u_old, v_old = u.copy(order="F"), v.copy(order="F")
-            du, dv = g.zeros_vec(), g.zeros_vec()
+            self._vop(0, UOLD, X)
+            self._vop(3, DU)
             alpha, L2_sav = 1.0, L2_NORM_LARGE
-            u_best = v_best = None
         criterion = np.inf
         counter = 0
+        rvd = ctypes.c_double()
         while criterion >= resid_target and counter < maxiter:
             counter += 1
             done = False
             while not done:
                 if accelerate and counter >= 2:
-                    self._set_x_halo(np.asfortranarray(u_old + alpha * du), np.asfortranarray(v_old + alpha * dv))
-                usav, vsav = self.sys.get_x()
+                    self._vop(1, X, UOLD, DU, alpha)
+                    _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+                self._vop(0, USAV, X)
                 ai = self.assemble()
                 res.assemble_ms.append(ai["total_ms"])
                 if ai["nnz"] == 0:          # nothing to solve (:2937-3032)
@@ -232,17 +247,17 @@ class BPSolver:
                 res.L2_norm.append(L2)
                 if accelerate and counter >= 2:
                     if L2 < RESID_REDUCTION_THRESHOLD * L2_sav and alpha + GAMMA_ACCEL <= ALPHA_ACCEL_MAX:
-                        u_best, v_best = usav, vsav
+                        self._vop(0, UBEST, X)
                         self.save_assembly()
                         alpha = alpha + GAMMA_ACCEL
                         L2_sav = L2
                     elif L2 < L2_sav:
-                        u_old, v_old = usav, vsav
+                        self._vop(0, UOLD, X)
                         alpha, L2_sav, done = 1.0, L2_NORM_LARGE, True
                     else:
-                        self.sys.set_x(u_best, v_best)
+                        self._vop(0, X, UBEST)
                         self.restore_assembly()
-                        u_old, v_old = u_best.copy(order="F"), v_best.copy(order="F")
+                        self._vop(0, UOLD, UBEST)
                         alpha, L2_sav, done = 1.0, L2_NORM_LARGE, True
                     res.alpha.append(alpha)
                 else:
@@ -254,11 +269,12 @@ class BPSolver:
             res.linear_err.append(si["err"])
             res.solve_ms.append(si["solve_ms"])
             _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
-            uu, vv = self.sys.get_x()
-            rv = residual_velocity(g, whichresid, usav, vsav, uu, vv)
+            _lib.check(self._L.cismgpu_system_residual_velocity(self.sys.handle, whichresid, USAV, ctypes.byref(rvd)),
+                       "residual_velocity")
+            rv = rvd.value
             res.resid_velo.append(rv)
             if accelerate:
-                du, dv = uu - u_old, vv - v_old
+                self._vop(2, DU, X, UOLD)
             criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
             if verbose:
                 print(f"picard {counter}: L2={L2:.4e} resid_velo={rv:.4e} linear iters={si['niters']} "
@@ -268,18 +284,3 @@ class BPSolver:
         res.uvel, res.vvel = self.sys.get_x()
         res.efvs = self.efvs()
         return res
-
-
-def residual_velocity(grid, whichresid, usav, vsav, uvel, vvel):
-    """compute_residual_velocity_3d (glissade_velo_higher.F90:10314-10446)."""
-    own = grid.owned_slice()
-    u, v, u0, v0 = uvel[own], vvel[own], usav[own], vsav[own]
-    if whichresid in (1, 2):
-        u, v, u0, v0 = u[:-1], v[:-1], u0[:-1], v0[:-1]
-    speed = np.sqrt(u * u + v * v)
-    old = np.sqrt(u0 * u0 + v0 * v0)
-    nz_ = speed != 0.0
-    if not nz_.any():
-        return 0.0
-    d = np.abs((old[nz_] - speed[nz_]) / speed[nz_])
-    return float(d.mean()) if whichresid == 2 else float(d.max())

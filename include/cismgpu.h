@@ -216,6 +216,19 @@ int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_i
  * iteration (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184). */
 int32_t cismgpu_system_save_assembly(cismgpu_system* sys);
 int32_t cismgpu_system_restore_assembly(cismgpu_system* sys);
+/* Device-resident Picard vectors (no reference counterpart: the reference
+ * keeps them in host arrays).  Slot -1 is the resident iterate x; slots
+ * 0..nslots-1 are work pairs (u, v) allocated by work_vectors (at most 16).
+ * vector_op: op 0 dst = a; 1 dst = a + alpha b; 2 dst = a - b; 3 dst = 0,
+ * over whole arrays (halo included), asynchronous on the system stream. */
+int32_t cismgpu_system_work_vectors(cismgpu_system* sys, int32_t nslots);
+int32_t cismgpu_system_vector_op(cismgpu_system* sys, int32_t op, int32_t dst, int32_t a, int32_t b,
+                                 double alpha);
+/* compute_residual_velocity_3d (glissade_velo_higher.F90:10314-10446) of the
+ * resident x against work slot old_slot: max (which_ho_resid 0, 1, 3, 4) or
+ * mean (2) relative speed change over owned vertices; 1 and 2 skip the bed. */
+int32_t cismgpu_system_residual_velocity(cismgpu_system* sys, int32_t whichresid, int32_t old_slot,
+                                         double* resid);
 /* efvs(nz-1, nx, ny) of the last assembly (mean over quadrature points) */
 int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
 /* resid = A x - b on owned active vertices, L2 norm and norm relative to

@@ -130,3 +130,42 @@ def test_accelerated_picard_matches_oracle(require_gpu, name):
     num = np.linalg.norm(np.stack([r.uvel - ref["uvel"], r.vvel - ref["vvel"]]))
     rel = num / np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]]))
     assert rel < 1e-7, rel
+
+
+@pytest.mark.parametrize("whichresid", [0, 1, 2])
+def test_device_residual_velocity_and_vector_ops(require_gpu, whichresid):
+    """compute_residual_velocity_3d on the device (work slot against the
+    resident iterate) equals the restatement; vector_op forms a + alpha b,
+    a - b, copies and zeros exactly as the host formulas do."""
+    import ctypes
+    p = PROBLEMS["stream"]
+    g = p.grid
+    s = BPSolver(p)
+    L = s._L
+    rng = np.random.default_rng(4)
+    u0, v0 = (np.asfortranarray(rng.standard_normal(g.vec_shape)) for _ in range(2))
+    u1, v1 = (np.asfortranarray(rng.standard_normal(g.vec_shape)) for _ in range(2))
+    u1[:, 5, 5] = 0.0
+    v1[:, 5, 5] = 0.0                     # zero speed is skipped
+    assert L.cismgpu_system_work_vectors(s.sys.handle, 3) == 0
+    s.sys.set_x(u0, v0)
+    s._vop(0, 0, -1)                      # slot 0 = x
+    s.sys.set_x(u1, v1)
+    out = ctypes.c_double()
+    assert L.cismgpu_system_residual_velocity(s.sys.handle, whichresid, 0, ctypes.byref(out)) == 0
+    ref = po.lib().orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                            po.dp(u0), po.dp(v0), po.dp(u1), po.dp(v1))
+    if whichresid == 2:
+        assert abs(out.value - ref) <= 1e-14 * abs(ref)
+    else:
+        assert out.value == ref
+    # x = slot0 + 0.7 (x - slot0), built through slots 1 and 2
+    s._vop(2, 1, -1, 0)                   # slot 1 = x - slot 0
+    s._vop(1, -1, 0, 1, 0.7)              # x = slot 0 + 0.7 slot 1
+    xu, xv = s.sys.get_x()
+    own = g.owned_slice()
+    assert np.array_equal(xu[own], (u0 + 0.7 * (u1 - u0))[own])
+    assert np.array_equal(xv[own], (v0 + 0.7 * (v1 - v0))[own])
+    s._vop(3, -1)
+    xu, xv = s.sys.get_x()
+    assert not xu.any() and not xv.any()
