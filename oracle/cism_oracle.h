@@ -56,6 +56,18 @@ void orc_matvec_3d(const orc_grid* g, const int32_t* active_vertex,
 void orc_tridiag_solver(int order, const double* sbdiag, const double* diag,
                         const double* spdiag, const double* rhs, double* soln);
 
+/* 2-D solvers (nz of the grid is ignored): A(nx-1,ny-1,9), m last; vectors (nx-1,ny-1) */
+int orc_pcg_solver_standard_2d(const orc_grid* g, const int32_t* active_vertex, const double* Auu,
+                               const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                               const double* bv, double* xu, double* xv, int32_t precond, int32_t ncheck,
+                               double tolerance, int32_t maxiters, double* err, int32_t* niters);
+int orc_pcg_solver_chrongear_2d(const orc_grid* g, const int32_t* active_vertex, const double* Auu,
+                                const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                const double* bv, double* xu, double* xv, int32_t precond, int32_t ncheck,
+                                double tolerance, int32_t maxiters, double* err, int32_t* niters);
+void orc_matvec_2d(const orc_grid* g, const int32_t* active_vertex, const double* Auu, const double* Auv,
+                   const double* Avu, const double* Avv, const double* xu, const double* xv, double* yu,
+                   double* yv);
 int orc_pcg_solver_standard_3d(const orc_grid* g, const int32_t* active_vertex,
                                const double* Auu, const double* Auv,
                                const double* Avu, const double* Avv,

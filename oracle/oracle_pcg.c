@@ -284,6 +284,7 @@ static void vsum_matrix(const orc_grid* g, const int32_t* active_vertex, const d
                     s = s + A[A4(m, k, i, j)] + A[A4(m + 9, k, i, j)] + A[A4(m + 18, k, i, j)];
                 vs[V2(i, j) + n2 * (m - 1)] = s;
             }
+    if (zero_offdiag_inactive < 0) return;   /* 2-D solves take the stencil as given */
     for (int j = 1; j <= NVY; ++j)
         for (int i = 1; i <= NVX; ++i)
             if (!active_vertex[V2(i, j)]) {
@@ -376,6 +377,7 @@ static int setup_tridiag_global_owned(int ilocal, int jlocal, double* Adiag, dou
 
 typedef struct pcstate {
     int precond;
+    int two_d;                    /* 2-D solve: tridiagonal preconditioners without the SIA column part */
     double *Adiagu, *Adiagv;      /* precond 1 */
     double *Muu, *Mvv;            /* precond 2, 3, 4 */
     tri2d t;                      /* precond 3 */
@@ -387,10 +389,11 @@ typedef struct pcstate {
 } pcstate;
 
 static int precond_setup(const orc_grid* g, const int32_t* active, const double* Auu, const double* Avv,
-                         int precond, pcstate* s)
+                         int precond, int two_d, pcstate* s)
 {
     memset(s, 0, sizeof(*s));
     s->precond = precond;
+    s->two_d = two_d;
     const size_t n = (size_t)NZ * NVX * NVY, n2 = (size_t)NVX * NVY;
     if (precond == PC_DIAG) {   /* setup_preconditioner_diag_3d (pcg.F90:3598-3643) */
         s->Adiagu = (double*)malloc(n * sizeof(double));
@@ -408,8 +411,8 @@ static int precond_setup(const orc_grid* g, const int32_t* active, const double*
             /* setup_preconditioner_tridiag_local_3d (pcg.F90:3737-3926) */
             double* vsu = (double*)malloc(9 * n2 * sizeof(double));
             double* vsv = (double*)malloc(9 * n2 * sizeof(double));
-            vsum_matrix(g, active, Auu, vsu, 1);
-            vsum_matrix(g, active, Avv, vsv, 1);
+            vsum_matrix(g, active, Auu, vsu, two_d ? -1 : 1);
+            vsum_matrix(g, active, Avv, vsv, two_d ? -1 : 1);
             tri2d_alloc(g, &s->t);
             int st = setup_tridiag_local_2d(g, vsu, vsv, &s->t);
             free(vsu); free(vsv);
@@ -418,8 +421,8 @@ static int precond_setup(const orc_grid* g, const int32_t* active, const double*
             /* setup_preconditioner_tridiag_global_3d (pcg.F90:3930-4195), one task */
             double* vsu = (double*)malloc(9 * n2 * sizeof(double));
             double* vsv = (double*)malloc(9 * n2 * sizeof(double));
-            vsum_matrix(g, active, Auu, vsu, 0);
-            vsum_matrix(g, active, Avv, vsv, 0);
+            vsum_matrix(g, active, Auu, vsu, two_d ? -1 : 0);
+            vsum_matrix(g, active, Avv, vsv, two_d ? -1 : 0);
             for (int m = 0; m < 9; ++m) { orc_halo3d(g, 1, vsu + n2 * m); orc_halo3d(g, 1, vsv + n2 * m); }
             const int il = g->ihi - g->ilo + 1, jl = g->jhi - g->jlo + 1;
             s->ilocal = il; s->jlocal = jl;
@@ -557,6 +560,11 @@ static void precond_apply(const orc_grid* g, const int32_t* active, pcstate* s,
         }
         orc_halo3d(g, 1, s->zu_vs);
         orc_halo3d(g, 1, s->zv_vs);
+        if (s->two_d) {   /* 2-D: z is the line solution (pcg.F90:1260-1320, :2352-2425) */
+            memcpy(zu, s->zu_vs, n2 * sizeof(double));
+            memcpy(zv, s->zv_vs, n2 * sizeof(double));
+            return;
+        }
         easy_sia_solver(g, active, s->Muu, ru, zu);
         easy_sia_solver(g, active, s->Mvv, rv, zv);
         for (int j = 1; j <= NVY; ++j)
@@ -568,9 +576,10 @@ static void precond_apply(const orc_grid* g, const int32_t* active, pcstate* s,
     }
 }
 
-static int check_args(const orc_grid* g, int precond, int ncheck)
+static int check_args(const orc_grid* g, int precond, int ncheck, int two_d)
 {
-    if (!g || g->nx < 6 || g->ny < 6 || g->nz < 2 || g->nz > 64) return ORC_ERR_ARG;
+    if (!g || g->nx < 6 || g->ny < 6 || g->nz < (two_d ? 1 : 2) || g->nz > 64) return ORC_ERR_ARG;
+    if (two_d && g->nz != 1) return ORC_ERR_ARG;
     if (g->ilo < 2 || g->ihi > g->nx - 2 || g->jlo < 2 || g->jhi > g->ny - 2) return ORC_ERR_ARG;
     if (g->ilo > g->ihi || g->jlo > g->jhi) return ORC_ERR_ARG;
     if (precond < 0 || precond > 4 || ncheck < 1) return ORC_ERR_ARG;
@@ -580,15 +589,15 @@ static int check_args(const orc_grid* g, int precond, int ncheck)
 /* ------------------------------------------------------------------ */
 /* pcg_solver_standard_3d (pcg.F90:827-1537)                           */
 /* ------------------------------------------------------------------ */
-int orc_pcg_solver_standard_3d(const orc_grid* g, const int32_t* active,
-                               const double* Auu, const double* Auv,
-                               const double* Avu, const double* Avv,
-                               const double* bu, const double* bv,
-                               double* xu, double* xv,
-                               int32_t precond, int32_t ncheck, double tolerance,
-                               int32_t maxiters, double* err, int32_t* niters)
+static int standard_core(const orc_grid* g, const int32_t* active,
+                         const double* Auu, const double* Auv,
+                         const double* Avu, const double* Avv,
+                         const double* bu, const double* bv,
+                         double* xu, double* xv,
+                         int32_t precond, int32_t ncheck, double tolerance,
+                         int32_t maxiters, double* err, int32_t* niters, int two_d)
 {
-    int st = check_args(g, precond, ncheck);
+    int st = check_args(g, precond, ncheck, two_d);
     if (st) return st;
     g_trace_len = 0;
     const size_t n = (size_t)NZ * NVX * NVY;
@@ -596,7 +605,7 @@ int orc_pcg_solver_standard_3d(const orc_grid* g, const int32_t* active,
     double *ru = buf, *rv = buf + n, *du = buf + 2 * n, *dv = buf + 3 * n, *qu = buf + 4 * n, *qv = buf + 5 * n;
     double *zu = buf + 6 * n, *zv = buf + 7 * n, *wu = buf + 8 * n, *wv = buf + 9 * n;
     pcstate pc;
-    st = precond_setup(g, active, Auu, Avv, precond, &pc);
+    st = precond_setup(g, active, Auu, Avv, precond, two_d, &pc);
     if (st) { precond_free(&pc); free(buf); return st; }
 
     orc_halo3d(g, NZ, xu);
@@ -649,15 +658,15 @@ int orc_pcg_solver_standard_3d(const orc_grid* g, const int32_t* active,
 /* ------------------------------------------------------------------ */
 /* pcg_solver_chrongear_3d (pcg.F90:2632-3594)                          */
 /* ------------------------------------------------------------------ */
-int orc_pcg_solver_chrongear_3d(const orc_grid* g, const int32_t* active,
-                                const double* Auu, const double* Auv,
-                                const double* Avu, const double* Avv,
-                                const double* bu, const double* bv,
-                                double* xu, double* xv,
-                                int32_t precond, int32_t ncheck, double tolerance,
-                                int32_t maxiters, double* err, int32_t* niters)
+static int chrongear_core(const orc_grid* g, const int32_t* active,
+                          const double* Auu, const double* Auv,
+                          const double* Avu, const double* Avv,
+                          const double* bu, const double* bv,
+                          double* xu, double* xv,
+                          int32_t precond, int32_t ncheck, double tolerance,
+                          int32_t maxiters, double* err, int32_t* niters, int two_d)
 {
-    int st = check_args(g, precond, ncheck);
+    int st = check_args(g, precond, ncheck, two_d);
     if (st) return st;
     g_trace_len = 0;
     const size_t n = (size_t)NZ * NVX * NVY;
@@ -667,7 +676,7 @@ int orc_pcg_solver_chrongear_3d(const orc_grid* g, const int32_t* active,
     double *wu = buf + 10 * n, *wv = buf + 11 * n, *w2u1 = buf + 12 * n, *w2v1 = buf + 13 * n;
     double *w2u2 = buf + 14 * n, *w2v2 = buf + 15 * n;
     pcstate pc;
-    st = precond_setup(g, active, Auu, Avv, precond, &pc);
+    st = precond_setup(g, active, Auu, Avv, precond, two_d, &pc);
     if (st) { precond_free(&pc); free(buf); return st; }
 
     *niters = maxiters;
@@ -740,4 +749,94 @@ done:
     precond_free(&pc);
     free(buf);
     return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* Public 3-D entry points                                              */
+/* ------------------------------------------------------------------ */
+int orc_pcg_solver_standard_3d(const orc_grid* g, const int32_t* active, const double* Auu, const double* Auv,
+                               const double* Avu, const double* Avv, const double* bu, const double* bv,
+                               double* xu, double* xv, int32_t precond, int32_t ncheck, double tolerance,
+                               int32_t maxiters, double* err, int32_t* niters)
+{
+    return standard_core(g, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tolerance, maxiters, err,
+                         niters, 0);
+}
+
+int orc_pcg_solver_chrongear_3d(const orc_grid* g, const int32_t* active, const double* Auu, const double* Auv,
+                                const double* Avu, const double* Avv, const double* bu, const double* bv,
+                                double* xu, double* xv, int32_t precond, int32_t ncheck, double tolerance,
+                                int32_t maxiters, double* err, int32_t* niters)
+{
+    return chrongear_core(g, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tolerance, maxiters, err,
+                          niters, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-D solvers: pcg_solver_standard_2d (pcg.F90:67-823),                */
+/* pcg_solver_chrongear_2d (:1541-2628), matvec_multiply_structured_2d  */
+/* (:5656-5773).  Their recurrences, check schedule and sums are those  */
+/* of the 3-D solvers on one level; the 9-point stencil                 */
+/* A(nx-1,ny-1,9) (m last, m = indxA_2d) is the kA = 0 plane of a       */
+/* one-level 27-point row, the sums run over (j, i) as the 3-D ones over */
+/* (j, i, k), and the preconditioners are diagonal or the line solves   */
+/* alone (no SIA column part; the tridiagonal setup takes the stencil as */
+/* given, :4247-4392).  which_ho_precond = 2 is replaced by the diagonal */
+/* preconditioner, as glide_setup.F90:1355-1367 does for 2-D solves.    */
+/* ------------------------------------------------------------------ */
+static double* stencil_2d_to_rows(const orc_grid* g, const double* A2)
+{
+    const size_t n2 = (size_t)NVX * NVY;
+    double* A3 = (double*)calloc(27 * n2, sizeof(double));
+    for (size_t p = 0; p < n2; ++p)
+        for (int m = 1; m <= 9; ++m) A3[27 * p + (m + 9) - 1] = A2[p + n2 * (m - 1)];
+    return A3;
+}
+
+static int solve_2d(int algo, const orc_grid* g2, const int32_t* active, const double* Auu, const double* Auv,
+                    const double* Avu, const double* Avv, const double* bu, const double* bv, double* xu, double* xv,
+                    int32_t precond, int32_t ncheck, double tolerance, int32_t maxiters, double* err,
+                    int32_t* niters)
+{
+    orc_grid g = *g2;
+    g.nz = 1;
+    const orc_grid* gp = &g;
+    double* A3[4] = {stencil_2d_to_rows(gp, Auu), stencil_2d_to_rows(gp, Auv), stencil_2d_to_rows(gp, Avu),
+                     stencil_2d_to_rows(gp, Avv)};
+    if (precond == PC_SIA) precond = PC_DIAG;
+    int st = algo == 2 ? standard_core(gp, active, A3[0], A3[1], A3[2], A3[3], bu, bv, xu, xv, precond, ncheck,
+                                       tolerance, maxiters, err, niters, 1)
+                       : chrongear_core(gp, active, A3[0], A3[1], A3[2], A3[3], bu, bv, xu, xv, precond, ncheck,
+                                        tolerance, maxiters, err, niters, 1);
+    for (int q = 0; q < 4; ++q) free(A3[q]);
+    return st;
+}
+
+int orc_pcg_solver_standard_2d(const orc_grid* g, const int32_t* active, const double* Auu, const double* Auv,
+                               const double* Avu, const double* Avv, const double* bu, const double* bv,
+                               double* xu, double* xv, int32_t precond, int32_t ncheck, double tolerance,
+                               int32_t maxiters, double* err, int32_t* niters)
+{
+    return solve_2d(2, g, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tolerance, maxiters, err,
+                    niters);
+}
+
+int orc_pcg_solver_chrongear_2d(const orc_grid* g, const int32_t* active, const double* Auu, const double* Auv,
+                                const double* Avu, const double* Avv, const double* bu, const double* bv,
+                                double* xu, double* xv, int32_t precond, int32_t ncheck, double tolerance,
+                                int32_t maxiters, double* err, int32_t* niters)
+{
+    return solve_2d(3, g, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tolerance, maxiters, err,
+                    niters);
+}
+
+void orc_matvec_2d(const orc_grid* g2, const int32_t* active, const double* Auu, const double* Auv,
+                   const double* Avu, const double* Avv, const double* xu, const double* xv, double* yu, double* yv)
+{
+    orc_grid g = *g2;
+    g.nz = 1;
+    double* A3[4] = {stencil_2d_to_rows(&g, Auu), stencil_2d_to_rows(&g, Auv), stencil_2d_to_rows(&g, Avu),
+                     stencil_2d_to_rows(&g, Avv)};
+    orc_matvec_3d(&g, active, A3[0], A3[1], A3[2], A3[3], xu, xv, yu, yv);
+    for (int q = 0; q < 4; ++q) free(A3[q]);
 }

@@ -47,6 +47,10 @@ def _declare(L):
     L.orc_pcg_solver_chrongear_3d_omp.argtypes = solver_args[:-2] + [ctypes.c_int32, _D, _I]
     L.orc_pcg_solver_chrongear_3d_omp.restype = ctypes.c_int
     L.orc_matvec_3d.argtypes = [G, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_matvec_2d.argtypes = [G, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    for name in ("orc_pcg_solver_standard_2d", "orc_pcg_solver_chrongear_2d"):
+        getattr(L, name).argtypes = solver_args
+        getattr(L, name).restype = ctypes.c_int
     L.orc_halo3d.argtypes = [G, ctypes.c_int, _D]
     L.orc_global_sum.argtypes = [G, _D, _D]
     L.orc_global_sum.restype = ctypes.c_double
@@ -134,6 +138,43 @@ def matvec(grid, active_vertex, Auu, Auv, Avu, Avv, xu, xv):
     yu = grid.zeros_vec()
     yv = grid.zeros_vec()
     L.orc_matvec_3d(ctypes.byref(g), ip(as_mask(active_vertex)), dp(Auu), dp(Auv), dp(Avu), dp(Avv),
+                    dp(xu), dp(xv), dp(yu), dp(yv))
+    return yu, yv
+
+
+def pcg_solve_2d(grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, *, algo="chrongear", precond=1,
+                 ncheck=5, tolerance=1e-8, maxiters=200, trace=False):
+    """Restated pcg_solver_{standard,chrongear}_2d in place on xu, xv (nx-1, ny-1);
+    A(nx-1, ny-1, 9).  Returns (status, err, niters[, trace])."""
+    L = lib()
+    g = grid.cstruct()
+    err = ctypes.c_double(float("nan"))
+    nit = ctypes.c_int32(0)
+    act = as_mask(active_vertex)
+    tbuf = None
+    if trace:
+        tbuf = np.zeros(3 * (maxiters + 1))
+        L.orc_set_trace(dp(tbuf), maxiters + 1)
+    else:
+        L.orc_set_trace(None, 0)
+    args = [ctypes.byref(g), ip(act), dp(Auu), dp(Auv), dp(Avu), dp(Avv), dp(bu), dp(bv), dp(xu), dp(xv),
+            int(precond), int(ncheck), float(tolerance), int(maxiters), ctypes.byref(err), ctypes.byref(nit)]
+    f = L.orc_pcg_solver_chrongear_2d if algo == "chrongear" else L.orc_pcg_solver_standard_2d
+    st = f(*args)
+    out = (st, err.value, nit.value)
+    if trace:
+        n = L.orc_trace_len()
+        out = out + (tbuf[: 3 * n].reshape(n, 3).copy(),)
+        L.orc_set_trace(None, 0)
+    return out
+
+
+def matvec_2d(grid, active_vertex, Auu, Auv, Avu, Avv, xu, xv):
+    L = lib()
+    g = grid.cstruct()
+    yu = np.zeros((grid.nvx, grid.nvy), order="F")
+    yv = np.zeros((grid.nvx, grid.nvy), order="F")
+    L.orc_matvec_2d(ctypes.byref(g), ip(as_mask(active_vertex)), dp(Auu), dp(Auv), dp(Avu), dp(Avv),
                     dp(xu), dp(xv), dp(yu), dp(yv))
     return yu, yv
 

@@ -563,3 +563,127 @@ def test_accelerated_picard_same_fixed_point():
     ladder = [1.0 + 0.4 * m for m in range(6)]
     assert all(any(abs(a - l) < 1e-12 for l in ladder) for a in r1["alpha"])
     assert max(r1["alpha"]) <= 3.0 and 1.0 in r1["alpha"]
+
+
+# ----------------------------------------------------------------- 2-D solvers
+from systems import random_spd_system_2d   # noqa: E402
+
+SPD2 = {bc: random_spd_system_2d(bc=bc) for bc in (BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW)}
+
+
+def owned_operator_2d(g, act, A):
+    """Sparse operator of matvec_multiply_structured_2d (pcg.F90:5656-5773) on
+    owned unknowns (comp, j, i), halo rule as in owned_operator."""
+    ni, nj = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1
+    periodic = g.global_bc in (0, 2)
+    uid = lambda c, i, j: (c * nj + (j - g.jlo)) * ni + (i - g.ilo)
+    rows, cols, vals = [], [], []
+    for j in range(g.jlo, g.jhi + 1):
+        for i in range(g.ilo, g.ihi + 1):
+            if not act[i - 1, j - 1]:
+                continue
+            for jA in (-1, 0, 1):
+                for iA in (-1, 0, 1):
+                    qi, qj = i + iA, j + jA
+                    if not (g.ilo <= qi <= g.ihi and g.jlo <= qj <= g.jhi):
+                        if not periodic:
+                            continue
+                        qi = g.ilo + (qi - g.ilo) % ni
+                        qj = g.jlo + (qj - g.jlo) % nj
+                    m = 1 + (iA + 1) + 3 * (jA + 1) - 1
+                    for cr, blkpair in ((0, (0, 1)), (1, (2, 3))):
+                        for cc, blk in enumerate(blkpair):
+                            v = A[blk][i - 1, j - 1, m]
+                            if v != 0.0:
+                                rows.append(uid(cr, i, j))
+                                cols.append(uid(cc, qi, qj))
+                                vals.append(v)
+    n = 2 * ni * nj
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def to_owned_2d(g, u, v):
+    s = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    return np.concatenate([u[s].T.ravel(), v[s].T.ravel()])
+
+
+@pytest.mark.parametrize("bc", [BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW])
+def test_matvec_2d_matches_sparse_operator(bc):
+    g, act, A, bu, bv = SPD2[bc]
+    M = owned_operator_2d(g, act, A)
+    rng = np.random.default_rng(6)
+    xu = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)))
+    xv = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)))
+    po.halo(g, xu)
+    po.halo(g, xv)
+    yu, yv = po.matvec_2d(g, act, *A, xu, xv)
+    ref = M @ to_owned_2d(g, xu, xv)
+    assert np.allclose(to_owned_2d(g, yu, yv), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("algo", ["standard", "chrongear"])
+@pytest.mark.parametrize("bc", [BC_PERIODIC, BC_NO_ICE])
+def test_2d_iterates_equal_textbook_pcg(bc, algo):
+    """The 2-D solvers' k-th iterates equal a textbook PCG with the diagonal
+    preconditioner on the same operator."""
+    g, act, A, bu, bv = SPD2[bc]
+    M = owned_operator_2d(g, act, A)
+    b = to_owned_2d(g, bu, bv)
+    dg = to_owned_2d(g, A[0][:, :, 4], A[3][:, :, 4])
+    ref = textbook_pcg(M, b, lambda r: np.where(dg != 0, r / np.where(dg != 0, dg, 1.0), 0.0), 10)
+    for k in (1, 2, 5, 7, 10):
+        xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+        st, err, nit = po.pcg_solve_2d(g, act, *A, bu, bv, xu, xv, algo=algo, precond=1, tolerance=0.0,
+                                       maxiters=k)
+        assert st == 0 and nit == k
+        x = to_owned_2d(g, xu, xv)
+        assert np.linalg.norm(x - ref[k - 1]) <= 1e-10 * np.linalg.norm(ref[k - 1]), k
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("algo", ["standard", "chrongear"])
+def test_2d_converged_solution_matches_sparse_solve(algo, precond):
+    for bc in (BC_PERIODIC, BC_NO_ICE, BC_OUTFLOW):
+        g, act, A, bu, bv = SPD2[bc]
+        M = owned_operator_2d(g, act, A)
+        b = to_owned_2d(g, bu, bv)
+        keep = np.asarray(M.getnnz(axis=1) > 0)
+        xs = np.zeros_like(b)
+        xs[keep] = spla.spsolve(M[keep][:, keep].tocsc(), b[keep])
+        xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+        st, err, nit = po.pcg_solve_2d(g, act, *A, bu, bv, xu, xv, algo=algo, precond=precond, tolerance=1e-12,
+                                       maxiters=3000)
+        assert st == 0 and err < 1e-12
+        assert np.linalg.norm(to_owned_2d(g, xu, xv) - xs) <= 1e-10 * np.linalg.norm(xs), (bc, precond)
+
+
+def test_2d_tridiagonal_preconditioner_first_step():
+    """2-D precond 3 is the row (u) and column (v) line solves of the stencil
+    itself, without a column part: the first standard iterate is
+    x1 = alpha z0, z0 = PC(b), alpha = (b, z0) / (z0, A z0)."""
+    g, act, A, bu, bv = SPD2[BC_NO_ICE]
+    M = owned_operator_2d(g, act, A)
+    b = to_owned_2d(g, bu, bv)
+    ni, nj = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    z = np.zeros_like(b)
+    Z = z.reshape(2, nj, ni)
+    Bv = b.reshape(2, nj, ni)
+    for c, blk, m_sub, m_sup in ((0, 0, 3, 5), (1, 3, 1, 7)):
+        sub, dia, sup = (A[blk][:, :, m][own] for m in (m_sub, 4, m_sup))
+        dia = np.where(dia == 0, 1.0, dia)
+        if c == 0:       # rows along i for each j
+            for jo in range(nj):
+                T = np.diag(dia[:, jo]) + np.diag(sub[1:, jo], -1) + np.diag(sup[:-1, jo], 1)
+                Z[c, jo, :] = np.linalg.solve(T, Bv[c, jo, :])
+        else:            # columns along j for each i
+            for io in range(ni):
+                T = np.diag(dia[io, :]) + np.diag(sub[io, 1:], -1) + np.diag(sup[io, :-1], 1)
+                Z[c, :, io] = np.linalg.solve(T, Bv[c, :, io])
+    alpha = (b @ z) / (z @ (M @ z))
+    xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+    st, err, nit = po.pcg_solve_2d(g, act, *A, bu, bv, xu, xv, algo="standard", precond=3, tolerance=0.0,
+                                   maxiters=1)
+    assert st == 0
+    x = to_owned_2d(g, xu, xv)
+    assert np.linalg.norm(x - alpha * z) <= 1e-11 * np.linalg.norm(alpha * z)
