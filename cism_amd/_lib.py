@@ -36,6 +36,8 @@ EXPORTS = (
     "cismgpu_ensemble_get_x", "cismgpu_ensemble_member_info", "cismgpu_bench_stream",
     "cismgpu_system_save_assembly", "cismgpu_system_restore_assembly",
     "cismgpu_system_work_vectors", "cismgpu_system_vector_op", "cismgpu_system_residual_velocity",
+    "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d", "cismgpu_matvec_multiply_structured_2d",
+    "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d",
 )
 
 
@@ -76,13 +78,17 @@ def _declare(L):
     L.cismgpu_device_count.restype = ctypes.c_int32
     solver = [_G, _I, _D, _D, _D, _D, _D, _D, _D, _D, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
               ctypes.c_int32, _D, _I]
-    for n in ("cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d"):
+    for n in ("cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d", "cismgpu_pcg_solver_standard_2d",
+              "cismgpu_pcg_solver_chrongear_2d"):
         getattr(L, n).argtypes = solver
         getattr(L, n).restype = ctypes.c_int32
-    L.cismgpu_matvec_multiply_structured_3d.argtypes = [_G, _I, _D, _D, _D, _D, _D, _D, _D, _D]
-    L.cismgpu_matvec_multiply_structured_3d.restype = ctypes.c_int32
+    for n in ("cismgpu_matvec_multiply_structured_3d", "cismgpu_matvec_multiply_structured_2d"):
+        getattr(L, n).argtypes = [_G, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+        getattr(L, n).restype = ctypes.c_int32
     vp = ctypes.c_void_p
     L.cismgpu_system_create.argtypes = [_G, ctypes.POINTER(vp)]
+    L.cismgpu_system_create_2d.argtypes = [_G, ctypes.POINTER(vp)]
+    L.cismgpu_system_set_matrix_2d.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int32]
     L.cismgpu_system_destroy.argtypes = [vp]
     L.cismgpu_system_set_matrix.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int32]
     L.cismgpu_system_get_matrix.argtypes = [vp, _D, _D, _D, _D, _D, _D]
@@ -95,6 +101,7 @@ def _declare(L):
     L.cismgpu_bench_stream.argtypes = [ctypes.c_int64, ctypes.c_int32, _D, _D, _D]
     L.cismgpu_bench_stream.restype = ctypes.c_int32
     for n in ("cismgpu_system_zero_x", "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix",
+              "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d",
               "cismgpu_system_get_matrix", "cismgpu_system_set_x", "cismgpu_system_get_x", "cismgpu_system_solve",
               "cismgpu_system_matvec", "cismgpu_system_bench_spmv"):
         getattr(L, n).restype = ctypes.c_int32

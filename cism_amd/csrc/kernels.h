@@ -18,6 +18,10 @@ __global__ void k_spmv(Dims d, const TA* A, const TV* xu, const TV* xv, TV* yu, 
 template <typename TA, int MODE>
 __global__ void k_spmv_q(Dims d, const TA* A, const float* xu, const float* xv, float* yu, float* yv,
                          const float* bu, const float* bv, double* partials);
+template <typename TA, typename TV, int MODE>
+__global__ void k_spmv_plane(Dims d, const TA* A, const TV* xu, const TV* xv, TV* yu, TV* yv, const TV* bu,
+                             const TV* bv, double* partials);
+__global__ void k_expand_stencil_2d(long long n2, const double* src, double* dst);
 template <typename TA>
 __global__ void k_convert_quad(long long nrows, const double* src, TA* dst);
 template <typename TV>

@@ -189,6 +189,76 @@ __global__ __launch_bounds__(256) void k_spmv_q(Dims d, const TA* __restrict__ A
     }
 }
 
+// ---------------------------------------------------------------------------
+// One-level SpMV of the 2-D (depth-integrated) systems: the 9-point stencil
+// is the kA = 0 plane (m = 9..17) of the 27-point row storage; the other two
+// planes hold zeros (they couple outside the single level) and are not read.
+// FP64 rows use the row-blocked SoA layout, FP32/BF16 rows the quad layout.
+// Same MODEs and partial sums as k_spmv.
+// ---------------------------------------------------------------------------
+template <typename TA, typename TV, int MODE>
+__global__ __launch_bounds__(256) void k_spmv_plane(Dims d, const TA* __restrict__ A,
+                                                    const TV* __restrict__ xu, const TV* __restrict__ xv,
+                                                    TV* __restrict__ yu, TV* __restrict__ yv,
+                                                    const TV* __restrict__ bu, const TV* __restrict__ bv,
+                                                    double* __restrict__ partials) {
+    const int nbx = d.nxp / WAVE;
+    const int nblk = nbx * ((d.nyo + 3) / 4);
+    const int lin = xcd_remap(blockIdx.x, nblk);
+    const int bx = lin % nbx, by = lin / nbx;
+    const int io = bx * WAVE + threadIdx.x % WAVE;
+    const int jo = by * 4 + threadIdx.x / WAVE;
+    dd acc = {0.0, 0.0};
+    if (io < d.nxo && jo < d.nyo) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+        const long long c0 = vidx(d, 0, i, j);
+        const long long r = (long long)jo * d.nxp + io;
+        TV su = TV(0), sv = TV(0);
+#pragma unroll
+        for (int jA = -1; jA <= 1; ++jA)
+#pragma unroll
+            for (int iA = -1; iA <= 1; ++iA) {
+                const int m = 9 + (iA + 1) + 3 * (jA + 1);
+                const long long n = c0 + jA * d.px + iA;
+                const TV xun = xu[n], xvn = xv[n];
+                TV cuu, cuv, cvu, cvv;
+                if constexpr (std::is_same<TA, double>::value) {
+                    const TA* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+                    cuu = a[(0 * 27 + m) * WAVE]; cuv = a[(1 * 27 + m) * WAVE];
+                    cvu = a[(2 * 27 + m) * WAVE]; cvv = a[(3 * 27 + m) * WAVE];
+                } else {
+                    float q0, q1, q2, q3;
+                    ld_quad(A + (r / WAVE) * (27 * 4 * WAVE) + m * (4 * WAVE) + (r % WAVE) * 4, q0, q1, q2, q3);
+                    cuu = q0; cuv = q1; cvu = q2; cvv = q3;
+                }
+                su = su + cuu * xun + cuv * xvn;
+                sv = sv + cvu * xun + cvv * xvn;
+            }
+        if (MODE == 2) {
+            const TV ru = bu[c0] - su, rv = bv[c0] - sv;
+            yu[c0] = ru;
+            yv[c0] = rv;
+            acc = dd_add_d(acc, (double)ru * (double)ru);
+            acc = dd_add_d(acc, (double)rv * (double)rv);
+        } else {
+            yu[c0] = su;
+            yv[c0] = sv;
+            if (MODE == 1) {
+                acc = dd_add_d(acc, (double)su * (double)xu[c0]);
+                acc = dd_add_d(acc, (double)sv * (double)xv[c0]);
+            }
+        }
+    }
+    if (MODE != 0) {
+        dd v[1] = {acc};
+        block_reduce_dd<1, 256>(v);
+        if (threadIdx.x == 0) {
+            partials[2 * lin + 0] = v[0].hi;
+            partials[2 * lin + 1] = v[0].lo;
+        }
+    }
+}
+
 // Non-owned ring of the vertex grid: y = 0 after a matvec (the reference
 // zeroes the whole output, pcg.F90:5605-5606), or r = b - 0 after a
 // residual.  One thread per ring column.
@@ -382,7 +452,8 @@ __global__ __launch_bounds__(256) void k_update(Dims d, const double* __restrict
 
 // Stand-alone preconditioner apply z = PC(r) with partial (r, z) and
 // (r, r) over owned columns.  kind: 0 none (copy), 1 diag, 2 SIA.
-// For kinds 3/4 the caller adds the horizontal line-solve correction.
+// For kinds 3/4 the caller adds the horizontal line-solve correction; kind 5
+// (one-level systems) takes that line solution alone.
 template <typename TV, int NZMAX>
 __global__ __launch_bounds__(256) void k_precond(Dims d, int kind, const double* __restrict__ scal,
                                                  const TV* __restrict__ ru, const TV* __restrict__ rv,
@@ -417,6 +488,12 @@ __global__ __launch_bounds__(256) void k_precond(Dims d, int kind, const double*
                     zU[k] = gu != TV(0) ? rU[k] / gu : TV(0);
                     zV[k] = gv != TV(0) ? rV[k] / gv : TV(0);
                 }
+        } else if (kind == 5) {
+            // 2-D solves: z is the horizontal line solution itself on every
+            // vertex (pcg.F90:456-620 standard, :2020-2140 and :2275-2420 C-G)
+            const long long c2 = (long long)(j - 1) * d.nvx + (i - 1);
+#pragma unroll
+            for (int k = 0; k < NZMAX; ++k) { zU[k] = zvs_u[c2]; zV[k] = zvs_v[c2]; }
         } else {
             if (active[t]) {
                 const long long vl = d.vlen;
@@ -774,6 +851,16 @@ __global__ void k_unpack_matrix(Dims d, int blk, const double* __restrict__ A, d
     dst[t] = v;
 }
 
+// 2-D stencil A(nvx, nvy, 9) (m last, m = 1 + (iA+1) + 3(jA+1)) -> the kA = 0
+// plane of a one-level reference row A(27, 1, nvx, nvy); other planes zero.
+__global__ void k_expand_stencil_2d(long long n2, const double* __restrict__ src, double* __restrict__ dst) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;   // over 27 * n2
+    if (t >= 27 * n2) return;
+    const int m = (int)(t % 27);
+    const long long p = t / 27;
+    dst[t] = (m >= 9 && m < 18) ? src[p + n2 * (m - 9)] : 0.0;
+}
+
 // Reference vector (nz, nvx, nvy) <-> padded level-major layout.
 template <typename TS, typename TD>
 __global__ void k_vec_in(Dims d, const TS* __restrict__ src, TD* __restrict__ dst) {
@@ -859,6 +946,16 @@ INST_SPMV(__hip_bfloat16, float)
                                              const float*, const float*, double*);
 INST_SPMV_Q(float)
 INST_SPMV_Q(__hip_bfloat16)
+#define INST_SPMV_PLANE(TA, TV)                                                                              \
+    template __global__ void k_spmv_plane<TA, TV, 0>(Dims, const TA*, const TV*, const TV*, TV*, TV*,       \
+                                                     const TV*, const TV*, double*);                        \
+    template __global__ void k_spmv_plane<TA, TV, 1>(Dims, const TA*, const TV*, const TV*, TV*, TV*,       \
+                                                     const TV*, const TV*, double*);                        \
+    template __global__ void k_spmv_plane<TA, TV, 2>(Dims, const TA*, const TV*, const TV*, TV*, TV*,       \
+                                                     const TV*, const TV*, double*);
+INST_SPMV_PLANE(double, double)
+INST_SPMV_PLANE(float, float)
+INST_SPMV_PLANE(__hip_bfloat16, float)
 template __global__ void k_convert_quad<float>(long long, const double*, float*);
 template __global__ void k_convert_quad<__hip_bfloat16>(long long, const double*, __hip_bfloat16*);
 

@@ -76,6 +76,7 @@ struct cismgpu_system {
     size_t saved_n = 0;          // doubles in `saved`
     bool have_saved = false;
     bool have_matrix = false;
+    bool two_d = false;          // one-level (depth-integrated) system: 9-point stencil, nz = 1
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
     int ev_used = 0;
@@ -88,9 +89,9 @@ struct cismgpu_system {
 
 namespace {
 
-int check_grid(const cismgpu_grid* g) {
+int check_grid(const cismgpu_grid* g, bool two_d = false) {
     if (!g) { cismgpu_set_error("null grid"); return CISMGPU_ERR_ARG; }
-    if (g->nx < 6 || g->ny < 6 || g->nz < 2 || g->nz > 64) {
+    if (g->nx < 6 || g->ny < 6 || g->nz < (two_d ? 1 : 2) || g->nz > (two_d ? 1 : 64)) {
         cismgpu_set_error("bad grid sizes nx=%d ny=%d nz=%d (nz must be 2..64)", g->nx, g->ny, g->nz);
         return CISMGPU_ERR_ARG;
     }
@@ -156,7 +157,9 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
           double* part) {
     const bool t = S->timing && S->ev_used + 2 <= (int)S->ev.size();
     if (t) hipEventRecord(S->ev[S->ev_used], S->s);
-    if constexpr (std::is_same<TA, double>::value)
+    if (S->two_d)   // one level: only the kA = 0 plane of the rows is read
+        k_spmv_plane<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    else if constexpr (std::is_same<TA, double>::value)
         k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     else   // FP32 / BF16 copies are stored in the quad layout
         k_spmv_q<TA, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
@@ -233,7 +236,7 @@ int reset_scal(cismgpu_system* S) {
 // ---------------------------------------------------------------- tridiagonal PC setup
 // Vertical sums of the 2-D stencil entries used by the line solves
 // (pcg.F90:3815-3840 local; :3996-4021 global) from the raw reference blocks.
-__global__ void k_tri_vsum(Dims d, int mode, const double* __restrict__ Auu, const double* __restrict__ Avv,
+__global__ void k_tri_vsum(Dims d, int mode, int two_d, const double* __restrict__ Auu, const double* __restrict__ Avv,
                            const int8_t* __restrict__ active, double* __restrict__ su_s, double* __restrict__ su_d,
                            double* __restrict__ su_p, double* __restrict__ sv_s, double* __restrict__ sv_d,
                            double* __restrict__ sv_p) {
@@ -252,7 +255,7 @@ __global__ void k_tri_vsum(Dims d, int mode, const double* __restrict__ Auu, con
         }
         u[q] = a; v[q] = b;
     }
-    if (!active[t]) {
+    if (!active[t] && !two_d) {   // 2-D solves take the stencil as given (pcg.F90:287-297, :343-380)
         if (mode == 3) { u[0] = u[2] = v[0] = v[2] = 0.0; }
         u[1] = 1.0; v[1] = 1.0;
     }
@@ -314,7 +317,7 @@ int setup_tri(cismgpu_system* S, int mode) {
     double *su_s = T.rsu, *su_d = T.rsv, *su_p = T.zsu, *sv_s = T.zsv, *sv_d = T.gam, *sv_p = T.Asv;
     // stage[0] holds Auu, stage[1] Avv (raw layout)
     const long long nraw = 27LL * d.nz * d.nvx * d.nvy;
-    k_tri_vsum<<<nblocks(n2, 256), 256, 0, S->s>>>(d, mode, S->stage, S->stage + nraw, S->active, su_s, su_d, su_p,
+    k_tri_vsum<<<nblocks(n2, 256), 256, 0, S->s>>>(d, mode, S->two_d ? 1 : 0, S->stage, S->stage + nraw, S->active, su_s, su_d, su_p,
                                                   sv_s, sv_d, sv_p);
     if (mode == 4) {
         halo2d<double>(S, su_s, su_d);
@@ -373,7 +376,7 @@ void apply_precond(cismgpu_system* S, int precond, const TV* ru, const TV* rv, T
     else { M = S->M32; Dg = S->M32 + 6 * d.vlen; }
     TV *zsu = nullptr, *zsv = nullptr;
     if (precond >= 3) tri_horizontal<TV>(S, ru, rv, &zsu, &zsv);
-    launch_precond<TV>(S, precond, ru, rv, zu, zv, M, Dg, zsu, zsv);
+    launch_precond<TV>(S, (S->two_d && precond >= 3) ? 5 : precond, ru, rv, zu, zv, M, Dg, zsu, zsv);
 }
 
 int finish_timing(cismgpu_system* S, cismgpu_solve_info* info) {
@@ -627,10 +630,10 @@ int check_opts(const cismgpu_solve_opts* o) {
 // ============================================================================
 extern "C" {
 
-int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out) {
+static int32_t system_create(const cismgpu_grid* grid, bool two_d, cismgpu_system** out) {
     if (!out) return CISMGPU_ERR_ARG;
     *out = nullptr;
-    int st = check_grid(grid);
+    int st = check_grid(grid, two_d);
     if (st) return st;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -639,6 +642,7 @@ int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out) {
     }
     cismgpu_system* S = new cismgpu_system();
     S->d = make_dims(grid);
+    S->two_d = two_d;
     const Dims& d = S->d;
     auto fail = [&](hipError_t e, const char* what) {
         cismgpu_set_error("%s: %s", what, hipGetErrorString(e));
@@ -670,6 +674,17 @@ int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out) {
     return CISMGPU_OK;
 }
 
+int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out) {
+    return system_create(grid, false, out);
+}
+
+int32_t cismgpu_system_create_2d(const cismgpu_grid* grid, cismgpu_system** out) {
+    if (!grid) { cismgpu_set_error("null grid"); return CISMGPU_ERR_ARG; }
+    cismgpu_grid g = *grid;
+    g.nz = 1;
+    return system_create(&g, true, out);
+}
+
 int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (!S) return CISMGPU_OK;
     if (S->s) hipStreamSynchronize(S->s);
@@ -684,19 +699,20 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     return CISMGPU_OK;
 }
 
-int32_t cismgpu_system_set_matrix(cismgpu_system* S, const int32_t* active_vertex, const double* Auu,
-                                  const double* Auv, const double* Avu, const double* Avv, const double* bu,
-                                  const double* bv, int32_t where) {
-    if (!S || !active_vertex || !Auu || !Auv || !Avu || !Avv || !bu || !bv) return CISMGPU_ERR_ARG;
+// Matrix, rhs and mask into the device layouts.  where_*: 0 host, 1 device.
+static int set_matrix_impl(cismgpu_system* S, const int32_t* active_vertex, int where_act, const double* Auu,
+                           const double* Auv, const double* Avu, const double* Avv, int where_A, const double* bu,
+                           const double* bv, int where_b) {
     const Dims& d = S->d;
     const long long n2 = (long long)d.nvx * d.nvy;
     const long long nraw = 27LL * d.nz * n2;
     const long long nvec = (long long)d.nz * n2;
-    const hipMemcpyKind kind = where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kind = where_A ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kindb = where_b ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
     // active mask (int32 -> int8)
     std::vector<int8_t> act(n2);
-    if (where) {
+    if (where_act) {
         std::vector<int32_t> tmp(n2);
         CG_CHECK(hipMemcpy(tmp.data(), active_vertex, n2 * sizeof(int32_t), hipMemcpyDeviceToHost));
         for (long long i = 0; i < n2; ++i) act[i] = tmp[i] ? 1 : 0;
@@ -719,9 +735,9 @@ int32_t cismgpu_system_set_matrix(cismgpu_system* S, const int32_t* active_verte
     k_setup_precond<double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, s0, s1, S->M64, S->M64 + 6 * d.vlen);
     // rhs
     double* tmp = S->V(4);   // r vectors as staging for b
-    CG_CHECK(hipMemcpyAsync(tmp, bu, nvec * sizeof(double), kind, S->s));
+    CG_CHECK(hipMemcpyAsync(tmp, bu, nvec * sizeof(double), kindb, S->s));
     k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(2));
-    CG_CHECK(hipMemcpyAsync(tmp, bv, nvec * sizeof(double), kind, S->s));
+    CG_CHECK(hipMemcpyAsync(tmp, bv, nvec * sizeof(double), kindb, S->s));
     k_vec_in<double, double><<<nblocks(nvec, 256), 256, 0, S->s>>>(d, tmp, S->V(3));
     CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
@@ -729,6 +745,44 @@ int32_t cismgpu_system_set_matrix(cismgpu_system* S, const int32_t* active_verte
     S->m32_ready = S->a32_ready = S->a16_ready = false;
     S->tri.mode = 0;
     return CISMGPU_OK;
+}
+
+int32_t cismgpu_system_set_matrix(cismgpu_system* S, const int32_t* active_vertex, const double* Auu,
+                                  const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                  const double* bv, int32_t where) {
+    if (!S || !active_vertex || !Auu || !Auv || !Avu || !Avv || !bu || !bv) return CISMGPU_ERR_ARG;
+    if (S->two_d) { cismgpu_set_error("a 2-D system takes its matrix through set_matrix_2d"); return CISMGPU_ERR_ARG; }
+    return set_matrix_impl(S, active_vertex, where, Auu, Auv, Avu, Avv, where, bu, bv, where);
+}
+
+int32_t cismgpu_system_set_matrix_2d(cismgpu_system* S, const int32_t* active_vertex, const double* Auu,
+                                     const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                     const double* bv, int32_t where) {
+    if (!S || !active_vertex || !Auu || !Auv || !Avu || !Avv || !bu || !bv) return CISMGPU_ERR_ARG;
+    if (!S->two_d) { cismgpu_set_error("set_matrix_2d needs a system made by cismgpu_system_create_2d"); return CISMGPU_ERR_ARG; }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    // each 9-point block becomes the kA = 0 plane of a one-level 27-point row
+    double* buf = nullptr;
+    CG_CHECK(hipMalloc(&buf, (4 * 27 + 9) * n2 * sizeof(double)));
+    double* in9 = buf + 4 * 27 * n2;
+    const double* src[4] = {Auu, Auv, Avu, Avv};
+    const hipMemcpyKind kind = where ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    int st = CISMGPU_OK;
+    for (int b = 0; b < 4 && !st; ++b) {
+        if (hipMemcpyAsync(in9, src[b], 9 * n2 * sizeof(double), kind, S->s) != hipSuccess) st = CISMGPU_ERR_HIP;
+        else k_expand_stencil_2d<<<nblocks(27 * n2, 256), 256, 0, S->s>>>(n2, in9, buf + b * 27 * n2);
+    }
+    if (!st && hipStreamSynchronize(S->s) != hipSuccess) st = CISMGPU_ERR_HIP;
+    if (st) {
+        cismgpu_set_error("set_matrix_2d: device copy failed");
+        hipFree(buf);
+        return st;
+    }
+    // vectors (nvx, nvy) are the one-level (1, nvx, nvy) layout
+    st = set_matrix_impl(S, active_vertex, where, buf, buf + 27 * n2, buf + 54 * n2, buf + 81 * n2, 1, bu, bv, where);
+    hipFree(buf);
+    return st;
 }
 
 int32_t cismgpu_system_get_matrix(cismgpu_system* S, double* Auu, double* Auv, double* Avu, double* Avv,
@@ -795,6 +849,11 @@ int32_t cismgpu_system_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cis
     if (!S || !S->have_matrix || !info) { cismgpu_set_error("system has no matrix"); return CISMGPU_ERR_ARG; }
     int st = check_opts(o);
     if (st) return st;
+    cismgpu_solve_opts eff = *o;
+    // one-level systems have no column part: which_ho_precond = 2 becomes the
+    // diagonal preconditioner (glide_setup.F90:1363-1366)
+    if (S->two_d && eff.precond == 2) eff.precond = 1;
+    o = &eff;
     memset(info, 0, sizeof(*info));
     info->err = NAN;
     if ((o->precond == 3 || o->precond == 4) && S->tri.mode != o->precond) {
@@ -874,7 +933,14 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     CG_CHECK(hipEventCreate(&e1));
     // the form the solvers launch: y = A x with the fused (y, x) partial sums
     auto launch = [&]() {
-        if (kind == 0)
+        if (S->two_d) {
+            if (kind == 0)
+                k_spmv_plane<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
+            else if (kind == 1)
+                k_spmv_plane<float, float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
+            else
+                k_spmv_plane<__hip_bfloat16, float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
+        } else if (kind == 0)
             k_spmv<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
         else if (kind == 1)
             k_spmv_q<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
@@ -893,7 +959,8 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     if (avg_ms) *avg_ms = ms / nrep;
     const double esz = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 2.0);
     const double vsz = kind == 0 ? 8.0 : 4.0;
-    if (bytes_per_launch) *bytes_per_launch = (double)nodes * (NCOEF * esz + 4.0 * vsz);
+    const double ncoef = S->two_d ? 36.0 : (double)NCOEF;   // 4 blocks x 9 or x 27 neighbours
+    if (bytes_per_launch) *bytes_per_launch = (double)nodes * (ncoef * esz + 4.0 * vsz);
     return CISMGPU_OK;
 }
 
@@ -1011,6 +1078,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         cismgpu_set_error("assemble_3d: missing input");
         return CISMGPU_ERR_ARG;
     }
+    if (S->two_d) { cismgpu_set_error("assemble_3d needs a 3-D system"); return CISMGPU_ERR_ARG; }
     int st = upload_fe_tables();
     if (st) return st;
     const Dims& d = S->d;
@@ -1492,6 +1560,7 @@ extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* mem
         cismgpu_system* S = members[m];
         if (!S || !S->have_matrix) return fail(CISMGPU_ERR_ARG, "member without an assembled matrix");
         const Dims& d = S->d;
+        if (S->two_d) return fail(CISMGPU_ERR_ARG, "members must be 3-D systems");
         if (d.nz > 16) return fail(CISMGPU_ERR_ARG, "members need nz <= 16");
         if (d.bc != 1 && d.bc != 3) return fail(CISMGPU_ERR_ARG, "members need the no_ice or outflow halo rule");
         // active vertices must avoid every row/column the halo rule writes
@@ -1726,15 +1795,17 @@ extern "C" int32_t cismgpu_ensemble_member_info(cismgpu_ensemble* E, int32_t m, 
 // ---------------------------------------------------------------- drop-in entry points
 static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, const double* Auu, const double* Auv,
                   const double* Avu, const double* Avv, const double* bu, const double* bv, double* xu, double* xv,
-                  int32_t precond, int32_t ncheck, double tol, int32_t maxiters, double* err, int32_t* niters) {
+                  int32_t precond, int32_t ncheck, double tol, int32_t maxiters, double* err, int32_t* niters,
+                  bool two_d = false) {
     if (!active || !Auu || !Auv || !Avu || !Avv || !bu || !bv || !xu || !xv || !err || !niters) {
         cismgpu_set_error("null argument");
         return CISMGPU_ERR_ARG;
     }
     cismgpu_system* S = nullptr;
-    int st = cismgpu_system_create(grid, &S);
+    int st = two_d ? cismgpu_system_create_2d(grid, &S) : cismgpu_system_create(grid, &S);
     if (st) return st;
-    st = cismgpu_system_set_matrix(S, active, Auu, Auv, Avu, Avv, bu, bv, 0);
+    st = two_d ? cismgpu_system_set_matrix_2d(S, active, Auu, Auv, Avu, Avv, bu, bv, 0)
+               : cismgpu_system_set_matrix(S, active, Auu, Auv, Avu, Avv, bu, bv, 0);
     if (!st) st = cismgpu_system_set_x(S, xu, xv);
     cismgpu_solve_info info{};
     if (!st) {
@@ -1781,6 +1852,40 @@ int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const in
     const Dims& d = S->d;
     std::vector<double> zero((size_t)d.nz * d.nvx * d.nvy, 0.0);
     st = cismgpu_system_set_matrix(S, active_vertex, Auu, Auv, Avu, Avv, zero.data(), zero.data(), 0);
+    if (!st) st = cismgpu_system_set_x(S, xu, xv);
+    if (!st) st = cismgpu_system_matvec(S, yu, yv);
+    cismgpu_system_destroy(S);
+    return st;
+}
+
+int32_t cismgpu_pcg_solver_standard_2d(const cismgpu_grid* grid, const int32_t* active_vertex, const double* Auu,
+                                       const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                       const double* bv, double* xu, double* xv, int32_t precond,
+                                       int32_t linear_solve_ncheck, double tolerance, int32_t maxiters, double* err,
+                                       int32_t* niters) {
+    return dropin(CISMGPU_ALGO_STANDARD, grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond,
+                  linear_solve_ncheck, tolerance, maxiters, err, niters, true);
+}
+
+int32_t cismgpu_pcg_solver_chrongear_2d(const cismgpu_grid* grid, const int32_t* active_vertex, const double* Auu,
+                                        const double* Auv, const double* Avu, const double* Avv, const double* bu,
+                                        const double* bv, double* xu, double* xv, int32_t precond,
+                                        int32_t linear_solve_ncheck, double tolerance, int32_t maxiters, double* err,
+                                        int32_t* niters) {
+    return dropin(CISMGPU_ALGO_CHRONGEAR, grid, active_vertex, Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond,
+                  linear_solve_ncheck, tolerance, maxiters, err, niters, true);
+}
+
+int32_t cismgpu_matvec_multiply_structured_2d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                              const double* Auu, const double* Auv, const double* Avu,
+                                              const double* Avv, const double* xu, const double* xv, double* yu,
+                                              double* yv) {
+    cismgpu_system* S = nullptr;
+    int st = cismgpu_system_create_2d(grid, &S);
+    if (st) return st;
+    const Dims& d = S->d;
+    std::vector<double> zero((size_t)d.nvx * d.nvy, 0.0);
+    st = cismgpu_system_set_matrix_2d(S, active_vertex, Auu, Auv, Avu, Avv, zero.data(), zero.data(), 0);
     if (!st) st = cismgpu_system_set_x(S, xu, xv);
     if (!st) st = cismgpu_system_matvec(S, yu, yv);
     cismgpu_system_destroy(S);

@@ -89,6 +89,43 @@ int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const in
                                               const double* xu, const double* xv,
                                               double* yu, double* yv);
 
+/* 2-D (depth-integrated: SSA, L1L2, DIVA) solves.  Matrices are the
+ * reference's (nx-1, ny-1, 9) stencils (m last, m = 1 + (iA+1) + 3(jA+1),
+ * indxA_2d), vectors (nx-1, ny-1); grid->nz is ignored.  precond 0 none,
+ * 1 diag, 3/4 tridiagonal line solves on the stencil as given; 2 (SIA) is
+ * taken as 1, the replacement glide_setup.F90:1363-1366 makes for 2-D
+ * solves. */
+
+/* Replaces pcg_solver_standard_2d (glissade_velo_higher_pcg.F90:67-823);
+ * called from glissade_velo_higher.F90:3260-3271.  Same contract as the
+ * 3-D entry points. */
+int32_t cismgpu_pcg_solver_standard_2d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                       const double* Auu, const double* Auv,
+                                       const double* Avu, const double* Avv,
+                                       const double* bu, const double* bv,
+                                       double* xu, double* xv,
+                                       int32_t precond, int32_t linear_solve_ncheck,
+                                       double tolerance, int32_t maxiters,
+                                       double* err, int32_t* niters);
+
+/* Replaces pcg_solver_chrongear_2d (glissade_velo_higher_pcg.F90:1541-2628);
+ * called from glissade_velo_higher.F90:3245-3256. */
+int32_t cismgpu_pcg_solver_chrongear_2d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                        const double* Auu, const double* Auv,
+                                        const double* Avu, const double* Avv,
+                                        const double* bu, const double* bv,
+                                        double* xu, double* xv,
+                                        int32_t precond, int32_t linear_solve_ncheck,
+                                        double tolerance, int32_t maxiters,
+                                        double* err, int32_t* niters);
+
+/* Replaces matvec_multiply_structured_2d (glissade_velo_higher_pcg.F90:5656-5773). */
+int32_t cismgpu_matvec_multiply_structured_2d(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                              const double* Auu, const double* Auv,
+                                              const double* Avu, const double* Avv,
+                                              const double* xu, const double* xv,
+                                              double* yu, double* yv);
+
 /* ------------------------------------------------------------------ */
 /* Device-resident system: the matrix, rhs and iterate stay in HBM     */
 /* across calls (Picard iterations, benchmarks).  No reference         */
@@ -119,6 +156,11 @@ typedef struct cismgpu_solve_info {
 } cismgpu_solve_info;
 
 int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out);
+/* One-level system for the 2-D solves (grid->nz ignored).  Its matrix is
+ * loaded with cismgpu_system_set_matrix_2d; vectors are (nx-1, ny-1); the
+ * SpMV reads the 9-point stencil only.  Assembly and ensembles need 3-D
+ * systems. */
+int32_t cismgpu_system_create_2d(const cismgpu_grid* grid, cismgpu_system** out);
 int32_t cismgpu_system_destroy(cismgpu_system* sys);
 
 /* Load A (reference layout) and b.  where = 0: host pointers; 1: device pointers. */
@@ -126,7 +168,14 @@ int32_t cismgpu_system_set_matrix(cismgpu_system* sys, const int32_t* active_ver
                                   const double* Auu, const double* Auv,
                                   const double* Avu, const double* Avv,
                                   const double* bu, const double* bv, int32_t where);
-/* Copy A and b back to reference layout (host pointers).  Rows that are not
+/* Load a 2-D stencil A(nx-1, ny-1, 9) and b(nx-1, ny-1) into a system from
+ * cismgpu_system_create_2d.  where as above. */
+int32_t cismgpu_system_set_matrix_2d(cismgpu_system* sys, const int32_t* active_vertex,
+                                     const double* Auu, const double* Auv,
+                                     const double* Avu, const double* Avv,
+                                     const double* bu, const double* bv, int32_t where);
+/* Copy A and b back to reference layout (host pointers); for a 2-D system
+ * the rows are (27, 1, nx-1, ny-1) with the stencil in m = 10..18.  Rows that are not
  * owned are returned as zero. */
 int32_t cismgpu_system_get_matrix(cismgpu_system* sys, double* Auu, double* Auv,
                                   double* Avu, double* Avv, double* bu, double* bv);

@@ -22,7 +22,9 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for n in ("cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d",
-              "cismgpu_matvec_multiply_structured_3d", "cismgpu_system_assemble_3d"):
+              "cismgpu_matvec_multiply_structured_3d", "cismgpu_system_assemble_3d",
+              "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d",
+              "cismgpu_matvec_multiply_structured_2d", "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d"):
         assert n in names
 
 
@@ -54,3 +56,19 @@ def test_bad_grid_is_rejected():
     g.ilo = 1   # owned bounds inside the halo
     h = ctypes.c_void_p()
     assert L.cismgpu_system_create(ctypes.byref(g.cstruct()), ctypes.byref(h)) == 3
+
+
+def test_2d_wrappers_check_shapes_before_any_device_call():
+    """The 2-D host functions take (nx-1, ny-1, 9) stencils and (nx-1, ny-1)
+    vectors; 3-D-shaped arrays are rejected on the host."""
+    import numpy as np
+    from cism_amd import pcg
+    g = Grid.from_owned(8, 8, 5)
+    act = np.ones((g.nvx, g.nvy), np.int32, order="F")
+    A3 = [g.zeros_mat() for _ in range(4)]
+    v3 = g.zeros_vec()
+    with pytest.raises(ValueError):
+        pcg.pcg_solver_chrongear_2d(g, act, *A3, v3, v3, v3.copy(order="F"), v3.copy(order="F"), 1, 5, 1e-8, 10)
+    A2 = [np.zeros((g.nvx, g.nvy, 9), order="F") for _ in range(4)]
+    with pytest.raises(ValueError):
+        pcg.matvec_multiply_structured_2d(g, act, *A2, v3, v3)
