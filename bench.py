@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--no-cpu-all", action="store_true", help="skip the all-cores CPU leg")
     ap.add_argument("--ensemble", type=int, default=256, help="members of the batched-ensemble run (0: skip)")
     ap.add_argument("--picard-n", type=int, default=1024, help="marine-sheet Picard run size (0: skip)")
+    ap.add_argument("--no-twod", action="store_true", help="skip the depth-integrated 2-D solver leg")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     args = ap.parse_args()
 
@@ -161,6 +162,14 @@ def main():
     # system, a bounded sample of iterations, extrapolated to the FP64 iteration count
     cpu = None
     dropin = None
+    twod_sys = None
+    if not args.no_twod:
+        try:
+            (Auu, Auv, Avu, Avv), (bu, bv) = s.get_matrix()
+            twod_sys = synthetic.depth_integrated_system(Auu, Auv, Avu, Avv, bu, bv)
+            del Auu, Auv, Avu, Avv, bu, bv
+        except Exception as e:
+            log(f"2-D system export failed: {e!r}")
     if not args.no_cpu:
         try:
             from oracle import pyoracle as po
@@ -205,7 +214,8 @@ def main():
         except Exception as e:   # the CPU leg must not hide the GPU result
             log(f"CPU baseline failed: {e!r}")
 
-    del s, bp
+    twod = run_twod(args, g, bp.active_vertex, twod_sys) if twod_sys is not None else None
+    del s, bp, twod_sys
     ens = run_ensemble(args) if args.ensemble > 0 else None
     pic = run_marine_picard(args) if args.picard_n > 0 else None
 
@@ -247,6 +257,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ens,
         "picard_marine": pic,
+        "twod": twod,
         "wall_s": round(time.time() - t0, 1),
     }
     line = json.dumps(out)
@@ -291,6 +302,74 @@ def dome_picard_legs():
                 np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]])))
     return dict(cpu_s=round(t_cpu, 3), gpu_s=round(t_gpu, 3), cpu_iterations=ref["iterations"],
                 gpu_iterations=r.iterations, rel_diff=rel, cores=1)
+
+
+def run_twod(args, g, act, system):
+    """Depth-integrated 2-D solves (pcg_solver_chrongear_2d path) on the
+    P^T A P collapse of the assembled ice-stream system: FP64 drop-in solves
+    with the diagonal (which_ho_precond 1, and 2 as replaced for 2-D solves)
+    and local tridiagonal preconditioners, the mixed FP32 solve, isolated
+    9-point SpMV rates, and the oracle's 2-D C-G on the host as the CPU
+    sample."""
+    import numpy as np
+
+    from cism_amd import pcg
+    try:
+        (A2, bu, bv) = system
+        nv = (g.ihi - g.ilo + 1) * (g.jhi - g.jlo + 1)
+        s2 = pcg.System(g, two_d=True)
+        s2.set_matrix(act, *A2, bu, bv)
+        s2.zero_x()
+        s2.solve(precond=1, tolerance=1e-12, maxiters=200000)
+        xref = np.stack(s2.get_x())
+
+        def err():
+            x = np.stack(s2.get_x())
+            return float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
+
+        legs = {}
+        for name, kw in (("fp64_diag", dict(precond=1)), ("fp64_tridiag_local", dict(precond=3)),
+                         ("mixed_fp32_diag", dict(precond=1, precision=pcg.MIXED_FP32, inner_tol=1e-6,
+                                                  max_outer=40))):
+            s2.zero_x()
+            s2.solve(tolerance=args.tol, maxiters=200000, **kw)       # warm-up
+            ms, its = [], []
+            for _ in range(max(args.steps, 3)):
+                s2.zero_x()
+                r = s2.solve(tolerance=args.tol, maxiters=200000, time_spmv=True, **kw)
+                ms.append(r["solve_ms"])
+                its.append(r["niters"])
+            legs[name] = dict(ms_per_solve=round(float(np.mean(ms)), 3), solves_per_s=round(float(1e3 / np.mean(ms)), 2),
+                              iterations=its, outer=r["outer_iters"], rel_err_vs_tight_fp64=err(),
+                              spmv_avg_ms=round(r["spmv_ms"] / max(r["spmv_count"], 1), 4))
+            log(f"2-D {name}: {legs[name]}")
+        spmv = {}
+        for kind, name in ((0, "fp64"), (1, "fp32"), (2, "bf16")):
+            ms, by = s2.bench_spmv(kind, 50)
+            spmv[name] = dict(avg_ms=round(ms, 4), algorithmic_bytes=int(by), gbps=round(by / (ms * 1e-3) / 1e9, 1),
+                              frac_of_peak=round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3))
+        log(f"2-D isolated SpMV: {spmv}")
+        s2.close()
+        cpu = None
+        if not args.no_cpu:
+            from oracle import pyoracle as po
+            k = 20
+            xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+            tc = time.perf_counter()
+            po.pcg_solve_2d(g, act, *A2, bu, bv, xu, xv, algo="chrongear", precond=1, tolerance=0.0, maxiters=k)
+            per = (time.perf_counter() - tc) / k
+            n_it = int(np.mean(legs["fp64_diag"]["iterations"]))
+            cpu = dict(value=round(1.0 / (per * n_it), 4), unit="solves/s", cores=1, kind="port",
+                       sample=f"{k} Chronopoulos-Gear iterations (diagonal preconditioner) of the same 2-D system by "
+                              f"the C restatement in oracle/ on 1 thread; {per * 1e3:.1f} ms/iteration x {n_it} "
+                              f"iterations", seconds_per_solve=round(per * n_it, 3))
+            log(f"2-D CPU baseline: {cpu}")
+        return dict(workload=f"depth-integrated (P^T A P) ice stream {args.n}x{args.n}, 9-point 2x2-block stencil",
+                    owned_vertices=nv, unknowns=2 * nv, tolerance=args.tol, legs=legs, spmv_isolated=spmv,
+                    cpu_baseline=cpu)
+    except Exception as e:      # a failing side leg must not hide the headline result
+        log(f"2-D leg failed: {e!r}")
+        return {"error": repr(e)}
 
 
 def run_ensemble(args):

@@ -1,13 +1,14 @@
 !-----------------------------------------------------------------------------
 ! glissade_velo_higher_pcg_gpu.F90
 !
-! Drop-in Fortran entry points for the structured 3-D velocity solvers of
-! CISM's Glissade dycore, backed by the MI355X library libcismgpu.so through
-! its C ABI (include/cismgpu.h).  The public subroutines keep the names and
-! argument lists of module glissade_velo_higher_pcg
-! (glissade_velo_higher_pcg.F90:827-838, :2632-2643, :5528-5540) and the
-! Auu/Auv/Avu/Avv(27,nz,nx-1,ny-1) block layout, so a caller switches by
-! changing the module named in its use statement.
+! Drop-in Fortran entry points for the structured 3-D and 2-D velocity
+! solvers of CISM's Glissade dycore, backed by the MI355X library
+! libcismgpu.so through its C ABI (include/cismgpu.h).  The public
+! subroutines keep the names and argument lists of module
+! glissade_velo_higher_pcg (glissade_velo_higher_pcg.F90:67-78, :827-838,
+! :1541-1552, :2632-2643, :5528-5540, :5656-5663) and the
+! Auu/Auv/Avu/Avv(27,nz,nx-1,ny-1) and (nx-1,ny-1,9) block layouts, so a
+! caller switches by changing the module named in its use statement.
 !
 ! Arrays are passed by address (no copies on the host side).  A nonzero
 ! library status is reported through write_log(..., GM_FATAL), as the
@@ -25,6 +26,7 @@ module glissade_velo_higher_pcg_gpu
   implicit none
   private
   public :: pcg_solver_standard_3d, pcg_solver_chrongear_3d, matvec_multiply_structured_3d
+  public :: pcg_solver_standard_2d, pcg_solver_chrongear_2d, matvec_multiply_structured_2d
 
   ! mirror of cismgpu_grid
   type, bind(c) :: cismgpu_grid
@@ -72,6 +74,43 @@ module glissade_velo_higher_pcg_gpu
        real(c_double), intent(inout) :: yu(*), yv(*)
      end function c_matvec
 
+     integer(c_int32_t) function c_pcg_standard_2d(grid, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+          precond, ncheck, tolerance, maxiters, err, niters) bind(c, name='cismgpu_pcg_solver_standard_2d')
+       import :: c_int32_t, c_double, cismgpu_grid
+       type(cismgpu_grid), intent(in) :: grid
+       integer(c_int32_t), intent(in) :: active(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*)
+       real(c_double), intent(inout) :: xu(*), xv(*)
+       integer(c_int32_t), value :: precond, ncheck
+       real(c_double), value :: tolerance
+       integer(c_int32_t), value :: maxiters
+       real(c_double), intent(inout) :: err
+       integer(c_int32_t), intent(inout) :: niters
+     end function c_pcg_standard_2d
+
+     integer(c_int32_t) function c_pcg_chrongear_2d(grid, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+          precond, ncheck, tolerance, maxiters, err, niters) bind(c, name='cismgpu_pcg_solver_chrongear_2d')
+       import :: c_int32_t, c_double, cismgpu_grid
+       type(cismgpu_grid), intent(in) :: grid
+       integer(c_int32_t), intent(in) :: active(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), bu(*), bv(*)
+       real(c_double), intent(inout) :: xu(*), xv(*)
+       integer(c_int32_t), value :: precond, ncheck
+       real(c_double), value :: tolerance
+       integer(c_int32_t), value :: maxiters
+       real(c_double), intent(inout) :: err
+       integer(c_int32_t), intent(inout) :: niters
+     end function c_pcg_chrongear_2d
+
+     integer(c_int32_t) function c_matvec_2d(grid, active, Auu, Auv, Avu, Avv, xu, xv, yu, yv) &
+          bind(c, name='cismgpu_matvec_multiply_structured_2d')
+       import :: c_int32_t, c_double, cismgpu_grid
+       type(cismgpu_grid), intent(in) :: grid
+       integer(c_int32_t), intent(in) :: active(*)
+       real(c_double), intent(in) :: Auu(*), Auv(*), Avu(*), Avv(*), xu(*), xv(*)
+       real(c_double), intent(inout) :: yu(*), yv(*)
+     end function c_matvec_2d
+
      type(c_ptr) function c_last_error() bind(c, name='cismgpu_last_error')
        import :: c_ptr
      end function c_last_error
@@ -79,18 +118,11 @@ module glissade_velo_higher_pcg_gpu
 
 contains
 
-  ! Grid bounds and halo rule of the one task this library supports.
-  subroutine make_grid(nx, ny, nz, parallel, indxA_3d, grid)
-    integer, intent(in) :: nx, ny, nz
-    type(parallel_type), intent(in) :: parallel
+  ! The kernels use the neighbour numbering of glissade_velo_higher_init
+  ! (glissade_velo_higher.F90:607-640).
+  subroutine check_indxA_3d(indxA_3d)
     integer, dimension(-1:1,-1:1,-1:1), intent(in) :: indxA_3d
-    type(cismgpu_grid), intent(out) :: grid
     integer :: iA, jA, kA
-
-    if (parallel%tasks_row /= 1 .or. parallel%tasks_col /= 1) then
-       call write_log('GPU PCG solver: one task per row and column is supported', GM_FATAL)
-    endif
-    ! the kernels use the neighbour numbering of glissade_velo_higher_init
     do kA = -1, 1
        do jA = -1, 1
           do iA = -1, 1
@@ -100,6 +132,29 @@ contains
           enddo
        enddo
     enddo
+  end subroutine check_indxA_3d
+
+  subroutine check_indxA_2d(indxA_2d)
+    integer, dimension(-1:1,-1:1), intent(in) :: indxA_2d
+    integer :: iA, jA
+    do jA = -1, 1
+       do iA = -1, 1
+          if (indxA_2d(iA,jA) /= 1 + (iA+1) + 3*(jA+1)) then
+             call write_log('GPU PCG solver: unexpected indxA_2d ordering', GM_FATAL)
+          endif
+       enddo
+    enddo
+  end subroutine check_indxA_2d
+
+  ! Grid bounds and halo rule of the one task this library supports.
+  subroutine make_grid(nx, ny, nz, parallel, grid)
+    integer, intent(in) :: nx, ny, nz
+    type(parallel_type), intent(in) :: parallel
+    type(cismgpu_grid), intent(out) :: grid
+
+    if (parallel%tasks_row /= 1 .or. parallel%tasks_col /= 1) then
+       call write_log('GPU PCG solver: one task per row and column is supported', GM_FATAL)
+    endif
     grid%nx = nx
     grid%ny = ny
     grid%nz = nz
@@ -179,7 +234,8 @@ contains
     integer(c_int32_t) :: status, nit
     real(c_double) :: e
 
-    call make_grid(nx, ny, nz, parallel, indxA_3d, grid)
+    call check_indxA_3d(indxA_3d)
+    call make_grid(nx, ny, nz, parallel, grid)
     active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     e = ieee_value(e, ieee_quiet_nan)   ! left NaN when no convergence check ran
     nit = 0
@@ -224,7 +280,8 @@ contains
     integer(c_int32_t) :: status, nit
     real(c_double) :: e
 
-    call make_grid(nx, ny, nz, parallel, indxA_3d, grid)
+    call check_indxA_3d(indxA_3d)
+    call make_grid(nx, ny, nz, parallel, grid)
     active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     e = ieee_value(e, ieee_quiet_nan)   ! left NaN when no convergence check ran
     nit = 0
@@ -255,10 +312,125 @@ contains
     integer(c_int32_t), dimension(nx-1,ny-1) :: active
     integer(c_int32_t) :: status
 
-    call make_grid(nx, ny, nz, parallel, indxA_3d, grid)
+    call check_indxA_3d(indxA_3d)
+    call make_grid(nx, ny, nz, parallel, grid)
     active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
     status = c_matvec(grid, active, Auu, Auv, Avu, Avv, xu, xv, yu, yv)
     if (status /= 0) call report(status, 'matvec_multiply_structured_3d')
   end subroutine matvec_multiply_structured_3d
+
+  subroutine pcg_solver_standard_2d(nx,        ny,            &
+                                    parallel,                 &
+                                    indxA_2d,  active_vertex, &
+                                    Auu,       Auv,           &
+                                    Avu,       Avv,           &
+                                    bu,        bv,            &
+                                    xu,        xv,            &
+                                    precond,   linear_solve_ncheck,  &
+                                    tolerance, maxiters,      &
+                                    err,       niters,        &
+                                    itest, jtest, rtest)
+    integer, intent(in) :: nx, ny
+    type(parallel_type), intent(in) :: parallel
+    integer, dimension(-1:1,-1:1), intent(in) :: indxA_2d
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(nx-1,ny-1,9), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nx-1,ny-1), intent(in) :: bu, bv
+    real(dp), dimension(nx-1,ny-1), intent(inout) :: xu, xv
+    integer, intent(in) :: precond
+    integer, intent(in) :: linear_solve_ncheck
+    integer, intent(in) :: maxiters
+    real(dp), intent(in) :: tolerance
+    real(dp), intent(out) :: err
+    integer, intent(out) :: niters
+    integer, intent(in) :: itest, jtest, rtest
+
+    type(cismgpu_grid) :: grid
+    integer(c_int32_t), dimension(nx-1,ny-1) :: active
+    integer(c_int32_t) :: status, nit
+    real(c_double) :: e
+
+    call check_indxA_2d(indxA_2d)
+    call make_grid(nx, ny, 1, parallel, grid)
+    active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
+    e = ieee_value(e, ieee_quiet_nan)   ! left NaN when no convergence check ran
+    nit = 0
+    status = c_pcg_standard_2d(grid, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+                               int(precond, c_int32_t), int(linear_solve_ncheck, c_int32_t), &
+                               real(tolerance, c_double), int(maxiters, c_int32_t), e, nit)
+    if (status /= 0) call report(status, 'PCG solver')
+    err = e
+    niters = nit
+  end subroutine pcg_solver_standard_2d
+
+  subroutine pcg_solver_chrongear_2d(nx,        ny,            &
+                                     parallel,                 &
+                                     indxA_2d,  active_vertex, &
+                                     Auu,       Auv,           &
+                                     Avu,       Avv,           &
+                                     bu,        bv,            &
+                                     xu,        xv,            &
+                                     precond,   linear_solve_ncheck,  &
+                                     tolerance, maxiters,      &
+                                     err,       niters,        &
+                                     itest, jtest, rtest)
+    integer, intent(in) :: nx, ny
+    type(parallel_type), intent(in) :: parallel
+    integer, dimension(-1:1,-1:1), intent(in) :: indxA_2d
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(nx-1,ny-1,9), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nx-1,ny-1), intent(in) :: bu, bv
+    real(dp), dimension(nx-1,ny-1), intent(inout) :: xu, xv
+    integer, intent(in) :: precond
+    integer, intent(in) :: linear_solve_ncheck
+    integer, intent(in) :: maxiters
+    real(dp), intent(in) :: tolerance
+    real(dp), intent(out) :: err
+    integer, intent(out) :: niters
+    integer, intent(in) :: itest, jtest, rtest
+
+    type(cismgpu_grid) :: grid
+    integer(c_int32_t), dimension(nx-1,ny-1) :: active
+    integer(c_int32_t) :: status, nit
+    real(c_double) :: e
+
+    call check_indxA_2d(indxA_2d)
+    call make_grid(nx, ny, 1, parallel, grid)
+    active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
+    e = ieee_value(e, ieee_quiet_nan)   ! left NaN when no convergence check ran
+    nit = 0
+    status = c_pcg_chrongear_2d(grid, active, Auu, Auv, Avu, Avv, bu, bv, xu, xv, &
+                                int(precond, c_int32_t), int(linear_solve_ncheck, c_int32_t), &
+                                real(tolerance, c_double), int(maxiters, c_int32_t), e, nit)
+    if (status /= 0) call report(status, 'Chron-Gear PCG solver')
+    err = e
+    niters = nit
+  end subroutine pcg_solver_chrongear_2d
+
+  subroutine matvec_multiply_structured_2d(nx,            ny,            &
+                                           parallel,                     &
+                                           indxA_2d,      active_vertex, &
+                                           Auu,           Auv,           &
+                                           Avu,           Avv,           &
+                                           xu,            xv,            &
+                                           yu,            yv)
+    integer, intent(in) :: nx, ny
+    type(parallel_type), intent(in) :: parallel
+    integer, dimension(-1:1,-1:1), intent(in) :: indxA_2d
+    logical, dimension(nx-1,ny-1), intent(in) :: active_vertex
+    real(dp), dimension(nx-1,ny-1,9), intent(in) :: Auu, Auv, Avu, Avv
+    real(dp), dimension(nx-1,ny-1), intent(in) :: xu, xv
+    real(dp), dimension(nx-1,ny-1), intent(out) :: yu, yv
+
+    type(cismgpu_grid) :: grid
+    integer(c_int32_t), dimension(nx-1,ny-1) :: active
+    integer(c_int32_t) :: status
+
+    call check_indxA_2d(indxA_2d)
+    call make_grid(nx, ny, 1, parallel, grid)
+    active = merge(1_c_int32_t, 0_c_int32_t, active_vertex)
+    status = c_matvec_2d(grid, active, Auu, Auv, Avu, Avv, xu, xv, yu, yv)
+    if (status /= 0) call report(status, 'matvec_multiply_structured_2d')
+  end subroutine matvec_multiply_structured_2d
 
 end module glissade_velo_higher_pcg_gpu

@@ -273,3 +273,20 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
                   front_frac=front_frac, margin_frac=margin_frac),
     )
     return p
+
+
+def depth_integrated_system(Auu, Auv, Avu, Avv, bu, bv):
+    """2-D (9-point) system P^T A P of an assembled 3-D system, with P x(i,j) =
+    x on every level of column (i,j): block stencil entry (i,j; m2) = the sum
+    over k and kA of A(m2 + 9(kA+1), k, i, j); b2(i,j) = sum_k b(k,i,j).  It
+    is symmetric positive definite whenever A is, and carries the horizontal
+    coupling and conditioning of the assembled ice-flow problem, so it serves
+    as a synthetic depth-integrated (SSA-like) workload for the 2-D solvers.
+    Inputs in the reference layouts (27, nz, nx-1, ny-1) and (nz, nx-1, ny-1);
+    returns four (nx-1, ny-1, 9) stencils and two (nx-1, ny-1) vectors."""
+    out = []
+    for a in (Auu, Auv, Avu, Avv):
+        s = a.sum(axis=1)
+        s = s[0:9] + s[9:18] + s[18:27]
+        out.append(np.asfortranarray(np.moveaxis(s, 0, 2)))
+    return out, np.asfortranarray(bu.sum(axis=0)), np.asfortranarray(bv.sum(axis=0))

@@ -1,10 +1,12 @@
 ! Reads a system written by tests/test_fortran_shim.py, calls the drop-in
-! pcg_solver_{standard,chrongear}_3d exactly as glissade_velo_higher.F90
-! does (:3288-3316), and writes the solution, err and niters back.
+! pcg_solver_{standard,chrongear}_3d (nz > 0) or _2d (nz = 0 in the header)
+! exactly as glissade_velo_higher.F90 does (:3245-3271, :3288-3316), and
+! writes the solution, err and niters back.
 program shim_driver
   use glimmer_global, only: dp
   use cism_parallel, only: parallel_type
-  use glissade_velo_higher_pcg_gpu, only: pcg_solver_standard_3d, pcg_solver_chrongear_3d
+  use glissade_velo_higher_pcg_gpu, only: pcg_solver_standard_3d, pcg_solver_chrongear_3d, &
+                                          pcg_solver_standard_2d, pcg_solver_chrongear_2d
   implicit none
   integer :: hdr(12), nx, ny, nz, bc, algo, precond, ncheck, maxiters, niters, i, j, k, m, u
   real(dp) :: tol, err
@@ -14,6 +16,8 @@ program shim_driver
   logical, allocatable :: active_vertex(:,:)
   real(dp), allocatable :: Auu(:,:,:,:), Auv(:,:,:,:), Avu(:,:,:,:), Avv(:,:,:,:)
   real(dp), allocatable :: bu(:,:,:), bv(:,:,:), xu(:,:,:), xv(:,:,:)
+  real(dp), allocatable :: A2uu(:,:,:), A2uv(:,:,:), A2vu(:,:,:), A2vv(:,:,:)
+  real(dp), allocatable :: b2u(:,:), b2v(:,:), x2u(:,:), x2v(:,:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -29,10 +33,16 @@ program shim_driver
   parallel%outflow_bc = (bc == 1)
   parallel%no_ice_bc = (bc == 3)
   allocate(act(nx-1,ny-1), active_vertex(nx-1,ny-1))
-  allocate(Auu(27,nz,nx-1,ny-1), Auv(27,nz,nx-1,ny-1), Avu(27,nz,nx-1,ny-1), Avv(27,nz,nx-1,ny-1))
-  allocate(bu(nz,nx-1,ny-1), bv(nz,nx-1,ny-1), xu(nz,nx-1,ny-1), xv(nz,nx-1,ny-1))
   read(u) act
-  read(u) Auu, Auv, Avu, Avv, bu, bv, xu, xv
+  if (nz > 0) then
+     allocate(Auu(27,nz,nx-1,ny-1), Auv(27,nz,nx-1,ny-1), Avu(27,nz,nx-1,ny-1), Avv(27,nz,nx-1,ny-1))
+     allocate(bu(nz,nx-1,ny-1), bv(nz,nx-1,ny-1), xu(nz,nx-1,ny-1), xv(nz,nx-1,ny-1))
+     read(u) Auu, Auv, Avu, Avv, bu, bv, xu, xv
+  else
+     allocate(A2uu(nx-1,ny-1,9), A2uv(nx-1,ny-1,9), A2vu(nx-1,ny-1,9), A2vv(nx-1,ny-1,9))
+     allocate(b2u(nx-1,ny-1), b2v(nx-1,ny-1), x2u(nx-1,ny-1), x2v(nx-1,ny-1))
+     read(u) A2uu, A2uv, A2vu, A2vv, b2u, b2v, x2u, x2v
+  endif
   close(u)
   active_vertex = act /= 0
   ! neighbour numbering of glissade_velo_higher_init (glissade_velo_higher.F90:619-640)
@@ -52,7 +62,17 @@ program shim_driver
         indxA_2d(i,j) = m
      enddo
   enddo
-  if (algo == 3) then
+  if (nz == 0) then
+     if (algo == 3) then
+        call pcg_solver_chrongear_2d(nx, ny, parallel, indxA_2d, active_vertex, &
+                                     A2uu, A2uv, A2vu, A2vv, b2u, b2v, x2u, x2v, precond, ncheck, tol, &
+                                     maxiters, err, niters, 1, 1, 0)
+     else
+        call pcg_solver_standard_2d(nx, ny, parallel, indxA_2d, active_vertex, &
+                                    A2uu, A2uv, A2vu, A2vv, b2u, b2v, x2u, x2v, precond, ncheck, tol, &
+                                    maxiters, err, niters, 1, 1, 0)
+     endif
+  elseif (algo == 3) then
      call pcg_solver_chrongear_3d(nx, ny, nz, parallel, indxA_2d, indxA_3d, active_vertex, &
                                   Auu, Auv, Avu, Avv, bu, bv, xu, xv, precond, ncheck, tol, maxiters, &
                                   err, niters, 1, 1, 0)
@@ -64,6 +84,10 @@ program shim_driver
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) niters
   write(u) err
-  write(u) xu, xv
+  if (nz == 0) then
+     write(u) x2u, x2v
+  else
+     write(u) xu, xv
+  endif
   close(u)
 end program shim_driver

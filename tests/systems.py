@@ -121,16 +121,9 @@ def random_spd_system_2d(n=14, bc=BC_PERIODIC, seed=0):
     return g, act, A2, bu, bv
 
 
+
 def collapse_to_2d(g, act, A, bu, bv):
-    """Depth-integrated 2-D system P^T A P, P x(i,j) = x on every level of
-    column (i,j): the 9-point stencil of block b at (i,j) for horizontal
-    offset m2 is the sum over k and kA of A_b(m2 + 9(kA+1), k, i, j), and
-    b2(i,j) = sum_k b(k,i,j).  Symmetric positive definite whenever A is, so
-    the assembled ice-dynamics systems give 2-D test systems with their
-    conditioning.  Returns A2 list of (nvx, nvy, 9) and (nvx, nvy) vectors."""
-    A2 = []
-    for a in A:
-        s = a.sum(axis=1)                       # over k: (27, nvx, nvy)
-        s = s[0:9] + s[9:18] + s[18:27]          # over kA
-        A2.append(np.asfortranarray(np.moveaxis(s, 0, 2)))
-    return A2, np.asfortranarray(bu.sum(axis=0)), np.asfortranarray(bv.sum(axis=0))
+    """Depth-integrated 2-D form P^T A P of an assembled 3-D system (see
+    cism_amd.synthetic.depth_integrated_system)."""
+    from cism_amd.synthetic import depth_integrated_system
+    return depth_integrated_system(*A, bu, bv)

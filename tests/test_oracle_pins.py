@@ -687,3 +687,32 @@ def test_2d_tridiagonal_preconditioner_first_step():
     assert st == 0
     x = to_owned_2d(g, xu, xv)
     assert np.linalg.norm(x - alpha * z) <= 1e-11 * np.linalg.norm(alpha * z)
+
+
+def test_depth_integrated_system_is_projected_operator():
+    """depth_integrated_system gives P^T A P: for x2 spread over every level,
+    the level sum of the 3-D matvec equals the 2-D matvec of the collapsed
+    stencils (both oracle matvecs are pinned above)."""
+    from cism_amd.synthetic import depth_integrated_system
+    p, s = dome_system()
+    g, act = p.grid, s["active_vertex"]
+    A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+    A2, bu2, bv2 = depth_integrated_system(*A, s["bu"], s["bv"])
+    rng = np.random.default_rng(11)
+    xu2 = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)))
+    xv2 = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)))
+    xu3 = np.asfortranarray(np.broadcast_to(xu2, (g.nz, g.nvx, g.nvy)))
+    xv3 = np.asfortranarray(np.broadcast_to(xv2, (g.nz, g.nvx, g.nvy)))
+    yu3, yv3 = po.matvec(g, act, *A, xu3, xv3)
+    yu2, yv2 = po.matvec_2d(g, act, *A2, xu2, xv2)
+    sc = np.abs(yu3).sum(axis=0).max()
+    assert np.allclose(yu3.sum(axis=0), yu2, rtol=0, atol=1e-12 * sc)
+    assert np.allclose(yv3.sum(axis=0), yv2, rtol=0, atol=1e-12 * sc)
+    assert np.array_equal(bu2, s["bu"].sum(axis=0))
+    # symmetric: the (i,j)->(i+iA,j+jA) entry equals the transpose entry
+    own = slice(g.ilo, g.ihi - 1)
+    for iA, jA in ((1, 0), (0, 1), (1, 1), (1, -1)):
+        m, mt = 4 + iA + 3 * jA, 4 - iA - 3 * jA
+        a = A2[0][own, own, m]
+        b = A2[0][own.start + iA:own.stop + iA, own.start + jA:own.stop + jA, mt]
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(A2[0]).max())
