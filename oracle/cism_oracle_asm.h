@@ -58,6 +58,27 @@ void orc_grounded_fraction_no_glp(int nx, int ny, const int32_t* ice_mask, const
                                   const int32_t* land_mask, double* f_ground);
 double orc_residual_velocity_3d(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int whichresid,
                                 const double* usav, const double* vsav, const double* uvel, const double* vvel);
+void orc_vertical_average(int nx, int ny, int nz, const double* sigma, const double* var, double* var_2d);
+int orc_element_matrix_2d_ssa(const double* x, const double* y, const double* u, const double* v, const double* h,
+                              double flwafact_2d, int whichefvs, double efvs_constant, double effstrain_min,
+                              double n_glen, double Kuu[4][4], double Kuv[4][4], double Kvu[4][4], double Kvv[4][4],
+                              double* efvs_mean);
+int orc_assemble_stiffness_matrix_2d_ssa(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                         const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                         const double* uvel_2d, const double* vvel_2d, const double* stagthck,
+                                         const double* flwafact, int whichefvs, double efvs_constant,
+                                         double effstrain_min, double n_glen, double* efvs,
+                                         double* Auu, double* Auv, double* Avu, double* Avv);
+void orc_basal_sliding_bc_2d_local(int nx, int ny, double dx, double dy, const int32_t* active_vertex,
+                                   const double* beta, double* Auu, double* Avv);
+void orc_dirichlet_bc_2d(int nx, int ny, int nhalo, const int32_t* active_vertex, const int32_t* umask,
+                         const int32_t* vmask, const double* uvel, const double* vvel,
+                         double* Auu, double* Auv, double* Avu, double* Avv, double* bu, double* bv);
+int64_t orc_check_symmetry_assembled_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi,
+                                        const int32_t* active_vertex, const double* Auu, const double* Auv,
+                                        const double* Avu, const double* Avv);
+int64_t orc_count_nonzeros_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, const int32_t* active_vertex,
+                              const double* Auu, const double* Auv, const double* Avu, const double* Avv);
 
 #ifdef __cplusplus
 }

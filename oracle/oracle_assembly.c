@@ -741,3 +741,297 @@ void orc_grounded_fraction_no_glp(int nx, int ny, const int32_t* ice_mask, const
         }
 #undef CG_
 }
+
+/* ------------------------------------------------------------------ */
+/* 2-D (depth-integrated) assembly, SSA approximation                   */
+/* ------------------------------------------------------------------ */
+
+/* 2-D reference element (glissade_velo_higher.F90:528-570): 2x2 Gauss
+ * points (-r,-r), (r,-r), (r,r), (-r,r), weights 1; bilinear phi_2d and its
+ * reference derivatives, stored [n][p]. */
+static double phi2[4][4], dxr2[4][4], dyr2[4][4];
+static int tables2_ready = 0;
+/* node n of the element in cell (i,j) sits at vertex (i + di2[n], j + dj2[n])
+ * (ishift(3,n), jshift(3,n), :461-494) */
+static const int di2[4] = {-1, 0, 0, -1}, dj2[4] = {-1, -1, 0, 0};
+#define IDX2(iA, jA) (1 + ((iA) + 1) + 3 * ((jA) + 1))
+#define A2(m, i, j) (V2(i, j) + (size_t)NVX * NVY * (size_t)((m) - 1))
+
+static void init_tables_2d(void)
+{
+    if (tables2_ready) return;
+    const double r = 1.0 / sqrt(3.0);
+    const double xq[4] = {-r, r, r, -r}, yq[4] = {-r, -r, r, r};
+    for (int p = 0; p < 4; ++p) {
+        phi2[0][p] = (1.0 - xq[p]) * (1.0 - yq[p]) / 4.0;
+        phi2[1][p] = (1.0 + xq[p]) * (1.0 - yq[p]) / 4.0;
+        phi2[2][p] = (1.0 + xq[p]) * (1.0 + yq[p]) / 4.0;
+        phi2[3][p] = (1.0 - xq[p]) * (1.0 + yq[p]) / 4.0;
+        dxr2[0][p] = -(1.0 - yq[p]) / 4.0;
+        dxr2[1][p] = (1.0 - yq[p]) / 4.0;
+        dxr2[2][p] = (1.0 + yq[p]) / 4.0;
+        dxr2[3][p] = -(1.0 + yq[p]) / 4.0;
+        dyr2[0][p] = -(1.0 - xq[p]) / 4.0;
+        dyr2[1][p] = -(1.0 + xq[p]) / 4.0;
+        dyr2[2][p] = (1.0 + xq[p]) / 4.0;
+        dyr2[3][p] = (1.0 - xq[p]) / 4.0;
+    }
+    tables2_ready = 1;
+}
+
+/* get_basis_function_derivatives_2d (glissade_velo_higher.F90:6319-6504) */
+static int basis_derivs_2d(const double* x, const double* y, int p, double* dx, double* dy, double* detJ)
+{
+    double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+    for (int n = 0; n < 4; ++n) {
+        J11 = J11 + dxr2[n][p] * x[n];
+        J12 = J12 + dxr2[n][p] * y[n];
+        J21 = J21 + dyr2[n][p] * x[n];
+        J22 = J22 + dyr2[n][p] * y[n];
+    }
+    const double d = J11 * J22 - J12 * J21;
+    *detJ = d;
+    if (!(fabs(d) > 0.0)) return ORC_ERR_SINGULAR;
+    const double i11 = J22 / d, i12 = -J12 / d, i21 = -J21 / d, i22 = J11 / d;
+    for (int n = 0; n < 4; ++n) {
+        dx[n] = 0.0 + i11 * dxr2[n][p] + i12 * dyr2[n][p];
+        dy[n] = 0.0 + i21 * dxr2[n][p] + i22 * dyr2[n][p];
+    }
+    return ORC_OK;
+}
+
+/* glissade_vertical_average, no mask (glissade_grid_operators.F90:1882-1946):
+ * var (nz-1, nx, ny) layers -> var_2d (nx, ny) */
+void orc_vertical_average(int nx, int ny, int nz, const double* sigma, const double* var, double* var_2d)
+{
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            double s = 0.0;
+            for (int k = 1; k <= nz - 1; ++k) s = s + var[E3(k, i, j)] * (sigma[k] - sigma[k - 1]);
+            var_2d[C2(i, j)] = s;
+        }
+}
+
+/* compute_effective_viscosity, SSA branch (glissade_velo_higher.F90:7285-7470) */
+static double effective_viscosity_ssa(int whichefvs, double efvs_constant, double effstrain_min, double n_glen,
+                                      const double* dx, const double* dy, const double* u, const double* v,
+                                      double flwafact)
+{
+    const double p_effstr = (1.0 - n_glen) / n_glen;
+    if (whichefvs == 0) return efvs_constant;
+    if (whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
+        return flwafact * pow(effstrain, p_effstr);
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0;
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+    }
+    const double esq = effstrain_min * effstrain_min + (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+    return flwafact * pow(esq, p_effstr / 2.0);
+}
+
+/* One SSA element: quadrature loop of assemble_stiffness_matrix_2d
+ * (glissade_velo_higher.F90:5216-5337) with compute_element_matrix (SSA
+ * branch, :8001-8123): factor = (h_qp efvs_qp) w detJ / vol0, h_qp =
+ * sum_n phi_2d(n,p) h(n).  efvs_mean = mean over the 4 points (:5310-5318).
+ * K are [nr][nc]. */
+int orc_element_matrix_2d_ssa(const double* x, const double* y, const double* u, const double* v, const double* h,
+                              double flwafact_2d, int whichefvs, double efvs_constant, double effstrain_min,
+                              double n_glen, double Kuu[4][4], double Kuv[4][4], double Kvu[4][4], double Kvv[4][4],
+                              double* efvs_mean)
+{
+    init_tables_2d();
+    double dx[4], dy[4], detJ, efvs_qp[4];
+    memset(Kuu, 0, 16 * sizeof(double)); memset(Kuv, 0, 16 * sizeof(double));
+    memset(Kvu, 0, 16 * sizeof(double)); memset(Kvv, 0, 16 * sizeof(double));
+    for (int p = 0; p < 4; ++p) {
+        int st = basis_derivs_2d(x, y, p, dx, dy, &detJ);
+        if (st) return st;
+        efvs_qp[p] = effective_viscosity_ssa(whichefvs, efvs_constant, effstrain_min, n_glen, dx, dy, u, v,
+                                             flwafact_2d);
+        double h_qp = 0.0;
+        for (int n = 0; n < 4; ++n) h_qp = h_qp + phi2[n][p] * h[n];
+        const double f = (h_qp * efvs_qp[p]) * 1.0 * detJ / VOL0;
+        for (int nc = 0; nc < 4; ++nc)
+            for (int nr = 0; nr < 4; ++nr) {
+                Kuu[nr][nc] = Kuu[nr][nc] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc]);
+                Kuv[nr][nc] = Kuv[nr][nc] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                Kvu[nr][nc] = Kvu[nr][nc] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                Kvv[nr][nc] = Kvv[nr][nc] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc]);
+            }
+    }
+    double s = 0.0;
+    for (int p = 0; p < 4; ++p) s = s + efvs_qp[p];
+    *efvs_mean = s / 4.0;
+    /* check_symmetry_element_matrix (:10583-10644) */
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            if (fabs(Kuu[a][b] - Kuu[b][a]) > EPS11) return ORC_ERR_ASYM;
+            if (fabs(Kvv[a][b] - Kvv[b][a]) > EPS11) return ORC_ERR_ASYM;
+            if (fabs(Kuv[a][b] - Kvu[b][a]) > EPS11) return ORC_ERR_ASYM;
+        }
+    return ORC_OK;
+}
+
+/* assemble_stiffness_matrix_2d, SSA (glissade_velo_higher.F90:4961-5381)
+ * with element_to_global_matrix_2d (:8127-8196).  flwafact_2d is the
+ * vertical average of flwafact (:5187-5192); efvs(k,i,j) takes the element
+ * mean for every layer k.  A are (nx-1, ny-1, 9), m last. */
+int orc_assemble_stiffness_matrix_2d_ssa(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                         const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                         const double* uvel_2d, const double* vvel_2d, const double* stagthck,
+                                         const double* flwafact, int whichefvs, double efvs_constant,
+                                         double effstrain_min, double n_glen, double* efvs,
+                                         double* Auu, double* Auv, double* Avu, double* Avv)
+{
+    init_tables_2d();
+    const size_t nA = (size_t)9 * NVX * NVY;
+    memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    memset(Auu, 0, nA * sizeof(double)); memset(Auv, 0, nA * sizeof(double));
+    memset(Avu, 0, nA * sizeof(double)); memset(Avv, 0, nA * sizeof(double));
+    double* fl2 = (double*)malloc((size_t)nx * ny * sizeof(double));
+    orc_vertical_average(nx, ny, nz, sigma, flwafact, fl2);
+    double Kuu[4][4], Kuv[4][4], Kvu[4][4], Kvv[4][4];
+    double x[4], y[4], u[4], v[4], h[4];
+    int st = ORC_OK;
+    for (int j = nhalo + 1; j <= ny - nhalo + 1 && !st; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1 && !st; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int n = 0; n < 4; ++n) {
+                const int in = i + di2[n], jn = j + dj2[n];
+                x[n] = xVertex[V2(in, jn)];
+                y[n] = yVertex[V2(in, jn)];
+                u[n] = uvel_2d[V2(in, jn)];
+                v[n] = vvel_2d[V2(in, jn)];
+                h[n] = stagthck[V2(in, jn)];
+            }
+            double em;
+            st = orc_element_matrix_2d_ssa(x, y, u, v, h, fl2[C2(i, j)], whichefvs, efvs_constant, effstrain_min,
+                                           n_glen, Kuu, Kuv, Kvu, Kvv, &em);
+            if (st) break;
+            for (int k = 1; k <= nz - 1; ++k) efvs[E3(k, i, j)] = em;
+            for (int nr = 0; nr < 4; ++nr) {
+                const int ir = i + di2[nr], jr = j + dj2[nr];
+                for (int nc = 0; nc < 4; ++nc) {
+                    const int m = IDX2(di2[nc] - di2[nr], dj2[nc] - dj2[nr]);
+                    const size_t a = A2(m, ir, jr);
+                    Auu[a] = Auu[a] + Kuu[nr][nc];
+                    Auv[a] = Auv[a] + Kuv[nr][nc];
+                    Avu[a] = Avu[a] + Kvu[nr][nc];
+                    Avv[a] = Avv[a] + Kvv[nr][nc];
+                }
+            }
+        }
+    free(fl2);
+    return st;
+}
+
+/* basal_sliding_bc_2d, local option (glissade_velo_higher.F90:8281-8531):
+ * diag += dx dy / vol0 * beta on active vertices (no slope factor in 2-D) */
+void orc_basal_sliding_bc_2d_local(int nx, int ny, double dx, double dy, const int32_t* active_vertex,
+                                   const double* beta, double* Auu, double* Avv)
+{
+    const int m = IDX2(0, 0);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            const double inc = (dx * dy / VOL0) * beta[V2(i, j)];
+            Auu[A2(m, i, j)] = Auu[A2(m, i, j)] + inc;
+            Avv[A2(m, i, j)] = Avv[A2(m, i, j)] + inc;
+        }
+}
+
+/* dirichlet_boundary_conditions_2d (glissade_velo_higher.F90:9133-9301),
+ * loops (jA, iA) outside (j, i) as the reference; masks are the basal level
+ * of the 3-D masks (:2652-2657).  A u-prescribed row sets the diagonal of
+ * Auu, Auv and Avu to 1 (:9206-9210), as the reference does. */
+void orc_dirichlet_bc_2d(int nx, int ny, int nhalo, const int32_t* active_vertex, const int32_t* umask,
+                         const int32_t* vmask, const double* uvel, const double* vvel,
+                         double* Auu, double* Auv, double* Avu, double* Avv, double* bu, double* bv)
+{
+    const int md = IDX2(0, 0);
+    for (int jA = -1; jA <= 1; ++jA)
+        for (int iA = -1; iA <= 1; ++iA) {
+            const int m = IDX2(iA, jA), mm = IDX2(-iA, -jA);
+            for (int j = nhalo; j <= ny - nhalo + 1; ++j)
+                for (int i = nhalo; i <= nx - nhalo + 1; ++i) {
+                    if (!active_vertex[V2(i, j)]) continue;
+                    const int inb = (i + iA >= 1 && i + iA <= nx - 1) && (j + jA >= 1 && j + jA <= ny - 1);
+                    if (umask[V2(i, j)] == 1) {
+                        bu[V2(i, j)] = uvel[V2(i, j)];
+                        if (inb) {
+                            if (iA == 0 && jA == 0) {
+                                Auu[A2(md, i, j)] = 1.0; Auv[A2(md, i, j)] = 1.0; Avu[A2(md, i, j)] = 1.0;
+                            } else {
+                                Auu[A2(m, i, j)] = 0.0; Auv[A2(m, i, j)] = 0.0;
+                                const size_t nb = V2(i + iA, j + jA);
+                                const size_t a = A2(mm, i + iA, j + jA);
+                                if (umask[nb] != 1) { bu[nb] = bu[nb] - Auu[a] * uvel[V2(i, j)]; Auu[a] = 0.0; }
+                                if (vmask[nb] != 1) { bv[nb] = bv[nb] - Avu[a] * uvel[V2(i, j)]; Avu[a] = 0.0; }
+                            }
+                        }
+                    }
+                    if (vmask[V2(i, j)] == 1) {
+                        bv[V2(i, j)] = vvel[V2(i, j)];
+                        if (inb) {
+                            if (iA == 0 && jA == 0) {
+                                Auv[A2(md, i, j)] = 0.0; Avu[A2(md, i, j)] = 0.0; Avv[A2(md, i, j)] = 1.0;
+                            } else {
+                                Avu[A2(m, i, j)] = 0.0; Avv[A2(m, i, j)] = 0.0;
+                                const size_t nb = V2(i + iA, j + jA);
+                                const size_t a = A2(mm, i + iA, j + jA);
+                                if (umask[nb] != 1) { bu[nb] = bu[nb] - Auv[a] * vvel[V2(i, j)]; Auv[a] = 0.0; }
+                                if (vmask[nb] != 1) { bv[nb] = bv[nb] - Avv[a] * vvel[V2(i, j)]; Avv[a] = 0.0; }
+                            }
+                        }
+                    }
+                }
+        }
+}
+
+/* check_symmetry_assembled_matrix_2d (glissade_velo_higher.F90:10648-10850):
+ * the 2-D check only reports (its averaging lines are commented out in the
+ * reference); returns the number of pairs that differ by eps11 |diag| or
+ * more.  The partner entry is read at (i+iA, j+jA) as the reference does. */
+int64_t orc_check_symmetry_assembled_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi,
+                                        const int32_t* active_vertex, const double* Auu, const double* Auv,
+                                        const double* Avu, const double* Avv)
+{
+    int64_t nwarn = 0;
+    for (int jA = -1; jA <= 1; ++jA)
+        for (int iA = -1; iA <= 1; ++iA) {
+            const int m = IDX2(iA, jA), mm = IDX2(-iA, -jA);
+            for (int j = jlo; j <= jhi; ++j)
+                for (int i = ilo; i <= ihi; ++i) {
+                    if (!active_vertex[V2(i, j)]) continue;
+                    const double du = Auu[A2(IDX2(0, 0), i, j)], dv = Avv[A2(IDX2(0, 0), i, j)];
+                    const double pairs[4][3] = {
+                        {Auu[A2(m, i, j)], Auu[A2(mm, i + iA, j + jA)], du},
+                        {Auv[A2(m, i, j)], Avu[A2(mm, i + iA, j + jA)], du},
+                        {Avv[A2(m, i, j)], Avv[A2(mm, i + iA, j + jA)], dv},
+                        {Avu[A2(m, i, j)], Auv[A2(mm, i + iA, j + jA)], dv}};
+                    for (int s = 0; s < 4; ++s)
+                        if (pairs[s][1] != pairs[s][0] && !(fabs(pairs[s][1] - pairs[s][0]) < EPS11 * fabs(pairs[s][2])))
+                            ++nwarn;
+                }
+        }
+    return nwarn;
+}
+
+/* count_nonzeros_2d (glissade_velo_higher.F90:10450-10509) */
+int64_t orc_count_nonzeros_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, const int32_t* active_vertex,
+                              const double* Auu, const double* Auv, const double* Avu, const double* Avv)
+{
+    int64_t n = 0;
+    for (int m = 1; m <= 9; ++m)
+        for (int j = jlo; j <= jhi; ++j)
+            for (int i = ilo; i <= ihi; ++i) {
+                if (!active_vertex[V2(i, j)]) continue;
+                n += (Auu[A2(m, i, j)] != 0.0) + (Auv[A2(m, i, j)] != 0.0) + (Avu[A2(m, i, j)] != 0.0) +
+                     (Avv[A2(m, i, j)] != 0.0);
+            }
+    return n;
+}

@@ -85,6 +85,18 @@ def _declare(L):
     L.orc_stagger_masked.argtypes = [i, i, _D, _I, _D]
     L.orc_load_vector_lateral_bc.argtypes = [i, i, i, _D, i, i, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D]
     L.orc_grounded_fraction_no_glp.argtypes = [i, i, _I, _I, _I, _D]
+    L.orc_vertical_average.argtypes = [i, i, i, _D, _D, _D]
+    L.orc_element_matrix_2d_ssa.argtypes = [_D, _D, _D, _D, _D, d, i, d, d, d, _D, _D, _D, _D, _D]
+    L.orc_element_matrix_2d_ssa.restype = ctypes.c_int
+    L.orc_assemble_stiffness_matrix_2d_ssa.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, _D, i, d, d, d,
+                                                       _D, _D, _D, _D, _D]
+    L.orc_assemble_stiffness_matrix_2d_ssa.restype = ctypes.c_int
+    L.orc_basal_sliding_bc_2d_local.argtypes = [i, i, d, d, _I, _D, _D, _D]
+    L.orc_dirichlet_bc_2d.argtypes = [i, i, i, _I, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_check_symmetry_assembled_2d.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D]
+    L.orc_check_symmetry_assembled_2d.restype = ctypes.c_int64
+    L.orc_count_nonzeros_2d.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D]
+    L.orc_count_nonzeros_2d.restype = ctypes.c_int64
 
 
 def dp(a: np.ndarray):
@@ -353,6 +365,90 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     return dict(Auu=A[0], Auv=A[1], Avu=A[2], Avv=A[3], bu=bu, bv=bv, efvs=efvs, uvel=uvel, vvel=vvel,
                 active_vertex=av, loadu=loadu, loadv=loadv, umask=umask, vmask=vmask,
                 beta_internal=binternal, nwarn_asym=nwarn, nnz=nnz, geo=geo)
+
+
+def element_matrix_2d_ssa(x, y, u, v, h, flwafact_2d, whichefvs=2, efvs_constant=0.0, effstrain_min=1e-8,
+                          n_glen=3.0):
+    L = lib()
+    K = [np.zeros((4, 4)) for _ in range(4)]   # C order [nr][nc]
+    em = ctypes.c_double()
+    f = lambda a: dp(np.ascontiguousarray(a, dtype=np.float64))
+    st = L.orc_element_matrix_2d_ssa(f(x), f(y), f(u), f(v), f(h), flwafact_2d, whichefvs, efvs_constant,
+                                     effstrain_min, n_glen, *[dp(k) for k in K], ctypes.byref(em))
+    return st, K, em.value
+
+
+def vertical_average(sigma, var):
+    """glissade_vertical_average of a layer field (nz-1, nx, ny)."""
+    L = lib()
+    nz = len(sigma)
+    _, nx, ny = var.shape
+    out = np.zeros((nx, ny), order="F")
+    L.orc_vertical_average(nx, ny, nz, dp(np.ascontiguousarray(sigma, dtype=np.float64)),
+                           dp(np.asfortranarray(var)), dp(out))
+    return out
+
+
+def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None):
+    """One assembly of the 2-D SSA system in the reference's order
+    (glissade_velo_higher.F90:2353-2725): stiffness (assemble_stiffness_matrix_2d),
+    local basal BC (basal_sliding_bc_2d), b = the level sum of the 3-D load
+    (:2045-2058, :2636), Dirichlet rows with the basal-level masks
+    (dirichlet_boundary_conditions_2d), halo of A and b, symmetry report and
+    nonzero count.  The default 2-D velocity is the basal level of the 3-D
+    guess (:1362)."""
+    L = lib()
+    p = problem
+    g = p.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    geo = geo or geometry(p)
+    av = geo["active_vertex"]
+    u2 = np.asfortranarray(p.uvel[nz - 1].copy() if uvel_2d is None else uvel_2d, dtype=np.float64)
+    v2 = np.asfortranarray(p.vvel[nz - 1].copy() if vvel_2d is None else vvel_2d, dtype=np.float64)
+    xV, yV = p.vertex_coords()
+    # 3-D load (gravity + lateral), summed over levels in k order
+    loadu = g.zeros_vec(); loadv = g.zeros_vec()
+    from cism_amd.grid import stagwbndsigma
+    swb = stagwbndsigma(p.sigma)
+    L.orc_load_vector_gravity_local(nx, ny, nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
+                                    dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+    sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
+    L.orc_load_vector_lateral_bc(nx, ny, nz, dp(sigma), 2, int(p.assemble_lateral), ip(geo["ocean_mask"]),
+                                 ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["usrf"]), dp(p.thck),
+                                 dp(geo["stagusrf_lateral"]), dp(geo["stagthck_lateral"]), dp(loadu), dp(loadv))
+    lu2 = np.zeros((nx - 1, ny - 1), order="F"); lv2 = np.zeros((nx - 1, ny - 1), order="F")
+    for k in range(nz):
+        lu2 = lu2 + loadu[k]
+        lv2 = lv2 + loadv[k]
+    A = [np.zeros((nx - 1, ny - 1, 9), order="F") for _ in range(4)]
+    efvs = np.zeros((nz - 1, nx, ny), order="F")
+    st = L.orc_assemble_stiffness_matrix_2d_ssa(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                                dp(u2), dp(v2), dp(geo["stagthck"]), dp(geo["flwafact"]),
+                                                p.whichefvs, p.efvs_constant, p.effstrain_min, p.n_glen, dp(efvs),
+                                                *[dp(a) for a in A])
+    if st:
+        raise RuntimeError(f"oracle 2-D assembly failed with status {st}")
+    binternal = beta_internal(p, geo)
+    if p.whichbabc != 6:
+        L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
+    bu = np.asfortranarray(lu2.copy()); bv = np.asfortranarray(lv2.copy())
+    umask = np.zeros((nx - 1, ny - 1), np.int32, order="F")
+    if p.whichbabc == 6:
+        umask[:, :] = 1
+    vmask = umask.copy(order="F")
+    L.orc_dirichlet_bc_2d(nx, ny, 2, ip(av), ip(umask), ip(vmask), dp(u2), dp(v2), *[dp(a) for a in A],
+                          dp(bu), dp(bv))
+    for a in A:
+        for m in range(9):
+            sl = a[:, :, m]
+            halo(g, sl)
+    halo(g, bu)
+    halo(g, bv)
+    nwarn = L.orc_check_symmetry_assembled_2d(nx, ny, g.ilo, g.ihi, g.jlo, g.jhi, ip(av), *[dp(a) for a in A])
+    nnz = L.orc_count_nonzeros_2d(nx, ny, g.ilo, g.ihi, g.jlo, g.jhi, ip(av), *[dp(a) for a in A])
+    return dict(Auu=A[0], Auv=A[1], Avu=A[2], Avv=A[3], bu=bu, bv=bv, efvs=efvs, uvel_2d=u2, vvel_2d=v2,
+                active_vertex=av, loadu_2d=lu2, loadv_2d=lv2, beta_internal=binternal, nwarn_asym=nwarn,
+                nnz=nnz, geo=geo)
 
 
 def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):

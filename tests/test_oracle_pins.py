@@ -716,3 +716,170 @@ def test_depth_integrated_system_is_projected_operator():
         a = A2[0][own, own, m]
         b = A2[0][own.start + iA:own.stop + iA, own.start + jA:own.stop + jA, mt]
         assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(A2[0]).max())
+
+
+# ----------------------------------------------------------- 2-D SSA assembly
+def _exact_rect_integrals(a, b, npts=6):
+    """Exact integrals over [0,a]x[0,b] of products of bilinear-basis
+    derivatives (nodes (0,0), (a,0), (a,b), (0,b)), by a 6-point
+    Gauss-Legendre rule (exact for these polynomials)."""
+    xg, wg = np.polynomial.legendre.leggauss(npts)
+    X = 0.5 * a * (xg + 1.0)
+    Y = 0.5 * b * (xg + 1.0)
+    W = np.outer(wg, wg) * (0.25 * a * b)
+    xx, yy = np.meshgrid(X, Y, indexing="ij")
+    sx = np.array([-1, 1, 1, -1]) / a
+    sy = np.array([-1, -1, 1, 1]) / b
+    fx = [(xx if s > 0 else a - xx) / a for s in (-1, 1, 1, -1)]
+    fy = [(yy if s > 0 else b - yy) / b for s in (-1, -1, 1, 1)]
+    phix = [sx[n] * fy[n] for n in range(4)]     # d phi_n / dx
+    phiy = [sy[n] * fx[n] for n in range(4)]     # d phi_n / dy
+    I = lambda f, g_: float((f * g_ * W).sum())
+    Ixx = np.array([[I(phix[r], phix[c]) for c in range(4)] for r in range(4)])
+    Iyy = np.array([[I(phiy[r], phiy[c]) for c in range(4)] for r in range(4)])
+    Ixy = np.array([[I(phix[r], phiy[c]) for c in range(4)] for r in range(4)])
+    return Ixx, Iyy, Ixy
+
+
+@pytest.mark.parametrize("a,b", [(1000.0, 1000.0), (2000.0, 500.0)])
+def test_ssa_element_matrix_is_exact_weak_form_on_rectangles(a, b):
+    """SSA weak form (MacAyeal 1989): K_uu = int eta h (4 phi_x phi_x + phi_y phi_y),
+    K_uv = int eta h (2 phi_x phi_y' + phi_y phi_x'), ...; the 2x2 Gauss rule is
+    exact for bilinear elements on rectangles, so the element matrix must equal
+    the exact integrals (scaled by 1/vol0)."""
+    x = np.array([0.0, a, a, 0.0]) + 3000.0
+    y = np.array([0.0, 0.0, b, b]) - 7000.0
+    h = np.full(4, 800.0)
+    eta = 3.0e13
+    st, (Kuu, Kuv, Kvu, Kvv), em = po.element_matrix_2d_ssa(x, y, np.zeros(4), np.zeros(4), h, 0.0, whichefvs=0,
+                                                            efvs_constant=eta)
+    assert st == 0 and em == eta
+    Ixx, Iyy, Ixy = _exact_rect_integrals(a, b)
+    f = eta * 800.0 / 1.0e9
+    sc = f * max(np.abs(Ixx).max(), np.abs(Iyy).max())
+    assert np.allclose(Kuu, f * (4 * Ixx + Iyy), rtol=0, atol=1e-13 * sc)
+    assert np.allclose(Kvv, f * (4 * Iyy + Ixx), rtol=0, atol=1e-13 * sc)
+    assert np.allclose(Kuv, f * (2 * Ixy + Ixy.T), rtol=0, atol=1e-13 * sc)
+    assert np.allclose(Kvu, f * (2 * Ixy.T + Ixy), rtol=0, atol=1e-13 * sc)
+
+
+def test_ssa_element_rotation_covariance_and_nonlinear_viscosity():
+    """On a distorted quad with a linear velocity field: efvs is Glen's law of
+    the SSA effective strain rate u_x^2 + v_y^2 + u_x v_y + (u_y + v_x)^2 / 4;
+    rotating coordinates and velocities by 90 degrees maps the blocks as
+    K'uu = Kvv, K'vv = Kuu, K'uv = -Kvu, K'vu = -Kuv."""
+    x = np.array([0.0, 1100.0, 1250.0, -80.0])
+    y = np.array([0.0, 60.0, 900.0, 1010.0])
+    A_, B_, C_, D_ = 2.0e-3, -1.0e-3, 5.0e-4, -3.0e-3    # u = A x + B y, v = C x + D y (1/yr)
+    u = A_ * x + B_ * y
+    v = C_ * x + D_ * y
+    h = np.array([900.0, 950.0, 1000.0, 1020.0])
+    fl, emin, n = 1.7e5, 1.0e-8, 3.0
+    st, K, em = po.element_matrix_2d_ssa(x, y, u, v, h, fl, whichefvs=2, effstrain_min=emin, n_glen=n)
+    assert st == 0
+    esq = emin ** 2 + A_ ** 2 + D_ ** 2 + A_ * D_ + 0.25 * (B_ + C_) ** 2
+    assert em == pytest.approx(fl * esq ** ((1.0 - n) / (2.0 * n)), rel=1e-13)
+    st, Kr, emr = po.element_matrix_2d_ssa(-y, x, -v, u, h, fl, whichefvs=2, effstrain_min=emin, n_glen=n)
+    assert st == 0 and emr == pytest.approx(em, rel=1e-13)
+    sc = max(np.abs(k).max() for k in K)
+    assert np.allclose(Kr[0], K[3], atol=1e-12 * sc) and np.allclose(Kr[3], K[0], atol=1e-12 * sc)
+    assert np.allclose(Kr[1], -K[2], atol=1e-12 * sc) and np.allclose(Kr[2], -K[1], atol=1e-12 * sc)
+    # a degenerate (collapsed) element is a singular Jacobian
+    st, _, _ = po.element_matrix_2d_ssa(np.zeros(4), y, u, v, h, fl)
+    assert st == 4
+
+
+def test_vertical_average_of_layer_field():
+    sigma = np.array([0.0, 0.1, 0.3, 0.6, 1.0])
+    var = np.zeros((4, 6, 6), order="F")
+    var[:, :, :] = 2.5
+    var[:, 2, 3] = [1.0, 2.0, 3.0, 4.0]
+    out = po.vertical_average(sigma, var)
+    assert np.allclose(out[0, 0], 2.5, rtol=1e-15)
+    assert out[2, 3] == pytest.approx(1.0 * 0.1 + 2.0 * 0.2 + 3.0 * 0.3 + 4.0 * 0.4, rel=1e-15)
+
+
+def test_ssa_slab_closed_form_and_patch_test():
+    """Periodic inclined slab (uniform H, beta): the 2-D SSA solution is the
+    uniform sliding velocity u = -rho_i g H ds/dx / beta (the stiffness
+    annihilates constant fields; the level-summed gravity load is
+    -rho_i g H ds/dx dx dy / vol0); and the assembled operator maps a linear
+    velocity field to zero at interior vertices (consistency)."""
+    p = synthetic.slab(n_cells=8, nz=5)
+    s = po.assemble_2d(p)
+    g = p.grid
+    geo = s["geo"]
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    dsdx = geo["dusrf_dx"][own]
+    assert np.ptp(dsdx) < 1e-12 * abs(dsdx).max()
+    ucf = -910.0 * 9.81 * 1000.0 * dsdx.mean() / 2000.0
+    xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+    st, err, nit = po.pcg_solve_2d(g, s["active_vertex"], s["Auu"], s["Auv"], s["Avu"], s["Avv"], s["bu"], s["bv"],
+                                   xu, xv, precond=1, tolerance=1e-9, maxiters=20000)
+    assert st == 0
+    assert np.allclose(xu[own], ucf, rtol=1e-6) and np.abs(xv[own]).max() < 1e-6 * ucf
+    # patch test with constant viscosity on the same grid
+    p.whichefvs, p.efvs_constant = 0, 1.0e13
+    s = po.assemble_2d(p)
+    xV, yV = p.vertex_coords()
+    lu = np.asfortranarray(3e-3 * xV - 1e-3 * yV)
+    lv = np.asfortranarray(2e-3 * xV + 4e-3 * yV)
+    Az = [s[k].copy(order="F") for k in ("Auu", "Auv", "Avu", "Avv")]
+    Az[0][:, :, 4] -= s["beta_internal"] * p.dx * p.dy / 1e9 * (s["active_vertex"] != 0)
+    Az[3][:, :, 4] -= s["beta_internal"] * p.dx * p.dy / 1e9 * (s["active_vertex"] != 0)
+    yu, yv = po.matvec_2d(g, s["active_vertex"], *Az, lu, lv)
+    inner = (slice(g.ilo, g.ihi - 1), slice(g.jlo, g.jhi - 1))
+    scale = np.abs(Az[0]).max() * np.abs(lu).max()
+    assert np.abs(yu[inner]).max() < 1e-11 * scale and np.abs(yv[inner]).max() < 1e-11 * scale
+
+
+def test_ssa_basal_dirichlet_count_and_symmetry_report():
+    """Basal term = dx dy beta / vol0 on the diagonal of active rows; with u
+    and v prescribed on a set of vertices, the Dirichlet-modified system gives
+    those values there and satisfies the unmodified equations elsewhere;
+    count_nonzeros and the symmetry report against numpy."""
+    p = synthetic.ice_stream(24, nz=5, seed=4)
+    s = po.assemble_2d(p)
+    g = p.grid
+    av = s["active_vertex"]
+    A = [s[k] for k in ("Auu", "Auv", "Avu", "Avv")]
+    pb = synthetic.ice_stream(24, nz=5, seed=4)
+    pb.beta = np.zeros_like(pb.beta)
+    s0 = po.assemble_2d(pb)
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    d = (s["Auu"] - s0["Auu"])[:, :, 4][own]
+    want = (p.dx * p.dy / 1e9 * s["beta_internal"] * (av != 0))[own]
+    assert np.allclose(d, want, rtol=1e-9, atol=1e-12 * np.abs(s["Auu"]).max())
+    # count and symmetry report
+    nnz = sum(int(np.count_nonzero(a[own][np.asarray(av, bool)[own]])) for a in A)
+    assert s["nnz"] == nnz and s["nwarn_asym"] == 0
+    B = [a.copy(order="F") for a in A]
+    i0, j0 = g.ilo + 5, g.jlo + 5
+    B[0][i0 - 1, j0 - 1, 5] *= 1.5       # plant one asymmetric pair (row (i0,j0), neighbour i0+1)
+    L = po.lib()
+    nw = L.orc_check_symmetry_assembled_2d(g.nx, g.ny, g.ilo, g.ihi, g.jlo, g.jhi, po.ip(av), *[po.dp(b) for b in B])
+    assert nw == 2
+    # Dirichlet elimination
+    rng = np.random.default_rng(9)
+    mask = np.zeros((g.nvx, g.nvy), np.int32, order="F")
+    pick = rng.random((g.nvx, g.nvy)) < 0.15
+    mask[pick & (np.asarray(av) != 0)] = 1
+    uval = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)) * 50.0)
+    vval = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)) * 50.0)
+    Ad = [a.copy(order="F") for a in A]
+    bu, bv = s["bu"].copy(order="F"), s["bv"].copy(order="F")
+    L.orc_dirichlet_bc_2d(g.nx, g.ny, 2, po.ip(av), po.ip(mask), po.ip(mask), po.dp(uval), po.dp(vval),
+                          *[po.dp(a) for a in Ad], po.dp(bu), po.dp(bv))
+    xu, xv = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+    st, err, nit = po.pcg_solve_2d(g, av, *Ad, bu, bv, xu, xv, precond=1, tolerance=1e-13, maxiters=20000)
+    assert st == 0
+    m_own = (mask[own] == 1) & (np.asarray(av)[own] != 0)
+    assert np.allclose(xu[own][m_own], uval[own][m_own], rtol=1e-9) and np.allclose(xv[own][m_own], vval[own][m_own], rtol=1e-9)
+    yu, yv = po.matvec_2d(g, av, *A, xu, xv)
+    free = (mask[own] == 0) & (np.asarray(av)[own] != 0)
+    # the halo rule (no_ice) zeroes the last owned row/column of x and b
+    free[-1, :] = False
+    free[:, -1] = False
+    scale = np.abs(s["bu"][own]).max()
+    assert np.abs((yu - s["bu"])[own][free]).max() < 1e-7 * scale
+    assert np.abs((yv - s["bv"])[own][free]).max() < 1e-7 * scale
