@@ -15,6 +15,7 @@
 #include "kernels.h"
 #include "kernels.hip"
 #include "assembly.hip"
+#include "assembly2d.hip"
 #include "ensemble.hip"
 
 using namespace cg;
@@ -77,6 +78,8 @@ struct cismgpu_system {
     bool have_saved = false;
     bool have_matrix = false;
     bool two_d = false;          // one-level (depth-integrated) system: 9-point stencil, nz = 1
+    int nz_levels = 0;           // 2-D system: sigma levels of the assembly inputs
+    double* load3 = nullptr;     // 2-D system: nz_levels-level load work pair
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
     int ev_used = 0;
@@ -682,7 +685,9 @@ int32_t cismgpu_system_create_2d(const cismgpu_grid* grid, cismgpu_system** out)
     if (!grid) { cismgpu_set_error("null grid"); return CISMGPU_ERR_ARG; }
     cismgpu_grid g = *grid;
     g.nz = 1;
-    return system_create(&g, true, out);
+    const int st = system_create(&g, true, out);
+    if (!st) (*out)->nz_levels = grid->nz;
+    return st;
 }
 
 int32_t cismgpu_system_destroy(cismgpu_system* S) {
@@ -690,7 +695,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -1067,28 +1072,42 @@ static int upload_fe_tables() {
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dxr), dxr, sizeof(dxr)));
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dyr), dyr, sizeof(dyr)));
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dzr), dzr, sizeof(dzr)));
+    // 2-D element (glissade_velo_higher.F90:528-570): points (-r,-r), (r,-r), (r,r), (-r,r)
+    const double xq2[4] = {-r, r, r, -r}, yq2[4] = {-r, -r, r, r};
+    double phi2[4][4], dxr2[4][4], dyr2[4][4];
+    for (int p = 0; p < 4; ++p) {
+        phi2[0][p] = (1.0 - xq2[p]) * (1.0 - yq2[p]) / 4.0;
+        phi2[1][p] = (1.0 + xq2[p]) * (1.0 - yq2[p]) / 4.0;
+        phi2[2][p] = (1.0 + xq2[p]) * (1.0 + yq2[p]) / 4.0;
+        phi2[3][p] = (1.0 - xq2[p]) * (1.0 + yq2[p]) / 4.0;
+        dxr2[0][p] = -(1.0 - yq2[p]) / 4.0;
+        dxr2[1][p] = (1.0 - yq2[p]) / 4.0;
+        dxr2[2][p] = (1.0 + yq2[p]) / 4.0;
+        dxr2[3][p] = -(1.0 + yq2[p]) / 4.0;
+        dyr2[0][p] = -(1.0 - xq2[p]) / 4.0;
+        dyr2[1][p] = -(1.0 + xq2[p]) / 4.0;
+        dyr2[2][p] = (1.0 + xq2[p]) / 4.0;
+        dyr2[3][p] = (1.0 - xq2[p]) / 4.0;
+    }
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_phi2), phi2, sizeof(phi2)));
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dxr2), dxr2, sizeof(dxr2)));
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dyr2), dyr2, sizeof(dyr2)));
     g_tables_ready = true;
     return CISMGPU_OK;
 }
 
-extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_assembly_in* in,
-                                              cismgpu_assembly_info* info) {
-    if (!S || !in || !in->sigma || !in->stagwbndsigma || !in->active_cell || !in->active_vertex || !in->xVertex ||
-        !in->yVertex || !in->stagusrf || !in->stagthck || !in->dusrf_dx || !in->dusrf_dy || !in->flwafact) {
-        cismgpu_set_error("assemble_3d: missing input");
-        return CISMGPU_ERR_ARG;
-    }
-    if (S->two_d) { cismgpu_set_error("assemble_3d needs a 3-D system"); return CISMGPU_ERR_ARG; }
-    int st = upload_fe_tables();
-    if (st) return st;
-    const Dims& d = S->d;
+// Inputs of an assembly into the system's device block.  d describes the
+// input fields: the system's grid for a 3-D system, the grid with the
+// input's sigma levels for a 2-D one.
+static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgpu_assembly_in* in, AsmDev& a,
+                                  LatDev& lat, bool& lateral) {
     const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
     const long long nvec = (long long)d.nz * n2, nfl = (long long)(d.nz - 1) * nc;
     // device input block: doubles first, then int32s
-    const bool lateral = in->ocean_mask != nullptr;
+    lateral = in->ocean_mask != nullptr;
     if (lateral && (!in->usrf || !in->thck ||
                     (in->which_assemble_lateral == 0 && (!in->stagusrf_lateral || !in->stagthck_lateral)))) {
-        cismgpu_set_error("assemble_3d: lateral BC needs usrf, thck and (standard option) the staggered fields");
+        cismgpu_set_error("assembly: lateral BC needs usrf, thck and (standard option) the staggered fields");
         return CISMGPU_ERR_ARG;
     }
     const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? 2 * nc + 2 * n2 : 0)),
@@ -1103,7 +1122,6 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     if (!S->efvs) CG_CHECK(hipMalloc(&S->efvs, nfl * sizeof(double)));
     double* db = (double*)S->asm_buf;
     int32_t* ib = (int32_t*)(db + nd);
-    AsmDev a{};
     double* p = db;
     auto putd = [&](const double* h, long long n, bool zero_if_null) -> const double* {
         double* dst = p;
@@ -1123,7 +1141,6 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     a.beta = putd(in->beta_internal, n2, true);
     a.theta = putd(in->theta_basal, n2, true);
     a.flwafact = putd(in->flwafact, nfl, true);
-    LatDev lat{};
     if (lateral) {
         lat.usrf = putd(in->usrf, nc, true);
         lat.thck = putd(in->thck, nc, true);
@@ -1154,6 +1171,27 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         CG_CHECK(hipMemcpyAsync(S->active, act.data(), n2, hipMemcpyHostToDevice, S->s));
         CG_CHECK(hipStreamSynchronize(S->s));
     }
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_assembly_in* in,
+                                              cismgpu_assembly_info* info) {
+    if (!S || !in || !in->sigma || !in->stagwbndsigma || !in->active_cell || !in->active_vertex || !in->xVertex ||
+        !in->yVertex || !in->stagusrf || !in->stagthck || !in->dusrf_dx || !in->dusrf_dy || !in->flwafact) {
+        cismgpu_set_error("assemble_3d: missing input");
+        return CISMGPU_ERR_ARG;
+    }
+    if (S->two_d) { cismgpu_set_error("assemble_3d needs a 3-D system"); return CISMGPU_ERR_ARG; }
+    int st = upload_fe_tables();
+    if (st) return st;
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
+    const long long nfl = (long long)(d.nz - 1) * nc;
+    AsmDev a{};
+    LatDev lat{};
+    bool lateral = false;
+    st = upload_assembly_inputs(S, d, in, a, lat, lateral);
+    if (st) return st;
     hipEvent_t e0, e1, e2;
     hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2);
     { const int st_ = reset_scal(S); if (st_) return st_; }
@@ -1184,6 +1222,93 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     for (int f = 0; f < 8; f += 2) halo<double>(S, S->M64 + f * d.vlen, S->M64 + (f + 1) * d.vlen, d.nz);
     // raw Auu/Avv for the horizontal tridiagonal preconditioners
     const long long nraw = 27LL * d.nz * n2;
+    if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
+    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
+    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw);
+    hipEventRecord(e2, S->s);
+    unsigned long long hc[2] = {0, 0};
+    CG_CHECK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(sync_scal(S) ? hipErrorUnknown : hipSuccess);
+    hipFree(cnt);
+    float ms1 = 0.f, ms2 = 0.f;
+    hipEventElapsedTime(&ms1, e0, e1);
+    hipEventElapsedTime(&ms2, e0, e2);
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+    if (info) {
+        info->nnz = (int64_t)hc[0];
+        info->asym_pairs = (int64_t)hc[1];
+        info->assemble_ms = ms1;
+        info->total_ms = ms2;
+    }
+    S->have_matrix = true;
+    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->tri.mode = 0;
+    if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
+    return CISMGPU_OK;
+}
+
+// 2-D (depth-integrated) assembly on a system from cismgpu_system_create_2d:
+// the inputs are those of the 3-D assembly with the grid's nz sigma levels;
+// the velocity is the resident 2-D x.
+extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_assembly_in* in, int32_t whichapprox,
+                                              cismgpu_assembly_info* info) {
+    if (!S || !in || !in->sigma || !in->stagwbndsigma || !in->active_cell || !in->active_vertex || !in->xVertex ||
+        !in->yVertex || !in->stagusrf || !in->stagthck || !in->dusrf_dx || !in->dusrf_dy || !in->flwafact) {
+        cismgpu_set_error("assemble_2d: missing input");
+        return CISMGPU_ERR_ARG;
+    }
+    if (!S->two_d) { cismgpu_set_error("assemble_2d needs a system made by cismgpu_system_create_2d"); return CISMGPU_ERR_ARG; }
+    if (S->nz_levels < 2 || S->nz_levels > 64) {
+        cismgpu_set_error("assemble_2d: the grid's nz (sigma levels of the inputs) must be 2..64, got %d", S->nz_levels);
+        return CISMGPU_ERR_ARG;
+    }
+    if (whichapprox != CISMGPU_APPROX_SSA) {
+        cismgpu_set_error("assemble_2d: which_ho_approx %d is not supported (SSA = 1 is)", whichapprox);
+        return CISMGPU_ERR_UNSUPPORTED;
+    }
+    int st = upload_fe_tables();
+    if (st) return st;
+    const Dims& d = S->d;
+    Dims d3 = d;                       // the nz-level fields of the inputs
+    d3.nz = S->nz_levels;
+    d3.vlen = (long long)(d3.nz + 2) * d3.plane;
+    d3.nrows = (long long)d3.nz * d3.nyo * d3.nxp;
+    const Lev L{d3.nz};
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
+    AsmDev a{};
+    LatDev lat{};
+    bool lateral = false;
+    st = upload_assembly_inputs(S, d3, in, a, lat, lateral);
+    if (st) return st;
+    if (!S->load3) CG_CHECK(hipMalloc(&S->load3, 2 * d3.vlen * sizeof(double)));
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2);
+    { const int st_ = reset_scal(S); if (st_) return st_; }
+    const double* uv = S->V(0);
+    const double* vv = S->V(1);
+    hipEventRecord(e0, S->s);
+    k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->scal + S_NAN);
+    hipEventRecord(e1, S->s);
+    k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
+    if (in->beta_internal) k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+    // b = level sum of the 3-D load (gravity, then the lateral BC)
+    double *lu = S->load3, *lv = S->load3 + d3.vlen;
+    CG_CHECK(hipMemsetAsync(S->load3, 0, 2 * d3.vlen * sizeof(double), S->s));
+    k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, a, lu, lv);
+    if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d3, a, lat, lu, lv);
+    k_sum_levels<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, lu, lv, d, S->V(2), S->V(3));
+    k_dirichlet_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->V(2), S->V(3));
+    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
+    halo<double>(S, S->V(2), S->V(3), 1);
+    unsigned long long* cnt = nullptr;
+    CG_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
+    CG_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), S->s));
+    if (in->check_symmetry) k_symmetry_report_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a.avert, S->A64, cnt + 1);
+    k_count_nnz<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt);
+    CG_CHECK(hipMemsetAsync(S->M64, 0, 8 * d.vlen * sizeof(double), S->s));
+    k_precond_from_rows<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, S->M64, S->M64 + 6 * d.vlen);
+    for (int f = 0; f < 8; f += 2) halo<double>(S, S->M64 + f * d.vlen, S->M64 + (f + 1) * d.vlen, 1);
+    const long long nraw = 27LL * n2;
     if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
     k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
     k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw);
@@ -1273,7 +1398,8 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
 
 extern "C" int32_t cismgpu_system_get_efvs(cismgpu_system* S, double* efvs) {
     if (!S || !S->efvs || !efvs) return CISMGPU_ERR_ARG;
-    const long long nfl = (long long)(S->d.nz - 1) * S->d.nx * S->d.ny;
+    const int nzl = S->two_d ? S->nz_levels : S->d.nz;
+    const long long nfl = (long long)(nzl - 1) * S->d.nx * S->d.ny;
     CG_CHECK(hipMemcpyAsync(efvs, S->efvs, nfl * sizeof(double), hipMemcpyDeviceToHost, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     return CISMGPU_OK;
@@ -1434,7 +1560,8 @@ extern "C" int32_t cismgpu_system_residual_velocity(cismgpu_system* S, int32_t w
     const int nb = (int)nblocks((long long)d.nxo * d.nyo, 256);
     double* part = nullptr;
     CG_CHECK(hipMalloc(&part, 4 * (size_t)nb * sizeof(double)));
-    const int skip_bed = (whichresid == 1 || whichresid == 2) ? 1 : 0;
+    // compute_residual_velocity_2d has no bed level to leave out (:10188-10310)
+    const int skip_bed = (!S->two_d && (whichresid == 1 || whichresid == 2)) ? 1 : 0;
     k_resid_velo<<<nb, 256, 0, S->s>>>(d, skip_bed, S->V(0), S->V(1), O, O + d.vlen, part);
     std::vector<double> h(4 * (size_t)nb);
     CG_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->s));

@@ -46,6 +46,9 @@ def _declare(L):
         return
     L.cismgpu_system_assemble_3d.argtypes = [ctypes.c_void_p, ctypes.POINTER(AssemblyIn), ctypes.POINTER(AssemblyInfo)]
     L.cismgpu_system_assemble_3d.restype = ctypes.c_int32
+    L.cismgpu_system_assemble_2d.argtypes = [ctypes.c_void_p, ctypes.POINTER(AssemblyIn), ctypes.c_int32,
+                                             ctypes.POINTER(AssemblyInfo)]
+    L.cismgpu_system_assemble_2d.restype = ctypes.c_int32
     D = ctypes.POINTER(ctypes.c_double)
     L.cismgpu_system_get_efvs.argtypes = [ctypes.c_void_p, D]
     L.cismgpu_system_get_efvs.restype = ctypes.c_int32
@@ -143,6 +146,14 @@ class BPSolver:
             self.ocean_mask.ctypes.data if has_front else None, self.usrf.ctypes.data, self.thck.ctypes.data,
             self.stagusrf_lat.ctypes.data, self.stagthck_lat.ctypes.data, int(p.assemble_lateral))
 
+    def _initial_velocity(self):
+        u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
+        v = np.asfortranarray(np.array(self.p.vvel, dtype=np.float64))
+        return u, v
+
+    def _zeros(self):
+        return self.grid.zeros_vec()
+
     def zero_inactive(self, uvel, vvel):
         """Velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""
         g = self.grid
@@ -180,7 +191,7 @@ class BPSolver:
         L2r = ctypes.c_double()
         ru = rv = None
         if want_vectors:
-            ru, rv = g.zeros_vec(), g.zeros_vec()
+            ru, rv = self._zeros(), self._zeros()
         _lib.check(self._L.cismgpu_system_residual(self.sys.handle, ru.ctypes.data_as(D) if ru is not None else None,
                                                    rv.ctypes.data_as(D) if rv is not None else None,
                                                    ctypes.byref(L2), ctypes.byref(L2r)), "residual")
@@ -213,9 +224,7 @@ class BPSolver:
         its assembled system.  Convergence: resid_velo < resid_target for
         which_ho_resid 0/1/2, L2 < resid_target for 3, L2 / |b| for 4
         (:3652-3661)."""
-        g = self.grid
-        u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
-        v = np.asfortranarray(np.array(self.p.vvel, dtype=np.float64))
+        u, v = self._initial_velocity()
         self.zero_inactive(u, v)
         self.sys.set_x(u, v)
         _lib.check(self._L.cismgpu_system_work_vectors(self.sys.handle, 4), "work_vectors")
@@ -239,7 +248,7 @@ class BPSolver:
                 ai = self.assemble()
                 res.assemble_ms.append(ai["total_ms"])
                 if ai["nnz"] == 0:          # nothing to solve (:2937-3032)
-                    res.uvel, res.vvel = g.zeros_vec(), g.zeros_vec()
+                    res.uvel, res.vvel = self._zeros(), self._zeros()
                     res.converged = True
                     res.iterations = counter
                     return res
@@ -284,3 +293,59 @@ class BPSolver:
         res.uvel, res.vvel = self.sys.get_x()
         res.efvs = self.efvs()
         return res
+
+
+class SSASolver(BPSolver):
+    """Device-resident 2-D shallow-shelf (SSA) velocity solver: the
+    ``solve_2d`` branch of glissade_velo_higher_solve with which_ho_approx = 1
+    (glissade_velo_higher.F90:2353-2725 assembly and BCs, :3245-3272 solve,
+    compute_residual_velocity_2d :10188-10310).  The setup (masks, staggered
+    geometry, flow factor, beta, load inputs) is the 3-D one; the unknowns
+    are the (nx-1, ny-1) velocities, started from the basal level of the 3-D
+    guess (:1362).  The Picard loop and its acceleration are those of
+    :class:`BPSolver`."""
+
+    APPROX_SSA = 1
+
+    def __init__(self, problem, system: System | None = None):
+        super().__init__(problem, system or System(problem.grid, two_d=True))
+        if not self.sys.two_d:
+            raise ValueError("SSASolver needs a 2-D system")
+
+    def _initial_velocity(self):
+        nz = self.grid.nz
+        u = np.asfortranarray(np.array(self.p.uvel[nz - 1], dtype=np.float64))
+        v = np.asfortranarray(np.array(self.p.vvel[nz - 1], dtype=np.float64))
+        return u, v
+
+    def _zeros(self):
+        g = self.grid
+        return np.zeros((g.nvx, g.nvy), order="F")
+
+    def zero_inactive(self, uvel, vvel):
+        """2-D velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""
+        g = self.grid
+        il, jl = slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi)
+        inact = self.active_vertex[il, jl] == 0
+        for a in (uvel, vvel):
+            o = a[il, jl]
+            o[inact] = 0.0
+            a[il, jl] = o
+
+    def assemble(self, uvel=None, vvel=None):
+        """Assemble the 2-D SSA A, b for the viscosity of (uvel, vvel) (nx-1, ny-1);
+        None keeps the resident x."""
+        if uvel is not None:
+            u = np.asfortranarray(np.array(uvel, dtype=np.float64))
+            v = np.asfortranarray(np.array(vvel, dtype=np.float64))
+            self.zero_inactive(u, v)
+            self.sys.set_x(u, v)
+        info = AssemblyInfo()
+        _lib.check(self._L.cismgpu_system_assemble_2d(self.sys.handle, ctypes.byref(self._ain), self.APPROX_SSA,
+                                                      ctypes.byref(info)), "assemble_2d")
+        return dict(nnz=info.nnz, asym_pairs=info.asym_pairs, assemble_ms=info.assemble_ms, total_ms=info.total_ms)
+
+    def picard(self, *, precond=1, **kw) -> PicardResult:
+        """Picard iteration of the 2-D solve; which_ho_precond 2 is replaced by
+        the diagonal preconditioner for 2-D solves (glide_setup.F90:1363-1366)."""
+        return super().picard(precond=1 if precond == HO_PRECOND_SIA else precond, **kw)

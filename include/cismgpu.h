@@ -156,10 +156,11 @@ typedef struct cismgpu_solve_info {
 } cismgpu_solve_info;
 
 int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out);
-/* One-level system for the 2-D solves (grid->nz ignored).  Its matrix is
- * loaded with cismgpu_system_set_matrix_2d; vectors are (nx-1, ny-1); the
- * SpMV reads the 9-point stencil only.  Assembly and ensembles need 3-D
- * systems. */
+/* One-level system for the 2-D solves.  Its matrix is loaded with
+ * cismgpu_system_set_matrix_2d or assembled by cismgpu_system_assemble_2d
+ * (grid->nz: the sigma levels of the assembly inputs; the solves ignore it);
+ * vectors are (nx-1, ny-1); the SpMV reads the 9-point stencil only.
+ * Ensembles need 3-D systems. */
 int32_t cismgpu_system_create_2d(const cismgpu_grid* grid, cismgpu_system** out);
 int32_t cismgpu_system_destroy(cismgpu_system* sys);
 
@@ -259,6 +260,22 @@ typedef struct cismgpu_assembly_info {
 } cismgpu_assembly_info;
 
 int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_in* in,
+                                   cismgpu_assembly_info* info);
+/* which_ho_approx values (glide_types.F90:335-339) */
+#define CISMGPU_APPROX_SSA   1
+#define CISMGPU_APPROX_L1L2  3
+#define CISMGPU_APPROX_DIVA  4
+/* 2-D (depth-integrated) assembly on a system from cismgpu_system_create_2d,
+ * whose grid nz gives the sigma levels of the inputs (the same fields as
+ * for the 3-D assembly; the Dirichlet masks' basal level is used).  Velocity:
+ * the resident 2-D x.  Covers assemble_stiffness_matrix_2d
+ * (glissade_velo_higher.F90:4961-5381) for which_ho_approx = SSA, the local
+ * basal_sliding_bc_2d (:8281-8531), b = the level sum of the 3-D load
+ * (:2045-2058, :2636), dirichlet_boundary_conditions_2d (:9133-9301), the
+ * matrix and rhs halo, check_symmetry_assembled_matrix_2d (report only,
+ * :10648-10850) and count_nonzeros_2d (:10450-10509).  L1L2 and DIVA return
+ * CISMGPU_ERR_UNSUPPORTED. */
+int32_t cismgpu_system_assemble_2d(cismgpu_system* sys, const cismgpu_assembly_in* in, int32_t whichapprox,
                                    cismgpu_assembly_info* info);
 /* Snapshot of the assembled system (A, b, preconditioner) kept on the
  * device, and its restoration: the matrix copies of the accelerated Picard

@@ -79,6 +79,8 @@ int64_t orc_check_symmetry_assembled_2d(int nx, int ny, int ilo, int ihi, int jl
                                         const double* Avu, const double* Avv);
 int64_t orc_count_nonzeros_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, const int32_t* active_vertex,
                               const double* Auu, const double* Auv, const double* Avu, const double* Avv);
+double orc_residual_velocity_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, int whichresid,
+                                const double* usav, const double* vsav, const double* uvel, const double* vvel);
 
 #ifdef __cplusplus
 }

@@ -1035,3 +1035,26 @@ int64_t orc_count_nonzeros_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi
             }
     return n;
 }
+
+/* compute_residual_velocity_2d (glissade_velo_higher.F90:10188-10310):
+ * which_ho_resid 2 is the mean relative speed change, every other value
+ * the maximum, over all owned vertices (no bed level to leave out). */
+double orc_residual_velocity_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, int whichresid,
+                                const double* usav, const double* vsav, const double* uvel, const double* vvel)
+{
+    double r = 0.0;
+    int64_t count = 0;
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i) {
+            const size_t p = V2(i, j);
+            const double speed = sqrt(uvel[p] * uvel[p] + vvel[p] * vvel[p]);
+            if (speed != 0.0) {
+                const double old = sqrt(usav[p] * usav[p] + vsav[p] * vsav[p]);
+                const double d = fabs((old - speed) / speed);
+                if (whichresid == 2) { count++; r = r + d; }
+                else if (d > r) r = d;
+            }
+        }
+    if (whichresid == 2 && count > 0) r = r / (double)count;
+    return r;
+}

@@ -97,6 +97,8 @@ def _declare(L):
     L.orc_check_symmetry_assembled_2d.restype = ctypes.c_int64
     L.orc_count_nonzeros_2d.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D]
     L.orc_count_nonzeros_2d.restype = ctypes.c_int64
+    L.orc_residual_velocity_2d.argtypes = [i, i, i, i, i, i, i, _D, _D, _D, _D]
+    L.orc_residual_velocity_2d.restype = ctypes.c_double
 
 
 def dp(a: np.ndarray):
@@ -467,22 +469,53 @@ def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
     return L2, L2 / np.sqrt(global_sum(grid, bu2, bv2))
 
 
+def residual_L2_2d(grid, active_vertex, A, bu, bv, uvel, vvel):
+    """compute_residual_vector_2d (glissade_velo_higher.F90:9502-9689): L2 of
+    A u - b over owned active vertices, and relative to |b|."""
+    import dataclasses
+    g1 = dataclasses.replace(grid, nz=1)
+    yu, yv = matvec_2d(grid, active_vertex, *A, uvel, vvel)
+    il, jl = slice(grid.ilo - 1, grid.ihi), slice(grid.jlo - 1, grid.jhi)
+    act = np.asarray(active_vertex, bool)[il, jl]
+    w = [np.zeros((1, grid.nvx, grid.nvy), order="F") for _ in range(4)]
+    w[0][0, il, jl] = np.where(act, (yu - bu)[il, jl] ** 2, 0.0)
+    w[1][0, il, jl] = np.where(act, (yv - bv)[il, jl] ** 2, 0.0)
+    w[2][0, il, jl] = np.where(act, bu[il, jl] * bu[il, jl], 0.0)
+    w[3][0, il, jl] = np.where(act, bv[il, jl] * bv[il, jl], 0.0)
+    L2 = np.sqrt(global_sum(g1, w[0], w[1]))
+    rhs = np.sqrt(global_sum(g1, w[2], w[3]))
+    return L2, (L2 / rhs if rhs > 0.0 else 0.0)
+
+
 def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chrongear", precond=2, ncheck=5,
-           linear_tol=1e-8, linear_maxiters=200, accelerate=False):
+           linear_tol=1e-8, linear_maxiters=200, accelerate=False, two_d=False):
     """Outer Picard loop of glissade_velo_higher_solve for the BP path
     (glissade_velo_higher.F90:2163-3663), which_ho_nonlinear = 0, or with
     Picard acceleration (:2186-2263, :3108-3176, :3624-3630;
-    evaluate_accelerated_picard_3d :10052-10184)."""
+    evaluate_accelerated_picard_3d :10052-10184).  two_d: the SSA branch
+    (solve_2d; assemble_2d, the 2-D solvers with which_ho_precond 2 taken as
+    1, compute_residual_velocity_2d), started from the basal level of the
+    3-D guess (:1362)."""
     L = lib()
     p = problem
     g = p.grid
     geo = geometry(p)
-    u = np.asfortranarray(p.uvel.copy())
-    v = np.asfortranarray(p.vvel.copy())
-    own = g.owned_slice()
-    inact = geo["active_vertex"][own[1], own[2]] == 0
-    for a in (u, v):
-        o = a[own]; o[:, inact] = 0.0; a[own] = o
+    if two_d:
+        u = np.asfortranarray(p.uvel[g.nz - 1].copy())
+        v = np.asfortranarray(p.vvel[g.nz - 1].copy())
+        il, jl = slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi)
+        inact = geo["active_vertex"][il, jl] == 0
+        for a in (u, v):
+            o = a[il, jl]; o[inact] = 0.0; a[il, jl] = o
+        if precond == 2:
+            precond = 1
+    else:
+        u = np.asfortranarray(p.uvel.copy())
+        v = np.asfortranarray(p.vvel.copy())
+        own = g.owned_slice()
+        inact = geo["active_vertex"][own[1], own[2]] == 0
+        for a in (u, v):
+            o = a[own]; o[:, inact] = 0.0; a[own] = o
     hist = dict(L2=[], resid_velo=[], linear_iters=[], linear_err=[], alpha=[])
     L2_large, gamma, alpha_max, thresh = 1.0e20, 0.40, 3.0, 0.85        # (:831, :1049-1056)
     if accelerate:
@@ -502,9 +535,9 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
                 v = np.asfortranarray(v_old + alpha * dv)
                 halo(g, u); halo(g, v)
             usav, vsav = u.copy(order="F"), v.copy(order="F")
-            s = assemble(p, u, v, geo)
+            s = assemble_2d(p, u, v, geo) if two_d else assemble(p, u, v, geo)
             A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
-            L2, L2r = residual_L2(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
+            L2, L2r = (residual_L2_2d if two_d else residual_L2)(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
             hist["L2"].append(L2)
             if accelerate and counter >= 2:
                 if L2 < thresh * L2_sav and alpha + gamma <= alpha_max:
@@ -522,14 +555,19 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
                 hist["alpha"].append(alpha)
             else:
                 done = True
-        st, err, nit = pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], u, v, algo=algo, precond=precond,
-                                 ncheck=ncheck, tolerance=linear_tol, maxiters=linear_maxiters)
+        st, err, nit = (pcg_solve_2d if two_d else pcg_solve)(g, s["active_vertex"], *A, s["bu"], s["bv"], u, v,
+                                                              algo=algo, precond=precond, ncheck=ncheck,
+                                                              tolerance=linear_tol, maxiters=linear_maxiters)
         if st:
             raise RuntimeError(f"oracle solver failed with status {st}")
         hist["linear_iters"].append(nit); hist["linear_err"].append(err)
         halo(g, u); halo(g, v)
-        rv = L.orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
-                                        dp(usav), dp(vsav), dp(u), dp(v))
+        if two_d:
+            rv = L.orc_residual_velocity_2d(g.nx, g.ny, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                            dp(usav), dp(vsav), dp(u), dp(v))
+        else:
+            rv = L.orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                            dp(usav), dp(vsav), dp(u), dp(v))
         hist["resid_velo"].append(rv)
         if accelerate:
             du, dv = u - u_old, v - v_old
