@@ -131,14 +131,16 @@ def main():
     # dominant kernel of the mixed solve: the low-precision SpMV, timed live
     sp_ms = sum(a for a, _ in spm)
     sp_n = sum(b for _, b in spm)
-    esz = 4 if lowp == pcg.MIXED_FP32 else 2
-    bytes_launch = nodes * (108 * esz + 4 * 4)
+    # storage of the inner SpMV: 1 FP32 quad, 2 BF16 quad, 3 FP32 half (symmetric) storage
+    spmv_kind = r["spmv_kind"]
+    coef_bytes = {1: 108 * 4, 2: 108 * 2, 3: 56 * 4}[spmv_kind]
+    bytes_launch = nodes * (coef_bytes + 4 * 4)
     spmv_avg_ms = sp_ms / max(sp_n, 1)
     achieved = bytes_launch / (spmv_avg_ms * 1e-3) / 1e9
     # HBM bytes per launch of the same kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     # passes of this workload, scaled by the counters' reading of a copy kernel of known size
     traffic = None
-    kname = "cg::k_spmv_q<float, 1>" if lowp == pcg.MIXED_FP32 else "cg::k_spmv_q<__hip_bfloat16, 1>"
+    kname = {1: "cg::k_spmv_q<float, 1>", 2: "cg::k_spmv_q<__hip_bfloat16, 1>", 3: "cg::k_spmv_sym<float, 1>"}[spmv_kind]
     pmc_path = os.path.join(ROOT, "profiles", "spmv_pmc_icestream1024.json")
     if args.n == 1024 and args.nz == 11 and os.path.exists(pmc_path):
         with open(pmc_path) as f:
@@ -247,7 +249,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "traffic_source": "profiles/spmv_pmc_icestream1024.json" if traffic else None,
-                     "kernel": f"{kname} (quad-layout inner SpMV + (Az,z) partials)",
+                     "kernel": f"{kname} (inner SpMV + (Az,z) partials; "
+                               f"{'half (symmetric) storage' if spmv_kind == 3 else 'quad layout'})",
                      "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": round(spmv_avg_ms, 4),
                      "launches_timed": sp_n},
         "spmv_isolated": spmv_iso,

@@ -16,6 +16,7 @@
 #include "kernels.hip"
 #include "assembly.hip"
 #include "assembly2d.hip"
+#include "spmv_sym.hip"
 #include "ensemble.hip"
 
 using namespace cg;
@@ -81,6 +82,7 @@ struct cismgpu_system {
     int nz_levels = 0;           // 2-D system: sigma levels of the assembly inputs
     double* load3 = nullptr;     // 2-D system: nz_levels-level load work pair
     bool m32_ready = false, a32_ready = false, a16_ready = false;
+    bool a32_sym = false;        // FP32 copy in half (symmetric) storage
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
     int ev_used = 0;
     bool timing = false;
@@ -164,7 +166,9 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
         k_spmv_plane<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     else if constexpr (std::is_same<TA, double>::value)
         k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
-    else   // FP32 / BF16 copies are stored in the quad layout
+    else if (std::is_same<TA, float>::value && S->a32_sym)   // FP32 half (symmetric) storage
+        k_spmv_sym<float, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, (const float*)A, xu, xv, yu, yv, bu, bv, part);
+    else   // FP32 / BF16 copies in the quad layout
         k_spmv_q<TA, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     if (t) { hipEventRecord(S->ev[S->ev_used + 1], S->s); S->ev_used += 2; }
 }
@@ -532,9 +536,32 @@ int ensure_lowp(cismgpu_system* S, int precision) {
         k_cast<double, float><<<nblocks(8 * d.vlen, 256), 256, 0, S->s>>>(8 * d.vlen, S->M64, S->M32);
         S->m32_ready = true;
     }
+    // FP32: half (symmetric) storage when the rounded matrix is exactly
+    // symmetric (3-D systems; CISMGPU_SPMV_FULL=1 keeps the quad layout).
+    // BF16 keeps the quad layout: its half-storage SpMV is issue-bound
+    // (0.70 ms against 0.52 ms for the quad kernel at 1024^2 x 11).
+    auto sym_ok = [&](bool& ok) -> int {
+        ok = false;
+        const char* env = getenv("CISMGPU_SPMV_FULL");
+        if (S->two_d || (env && env[0] == '1')) return CISMGPU_OK;
+        unsigned long long* cnt = nullptr;
+        unsigned long long h = 1;
+        CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
+        CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
+        k_check_sym<float><<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, cnt);
+        CG_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, S->s));
+        CG_CHECK(hipStreamSynchronize(S->s));
+        hipFree(cnt);
+        ok = h == 0;
+        return CISMGPU_OK;
+    };
+    const long long ns = d.nrows * NSLOT * 4;
     if (precision == CISMGPU_MIXED_FP32 && !S->a32_ready) {
         if (!S->A32) CG_CHECK(hipMalloc(&S->A32, na * sizeof(float)));
-        k_convert_quad<float><<<nblocks(na, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A32);
+        const int st = sym_ok(S->a32_sym);
+        if (st) return st;
+        if (S->a32_sym) k_convert_sym<float><<<nblocks(ns, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A32);
+        else k_convert_quad<float><<<nblocks(na, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A32);
         S->a32_ready = true;
     }
     if (precision == CISMGPU_MIXED_BF16 && !S->a16_ready) {
@@ -604,7 +631,7 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
     info->err = err;
     info->niters = inner_total;
     info->outer_iters = outer;
-    info->spmv_kind = o->precision == CISMGPU_MIXED_BF16 ? 2 : 1;
+    info->spmv_kind = o->precision == CISMGPU_MIXED_BF16 ? 2 : (S->a32_sym ? 3 : 1);
     if (!(err < o->tolerance)) {
         cismgpu_set_error("iterative refinement stopped at relative residual %.3e > %.3e", err, o->tolerance);
         return CISMGPU_ERR_NOT_CONVERGED;
@@ -947,6 +974,8 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
                 k_spmv_plane<__hip_bfloat16, float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         } else if (kind == 0)
             k_spmv<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
+        else if (kind == 1 && S->a32_sym)
+            k_spmv_sym<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         else if (kind == 1)
             k_spmv_q<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         else
@@ -964,7 +993,9 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     if (avg_ms) *avg_ms = ms / nrep;
     const double esz = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 2.0);
     const double vsz = kind == 0 ? 8.0 : 4.0;
-    const double ncoef = S->two_d ? 36.0 : (double)NCOEF;   // 4 blocks x 9 or x 27 neighbours
+    const bool half = kind == 1 && S->a32_sym;
+    // 4 blocks x 9 or x 27 neighbours; half storage: 14 slots of 4
+    const double ncoef = S->two_d ? 36.0 : (half ? 4.0 * NSLOT : (double)NCOEF);
     if (bytes_per_launch) *bytes_per_launch = (double)nodes * (ncoef * esz + 4.0 * vsz);
     return CISMGPU_OK;
 }

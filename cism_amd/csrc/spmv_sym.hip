@@ -1,0 +1,214 @@
+// spmv_sym.hip -- half-storage SpMV for the inner FP32 iterations.
+//
+// The assembled operator is symmetric: Auu = Auu^T, Avv = Avv^T, Auv = Avu^T
+// (glissade_velo_higher.F90:10859-10864), so of the 27 neighbour couplings
+// of a node only the centre and the 13 "upper" ones (m = 14..27 in the
+// reference numbering, 0-based 13..26) need storing; the coupling of node r
+// to its lower neighbour q = r + off(m), m < 13, is the transposed 2x2
+// block of q's upper coupling 26 - m.  Storage (per owned row r, slot
+// s = m - 13 for m >= 13):
+//     Sq[((r/64)*14 + s)*256 + (r%64)*4 + c]
+// slot 0 holds (uu, uv, vv, 0) of the centre, slots 1..13 the quads
+// (uu, uv, vu, vv) of the upper couplings, so the matrix streams 224 B per
+// node instead of 432.
+//
+// The SpMV walks a column up the levels like k_spmv_q and loads the 13
+// lower quads from the partner rows; those rows are the neighbouring
+// columns of the same tile (same level) or of the previous level, read
+// moments earlier by the neighbouring threads, so they come from L2
+// (measured: HBM traffic 1.13x the half-storage bytes).  The products are
+// summed in the same order (m = 0..26) with the same values as the
+// full-storage kernel, so when the FP32 matrix is exactly symmetric
+// (k_check_sym finds no mismatch) both kernels give identical results; the
+// solver uses this layout only then.  x is read per neighbour (L1/L2 hits)
+// instead of being kept in a 3-level register window: the window cost 54
+// VGPRs and limited the kernel to 3 waves per SIMD (0.87 ms against 0.73 ms
+// at 1024^2 x 11).
+#include "cismgpu_internal.h"
+
+namespace cg {
+
+constexpr int NSLOT = 14;
+
+// partner column of row (i, j) at horizontal offset (iA, jA): its owned
+// image (periodic rules) as jo*nxp + io, or -1 if it has no stored row
+__device__ __forceinline__ int partner_col(const Dims& d, int i, int j) {
+    if (!row_image(d, i, j)) return -1;
+    return (j - d.jlo) * d.nxp + (i - d.ilo);
+}
+
+// Half-storage copy of the owned rows of the FP64 row-blocked matrix.
+template <typename TA>
+__global__ void k_convert_sym(long long nrows, const double* __restrict__ src, TA* __restrict__ dst) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;   // over nrows * 14 * 4
+    if (t >= nrows * NSLOT * 4) return;
+    const int c = (int)(t % 4);
+    const long long u = t / 4;
+    const int lane = (int)(u % WAVE);
+    const long long v = u / WAVE;
+    const int s = (int)(v % NSLOT);
+    const long long rb = v / NSLOT;
+    const double* row = src + rb * NCOEF * WAVE + lane;
+    double val;
+    if (s == 0) {   // centre: uu, uv, vv, 0
+        const int blk = c == 0 ? 0 : (c == 1 ? 1 : 3);
+        val = c == 3 ? 0.0 : row[(blk * 27 + 13) * WAVE];
+    } else {
+        val = row[(c * 27 + 13 + s) * WAVE];
+    }
+    dst[t] = to_lowp<TA>(val);
+}
+
+// Counts the couplings that the half storage cannot reproduce exactly after
+// rounding to TA: a lower coupling whose transposed partner differs, a
+// nonzero coupling to a node without a stored row, or Auv != Avu at the
+// centre.  count[0] += mismatches.
+template <typename TA>
+__global__ void k_check_sym(Dims d, const double* __restrict__ A, unsigned long long* __restrict__ count) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long bad = 0;
+    if (t < d.nrows) {
+        const int io = (int)(t % d.nxp);
+        const long long kj = t / d.nxp;
+        const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+        if (io < d.nxo) {
+            const int i = d.ilo + io, j = d.jlo + jo;
+            const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+            auto rd = [](double x) { return (float)to_lowp<TA>(x); };
+            if (rd(a[(1 * 27 + 13) * WAVE]) != rd(a[(2 * 27 + 13) * WAVE])) ++bad;
+            for (int m = 0; m < 13; ++m) {
+                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+                if (k0 + kA < 0) continue;                   // stored as zero
+                const double c0 = a[(0 * 27 + m) * WAVE], c1 = a[(1 * 27 + m) * WAVE];
+                const double c2_ = a[(2 * 27 + m) * WAVE], c3 = a[(3 * 27 + m) * WAVE];
+                int qi = i + iA, qj = j + jA;
+                if (!row_image(d, qi, qj)) {
+                    bad += (c0 != 0.0 || c1 != 0.0 || c2_ != 0.0 || c3 != 0.0) ? 1 : 0;
+                    continue;
+                }
+                const double* q = row_ptr(d, A, k0 + kA, qi, qj);
+                const int mm = 26 - m;
+                if (rd(c0) != rd(q[(0 * 27 + mm) * WAVE]) || rd(c1) != rd(q[(2 * 27 + mm) * WAVE]) ||
+                    rd(c2_) != rd(q[(1 * 27 + mm) * WAVE]) || rd(c3) != rd(q[(3 * 27 + mm) * WAVE]))
+                    ++bad;
+            }
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) bad += __shfl_down(bad, off, WAVE);
+    if (threadIdx.x % WAVE == 0 && bad) atomicAdd(count, bad);
+}
+
+__device__ __forceinline__ void ld_slot(const float* p, float& a, float& b, float& c, float& e) { ld_quad(p, a, b, c, e); }
+
+template <typename TA, int MODE>
+__global__ __launch_bounds__(256) void k_spmv_sym(Dims d, const TA* __restrict__ S, const float* __restrict__ xu,
+                                                  const float* __restrict__ xv, float* __restrict__ yu,
+                                                  float* __restrict__ yv, const float* __restrict__ bu,
+                                                  const float* __restrict__ bv, double* __restrict__ partials) {
+    // x of the tile (64 x 4 columns) plus a one-column ring, for levels k-1,
+    // k, k+1, staged in LDS (a 3-level ring)
+    __shared__ float xs[2][3][6][WAVE + 2];
+    const int nbx = d.nxp / WAVE;
+    const int nblk = nbx * ((d.nyo + 3) / 4);
+    const int lin = xcd_remap(blockIdx.x, nblk);     // neighbouring tiles share an XCD's L2
+    const int bx = lin % nbx, by = lin / nbx;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int io = bx * WAVE + lane;
+    const int jo = by * 4 + w;
+    const bool live = io < d.nxo && jo < d.nyo;
+    const int i0 = d.ilo + bx * WAVE - 1, j0 = d.jlo + by * 4 - 1;   // vertex of xs[.][.][0][0]
+    const long long pl = d.plane;
+    // stage level kk (-1..nz) of x into ring slot kk+1 mod 3
+    auto stage = [&](int kk) {
+        const int slot = (kk + 1) % 3;
+        for (int e = threadIdx.x; e < 6 * (WAVE + 2); e += 256) {
+            const int r = e / (WAVE + 2), c = e % (WAVE + 2);
+            const int vi = i0 + c, vj = j0 + r;
+            float a = 0.f, b = 0.f;
+            if (vi >= 1 && vi <= d.nvx && vj >= 1 && vj <= d.nvy) {
+                const long long n = vidx(d, kk, vi, vj);
+                a = xu[n];
+                b = xv[n];
+            }
+            xs[0][slot][r][c] = a;
+            xs[1][slot][r][c] = b;
+        }
+    };
+    double acc = 0.0;
+    int po[9];
+    const int LS = d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);   // elements per level
+    if (live) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+        // element offsets of the partner rows' slot 0 within a level, -1 when
+        // the neighbour has no stored row; rows per level are a multiple of 64
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            const int pc = partner_col(d, i + h % 3 - 1, j + h / 3 - 1);
+            po[h] = pc < 0 ? -1 : (pc / WAVE) * (NSLOT * 4 * WAVE) + (pc % WAVE) * 4;
+        }
+    }
+    stage(-1);
+    stage(0);
+    for (int k = 0; k < d.nz; ++k) {
+        stage(k + 1);
+        __syncthreads();
+        if (live) {
+            const long long ck = vidx(d, k, d.ilo + io, d.jlo + jo);
+            const TA* Sk = S + (long long)k * LS;     // level k
+            const TA* Sb = Sk - LS;                   // level k - 1
+            float su = 0.f, sv = 0.f;
+#pragma unroll
+            for (int m = 0; m < 27; ++m) {
+                float cuu = 0.f, cuv = 0.f, cvu = 0.f, cvv = 0.f;
+                bool have = true;
+                if (m < 13) {
+                    const int h = m % 9;
+                    have = po[h] >= 0 && (m >= 9 || k > 0);
+                    if (have) {
+                        float a, b, c, e;
+                        ld_slot((m < 9 ? Sb : Sk) + po[h] + (13 - m) * (4 * WAVE), a, b, c, e);
+                        cuu = a; cuv = c; cvu = b; cvv = e;     // transposed 2x2 block
+                    }
+                } else if (m == 13) {
+                    float a, b, c, e;
+                    ld_slot(Sk + po[4], a, b, c, e);
+                    cuu = a; cuv = b; cvu = b; cvv = c;
+                } else {
+                    ld_slot(Sk + po[4] + (m - 13) * (4 * WAVE), cuu, cuv, cvu, cvv);
+                }
+                if (have) {
+                    const int slot = (k + m / 9) % 3;          // level k + kA lives in slot (k + kA + 1) % 3
+                    const int r = w + 1 + ((m / 3) % 3 - 1), c = lane + 1 + (m % 3 - 1);
+                    const float xun = xs[0][slot][r][c], xvn = xs[1][slot][r][c];
+                    su = su + cuu * xun + cuv * xvn;
+                    sv = sv + cvu * xun + cvv * xvn;
+                }
+            }
+            if (MODE == 2) {
+                const float ru = bu[ck] - su, rv = bv[ck] - sv;
+                yu[ck] = ru;
+                yv[ck] = rv;
+                acc += (double)ru * (double)ru + (double)rv * (double)rv;
+            } else {
+                yu[ck] = su;
+                yv[ck] = sv;
+                if (MODE == 1) {
+                    const int slot = (k + 1) % 3;
+                    acc += (double)su * (double)xs[0][slot][w + 1][lane + 1] +
+                           (double)sv * (double)xs[1][slot][w + 1][lane + 1];
+                }
+            }
+        }
+        __syncthreads();   // the slot of level k-1 is restaged next
+    }
+    if (MODE != 0) {
+        dd v[1] = {{acc, 0.0}};
+        block_reduce_dd<1, 256>(v);
+        if (threadIdx.x == 0) {
+            partials[2 * lin + 0] = v[0].hi;
+            partials[2 * lin + 1] = v[0].lo;
+        }
+    }
+}
+
+}  // namespace cg

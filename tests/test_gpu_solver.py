@@ -140,3 +140,45 @@ def test_mixed_precision_matches_fp64(require_gpu, precision):
         xm = np.stack(s.get_x())
     assert rm["err"] < 1e-10
     assert _rel(xm, x64) < 1e-6
+
+
+def _mixed_run(g, act, A, bu, bv, precision, full):
+    import os
+    os.environ["CISMGPU_SPMV_FULL"] = "1" if full else "0"
+    try:
+        with pcg.System(g) as s:
+            s.set_matrix(act, *A, bu, bv)
+            s.set_x(g.zeros_vec(), g.zeros_vec())
+            r = s.solve(precond=2, tolerance=1e-10, maxiters=3000, precision=precision, inner_tol=1e-3,
+                        max_outer=30)
+            return r, np.stack(s.get_x()), s.bench_spmv(kind=1 if precision == pcg.MIXED_FP32 else 2, nrep=3)
+    finally:
+        os.environ.pop("CISMGPU_SPMV_FULL", None)
+
+
+@pytest.mark.parametrize("name", ["stream40", "dome", "random_bc0", "random_bc3"])
+def test_half_storage_spmv_gives_identical_solves(require_gpu, name):
+    """An exactly symmetric FP32 matrix is stored as its centre and 13 upper
+    couplings; the half-storage SpMV sums the same products in the same
+    order, so the mixed solve is bit-identical to the full-storage one and
+    streams 224 matrix bytes per node instead of 432."""
+    g, act, A, bu, bv = SYSTEMS[name]
+    rf, xf, (msf, byf) = _mixed_run(g, act, A, bu, bv, pcg.MIXED_FP32, full=True)
+    rh, xh, (msh, byh) = _mixed_run(g, act, A, bu, bv, pcg.MIXED_FP32, full=False)
+    assert rf["spmv_kind"] == 1 and rh["spmv_kind"] == 3
+    assert rh["niters"] == rf["niters"] and rh["outer_iters"] == rf["outer_iters"]
+    assert np.array_equal(xh, xf)
+    nodes = g.n_owned_nodes
+    assert byf == nodes * (108 * 4 + 16) and byh == nodes * (56 * 4 + 16)
+
+
+def test_half_storage_needs_exact_symmetry(require_gpu):
+    """A coupling whose transposed partner differs after rounding keeps the
+    full quad storage (spmv_kind 1)."""
+    g, act, A, bu, bv = SYSTEMS["random_bc3"]
+    B = [a.copy(order="F") for a in A]
+    own = g.owned_slice()
+    i0, j0 = g.ilo + 3, g.jlo + 3
+    B[1][14, 2, i0 - 1, j0 - 1] += 1e-3 * abs(B[0][13, 2, i0 - 1, j0 - 1])   # Auv, neighbour (+1, 0, 0)
+    r, x, _ = _mixed_run(g, act, B, bu, bv, pcg.MIXED_FP32, full=False)
+    assert r["spmv_kind"] == 1
