@@ -83,6 +83,8 @@ struct cismgpu_system {
     double* load3 = nullptr;     // 2-D system: nz_levels-level load work pair
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     bool a32_sym = false;        // FP32 copy in half (symmetric) storage
+    double* A64h = nullptr;      // FP64 half (symmetric) storage, when the FP64 matrix is exactly symmetric
+    bool a64h_ready = false, a64_sym = false;
     std::vector<hipEvent_t> ev;    // SpMV timing events (pairs)
     int ev_used = 0;
     bool timing = false;
@@ -164,8 +166,12 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
     if (t) hipEventRecord(S->ev[S->ev_used], S->s);
     if (S->two_d)   // one level: only the kA = 0 plane of the rows is read
         k_spmv_plane<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
-    else if constexpr (std::is_same<TA, double>::value)
-        k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    else if constexpr (std::is_same<TA, double>::value) {
+        if (S->a64_sym && (const double*)A == S->A64)   // FP64 half (symmetric) storage
+            k_spmv_sym64<MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, S->A64h, xu, xv, yu, yv, bu, bv, part);
+        else
+            k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    }
     else if (std::is_same<TA, float>::value && S->a32_sym)   // FP32 half (symmetric) storage
         k_spmv_sym<float, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, (const float*)A, xu, xv, yu, yv, bu, bv, part);
     else   // FP32 / BF16 copies in the quad layout
@@ -529,6 +535,31 @@ int std_solve(cismgpu_system* S, int precond, int ncheck, double tol, int maxite
     return CISMGPU_OK;
 }
 
+// FP64 half (symmetric) storage of the owned rows when the FP64 matrix is
+// exactly symmetric (3-D systems; CISMGPU_SPMV_FULL=1 keeps full rows only).
+int ensure_half64(cismgpu_system* S) {
+    if (S->a64h_ready) return CISMGPU_OK;
+    S->a64h_ready = true;
+    S->a64_sym = false;
+    const Dims& d = S->d;
+    const char* env = getenv("CISMGPU_SPMV_FULL");
+    if (S->two_d || (env && env[0] == '1')) return CISMGPU_OK;
+    unsigned long long* cnt = nullptr;
+    unsigned long long h = 1;
+    CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
+    CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
+    k_check_sym<double><<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, cnt);
+    CG_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    hipFree(cnt);
+    if (h != 0) return CISMGPU_OK;
+    const long long ns = d.nrows * NSLOT * 4;
+    if (!S->A64h) CG_CHECK(hipMalloc(&S->A64h, ns * sizeof(double)));
+    k_convert_sym64<<<nblocks(ns, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A64h);
+    S->a64_sym = true;
+    return CISMGPU_OK;
+}
+
 int ensure_lowp(cismgpu_system* S, int precision) {
     const Dims& d = S->d;
     const long long na = d.nrows * NCOEF;
@@ -722,7 +753,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -774,7 +805,8 @@ static int set_matrix_impl(cismgpu_system* S, const int32_t* active_vertex, int 
     CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_matrix = true;
-    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
+    S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
     return CISMGPU_OK;
 }
@@ -892,6 +924,8 @@ int32_t cismgpu_system_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cis
         st = setup_tri(S, o->precond);
         if (st) return st;
     }
+    st = ensure_half64(S);
+    if (st) return st;
     S->timing = o->time_spmv != 0;
     if (S->timing && S->ev.empty()) {
         S->ev.resize(8192);
@@ -959,6 +993,9 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
         st = ensure_lowp(S, kind);
         if (st) return st;
         k_cast<double, float><<<nblocks(2 * d.vlen, 256), 256, 0, S->s>>>(2 * d.vlen, S->V(0), S->W(10));
+    } else {
+        st = ensure_half64(S);
+        if (st) return st;
     }
     hipEvent_t e0, e1;
     CG_CHECK(hipEventCreate(&e0));
@@ -972,7 +1009,9 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
                 k_spmv_plane<float, float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
             else
                 k_spmv_plane<__hip_bfloat16, float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A16, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
-        } else if (kind == 0)
+        } else if (kind == 0 && S->a64_sym)
+            k_spmv_sym64<1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64h, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
+        else if (kind == 0)
             k_spmv<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
         else if (kind == 1 && S->a32_sym)
             k_spmv_sym<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
@@ -993,7 +1032,7 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     if (avg_ms) *avg_ms = ms / nrep;
     const double esz = kind == 0 ? 8.0 : (kind == 1 ? 4.0 : 2.0);
     const double vsz = kind == 0 ? 8.0 : 4.0;
-    const bool half = kind == 1 && S->a32_sym;
+    const bool half = (kind == 1 && S->a32_sym) || (kind == 0 && S->a64_sym);
     // 4 blocks x 9 or x 27 neighbours; half storage: 14 slots of 4
     const double ncoef = S->two_d ? 36.0 : (half ? 4.0 * NSLOT : (double)NCOEF);
     if (bytes_per_launch) *bytes_per_launch = (double)nodes * (ncoef * esz + 4.0 * vsz);
@@ -1272,7 +1311,8 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         info->total_ms = ms2;
     }
     S->have_matrix = true;
-    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
+    S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
     if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
     return CISMGPU_OK;
@@ -1359,7 +1399,8 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         info->total_ms = ms2;
     }
     S->have_matrix = true;
-    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
+    S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
     if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
     return CISMGPU_OK;
@@ -1422,7 +1463,8 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
         if (sizes[q]) CG_CHECK(hipMemcpyAsync(dst[q], parts[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_matrix = true;
-    S->m32_ready = S->a32_ready = S->a16_ready = false;
+    S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
+    S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
     return CISMGPU_OK;
 }

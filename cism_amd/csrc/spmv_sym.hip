@@ -59,8 +59,8 @@ __global__ void k_convert_sym(long long nrows, const double* __restrict__ src, T
     dst[t] = to_lowp<TA>(val);
 }
 
-// Counts the couplings that the half storage cannot reproduce exactly after
-// rounding to TA: a lower coupling whose transposed partner differs, a
+// Counts the couplings that the half storage cannot reproduce exactly in
+// precision TA (float or double): a lower coupling whose transposed partner differs, a
 // nonzero coupling to a node without a stored row, or Auv != Avu at the
 // centre.  count[0] += mismatches.
 template <typename TA>
@@ -74,7 +74,7 @@ __global__ void k_check_sym(Dims d, const double* __restrict__ A, unsigned long 
         if (io < d.nxo) {
             const int i = d.ilo + io, j = d.jlo + jo;
             const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
-            auto rd = [](double x) { return (float)to_lowp<TA>(x); };
+            auto rd = [](double x) { return to_lowp<TA>(x); };
             if (rd(a[(1 * 27 + 13) * WAVE]) != rd(a[(2 * 27 + 13) * WAVE])) ++bad;
             for (int m = 0; m < 13; ++m) {
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
@@ -203,6 +203,144 @@ __global__ __launch_bounds__(256) void k_spmv_sym(Dims d, const TA* __restrict__
     }
     if (MODE != 0) {
         dd v[1] = {{acc, 0.0}};
+        block_reduce_dd<1, 256>(v);
+        if (threadIdx.x == 0) {
+            partials[2 * lin + 0] = v[0].hi;
+            partials[2 * lin + 1] = v[0].lo;
+        }
+    }
+}
+
+// FP64 half storage: the same slots with double quads,
+//     Sd[((r/64)*14 + s)*256 + (r%64)*4 + c]
+// (448 B per node instead of 864), for the FP64 solves when the FP64 matrix
+// is exactly symmetric.  Sums in the order and with the values of k_spmv,
+// so the results are identical.
+__global__ void k_convert_sym64(long long nrows, const double* __restrict__ src, double* __restrict__ dst) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nrows * NSLOT * 4) return;
+    const int c = (int)(t % 4);
+    const long long u = t / 4;
+    const int lane = (int)(u % WAVE);
+    const long long v = u / WAVE;
+    const int s = (int)(v % NSLOT);
+    const long long rb = v / NSLOT;
+    const double* row = src + rb * NCOEF * WAVE + lane;
+    double val;
+    if (s == 0) {
+        const int blk = c == 0 ? 0 : (c == 1 ? 1 : 3);
+        val = c == 3 ? 0.0 : row[(blk * 27 + 13) * WAVE];
+    } else {
+        val = row[(c * 27 + 13 + s) * WAVE];
+    }
+    dst[t] = val;
+}
+
+__device__ __forceinline__ void ld_slot64(const double* p, double& a, double& b, double& c, double& e) {
+    const double2 q0 = *reinterpret_cast<const double2*>(p);
+    const double2 q1 = *reinterpret_cast<const double2*>(p + 2);
+    a = q0.x; b = q0.y; c = q1.x; e = q1.y;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spmv_sym64(Dims d, const double* __restrict__ S,
+                                                    const double* __restrict__ xu, const double* __restrict__ xv,
+                                                    double* __restrict__ yu, double* __restrict__ yv,
+                                                    const double* __restrict__ bu, const double* __restrict__ bv,
+                                                    double* __restrict__ partials) {
+    __shared__ double xs[2][3][6][WAVE + 2];
+    const int nbx = d.nxp / WAVE;
+    const int nblk = nbx * ((d.nyo + 3) / 4);
+    const int lin = xcd_remap(blockIdx.x, nblk);
+    const int bx = lin % nbx, by = lin / nbx;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int io = bx * WAVE + lane;
+    const int jo = by * 4 + w;
+    const bool live = io < d.nxo && jo < d.nyo;
+    const int i0 = d.ilo + bx * WAVE - 1, j0 = d.jlo + by * 4 - 1;
+    auto stage = [&](int kk) {
+        const int slot = (kk + 1) % 3;
+        for (int e = threadIdx.x; e < 6 * (WAVE + 2); e += 256) {
+            const int r = e / (WAVE + 2), c = e % (WAVE + 2);
+            const int vi = i0 + c, vj = j0 + r;
+            double a = 0.0, b = 0.0;
+            if (vi >= 1 && vi <= d.nvx && vj >= 1 && vj <= d.nvy) {
+                const long long n = vidx(d, kk, vi, vj);
+                a = xu[n];
+                b = xv[n];
+            }
+            xs[0][slot][r][c] = a;
+            xs[1][slot][r][c] = b;
+        }
+    };
+    dd acc = {0.0, 0.0};
+    int po[9];
+    const long long LS = (long long)d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);
+    if (live) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            const int pc = partner_col(d, i + h % 3 - 1, j + h / 3 - 1);
+            po[h] = pc < 0 ? -1 : (pc / WAVE) * (NSLOT * 4 * WAVE) + (pc % WAVE) * 4;
+        }
+    }
+    stage(-1);
+    stage(0);
+    for (int k = 0; k < d.nz; ++k) {
+        stage(k + 1);
+        __syncthreads();
+        if (live) {
+            const long long ck = vidx(d, k, d.ilo + io, d.jlo + jo);
+            const double* Sk = S + (long long)k * LS;
+            const double* Sb = Sk - LS;
+            double su = 0.0, sv = 0.0;
+#pragma unroll
+            for (int m = 0; m < 27; ++m) {
+                double cuu = 0.0, cuv = 0.0, cvu = 0.0, cvv = 0.0;
+                bool have = true;
+                if (m < 13) {
+                    const int h = m % 9;
+                    have = po[h] >= 0 && (m >= 9 || k > 0);
+                    if (have) {
+                        double a, b, c, e;
+                        ld_slot64((m < 9 ? Sb : Sk) + po[h] + (13 - m) * (4 * WAVE), a, b, c, e);
+                        cuu = a; cuv = c; cvu = b; cvv = e;
+                    }
+                } else if (m == 13) {
+                    double a, b, c, e;
+                    ld_slot64(Sk + po[4], a, b, c, e);
+                    cuu = a; cuv = b; cvu = b; cvv = c;
+                } else {
+                    ld_slot64(Sk + po[4] + (m - 13) * (4 * WAVE), cuu, cuv, cvu, cvv);
+                }
+                if (have) {
+                    const int slot = (k + m / 9) % 3;
+                    const int r = w + 1 + ((m / 3) % 3 - 1), c = lane + 1 + (m % 3 - 1);
+                    const double xun = xs[0][slot][r][c], xvn = xs[1][slot][r][c];
+                    su = su + cuu * xun + cuv * xvn;
+                    sv = sv + cvu * xun + cvv * xvn;
+                }
+            }
+            if (MODE == 2) {
+                const double ru = bu[ck] - su, rv = bv[ck] - sv;
+                yu[ck] = ru;
+                yv[ck] = rv;
+                acc = dd_add_d(acc, ru * ru);
+                acc = dd_add_d(acc, rv * rv);
+            } else {
+                yu[ck] = su;
+                yv[ck] = sv;
+                if (MODE == 1) {
+                    const int slot = (k + 1) % 3;
+                    acc = dd_add_d(acc, su * xs[0][slot][w + 1][lane + 1]);
+                    acc = dd_add_d(acc, sv * xs[1][slot][w + 1][lane + 1]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE != 0) {
+        dd v[1] = {acc};
         block_reduce_dd<1, 256>(v);
         if (threadIdx.x == 0) {
             partials[2 * lin + 0] = v[0].hi;

@@ -182,3 +182,30 @@ def test_half_storage_needs_exact_symmetry(require_gpu):
     B[1][14, 2, i0 - 1, j0 - 1] += 1e-3 * abs(B[0][13, 2, i0 - 1, j0 - 1])   # Auv, neighbour (+1, 0, 0)
     r, x, _ = _mixed_run(g, act, B, bu, bv, pcg.MIXED_FP32, full=False)
     assert r["spmv_kind"] == 1
+
+
+@pytest.mark.parametrize("algo", [pcg.ALGO_STANDARD, pcg.ALGO_CHRONGEAR])
+@pytest.mark.parametrize("name", ["stream40", "random_bc0", "random_bc1"])
+def test_fp64_half_storage_gives_identical_solves(require_gpu, name, algo):
+    """The FP64 solves use half (symmetric) storage when the FP64 matrix is
+    exactly symmetric: identical iterates, 448 instead of 864 matrix bytes
+    per node."""
+    import os
+    g, act, A, bu, bv = SYSTEMS[name]
+    out = {}
+    for full in ("1", "0"):
+        os.environ["CISMGPU_SPMV_FULL"] = full
+        try:
+            with pcg.System(g) as s:
+                s.set_matrix(act, *A, bu, bv)
+                s.set_x(g.zeros_vec(), g.zeros_vec())
+                r = s.solve(algo=algo, precond=2, tolerance=1e-11, maxiters=3000)
+                out[full] = (r, np.stack(s.get_x()), s.bench_spmv(kind=0, nrep=3), s.matvec())
+        finally:
+            os.environ.pop("CISMGPU_SPMV_FULL", None)
+    (rf, xf, (_, byf), yf), (rh, xh, (_, byh), yh) = out["1"], out["0"]
+    assert rh["niters"] == rf["niters"] and rh["err"] == rf["err"]
+    assert np.array_equal(xh, xf)
+    assert np.array_equal(yh[0], yf[0]) and np.array_equal(yh[1], yf[1])
+    nodes = g.n_owned_nodes
+    assert byf == nodes * (108 * 8 + 32) and byh == nodes * (56 * 8 + 32)
