@@ -110,6 +110,7 @@ def main():
         t64.append(time.perf_counter() - tw)
         it64.append(r["niters"])
         sp64.append((r["spmv_ms"], r["spmv_count"]))
+        r64_kind = r["spmv_kind"]
         log(f"fp64 step {k}: {r['niters']} it, device {r['solve_ms']:.1f} ms, wall {t64[-1] * 1e3:.1f} ms")
     err64 = rel_err()
 
@@ -146,7 +147,8 @@ def main():
         with open(pmc_path) as f:
             traffic = json.load(f)["kernels"].get(kname, {}).get("traffic_bytes")
     sp64_ms = sum(a for a, _ in sp64) / max(sum(b for _, b in sp64), 1)
-    ach64 = nodes * (108 * 8 + 4 * 8) / (sp64_ms * 1e-3) / 1e9
+    coef64 = 56 * 8 if r64_kind == 4 else 108 * 8      # FP64 half (symmetric) or full rows
+    ach64 = nodes * (coef64 + 4 * 8) / (sp64_ms * 1e-3) / 1e9
 
     # isolated SpMV rates (back-to-back launches, HIP events)
     spmv_iso = {}
@@ -242,7 +244,8 @@ def main():
         },
         "fp64": {"solves_per_s": round(1e3 / ms_fp64, 4), "ms_per_solve": round(ms_fp64, 2),
                  "iterations": it64, "rel_err_vs_tight_fp64": err64,
-                 "spmv_avg_ms": round(sp64_ms, 4), "spmv_gbps": round(ach64, 1)},
+                 "spmv_avg_ms": round(sp64_ms, 4), "spmv_gbps": round(ach64, 1),
+                 "spmv_storage": "half (symmetric)" if r64_kind == 4 else "full rows"},
         "mixed": {"inner_iterations": itm, "outer_iterations": outm, "rel_err_vs_tight_fp64": errm,
                   "meets_1e-6": bool(errm <= 1e-6)},
         "speedup_mixed_vs_fp64": round(ms_fp64 / ms_mixed, 3),

@@ -948,7 +948,7 @@ int32_t cismgpu_system_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cis
         }
         info->err = err;
         info->niters = nit;
-        info->spmv_kind = 0;
+        info->spmv_kind = S->a64_sym ? 4 : 0;
     } else {
         st = mixed_solve(S, o, info);
     }

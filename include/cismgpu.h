@@ -152,7 +152,7 @@ typedef struct cismgpu_solve_info {
     double  solve_ms;        /* device time of the solve (HIP events on the system stream) */
     double  spmv_ms;         /* summed duration of the timed SpMV launches */
     int32_t spmv_count;      /* number of timed SpMV launches */
-    int32_t spmv_kind;       /* storage of the dominant SpMV: 0 fp64, 1 fp32, 2 bf16, 3 fp32 half (symmetric) storage */
+    int32_t spmv_kind;       /* storage of the dominant SpMV: 0 fp64, 1 fp32, 2 bf16, 3 fp32 half (symmetric), 4 fp64 half */
 } cismgpu_solve_info;
 
 int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out);
