@@ -81,6 +81,32 @@ int64_t orc_count_nonzeros_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi
                               const double* Auu, const double* Auv, const double* Avu, const double* Avv);
 double orc_residual_velocity_2d(int nx, int ny, int ilo, int ihi, int jlo, int jhi, int whichresid,
                                 const double* usav, const double* vsav, const double* uvel, const double* vvel);
+void orc_slope_angle_xy(int nx, int ny, double dx, double dy, const double* zsfc, double* theta, double* thx,
+                        double* thy);
+void orc_slope_angle_staggered(int nx, int ny, double dx, double dy, const double* zsfc, double* theta,
+                               double* thx, double* thy);
+void orc_cell_halo(int nx, int ny, int global_bc, double* a);
+int orc_assemble_stiffness_matrix_2d_diva(int nx, int ny, int nz, const double* sigma, const double* stagsigma,
+                                          int nhalo, const int32_t* active_cell, const double* xVertex,
+                                          const double* yVertex, const double* uvel_2d, const double* vvel_2d,
+                                          const double* stagthck, const double* thck, const double* flwafact,
+                                          const double* btractx, const double* btracty, const double* sfx_cell,
+                                          const double* sfy_cell, int whichefvs, double efvs_constant,
+                                          double effstrain_min, double n_glen, double* efvs_qp_3d, double* efvs,
+                                          double* omega_k, double* omega, double* Auu, double* Auv, double* Avu,
+                                          double* Avv);
+void orc_diva_beta_eff(int nx, int ny, int no_slip, const double* stag_omega, const double* beta,
+                       const double* ssfx, const double* ssfy, const double* sthx, const double* sthy,
+                       double* beta_eff_x, double* beta_eff_y);
+void orc_basal_sliding_bc_2d_diva_local(int nx, int ny, double dx, double dy, const int32_t* active_vertex,
+                                        const double* beta_eff_x, const double* beta_eff_y, const double* tbx,
+                                        const double* tby, double* Auu, double* Avv);
+void orc_compute_3d_velocity_diva(int nx, int ny, int nz, const int32_t* active_vertex, const int32_t* ipl_mask,
+                                  const double* omega_k, const double* beta, const double* beta_eff_x,
+                                  const double* beta_eff_y, const double* sthx, const double* sthy,
+                                  const double* ssfx, const double* ssfy, const double* uvel_2d,
+                                  const double* vvel_2d, double* btractx, double* btracty, double* uvel,
+                                  double* vvel);
 
 #ifdef __cplusplus
 }

@@ -1058,3 +1058,289 @@ double orc_residual_velocity_2d(int nx, int ny, int ilo, int ihi, int jlo, int j
     if (whichresid == 2 && count > 0) r = r / (double)count;
     return r;
 }
+
+/* ------------------------------------------------------------------ */
+/* 2-D assembly, DIVA approximation (Goldberg 2011)                     */
+/* ------------------------------------------------------------------ */
+
+/* glissade_slope_angle with the x and y angles (glissade_grid_operators.F90:
+ * 1264-1410): theta_x = atan|dz/dx|, theta_y = atan|dz/dy| of the
+ * unstaggered gradient; zero on the outer ring. */
+void orc_slope_angle_xy(int nx, int ny, double dx, double dy, const double* zsfc, double* theta, double* thx,
+                        double* thy)
+{
+    double* gx = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    double* gy = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    gradient_full(nx, ny, dx, dy, zsfc, gx, gy);
+    for (size_t p = 0; p < (size_t)nx * ny; ++p) { theta[p] = 0.0; thx[p] = 0.0; thy[p] = 0.0; }
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const double dzx = (gx[V2(i, j)] + gx[V2(i - 1, j)] + gx[V2(i, j - 1)] + gx[V2(i - 1, j - 1)]) / 4.0;
+            const double dzy = (gy[V2(i, j)] + gy[V2(i - 1, j)] + gy[V2(i, j - 1)] + gy[V2(i - 1, j - 1)]) / 4.0;
+            const double s = sqrt(dzx * dzx + dzy * dzy);
+            theta[C2(i, j)] = (s >= 0.0) ? atan(s) : 0.0;
+            thx[C2(i, j)] = atan(fabs(dzx));
+            thy[C2(i, j)] = atan(fabs(dzy));
+        }
+    free(gx); free(gy);
+}
+
+/* glissade_slope_angle_staggered (glissade_grid_operators.F90:1414-1552):
+ * angles of the vertex gradient (glissade_gradient, margin 1, full mask) */
+void orc_slope_angle_staggered(int nx, int ny, double dx, double dy, const double* zsfc, double* theta,
+                               double* thx, double* thy)
+{
+    double* gx = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    double* gy = (double*)calloc((size_t)(nx - 1) * (ny - 1), sizeof(double));
+    gradient_full(nx, ny, dx, dy, zsfc, gx, gy);
+    for (size_t p = 0; p < (size_t)(nx - 1) * (ny - 1); ++p) {
+        const double s = sqrt(gx[p] * gx[p] + gy[p] * gy[p]);
+        theta[p] = (s >= 0.0) ? atan(s) : 0.0;
+        thx[p] = atan(fabs(gx[p]));
+        thy[p] = atan(fabs(gy[p]));
+    }
+    free(gx); free(gy);
+}
+
+/* parallel_halo_real8_2d for a cell field on one task, lhalo = uhalo = 2
+ * (parallel_mpi.F90:7720-7912): periodic wrap (E/W over owned rows, then N/S
+ * over all columns); outflow zeroes the 2 halo rows; no_ice zeroes 3 rows
+ * on every side (the halo and the first/last owned row). */
+void orc_cell_halo(int nx, int ny, int global_bc, double* a)
+{
+    if (global_bc == 0 || global_bc == 2) {
+        for (int j = 3; j <= ny - 2; ++j) {
+            a[C2(1, j)] = a[C2(nx - 3, j)]; a[C2(2, j)] = a[C2(nx - 2, j)];
+            a[C2(nx - 1, j)] = a[C2(3, j)]; a[C2(nx, j)] = a[C2(4, j)];
+        }
+        for (int i = 1; i <= nx; ++i) {
+            a[C2(i, 1)] = a[C2(i, ny - 3)]; a[C2(i, 2)] = a[C2(i, ny - 2)];
+            a[C2(i, ny - 1)] = a[C2(i, 3)]; a[C2(i, ny)] = a[C2(i, 4)];
+        }
+    } else {
+        const int w = (global_bc == 1) ? 2 : 3;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                if (i <= w || i >= nx - w + 1 || j <= w || j >= ny - w + 1) a[C2(i, j)] = 0.0;
+    }
+}
+
+/* compute_effective_viscosity_diva (glissade_velo_higher.F90:7730-7997),
+ * the default (non-cubic) branch: layer k's vertical shear is estimated from
+ * the basal traction and the previous efvs(k) (initialised from the minimum
+ * strain rate when zero), efvs(k) is updated in place. */
+static void efvs_diva(int whichefvs, double efvs_constant, double effstrain_min, double n_glen, int nz,
+                      const double* stagsigma, int p, const double* dx, const double* dy, const double* u,
+                      const double* v, const double* bx, const double* by, double sfx, double sfy, const double* h,
+                      const double* flwafact, double* efvs)
+{
+    const double p_effstr = (1.0 - n_glen) / n_glen;
+    if (whichefvs == 0) { for (int k = 0; k < nz - 1; ++k) efvs[k] = efvs_constant; return; }
+    if (whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
+        for (int k = 0; k < nz - 1; ++k) efvs[k] = flwafact[k] * pow(effstrain, p_effstr);
+        return;
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0, thck = 0, taux = 0, tauy = 0;
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+        taux = taux + phi2[n][p] * bx[n];
+        tauy = tauy + phi2[n][p] * by[n];
+        thck = thck + phi2[n][p] * h[n];
+    }
+    (void)thck;
+    for (int k = 0; k < nz - 1; ++k) {
+        if (efvs[k] == 0.0) efvs[k] = flwafact[k] * pow(effstrain_min, p_effstr);
+        const double du_dz = taux * stagsigma[k] / (efvs[k] * sfx);
+        const double dv_dz = tauy * stagsigma[k] / (efvs[k] * sfy);
+        const double esq = effstrain_min * effstrain_min + ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)) +
+                           0.25 * (du_dz * du_dz) * sfx + 0.25 * (dv_dz * dv_dz) * sfy;
+        efvs[k] = flwafact[k] * pow(esq, p_effstr / 2.0);
+    }
+}
+
+/* compute_integrals_diva (glissade_velo_higher.F90:5649-5775): omega_k(k) =
+ * mean over the quadrature points of the integral from the bed up to level
+ * k of depth/efvs dz (Goldberg 2011 eq. 32), omega = its vertical mean
+ * (eq. 35, divided by H) */
+static void integrals_diva(int nz, const double* sigma, double thck, const double* efvs_qp /* [p][k] */,
+                           double* omega_k, double* omega)
+{
+    double okp[4][65];
+    for (int p = 0; p < 4; ++p) {
+        okp[p][nz - 1] = 0.0;
+        for (int k = nz - 2; k >= 0; --k) {
+            const double depth = 0.5 * (sigma[k] + sigma[k + 1]);
+            const double dz = (sigma[k + 1] - sigma[k]) * thck;
+            okp[p][k] = okp[p][k + 1] + depth / efvs_qp[p * (nz - 1) + k] * dz;
+        }
+    }
+    for (int k = 0; k < nz; ++k) omega_k[k] = (((okp[0][k] + okp[1][k]) + okp[2][k]) + okp[3][k]) / 4.0;
+    double om = 0.0;
+    for (int k = 0; k < nz - 1; ++k) om = om + 0.5 * (omega_k[k] + omega_k[k + 1]) * (sigma[k + 1] - sigma[k]);
+    *omega = om;
+}
+
+/* assemble_stiffness_matrix_2d, DIVA (glissade_velo_higher.F90:4961-5381):
+ * per active cell and quadrature point the layer viscosities of the state
+ * efvs_qp_3d (nz-1, 4, nx, ny) are updated from the 2-D velocity and the
+ * basal traction (btractx/y at vertices), their vertical mean times the
+ * interpolated thickness weights the element matrix (the BP form with
+ * dphi_dz = 0), and omega_k (nz, nx, ny), omega (nx, ny) are formed from
+ * the cell thickness thck (nx, ny).  efvs(k,i,j) = mean over the points. */
+int orc_assemble_stiffness_matrix_2d_diva(int nx, int ny, int nz, const double* sigma, const double* stagsigma,
+                                          int nhalo, const int32_t* active_cell, const double* xVertex,
+                                          const double* yVertex, const double* uvel_2d, const double* vvel_2d,
+                                          const double* stagthck, const double* thck, const double* flwafact,
+                                          const double* btractx, const double* btracty, const double* sfx_cell,
+                                          const double* sfy_cell, int whichefvs, double efvs_constant,
+                                          double effstrain_min, double n_glen, double* efvs_qp_3d, double* efvs,
+                                          double* omega_k, double* omega, double* Auu, double* Auv, double* Avu,
+                                          double* Avv)
+{
+    init_tables_2d();
+    const size_t nA = (size_t)9 * NVX * NVY;
+    memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    memset(omega_k, 0, (size_t)nz * nx * ny * sizeof(double));
+    memset(omega, 0, (size_t)nx * ny * sizeof(double));
+    memset(Auu, 0, nA * sizeof(double)); memset(Auv, 0, nA * sizeof(double));
+    memset(Avu, 0, nA * sizeof(double)); memset(Avv, 0, nA * sizeof(double));
+    double x[4], y[4], u[4], v[4], h[4], bx[4], by[4];
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int n = 0; n < 4; ++n) {
+                const size_t q = V2(i + di2[n], j + dj2[n]);
+                x[n] = xVertex[q]; y[n] = yVertex[q]; u[n] = uvel_2d[q]; v[n] = vvel_2d[q];
+                h[n] = stagthck[q]; bx[n] = btractx[q]; by[n] = btracty[q];
+            }
+            double* st = efvs_qp_3d + (size_t)(nz - 1) * 4 * C2(i, j);   /* [p][k] */
+            const double* fl = flwafact + (size_t)(nz - 1) * C2(i, j);
+            double Kuu[4][4] = {{0}}, Kuv[4][4] = {{0}}, Kvu[4][4] = {{0}}, Kvv[4][4] = {{0}};
+            for (int p = 0; p < 4; ++p) {
+                double dx[4], dy[4], detJ;
+                if (basis_derivs_2d(x, y, p, dx, dy, &detJ)) return ORC_ERR_SINGULAR;
+                efvs_diva(whichefvs, efvs_constant, effstrain_min, n_glen, nz, stagsigma, p, dx, dy, u, v, bx, by,
+                          sfx_cell[C2(i, j)], sfy_cell[C2(i, j)], h, fl, st + (size_t)p * (nz - 1));
+                double va = 0.0;
+                for (int k = 0; k < nz - 1; ++k) va = va + st[p * (nz - 1) + k] * (sigma[k + 1] - sigma[k]);
+                double h_qp = 0.0;
+                for (int n = 0; n < 4; ++n) h_qp = h_qp + phi2[n][p] * h[n];
+                const double f = (h_qp * va) * 1.0 * detJ / VOL0;
+                for (int nc = 0; nc < 4; ++nc)
+                    for (int nr = 0; nr < 4; ++nr) {
+                        Kuu[nr][nc] = Kuu[nr][nc] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc]);
+                        Kuv[nr][nc] = Kuv[nr][nc] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                        Kvu[nr][nc] = Kvu[nr][nc] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                        Kvv[nr][nc] = Kvv[nr][nc] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc]);
+                    }
+            }
+            integrals_diva(nz, sigma, thck[C2(i, j)], st, omega_k + (size_t)nz * C2(i, j), omega + C2(i, j));
+            for (int k = 0; k < nz - 1; ++k) {
+                double s = 0.0;
+                for (int p = 0; p < 4; ++p) s = s + st[p * (nz - 1) + k];
+                efvs[(size_t)k + (size_t)(nz - 1) * C2(i, j)] = s / 4.0;
+            }
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    if (fabs(Kuu[a][b] - Kuu[b][a]) > EPS11 || fabs(Kvv[a][b] - Kvv[b][a]) > EPS11 ||
+                        fabs(Kuv[a][b] - Kvu[b][a]) > EPS11)
+                        return ORC_ERR_ASYM;
+            for (int nr = 0; nr < 4; ++nr) {
+                const int ir = i + di2[nr], jr = j + dj2[nr];
+                for (int nc = 0; nc < 4; ++nc) {
+                    const int m = IDX2(di2[nc] - di2[nr], dj2[nc] - dj2[nr]);
+                    const size_t a = A2(m, ir, jr);
+                    Auu[a] = Auu[a] + Kuu[nr][nc];
+                    Auv[a] = Auv[a] + Kuv[nr][nc];
+                    Avu[a] = Avu[a] + Kvu[nr][nc];
+                    Avv[a] = Avv[a] + Kvv[nr][nc];
+                }
+            }
+        }
+    return ORC_OK;
+}
+
+/* beta_eff of the DIVA solve (glissade_velo_higher.F90:2469-2560), slope
+ * correction on: slip  beta_eff_x = sfx beta / (sfx + beta stag_omega cos(theta_x));
+ * no-slip (where stag_omega > 0)  beta_eff_x = sfx / (stag_omega cos(theta_x)). */
+void orc_diva_beta_eff(int nx, int ny, int no_slip, const double* stag_omega, const double* beta,
+                       const double* ssfx, const double* ssfy, const double* sthx, const double* sthy,
+                       double* beta_eff_x, double* beta_eff_y)
+{
+    for (size_t p = 0; p < (size_t)NVX * NVY; ++p) {
+        beta_eff_x[p] = 0.0; beta_eff_y[p] = 0.0;
+        if (no_slip) {
+            if (stag_omega[p] > 0.0) {
+                beta_eff_x[p] = ssfx[p] / (stag_omega[p] * cos(sthx[p]));
+                beta_eff_y[p] = ssfy[p] / (stag_omega[p] * cos(sthy[p]));
+            }
+        } else {
+            beta_eff_x[p] = (ssfx[p] * beta[p]) / (ssfx[p] + beta[p] * stag_omega[p] * cos(sthx[p]));
+            beta_eff_y[p] = (ssfy[p] * beta[p]) / (ssfy[p] + beta[p] * stag_omega[p] * cos(sthy[p]));
+        }
+    }
+}
+
+/* basal_sliding_bc_2d_diva, local option (glissade_velo_higher.F90:8535-8811):
+ * diag += dx dy / vol0 * beta_eff / cos(theta_basal) per direction */
+void orc_basal_sliding_bc_2d_diva_local(int nx, int ny, double dx, double dy, const int32_t* active_vertex,
+                                        const double* beta_eff_x, const double* beta_eff_y, const double* tbx,
+                                        const double* tby, double* Auu, double* Avv)
+{
+    const int m = IDX2(0, 0);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (!active_vertex[V2(i, j)]) continue;
+            Auu[A2(m, i, j)] = Auu[A2(m, i, j)] + (dx * dy / VOL0) * beta_eff_x[V2(i, j)] / cos(tbx[V2(i, j)]);
+            Avv[A2(m, i, j)] = Avv[A2(m, i, j)] + (dx * dy / VOL0) * beta_eff_y[V2(i, j)] / cos(tby[V2(i, j)]);
+        }
+}
+
+/* compute_3d_velocity_diva (glissade_velo_higher.F90:5779-5900): basal
+ * traction btract = beta_eff u_2d (every vertex), then on active vertices
+ * the basal velocity u_b = u_2d beta_eff / beta (u_2d where beta = 0) and
+ * u(k) = u_b + btract stag_omega_k(k) cos(theta) / slope_factor, with
+ * stag_omega_k the masked stagger (ice or land cells) of omega_k. */
+void orc_compute_3d_velocity_diva(int nx, int ny, int nz, const int32_t* active_vertex, const int32_t* ipl_mask,
+                                  const double* omega_k, const double* beta, const double* beta_eff_x,
+                                  const double* beta_eff_y, const double* sthx, const double* sthy,
+                                  const double* ssfx, const double* ssfy, const double* uvel_2d,
+                                  const double* vvel_2d, double* btractx, double* btracty, double* uvel,
+                                  double* vvel)
+{
+    const size_t n2 = (size_t)NVX * NVY, nc = (size_t)nx * ny;
+    for (size_t p = 0; p < n2; ++p) {
+        btractx[p] = beta_eff_x[p] * uvel_2d[p];
+        btracty[p] = beta_eff_y[p] * vvel_2d[p];
+    }
+    double* ok = (double*)malloc(nc * sizeof(double));
+    double* sok = (double*)malloc((size_t)nz * n2 * sizeof(double));
+    for (int k = 0; k < nz; ++k) {
+        for (size_t c = 0; c < nc; ++c) ok[c] = omega_k[(size_t)k + (size_t)nz * c];
+        orc_stagger_masked(nx, ny, ok, ipl_mask, sok + (size_t)k * n2);
+    }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const size_t p = V2(i, j);
+            if (!active_vertex[p]) continue;
+            double ub, vb;
+            if (beta[p] > 0.0) {
+                ub = uvel_2d[p] * beta_eff_x[p] / beta[p];
+                vb = vvel_2d[p] * beta_eff_y[p] / beta[p];
+            } else {
+                ub = uvel_2d[p];
+                vb = vvel_2d[p];
+            }
+            uvel[V3(nz, i, j)] = ub;
+            vvel[V3(nz, i, j)] = vb;
+            const double cx = cos(sthx[p]) / ssfx[p], cy = cos(sthy[p]) / ssfy[p];
+            for (int k = 1; k <= nz - 1; ++k) {
+                uvel[V3(k, i, j)] = uvel[V3(nz, i, j)] + btractx[p] * sok[(size_t)(k - 1) * n2 + p] * cx;
+                vvel[V3(k, i, j)] = vvel[V3(nz, i, j)] + btracty[p] * sok[(size_t)(k - 1) * n2 + p] * cy;
+            }
+        }
+    free(ok); free(sok);
+}

@@ -99,6 +99,16 @@ def _declare(L):
     L.orc_count_nonzeros_2d.restype = ctypes.c_int64
     L.orc_residual_velocity_2d.argtypes = [i, i, i, i, i, i, i, _D, _D, _D, _D]
     L.orc_residual_velocity_2d.restype = ctypes.c_double
+    L.orc_slope_angle_xy.argtypes = [i, i, d, d, _D, _D, _D, _D]
+    L.orc_slope_angle_staggered.argtypes = [i, i, d, d, _D, _D, _D, _D]
+    L.orc_cell_halo.argtypes = [i, i, i, _D]
+    L.orc_assemble_stiffness_matrix_2d_diva.argtypes = [i, i, i, _D, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D,
+                                                        _D, _D, i, d, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_assemble_stiffness_matrix_2d_diva.restype = ctypes.c_int
+    L.orc_diva_beta_eff.argtypes = [i, i, i, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_basal_sliding_bc_2d_diva_local.argtypes = [i, i, d, d, _I, _D, _D, _D, _D, _D, _D]
+    L.orc_compute_3d_velocity_diva.argtypes = [i, i, i, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D,
+                                               _D, _D]
 
 
 def dp(a: np.ndarray):
@@ -391,14 +401,54 @@ def vertical_average(sigma, var):
     return out
 
 
-def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None):
-    """One assembly of the 2-D SSA system in the reference's order
+def geometry_diva(problem, geo):
+    """DIVA inputs of glissade_velo_higher_solve with diva_slope_correction
+    (glissade_velo_higher.F90:2078-2140): slope angles of the mid-column
+    surface z_mean = (usrf + lsrf)/2 at cells (glissade_slope_angle, with the
+    cell halo) and vertices (glissade_slope_angle_staggered), the slope
+    factors cos^2 + 4 sin^2, the basal slope angles of the staggered lower
+    surface used by basal_sliding_bc_2d_diva (:8591-8605), and the
+    ice-or-land cell mask (:1510-1515)."""
+    L = lib()
+    p = problem
+    g = p.grid
+    nx, ny = g.nx, g.ny
+    zmean = np.asfortranarray(0.5 * (geo["usrf"] + geo["lsrf"]))
+    th = np.zeros((nx, ny), order="F"); thx = np.zeros_like(th); thy = np.zeros_like(th)
+    L.orc_slope_angle_xy(nx, ny, p.dx, p.dy, dp(zmean), dp(th), dp(thx), dp(thy))
+    L.orc_cell_halo(nx, ny, g.global_bc, dp(thx))
+    L.orc_cell_halo(nx, ny, g.global_bc, dp(thy))
+    sth = np.zeros((nx - 1, ny - 1), order="F"); sthx = np.zeros_like(sth); sthy = np.zeros_like(sth)
+    L.orc_slope_angle_staggered(nx, ny, p.dx, p.dy, dp(zmean), dp(sth), dp(sthx), dp(sthy))
+    sf = lambda t: np.asfortranarray(np.cos(t) ** 2 + 4.0 * np.sin(t) ** 2)
+    staglsrf = np.zeros((nx - 1, ny - 1), order="F")
+    L.orc_stagger(nx, ny, dp(geo["lsrf"]), dp(staglsrf))
+    tb = np.zeros((nx - 1, ny - 1), order="F"); tbx = np.zeros_like(tb); tby = np.zeros_like(tb)
+    L.orc_slope_angle_xy(nx - 1, ny - 1, p.dx, p.dy, dp(staglsrf), dp(tb), dp(tbx), dp(tby))
+    ipl = np.asfortranarray(((geo["ice_mask"] == 1) | (geo["land_mask"] == 1)).astype(np.int32))
+    return dict(theta_slope_x=thx, theta_slope_y=thy, slope_factor_x=sf(thx), slope_factor_y=sf(thy),
+                stag_theta_slope_x=sthx, stag_theta_slope_y=sthy, stag_slope_factor_x=sf(sthx),
+                stag_slope_factor_y=sf(sthy), theta_basal_x=tbx, theta_basal_y=tby, ice_plus_land_mask=ipl)
+
+
+def diva_state(problem):
+    """Cold-start DIVA state: efvs_qp_3d (nz-1, 4, nx, ny) and the basal
+    traction (nx-1, ny-1) start at zero."""
+    g = problem.grid
+    return dict(efvs_qp_3d=np.zeros((g.nz - 1, 4, g.nx, g.ny), order="F"),
+                btractx=np.zeros((g.nx - 1, g.ny - 1), order="F"),
+                btracty=np.zeros((g.nx - 1, g.ny - 1), order="F"))
+
+
+def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", state=None):
+    """One assembly of the 2-D system in the reference's order
     (glissade_velo_higher.F90:2353-2725): stiffness (assemble_stiffness_matrix_2d),
-    local basal BC (basal_sliding_bc_2d), b = the level sum of the 3-D load
-    (:2045-2058, :2636), Dirichlet rows with the basal-level masks
-    (dirichlet_boundary_conditions_2d), halo of A and b, symmetry report and
-    nonzero count.  The default 2-D velocity is the basal level of the 3-D
-    guess (:1362)."""
+    basal BC (basal_sliding_bc_2d; for DIVA beta_eff from omega and
+    basal_sliding_bc_2d_diva), b = the level sum of the 3-D load (:2045-2058,
+    :2636), Dirichlet rows with the basal-level masks (none for DIVA),
+    halo of A and b, symmetry report and nonzero count.  The default 2-D
+    velocity is the basal level of the 3-D guess (:1362).  approx "diva"
+    updates state (efvs_qp_3d; btractx/y are read) in place."""
     L = lib()
     p = problem
     g = p.grid
@@ -424,18 +474,46 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None):
         lv2 = lv2 + loadv[k]
     A = [np.zeros((nx - 1, ny - 1, 9), order="F") for _ in range(4)]
     efvs = np.zeros((nz - 1, nx, ny), order="F")
-    st = L.orc_assemble_stiffness_matrix_2d_ssa(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV), dp(yV),
-                                                dp(u2), dp(v2), dp(geo["stagthck"]), dp(geo["flwafact"]),
-                                                p.whichefvs, p.efvs_constant, p.effstrain_min, p.n_glen, dp(efvs),
-                                                *[dp(a) for a in A])
-    if st:
-        raise RuntimeError(f"oracle 2-D assembly failed with status {st}")
+    extra = {}
     binternal = beta_internal(p, geo)
-    if p.whichbabc != 6:
-        L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
+    if approx == "diva":
+        dg = geo.get("diva") or geometry_diva(p, geo)
+        geo["diva"] = dg
+        state = state if state is not None else diva_state(p)
+        stagsigma = np.ascontiguousarray((p.sigma[:-1] + p.sigma[1:]) / 2.0)
+        omega_k = np.zeros((nz, nx, ny), order="F")
+        omega = np.zeros((nx, ny), order="F")
+        thck = np.asfortranarray(np.array(p.thck, dtype=np.float64))
+        st = L.orc_assemble_stiffness_matrix_2d_diva(
+            nx, ny, nz, dp(sigma), dp(stagsigma), 2, ip(geo["active_cell"]), dp(xV), dp(yV), dp(u2), dp(v2),
+            dp(geo["stagthck"]), dp(thck), dp(geo["flwafact"]), dp(state["btractx"]), dp(state["btracty"]),
+            dp(dg["slope_factor_x"]), dp(dg["slope_factor_y"]), p.whichefvs, p.efvs_constant, p.effstrain_min,
+            p.n_glen, dp(state["efvs_qp_3d"]), dp(efvs), dp(omega_k), dp(omega), *[dp(a) for a in A])
+        if st:
+            raise RuntimeError(f"oracle 2-D DIVA assembly failed with status {st}")
+        L.orc_cell_halo(nx, ny, g.global_bc, dp(omega))
+        stag_omega = np.zeros((nx - 1, ny - 1), order="F")
+        L.orc_stagger_masked(nx, ny, dp(omega), ip(dg["ice_plus_land_mask"]), dp(stag_omega))
+        bex = np.zeros((nx - 1, ny - 1), order="F"); bey = np.zeros_like(bex)
+        L.orc_diva_beta_eff(nx, ny, 1 if p.whichbabc == 6 else 0, dp(stag_omega), dp(binternal),
+                            dp(dg["stag_slope_factor_x"]), dp(dg["stag_slope_factor_y"]),
+                            dp(dg["stag_theta_slope_x"]), dp(dg["stag_theta_slope_y"]), dp(bex), dp(bey))
+        L.orc_basal_sliding_bc_2d_diva_local(nx, ny, p.dx, p.dy, ip(av), dp(bex), dp(bey), dp(dg["theta_basal_x"]),
+                                             dp(dg["theta_basal_y"]), dp(A[0]), dp(A[3]))
+        extra = dict(omega_k=omega_k, omega=omega, stag_omega=stag_omega, beta_eff_x=bex, beta_eff_y=bey,
+                     state=state)
+    else:
+        st = L.orc_assemble_stiffness_matrix_2d_ssa(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV),
+                                                    dp(yV), dp(u2), dp(v2), dp(geo["stagthck"]),
+                                                    dp(geo["flwafact"]), p.whichefvs, p.efvs_constant,
+                                                    p.effstrain_min, p.n_glen, dp(efvs), *[dp(a) for a in A])
+        if st:
+            raise RuntimeError(f"oracle 2-D assembly failed with status {st}")
+        if p.whichbabc != 6:
+            L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
     bu = np.asfortranarray(lu2.copy()); bv = np.asfortranarray(lv2.copy())
     umask = np.zeros((nx - 1, ny - 1), np.int32, order="F")
-    if p.whichbabc == 6:
+    if p.whichbabc == 6 and approx != "diva":     # DIVA: no-slip through beta_eff (:1423-1428)
         umask[:, :] = 1
     vmask = umask.copy(order="F")
     L.orc_dirichlet_bc_2d(nx, ny, 2, ip(av), ip(umask), ip(vmask), dp(u2), dp(v2), *[dp(a) for a in A],
@@ -450,7 +528,24 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None):
     nnz = L.orc_count_nonzeros_2d(nx, ny, g.ilo, g.ihi, g.jlo, g.jhi, ip(av), *[dp(a) for a in A])
     return dict(Auu=A[0], Auv=A[1], Avu=A[2], Avv=A[3], bu=bu, bv=bv, efvs=efvs, uvel_2d=u2, vvel_2d=v2,
                 active_vertex=av, loadu_2d=lu2, loadv_2d=lv2, beta_internal=binternal, nwarn_asym=nwarn,
-                nnz=nnz, geo=geo)
+                nnz=nnz, geo=geo, **extra)
+
+
+def diva_velocity(problem, s, u2, v2):
+    """compute_3d_velocity_diva after a 2-D DIVA solve: updates the basal
+    traction in s["state"] and returns the 3-D velocity (nz, nx-1, ny-1)."""
+    L = lib()
+    p = problem
+    g = p.grid
+    dg = s["geo"]["diva"]
+    u3 = g.zeros_vec(); v3 = g.zeros_vec()
+    st = s["state"]
+    L.orc_compute_3d_velocity_diva(g.nx, g.ny, g.nz, ip(s["active_vertex"]), ip(dg["ice_plus_land_mask"]),
+                                   dp(s["omega_k"]), dp(s["beta_internal"]), dp(s["beta_eff_x"]), dp(s["beta_eff_y"]),
+                                   dp(dg["stag_theta_slope_x"]), dp(dg["stag_theta_slope_y"]),
+                                   dp(dg["stag_slope_factor_x"]), dp(dg["stag_slope_factor_y"]), dp(u2), dp(v2),
+                                   dp(st["btractx"]), dp(st["btracty"]), dp(u3), dp(v3))
+    return u3, v3
 
 
 def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
@@ -488,18 +583,25 @@ def residual_L2_2d(grid, active_vertex, A, bu, bv, uvel, vvel):
 
 
 def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chrongear", precond=2, ncheck=5,
-           linear_tol=1e-8, linear_maxiters=200, accelerate=False, two_d=False):
+           linear_tol=1e-8, linear_maxiters=200, accelerate=False, two_d=False, approx="ssa"):
     """Outer Picard loop of glissade_velo_higher_solve for the BP path
     (glissade_velo_higher.F90:2163-3663), which_ho_nonlinear = 0, or with
     Picard acceleration (:2186-2263, :3108-3176, :3624-3630;
     evaluate_accelerated_picard_3d :10052-10184).  two_d: the SSA branch
     (solve_2d; assemble_2d, the 2-D solvers with which_ho_precond 2 taken as
     1, compute_residual_velocity_2d), started from the basal level of the
-    3-D guess (:1362)."""
+    3-D guess (:1362).  approx "diva" (two_d): DIVA assembly with its state
+    and compute_3d_velocity_diva after each solve (:3566-3590), plain Picard
+    only; the result adds the 3-D velocity (uvel3d, vvel3d)."""
     L = lib()
     p = problem
     g = p.grid
     geo = geometry(p)
+    diva = two_d and approx == "diva"
+    if diva and accelerate:
+        raise NotImplementedError("accelerated Picard is restated for BP and SSA only")
+    dstate = diva_state(p) if diva else None
+    u3 = v3 = None
     if two_d:
         u = np.asfortranarray(p.uvel[g.nz - 1].copy())
         v = np.asfortranarray(p.vvel[g.nz - 1].copy())
@@ -535,7 +637,7 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
                 v = np.asfortranarray(v_old + alpha * dv)
                 halo(g, u); halo(g, v)
             usav, vsav = u.copy(order="F"), v.copy(order="F")
-            s = assemble_2d(p, u, v, geo) if two_d else assemble(p, u, v, geo)
+            s = (assemble_2d(p, u, v, geo, approx=approx, state=dstate) if two_d else assemble(p, u, v, geo))
             A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
             L2, L2r = (residual_L2_2d if two_d else residual_L2)(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
             hist["L2"].append(L2)
@@ -562,6 +664,9 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
             raise RuntimeError(f"oracle solver failed with status {st}")
         hist["linear_iters"].append(nit); hist["linear_err"].append(err)
         halo(g, u); halo(g, v)
+        if diva:
+            u3, v3 = diva_velocity(p, s, u, v)
+            halo(g, u3); halo(g, v3)
         if two_d:
             rv = L.orc_residual_velocity_2d(g.nx, g.ny, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
                                             dp(usav), dp(vsav), dp(u), dp(v))
@@ -573,4 +678,4 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
             du, dv = u - u_old, v - v_old
         criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
     return dict(uvel=u, vvel=v, efvs=s["efvs"] if s else None, iterations=counter,
-                converged=criterion < resid_target, **hist)
+                converged=criterion < resid_target, uvel3d=u3, vvel3d=v3, state=dstate, **hist)

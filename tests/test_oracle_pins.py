@@ -883,3 +883,76 @@ def test_ssa_basal_dirichlet_count_and_symmetry_report():
     scale = np.abs(s["bu"][own]).max()
     assert np.abs((yu - s["bu"])[own][free]).max() < 1e-7 * scale
     assert np.abs((yv - s["bv"])[own][free]).max() < 1e-7 * scale
+
+
+# ------------------------------------------------------------------- 2-D DIVA
+def _diva_slab(nz):
+    th = 5.0 * pi / 180
+    gn, rhoi, grav, beta, Hp = 3.0, 910.0, 9.81, 2000.0, 1000.0
+    mu_n_pwrn = 2.0 ** ((3.0 - gn) / 2.0) / (gn + 1.0) * sin(th) ** (gn - 1.0) * (rhoi * grav * Hp) ** (gn - 1.0) * 1e6
+    flwa = 1.0 / (2.0 ** ((1.0 + gn) / 2.0) * mu_n_pwrn)
+    p = synthetic.slab(n_cells=6, nz=nz, dx=50.0, thck=Hp / cos(th), theta_deg=5.0, beta=beta, flwa=flwa)
+    r = po.picard(p, two_d=True, approx="diva", precond=1, linear_tol=1e-11, linear_maxiters=3000,
+                  resid_target=1e-9, maxiter=300)
+    assert r["converged"]
+    g = p.grid
+    H = p.thck[0, 0] * cos(th)
+    mu_n = 1.0 / (2.0 ** ((1.0 + gn) / (2.0 * gn)) * flwa ** (1.0 / gn))
+    eta = (beta * H / mu_n ** gn) * (rhoi * grav * H) ** (gn - 1)
+    velscale = (rhoi * grav * H / mu_n) ** gn * H
+    zp = (1.0 - p.sigma) * p.thck[0, 0] / H * cos(th)
+    ufo = tan(th) / eta - 2 ** ((1.0 - gn) / 2.0) * tan(th) ** gn / (
+        (gn + 1) * (1.0 + 3.0 * sin(th) ** 2) ** ((gn + 1.0) / 2.0)) * ((1.0 - zp) ** (gn + 1) - 1.0)
+    u = r["uvel3d"][:, g.ilo + 1, g.jlo + 1]
+    return float(np.abs(u / velscale - ufo).max() / np.abs(ufo).max()), abs(u[-1] / velscale / ufo[-1] - 1.0)
+
+
+def test_diva_slab_matches_first_order_analytic_solution():
+    """DIVA (Goldberg 2011) with the slope correction reproduces the
+    first-order slab solution of tests/slab/plotSlab.py:242-250 through the
+    whole path -- layer viscosities from the basal traction, the omega
+    integrals, beta_eff, the basal term with the bed slope and the 3-D
+    velocity reconstruction: basal velocity to rounding, the profile at
+    second order in the level spacing."""
+    e1, b1 = _diva_slab(11)
+    e2, b2 = _diva_slab(21)
+    assert e2 < 1e-3 and 3.5 < e1 / e2 < 4.5
+    assert b1 < 1e-9 and b2 < 1e-9
+
+
+def test_diva_omega_integrals_closed_form():
+    """With a constant viscosity eta the single integral is omega_k(sigma) =
+    H (1 - sigma^2) / (2 eta) exactly (midpoint rule on a linear integrand)
+    and omega is its trapezoidal vertical mean."""
+    p = synthetic.ice_stream(16, nz=7, seed=5)
+    p.whichefvs, p.efvs_constant = 0, 2.5e13
+    s = po.assemble_2d(p, approx="diva")
+    g = p.grid
+    acell = s["geo"]["active_cell"]
+    sig = p.sigma
+    for (i, j) in [(4, 5), (9, 7), (12, 12)]:
+        if not acell[i - 1, j - 1]:
+            continue
+        H = p.thck[i - 1, j - 1]
+        want = H * (1.0 - sig ** 2) / (2.0 * 2.5e13)
+        assert np.allclose(s["omega_k"][:, i - 1, j - 1], want, rtol=1e-13, atol=1e-30)
+        trap = sum(0.5 * (want[k] + want[k + 1]) * (sig[k + 1] - sig[k]) for k in range(g.nz - 1))
+        assert s["omega"][i - 1, j - 1] == pytest.approx(trap, rel=1e-13)
+
+
+def test_diva_without_basal_traction_is_ssa_stiffness():
+    """With zero basal traction the DIVA layer viscosities have no vertical
+    shear, so for a vertically uniform flow factor the DIVA stiffness equals
+    the SSA stiffness (the vertical mean of the layer viscosities equals the
+    viscosity of the vertically averaged flow factor)."""
+    p = synthetic.ice_stream(16, nz=7, seed=6)
+    p.beta = np.zeros_like(p.beta)
+    g = p.grid
+    rng = np.random.default_rng(3)
+    u2 = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)) * 30.0)
+    v2 = np.asfortranarray(rng.standard_normal((g.nvx, g.nvy)) * 30.0)
+    ssa = po.assemble_2d(p, u2, v2)
+    diva = po.assemble_2d(p, u2, v2, approx="diva")
+    for key in ("Auu", "Auv", "Avu", "Avv"):
+        a, b = ssa[key], diva[key]
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-11 * np.abs(a).max()), key
