@@ -99,13 +99,17 @@ __device__ __forceinline__ double flwafact_2d(const Dims& d, const AsmDev& in, L
 // Row NR of the SSA element matrix of cell (ci,cj), summed over the 4
 // quadrature points as compute_element_matrix accumulates K, then added
 // into the vertex row accumulators acc[blk][m2].
+// efva != nullptr (DIVA): the vertical-mean viscosity of each quadrature
+// point comes from k_diva_cell (efva[cell*4 + p]); otherwise the SSA
+// viscosity of the vertically averaged flow factor.
 template <int NR>
 __device__ __forceinline__ bool add_element_row_2d(const Dims& d, const AsmDev& in, Lev L,
                                                    const double* __restrict__ uv, const double* __restrict__ vv,
-                                                   int ci, int cj, double (&acc)[4][9]) {
+                                                   const double* __restrict__ efva, int ci, int cj,
+                                                   double (&acc)[4][9]) {
     Elem2 e;
     load_elem2(d, in, uv, vv, ci, cj, e);
-    const double fl = flwafact_2d(d, in, L, ci, cj);
+    const double fl = efva ? 0.0 : flwafact_2d(d, in, L, ci, cj);
     double K[4][4];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -115,7 +119,7 @@ __device__ __forceinline__ bool add_element_row_2d(const Dims& d, const AsmDev& 
     for (int p = 0; p < 4; ++p) {
         double dx[4], dy[4], detJ;
         if (!basis_derivs_2d(e, p, dx, dy, detJ)) { ok = false; break; }
-        const double ef = efvs_ssa(in, e, dx, dy, fl);
+        const double ef = efva ? efva[4 * c2(d, ci, cj) + p] : efvs_ssa(in, e, dx, dy, fl);
         double hq = 0.0;
 #pragma unroll
         for (int n = 0; n < 4; ++n) hq = hq + c_phi2[n][p] * e.h[n];
@@ -141,8 +145,8 @@ __device__ __forceinline__ bool add_element_row_2d(const Dims& d, const AsmDev& 
 // One thread per owned vertex; writes the whole one-level row (the kA = +-1
 // planes are zero).  flag[0] = 1 on a singular Jacobian.
 __global__ __launch_bounds__(128) void k_assemble_2d(Dims d, AsmDev in, Lev L, const double* __restrict__ uv,
-                                                     const double* __restrict__ vv, double* __restrict__ A,
-                                                     double* __restrict__ flag) {
+                                                     const double* __restrict__ vv, const double* __restrict__ efva,
+                                                     double* __restrict__ A, double* __restrict__ flag) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= d.nrows) return;
     const int io = (int)(t % d.nxp), jo = (int)(t / d.nxp);
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(128) void k_assemble_2d(Dims d, AsmDev in, Lev L, c
     {                                                                                     \
         const int ci = i + DI, cj = j + DJ;                                               \
         if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)])                \
-            ok = add_element_row_2d<NR>(d, in, L, uv, vv, ci, cj, acc) && ok;             \
+            ok = add_element_row_2d<NR>(d, in, L, uv, vv, efva, ci, cj, acc) && ok;       \
     }
             CELL2(0, 0, 2)
             CELL2(1, 0, 3)
@@ -334,6 +338,238 @@ __global__ void k_symmetry_report_2d(Dims d, const int32_t* __restrict__ avert, 
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) warns += __shfl_down(warns, off, WAVE);
     if (threadIdx.x % WAVE == 0 && warns) atomicAdd(count, warns);
+}
+
+// ---------------------------------------------------------------------------
+// DIVA (Goldberg 2011) for the 2-D system, slope correction on
+// (glissade_velo_higher.F90:2078-2140, 2469-2560, 5649-5900, 7730-7997,
+// 8535-8811).  State kept on the device between Picard iterations: the
+// layer viscosities efvs_qp_3d (nz-1, 4, nx, ny) and the basal traction.
+// ---------------------------------------------------------------------------
+struct DivaDev {
+    const double *sfx, *sfy;           // (nx, ny) slope factors at cells
+    const double *ssfx, *ssfy;         // (nvx, nvy) slope factors at vertices
+    const double *sthx, *sthy;         // (nvx, nvy) slope angles at vertices
+    const double *tbx, *tby;           // (nvx, nvy) basal slope angles
+    const int32_t* ipl;                // (nx, ny) ice or land cells
+    double* efvs_qp;                   // (nz-1, 4, nx, ny) state
+    double *btx, *bty;                 // (nvx, nvy) basal traction (state)
+    double* omega_k;                   // (nz, nx, ny)
+    double* omega;                     // (nx, ny)
+    double* efva;                      // (4, nx, ny) vertical-mean viscosity per quadrature point
+    double* somega;                    // (nvx, nvy) staggered omega
+    double *bex, *bey;                 // (nvx, nvy) beta_eff
+    int no_slip;
+};
+
+// compute_effective_viscosity_diva, default (non-cubic) branch, for point p;
+// ef (nz-1 values, stride 1) updated in place
+__device__ __forceinline__ void efvs_diva(const AsmDev& in, Lev L, const Elem2& e, const double* bx,
+                                          const double* by, int p, const double* dx, const double* dy, double sfx,
+                                          double sfy, const double* fl, double* ef) {
+#pragma clang fp contract(off)
+    const double p_effstr = (1.0 - in.n_glen) / in.n_glen;
+    if (in.whichefvs == 0) { for (int k = 0; k < L.nz - 1; ++k) ef[k] = in.efvs_constant; return; }
+    if (in.whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
+        for (int k = 0; k < L.nz - 1; ++k) ef[k] = fl[k] * pow(effstrain, p_effstr);
+        return;
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0, taux = 0, tauy = 0;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dx[n] * e.u[n]; uy = uy + dy[n] * e.u[n];
+        vx = vx + dx[n] * e.v[n]; vy = vy + dy[n] * e.v[n];
+        taux = taux + c_phi2[n][p] * bx[n];
+        tauy = tauy + c_phi2[n][p] * by[n];
+    }
+    for (int k = 0; k < L.nz - 1; ++k) {
+        // sigma of layer k (the midpoint of levels k, k+1)
+        const double ss = (in.sigma[k] + in.sigma[k + 1]) / 2.0;
+        double efk = ef[k];
+        if (efk == 0.0) efk = fl[k] * pow(in.effstrain_min, p_effstr);
+        const double du_dz = taux * ss / (efk * sfx);
+        const double dv_dz = tauy * ss / (efk * sfy);
+        const double esq = in.effstrain_min * in.effstrain_min + ux * ux + vy * vy + ux * vy +
+                           0.25 * ((vx + uy) * (vx + uy)) + 0.25 * (du_dz * du_dz) * sfx + 0.25 * (dv_dz * dv_dz) * sfy;
+        ef[k] = fl[k] * pow(esq, p_effstr / 2.0);
+    }
+}
+
+// One thread per cell: layer viscosities (state), their vertical means per
+// quadrature point, omega_k / omega (compute_integrals_diva) and efvs.
+__global__ void k_diva_cell(Dims d, AsmDev in, Lev L, DivaDev dv, const double* __restrict__ uv,
+                            const double* __restrict__ vv, double* __restrict__ efvs, double* __restrict__ flag) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nx * d.ny) return;
+    const int ci = 1 + (int)(t % d.nx), cj = 1 + (int)(t / d.nx);
+    const int nl = L.nz - 1;
+    double* st = dv.efvs_qp + (long long)nl * 4 * t;
+    double* ok = dv.omega_k + (long long)L.nz * t;
+    if (!(cell_in_range(d, in.nhalo, ci, cj) && in.acell[t])) {
+        for (int p = 0; p < 4; ++p) dv.efva[4 * t + p] = 0.0;
+        for (int k = 0; k < L.nz; ++k) ok[k] = 0.0;
+        dv.omega[t] = 0.0;
+        for (int k = 0; k < nl; ++k) efvs[(long long)k + (long long)nl * t] = 0.0;
+        return;
+    }
+    Elem2 e;
+    load_elem2(d, in, uv, vv, ci, cj, e);
+    double bx[4], by[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const long long q = v2(d, ci + nd2_i(n), cj + nd2_j(n));
+        bx[n] = dv.btx[q];
+        by[n] = dv.bty[q];
+    }
+    const double* fl = in.flwafact + (long long)nl * t;
+    for (int p = 0; p < 4; ++p) {
+        double dx[4], dy[4], detJ;
+        if (!basis_derivs_2d(e, p, dx, dy, detJ)) { flag[0] = 1.0; return; }
+        efvs_diva(in, L, e, bx, by, p, dx, dy, dv.sfx[t], dv.sfy[t], fl, st + p * nl);
+        double va = 0.0;
+        for (int k = 0; k < nl; ++k) va = va + st[p * nl + k] * (in.sigma[k + 1] - in.sigma[k]);
+        dv.efva[4 * t + p] = va;
+    }
+    // compute_integrals_diva with the cell thickness
+    const double H = in.thck_cell[t];
+    double okp[4];
+    for (int p = 0; p < 4; ++p) okp[p] = 0.0;
+    ok[L.nz - 1] = 0.0;
+    for (int k = L.nz - 2; k >= 0; --k) {
+        const double depth = 0.5 * (in.sigma[k] + in.sigma[k + 1]);
+        const double dz = (in.sigma[k + 1] - in.sigma[k]) * H;
+        for (int p = 0; p < 4; ++p) okp[p] = okp[p] + depth / st[p * nl + k] * dz;
+        ok[k] = (((okp[0] + okp[1]) + okp[2]) + okp[3]) / 4.0;
+    }
+    double om = 0.0;
+    for (int k = 0; k < L.nz - 1; ++k) om = om + 0.5 * (ok[k] + ok[k + 1]) * (in.sigma[k + 1] - in.sigma[k]);
+    dv.omega[t] = om;
+    for (int k = 0; k < nl; ++k) {
+        double sm = 0.0;
+        for (int p = 0; p < 4; ++p) sm = sm + st[p * nl + k];
+        efvs[(long long)k + (long long)nl * t] = sm / 4.0;
+    }
+}
+
+// parallel_halo_real8_2d for a cell field, one task, 2 halo cells: periodic
+// wrap; outflow zeroes the halo; no_ice zeroes 3 rows on each side.  Run as
+// two launches (E/W, then N/S) for the periodic rules, one otherwise.
+__global__ void k_cell_halo(Dims d, int pass, double* __restrict__ a) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int nx = d.nx, ny = d.ny;
+    if (d.bc == 0 || d.bc == 2) {
+        if (pass == 0) {
+            if (t >= ny - 4) return;
+            const int j = 3 + (int)t;
+            auto c = [&](int i) { return (long long)(i - 1) + (long long)nx * (j - 1); };
+            const double a1 = a[c(nx - 3)], a2 = a[c(nx - 2)], a3 = a[c(3)], a4 = a[c(4)];
+            a[c(1)] = a1; a[c(2)] = a2; a[c(nx - 1)] = a3; a[c(nx)] = a4;
+        } else {
+            if (t >= nx) return;
+            const int i = 1 + (int)t;
+            auto c = [&](int j) { return (long long)(i - 1) + (long long)nx * (j - 1); };
+            const double a1 = a[c(ny - 3)], a2 = a[c(ny - 2)], a3 = a[c(3)], a4 = a[c(4)];
+            a[c(1)] = a1; a[c(2)] = a2; a[c(ny - 1)] = a3; a[c(ny)] = a4;
+        }
+    } else {
+        if (pass != 0 || t >= (long long)nx * ny) return;
+        const int i = 1 + (int)(t % nx), j = 1 + (int)(t / nx);
+        const int w = d.bc == 1 ? 2 : 3;
+        if (i <= w || i >= nx - w + 1 || j <= w || j >= ny - w + 1) a[t] = 0.0;
+    }
+}
+
+// masked stagger (glissade_stagger, stagger_margin 1) of a cell field
+__device__ __forceinline__ double stagger_masked_at(const Dims& d, const int32_t* __restrict__ m,
+                                                    const double* __restrict__ a, int i, int j) {
+#pragma clang fp contract(off)
+    const long long c00 = c2(d, i, j), c10 = c2(d, i + 1, j), c01 = c2(d, i, j + 1), c11 = c2(d, i + 1, j + 1);
+    const double sv = (double)m[c01] * a[c01] + (double)m[c11] * a[c11] + (double)m[c00] * a[c00] + (double)m[c10] * a[c10];
+    const double sm = (double)(m[c01] + m[c11] + m[c00] + m[c10]);
+    return sm > 0.0 ? sv / sm : 0.0;
+}
+
+// stag_omega and beta_eff at every vertex
+__global__ void k_diva_beta(Dims d, AsmDev in, DivaDev dv) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const double so = stagger_masked_at(d, dv.ipl, dv.omega, i, j);
+    dv.somega[t] = so;
+    double bex = 0.0, bey = 0.0;
+    if (dv.no_slip) {
+        if (so > 0.0) {
+            bex = dv.ssfx[t] / (so * cos(dv.sthx[t]));
+            bey = dv.ssfy[t] / (so * cos(dv.sthy[t]));
+        }
+    } else {
+        const double b = in.beta[t];
+        bex = (dv.ssfx[t] * b) / (dv.ssfx[t] + b * so * cos(dv.sthx[t]));
+        bey = (dv.ssfy[t] * b) / (dv.ssfy[t] + b * so * cos(dv.sthy[t]));
+    }
+    dv.bex[t] = bex;
+    dv.bey[t] = bey;
+}
+
+// basal_sliding_bc_2d_diva, local option, on active owned rows
+__global__ void k_basal_diva_2d(Dims d, AsmDev in, DivaDev dv, double* __restrict__ A) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nxo * d.nyo) return;
+    const int io = (int)(t % d.nxo), jo = (int)(t / d.nxo);
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const long long p = v2(d, i, j);
+    if (!in.avert[p]) return;
+    const long long r = (long long)jo * d.nxp + io;
+    double* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+    const double f = in.dx * in.dy / 1.0e9;
+    a[(0 * 27 + 13) * WAVE] = a[(0 * 27 + 13) * WAVE] + f * dv.bex[p] / cos(dv.tbx[p]);
+    a[(3 * 27 + 13) * WAVE] = a[(3 * 27 + 13) * WAVE] + f * dv.bey[p] / cos(dv.tby[p]);
+}
+
+// compute_3d_velocity_diva: basal traction at every vertex; the 3-D velocity
+// (reference layout (nz, nvx, nvy), optional) on active vertices
+__global__ void k_diva_velocity(Dims d, AsmDev in, Lev L, DivaDev dv, const double* __restrict__ uv,
+                                const double* __restrict__ vv, double* __restrict__ u3, double* __restrict__ v3) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const long long c = vidx(d, 0, i, j);
+    const double u = uv[c], v = vv[c];
+    const double btx = dv.bex[t] * u, bty = dv.bey[t] * v;
+    dv.btx[t] = btx;
+    dv.bty[t] = bty;
+    if (!u3 || !in.avert[t]) return;
+    double ub, vb;
+    const double b = in.beta[t];
+    if (b > 0.0) {
+        ub = u * dv.bex[t] / b;
+        vb = v * dv.bey[t] / b;
+    } else {
+        ub = u;
+        vb = v;
+    }
+    const int nz = L.nz;
+    u3[(long long)(nz - 1) + (long long)nz * t] = ub;
+    v3[(long long)(nz - 1) + (long long)nz * t] = vb;
+    const double cx = cos(dv.sthx[t]) / dv.ssfx[t], cy = cos(dv.sthy[t]) / dv.ssfy[t];
+    for (int k = 0; k < nz - 1; ++k) {
+        // masked stagger of omega_k(k, :, :)
+        const long long c00 = c2(d, i, j), c10 = c2(d, i + 1, j), c01 = c2(d, i, j + 1), c11 = c2(d, i + 1, j + 1);
+        const int32_t* m = dv.ipl;
+        const double* o = dv.omega_k;
+        const double sv = (double)m[c01] * o[(long long)k + nz * c01] + (double)m[c11] * o[(long long)k + nz * c11] +
+                          (double)m[c00] * o[(long long)k + nz * c00] + (double)m[c10] * o[(long long)k + nz * c10];
+        const double sm = (double)(m[c01] + m[c11] + m[c00] + m[c10]);
+        const double sok = sm > 0.0 ? sv / sm : 0.0;
+        u3[(long long)k + (long long)nz * t] = ub + btx * sok * cx;
+        v3[(long long)k + (long long)nz * t] = vb + bty * sok * cy;
+    }
 }
 
 }  // namespace cg

@@ -81,6 +81,10 @@ struct cismgpu_system {
     bool two_d = false;          // one-level (depth-integrated) system: 9-point stencil, nz = 1
     int nz_levels = 0;           // 2-D system: sigma levels of the assembly inputs
     double* load3 = nullptr;     // 2-D system: nz_levels-level load work pair
+    double* diva_buf = nullptr;  // 2-D system: DIVA inputs and state
+    DivaDev dv{};
+    AsmDev asm_in{};             // device inputs of the last 2-D assembly
+    bool diva_ready = false;
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     bool a32_sym = false;        // FP32 copy in half (symmetric) storage
     double* A64h = nullptr;      // FP64 half (symmetric) storage, when the FP64 matrix is exactly symmetric
@@ -753,7 +757,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -1180,7 +1184,7 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
         cismgpu_set_error("assembly: lateral BC needs usrf, thck and (standard option) the staggered fields");
         return CISMGPU_ERR_ARG;
     }
-    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? 2 * nc + 2 * n2 : 0)),
+    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? nc + 2 * n2 : 0) + (in->thck ? nc : 0)),
                  ni = (size_t)(nc + n2 + 2 * nvec + (lateral ? nc : 0));
     const size_t bytes = nd * sizeof(double) + ni * sizeof(int32_t);
     if (S->asm_bytes < bytes) {
@@ -1211,9 +1215,10 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     a.beta = putd(in->beta_internal, n2, true);
     a.theta = putd(in->theta_basal, n2, true);
     a.flwafact = putd(in->flwafact, nfl, true);
+    a.thck_cell = in->thck ? putd(in->thck, nc, true) : nullptr;
     if (lateral) {
         lat.usrf = putd(in->usrf, nc, true);
-        lat.thck = putd(in->thck, nc, true);
+        lat.thck = a.thck_cell;
         lat.susrf = putd(in->stagusrf_lateral, n2, true);
         lat.sthck = putd(in->stagthck_lateral, n2, true);
         lat.local = in->which_assemble_lateral == 1 ? 1 : 0;
@@ -1333,9 +1338,14 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         cismgpu_set_error("assemble_2d: the grid's nz (sigma levels of the inputs) must be 2..64, got %d", S->nz_levels);
         return CISMGPU_ERR_ARG;
     }
-    if (whichapprox != CISMGPU_APPROX_SSA) {
-        cismgpu_set_error("assemble_2d: which_ho_approx %d is not supported (SSA = 1 is)", whichapprox);
+    const bool diva = whichapprox == CISMGPU_APPROX_DIVA;
+    if (whichapprox != CISMGPU_APPROX_SSA && !diva) {
+        cismgpu_set_error("assemble_2d: which_ho_approx %d is not supported (SSA = 1 and DIVA = 4 are)", whichapprox);
         return CISMGPU_ERR_UNSUPPORTED;
+    }
+    if (diva && (!S->diva_ready || !in->thck)) {
+        cismgpu_set_error("assemble_2d: DIVA needs cismgpu_system_diva_setup and the cell thickness (in->thck)");
+        return CISMGPU_ERR_ARG;
     }
     int st = upload_fe_tables();
     if (st) return st;
@@ -1358,10 +1368,27 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     const double* uv = S->V(0);
     const double* vv = S->V(1);
     hipEventRecord(e0, S->s);
-    k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->scal + S_NAN);
+    S->asm_in = a;
+    if (diva) {   // layer viscosities (state), omega integrals, efvs
+        k_diva_cell<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, S->dv, uv, vv, S->efvs, S->scal + S_NAN);
+        k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->dv.efva, S->A64, S->scal + S_NAN);
+    } else {
+        k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, nullptr, S->A64, S->scal + S_NAN);
+    }
     hipEventRecord(e1, S->s);
-    k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
-    if (in->beta_internal) k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+    if (diva) {   // omega halo, beta_eff, basal_sliding_bc_2d_diva
+        if (d.bc == 0 || d.bc == 2) {
+            k_cell_halo<<<nblocks(d.ny, 256), 256, 0, S->s>>>(d, 0, S->dv.omega);
+            k_cell_halo<<<nblocks(d.nx, 256), 256, 0, S->s>>>(d, 1, S->dv.omega);
+        } else {
+            k_cell_halo<<<nblocks(nc, 256), 256, 0, S->s>>>(d, 0, S->dv.omega);
+        }
+        k_diva_beta<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->dv);
+        k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
+    } else {
+        k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
+        if (in->beta_internal) k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+    }
     // b = level sum of the 3-D load (gravity, then the lateral BC)
     double *lu = S->load3, *lv = S->load3 + d3.vlen;
     CG_CHECK(hipMemsetAsync(S->load3, 0, 2 * d3.vlen * sizeof(double), S->s));
@@ -1404,6 +1431,88 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     S->tri.mode = 0;
     if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
     return CISMGPU_OK;
+}
+
+// DIVA inputs and state of a 2-D system (glissade_velo_higher.F90:2078-2140).
+extern "C" int32_t cismgpu_system_diva_setup(cismgpu_system* S, const cismgpu_diva_in* in) {
+    if (!S || !in || !S->two_d || !in->slope_factor_x || !in->slope_factor_y || !in->stag_slope_factor_x ||
+        !in->stag_slope_factor_y || !in->stag_theta_slope_x || !in->stag_theta_slope_y || !in->theta_basal_x ||
+        !in->theta_basal_y || !in->ice_plus_land_mask) {
+        cismgpu_set_error("diva_setup: needs a 2-D system and every slope/mask field");
+        return CISMGPU_ERR_ARG;
+    }
+    if (S->nz_levels < 2) { cismgpu_set_error("diva_setup: the grid's nz must be >= 2"); return CISMGPU_ERR_ARG; }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny, nz = S->nz_levels;
+    // doubles: sfx, sfy (nc); ssfx, ssfy, sthx, sthy, tbx, tby, btx, bty, somega, bex, bey (n2);
+    // efvs_qp (nz-1)*4*nc; omega_k nz*nc; omega nc; efva 4*nc; then the int mask (nc)
+    const long long ndbl = 2 * nc + 11 * n2 + (nz - 1) * 4 * nc + nz * nc + nc + 4 * nc;
+    if (!S->diva_buf) CG_CHECK(hipMalloc(&S->diva_buf, ndbl * sizeof(double) + nc * sizeof(int32_t)));
+    double* p = S->diva_buf;
+    DivaDev& dv = S->dv;
+    auto take = [&](long long n) { double* q = p; p += n; return q; };
+    auto up = [&](const double* h, long long n, bool zero) -> double* {
+        double* q = take(n);
+        if (h) hipMemcpyAsync(q, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
+        else if (zero) hipMemsetAsync(q, 0, n * sizeof(double), S->s);
+        return q;
+    };
+    dv.sfx = up(in->slope_factor_x, nc, true);
+    dv.sfy = up(in->slope_factor_y, nc, true);
+    dv.ssfx = up(in->stag_slope_factor_x, n2, true);
+    dv.ssfy = up(in->stag_slope_factor_y, n2, true);
+    dv.sthx = up(in->stag_theta_slope_x, n2, true);
+    dv.sthy = up(in->stag_theta_slope_y, n2, true);
+    dv.tbx = up(in->theta_basal_x, n2, true);
+    dv.tby = up(in->theta_basal_y, n2, true);
+    dv.btx = up(in->btractx, n2, true);
+    dv.bty = up(in->btracty, n2, true);
+    dv.somega = take(n2);
+    dv.bex = up(nullptr, n2, true);
+    dv.bey = up(nullptr, n2, true);
+    dv.efvs_qp = up(in->efvs_qp_3d, (nz - 1) * 4 * nc, true);
+    dv.omega_k = up(nullptr, nz * nc, true);
+    dv.omega = up(nullptr, nc, true);
+    dv.efva = take(4 * nc);
+    int32_t* m = (int32_t*)p;
+    CG_CHECK(hipMemcpyAsync(m, in->ice_plus_land_mask, nc * sizeof(int32_t), hipMemcpyHostToDevice, S->s));
+    dv.ipl = m;
+    dv.no_slip = in->no_slip ? 1 : 0;
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->diva_ready = true;
+    return CISMGPU_OK;
+}
+
+// compute_3d_velocity_diva (glissade_velo_higher.F90:5779-5900) from the
+// resident 2-D x and the beta_eff of the last DIVA assembly: updates the
+// basal traction state; optionally returns the 3-D velocity (nz, nx-1, ny-1)
+// and the traction (nx-1, ny-1) to the host.  a.beta is the last assembly's.
+extern "C" int32_t cismgpu_system_diva_velocity(cismgpu_system* S, double* uvel, double* vvel, double* btractx,
+                                                double* btracty) {
+    if (!S || !S->diva_ready || !S->asm_buf) { cismgpu_set_error("diva_velocity: no DIVA assembly"); return CISMGPU_ERR_ARG; }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nz = S->nz_levels;
+    double* u3 = nullptr;
+    if (uvel && vvel) {
+        CG_CHECK(hipMalloc(&u3, 2 * nz * n2 * sizeof(double)));
+        CG_CHECK(hipMemsetAsync(u3, 0, 2 * nz * n2 * sizeof(double), S->s));   // inactive vertices stay zero
+    }
+    k_diva_velocity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, S->asm_in, Lev{(int)nz}, S->dv, S->V(0), S->V(1), u3,
+                                                        u3 ? u3 + nz * n2 : nullptr);
+    int st = CISMGPU_OK;
+    if (u3) {
+        if (hipMemcpyAsync(uvel, u3, nz * n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+            hipMemcpyAsync(vvel, u3 + nz * n2, nz * n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess)
+            st = CISMGPU_ERR_HIP;
+    }
+    if (!st && btractx && btracty &&
+        (hipMemcpyAsync(btractx, S->dv.btx, n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+         hipMemcpyAsync(btracty, S->dv.bty, n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess))
+        st = CISMGPU_ERR_HIP;
+    if (hipStreamSynchronize(S->s) != hipSuccess) st = CISMGPU_ERR_HIP;
+    if (u3) hipFree(u3);
+    if (st) cismgpu_set_error("diva_velocity: device copy failed");
+    return st;
 }
 
 // Saved-assembly snapshot for the accelerated Picard iteration

@@ -99,6 +99,79 @@ def slope_angle(dx, dy, zsfc):
     return np.asfortranarray(theta)
 
 
+def _vertex_gradient(dx, dy, z):
+    """glissade_gradient with a full mask (glissade_grid_operators.F90:309-468):
+    the centred vertex gradient of a cell field."""
+    gx = ((z[1:, 1:] - z[:-1, 1:]) / dx + (z[1:, :-1] - z[:-1, :-1]) / dx) / 2.0
+    gy = ((z[1:, 1:] - z[1:, :-1]) / dy + (z[:-1, 1:] - z[:-1, :-1]) / dy) / 2.0
+    return gx, gy
+
+
+def slope_angle_xy(dx, dy, zsfc):
+    """glissade_slope_angle with its optional x/y outputs (glissade_grid_operators.F90:
+    1264-1395): theta = atan|grad|, theta_x = atan|dz/dx|, theta_y = atan|dz/dy|
+    of the vertex gradient unstaggered to cell centres; the outer ring is zero."""
+    z = np.asarray(zsfc, dtype=np.float64)
+    nx, ny = z.shape
+    gx, gy = _vertex_gradient(dx, dy, z)
+    ux = (gx[1:, 1:] + gx[:-1, 1:] + gx[1:, :-1] + gx[:-1, :-1]) / 4.0
+    uy = (gy[1:, 1:] + gy[:-1, 1:] + gy[1:, :-1] + gy[:-1, :-1]) / 4.0
+    out = [np.zeros((nx, ny), order="F") for _ in range(3)]
+    out[0][1:-1, 1:-1] = np.arctan(np.sqrt(ux * ux + uy * uy))
+    out[1][1:-1, 1:-1] = np.arctan(np.abs(ux))
+    out[2][1:-1, 1:-1] = np.arctan(np.abs(uy))
+    return tuple(out)
+
+
+def slope_angle_staggered(dx, dy, zsfc):
+    """glissade_slope_angle_staggered (glissade_grid_operators.F90:1414-1552):
+    the angles of the vertex gradient of a cell field, (nx-1, ny-1)."""
+    gx, gy = _vertex_gradient(dx, dy, np.asarray(zsfc, dtype=np.float64))
+    return (np.asfortranarray(np.arctan(np.sqrt(gx * gx + gy * gy))), np.asfortranarray(np.arctan(np.abs(gx))),
+            np.asfortranarray(np.arctan(np.abs(gy))))
+
+
+def cell_halo_2d(grid: Grid, a: np.ndarray) -> None:
+    """parallel_halo for a cell field on one task, 2 halo cells (parallel_mpi.F90:
+    7720-7912): periodic wrap (east/west over the owned rows, then north/south
+    over every column); outflow zeroes the halo; no_ice zeroes 3 rows on each
+    side."""
+    nx, ny = a.shape
+    if grid.global_bc in (0, 2):
+        r = slice(2, ny - 2)
+        w, e = a[nx - 4:nx - 2, r].copy(), a[2:4, r].copy()
+        a[0:2, r] = w
+        a[nx - 2:nx, r] = e
+        s, n = a[:, ny - 4:ny - 2].copy(), a[:, 2:4].copy()
+        a[:, 0:2] = s
+        a[:, ny - 2:ny] = n
+    else:
+        w = 2 if grid.global_bc == BC_OUTFLOW else 3
+        a[:w, :] = 0.0
+        a[nx - w:, :] = 0.0
+        a[:, :w] = 0.0
+        a[:, ny - w:] = 0.0
+
+
+def diva_geometry(grid: Grid, dx, dy, usrf, lsrf, ice_mask, land_mask):
+    """DIVA inputs of glissade_velo_higher_solve with diva_slope_correction
+    (glissade_velo_higher.F90:2078-2154): slope angles of z_mean = (usrf +
+    lsrf)/2 at cells (with their halo) and vertices, the slope factors
+    cos^2 + 4 sin^2, the basal slope angles (of the staggered lower surface,
+    :8591-8605) and the ice-or-land cell mask (:1510-1515)."""
+    zmean = 0.5 * (np.asarray(usrf) + np.asarray(lsrf))
+    _, thx, thy = slope_angle_xy(dx, dy, zmean)
+    cell_halo_2d(grid, thx)
+    cell_halo_2d(grid, thy)
+    _, sthx, sthy = slope_angle_staggered(dx, dy, zmean)
+    _, tbx, tby = slope_angle_xy(dx, dy, stagger(lsrf))
+    sf = lambda t: np.asfortranarray(np.cos(t) ** 2 + 4.0 * np.sin(t) ** 2)
+    ipl = np.asfortranarray(((ice_mask == 1) | (land_mask == 1)).astype(np.int32))
+    return dict(slope_factor_x=sf(thx), slope_factor_y=sf(thy), stag_slope_factor_x=sf(sthx),
+                stag_slope_factor_y=sf(sthy), stag_theta_slope_x=sthx, stag_theta_slope_y=sthy,
+                theta_basal_x=tbx, theta_basal_y=tby, ice_plus_land_mask=ipl)
+
+
 def vertex_masks(ice_mask, nhalo=2):
     """active_cell / active_vertex of get_vertex_geometry (glissade_velo_higher.F90:4164-4183)."""
     nx, ny = ice_mask.shape

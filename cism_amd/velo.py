@@ -36,6 +36,14 @@ class AssemblyIn(ctypes.Structure):
                 ("which_assemble_lateral", ctypes.c_int32)]
 
 
+class DivaIn(ctypes.Structure):
+    _P = ctypes.c_void_p
+    _fields_ = [("slope_factor_x", _P), ("slope_factor_y", _P), ("stag_slope_factor_x", _P),
+                ("stag_slope_factor_y", _P), ("stag_theta_slope_x", _P), ("stag_theta_slope_y", _P),
+                ("theta_basal_x", _P), ("theta_basal_y", _P), ("ice_plus_land_mask", _P), ("efvs_qp_3d", _P),
+                ("btractx", _P), ("btracty", _P), ("no_slip", ctypes.c_int32)]
+
+
 class AssemblyInfo(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("asym_pairs", ctypes.c_int64), ("assemble_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double)]
@@ -62,8 +70,11 @@ def _declare(L):
     L.cismgpu_system_work_vectors.argtypes = [vp, i32]
     L.cismgpu_system_vector_op.argtypes = [vp, i32, i32, i32, i32, ctypes.c_double]
     L.cismgpu_system_residual_velocity.argtypes = [vp, i32, i32, D]
+    L.cismgpu_system_diva_setup.argtypes = [vp, ctypes.POINTER(DivaIn)]
+    L.cismgpu_system_diva_velocity.argtypes = [vp, D, D, D, D]
     for n in ("cismgpu_system_save_assembly", "cismgpu_system_restore_assembly", "cismgpu_system_work_vectors",
-              "cismgpu_system_vector_op", "cismgpu_system_residual_velocity"):
+              "cismgpu_system_vector_op", "cismgpu_system_residual_velocity", "cismgpu_system_diva_setup",
+              "cismgpu_system_diva_velocity"):
         getattr(L, n).restype = ctypes.c_int32
     L._velo_declared = True
 
@@ -153,6 +164,9 @@ class BPSolver:
 
     def _zeros(self):
         return self.grid.zeros_vec()
+
+    def _after_solve(self):
+        """Work after each linear solve and the halo of x (none for BP and SSA)."""
 
     def zero_inactive(self, uvel, vvel):
         """Velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""
@@ -278,6 +292,7 @@ class BPSolver:
             res.linear_err.append(si["err"])
             res.solve_ms.append(si["solve_ms"])
             _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+            self._after_solve()
             _lib.check(self._L.cismgpu_system_residual_velocity(self.sys.handle, whichresid, USAV, ctypes.byref(rvd)),
                        "residual_velocity")
             rv = rvd.value
@@ -305,7 +320,7 @@ class SSASolver(BPSolver):
     guess (:1362).  The Picard loop and its acceleration are those of
     :class:`BPSolver`."""
 
-    APPROX_SSA = 1
+    APPROX = 1
 
     def __init__(self, problem, system: System | None = None):
         super().__init__(problem, system or System(problem.grid, two_d=True))
@@ -341,7 +356,7 @@ class SSASolver(BPSolver):
             self.zero_inactive(u, v)
             self.sys.set_x(u, v)
         info = AssemblyInfo()
-        _lib.check(self._L.cismgpu_system_assemble_2d(self.sys.handle, ctypes.byref(self._ain), self.APPROX_SSA,
+        _lib.check(self._L.cismgpu_system_assemble_2d(self.sys.handle, ctypes.byref(self._ain), self.APPROX,
                                                       ctypes.byref(info)), "assemble_2d")
         return dict(nnz=info.nnz, asym_pairs=info.asym_pairs, assemble_ms=info.assemble_ms, total_ms=info.total_ms)
 
@@ -349,3 +364,74 @@ class SSASolver(BPSolver):
         """Picard iteration of the 2-D solve; which_ho_precond 2 is replaced by
         the diagonal preconditioner for 2-D solves (glide_setup.F90:1363-1366)."""
         return super().picard(precond=1 if precond == HO_PRECOND_SIA else precond, **kw)
+
+
+class DIVASolver(SSASolver):
+    """Device-resident depth-integrated viscosity approximation (DIVA,
+    Goldberg 2011; which_ho_approx = 4, diva_slope_correction on): the
+    ``solve_2d`` branch of glissade_velo_higher_solve with the DIVA
+    assembly (layer viscosities from the basal traction, their vertical
+    mean in the element matrix; glissade_velo_higher.F90:5198-5300,
+    compute_effective_viscosity_diva :7730-7997), omega and beta_eff
+    (compute_integrals_diva :5649-5775, :2469-2610), basal_sliding_bc_2d_diva
+    (:8535-8811) and compute_3d_velocity_diva (:5779-5900) after each solve
+    (:3566-3590).  No Dirichlet rows: no-slip enters through beta_eff
+    (:1423-1428).  The state (efvs_qp_3d, btractx/y) stays on the device and
+    starts at zero unless given.  Plain Picard only."""
+
+    APPROX = 4
+
+    def __init__(self, problem, system: System | None = None, state: dict | None = None):
+        super().__init__(problem, system)
+        p = self.p
+        self.diva = ge.diva_geometry(self.grid, p.dx, p.dy, self.usrf, self.lsrf, self.ice_mask, self.land_mask)
+        self.no_slip = p.whichbabc == 6
+        self._ain.beta_internal = self.beta_internal.ctypes.data
+        self._ain.umask_dirichlet = None
+        self._ain.vmask_dirichlet = None
+        self.set_state(state)
+
+    def set_state(self, state: dict | None = None):
+        """Load the DIVA inputs and state (efvs_qp_3d (nz-1, 4, nx, ny),
+        btractx/y (nx-1, ny-1); None: zero)."""
+        g = self.grid
+        st = state or {}
+        keep = {k: np.asfortranarray(np.array(st[k], dtype=np.float64))
+                for k in ("efvs_qp_3d", "btractx", "btracty") if st.get(k) is not None}
+        if "efvs_qp_3d" in keep and keep["efvs_qp_3d"].shape != (g.nz - 1, 4, g.nx, g.ny):
+            raise ValueError("efvs_qp_3d must be (nz-1, 4, nx, ny)")
+        dg = self.diva
+        ptr = lambda a: None if a is None else a.ctypes.data
+        din = DivaIn(*(ptr(dg[k]) for k in ("slope_factor_x", "slope_factor_y", "stag_slope_factor_x",
+                                            "stag_slope_factor_y", "stag_theta_slope_x", "stag_theta_slope_y",
+                                            "theta_basal_x", "theta_basal_y", "ice_plus_land_mask")),
+                     ptr(keep.get("efvs_qp_3d")), ptr(keep.get("btractx")), ptr(keep.get("btracty")),
+                     1 if self.no_slip else 0)
+        _lib.check(self._L.cismgpu_system_diva_setup(self.sys.handle, ctypes.byref(din)), "diva_setup")
+
+    def velocity_3d(self, want_velocity=True):
+        """compute_3d_velocity_diva from the resident 2-D x: updates the basal
+        traction state; returns (uvel, vvel) (nz, nx-1, ny-1) and (btractx, btracty)."""
+        g = self.grid
+        D = ctypes.POINTER(ctypes.c_double)
+        bx, by = np.zeros((g.nvx, g.nvy), order="F"), np.zeros((g.nvx, g.nvy), order="F")
+        u3 = v3 = None
+        if want_velocity:
+            u3, v3 = g.zeros_vec(), g.zeros_vec()
+        _lib.check(self._L.cismgpu_system_diva_velocity(
+            self.sys.handle, u3.ctypes.data_as(D) if u3 is not None else None,
+            v3.ctypes.data_as(D) if v3 is not None else None, bx.ctypes.data_as(D), by.ctypes.data_as(D)),
+            "diva_velocity")
+        return u3, v3, bx, by
+
+    def _after_solve(self):
+        _lib.check(self._L.cismgpu_system_diva_velocity(self.sys.handle, None, None, None, None), "diva_velocity")
+
+    def picard(self, *, precond=1, accelerate=False, **kw) -> PicardResult:
+        """Plain Picard iteration of the DIVA solve; the result adds the 3-D
+        velocity of the last iterate (uvel3d, vvel3d) and the basal traction."""
+        if accelerate:
+            raise NotImplementedError("Picard acceleration is not used with DIVA here")
+        r = super().picard(precond=precond, accelerate=False, **kw)
+        r.uvel3d, r.vvel3d, r.btractx, r.btracty = self.velocity_3d()
+        return r

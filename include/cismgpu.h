@@ -273,10 +273,46 @@ int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_i
  * basal_sliding_bc_2d (:8281-8531), b = the level sum of the 3-D load
  * (:2045-2058, :2636), dirichlet_boundary_conditions_2d (:9133-9301), the
  * matrix and rhs halo, check_symmetry_assembled_matrix_2d (report only,
- * :10648-10850) and count_nonzeros_2d (:10450-10509).  L1L2 and DIVA return
- * CISMGPU_ERR_UNSUPPORTED. */
+ * :10648-10850) and count_nonzeros_2d (:10450-10509).  DIVA (after
+ * cismgpu_system_diva_setup, in->thck required) updates the layer
+ * viscosities of the state from the basal traction, weights the elements
+ * with their vertical mean (:5198-5300, compute_effective_viscosity_diva
+ * :7730-7997), forms omega (compute_integrals_diva :5649-5775), beta_eff
+ * (:2469-2610) and basal_sliding_bc_2d_diva (:8535-8811), with no Dirichlet
+ * rows for no-slip (:1423-1428).  L1L2 returns CISMGPU_ERR_UNSUPPORTED. */
 int32_t cismgpu_system_assemble_2d(cismgpu_system* sys, const cismgpu_assembly_in* in, int32_t whichapprox,
                                    cismgpu_assembly_info* info);
+
+/* DIVA inputs and state of a 2-D system (glissade_velo_higher.F90:2078-2154
+ * slope factors cos^2 + 4 sin^2 of the slope angles of (usrf+lsrf)/2 at cells
+ * and vertices; :8591-8605 basal slope angles; :1510-1515 ice-or-land
+ * mask).  Cell fields are (nx, ny), vertex fields (nx-1, ny-1).  The state
+ * (efvs_qp_3d (nz-1, 4, nx, ny), :1342-1352; btractx/y) stays on the device
+ * and is updated by every DIVA assembly and cismgpu_system_diva_velocity;
+ * NULL state arrays start at zero. */
+typedef struct cismgpu_diva_in {
+    const double*  slope_factor_x;        /* (nx, ny) diva_slope_factor_x */
+    const double*  slope_factor_y;
+    const double*  stag_slope_factor_x;   /* (nx-1, ny-1) */
+    const double*  stag_slope_factor_y;
+    const double*  stag_theta_slope_x;    /* (nx-1, ny-1) radians */
+    const double*  stag_theta_slope_y;
+    const double*  theta_basal_x;         /* (nx-1, ny-1) slope of the staggered lower surface */
+    const double*  theta_basal_y;
+    const int32_t* ice_plus_land_mask;    /* (nx, ny) */
+    const double*  efvs_qp_3d;            /* state in, or NULL */
+    const double*  btractx;               /* state in, or NULL */
+    const double*  btracty;
+    int32_t no_slip;                      /* which_ho_babc == no-slip: beta_eff = sf/(stag_omega cos) */
+} cismgpu_diva_in;
+
+int32_t cismgpu_system_diva_setup(cismgpu_system* sys, const cismgpu_diva_in* in);
+/* compute_3d_velocity_diva (glissade_velo_higher.F90:5779-5900) from the
+ * resident 2-D x and the beta_eff of the last DIVA assembly: updates the
+ * basal traction state; uvel/vvel (nz, nx-1, ny-1) and btractx/y
+ * (nx-1, ny-1) are optional host outputs. */
+int32_t cismgpu_system_diva_velocity(cismgpu_system* sys, double* uvel, double* vvel, double* btractx,
+                                     double* btracty);
 /* Snapshot of the assembled system (A, b, preconditioner) kept on the
  * device, and its restoration: the matrix copies of the accelerated Picard
  * iteration (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184). */

@@ -57,3 +57,35 @@ def test_grounded_fraction_and_masked_stagger_bitwise():
     assert np.array_equal(ge.stagger_masked(geo["usrf"], ipl), geo["stagusrf_lateral"])
     b = ge.beta_internal(p, geo["ice_mask"], geo["floating_mask"], geo["land_mask"])
     assert np.array_equal(b, po.beta_internal(p, geo))
+
+
+@pytest.mark.parametrize("make", [lambda: synthetic.dome(), lambda: synthetic.slab(n_cells=8, nz=9),
+                                  lambda: synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3, h_max=1000.0,
+                                                                 h_shelf=200.0)])
+def test_diva_geometry_bitwise(make):
+    """Slope factors, slope angles, basal slope angles and the ice-or-land
+    mask of the DIVA set-up equal the restated operators: bit for bit, except
+    that numpy's arctan and the C library's atan may differ by one ulp."""
+    p = make()
+    geo = po.geometry(p)
+    want = po.geometry_diva(p, geo)
+    got = ge.diva_geometry(p.grid, p.dx, p.dy, geo["usrf"], geo["lsrf"], geo["ice_mask"], geo["land_mask"])
+    for k, v in got.items():
+        w = want[k]
+        if k.startswith("ice"):
+            assert np.array_equal(v, w), k
+        else:
+            assert np.all(np.abs(v - w) <= np.spacing(np.abs(w))), k
+
+
+@pytest.mark.parametrize("bc", [0, 1, 2, 3])
+def test_cell_halo_rules(bc):
+    """cell_halo_2d equals the restated one-task halo for every global BC."""
+    p = synthetic.ice_stream(24, seed=1)
+    p.grid.global_bc = bc
+    rng = np.random.default_rng(bc)
+    a = np.asfortranarray(rng.standard_normal((p.grid.nx, p.grid.ny)))
+    b = a.copy(order="F")
+    ge.cell_halo_2d(p.grid, a)
+    po.lib().orc_cell_halo(p.grid.nx, p.grid.ny, bc, po.dp(b))
+    assert np.array_equal(a, b)
