@@ -38,7 +38,7 @@ EXPORTS = (
     "cismgpu_system_work_vectors", "cismgpu_system_vector_op", "cismgpu_system_residual_velocity",
     "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d", "cismgpu_matvec_multiply_structured_2d",
     "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d", "cismgpu_system_assemble_2d", "cismgpu_system_diva_setup",
-    "cismgpu_system_diva_velocity",
+    "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity",
 )
 
 

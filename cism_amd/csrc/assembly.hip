@@ -27,6 +27,7 @@ __host__ __device__ constexpr int nd_k(int n) { return n < 4 ? 1 : 0; }
 struct AsmDev {
     const double *xV, *yV, *susrf, *sthck, *sigma, *swbs, *flwafact, *dsdx, *dsdy, *beta, *theta;
     const double* thck_cell;     // (nx, ny) ice thickness, when given
+    const double* flwa;          // (nz-1, nx, ny) rate factor, when given
     const int32_t *acell, *avert, *umask, *vmask;
     double* efvs;
     double dx, dy, efvs_constant, effstrain_min, n_glen;

@@ -572,4 +572,256 @@ __global__ void k_diva_velocity(Dims d, AsmDev in, Lev L, DivaDev dv, const doub
     }
 }
 
+// ------------------------------------------------------------------ L1L2
+// compute_effective_viscosity_L1L2, layer k at one quadrature point of
+// nonlinear viscosity: tau_perp = rhoi g (h sigma(k+1)) |grad s|;
+// tau_parallel solves x^3 + tau_perp^2 x - es/A = 0 (Cardano, with the
+// first-order form of the second cube root when a^3/27 <= 1e-6 b^2/4).
+// n is 1 or 3 (checked on the host).
+__device__ __forceinline__ double efvs_l1l2_layer(const AsmDev& in, double es, double thck, double grads,
+                                                  double sig1, double A) {
+#pragma clang fp contract(off)
+    const double depth = thck * sig1;
+    const double tau_perp = 910.0 * 9.81 * depth * grads;
+    const double a = tau_perp * tau_perp;
+    const double b = -es / A;
+    const double c = sqrt(b * b / 4.0 + a * a * a / 27.0);
+    const double rootA = pow(-b / 2.0 + c, 1.0 / 3.0);
+    const double rootB = (a * a * a / 27.0 > 1.0e-6 * (b * b / 4.0)) ? -pow(b / 2.0 + c, 1.0 / 3.0)
+                                                                     : -a / (3.0 * pow(fabs(b), 1.0 / 3.0));
+    const double tp = rootA + rootB;
+    if (fabs(in.n_glen - 1.0) < 1.0e-10) return 1.0 / (2.0 * A);
+    return 1.0 / (2.0 * A * (tp * tp + tau_perp * tau_perp));
+}
+
+// One thread per cell: the L1L2 layer viscosities at the four quadrature
+// points, their vertical means (efva) and the element means per layer (efvs).
+__global__ void k_l1l2_cell(Dims d, AsmDev in, Lev L, const double* __restrict__ uv, const double* __restrict__ vv,
+                            double* __restrict__ efva, double* __restrict__ efvs, double* __restrict__ flag) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nx * d.ny) return;
+    const int ci = 1 + (int)(t % d.nx), cj = 1 + (int)(t / d.nx);
+    const int nl = L.nz - 1;
+    double* out = efvs + (long long)nl * t;
+    if (!(cell_in_range(d, in.nhalo, ci, cj) && in.acell[t])) {
+        for (int p = 0; p < 4; ++p) efva[4 * t + p] = 0.0;
+        for (int k = 0; k < nl; ++k) out[k] = 0.0;
+        return;
+    }
+    Elem2 e;
+    load_elem2(d, in, uv, vv, ci, cj, e);
+    double sx[4], sy[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const long long q = v2(d, ci + nd2_i(n), cj + nd2_j(n));
+        sx[n] = in.dsdx[q];
+        sy[n] = in.dsdy[q];
+    }
+    const double* fa = in.flwa + (long long)nl * t;
+    const double* fl = in.flwafact + (long long)nl * t;
+    const double p_effstr = (1.0 - in.n_glen) / in.n_glen;
+    for (int p = 0; p < 4; ++p) {
+        double dx[4], dy[4], detJ;
+        if (!basis_derivs_2d(e, p, dx, dy, detJ)) { flag[0] = 1.0; return; }
+        double es = 0.0, grads = 0.0, thck = 0.0;
+        if (in.whichefvs == 2) {
+            double ux = 0, uy = 0, vx = 0, vy = 0, dsx = 0, dsy = 0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                ux = ux + dx[n] * e.u[n]; uy = uy + dy[n] * e.u[n];
+                vx = vx + dx[n] * e.v[n]; vy = vy + dy[n] * e.v[n];
+                dsx = dsx + c_phi2[n][p] * sx[n];
+                dsy = dsy + c_phi2[n][p] * sy[n];
+                thck = thck + c_phi2[n][p] * e.h[n];
+            }
+            const double esq = in.effstrain_min * in.effstrain_min + ux * ux + vy * vy + ux * vy +
+                               0.25 * ((vx + uy) * (vx + uy));
+            es = sqrt(esq);
+            grads = sqrt(dsx * dsx + dsy * dsy);
+        }
+        double va = 0.0;
+        for (int k = 0; k < nl; ++k) {
+            double ek;
+            if (in.whichefvs == 0) {
+                ek = in.efvs_constant;
+            } else if (in.whichefvs == 1) {
+                const double scyr = 31536000.0;
+                ek = fl[k] * pow((500.0 / scyr) / 200.0e3 * scyr, p_effstr);
+            } else {
+                ek = efvs_l1l2_layer(in, es, thck, grads, in.sigma[k + 1], fa[k]);
+            }
+            va = va + ek * (in.sigma[k + 1] - in.sigma[k]);
+            out[k] = p == 0 ? ek : out[k] + ek;
+        }
+        efva[4 * t + p] = va;
+    }
+    for (int k = 0; k < nl; ++k) out[k] = out[k] / 4.0;
+}
+
+// parallel_halo of nlay cell fields stored one after another (blockIdx.y
+// selects the field), with the rules of k_cell_halo.
+__global__ void k_cell_halo_layers(Dims d, int pass, double* __restrict__ a) {
+    const long long nc = (long long)d.nx * d.ny;
+    double* f = a + (long long)blockIdx.y * nc;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int nx = d.nx, ny = d.ny;
+    if (d.bc == 0 || d.bc == 2) {
+        if (pass == 0) {
+            if (t >= ny - 4) return;
+            const int j = 3 + (int)t;
+            auto c = [&](int i) { return (long long)(i - 1) + (long long)nx * (j - 1); };
+            const double a1 = f[c(nx - 3)], a2 = f[c(nx - 2)], a3 = f[c(3)], a4 = f[c(4)];
+            f[c(1)] = a1; f[c(2)] = a2; f[c(nx - 1)] = a3; f[c(nx)] = a4;
+        } else {
+            if (t >= nx) return;
+            const int i = 1 + (int)t;
+            auto c = [&](int j) { return (long long)(i - 1) + (long long)nx * (j - 1); };
+            const double a1 = f[c(ny - 3)], a2 = f[c(ny - 2)], a3 = f[c(3)], a4 = f[c(4)];
+            f[c(1)] = a1; f[c(2)] = a2; f[c(ny - 1)] = a3; f[c(ny)] = a4;
+        }
+    } else {
+        if (pass != 0 || t >= nc) return;
+        const int i = 1 + (int)(t % nx), j = 1 + (int)(t / nx);
+        const int w = d.bc == 1 ? 2 : 3;
+        if (i <= w || i >= nx - w + 1 || j <= w || j >= ny - w + 1) f[t] = 0.0;
+    }
+}
+
+// compute_3d_velocity_L1L2, cell part: membrane strain rate at the cell
+// centre, tau_parallel(k) = 2 efvs(k) eps_parallel and the efvs integral
+// from the layer midpoint to the surface times the strain combinations
+// (work1..3).  Layer-major outputs: f[(q*nl + k)*nc + c], q = 0 tau_parallel,
+// 1..3 work1..3.
+__global__ void k_l1l2_columns(Dims d, AsmDev in, Lev L, const double* __restrict__ uv, const double* __restrict__ vv,
+                               const double* __restrict__ efvs, double* __restrict__ f, double* __restrict__ flag) {
+#pragma clang fp contract(off)
+    const long long nc = (long long)d.nx * d.ny;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const int ci = 1 + (int)(t % d.nx), cj = 1 + (int)(t / d.nx);
+    const int nl = L.nz - 1;
+    double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0, eps = 0.0;
+    const bool act = cell_in_range(d, in.nhalo, ci, cj) && in.acell[t];
+    const double* ef = efvs + (long long)nl * t;
+    if (act) {
+        Elem2 e;
+        load_elem2(d, in, uv, vv, ci, cj, e);
+        const double cdx[4] = {-0.25, 0.25, 0.25, -0.25}, cdy[4] = {-0.25, -0.25, 0.25, 0.25};
+        double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            J11 = J11 + cdx[n] * e.x[n]; J12 = J12 + cdx[n] * e.y[n];
+            J21 = J21 + cdy[n] * e.x[n]; J22 = J22 + cdy[n] * e.y[n];
+        }
+        const double det = J11 * J22 - J12 * J21;
+        if (!(fabs(det) > 0.0)) { flag[0] = 1.0; return; }
+        const double i11 = J22 / det, i12 = -J12 / det, i21 = -J21 / det, i22 = J11 / det;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const double ddx = 0.0 + i11 * cdx[n] + i12 * cdy[n];
+            const double ddy = 0.0 + i21 * cdx[n] + i22 * cdy[n];
+            ux = ux + ddx * e.u[n]; uy = uy + ddy * e.u[n];
+            vx = vx + ddx * e.v[n]; vy = vy + ddy * e.v[n];
+        }
+        eps = sqrt(ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+    }
+    const double H = in.thck_cell[t];
+    double ei = 0.0;
+    for (int k = 0; k < nl; ++k) {
+        double tp = 0.0;
+        if (act) {
+            const double ssk = (in.sigma[k] + in.sigma[k + 1]) / 2.0;
+            tp = 2.0 * ef[k] * eps;
+            if (k == 0) {
+                ei = ef[0] * ssk * H;
+            } else {
+                const double ssm = (in.sigma[k - 1] + in.sigma[k]) / 2.0;
+                ei = ei + ef[k - 1] * (in.sigma[k] - ssm) * H + ef[k] * (ssk - in.sigma[k]) * H;
+            }
+        }
+        f[(0LL * nl + k) * nc + t] = tp;
+        f[(1LL * nl + k) * nc + t] = ei * (2.0 * ux + vy);
+        f[(2LL * nl + k) * nc + t] = ei * (uy + vx);
+        f[(3LL * nl + k) * nc + t] = ei * (2.0 * vy + ux);
+    }
+}
+
+// glissade_gradient, gradient_margin 1, of cell field f at vertex (i, j)
+__device__ __forceinline__ void gradient_m1_at(const Dims& d, const int32_t* __restrict__ ice,
+                                               const double* __restrict__ f, double dx, double dy, int i, int j,
+                                               double& gx, double& gy) {
+#pragma clang fp contract(off)
+    const long long c00 = c2(d, i, j), c10 = c2(d, i + 1, j), c01 = c2(d, i, j + 1), c11 = c2(d, i + 1, j + 1);
+    const bool es = ice[c00] == 1 && ice[c10] == 1, en = ice[c01] == 1 && ice[c11] == 1;
+    const bool ew = ice[c00] == 1 && ice[c01] == 1, ee = ice[c10] == 1 && ice[c11] == 1;
+    const double n_ = (f[c11] - f[c01]) / dx, s_ = (f[c10] - f[c00]) / dx;
+    gx = (es && en) ? (n_ + s_) / 2.0 : (es ? s_ : (en ? n_ : 0.0));
+    const double e_ = (f[c11] - f[c10]) / dy, w_ = (f[c01] - f[c00]) / dy;
+    gy = (ew && ee) ? (e_ + w_) / 2.0 : (ew ? w_ : (ee ? e_ : 0.0));
+}
+
+// compute_3d_velocity_L1L2, vertex part: from the bed up at owned active
+// vertices u(k) = u(k+1) + fact tau_xz(k) (u(nz) where the basal Dirichlet
+// mask is set); the basal level is x at every vertex; output (nz, nvx, nvy).
+__global__ void k_l1l2_velocity(Dims d, AsmDev in, Lev L, const double* __restrict__ uv,
+                                const double* __restrict__ vv, const int32_t* __restrict__ ice,
+                                const double* __restrict__ efvs, const double* __restrict__ f,
+                                double* __restrict__ u3, double* __restrict__ v3) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const int nz = L.nz, nl = nz - 1;
+    const long long nc = (long long)d.nx * d.ny;
+    const long long b = vidx(d, 0, i, j);
+    const double ub = uv[b], vb = vv[b];
+    u3[(long long)(nz - 1) + (long long)nz * t] = ub;
+    v3[(long long)(nz - 1) + (long long)nz * t] = vb;
+    const bool own = i >= d.ilo && i <= d.ihi && j >= d.jlo && j <= d.jhi && in.avert[t];
+    if (!own) {
+        for (int k = 0; k < nl; ++k) {
+            u3[(long long)k + (long long)nz * t] = 0.0;
+            v3[(long long)k + (long long)nz * t] = 0.0;
+        }
+        return;
+    }
+    const bool um = in.umask[(long long)(nz - 1) + (long long)nz * t] == 1;
+    const bool vm = in.vmask[(long long)(nz - 1) + (long long)nz * t] == 1;
+    const long long c00 = c2(d, i, j), c10 = c2(d, i + 1, j), c01 = c2(d, i, j + 1), c11 = c2(d, i + 1, j + 1);
+    const double sm = (double)(ice[c01] + ice[c11] + ice[c00] + ice[c10]);
+    const double H = in.sthck[t];
+    double uk = ub, vk = vb;
+    for (int k = nl - 1; k >= 0; --k) {
+        double sq = stagger_masked_at(d, ice, f + (long long)k * nc, i, j);
+        sq = sq * sq;
+        const double* fa = in.flwa;
+        const double sv = (double)ice[c01] * fa[(long long)k + nl * c01] + (double)ice[c11] * fa[(long long)k + nl * c11] +
+                          (double)ice[c00] * fa[(long long)k + nl * c00] + (double)ice[c10] * fa[(long long)k + nl * c10];
+        const double sf = sm > 0.0 ? sv / sm : 0.0;
+        double g1x, g1y, g2x, g2y, g3x, g3y;
+        gradient_m1_at(d, ice, f + (1LL * nl + k) * nc, in.dx, in.dy, i, j, g1x, g1y);
+        gradient_m1_at(d, ice, f + (2LL * nl + k) * nc, in.dx, in.dy, i, j, g2x, g2y);
+        gradient_m1_at(d, ice, f + (3LL * nl + k) * nc, in.dx, in.dy, i, j, g3x, g3y);
+        const double ssk = (in.sigma[k] + in.sigma[k + 1]) / 2.0;
+        const double depth = ssk * H;
+        const double txs = -(910.0 * 9.81 * depth * in.dsdx[t]);
+        const double tys = -(910.0 * 9.81 * depth * in.dsdy[t]);
+        const double txz = txs + 2.0 * g1x + g2y;
+        const double tyz = tys + g2x + 2.0 * g3y;
+        const double te = sq + txz * txz + tyz * tyz;
+        double fact;
+        if (in.whichefvs == 2) {
+            fact = 2.0 * sf * pow(te, (in.n_glen - 1.0) / 2.0) * (in.sigma[k + 1] - in.sigma[k]) * H;
+        } else {
+            const double ev = efvs[(long long)k + (long long)nl * c2(d, i, j)];
+            fact = ev > 0.0 ? (in.sigma[k + 1] - in.sigma[k]) * H / ev : 0.0;
+        }
+        uk = um ? ub : uk + fact * txz;
+        vk = vm ? vb : vk + fact * tyz;
+        u3[(long long)k + (long long)nz * t] = uk;
+        v3[(long long)k + (long long)nz * t] = vk;
+    }
+}
+
 }  // namespace cg

@@ -84,6 +84,8 @@ struct cismgpu_system {
     double* diva_buf = nullptr;  // 2-D system: DIVA inputs and state
     DivaDev dv{};
     AsmDev asm_in{};             // device inputs of the last 2-D assembly
+    double* efva2 = nullptr;     // 2-D system: (4, nx, ny) vertical-mean viscosities (L1L2)
+    int last_approx = 0;         // which_ho_approx of the last 2-D assembly
     bool diva_ready = false;
     bool m32_ready = false, a32_ready = false, a16_ready = false;
     bool a32_sym = false;        // FP32 copy in half (symmetric) storage
@@ -757,7 +759,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf, (void*)S->efva2})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -1184,7 +1186,8 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
         cismgpu_set_error("assembly: lateral BC needs usrf, thck and (standard option) the staggered fields");
         return CISMGPU_ERR_ARG;
     }
-    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? nc + 2 * n2 : 0) + (in->thck ? nc : 0)),
+    const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? nc + 2 * n2 : 0) + (in->thck ? nc : 0) +
+                             (in->flwa ? nfl : 0)),
                  ni = (size_t)(nc + n2 + 2 * nvec + (lateral ? nc : 0));
     const size_t bytes = nd * sizeof(double) + ni * sizeof(int32_t);
     if (S->asm_bytes < bytes) {
@@ -1216,6 +1219,7 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     a.theta = putd(in->theta_basal, n2, true);
     a.flwafact = putd(in->flwafact, nfl, true);
     a.thck_cell = in->thck ? putd(in->thck, nc, true) : nullptr;
+    a.flwa = in->flwa ? putd(in->flwa, nfl, true) : nullptr;
     if (lateral) {
         lat.usrf = putd(in->usrf, nc, true);
         lat.thck = a.thck_cell;
@@ -1338,10 +1342,15 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         cismgpu_set_error("assemble_2d: the grid's nz (sigma levels of the inputs) must be 2..64, got %d", S->nz_levels);
         return CISMGPU_ERR_ARG;
     }
-    const bool diva = whichapprox == CISMGPU_APPROX_DIVA;
-    if (whichapprox != CISMGPU_APPROX_SSA && !diva) {
-        cismgpu_set_error("assemble_2d: which_ho_approx %d is not supported (SSA = 1 and DIVA = 4 are)", whichapprox);
+    const bool diva = whichapprox == CISMGPU_APPROX_DIVA, l1l2 = whichapprox == CISMGPU_APPROX_L1L2;
+    if (whichapprox != CISMGPU_APPROX_SSA && !diva && !l1l2) {
+        cismgpu_set_error("assemble_2d: which_ho_approx %d is not supported (SSA 1, L1L2 3, DIVA 4)", whichapprox);
         return CISMGPU_ERR_UNSUPPORTED;
+    }
+    if (l1l2 && (!in->flwa || (in->whichefvs == 2 && !(fabs(in->n_glen - 1.0) < 1.0e-10) &&
+                               !(fabs(in->n_glen - 3.0) < 1.0e-10)))) {
+        cismgpu_set_error("assemble_2d: L1L2 needs the rate factor (in->flwa) and n = 1 or 3 for the nonlinear viscosity");
+        return CISMGPU_ERR_ARG;
     }
     if (diva && (!S->diva_ready || !in->thck)) {
         cismgpu_set_error("assemble_2d: DIVA needs cismgpu_system_diva_setup and the cell thickness (in->thck)");
@@ -1362,6 +1371,7 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     st = upload_assembly_inputs(S, d3, in, a, lat, lateral);
     if (st) return st;
     if (!S->load3) CG_CHECK(hipMalloc(&S->load3, 2 * d3.vlen * sizeof(double)));
+    if (l1l2 && !S->efva2) CG_CHECK(hipMalloc(&S->efva2, 4 * nc * sizeof(double)));
     hipEvent_t e0, e1, e2;
     hipEventCreate(&e0); hipEventCreate(&e1); hipEventCreate(&e2);
     { const int st_ = reset_scal(S); if (st_) return st_; }
@@ -1369,7 +1379,11 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     const double* vv = S->V(1);
     hipEventRecord(e0, S->s);
     S->asm_in = a;
-    if (diva) {   // layer viscosities (state), omega integrals, efvs
+    S->last_approx = whichapprox;
+    if (l1l2) {   // L1L2 layer viscosities and their vertical means
+        k_l1l2_cell<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efva2, S->efvs, S->scal + S_NAN);
+        k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efva2, S->A64, S->scal + S_NAN);
+    } else if (diva) {   // layer viscosities (state), omega integrals, efvs
         k_diva_cell<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, S->dv, uv, vv, S->efvs, S->scal + S_NAN);
         k_assemble_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->dv.efva, S->A64, S->scal + S_NAN);
     } else {
@@ -1386,7 +1400,7 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         k_diva_beta<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->dv);
         k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
     } else {
-        k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
+        if (!l1l2) k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
         if (in->beta_internal) k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
     }
     // b = level sum of the 3-D load (gravity, then the lateral BC)
@@ -1431,6 +1445,50 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     S->tri.mode = 0;
     if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
     return CISMGPU_OK;
+}
+
+// compute_3d_velocity_L1L2 (glissade_velo_higher.F90:5904-6315) from the
+// resident 2-D x and the inputs and layer viscosities of the last L1L2
+// assembly.
+extern "C" int32_t cismgpu_system_l1l2_velocity(cismgpu_system* S, const int32_t* ice_mask, double* uvel,
+                                                double* vvel) {
+    if (!S || !ice_mask || !uvel || !vvel) { cismgpu_set_error("l1l2_velocity: null argument"); return CISMGPU_ERR_ARG; }
+    if (!S->two_d || S->last_approx != CISMGPU_APPROX_L1L2 || !S->asm_in.thck_cell || !S->asm_in.flwa) {
+        cismgpu_set_error("l1l2_velocity: the last 2-D assembly was not an L1L2 one with the cell thickness");
+        return CISMGPU_ERR_ARG;
+    }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny, nz = S->nz_levels, nl = nz - 1;
+    if (4 * nl > 65535) { cismgpu_set_error("l1l2_velocity: too many levels"); return CISMGPU_ERR_ARG; }
+    double* f = nullptr;
+    CG_CHECK(hipMalloc(&f, (4 * nl * nc + 2 * nz * n2) * sizeof(double) + nc * sizeof(int32_t)));
+    double* u3 = f + 4 * nl * nc;
+    double* v3 = u3 + nz * n2;
+    int32_t* ice = (int32_t*)(v3 + nz * n2);
+    int st = CISMGPU_OK;
+    const Lev L{(int)nz};
+    const double* uv = S->V(0);
+    const double* vv = S->V(1);
+    if (hipMemcpyAsync(ice, ice_mask, nc * sizeof(int32_t), hipMemcpyHostToDevice, S->s) != hipSuccess) st = CISMGPU_ERR_HIP;
+    if (!st) st = reset_scal(S);
+    if (!st) {
+        k_l1l2_columns<<<nblocks(nc, 128), 128, 0, S->s>>>(d, S->asm_in, L, uv, vv, S->efvs, f, S->scal + S_NAN);
+        if (d.bc == 0 || d.bc == 2) {
+            k_cell_halo_layers<<<dim3((unsigned)nblocks(d.ny, 256), (unsigned)(4 * nl)), 256, 0, S->s>>>(d, 0, f);
+            k_cell_halo_layers<<<dim3((unsigned)nblocks(d.nx, 256), (unsigned)(4 * nl)), 256, 0, S->s>>>(d, 1, f);
+        } else {
+            k_cell_halo_layers<<<dim3((unsigned)nblocks(nc, 256), (unsigned)(4 * nl)), 256, 0, S->s>>>(d, 0, f);
+        }
+        k_l1l2_velocity<<<nblocks(n2, 128), 128, 0, S->s>>>(d, S->asm_in, L, uv, vv, ice, S->efvs, f, u3, v3);
+        if (hipMemcpyAsync(uvel, u3, nz * n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+            hipMemcpyAsync(vvel, v3, nz * n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess)
+            st = CISMGPU_ERR_HIP;
+    }
+    if (!st && sync_scal(S)) st = CISMGPU_ERR_HIP;
+    hipFree(f);
+    if (st == CISMGPU_ERR_HIP) { cismgpu_set_error("l1l2_velocity: device failure"); return st; }
+    if (!st && S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("Jacobian matrix is singular"); return CISMGPU_ERR_SINGULAR; }
+    return st;
 }
 
 // DIVA inputs and state of a 2-D system (glissade_velo_higher.F90:2078-2140).

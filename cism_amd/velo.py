@@ -33,7 +33,7 @@ class AssemblyIn(ctypes.Structure):
                 ("effstrain_min", ctypes.c_double), ("n_glen", ctypes.c_double),
                 ("check_symmetry", ctypes.c_int32),
                 ("ocean_mask", _P), ("usrf", _P), ("thck", _P), ("stagusrf_lateral", _P), ("stagthck_lateral", _P),
-                ("which_assemble_lateral", ctypes.c_int32)]
+                ("which_assemble_lateral", ctypes.c_int32), ("flwa", _P)]
 
 
 class DivaIn(ctypes.Structure):
@@ -72,9 +72,10 @@ def _declare(L):
     L.cismgpu_system_residual_velocity.argtypes = [vp, i32, i32, D]
     L.cismgpu_system_diva_setup.argtypes = [vp, ctypes.POINTER(DivaIn)]
     L.cismgpu_system_diva_velocity.argtypes = [vp, D, D, D, D]
+    L.cismgpu_system_l1l2_velocity.argtypes = [vp, ctypes.c_void_p, D, D]
     for n in ("cismgpu_system_save_assembly", "cismgpu_system_restore_assembly", "cismgpu_system_work_vectors",
               "cismgpu_system_vector_op", "cismgpu_system_residual_velocity", "cismgpu_system_diva_setup",
-              "cismgpu_system_diva_velocity"):
+              "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity"):
         getattr(L, n).restype = ctypes.c_int32
     L._velo_declared = True
 
@@ -364,6 +365,43 @@ class SSASolver(BPSolver):
         """Picard iteration of the 2-D solve; which_ho_precond 2 is replaced by
         the diagonal preconditioner for 2-D solves (glide_setup.F90:1363-1366)."""
         return super().picard(precond=1 if precond == HO_PRECOND_SIA else precond, **kw)
+
+
+class L1L2Solver(SSASolver):
+    """Device-resident L1L2 solver (Perego, Gunzburger and Burkardt 2012;
+    which_ho_approx = 3): the ``solve_2d`` branch of glissade_velo_higher_solve
+    with the layer viscosities of compute_effective_viscosity_L1L2
+    (glissade_velo_higher.F90:7474-7726) weighting the 2-D elements through
+    their vertical mean (:5228-5248); basal friction, Dirichlet rows, the
+    Picard loop and its acceleration are those of the SSA solve; after the
+    loop compute_3d_velocity_L1L2 (:5904-6315, called at :3698-3724) forms
+    the 3-D velocity."""
+
+    APPROX = 3
+
+    def __init__(self, problem, system: System | None = None):
+        super().__init__(problem, system)
+        self.flwa = np.asfortranarray(np.array(self.p.flwa, dtype=np.float64))
+        self._ain.flwa = self.flwa.ctypes.data
+
+    def velocity_3d(self):
+        """compute_3d_velocity_L1L2 from the resident 2-D x and the last
+        assembly: (uvel, vvel) (nz, nx-1, ny-1), halo not applied."""
+        g = self.grid
+        D = ctypes.POINTER(ctypes.c_double)
+        u3, v3 = g.zeros_vec(), g.zeros_vec()
+        _lib.check(self._L.cismgpu_system_l1l2_velocity(self.sys.handle, self.ice_mask.ctypes.data,
+                                                         u3.ctypes.data_as(D), v3.ctypes.data_as(D)), "l1l2_velocity")
+        return u3, v3
+
+    def picard(self, *, precond=1, **kw) -> PicardResult:
+        """Picard iteration of the L1L2 solve (plain or accelerated); the result
+        adds the 3-D velocity of the converged 2-D velocity (uvel3d, vvel3d)."""
+        r = super().picard(precond=precond, **kw)
+        r.uvel3d = r.vvel3d = None
+        if r.efvs is not None:
+            r.uvel3d, r.vvel3d = self.velocity_3d()
+        return r
 
 
 class DIVASolver(SSASolver):

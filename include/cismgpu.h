@@ -250,6 +250,7 @@ typedef struct cismgpu_assembly_in {
     const double*  stagusrf_lateral; /* (nx-1, ny-1): usrf, thck averaged over ice-covered or */
     const double*  stagthck_lateral; /*   land cells (glissade_velo_higher.F90:2006-2016)     */
     int32_t which_assemble_lateral;  /* 0 standard (staggered values), 1 local (cell values) */
+    const double*  flwa;             /* (nz-1, nx, ny) rate factor A (Pa^-n yr^-1): the L1L2 viscosity */
 } cismgpu_assembly_in;
 
 typedef struct cismgpu_assembly_info {
@@ -279,7 +280,10 @@ int32_t cismgpu_system_assemble_3d(cismgpu_system* sys, const cismgpu_assembly_i
  * with their vertical mean (:5198-5300, compute_effective_viscosity_diva
  * :7730-7997), forms omega (compute_integrals_diva :5649-5775), beta_eff
  * (:2469-2610) and basal_sliding_bc_2d_diva (:8535-8811), with no Dirichlet
- * rows for no-slip (:1423-1428).  L1L2 returns CISMGPU_ERR_UNSUPPORTED. */
+ * rows for no-slip (:1423-1428).  L1L2 (in->flwa required; n = 1 or 3 for
+ * the nonlinear viscosity) takes the layer viscosities of
+ * compute_effective_viscosity_L1L2 (:7474-7726) at every quadrature point
+ * and weights the elements with their vertical mean (:5228-5248). */
 int32_t cismgpu_system_assemble_2d(cismgpu_system* sys, const cismgpu_assembly_in* in, int32_t whichapprox,
                                    cismgpu_assembly_info* info);
 
@@ -313,6 +317,14 @@ int32_t cismgpu_system_diva_setup(cismgpu_system* sys, const cismgpu_diva_in* in
  * (nx-1, ny-1) are optional host outputs. */
 int32_t cismgpu_system_diva_velocity(cismgpu_system* sys, double* uvel, double* vvel, double* btractx,
                                      double* btracty);
+/* compute_3d_velocity_L1L2 (glissade_velo_higher.F90:5904-6315, called at
+ * :3698-3724) from the resident 2-D x (the basal velocity) and the layer
+ * viscosities, surface gradient, thickness and rate factor of the last L1L2
+ * assembly; ice_mask (nx, ny) gives the masked stagger and the one-sided
+ * gradients.  uvel/vvel (nz, nx-1, ny-1): the basal level is x everywhere,
+ * the levels above are set on owned active vertices and zero elsewhere;
+ * the caller applies the halo (staggered_parallel_halo, :3723-3724). */
+int32_t cismgpu_system_l1l2_velocity(cismgpu_system* sys, const int32_t* ice_mask, double* uvel, double* vvel);
 /* Snapshot of the assembled system (A, b, preconditioner) kept on the
  * device, and its restoration: the matrix copies of the accelerated Picard
  * iteration (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184). */

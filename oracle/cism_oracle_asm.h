@@ -107,6 +107,21 @@ void orc_compute_3d_velocity_diva(int nx, int ny, int nz, const int32_t* active_
                                   const double* ssfx, const double* ssfy, const double* uvel_2d,
                                   const double* vvel_2d, double* btractx, double* btracty, double* uvel,
                                   double* vvel);
+double orc_l1l2_tau_parallel(double effstrain, double flwa, double tau_perp);
+int orc_assemble_stiffness_matrix_2d_l1l2(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                          const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                          const double* uvel_2d, const double* vvel_2d, const double* stagthck,
+                                          const double* dusrf_dx, const double* dusrf_dy, const double* flwa,
+                                          const double* flwafact, int whichefvs, double efvs_constant,
+                                          double effstrain_min, double n_glen, double* efvs, double* Auu,
+                                          double* Auv, double* Avu, double* Avv);
+int orc_compute_3d_velocity_l1l2(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int global_bc,
+                                 int nhalo, const double* sigma, double dx, double dy, const int32_t* ice_mask,
+                                 const int32_t* active_cell, const int32_t* active_vertex, const int32_t* umask,
+                                 const int32_t* vmask, const double* xVertex, const double* yVertex,
+                                 const double* thck, const double* stagthck, const double* dusrf_dx,
+                                 const double* dusrf_dy, const double* flwa, const double* efvs, int whichefvs,
+                                 double n_glen, double* uvel, double* vvel);
 
 #ifdef __cplusplus
 }

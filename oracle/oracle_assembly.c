@@ -1344,3 +1344,286 @@ void orc_compute_3d_velocity_diva(int nx, int ny, int nz, const int32_t* active_
         }
     free(ok); free(sok);
 }
+
+/* ------------------------------------------------------------------ */
+/* L1L2 (Perego, Gunzburger and Burkardt 2012)                         */
+/* ------------------------------------------------------------------ */
+
+/* compute_effective_viscosity_L1L2 (glissade_velo_higher.F90:7474-7726) at
+ * quadrature point p: the membrane strain rate and surface slope at the
+ * point give, per layer k, tau_perp = rhoi g (h sigma(k+1)) |grad s| and
+ * tau_parallel, the real root of x^3 + tau_perp^2 x - effstrain/A = 0
+ * (Cardano, with the first-order form of the second cube root when
+ * a^3/27 <= 1e-6 b^2/4); efvs = 1/(2A) for n = 1, 1/(2A(tau_par^2 +
+ * tau_perp^2)) for n = 3, an error otherwise. */
+static int efvs_l1l2(int whichefvs, double efvs_constant, double effstrain_min, double n_glen, int nz,
+                     const double* sigma, int p, const double* dx, const double* dy, const double* u,
+                     const double* v, const double* h, const double* sx, const double* sy, const double* flwa,
+                     const double* flwafact, double* efvs)
+{
+    const double p_effstr = (1.0 - n_glen) / n_glen;
+    if (whichefvs == 0) { for (int k = 0; k < nz - 1; ++k) efvs[k] = efvs_constant; return ORC_OK; }
+    if (whichefvs == 1) {
+        const double scyr = 31536000.0;
+        const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
+        for (int k = 0; k < nz - 1; ++k) efvs[k] = flwafact[k] * pow(effstrain, p_effstr);
+        return ORC_OK;
+    }
+    double ux = 0, uy = 0, vx = 0, vy = 0, dsx = 0, dsy = 0, thck = 0;
+    for (int n = 0; n < 4; ++n) {
+        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+        dsx = dsx + phi2[n][p] * sx[n];
+        dsy = dsy + phi2[n][p] * sy[n];
+        thck = thck + phi2[n][p] * h[n];
+    }
+    const double esq = effstrain_min * effstrain_min + ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
+    const double es = sqrt(esq);
+    for (int k = 0; k < nz - 1; ++k) {
+        const double depth = thck * sigma[k + 1];
+        const double grads = sqrt(dsx * dsx + dsy * dsy);
+        const double tau_perp = RHOI * GRAV * depth * grads;
+        const double a = tau_perp * tau_perp;
+        const double b = -es / flwa[k];
+        const double c = sqrt(b * b / 4.0 + a * a * a / 27.0);
+        const double rootA = pow(-b / 2.0 + c, 1.0 / 3.0);
+        double rootB;
+        if (a * a * a / 27.0 > 1.0e-6 * (b * b / 4.0)) rootB = -pow(b / 2.0 + c, 1.0 / 3.0);
+        else rootB = -a / (3.0 * pow(fabs(b), 1.0 / 3.0));
+        const double tau_par = rootA + rootB;
+        if (fabs(n_glen - 1.0) < 1.0e-10) efvs[k] = 1.0 / (2.0 * flwa[k]);
+        else if (fabs(n_glen - 3.0) < 1.0e-10) efvs[k] = 1.0 / (2.0 * flwa[k] * (tau_par * tau_par + tau_perp * tau_perp));
+        else return ORC_ERR_ARG;
+    }
+    return ORC_OK;
+}
+
+/* One point of compute_effective_viscosity_L1L2's cubic for the pins:
+ * tau_parallel for effstrain, A and tau_perp. */
+double orc_l1l2_tau_parallel(double effstrain, double flwa, double tau_perp)
+{
+    const double a = tau_perp * tau_perp;
+    const double b = -effstrain / flwa;
+    const double c = sqrt(b * b / 4.0 + a * a * a / 27.0);
+    const double rootA = pow(-b / 2.0 + c, 1.0 / 3.0);
+    const double rootB = (a * a * a / 27.0 > 1.0e-6 * (b * b / 4.0)) ? -pow(b / 2.0 + c, 1.0 / 3.0)
+                                                                     : -a / (3.0 * pow(fabs(b), 1.0 / 3.0));
+    return rootA + rootB;
+}
+
+/* assemble_stiffness_matrix_2d, L1L2 (glissade_velo_higher.F90:5228-5248,
+ * 5300-5346): per active cell and quadrature point the L1L2 layer
+ * viscosities from the 2-D velocity, the vertex surface gradient and
+ * thickness, and the layer rate factor flwa (nz-1, nx, ny); their vertical
+ * mean times the interpolated thickness weights the element matrix;
+ * efvs(k,i,j) = mean over the points. */
+int orc_assemble_stiffness_matrix_2d_l1l2(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                          const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                          const double* uvel_2d, const double* vvel_2d, const double* stagthck,
+                                          const double* dusrf_dx, const double* dusrf_dy, const double* flwa,
+                                          const double* flwafact, int whichefvs, double efvs_constant,
+                                          double effstrain_min, double n_glen, double* efvs, double* Auu,
+                                          double* Auv, double* Avu, double* Avv)
+{
+    init_tables_2d();
+    const size_t nA = (size_t)9 * NVX * NVY;
+    memset(efvs, 0, (size_t)(nz - 1) * nx * ny * sizeof(double));
+    memset(Auu, 0, nA * sizeof(double)); memset(Auv, 0, nA * sizeof(double));
+    memset(Avu, 0, nA * sizeof(double)); memset(Avv, 0, nA * sizeof(double));
+    double x[4], y[4], u[4], v[4], h[4], sx[4], sy[4];
+    double* eq = (double*)malloc((size_t)4 * (nz - 1) * sizeof(double));   /* [p][k] */
+    int st = ORC_OK;
+    for (int j = nhalo + 1; j <= ny - nhalo + 1 && !st; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1 && !st; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int n = 0; n < 4; ++n) {
+                const size_t q = V2(i + di2[n], j + dj2[n]);
+                x[n] = xVertex[q]; y[n] = yVertex[q]; u[n] = uvel_2d[q]; v[n] = vvel_2d[q];
+                h[n] = stagthck[q]; sx[n] = dusrf_dx[q]; sy[n] = dusrf_dy[q];
+            }
+            const double* fa = flwa + (size_t)(nz - 1) * C2(i, j);
+            const double* fl = flwafact + (size_t)(nz - 1) * C2(i, j);
+            double Kuu[4][4] = {{0}}, Kuv[4][4] = {{0}}, Kvu[4][4] = {{0}}, Kvv[4][4] = {{0}};
+            for (int p = 0; p < 4 && !st; ++p) {
+                double dx[4], dy[4], detJ;
+                st = basis_derivs_2d(x, y, p, dx, dy, &detJ);
+                if (st) break;
+                double* e = eq + (size_t)p * (nz - 1);
+                st = efvs_l1l2(whichefvs, efvs_constant, effstrain_min, n_glen, nz, sigma, p, dx, dy, u, v, h, sx, sy,
+                               fa, fl, e);
+                if (st) break;
+                double va = 0.0;
+                for (int k = 0; k < nz - 1; ++k) va = va + e[k] * (sigma[k + 1] - sigma[k]);
+                double h_qp = 0.0;
+                for (int n = 0; n < 4; ++n) h_qp = h_qp + phi2[n][p] * h[n];
+                const double f = (h_qp * va) * 1.0 * detJ / VOL0;
+                for (int nc = 0; nc < 4; ++nc)
+                    for (int nr = 0; nr < 4; ++nr) {
+                        Kuu[nr][nc] = Kuu[nr][nc] + f * (4.0 * dx[nr] * dx[nc] + dy[nr] * dy[nc]);
+                        Kuv[nr][nc] = Kuv[nr][nc] + f * (2.0 * dx[nr] * dy[nc] + dy[nr] * dx[nc]);
+                        Kvu[nr][nc] = Kvu[nr][nc] + f * (2.0 * dy[nr] * dx[nc] + dx[nr] * dy[nc]);
+                        Kvv[nr][nc] = Kvv[nr][nc] + f * (4.0 * dy[nr] * dy[nc] + dx[nr] * dx[nc]);
+                    }
+            }
+            if (st) break;
+            for (int k = 0; k < nz - 1; ++k) {
+                double s = 0.0;
+                for (int p = 0; p < 4; ++p) s = s + eq[p * (nz - 1) + k];
+                efvs[(size_t)k + (size_t)(nz - 1) * C2(i, j)] = s / 4.0;
+            }
+            for (int nr = 0; nr < 4; ++nr) {
+                const int ir = i + di2[nr], jr = j + dj2[nr];
+                for (int nc = 0; nc < 4; ++nc) {
+                    const int m = IDX2(di2[nc] - di2[nr], dj2[nc] - dj2[nr]);
+                    const size_t a = A2(m, ir, jr);
+                    Auu[a] = Auu[a] + Kuu[nr][nc];
+                    Auv[a] = Auv[a] + Kuv[nr][nc];
+                    Avu[a] = Avu[a] + Kvu[nr][nc];
+                    Avv[a] = Avv[a] + Kvv[nr][nc];
+                }
+            }
+        }
+    free(eq);
+    return st;
+}
+
+/* glissade_gradient with gradient_margin 1 (glissade_grid_operators.F90:
+ * 309-468): an edge difference counts where both cells of the edge have
+ * ice; the vertex value is the mean of the two edges in a direction, the one
+ * edge that counts, or zero. */
+static void gradient_margin1(int nx, int ny, double dx, double dy, const double* f, const int32_t* ice,
+                             double* gx, double* gy)
+{
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const int es = ice[C2(i, j)] == 1 && ice[C2(i + 1, j)] == 1;
+            const int en = ice[C2(i, j + 1)] == 1 && ice[C2(i + 1, j + 1)] == 1;
+            const int ew = ice[C2(i, j)] == 1 && ice[C2(i, j + 1)] == 1;
+            const int ee = ice[C2(i + 1, j)] == 1 && ice[C2(i + 1, j + 1)] == 1;
+            const double n_ = (f[C2(i + 1, j + 1)] - f[C2(i, j + 1)]) / dx;
+            const double s_ = (f[C2(i + 1, j)] - f[C2(i, j)]) / dx;
+            gx[V2(i, j)] = (es && en) ? (n_ + s_) / 2.0 : (es ? s_ : (en ? n_ : 0.0));
+            const double e_ = (f[C2(i + 1, j + 1)] - f[C2(i + 1, j)]) / dy;
+            const double w_ = (f[C2(i, j + 1)] - f[C2(i, j)]) / dy;
+            gy[V2(i, j)] = (ew && ee) ? (e_ + w_) / 2.0 : (ew ? w_ : (ee ? e_ : 0.0));
+        }
+}
+
+/* compute_3d_velocity_L1L2 (glissade_velo_higher.F90:5904-6315): from the
+ * basal (2-D) velocity, uvel(nz) on input, and the layer viscosities efvs
+ * (nz-1, nx, ny) of the last assembly.  Per active cell: the membrane
+ * strain rate at the cell centre, tau_parallel(k) = 2 efvs eps_parallel and
+ * the integral of efvs from the layer midpoint to the surface; halos; then
+ * from the bed up at owned active vertices
+ *     u(k) = u(k+1) + fact(k) tau_xz(k),
+ * tau_xz = -rhoi g depth ds/dx + 2 d(work1)/dx + d(work2)/dy (membrane
+ * terms included), fact = 2 A tau_eff^(n-1) dz (nonlinear viscosity) or
+ * dz / efvs(k, i, j) (the cell of the vertex's index, as the reference), and
+ * u(k) = u(nz) where the basal Dirichlet mask is set.  uvel/vvel (nz, nx-1,
+ * ny-1): levels 1..nz-1 are overwritten (zero off the owned active
+ * vertices); no halo. */
+int orc_compute_3d_velocity_l1l2(int nx, int ny, int nz, int ilo, int ihi, int jlo, int jhi, int global_bc,
+                                 int nhalo, const double* sigma, double dx, double dy, const int32_t* ice_mask,
+                                 const int32_t* active_cell, const int32_t* active_vertex, const int32_t* umask,
+                                 const int32_t* vmask, const double* xVertex, const double* yVertex,
+                                 const double* thck, const double* stagthck, const double* dusrf_dx,
+                                 const double* dusrf_dy, const double* flwa, const double* efvs, int whichefvs,
+                                 double n_glen, double* uvel, double* vvel)
+{
+    const size_t nc = (size_t)nx * ny, n2 = (size_t)NVX * NVY;
+    const int nl = nz - 1;
+    double* ss = (double*)malloc((size_t)nl * sizeof(double));
+    for (int k = 0; k < nl; ++k) ss[k] = (sigma[k] + sigma[k + 1]) / 2.0;
+    double* ux = (double*)calloc(nc, sizeof(double));
+    double* uy = (double*)calloc(nc, sizeof(double));
+    double* vx = (double*)calloc(nc, sizeof(double));
+    double* vy = (double*)calloc(nc, sizeof(double));
+    double* tp = (double*)calloc((size_t)nl * nc, sizeof(double));    /* [k][c] */
+    double* ei = (double*)calloc((size_t)nl * nc, sizeof(double));    /* [k][c] */
+    double* w = (double*)malloc(3 * nc * sizeof(double));
+    double* g = (double*)malloc(8 * n2 * sizeof(double));
+    const double cdx[4] = {-0.25, 0.25, 0.25, -0.25}, cdy[4] = {-0.25, -0.25, 0.25, 0.25};
+    int st = ORC_OK;
+    for (size_t p = 0; p < n2; ++p)
+        for (int k = 1; k <= nz - 1; ++k) { uvel[(size_t)(k - 1) + (size_t)nz * p] = 0.0; vvel[(size_t)(k - 1) + (size_t)nz * p] = 0.0; }
+    for (int j = nhalo + 1; j <= ny - nhalo + 1 && !st; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            const size_t c = C2(i, j);
+            if (!active_cell[c]) continue;
+            double x[4], y[4], u[4], v[4];
+            for (int n = 0; n < 4; ++n) {
+                const size_t q = V2(i + di2[n], j + dj2[n]);
+                x[n] = xVertex[q]; y[n] = yVertex[q];
+                u[n] = uvel[(size_t)(nz - 1) + (size_t)nz * q];
+                v[n] = vvel[(size_t)(nz - 1) + (size_t)nz * q];
+            }
+            double J11 = 0.0, J12 = 0.0, J21 = 0.0, J22 = 0.0;
+            for (int n = 0; n < 4; ++n) {
+                J11 = J11 + cdx[n] * x[n]; J12 = J12 + cdx[n] * y[n];
+                J21 = J21 + cdy[n] * x[n]; J22 = J22 + cdy[n] * y[n];
+            }
+            const double d = J11 * J22 - J12 * J21;
+            if (!(fabs(d) > 0.0)) { st = ORC_ERR_SINGULAR; break; }
+            const double i11 = J22 / d, i12 = -J12 / d, i21 = -J21 / d, i22 = J11 / d;
+            for (int n = 0; n < 4; ++n) {
+                const double ddx = 0.0 + i11 * cdx[n] + i12 * cdy[n];
+                const double ddy = 0.0 + i21 * cdx[n] + i22 * cdy[n];
+                ux[c] = ux[c] + ddx * u[n]; uy[c] = uy[c] + ddy * u[n];
+                vx[c] = vx[c] + ddx * v[n]; vy[c] = vy[c] + ddy * v[n];
+            }
+            const double eps = sqrt(ux[c] * ux[c] + vy[c] * vy[c] + ux[c] * vy[c] + 0.25 * ((vx[c] + uy[c]) * (vx[c] + uy[c])));
+            const double* ef = efvs + (size_t)nl * c;
+            for (int k = 0; k < nl; ++k) tp[(size_t)k * nc + c] = 2.0 * ef[k] * eps;
+            ei[c] = ef[0] * ss[0] * thck[c];
+            for (int k = 1; k < nl; ++k)
+                ei[(size_t)k * nc + c] = ei[(size_t)(k - 1) * nc + c] + ef[k - 1] * (sigma[k] - ss[k - 1]) * thck[c] +
+                                         ef[k] * (ss[k] - sigma[k]) * thck[c];
+        }
+    if (st) goto done;
+    for (int k = 0; k < nl; ++k) orc_cell_halo(nx, ny, global_bc, tp + (size_t)k * nc);
+    for (int k = nl - 1; k >= 0; --k) {
+        double *sq = g, *sf = g + n2, *g1x = g + 2 * n2, *g1y = g + 3 * n2, *g2x = g + 4 * n2, *g2y = g + 5 * n2,
+               *g3x = g + 6 * n2, *g3y = g + 7 * n2;
+        orc_stagger_masked(nx, ny, tp + (size_t)k * nc, ice_mask, sq);
+        for (size_t p = 0; p < n2; ++p) sq[p] = sq[p] * sq[p];
+        double* fk = (double*)malloc(nc * sizeof(double));
+        for (size_t c = 0; c < nc; ++c) fk[c] = flwa[(size_t)k + (size_t)nl * c];
+        orc_stagger_masked(nx, ny, fk, ice_mask, sf);
+        free(fk);
+        const double* e = ei + (size_t)k * nc;
+        for (size_t c = 0; c < nc; ++c) {
+            w[c] = e[c] * (2.0 * ux[c] + vy[c]);
+            w[nc + c] = e[c] * (uy[c] + vx[c]);
+            w[2 * nc + c] = e[c] * (2.0 * vy[c] + ux[c]);
+        }
+        for (int q = 0; q < 3; ++q) orc_cell_halo(nx, ny, global_bc, w + (size_t)q * nc);
+        gradient_margin1(nx, ny, dx, dy, w, ice_mask, g1x, g1y);
+        gradient_margin1(nx, ny, dx, dy, w + nc, ice_mask, g2x, g2y);
+        gradient_margin1(nx, ny, dx, dy, w + 2 * nc, ice_mask, g3x, g3y);
+        for (int j = jlo; j <= jhi; ++j)
+            for (int i = ilo; i <= ihi; ++i) {
+                const size_t p = V2(i, j);
+                if (!active_vertex[p]) continue;
+                const double depth = ss[k] * stagthck[p];
+                const double txs = -RHOI * GRAV * depth * dusrf_dx[p];
+                const double tys = -RHOI * GRAV * depth * dusrf_dy[p];
+                const double txz = txs + 2.0 * g1x[p] + g2y[p];
+                const double tyz = tys + g2x[p] + 2.0 * g3y[p];
+                const double te = sq[p] + txz * txz + tyz * tyz;
+                double fact;
+                if (whichefvs == 2) {
+                    fact = 2.0 * sf[p] * pow(te, (n_glen - 1.0) / 2.0) * (sigma[k + 1] - sigma[k]) * stagthck[p];
+                } else {
+                    const double ev = efvs[(size_t)k + (size_t)nl * C2(i, j)];
+                    fact = ev > 0.0 ? (sigma[k + 1] - sigma[k]) * stagthck[p] / ev : 0.0;
+                }
+                const size_t a = (size_t)k + (size_t)nz * p, b = (size_t)(k + 1) + (size_t)nz * p,
+                             base = (size_t)(nz - 1) + (size_t)nz * p;
+                uvel[a] = umask[p] == 1 ? uvel[base] : uvel[b] + fact * txz;
+                vvel[a] = vmask[p] == 1 ? vvel[base] : vvel[b] + fact * tyz;
+            }
+    }
+done:
+    free(ss); free(ux); free(uy); free(vx); free(vy); free(tp); free(ei); free(w); free(g);
+    return st;
+}

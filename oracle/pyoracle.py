@@ -109,6 +109,14 @@ def _declare(L):
     L.orc_basal_sliding_bc_2d_diva_local.argtypes = [i, i, d, d, _I, _D, _D, _D, _D, _D, _D]
     L.orc_compute_3d_velocity_diva.argtypes = [i, i, i, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D,
                                                _D, _D]
+    L.orc_l1l2_tau_parallel.argtypes = [d, d, d]
+    L.orc_l1l2_tau_parallel.restype = ctypes.c_double
+    L.orc_assemble_stiffness_matrix_2d_l1l2.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, i, d,
+                                                        d, d, _D, _D, _D, _D, _D]
+    L.orc_assemble_stiffness_matrix_2d_l1l2.restype = ctypes.c_int
+    L.orc_compute_3d_velocity_l1l2.argtypes = [i, i, i, i, i, i, i, i, i, _D, d, d, _I, _I, _I, _I, _I, _D, _D, _D,
+                                               _D, _D, _D, _D, _D, i, d, _D, _D]
+    L.orc_compute_3d_velocity_l1l2.restype = ctypes.c_int
 
 
 def dp(a: np.ndarray):
@@ -502,6 +510,17 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
                                              dp(dg["theta_basal_y"]), dp(A[0]), dp(A[3]))
         extra = dict(omega_k=omega_k, omega=omega, stag_omega=stag_omega, beta_eff_x=bex, beta_eff_y=bey,
                      state=state)
+    elif approx == "l1l2":
+        flwa = np.asfortranarray(np.array(p.flwa, dtype=np.float64))
+        st = L.orc_assemble_stiffness_matrix_2d_l1l2(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                                     dp(u2), dp(v2), dp(geo["stagthck"]), dp(geo["dusrf_dx"]),
+                                                     dp(geo["dusrf_dy"]), dp(flwa), dp(geo["flwafact"]), p.whichefvs,
+                                                     p.efvs_constant, p.effstrain_min, p.n_glen, dp(efvs),
+                                                     *[dp(a) for a in A])
+        if st:
+            raise RuntimeError(f"oracle 2-D L1L2 assembly failed with status {st}")
+        if p.whichbabc != 6:
+            L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
     else:
         st = L.orc_assemble_stiffness_matrix_2d_ssa(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV),
                                                     dp(yV), dp(u2), dp(v2), dp(geo["stagthck"]),
@@ -548,6 +567,35 @@ def diva_velocity(problem, s, u2, v2):
     return u3, v3
 
 
+def l1l2_velocity(problem, s, u2, v2):
+    """compute_3d_velocity_L1L2 after a 2-D L1L2 solve (glissade_velo_higher.F90:
+    3698-3724): the basal level is the 2-D velocity, the levels above follow
+    from the layer viscosities of the last assembly s; returns the 3-D
+    velocity (nz, nx-1, ny-1) before its halo."""
+    L = lib()
+    p = problem
+    g = p.grid
+    geo = s["geo"]
+    u3 = g.zeros_vec(); v3 = g.zeros_vec()
+    u3[g.nz - 1] = u2
+    v3[g.nz - 1] = v2
+    mask = np.zeros((g.nx - 1, g.ny - 1), np.int32, order="F")
+    if p.whichbabc == 6:
+        mask[:, :] = 1
+    xV, yV = p.vertex_coords()
+    flwa = np.asfortranarray(np.array(p.flwa, dtype=np.float64))
+    thck = np.asfortranarray(np.array(p.thck, dtype=np.float64))
+    st = L.orc_compute_3d_velocity_l1l2(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, g.global_bc, 2,
+                                        dp(np.ascontiguousarray(p.sigma, dtype=np.float64)), p.dx, p.dy,
+                                        ip(geo["ice_mask"]), ip(geo["active_cell"]), ip(s["active_vertex"]), ip(mask),
+                                        ip(mask), dp(xV), dp(yV), dp(thck), dp(geo["stagthck"]), dp(geo["dusrf_dx"]),
+                                        dp(geo["dusrf_dy"]), dp(flwa), dp(s["efvs"]), p.whichefvs, p.n_glen, dp(u3),
+                                        dp(v3))
+    if st:
+        raise RuntimeError(f"oracle L1L2 3-D velocity failed with status {st}")
+    return u3, v3
+
+
 def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
     """compute_residual_vector_3d (glissade_velo_higher.F90:9693-9894): L2 of A u - b
     over owned active vertices, and relative to |b|."""
@@ -590,7 +638,8 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
     evaluate_accelerated_picard_3d :10052-10184).  two_d: the SSA branch
     (solve_2d; assemble_2d, the 2-D solvers with which_ho_precond 2 taken as
     1, compute_residual_velocity_2d), started from the basal level of the
-    3-D guess (:1362).  approx "diva" (two_d): DIVA assembly with its state
+    3-D guess (:1362).  approx "l1l2" (two_d): the L1L2 viscosities, and
+    compute_3d_velocity_L1L2 after the loop (:3698-3724).  approx "diva" (two_d): DIVA assembly with its state
     and compute_3d_velocity_diva after each solve (:3566-3590), plain Picard
     only; the result adds the 3-D velocity (uvel3d, vvel3d)."""
     L = lib()
@@ -677,5 +726,7 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
         if accelerate:
             du, dv = u - u_old, v - v_old
         criterion = L2 if whichresid == 3 else (L2r if whichresid == 4 else rv)
+    if two_d and approx == "l1l2" and s is not None:
+        u3, v3 = l1l2_velocity(p, s, u, v)
     return dict(uvel=u, vvel=v, efvs=s["efvs"] if s else None, iterations=counter,
                 converged=criterion < resid_target, uvel3d=u3, vvel3d=v3, state=dstate, **hist)

@@ -112,8 +112,8 @@ def test_ssa_picard_matches_oracle(require_gpu, name, accelerate):
 
 
 def test_assemble_2d_guards(require_gpu):
-    """The 2-D assembly refuses a 3-D system, the L1L2 approximation and DIVA
-    without its set-up."""
+    """The 2-D assembly refuses a 3-D system, approximations other than SSA,
+    L1L2 and DIVA, L1L2 without the rate factor and DIVA without its set-up."""
     from cism_amd import _lib
     from cism_amd.velo import AssemblyInfo, BPSolver
     import ctypes
@@ -122,7 +122,7 @@ def test_assemble_2d_guards(require_gpu):
     st = b3._L.cismgpu_system_assemble_2d(b3.sys.handle, ctypes.byref(b3._ain), 1, ctypes.byref(AssemblyInfo()))
     assert st == 3
     s = SSASolver(p)
-    for approx, code in ((3, 11), (4, 3)):
+    for approx, code in ((0, 11), (2, 11), (3, 3), (4, 3)):
         st = s._L.cismgpu_system_assemble_2d(s.sys.handle, ctypes.byref(s._ain), approx, ctypes.byref(AssemblyInfo()))
         assert st == code
     st = s._L.cismgpu_system_assemble_3d(s.sys.handle, ctypes.byref(s._ain), ctypes.byref(AssemblyInfo()))

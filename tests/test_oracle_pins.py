@@ -14,7 +14,7 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from cism_amd import synthetic
+from cism_amd import geometry as ge, synthetic
 from cism_amd.grid import BC_NO_ICE, BC_OUTFLOW, BC_PERIODIC, glide_sigma, indxA_3d
 from oracle import pyoracle as po
 from systems import dome_system, random_spd_system, stream_system
@@ -956,3 +956,105 @@ def test_diva_without_basal_traction_is_ssa_stiffness():
     for key in ("Auu", "Auv", "Avu", "Avv"):
         a, b = ssa[key], diva[key]
         assert np.allclose(a, b, rtol=1e-11, atol=1e-11 * np.abs(a).max()), key
+
+
+# ------------------------------------------------------------------- 2-D L1L2
+@pytest.mark.parametrize("tau_perp", [0.0, 1.0e-3, 3.0e2, 1.0e4, 2.0e5, 1.0e6])
+@pytest.mark.parametrize("effstrain", [1.0e-6, 1.0e-3, 0.3])
+def test_l1l2_cubic_root_is_the_polynomial_root(effstrain, tau_perp):
+    """tau_parallel of compute_effective_viscosity_L1L2 is the positive real
+    root of x^3 + tau_perp^2 x - effstrain/A = 0 (numpy's companion-matrix
+    roots), in both branches of the second cube root."""
+    A = 1.0e-16
+    got = po.lib().orc_l1l2_tau_parallel(effstrain, A, tau_perp)
+    r = np.roots([1.0, 0.0, tau_perp * tau_perp, -effstrain / A])
+    want = max(float(z.real) for z in r if abs(z.imag) <= 1e-9 * abs(z))
+    assert got == pytest.approx(want, rel=1e-8)
+    assert abs(got ** 3 + tau_perp ** 2 * got - effstrain / A) <= 1e-8 * (effstrain / A)
+
+
+def _l1l2_velocity_direct(p, u2, v2, thck, dsdx, dsdy, eta):
+    """compute_3d_velocity_L1L2 of the restatement on given fields (ice
+    everywhere, viscosity eta in every layer)."""
+    g = p.grid
+    ice = np.ones((g.nx, g.ny), np.int32, order="F")
+    acell, avert = ge.vertex_masks(ice)
+    xV, yV = p.vertex_coords()
+    u3, v3 = g.zeros_vec(), g.zeros_vec()
+    u3[g.nz - 1] = u2
+    v3[g.nz - 1] = v2
+    efvs = np.full((g.nz - 1, g.nx, g.ny), eta, order="F")
+    zero = np.zeros((g.nvx, g.nvy), np.int32, order="F")
+    st = po.lib().orc_compute_3d_velocity_l1l2(
+        g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, g.global_bc, 2, po.dp(np.ascontiguousarray(p.sigma)), p.dx,
+        p.dy, po.ip(ice), po.ip(acell), po.ip(avert), po.ip(zero), po.ip(zero), po.dp(xV), po.dp(yV),
+        po.dp(np.asfortranarray(thck)), po.dp(ge.stagger(thck)), po.dp(np.asfortranarray(dsdx)),
+        po.dp(np.asfortranarray(dsdy)), po.dp(np.full((g.nz - 1, g.nx, g.ny), 1e-16, order="F")), po.dp(efvs), 0,
+        3.0, po.dp(u3), po.dp(v3))
+    assert st == 0
+    return u3, v3
+
+
+def test_l1l2_reconstruction_membrane_and_shear_closed_form():
+    """With a constant viscosity eta, linear 2-D velocities u = a1 x + a2 y,
+    v = b1 x + b2 y, thickness H = H0 + cx x + cy y and a uniform surface
+    slope, the membrane terms of compute_3d_velocity_L1L2 are constant in
+    each layer and the midpoint rule is exact, so
+        u(sigma) - u_b = H (1 - sigma^2)/2 [2(2 a1 + b2) cx + (a2 + b1) cy - rho g H s_x / eta]
+        v(sigma) - v_b = H (1 - sigma^2)/2 [(a2 + b1) cx + 2(2 b2 + a1) cy - rho g H s_y / eta]
+    at every interior vertex (H at the vertex)."""
+    p = synthetic.ice_stream(20, nz=9, seed=2)
+    g = p.grid
+    g.global_bc = 1
+    a1, a2, b1, b2 = 3e-3, -1e-3, 2e-3, 5e-4
+    cx, cy, H0, eta = 2e-3, -1e-3, 800.0, 3e13
+    sx, sy = -2e-3, 1e-3
+    xV, yV = p.vertex_coords()
+    xc = (np.arange(g.nx) - 0.5) * p.dx
+    yc = (np.arange(g.ny) - 0.5) * p.dy
+    thck = np.asfortranarray(H0 + cx * xc[:, None] + cy * yc[None, :])
+    u2 = np.asfortranarray(a1 * xV + a2 * yV)
+    v2 = np.asfortranarray(b1 * xV + b2 * yV)
+    dsdx = np.full((g.nvx, g.nvy), sx, order="F")
+    dsdy = np.full((g.nvx, g.nvy), sy, order="F")
+    u3, v3 = _l1l2_velocity_direct(p, u2, v2, thck, dsdx, dsdy, eta)
+    Hv = ge.stagger(thck)
+    rg = 910.0 * 9.81
+    fu = 2 * (2 * a1 + b2) * cx + (a2 + b1) * cy - rg * Hv * sx / eta
+    fv = (a2 + b1) * cx + 2 * (2 * b2 + a1) * cy - rg * Hv * sy / eta
+    sig = p.sigma[:, None, None]
+    wu = u2[None] + Hv[None] * (1 - sig ** 2) / 2 * fu[None]
+    wv = v2[None] + Hv[None] * (1 - sig ** 2) / 2 * fv[None]
+    inner = (slice(None), slice(g.ilo, g.ihi - 1), slice(g.jlo, g.jhi - 1))
+    du, dv = (u3 - u2[None])[inner], (v3 - v2[None])[inner]
+    assert np.abs(du - (wu - u2[None])[inner]).max() < 1e-9 * np.abs(du).max()
+    assert np.abs(dv - (wv - v2[None])[inner]).max() < 1e-9 * np.abs(dv).max()
+
+
+def _l1l2_slab(nz):
+    th = 5.0 * pi / 180
+    beta, Hp, A = 2000.0, 1000.0, 1.0e-16
+    p = synthetic.slab(n_cells=6, nz=nz, dx=50.0, thck=Hp / cos(th), theta_deg=5.0, beta=beta, flwa=A)
+    r = po.picard(p, two_d=True, approx="l1l2", precond=1, linear_tol=1e-11, linear_maxiters=3000,
+                  resid_target=1e-9, maxiter=300)
+    assert r["converged"]
+    g = p.grid
+    geo = po.geometry(p)
+    i, j = g.ilo + 1, g.jlo + 1
+    H = geo["stagthck"][i, j]
+    sx = geo["dusrf_dx"][i, j]
+    ub = -910.0 * 9.81 * H * sx / beta
+    u = r["uvel3d"][:, i, j]
+    want = ub + 2 * A * (910.0 * 9.81 * abs(sx)) ** 3 * H ** 4 * (1 - p.sigma ** 4) / 4
+    return float(np.abs(u - want).max() / np.abs(want - ub).max()), abs(u[-1] / ub - 1.0)
+
+
+def test_l1l2_slab_sia_profile_and_sliding():
+    """On the periodic inclined slab the L1L2 solve slides at the closed-form
+    u_b = -rho g H s_x / beta (membrane stresses vanish) and the 3-D velocity
+    approaches the shallow-ice profile u_b + 2A (rho g |s_x|)^3 H^4 (1 -
+    sigma^4)/4 at second order in the level spacing."""
+    e1, b1 = _l1l2_slab(11)
+    e2, b2 = _l1l2_slab(21)
+    assert b1 < 1e-6 and b2 < 1e-6
+    assert e2 < 2e-3 and 3.5 < e1 / e2 < 4.5
