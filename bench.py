@@ -141,7 +141,7 @@ def main():
     # HBM bytes per launch of the same kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     # passes of this workload, scaled by the counters' reading of a copy kernel of known size
     traffic = None
-    kname = {1: "cg::k_spmv_q<float, 1>", 2: "cg::k_spmv_q<__hip_bfloat16, 1>", 3: "cg::k_spmv_sym<float, 1>"}[spmv_kind]
+    kname = {1: "cg::k_spmv_q<float, 1>", 2: "cg::k_spmv_q<__hip_bfloat16, 1>", 3: "cg::k_spmv_symx<1>"}[spmv_kind]
     pmc_path = os.path.join(ROOT, "profiles", "spmv_pmc_icestream1024.json")
     if args.n == 1024 and args.nz == 11 and os.path.exists(pmc_path):
         with open(pmc_path) as f:

@@ -165,6 +165,22 @@ void halo2d(cismgpu_system* S, TV* a, TV* b) {
     }
 }
 
+// FP32 half storage: the kernel with the level-to-level LDS exchange of the
+// partner slots (default) or the one that re-reads them (CISMGPU_SPMV_SYMX=0)
+static bool use_symx() {
+    const char* e = getenv("CISMGPU_SPMV_SYMX");
+    return !(e && e[0] == '0');
+}
+
+template <int MODE>
+void spmv_sym32(cismgpu_system* S, const float* A, const float* xu, const float* xv, float* yu, float* yv,
+                const float* bu, const float* bv, double* part) {
+    if (use_symx())
+        k_spmv_symx<MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+    else
+        k_spmv_sym<float, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
+}
+
 template <typename TA, typename TV, int MODE>
 void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV* yv, const TV* bu, const TV* bv,
           double* part) {
@@ -179,7 +195,8 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
             k_spmv<TA, TV, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     }
     else if (std::is_same<TA, float>::value && S->a32_sym)   // FP32 half (symmetric) storage
-        k_spmv_sym<float, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, (const float*)A, xu, xv, yu, yv, bu, bv, part);
+        spmv_sym32<MODE>(S, (const float*)A, (const float*)xu, (const float*)xv, (float*)yu, (float*)yv,
+                         (const float*)bu, (const float*)bv, part);
     else   // FP32 / BF16 copies in the quad layout
         k_spmv_q<TA, MODE><<<spmv_blocks(S->d), 256, 0, S->s>>>(S->d, A, xu, xv, yu, yv, bu, bv, part);
     if (t) { hipEventRecord(S->ev[S->ev_used + 1], S->s); S->ev_used += 2; }
@@ -1020,7 +1037,7 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
         else if (kind == 0)
             k_spmv<double, double, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), S->V(12), S->V(13), nullptr, nullptr, S->partB);
         else if (kind == 1 && S->a32_sym)
-            k_spmv_sym<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
+            spmv_sym32<1>(S, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         else if (kind == 1)
             k_spmv_q<float, 1><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A32, S->W(10), S->W(11), S->W(12), S->W(13), nullptr, nullptr, S->partB);
         else

@@ -12,15 +12,15 @@
 // (uu, uv, vu, vv) of the upper couplings, so the matrix streams 224 B per
 // node instead of 432.
 //
-// The SpMV walks a column up the levels like k_spmv_q and loads the 13
-// lower quads from the partner rows; those rows are the neighbouring
-// columns of the same tile (same level) or of the previous level, read
-// moments earlier by the neighbouring threads, so they come from L2
-// (measured: HBM traffic 1.13x the half-storage bytes).  The products are
-// summed in the same order (m = 0..26) with the same values as the
-// full-storage kernel, so when the FP32 matrix is exactly symmetric
-// (k_check_sym finds no mismatch) both kernels give identical results; the
-// solver uses this layout only then.  x is read per neighbour (L1/L2 hits)
+// k_spmv_sym walks a column up the levels like k_spmv_q and loads the 13
+// lower quads from the partner rows, the neighbouring columns of the same
+// level or of the previous level; k_spmv_symx (the default) takes the nine
+// previous-level quads from the tile's own threads through LDS and lane
+// shuffles instead (0.651 against 0.669 ms at 1024^2 x 11 on MI355X).  The
+// products are summed in the same order (m = 0..26) with the same values as
+// the full-storage kernel, so when the FP32 matrix is exactly symmetric
+// (k_check_sym finds no mismatch) all three kernels give identical results;
+// the solver uses this layout only then.  x is read per neighbour (L1/L2 hits)
 // instead of being kept in a 3-level register window: the window cost 54
 // VGPRs and limited the kernel to 3 waves per SIMD (0.87 ms against 0.73 ms
 // at 1024^2 x 11).
@@ -200,6 +200,188 @@ __global__ __launch_bounds__(256) void k_spmv_sym(Dims d, const TA* __restrict__
             }
         }
         __syncthreads();   // the slot of level k-1 is restaged next
+    }
+    if (MODE != 0) {
+        dd v[1] = {{acc, 0.0}};
+        block_reduce_dd<1, 256>(v);
+        if (threadIdx.x == 0) {
+            partials[2 * lin + 0] = v[0].hi;
+            partials[2 * lin + 1] = v[0].lo;
+        }
+    }
+}
+
+// FP32 half storage with a level-to-level exchange of the partner slots.
+// The lower couplings of row (k, i, j) to level k-1 (m = 0..8) are the
+// transposed slots 13 - m of the rows (k-1, i+iA, j+jA), which the tile's
+// own threads loaded one level earlier as their couplings to level k
+// (slots 5..13).  Instead of re-reading them (they fall out of L2 between
+// two levels: one level step of all resident tiles exceeds an XCD's 4 MB),
+// each thread keeps its slots 8, 9, 10 in registers for itself and its two
+// lane neighbours (cross-lane shuffles) and writes slots 5, 6, 7 and 11, 12,
+// 13 to LDS for the threads of the neighbouring tile rows; only partners
+// outside the 64 x 4 tile are read from memory.  The products are summed in
+// the order and with the values of k_spmv_sym, so the results are identical.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restrict__ S, const float* __restrict__ xu,
+                                                   const float* __restrict__ xv, float* __restrict__ yu,
+                                                   float* __restrict__ yv, const float* __restrict__ bu,
+                                                   const float* __restrict__ bv, double* __restrict__ partials) {
+    __shared__ float xs[2][3][6][WAVE + 2];
+    // ex[q][c][w][lane]: q 0..2 slots 11, 12, 13 (read by tile row w + 1),
+    // q 3..5 slots 5, 6, 7 (read by tile row w - 1), of the previous level
+    __shared__ float ex[6][4][4][WAVE];
+    const int nbx = d.nxp / WAVE;
+    const int nblk = nbx * ((d.nyo + 3) / 4);
+    const int lin = xcd_remap(blockIdx.x, nblk);
+    const int bx = lin % nbx, by = lin / nbx;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int io = bx * WAVE + lane;
+    const int jo = by * 4 + w;
+    const bool live = io < d.nxo && jo < d.nyo;
+    const int i0 = d.ilo + bx * WAVE - 1, j0 = d.jlo + by * 4 - 1;
+    auto stage = [&](int kk) {
+        const int slot = (kk + 1) % 3;
+        for (int e = threadIdx.x; e < 6 * (WAVE + 2); e += 256) {
+            const int r = e / (WAVE + 2), c = e % (WAVE + 2);
+            const int vi = i0 + c, vj = j0 + r;
+            float a = 0.f, b = 0.f;
+            if (vi >= 1 && vi <= d.nvx && vj >= 1 && vj <= d.nvy) {
+                const long long n = vidx(d, kk, vi, vj);
+                a = xu[n];
+                b = xv[n];
+            }
+            xs[0][slot][r][c] = a;
+            xs[1][slot][r][c] = b;
+        }
+    };
+    double acc = 0.0;
+    int po[9];
+    unsigned inl = 0;   // bit h: partner h is the tile's own neighbouring thread
+    const int LS = d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);
+    if (live) {
+        const int i = d.ilo + io, j = d.jlo + jo;
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            const int iA = h % 3 - 1, jA = h / 3 - 1;
+            const int pc = partner_col(d, i + iA, j + jA);
+            po[h] = pc < 0 ? -1 : (pc / WAVE) * (NSLOT * 4 * WAVE) + (pc % WAVE) * 4;
+            const int li = lane + iA, wj = w + jA;
+            if (pc >= 0 && li >= 0 && li < WAVE && wj >= 0 && wj < 4 && pc == (jo + jA) * d.nxp + (io + iA))
+                inl |= 1u << h;
+        }
+    }
+    float p8[4] = {0.f, 0.f, 0.f, 0.f}, p9[4] = {0.f, 0.f, 0.f, 0.f}, p10[4] = {0.f, 0.f, 0.f, 0.f};
+    stage(-1);
+    stage(0);
+    for (int k = 0; k < d.nz; ++k) {
+        stage(k + 1);
+        __syncthreads();   // x of levels k-1..k+1 staged; ex holds level k-1
+        float l10[4], r8[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            l10[c] = __shfl_up(p10[c], 1, WAVE);     // slot 10 of lane - 1
+            r8[c] = __shfl_down(p8[c], 1, WAVE);     // slot 8 of lane + 1
+        }
+        float su = 0.f, sv = 0.f;
+        const float* Sk = S + (long long)k * LS;
+        const float* Sb = Sk - LS;
+        // this level's own slots and same-level partner slots, loaded before
+        // the level k-1 part so that they are in flight across the barrier
+        float own[NSLOT][4];
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < NSLOT; ++q) ld_slot(Sk + po[4] + q * (4 * WAVE), own[q][0], own[q][1], own[q][2], own[q][3]);
+        }
+        if (live && k > 0) {
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {   // level k-1: transposed slots 13 - m of the partners
+                if (po[m] < 0) continue;
+                float a, b, c, e;
+                if (inl & (1u << m)) {
+                    const int li = lane + m % 3 - 1;
+                    if (m == 4) {
+                        a = p9[0]; b = p9[1]; c = p9[2]; e = p9[3];
+                    } else if (m == 3) {
+                        a = l10[0]; b = l10[1]; c = l10[2]; e = l10[3];
+                    } else if (m == 5) {
+                        a = r8[0]; b = r8[1]; c = r8[2]; e = r8[3];
+                    } else if (m < 3) {     // row j-1 (wave w-1), slot 13 - m = 11 + (2 - m)
+                        const int q = 2 - m;
+                        a = ex[q][0][w - 1][li]; b = ex[q][1][w - 1][li];
+                        c = ex[q][2][w - 1][li]; e = ex[q][3][w - 1][li];
+                    } else {                // row j+1 (wave w+1), slot 13 - m = 5 + (8 - m)
+                        const int q = 3 + (8 - m);
+                        a = ex[q][0][w + 1][li]; b = ex[q][1][w + 1][li];
+                        c = ex[q][2][w + 1][li]; e = ex[q][3][w + 1][li];
+                    }
+                } else {
+                    ld_slot(Sb + po[m] + (13 - m) * (4 * WAVE), a, b, c, e);
+                }
+                const float cuu = a, cuv = c, cvu = b, cvv = e;     // transposed 2x2 block
+                const int slot = k % 3;                              // level k-1
+                const int r = w + 1 + ((m / 3) % 3 - 1), cc = lane + 1 + (m % 3 - 1);
+                const float xun = xs[0][slot][r][cc], xvn = xs[1][slot][r][cc];
+                su = su + cuu * xun + cuv * xvn;
+                sv = sv + cvu * xun + cvv * xvn;
+            }
+        }
+        __syncthreads();   // every read of ex (level k-1) is done
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    ex[q][c][w][lane] = own[11 + q][c];
+                    ex[3 + q][c][w][lane] = own[5 + q][c];
+                }
+                p8[c] = own[8][c];
+                p9[c] = own[9][c];
+                p10[c] = own[10][c];
+            }
+#pragma unroll
+            for (int m = 9; m < 27; ++m) {
+                float cuu, cuv, cvu, cvv;
+                bool have = true;
+                if (m < 13) {   // same level: transposed slots of the partners
+                    const int h = m % 9;
+                    have = po[h] >= 0;
+                    cuu = cuv = cvu = cvv = 0.f;
+                    if (have) {
+                        float a, b, c, e;
+                        ld_slot(Sk + po[h] + (13 - m) * (4 * WAVE), a, b, c, e);
+                        cuu = a; cuv = c; cvu = b; cvv = e;
+                    }
+                } else if (m == 13) {
+                    cuu = own[0][0]; cuv = own[0][1]; cvu = own[0][1]; cvv = own[0][2];
+                } else {
+                    cuu = own[m - 13][0]; cuv = own[m - 13][1]; cvu = own[m - 13][2]; cvv = own[m - 13][3];
+                }
+                if (have) {
+                    const int slot = (k + m / 9) % 3;
+                    const int r = w + 1 + ((m / 3) % 3 - 1), c = lane + 1 + (m % 3 - 1);
+                    const float xun = xs[0][slot][r][c], xvn = xs[1][slot][r][c];
+                    su = su + cuu * xun + cuv * xvn;
+                    sv = sv + cvu * xun + cvv * xvn;
+                }
+            }
+            const long long ck = vidx(d, k, d.ilo + io, d.jlo + jo);
+            if (MODE == 2) {
+                const float ru = bu[ck] - su, rv = bv[ck] - sv;
+                yu[ck] = ru;
+                yv[ck] = rv;
+                acc += (double)ru * (double)ru + (double)rv * (double)rv;
+            } else {
+                yu[ck] = su;
+                yv[ck] = sv;
+                if (MODE == 1) {
+                    const int slot = (k + 1) % 3;
+                    acc += (double)su * (double)xs[0][slot][w + 1][lane + 1] +
+                           (double)sv * (double)xs[1][slot][w + 1][lane + 1];
+                }
+            }
+        }
+        __syncthreads();   // ex holds level k; the x slot of level k-1 is restaged next
     }
     if (MODE != 0) {
         dd v[1] = {{acc, 0.0}};

@@ -172,6 +172,26 @@ def test_half_storage_spmv_gives_identical_solves(require_gpu, name):
     assert byf == nodes * (108 * 4 + 16) and byh == nodes * (56 * 4 + 16)
 
 
+@pytest.mark.parametrize("name", ["stream40", "dome", "random_bc0", "random_bc3"])
+def test_half_storage_exchange_kernel_identical(require_gpu, name):
+    """The half-storage SpMV that passes the partner slots from level to level
+    through LDS and lane shuffles gives the same solve, bit for bit, as the one
+    that re-reads them from memory (CISMGPU_SPMV_SYMX=0)."""
+    import os
+    g, act, A, bu, bv = SYSTEMS[name]
+    out = []
+    for flag in ("0", "1"):
+        os.environ["CISMGPU_SPMV_SYMX"] = flag
+        try:
+            out.append(_mixed_run(g, act, A, bu, bv, pcg.MIXED_FP32, full=False))
+        finally:
+            os.environ.pop("CISMGPU_SPMV_SYMX", None)
+    (r0, x0, _), (r1, x1, _) = out
+    assert r0["spmv_kind"] == 3 and r1["spmv_kind"] == 3
+    assert r1["niters"] == r0["niters"] and r1["outer_iters"] == r0["outer_iters"]
+    assert np.array_equal(x1, x0)
+
+
 def test_half_storage_needs_exact_symmetry(require_gpu):
     """A coupling whose transposed partner differs after rounding keeps the
     full quad storage (spmv_kind 1)."""
