@@ -219,17 +219,14 @@ def test_stream_solution_matches_sparse_solve():
     assert np.linalg.norm(to_owned(g, xu, xv) - xs) <= 1e-9 * np.linalg.norm(xs)
 
 
-def test_tridiagonal_preconditioner_first_step():
-    """precond 3: z = SIA^-1 r + (column-broadcast of the line solves of the
-    vertically summed system).  The first standard-PCG iterate is
-    x1 = alpha z0 with alpha = (b,z0)/(z0,A z0), z0 = PC(b)."""
-    g, act, A, bu, bv = SPD[BC_NO_ICE]
-    M, uid = owned_operator(g, act, A)
-    b = to_owned(g, bu, bv)
+def line_preconditioner(g, act, A):
+    """precond 3 as an operator built here from the matrix: z = SIA^-1 r +
+    the column broadcast of the line solves of the vertically summed system
+    (u along i for each j, v along j for each i; inactive or zero-pivot
+    points solve the identity)."""
+    sia = sia_block_inverse(g, act, A)
     ni, nj, nz = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1, g.nz
-    z = sia_block_inverse(g, act, A)(b)
-    B = b.reshape(2, nz, nj, ni).sum(axis=1)
-    Z = z.reshape(2, nz, nj, ni)
+    lines = []
     for c, blk, m_sub, m_sup in ((0, 0, 4, 6), (1, 3, 2, 8)):
         vs = [A[blk][[m - 1, m + 8, m + 17]].sum(axis=(0, 1)) for m in (m_sub, 5, m_sup)]
         sub, dia, sup = [v[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi] for v in vs]
@@ -237,20 +234,57 @@ def test_tridiagonal_preconditioner_first_step():
         sub = np.where(a, sub, 0.0); sup = np.where(a, sup, 0.0); dia = np.where(a, dia, 1.0)
         bad = dia == 0.0
         dia = np.where(bad, 1.0, dia); sub = np.where(bad, 0.0, sub); sup = np.where(bad, 0.0, sup)
-        if c == 0:   # u: lines along i for each j
-            for jo in range(nj):
-                T = np.diag(dia[:, jo]) + np.diag(sub[1:, jo], -1) + np.diag(sup[:-1, jo], 1)
-                Z[c, :, jo, :] += np.linalg.solve(T, B[c, jo, :])[None, :]
-        else:        # v: lines along j for each i
-            for io in range(ni):
-                T = np.diag(dia[io, :]) + np.diag(sub[io, 1:], -1) + np.diag(sup[io, :-1], 1)
-                Z[c, :, :, io] += np.linalg.solve(T, B[c, :, io])[None, :]
-    z = Z.ravel()
+        if c == 0:
+            lines.append([np.linalg.inv(np.diag(dia[:, jo]) + np.diag(sub[1:, jo], -1) + np.diag(sup[:-1, jo], 1))
+                          for jo in range(nj)])
+        else:
+            lines.append([np.linalg.inv(np.diag(dia[io, :]) + np.diag(sub[io, 1:], -1) + np.diag(sup[io, :-1], 1))
+                          for io in range(ni)])
+
+    def apply(r):
+        Z = sia(r).reshape(2, nz, nj, ni).copy()
+        B = r.reshape(2, nz, nj, ni).sum(axis=1)
+        for jo in range(nj):
+            Z[0, :, jo, :] += (lines[0][jo] @ B[0, jo, :])[None, :]
+        for io in range(ni):
+            Z[1, :, :, io] += (lines[1][io] @ B[1, :, io])[None, :]
+        return Z.ravel()
+    return apply
+
+
+def test_tridiagonal_preconditioner_first_step():
+    """precond 3: z = SIA^-1 r + (column-broadcast of the line solves of the
+    vertically summed system).  The first standard-PCG iterate is
+    x1 = alpha z0 with alpha = (b,z0)/(z0,A z0), z0 = PC(b)."""
+    g, act, A, bu, bv = SPD[BC_NO_ICE]
+    M, uid = owned_operator(g, act, A)
+    b = to_owned(g, bu, bv)
+    z = line_preconditioner(g, act, A)(b)
     alpha = (b @ z) / (z @ (M @ z))
     for precond in (3, 4):
         xu, xv = g.zeros_vec(), g.zeros_vec()
         po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo="standard", precond=precond, tolerance=0.0, maxiters=1)
         assert np.linalg.norm(to_owned(g, xu, xv) - alpha * z) <= 1e-11 * np.linalg.norm(alpha * z)
+
+
+@pytest.mark.parametrize("algo", ["standard", "chrongear"])
+@pytest.mark.parametrize("bc", [BC_NO_ICE, BC_OUTFLOW, BC_PERIODIC])
+def test_tridiagonal_preconditioner_iterates_equal_textbook_pcg(bc, algo):
+    """precond 3 and 4 (one task: the same local line solves) over many
+    iterations: the restated solvers' k-th iterates equal a textbook PCG
+    whose preconditioner is the line-solve operator built above."""
+    g, act, A, bu, bv = SPD[bc]
+    M, _ = owned_operator(g, act, A)
+    b = to_owned(g, bu, bv)
+    ref = textbook_pcg(M, b, line_preconditioner(g, act, A), 12)
+    for precond in (3, 4):
+        for k in (2, 5, 8, 12):
+            xu, xv = g.zeros_vec(), g.zeros_vec()
+            st, err, nit = po.pcg_solve(g, act, *A, bu, bv, xu, xv, algo=algo, precond=precond, tolerance=0.0,
+                                        maxiters=k)
+            assert st == 0 and nit == k
+            x = to_owned(g, xu, xv)
+            assert np.linalg.norm(x - ref[k - 1]) <= 1e-9 * np.linalg.norm(ref[k - 1]), (precond, k)
 
 
 def test_nan_and_nonconvergence_semantics():
