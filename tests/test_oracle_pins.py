@@ -1092,3 +1092,123 @@ def test_l1l2_slab_sia_profile_and_sliding():
     e2, b2 = _l1l2_slab(21)
     assert b1 < 1e-6 and b2 < 1e-6
     assert e2 < 2e-3 and 3.5 < e1 / e2 < 4.5
+
+
+# ------------------------------------------------- surface gradient margin rules
+def _margin_problem():
+    """Ice with a planar surface s = s0 + a x + b y, cut by an ice-free stripe
+    of three cell columns: land (bed 50 m above sea level) in the south half,
+    ocean (bed 500 m below) in the north half.  The ice surface is higher
+    than the stripe everywhere."""
+    p = synthetic.ice_stream(24, nz=5, seed=1)
+    g = p.grid
+    p.dx = p.dy = 2000.0
+    a, b, s0 = -4e-3, 1e-3, 1400.0
+    xc = (np.arange(g.nx) + 0.5) * p.dx
+    yc = (np.arange(g.ny) + 0.5) * p.dy
+    s = s0 + a * xc[:, None] + b * yc[None, :]
+    topg = np.full((g.nx, g.ny), -200.0)
+    thck = s - topg
+    c1, c2 = 12, 15                         # 0-based stripe columns c1..c2-1
+    mid = g.ny // 2
+    topg[c1:c2, :mid] = 50.0
+    topg[c1:c2, mid:] = -500.0
+    thck[c1:c2, :] = 0.0
+    p.topg, p.thck = np.asfortranarray(topg), np.asfortranarray(thck)
+    p.eus, p.thck_gradient_ramp, p.max_slope = 0.0, 0.0, 10.0
+    return p, a, b, (c1, c2, mid)
+
+
+@pytest.mark.parametrize("margin", [0, 1, 2])
+def test_surface_gradient_margin_rules(margin):
+    """glissade_surface_elevation_gradient (glissade_grid_operators.F90:834-1260),
+    centred: every vertex surrounded by ice has the plane's slope (a, b)
+    under every margin rule; at the stripe, an x-edge between ice and an
+    ice-free cell counts for margin 0 (all edges), never for margin 2 (ice on
+    both sides), and for margin 1 only when the lower cell is land -- so the
+    vertex value is the mean of the counted edges' differences, or zero."""
+    p, a, b, (c1, c2, mid) = _margin_problem()
+    p.gradient_margin = margin
+    geo = po.geometry(p)
+    g = p.grid
+    usrf, dsdx, dsdy = geo["usrf"], geo["dusrf_dx"], geo["dusrf_dy"]
+    ice = geo["ice_mask"] == 1
+    land = geo["land_mask"] == 1
+    assert not ice[c1:c2].any() and ice[c1 - 1, 4] and ice[c2, 4]
+    checked = 0
+    for i in range(3, g.nx - 4):           # 0-based vertex (i, j): NE corner of cell (i, j)
+        for j in range(3, g.ny - 4):
+            cells = [(i, j), (i + 1, j), (i, j + 1), (i + 1, j + 1)]
+            if all(ice[c] for c in cells):
+                assert dsdx[i, j] == pytest.approx(a, rel=1e-9) and dsdy[i, j] == pytest.approx(b, rel=1e-9)
+                checked += 1
+                continue
+            # x-edges south (j) and north (j+1) of the vertex
+            vals = []
+            for jj in (j, j + 1):
+                lo, hi = (i, jj), (i + 1, jj)
+                up, dn = (lo, hi) if usrf[lo] > usrf[hi] else (hi, lo)
+                if margin == 0:
+                    keep = True
+                elif margin == 2:
+                    keep = ice[lo] and ice[hi]
+                else:
+                    keep = (ice[lo] and ice[hi]) or (ice[up] and land[dn])
+                vals.append((usrf[hi] - usrf[lo]) / p.dx if keep else 0.0)
+            assert dsdx[i, j] == pytest.approx(0.5 * (vals[0] + vals[1]), rel=1e-12, abs=1e-15), (i, j)
+            checked += 1
+    assert checked > 200
+    # the stripe is seen where it should be: hybrid keeps ice->land edges only
+    i = c1 - 1
+    if margin == 1:
+        assert dsdx[i, 4] < -0.3 and dsdx[i, g.ny - 6] == 0.0
+    if margin == 2:
+        assert dsdx[i, 4] == 0.0 and dsdx[i, g.ny - 6] == 0.0
+
+
+def test_symmetry_fix_averages_small_and_keeps_large_asymmetries():
+    """check_symmetry_assembled_matrix_3d (glissade_velo_higher.F90:10854-11117):
+    a coupling pair that differs by less than 1e-11 of the row's diagonal is
+    replaced by its mean on both sides; a larger difference is reported and
+    left; every other entry is untouched."""
+    p = synthetic.ice_stream(16, nz=5, seed=2)
+    s = po.assemble(p)
+    g = p.grid
+    A = [s[k].copy(order="F") for k in ("Auu", "Auv", "Avu", "Avv")]
+    orig = [a.copy(order="F") for a in A]
+    av = s["active_vertex"]
+    rng = np.random.default_rng(5)
+    planted = []
+    while len(planted) < 12:
+        i, j, k = (int(rng.integers(g.ilo + 1, g.ihi - 1)), int(rng.integers(g.jlo + 1, g.jhi - 1)),
+                   int(rng.integers(2, g.nz)))
+        m = int(rng.integers(0, 27))
+        if m == 13 or not av[i - 1, j - 1] or any(t[:4] == (i, j, k, m) for t in planted):
+            continue
+        iA, jA, kA = m % 3 - 1, (m // 3) % 3 - 1, m // 9 - 1
+        mm = 26 - m
+        if any(t[:3] == (i + iA, j + jA, k + kA) and t[3] == mm for t in planted):
+            continue
+        blk = len(planted) % 2 * 3          # Auu or Avv
+        big = len(planted) >= 9
+        dg = min(abs(orig[blk][13, k - 1, i - 1, j - 1]), abs(orig[blk][13, k + kA - 1, i + iA - 1, j + jA - 1]))
+        delta = (1e-6 if big else 1e-13) * dg
+        A[blk][m, k - 1, i - 1, j - 1] += delta
+        planted.append((i, j, k, m, blk, big, delta))
+    before = [a.copy(order="F") for a in A]
+    nwarn = po.lib().orc_check_symmetry_assembled_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, po.ip(av),
+                                                    *[po.dp(a) for a in A])
+    assert nwarn >= 3
+    touched = [np.zeros(a.shape, bool) for a in A]
+    for (i, j, k, m, blk, big, delta) in planted:
+        iA, jA, kA = m % 3 - 1, (m // 3) % 3 - 1, m // 9 - 1
+        r = (m, k - 1, i - 1, j - 1)
+        c = (26 - m, k + kA - 1, i + iA - 1, j + jA - 1)
+        v1, v2 = before[blk][r], before[blk][c]
+        if big:
+            assert A[blk][r] == v1 and A[blk][c] == v2
+        else:
+            assert A[blk][r] == A[blk][c] == 0.5 * (v1 + v2)
+        touched[blk][r] = touched[blk][c] = True
+    for a, b, t in zip(A, before, touched):
+        assert np.array_equal(a[~t], b[~t])
