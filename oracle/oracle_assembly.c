@@ -6,6 +6,13 @@
  *
  * TEST INFRASTRUCTURE ONLY (the checker for the HIP product).  Layouts as
  * in cism_oracle.h; each function cites the reference lines it follows.
+ *
+ * Pinned by tests/test_oracle_pins.py (closed forms, invariants, brute
+ * force, the reference's slab solution and EISMINT-2 files) except, parity
+ * unpinned: the vertex-mask rule of orc_grounded_fraction_no_glp (land
+ * cell (i+1, j+1) tested twice, (i+1, j) never, as the reference) and the
+ * thickness ramp branch (thck_gradient_ramp > 0) of
+ * orc_surface_elevation_gradient.
  */
 #include "cism_oracle.h"
 #include "cism_oracle_asm.h"
