@@ -17,7 +17,7 @@
 namespace cg {
 
 // Reference-element tables [n][p] (glissade_velo_higher.F90:352-388)
-__constant__ double c_dxr[8][8], c_dyr[8][8], c_dzr[8][8];
+__constant__ double c_dxr[8][8], c_dyr[8][8], c_dzr[8][8], c_phi3[8][8];
 
 // node offsets from node 7 of an element (ishift/jshift/kshift(7,:), :461-494)
 __host__ __device__ constexpr int nd_i(int n) { return (n == 0 || n == 3 || n == 4 || n == 7) ? -1 : 0; }
@@ -28,6 +28,8 @@ struct AsmDev {
     const double *xV, *yV, *susrf, *sthck, *sigma, *swbs, *flwafact, *dsdx, *dsdy, *beta, *theta;
     const double* thck_cell;     // (nx, ny) ice thickness, when given
     const double* flwa;          // (nz-1, nx, ny) rate factor, when given
+    const double* slsrf;         // (nvx, nvy) staggered lower surface (standard basal option)
+    int beta_std, taud_std;      // quadrature (standard) options of the basal term and the load
     const int32_t *acell, *avert, *umask, *vmask;
     double* efvs;
     double dx, dy, efvs_constant, effstrain_min, n_glen;
@@ -246,6 +248,59 @@ __global__ void k_load_gravity(Dims d, AsmDev in, double* __restrict__ bu, doubl
             lu = lu - rhoi * grav * in.dx * in.dy * dz / vol0 * in.dsdx[t];
             lv = lv - rhoi * grav * in.dx * in.dy * dz / vol0 * in.dsdy[t];
         }
+        bu[c] = lu;
+        bv[c] = lv;
+    }
+}
+
+// load_vector_gravity, standard option (glissade_velo_higher.F90:4350-4446),
+// gathered per node: node (k, i, j) takes the terms of the elements of the
+// active cells (i, j), (i+1, j), (i, j+1), (i+1, j+1) -- the order in which
+// the reference visits them -- layer k-1 before layer k, points in order:
+// load -= rhoi g w detJ / vol0 * ds/dx_qp * phi_n.  Every node is written.
+__global__ void k_load_gravity_std(Dims d, AsmDev in, double* __restrict__ bu, double* __restrict__ bv,
+                                   double* __restrict__ flag) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nvx * d.nvy) return;
+    const int i = 1 + (int)(t % d.nvx), j = 1 + (int)(t / d.nvx);
+    const double rhoi = 910.0, grav = 9.81, vol0 = 1.0e9;
+    for (int kn = 1; kn <= d.nz; ++kn) {
+        double lu = 0.0, lv = 0.0;
+        for (int cc = 0; cc < 4; ++cc) {
+            const int ci = i + (cc & 1), cj = j + (cc >> 1);
+            if (ci > d.nx || cj > d.ny || !cell_in_range(d, in.nhalo, ci, cj) || !in.acell[c2(d, ci, cj)]) continue;
+            for (int ke = kn - 1; ke <= kn; ++ke) {
+                if (ke < 1 || ke > d.nz - 1) continue;
+                int nl = 0;
+#pragma unroll
+                for (int n = 0; n < 8; ++n)
+                    if (nd_i(n) == i - ci && nd_j(n) == j - cj && nd_k(n) == kn - ke) nl = n;
+                Elem e;
+                double sx[8], sy[8];
+#pragma unroll
+                for (int n = 0; n < 8; ++n) {
+                    const long long q = v2(d, ci + nd_i(n), cj + nd_j(n));
+                    e.x[n] = in.xV[q];
+                    e.y[n] = in.yV[q];
+                    e.z[n] = in.susrf[q] - in.sigma[ke + nd_k(n) - 1] * in.sthck[q];
+                    sx[n] = in.dsdx[q];
+                    sy[n] = in.dsdy[q];
+                }
+                for (int p = 0; p < 8; ++p) {
+                    double dxn[8], dyn[8], dzn[8], detJ;
+                    if (!basis_derivs(e, p, dxn, dyn, dzn, detJ)) { flag[0] = 1.0; return; }
+                    double gx = 0.0, gy = 0.0;
+#pragma unroll
+                    for (int n = 0; n < 8; ++n) {
+                        gx = gx + c_phi3[n][p] * sx[n];
+                        gy = gy + c_phi3[n][p] * sy[n];
+                    }
+                    lu = lu - rhoi * grav * 1.0 * detJ / vol0 * gx * c_phi3[nl][p];
+                    lv = lv - rhoi * grav * 1.0 * detJ / vol0 * gy * c_phi3[nl][p];
+                }
+            }
+        }
+        const long long c = vidx(d, kn - 1, i, j);
         bu[c] = lu;
         bv[c] = lv;
     }

@@ -824,4 +824,68 @@ __global__ void k_l1l2_velocity(Dims d, AsmDev in, Lev L, const double* __restri
     }
 }
 
+// basal_sliding_bc_3d (glissade_velo_higher.F90:8955-9111) and
+// basal_sliding_bc_2d (:8386-8527), standard option, gathered per owned row
+// of level `level`: the cells (i, j), (i+1, j), (i, j+1), (i+1, j+1) in the
+// reference's order, points p in order, K(r,c) = beta_qp w detJ / vol0
+// phi_r phi_c added to the kA = 0 coupling of column node c.  curved: detJ
+// is the area factor of the face (x, y, staggered lower surface)
+// (get_area_scale_factor_curved_2d, :6736-6827), else the flat Jacobian.
+__global__ void k_basal_std(Dims d, AsmDev in, int level, int curved, double* __restrict__ A,
+                            double* __restrict__ flag) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)d.nxo * d.nyo) return;
+    const int io = (int)(t % d.nxo), jo = (int)(t / d.nxo);
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const long long r = ((long long)level * d.nyo + jo) * d.nxp + io;
+    double* a = A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+    const double vol0 = 1.0e9;
+    for (int cc = 0; cc < 4; ++cc) {
+        const int ci = i + (cc & 1), cj = j + (cc >> 1);
+        if (ci > d.nx || cj > d.ny || !cell_in_range(d, in.nhalo, ci, cj) || !in.acell[c2(d, ci, cj)]) continue;
+        int nr = 0;
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+            if (nd2_i(n) == i - ci && nd2_j(n) == j - cj) nr = n;
+        Elem2 e;
+        double z[4], b[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const long long q = v2(d, ci + nd2_i(n), cj + nd2_j(n));
+            e.x[n] = in.xV[q];
+            e.y[n] = in.yV[q];
+            z[n] = curved ? in.slsrf[q] : 0.0;
+            b[n] = in.beta[q];
+        }
+        for (int p = 0; p < 4; ++p) {
+            double detJ;
+            if (curved) {
+                double J11 = 0, J12 = 0, J13 = 0, J21 = 0, J22 = 0, J23 = 0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    J11 = J11 + c_dxr2[n][p] * e.x[n]; J12 = J12 + c_dxr2[n][p] * e.y[n]; J13 = J13 + c_dxr2[n][p] * z[n];
+                    J21 = J21 + c_dyr2[n][p] * e.x[n]; J22 = J22 + c_dyr2[n][p] * e.y[n]; J23 = J23 + c_dyr2[n][p] * z[n];
+                }
+                const double t1 = (J11 * J22 - J12 * J21);
+                const double t2 = -(J11 * J23 - J13 * J21);
+                const double t3 = (J12 * J23 - J13 * J22);
+                detJ = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+            } else {
+                double dxn[4], dyn[4];
+                if (!basis_derivs_2d(e, p, dxn, dyn, detJ)) { flag[0] = 1.0; return; }
+            }
+            double bq = 0.0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) bq = bq + c_phi2[n][p] * b[n];
+#pragma unroll
+            for (int nc = 0; nc < 4; ++nc) {
+                const double K = 0.0 + bq * 1.0 * detJ / vol0 * c_phi2[nr][p] * c_phi2[nc][p];
+                const int m = (nd2_i(nc) - nd2_i(nr) + 1) + 3 * (nd2_j(nc) - nd2_j(nr) + 1) + 9;
+                a[(0 * 27 + m) * WAVE] = a[(0 * 27 + m) * WAVE] + K;
+                a[(3 * 27 + m) * WAVE] = a[(3 * 27 + m) * WAVE] + K;
+            }
+        }
+    }
+}
+
 }  // namespace cg

@@ -1165,6 +1165,11 @@ static int upload_fe_tables() {
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dxr), dxr, sizeof(dxr)));
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dyr), dyr, sizeof(dyr)));
     CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dzr), dzr, sizeof(dzr)));
+    double phi3[8][8];
+    for (int p = 0; p < 8; ++p)
+        for (int n = 0; n < 8; ++n)
+            phi3[n][p] = (1.0 + sx[n] * xq[p]) * (1.0 + sy[n] * yq[p]) * (1.0 + sz[n] * zq[p]) / 8.0;
+    CG_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_phi3), phi3, sizeof(phi3)));
     // 2-D element (glissade_velo_higher.F90:528-570): points (-r,-r), (r,-r), (r,r), (-r,r)
     const double xq2[4] = {-r, r, r, -r}, yq2[4] = {-r, -r, r, r};
     double phi2[4][4], dxr2[4][4], dyr2[4][4];
@@ -1204,7 +1209,7 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
         return CISMGPU_ERR_ARG;
     }
     const size_t nd = (size_t)(d.nz + (d.nz + 1) + 8 * n2 + nfl + (lateral ? nc + 2 * n2 : 0) + (in->thck ? nc : 0) +
-                             (in->flwa ? nfl : 0)),
+                             (in->flwa ? nfl : 0) + (in->stag_lsrf ? n2 : 0)),
                  ni = (size_t)(nc + n2 + 2 * nvec + (lateral ? nc : 0));
     const size_t bytes = nd * sizeof(double) + ni * sizeof(int32_t);
     if (S->asm_bytes < bytes) {
@@ -1237,6 +1242,9 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     a.flwafact = putd(in->flwafact, nfl, true);
     a.thck_cell = in->thck ? putd(in->thck, nc, true) : nullptr;
     a.flwa = in->flwa ? putd(in->flwa, nfl, true) : nullptr;
+    a.slsrf = in->stag_lsrf ? putd(in->stag_lsrf, n2, true) : nullptr;
+    a.beta_std = in->assemble_beta_standard ? 1 : 0;
+    a.taud_std = in->assemble_taud_standard ? 1 : 0;
     if (lateral) {
         lat.usrf = putd(in->usrf, nc, true);
         lat.thck = a.thck_cell;
@@ -1278,6 +1286,10 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         return CISMGPU_ERR_ARG;
     }
     if (S->two_d) { cismgpu_set_error("assemble_3d needs a 3-D system"); return CISMGPU_ERR_ARG; }
+    if (in->assemble_beta_standard && in->beta_internal && !in->stag_lsrf) {
+        cismgpu_set_error("assemble_3d: the standard basal option needs stag_lsrf");
+        return CISMGPU_ERR_ARG;
+    }
     int st = upload_fe_tables();
     if (st) return st;
     const Dims& d = S->d;
@@ -1297,8 +1309,16 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
     hipEventRecord(e1, S->s);
     k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
-    if (in->beta_internal) k_basal_local<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
-    k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->V(2), S->V(3));
+    if (in->beta_internal) {
+        if (a.beta_std)
+            k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, d.nz - 1, 1, S->A64, S->scal + S_NAN);
+        else
+            k_basal_local<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+    }
+    if (a.taud_std)
+        k_load_gravity_std<<<nblocks(n2, 64), 64, 0, S->s>>>(d, a, S->V(2), S->V(3), S->scal + S_NAN);
+    else
+        k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->V(2), S->V(3));
     if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d, a, lat, S->V(2), S->V(3));
     k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
@@ -1369,6 +1389,10 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         cismgpu_set_error("assemble_2d: L1L2 needs the rate factor (in->flwa) and n = 1 or 3 for the nonlinear viscosity");
         return CISMGPU_ERR_ARG;
     }
+    if (diva && in->assemble_beta_standard) {
+        cismgpu_set_error("assemble_2d: the standard basal option is not built for DIVA");
+        return CISMGPU_ERR_UNSUPPORTED;
+    }
     if (diva && (!S->diva_ready || !in->thck)) {
         cismgpu_set_error("assemble_2d: DIVA needs cismgpu_system_diva_setup and the cell thickness (in->thck)");
         return CISMGPU_ERR_ARG;
@@ -1418,12 +1442,20 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
     } else {
         if (!l1l2) k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
-        if (in->beta_internal) k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+        if (in->beta_internal) {
+            if (a.beta_std)
+                k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, 0, 0, S->A64, S->scal + S_NAN);
+            else
+                k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
+        }
     }
     // b = level sum of the 3-D load (gravity, then the lateral BC)
     double *lu = S->load3, *lv = S->load3 + d3.vlen;
     CG_CHECK(hipMemsetAsync(S->load3, 0, 2 * d3.vlen * sizeof(double), S->s));
-    k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, a, lu, lv);
+    if (a.taud_std)
+        k_load_gravity_std<<<nblocks(n2, 64), 64, 0, S->s>>>(d3, a, lu, lv, S->scal + S_NAN);
+    else
+        k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, a, lu, lv);
     if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d3, a, lat, lu, lv);
     k_sum_levels<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, lu, lv, d, S->V(2), S->V(3));
     k_dirichlet_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->V(2), S->V(3));

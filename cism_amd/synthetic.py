@@ -48,6 +48,8 @@ class Problem:
     whichefvs: int = 2
     efvs_constant: float = 0.0
     assemble_lateral: int = 0    # which_ho_assemble_lateral (glide_types.F90:1067): 0 standard, 1 local
+    assemble_beta: int = 1       # which_ho_assemble_beta (glide_types.F90:359-360, 1050): 0 standard, 1 local
+    assemble_taud: int = 1       # which_ho_assemble_taud (glide_types.F90:362-363, 1042): 0 standard, 1 local
     meta: dict = field(default_factory=dict)
 
     def cell_centres(self):

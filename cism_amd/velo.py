@@ -33,7 +33,8 @@ class AssemblyIn(ctypes.Structure):
                 ("effstrain_min", ctypes.c_double), ("n_glen", ctypes.c_double),
                 ("check_symmetry", ctypes.c_int32),
                 ("ocean_mask", _P), ("usrf", _P), ("thck", _P), ("stagusrf_lateral", _P), ("stagthck_lateral", _P),
-                ("which_assemble_lateral", ctypes.c_int32), ("flwa", _P)]
+                ("which_assemble_lateral", ctypes.c_int32), ("flwa", _P), ("stag_lsrf", _P),
+                ("assemble_beta_standard", ctypes.c_int32), ("assemble_taud_standard", ctypes.c_int32)]
 
 
 class DivaIn(ctypes.Structure):
@@ -157,6 +158,12 @@ class BPSolver:
             p.n_glen, 1,
             self.ocean_mask.ctypes.data if has_front else None, self.usrf.ctypes.data, self.thck.ctypes.data,
             self.stagusrf_lat.ctypes.data, self.stagthck_lat.ctypes.data, int(p.assemble_lateral))
+        # quadrature (standard) options of the basal term and the gravity load
+        # (which_ho_assemble_beta / _taud = 0; glissade_velo_higher.F90:8955-9111, 4350-4446)
+        self.stag_lsrf = ge.stagger(self.lsrf)
+        self._ain.stag_lsrf = self.stag_lsrf.ctypes.data
+        self._ain.assemble_beta_standard = 1 if getattr(p, "assemble_beta", 1) == 0 else 0
+        self._ain.assemble_taud_standard = 1 if getattr(p, "assemble_taud", 1) == 0 else 0
 
     def _initial_velocity(self):
         u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))

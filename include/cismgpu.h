@@ -251,6 +251,12 @@ typedef struct cismgpu_assembly_in {
     const double*  stagthck_lateral; /*   land cells (glissade_velo_higher.F90:2006-2016)     */
     int32_t which_assemble_lateral;  /* 0 standard (staggered values), 1 local (cell values) */
     const double*  flwa;             /* (nz-1, nx, ny) rate factor A (Pa^-n yr^-1): the L1L2 viscosity */
+    const double*  stag_lsrf;        /* (nx-1, ny-1) lower surface on the staggered grid: the curved basal
+                                        Jacobian of the standard basal option (3-D) */
+    int32_t assemble_beta_standard;  /* 1: which_ho_assemble_beta = 0, quadrature of the basal term
+                                        (glissade_velo_higher.F90:8955-9111; 2-D :8386-8527); 0: local (default) */
+    int32_t assemble_taud_standard;  /* 1: which_ho_assemble_taud = 0, quadrature of the gravity load
+                                        (:4350-4446); 0: local (default) */
 } cismgpu_assembly_in;
 
 typedef struct cismgpu_assembly_info {

@@ -122,6 +122,13 @@ int orc_compute_3d_velocity_l1l2(int nx, int ny, int nz, int ilo, int ihi, int j
                                  const double* thck, const double* stagthck, const double* dusrf_dx,
                                  const double* dusrf_dy, const double* flwa, const double* efvs, int whichefvs,
                                  double n_glen, double* uvel, double* vvel);
+int orc_load_vector_gravity_standard(int nx, int ny, int nz, int nhalo, const double* sigma,
+                                     const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                     const double* stagusrf, const double* stagthck, const double* dusrf_dx,
+                                     const double* dusrf_dy, double* loadu, double* loadv);
+int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, int curved,
+                                  const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                  const double* staglsrf, const double* beta, double* Auu, double* Avv);
 
 #ifdef __cplusplus
 }

@@ -1634,3 +1634,110 @@ done:
     free(ss); free(ux); free(uy); free(vx); free(vy); free(tp); free(ei); free(w); free(g);
     return st;
 }
+
+/* ------------------------------------------------------------------ */
+/* Standard (quadrature) options of the load and the basal term        */
+/* ------------------------------------------------------------------ */
+
+/* load_vector_gravity, standard option (glissade_velo_higher.F90:4350-4446):
+ * per element of an active cell and per quadrature point, ds/dx (ds/dy)
+ * interpolated from the vertices; load(n) -= rhoi g w detJ / vol0 *
+ * ds/dx_qp * phi_n, accumulated in the loop order (cells j, i; layer k;
+ * point p; node n). */
+int orc_load_vector_gravity_standard(int nx, int ny, int nz, int nhalo, const double* sigma,
+                                     const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                     const double* stagusrf, const double* stagthck, const double* dusrf_dx,
+                                     const double* dusrf_dy, double* loadu, double* loadv)
+{
+    init_tables();
+    double x[8], y[8], z[8], sx[8], sy[8];
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                for (int n = 0; n < 8; ++n) {
+                    const size_t q = V2(i + node_di[n], j + node_dj[n]);
+                    const int kn = k + node_dk[n];
+                    x[n] = xVertex[q];
+                    y[n] = yVertex[q];
+                    z[n] = stagusrf[q] - sigma[kn - 1] * stagthck[q];
+                    sx[n] = dusrf_dx[q];
+                    sy[n] = dusrf_dy[q];
+                }
+                for (int p = 0; p < 8; ++p) {
+                    double dx[8], dy[8], dz[8], detJ;
+                    if (basis_derivs(x, y, z, p, dx, dy, dz, &detJ)) return ORC_ERR_SINGULAR;
+                    double gx = 0.0, gy = 0.0;
+                    for (int n = 0; n < 8; ++n) {
+                        gx = gx + phi3[n][p] * sx[n];
+                        gy = gy + phi3[n][p] * sy[n];
+                    }
+                    for (int n = 0; n < 8; ++n) {
+                        const size_t a = V3(k + node_dk[n], i + node_di[n], j + node_dj[n]);
+                        loadu[a] = loadu[a] - RHOI * GRAV * 1.0 * detJ / VOL0 * gx * phi3[n][p];
+                        loadv[a] = loadv[a] - RHOI * GRAV * 1.0 * detJ / VOL0 * gy * phi3[n][p];
+                    }
+                }
+            }
+        }
+    return ORC_OK;
+}
+
+/* basal_sliding_bc_3d standard option (glissade_velo_higher.F90:8955-9111)
+ * and basal_sliding_bc_2d standard option (:8386-8527): per active cell and
+ * 2x2 quadrature point, K(r,c) = beta_qp w detJ / vol0 phi_r phi_c is added
+ * to the kA = 0 couplings of the basal rows.  curved (3-D, the reference's
+ * curved_2d_basal_jacobian = .true., :8914): detJ is the area factor of
+ * the face (x, y, staglsrf) (get_area_scale_factor_curved_2d :6736-6827);
+ * otherwise the flat 2-D Jacobian.  two_d: A are (nx-1, ny-1, 9), else the
+ * level nz of (27, nz, nx-1, ny-1). */
+int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, int curved,
+                                  const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                  const double* staglsrf, const double* beta, double* Auu, double* Avv)
+{
+    init_tables_2d();
+    double x[4], y[4], z[4], b[4];
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int n = 0; n < 4; ++n) {
+                const size_t q = V2(i + di2[n], j + dj2[n]);
+                x[n] = xVertex[q];
+                y[n] = yVertex[q];
+                z[n] = staglsrf ? staglsrf[q] : 0.0;
+                b[n] = beta[q];
+            }
+            for (int p = 0; p < 4; ++p) {
+                double detJ;
+                if (curved) {
+                    double J11 = 0, J12 = 0, J13 = 0, J21 = 0, J22 = 0, J23 = 0;
+                    for (int n = 0; n < 4; ++n) {
+                        J11 = J11 + dxr2[n][p] * x[n]; J12 = J12 + dxr2[n][p] * y[n]; J13 = J13 + dxr2[n][p] * z[n];
+                        J21 = J21 + dyr2[n][p] * x[n]; J22 = J22 + dyr2[n][p] * y[n]; J23 = J23 + dyr2[n][p] * z[n];
+                    }
+                    const double t1 = (J11 * J22 - J12 * J21);
+                    const double t2 = -(J11 * J23 - J13 * J21);
+                    const double t3 = (J12 * J23 - J13 * J22);
+                    detJ = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+                } else {
+                    double dx[4], dy[4];
+                    if (basis_derivs_2d(x, y, p, dx, dy, &detJ)) return ORC_ERR_SINGULAR;
+                }
+                double bq = 0.0;
+                for (int n = 0; n < 4; ++n) bq = bq + phi2[n][p] * b[n];
+                double K[4][4];
+                for (int nc = 0; nc < 4; ++nc)
+                    for (int nr = 0; nr < 4; ++nr) K[nr][nc] = 0.0 + bq * 1.0 * detJ / VOL0 * phi2[nr][p] * phi2[nc][p];
+                for (int nr = 0; nr < 4; ++nr) {
+                    const int ii = i + di2[nr], jj = j + dj2[nr];
+                    for (int nc = 0; nc < 4; ++nc) {
+                        const int iA = di2[nc] - di2[nr], jA = dj2[nc] - dj2[nr];
+                        const size_t a = two_d ? A2(IDX2(iA, jA), ii, jj) : A4(IDX3(iA, jA, 0), nz, ii, jj);
+                        Auu[a] = Auu[a] + K[nr][nc];
+                        Avv[a] = Avv[a] + K[nr][nc];
+                    }
+                }
+            }
+        }
+    return ORC_OK;
+}

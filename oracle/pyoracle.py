@@ -109,6 +109,10 @@ def _declare(L):
     L.orc_basal_sliding_bc_2d_diva_local.argtypes = [i, i, d, d, _I, _D, _D, _D, _D, _D, _D]
     L.orc_compute_3d_velocity_diva.argtypes = [i, i, i, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D,
                                                _D, _D]
+    L.orc_load_vector_gravity_standard.argtypes = [i, i, i, i, _D, _I, _D, _D, _D, _D, _D, _D, _D, _D]
+    L.orc_load_vector_gravity_standard.restype = ctypes.c_int
+    L.orc_basal_sliding_bc_standard.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D, _D, _D]
+    L.orc_basal_sliding_bc_standard.restype = ctypes.c_int
     L.orc_l1l2_tau_parallel.argtypes = [d, d, d]
     L.orc_l1l2_tau_parallel.restype = ctypes.c_double
     L.orc_assemble_stiffness_matrix_2d_l1l2.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, i, d,
@@ -329,6 +333,48 @@ def beta_internal(problem, geo):
     return b
 
 
+def stagger(var):
+    """glissade_stagger, margin 0, of a cell field (orc_stagger)."""
+    nx, ny = var.shape
+    out = np.zeros((nx - 1, ny - 1), order="F")
+    lib().orc_stagger(nx, ny, dp(np.asfortranarray(var, dtype=np.float64)), dp(out))
+    return out
+
+
+def gravity_load(p, geo, xV, yV, loadu, loadv):
+    """load_vector_gravity with which_ho_assemble_taud (p.assemble_taud: 1 local,
+    the default, :4311-4342; 0 standard, :4350-4446)."""
+    L = lib()
+    g = p.grid
+    av = geo["active_vertex"]
+    if getattr(p, "assemble_taud", 1) == 0:
+        st = L.orc_load_vector_gravity_standard(g.nx, g.ny, g.nz, 2, dp(np.ascontiguousarray(p.sigma, dtype=np.float64)),
+                                                ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["stagusrf"]),
+                                                dp(geo["stagthck"]), dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]),
+                                                dp(loadu), dp(loadv))
+        if st:
+            raise RuntimeError(f"oracle standard gravity load failed with status {st}")
+        return
+    from cism_amd.grid import stagwbndsigma
+    swb = stagwbndsigma(p.sigma)
+    L.orc_load_vector_gravity_local(g.nx, g.ny, g.nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
+                                    dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+
+
+def basal_2d(p, geo, xV, yV, av, binternal, A):
+    """basal_sliding_bc_2d with which_ho_assemble_beta (p.assemble_beta: 1 local,
+    :8373-8384; 0 standard, flat Jacobian, :8386-8527)."""
+    L = lib()
+    g = p.grid
+    if getattr(p, "assemble_beta", 1) == 0:
+        st = L.orc_basal_sliding_bc_standard(g.nx, g.ny, g.nz, 2, 1, 0, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                             None, dp(binternal), dp(A[0]), dp(A[3]))
+        if st:
+            raise RuntimeError(f"oracle standard 2-D basal assembly failed with status {st}")
+    else:
+        L.orc_basal_sliding_bc_2d_local(g.nx, g.ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
+
+
 def assemble(problem, uvel=None, vvel=None, geo=None):
     """One assembly of the linear BP system in the reference's order
     (glissade_velo_higher.F90:2815-2931): stiffness, basal BC, b = load,
@@ -350,8 +396,7 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     loadu = g.zeros_vec(); loadv = g.zeros_vec()
     from cism_amd.grid import stagwbndsigma
     swb = stagwbndsigma(p.sigma)
-    L.orc_load_vector_gravity_local(nx, ny, nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
-                                    dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+    gravity_load(p, geo, xV, yV, loadu, loadv)
     sigma_ = np.ascontiguousarray(p.sigma, dtype=np.float64)
     L.orc_load_vector_lateral_bc(nx, ny, nz, dp(sigma_), 2, int(p.assemble_lateral), ip(geo["ocean_mask"]),
                                  ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["usrf"]), dp(p.thck),
@@ -367,8 +412,14 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
         raise RuntimeError(f"oracle assembly failed with status {st}")
     binternal = beta_internal(p, geo)
     if p.whichbabc != 6:
-        L.orc_basal_sliding_bc_local(nx, ny, nz, p.dx, p.dy, ip(av), dp(binternal), dp(geo["theta_basal"]),
-                                     dp(A[0]), dp(A[3]))
+        if getattr(p, "assemble_beta", 1) == 0:
+            st = L.orc_basal_sliding_bc_standard(nx, ny, nz, 2, 0, 1, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                                 dp(stagger(geo["lsrf"])), dp(binternal), dp(A[0]), dp(A[3]))
+            if st:
+                raise RuntimeError(f"oracle standard basal assembly failed with status {st}")
+        else:
+            L.orc_basal_sliding_bc_local(nx, ny, nz, p.dx, p.dy, ip(av), dp(binternal), dp(geo["theta_basal"]),
+                                         dp(A[0]), dp(A[3]))
     bu = loadu.copy(order="F"); bv = loadv.copy(order="F")
     umask = np.zeros(g.vec_shape, np.int32, order="F")
     if p.whichbabc == 6:
@@ -470,8 +521,7 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     loadu = g.zeros_vec(); loadv = g.zeros_vec()
     from cism_amd.grid import stagwbndsigma
     swb = stagwbndsigma(p.sigma)
-    L.orc_load_vector_gravity_local(nx, ny, nz, dp(swb), p.dx, p.dy, ip(av), dp(geo["stagthck"]),
-                                    dp(geo["dusrf_dx"]), dp(geo["dusrf_dy"]), dp(loadu), dp(loadv))
+    gravity_load(p, geo, xV, yV, loadu, loadv)
     sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
     L.orc_load_vector_lateral_bc(nx, ny, nz, dp(sigma), 2, int(p.assemble_lateral), ip(geo["ocean_mask"]),
                                  ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["usrf"]), dp(p.thck),
@@ -520,7 +570,7 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
         if st:
             raise RuntimeError(f"oracle 2-D L1L2 assembly failed with status {st}")
         if p.whichbabc != 6:
-            L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
+            basal_2d(p, geo, xV, yV, av, binternal, A)
     else:
         st = L.orc_assemble_stiffness_matrix_2d_ssa(nx, ny, nz, dp(sigma), 2, ip(geo["active_cell"]), dp(xV),
                                                     dp(yV), dp(u2), dp(v2), dp(geo["stagthck"]),
@@ -529,7 +579,7 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
         if st:
             raise RuntimeError(f"oracle 2-D assembly failed with status {st}")
         if p.whichbabc != 6:
-            L.orc_basal_sliding_bc_2d_local(nx, ny, p.dx, p.dy, ip(av), dp(binternal), dp(A[0]), dp(A[3]))
+            basal_2d(p, geo, xV, yV, av, binternal, A)
     bu = np.asfortranarray(lu2.copy()); bv = np.asfortranarray(lv2.copy())
     umask = np.zeros((nx - 1, ny - 1), np.int32, order="F")
     if p.whichbabc == 6 and approx != "diva":     # DIVA: no-slip through beta_eff (:1423-1428)

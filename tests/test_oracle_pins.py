@@ -1212,3 +1212,74 @@ def test_symmetry_fix_averages_small_and_keeps_large_asymmetries():
         touched[blk][r] = touched[blk][c] = True
     for a, b, t in zip(A, before, touched):
         assert np.array_equal(a[~t], b[~t])
+
+
+# ------------------------------------------- standard (quadrature) load and basal term
+def test_standard_gravity_load_slab_and_total_force():
+    """load_vector_gravity, standard option: on the uniform slab every node
+    whose eight elements are all ice gets the local option's load (the
+    consistent load of a constant equals the lumped one on boxes); with the
+    bed lowered and raised under a planar surface (thickness varying, ds/dx
+    uniform) the loads sum to -rho_i g ds/dx V / vol0, V the ice volume of
+    the active cells (bilinear thickness, integrated exactly)."""
+    p = synthetic.slab(n_cells=8, nz=6)
+    g = p.grid
+    geo = po.geometry(p)
+    xV, yV = p.vertex_coords()
+    lu_s, lv_s, lu_l, lv_l = g.zeros_vec(), g.zeros_vec(), g.zeros_vec(), g.zeros_vec()
+    p.assemble_taud = 0
+    po.gravity_load(p, geo, xV, yV, lu_s, lv_s)
+    p.assemble_taud = 1
+    po.gravity_load(p, geo, xV, yV, lu_l, lv_l)
+    inner = (slice(None), slice(3, g.nx - 3), slice(3, g.ny - 3))
+    assert np.abs(lu_s[inner] - lu_l[inner]).max() <= 1e-12 * np.abs(lu_l[inner]).max()
+    rng = np.random.default_rng(3)
+    dh = rng.uniform(-150.0, 150.0, (g.nx, g.ny))
+    p.topg = np.asfortranarray(p.topg + dh)
+    p.thck = np.asfortranarray(p.thck - dh)
+    geo = po.geometry(p)
+    sx = geo["dusrf_dx"]
+    assert np.ptp(sx) <= 1e-12 * abs(sx).max() and np.abs(geo["dusrf_dy"]).max() <= 1e-15
+    lu, lv = g.zeros_vec(), g.zeros_vec()
+    p.assemble_taud = 0
+    po.gravity_load(p, geo, xV, yV, lu, lv)
+    H = geo["stagthck"]
+    vol = 0.0
+    for j in range(3, g.ny):          # 1-based cells nhalo+1 .. ny-nhalo+1
+        for i in range(3, g.nx):
+            if geo["active_cell"][i - 1, j - 1]:
+                vol += p.dx * p.dy * (H[i - 2, j - 2] + H[i - 1, j - 2] + H[i - 1, j - 1] + H[i - 2, j - 1]) / 4.0
+    want = -910.0 * 9.81 * sx.mean() * vol / 1e9
+    assert lu.sum() == pytest.approx(want, rel=1e-12)
+    assert np.abs(lv).max() <= 1e-12 * np.abs(lu).max()
+
+
+def _basal_rows(p, beta_opt, two_d=False):
+    p.assemble_beta = beta_opt
+    s = po.assemble_2d(p) if two_d else po.assemble(p)
+    return s
+
+
+@pytest.mark.parametrize("two_d", [False, True])
+def test_standard_basal_term_row_sums_on_planar_bed(two_d):
+    """basal_sliding_bc_3d/_2d, standard option, on the periodic slab (planar
+    inclined bed, uniform beta): the consistent friction rows sum to the
+    lumped local term -- in 3-D beta dx dy / (vol0 cos theta), the curved
+    face's area factor being exactly 1/cos theta on a plane; in 2-D
+    beta dx dy / vol0 (flat Jacobian) -- the rows above the bed are
+    unchanged and the operator stays symmetric."""
+    p = synthetic.slab(n_cells=8, nz=6)
+    g = p.grid
+    loc = _basal_rows(p, 1, two_d)
+    std = _basal_rows(p, 0, two_d)
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    for key in ("Auu", "Avv"):
+        if two_d:
+            a, b = loc[key][own[0], own[1], :], std[key][own[0], own[1], :]
+            assert np.abs(a.sum(axis=-1) - b.sum(axis=-1)).max() <= 1e-12 * np.abs(a[..., 4]).max()
+            assert np.abs(b[..., [0, 1, 2, 3, 5, 6, 7, 8]]).max() > 0.0
+        else:
+            a, b = loc[key][:, :, own[0], own[1]], std[key][:, :, own[0], own[1]]
+            assert np.abs(a[:, -1].sum(axis=0) - b[:, -1].sum(axis=0)).max() <= 1e-12 * np.abs(a[13, -1]).max()
+            assert np.array_equal(a[:, :-1], b[:, :-1])
+    assert std["nwarn_asym"] == 0
