@@ -1625,23 +1625,35 @@ extern "C" int32_t cismgpu_system_diva_velocity(cismgpu_system* S, double* uvel,
 // Saved-assembly snapshot for the accelerated Picard iteration
 // (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184):
 // the owned rows of A, b and the preconditioner coefficients derived from A.
-static size_t saved_doubles(const cismgpu_system* S) {
-    const Dims& d = S->d;
-    const size_t nraw = 27ull * d.nz * d.nvx * d.nvy;
-    return (size_t)d.nrows * NCOEF + 2 * (size_t)d.vlen + 8 * (size_t)d.vlen + (S->stage ? 2 * nraw : 0);
-}
+// Saved assembly: A, b, the preconditioner, the raw rows and, for DIVA,
+// beta_eff and omega_k (evaluate_accelerated_picard_2d saves and restores
+// them with the matrix, glissade_velo_higher.F90:9898-10048).
+constexpr int NSAVED = 7;
 
-static void saved_parts(cismgpu_system* S, double* base, double* parts[4], size_t sizes[4]) {
+static void saved_sizes(const cismgpu_system* S, size_t sizes[NSAVED]) {
     const Dims& d = S->d;
     const size_t nraw = 27ull * d.nz * d.nvx * d.nvy;
+    const size_t n2 = (size_t)d.nvx * d.nvy, nc = (size_t)d.nx * d.ny;
     sizes[0] = (size_t)d.nrows * NCOEF;
     sizes[1] = 2 * (size_t)d.vlen;
     sizes[2] = 8 * (size_t)d.vlen;
     sizes[3] = S->stage ? 2 * nraw : 0;
+    sizes[4] = S->diva_ready ? n2 : 0;
+    sizes[5] = S->diva_ready ? n2 : 0;
+    sizes[6] = S->diva_ready ? (size_t)S->nz_levels * nc : 0;
+}
+
+static void saved_parts(const cismgpu_system* S, double* base, double* parts[NSAVED], size_t sizes[NSAVED]) {
+    saved_sizes(S, sizes);
     parts[0] = base;
-    parts[1] = parts[0] + sizes[0];
-    parts[2] = parts[1] + sizes[1];
-    parts[3] = parts[2] + sizes[2];
+    for (int q = 1; q < NSAVED; ++q) parts[q] = parts[q - 1] + sizes[q - 1];
+}
+
+static size_t saved_doubles(const cismgpu_system* S) {
+    size_t sizes[NSAVED], n = 0;
+    saved_sizes(S, sizes);
+    for (int q = 0; q < NSAVED; ++q) n += sizes[q];
+    return n;
 }
 
 extern "C" int32_t cismgpu_system_save_assembly(cismgpu_system* S) {
@@ -1655,11 +1667,11 @@ extern "C" int32_t cismgpu_system_save_assembly(cismgpu_system* S) {
         CG_CHECK(hipMalloc(&S->saved, need * sizeof(double)));
         S->saved_n = need;
     }
-    double* parts[4];
-    size_t sizes[4];
+    double* parts[NSAVED];
+    size_t sizes[NSAVED];
     saved_parts(S, S->saved, parts, sizes);
-    double* src[4] = {S->A64, S->V(2), S->M64, S->stage};
-    for (int q = 0; q < 4; ++q)
+    double* src[NSAVED] = {S->A64, S->V(2), S->M64, S->stage, S->dv.bex, S->dv.bey, S->dv.omega_k};
+    for (int q = 0; q < NSAVED; ++q)
         if (sizes[q]) CG_CHECK(hipMemcpyAsync(parts[q], src[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_saved = true;
@@ -1671,11 +1683,11 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
         cismgpu_set_error("restore_assembly: nothing saved");
         return CISMGPU_ERR_ARG;
     }
-    double* parts[4];
-    size_t sizes[4];
+    double* parts[NSAVED];
+    size_t sizes[NSAVED];
     saved_parts(S, S->saved, parts, sizes);
-    double* dst[4] = {S->A64, S->V(2), S->M64, S->stage};
-    for (int q = 0; q < 4; ++q)
+    double* dst[NSAVED] = {S->A64, S->V(2), S->M64, S->stage, S->dv.bex, S->dv.bey, S->dv.omega_k};
+    for (int q = 0; q < NSAVED; ++q)
         if (sizes[q]) CG_CHECK(hipMemcpyAsync(dst[q], parts[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_matrix = true;

@@ -176,6 +176,9 @@ class BPSolver:
     def _after_solve(self):
         """Work after each linear solve and the halo of x (none for BP and SSA)."""
 
+    def _after_extend(self):
+        """Work after the accelerated-Picard extension of x (none for BP and SSA)."""
+
     def zero_inactive(self, uvel, vvel):
         """Velocities at inactive owned vertices are zeroed (glissade_velo_higher.F90:1636-1647)."""
         g = self.grid
@@ -266,6 +269,7 @@ class BPSolver:
                 if accelerate and counter >= 2:
                     self._vop(1, X, UOLD, DU, alpha)
                     _lib.check(self._L.cismgpu_system_halo_x(self.sys.handle), "halo_x")
+                    self._after_extend()
                 self._vop(0, USAV, X)
                 ai = self.assemble()
                 res.assemble_ms.append(ai["total_ms"])
@@ -420,9 +424,11 @@ class DIVASolver(SSASolver):
     compute_effective_viscosity_diva :7730-7997), omega and beta_eff
     (compute_integrals_diva :5649-5775, :2469-2610), basal_sliding_bc_2d_diva
     (:8535-8811) and compute_3d_velocity_diva (:5779-5900) after each solve
-    (:3566-3590).  No Dirichlet rows: no-slip enters through beta_eff
+    (:3566-3590) and after each accelerated-Picard extension (:2188-2210),
+    where beta_eff and omega_k are saved and restored with the matrix
+    (:9898-10048).  No Dirichlet rows: no-slip enters through beta_eff
     (:1423-1428).  The state (efvs_qp_3d, btractx/y) stays on the device and
-    starts at zero unless given.  Plain Picard only."""
+    starts at zero unless given."""
 
     APPROX = 4
 
@@ -472,11 +478,13 @@ class DIVASolver(SSASolver):
     def _after_solve(self):
         _lib.check(self._L.cismgpu_system_diva_velocity(self.sys.handle, None, None, None, None), "diva_velocity")
 
-    def picard(self, *, precond=1, accelerate=False, **kw) -> PicardResult:
-        """Plain Picard iteration of the DIVA solve; the result adds the 3-D
-        velocity of the last iterate (uvel3d, vvel3d) and the basal traction."""
-        if accelerate:
-            raise NotImplementedError("Picard acceleration is not used with DIVA here")
-        r = super().picard(precond=precond, accelerate=False, **kw)
+    def _after_extend(self):
+        self._after_solve()
+
+    def picard(self, *, precond=1, **kw) -> PicardResult:
+        """Picard iteration of the DIVA solve (plain or accelerated); the result
+        adds the 3-D velocity of the last iterate (uvel3d, vvel3d) and the
+        basal traction."""
+        r = super().picard(precond=precond, **kw)
         r.uvel3d, r.vvel3d, r.btractx, r.btracty = self.velocity_3d()
         return r

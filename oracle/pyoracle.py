@@ -690,15 +690,14 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
     1, compute_residual_velocity_2d), started from the basal level of the
     3-D guess (:1362).  approx "l1l2" (two_d): the L1L2 viscosities, and
     compute_3d_velocity_L1L2 after the loop (:3698-3724).  approx "diva" (two_d): DIVA assembly with its state
-    and compute_3d_velocity_diva after each solve (:3566-3590), plain Picard
-    only; the result adds the 3-D velocity (uvel3d, vvel3d)."""
+    and compute_3d_velocity_diva after each solve (:3566-3590) and after each
+    accelerated extension (:2188-2210), beta_eff and omega_k kept with the
+    saved assembly (:9898-10048); the result adds the 3-D velocity (uvel3d, vvel3d)."""
     L = lib()
     p = problem
     g = p.grid
     geo = geometry(p)
     diva = two_d and approx == "diva"
-    if diva and accelerate:
-        raise NotImplementedError("accelerated Picard is restated for BP and SSA only")
     dstate = diva_state(p) if diva else None
     u3 = v3 = None
     if two_d:
@@ -735,6 +734,8 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
                 u = np.asfortranarray(u_old + alpha * du)
                 v = np.asfortranarray(v_old + alpha * dv)
                 halo(g, u); halo(g, v)
+                if diva:     # basal traction of the extended velocity (:2188-2210)
+                    diva_velocity(p, s, u, v)
             usav, vsav = u.copy(order="F"), v.copy(order="F")
             s = (assemble_2d(p, u, v, geo, approx=approx, state=dstate) if two_d else assemble(p, u, v, geo))
             A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]

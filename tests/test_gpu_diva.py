@@ -84,15 +84,18 @@ def test_diva_assembly_and_velocity_match_oracle(require_gpu, name, warm):
     assert _rel(u3[own], ou3[own]) < 1e-12 and _rel(v3[own], ov3[own]) < 1e-12
 
 
+@pytest.mark.parametrize("accelerate", [False, True])
 @pytest.mark.parametrize("name", ["stream", "marine", "dome_noslip"])
-def test_diva_picard_matches_oracle(require_gpu, name):
+def test_diva_picard_matches_oracle(require_gpu, name, accelerate):
     p = PROBLEMS[name]
     g = p.grid
-    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=60)
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=60, accelerate=accelerate)
     ref = po.picard(p, two_d=True, approx="diva", precond=1, **kw)
     r = DIVASolver(p).picard(precond=1, **kw)
     assert r.converged == ref["converged"]
     assert r.iterations == ref["iterations"], (r.iterations, ref["iterations"])
+    if accelerate:
+        assert r.alpha == pytest.approx(ref["alpha"])
     num = np.linalg.norm(np.stack([r.uvel - ref["uvel"], r.vvel - ref["vvel"]]))
     assert num / np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]])) < 1e-7
     own = g.owned_slice()
@@ -101,7 +104,7 @@ def test_diva_picard_matches_oracle(require_gpu, name):
 
 
 def test_diva_guards(require_gpu):
-    """DIVA needs its set-up (and the cell thickness); no Picard acceleration."""
+    """DIVA needs its set-up (and the cell thickness); its basal term is local only."""
     import ctypes
     from cism_amd import _lib
     from cism_amd.velo import AssemblyInfo
@@ -111,5 +114,7 @@ def test_diva_guards(require_gpu):
     assert st == 3 and "diva_setup" in _lib.last_error()
     st = s._L.cismgpu_system_diva_velocity(s.sys.handle, None, None, None, None)
     assert st == 3
-    with pytest.raises(NotImplementedError):
-        DIVASolver(p).picard(accelerate=True)
+    d = DIVASolver(p)
+    d._ain.assemble_beta_standard = 1
+    st = d._L.cismgpu_system_assemble_2d(d.sys.handle, ctypes.byref(d._ain), 4, ctypes.byref(AssemblyInfo()))
+    assert st == 11

@@ -1283,3 +1283,18 @@ def test_standard_basal_term_row_sums_on_planar_bed(two_d):
             assert np.abs(a[:, -1].sum(axis=0) - b[:, -1].sum(axis=0)).max() <= 1e-12 * np.abs(a[13, -1]).max()
             assert np.array_equal(a[:, :-1], b[:, :-1])
     assert std["nwarn_asym"] == 0
+
+
+@pytest.mark.parametrize("make", [lambda: synthetic.ice_stream(40, nz=11, seed=3), lambda: synthetic.dome()])
+def test_accelerated_diva_picard_same_fixed_point(make):
+    """Accelerated Picard with DIVA (basal traction recomputed after each
+    extension; beta_eff and omega_k kept with the saved assembly) reaches the
+    plain DIVA Picard fixed point in fewer iterations."""
+    p = make()
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-6, maxiter=100)
+    a = po.picard(p, two_d=True, approx="diva", precond=1, **kw)
+    b = po.picard(p, two_d=True, approx="diva", precond=1, accelerate=True, **kw)
+    assert a["converged"] and b["converged"] and b["iterations"] < a["iterations"]
+    num = np.linalg.norm(np.stack([a["uvel"] - b["uvel"], a["vvel"] - b["vvel"]]))
+    assert num / np.linalg.norm(np.stack([a["uvel"], a["vvel"]])) < 1e-5
+    assert max(b["alpha"]) > 1.0
