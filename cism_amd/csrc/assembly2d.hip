@@ -831,8 +831,10 @@ __global__ void k_l1l2_velocity(Dims d, AsmDev in, Lev L, const double* __restri
 // phi_r phi_c added to the kA = 0 coupling of column node c.  curved: detJ
 // is the area factor of the face (x, y, staggered lower surface)
 // (get_area_scale_factor_curved_2d, :6736-6827), else the flat Jacobian.
-__global__ void k_basal_std(Dims d, AsmDev in, int level, int curved, double* __restrict__ A,
-                            double* __restrict__ flag) {
+// bx weights Auu, by Avv (DIVA: beta_eff_x / beta_eff_y with the curved
+// Jacobian, basal_sliding_bc_2d_diva :8535-8811; otherwise both beta).
+__global__ void k_basal_std(Dims d, AsmDev in, int level, int curved, const double* __restrict__ bx,
+                            const double* __restrict__ by, double* __restrict__ A, double* __restrict__ flag) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)d.nxo * d.nyo) return;
     const int io = (int)(t % d.nxo), jo = (int)(t / d.nxo);
@@ -848,14 +850,15 @@ __global__ void k_basal_std(Dims d, AsmDev in, int level, int curved, double* __
         for (int n = 0; n < 4; ++n)
             if (nd2_i(n) == i - ci && nd2_j(n) == j - cj) nr = n;
         Elem2 e;
-        double z[4], b[4];
+        double z[4], b[4], c[4];
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
             const long long q = v2(d, ci + nd2_i(n), cj + nd2_j(n));
             e.x[n] = in.xV[q];
             e.y[n] = in.yV[q];
             z[n] = curved ? in.slsrf[q] : 0.0;
-            b[n] = in.beta[q];
+            b[n] = bx[q];
+            c[n] = by[q];
         }
         for (int p = 0; p < 4; ++p) {
             double detJ;
@@ -874,15 +877,19 @@ __global__ void k_basal_std(Dims d, AsmDev in, int level, int curved, double* __
                 double dxn[4], dyn[4];
                 if (!basis_derivs_2d(e, p, dxn, dyn, detJ)) { flag[0] = 1.0; return; }
             }
-            double bq = 0.0;
+            double bq = 0.0, cq = 0.0;
 #pragma unroll
-            for (int n = 0; n < 4; ++n) bq = bq + c_phi2[n][p] * b[n];
+            for (int n = 0; n < 4; ++n) {
+                bq = bq + c_phi2[n][p] * b[n];
+                cq = cq + c_phi2[n][p] * c[n];
+            }
 #pragma unroll
             for (int nc = 0; nc < 4; ++nc) {
-                const double K = 0.0 + bq * 1.0 * detJ / vol0 * c_phi2[nr][p] * c_phi2[nc][p];
+                const double Ku = 0.0 + bq * 1.0 * detJ / vol0 * c_phi2[nr][p] * c_phi2[nc][p];
+                const double Kv = 0.0 + cq * 1.0 * detJ / vol0 * c_phi2[nr][p] * c_phi2[nc][p];
                 const int m = (nd2_i(nc) - nd2_i(nr) + 1) + 3 * (nd2_j(nc) - nd2_j(nr) + 1) + 9;
-                a[(0 * 27 + m) * WAVE] = a[(0 * 27 + m) * WAVE] + K;
-                a[(3 * 27 + m) * WAVE] = a[(3 * 27 + m) * WAVE] + K;
+                a[(0 * 27 + m) * WAVE] = a[(0 * 27 + m) * WAVE] + Ku;
+                a[(3 * 27 + m) * WAVE] = a[(3 * 27 + m) * WAVE] + Kv;
             }
         }
     }

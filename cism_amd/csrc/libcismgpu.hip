@@ -1311,7 +1311,8 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
     if (in->beta_internal) {
         if (a.beta_std)
-            k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, d.nz - 1, 1, S->A64, S->scal + S_NAN);
+            k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, d.nz - 1, 1, a.beta, a.beta, S->A64,
+                                                                               S->scal + S_NAN);
         else
             k_basal_local<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
     }
@@ -1389,9 +1390,9 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
         cismgpu_set_error("assemble_2d: L1L2 needs the rate factor (in->flwa) and n = 1 or 3 for the nonlinear viscosity");
         return CISMGPU_ERR_ARG;
     }
-    if (diva && in->assemble_beta_standard) {
-        cismgpu_set_error("assemble_2d: the standard basal option is not built for DIVA");
-        return CISMGPU_ERR_UNSUPPORTED;
+    if (diva && in->assemble_beta_standard && !in->stag_lsrf) {
+        cismgpu_set_error("assemble_2d: the standard basal option of DIVA needs stag_lsrf");
+        return CISMGPU_ERR_ARG;
     }
     if (diva && (!S->diva_ready || !in->thck)) {
         cismgpu_set_error("assemble_2d: DIVA needs cismgpu_system_diva_setup and the cell thickness (in->thck)");
@@ -1439,12 +1440,17 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
             k_cell_halo<<<nblocks(nc, 256), 256, 0, S->s>>>(d, 0, S->dv.omega);
         }
         k_diva_beta<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->dv);
-        k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
+        if (a.beta_std)   // standard option: curved Jacobian, beta_eff_x / beta_eff_y
+            k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, 0, 1, S->dv.bex, S->dv.bey, S->A64,
+                                                                               S->scal + S_NAN);
+        else
+            k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
     } else {
         if (!l1l2) k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
         if (in->beta_internal) {
             if (a.beta_std)
-                k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, 0, 0, S->A64, S->scal + S_NAN);
+                k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, 0, 0, a.beta, a.beta, S->A64,
+                                                                                   S->scal + S_NAN);
             else
                 k_basal_local_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->A64);
         }

@@ -128,7 +128,8 @@ int orc_load_vector_gravity_standard(int nx, int ny, int nz, int nhalo, const do
                                      const double* dusrf_dy, double* loadu, double* loadv);
 int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, int curved,
                                   const int32_t* active_cell, const double* xVertex, const double* yVertex,
-                                  const double* staglsrf, const double* beta, double* Auu, double* Avv);
+                                  const double* staglsrf, const double* beta, const double* beta_y, double* Auu,
+                                  double* Avv);
 
 #ifdef __cplusplus
 }

@@ -1690,13 +1690,16 @@ int orc_load_vector_gravity_standard(int nx, int ny, int nz, int nhalo, const do
  * curved_2d_basal_jacobian = .true., :8914): detJ is the area factor of
  * the face (x, y, staglsrf) (get_area_scale_factor_curved_2d :6736-6827);
  * otherwise the flat 2-D Jacobian.  two_d: A are (nx-1, ny-1, 9), else the
- * level nz of (27, nz, nx-1, ny-1). */
+ * level nz of (27, nz, nx-1, ny-1).  beta_y (NULL: beta) weights Avv; DIVA
+ * passes beta_eff_x / beta_eff_y with the curved Jacobian
+ * (basal_sliding_bc_2d_diva standard option, :8535-8811). */
 int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, int curved,
                                   const int32_t* active_cell, const double* xVertex, const double* yVertex,
-                                  const double* staglsrf, const double* beta, double* Auu, double* Avv)
+                                  const double* staglsrf, const double* beta, const double* beta_y, double* Auu,
+                                  double* Avv)
 {
     init_tables_2d();
-    double x[4], y[4], z[4], b[4];
+    double x[4], y[4], z[4], b[4], by[4];
     for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
         for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
             if (!active_cell[C2(i, j)]) continue;
@@ -1706,6 +1709,7 @@ int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, 
                 y[n] = yVertex[q];
                 z[n] = staglsrf ? staglsrf[q] : 0.0;
                 b[n] = beta[q];
+                by[n] = beta_y ? beta_y[q] : beta[q];
             }
             for (int p = 0; p < 4; ++p) {
                 double detJ;
@@ -1723,18 +1727,24 @@ int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, 
                     double dx[4], dy[4];
                     if (basis_derivs_2d(x, y, p, dx, dy, &detJ)) return ORC_ERR_SINGULAR;
                 }
-                double bq = 0.0;
-                for (int n = 0; n < 4; ++n) bq = bq + phi2[n][p] * b[n];
-                double K[4][4];
+                double bq = 0.0, bqy = 0.0;
+                for (int n = 0; n < 4; ++n) {
+                    bq = bq + phi2[n][p] * b[n];
+                    bqy = bqy + phi2[n][p] * by[n];
+                }
+                double K[4][4], Ky[4][4];
                 for (int nc = 0; nc < 4; ++nc)
-                    for (int nr = 0; nr < 4; ++nr) K[nr][nc] = 0.0 + bq * 1.0 * detJ / VOL0 * phi2[nr][p] * phi2[nc][p];
+                    for (int nr = 0; nr < 4; ++nr) {
+                        K[nr][nc] = 0.0 + bq * 1.0 * detJ / VOL0 * phi2[nr][p] * phi2[nc][p];
+                        Ky[nr][nc] = 0.0 + bqy * 1.0 * detJ / VOL0 * phi2[nr][p] * phi2[nc][p];
+                    }
                 for (int nr = 0; nr < 4; ++nr) {
                     const int ii = i + di2[nr], jj = j + dj2[nr];
                     for (int nc = 0; nc < 4; ++nc) {
                         const int iA = di2[nc] - di2[nr], jA = dj2[nc] - dj2[nr];
                         const size_t a = two_d ? A2(IDX2(iA, jA), ii, jj) : A4(IDX3(iA, jA, 0), nz, ii, jj);
                         Auu[a] = Auu[a] + K[nr][nc];
-                        Avv[a] = Avv[a] + K[nr][nc];
+                        Avv[a] = Avv[a] + Ky[nr][nc];
                     }
                 }
             }

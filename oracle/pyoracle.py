@@ -111,7 +111,7 @@ def _declare(L):
                                                _D, _D]
     L.orc_load_vector_gravity_standard.argtypes = [i, i, i, i, _D, _I, _D, _D, _D, _D, _D, _D, _D, _D]
     L.orc_load_vector_gravity_standard.restype = ctypes.c_int
-    L.orc_basal_sliding_bc_standard.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D, _D, _D]
+    L.orc_basal_sliding_bc_standard.argtypes = [i, i, i, i, i, i, _I, _D, _D, _D, _D, _D, _D, _D]
     L.orc_basal_sliding_bc_standard.restype = ctypes.c_int
     L.orc_l1l2_tau_parallel.argtypes = [d, d, d]
     L.orc_l1l2_tau_parallel.restype = ctypes.c_double
@@ -368,7 +368,7 @@ def basal_2d(p, geo, xV, yV, av, binternal, A):
     g = p.grid
     if getattr(p, "assemble_beta", 1) == 0:
         st = L.orc_basal_sliding_bc_standard(g.nx, g.ny, g.nz, 2, 1, 0, ip(geo["active_cell"]), dp(xV), dp(yV),
-                                             None, dp(binternal), dp(A[0]), dp(A[3]))
+                                             None, dp(binternal), None, dp(A[0]), dp(A[3]))
         if st:
             raise RuntimeError(f"oracle standard 2-D basal assembly failed with status {st}")
     else:
@@ -414,7 +414,7 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     if p.whichbabc != 6:
         if getattr(p, "assemble_beta", 1) == 0:
             st = L.orc_basal_sliding_bc_standard(nx, ny, nz, 2, 0, 1, ip(geo["active_cell"]), dp(xV), dp(yV),
-                                                 dp(stagger(geo["lsrf"])), dp(binternal), dp(A[0]), dp(A[3]))
+                                                 dp(stagger(geo["lsrf"])), dp(binternal), None, dp(A[0]), dp(A[3]))
             if st:
                 raise RuntimeError(f"oracle standard basal assembly failed with status {st}")
         else:
@@ -556,8 +556,14 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
         L.orc_diva_beta_eff(nx, ny, 1 if p.whichbabc == 6 else 0, dp(stag_omega), dp(binternal),
                             dp(dg["stag_slope_factor_x"]), dp(dg["stag_slope_factor_y"]),
                             dp(dg["stag_theta_slope_x"]), dp(dg["stag_theta_slope_y"]), dp(bex), dp(bey))
-        L.orc_basal_sliding_bc_2d_diva_local(nx, ny, p.dx, p.dy, ip(av), dp(bex), dp(bey), dp(dg["theta_basal_x"]),
-                                             dp(dg["theta_basal_y"]), dp(A[0]), dp(A[3]))
+        if getattr(p, "assemble_beta", 1) == 0:     # standard option, curved Jacobian (:8535-8811)
+            st = L.orc_basal_sliding_bc_standard(nx, ny, nz, 2, 1, 1, ip(geo["active_cell"]), dp(xV), dp(yV),
+                                                 dp(stagger(geo["lsrf"])), dp(bex), dp(bey), dp(A[0]), dp(A[3]))
+            if st:
+                raise RuntimeError(f"oracle standard DIVA basal assembly failed with status {st}")
+        else:
+            L.orc_basal_sliding_bc_2d_diva_local(nx, ny, p.dx, p.dy, ip(av), dp(bex), dp(bey), dp(dg["theta_basal_x"]),
+                                                 dp(dg["theta_basal_y"]), dp(A[0]), dp(A[3]))
         extra = dict(omega_k=omega_k, omega=omega, stag_omega=stag_omega, beta_eff_x=bex, beta_eff_y=bey,
                      state=state)
     elif approx == "l1l2":

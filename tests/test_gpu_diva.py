@@ -59,11 +59,20 @@ def _check_assembly(p, s, ref):
     assert _rel(s.efvs(), ref["efvs"]) < 1e-12
 
 
+@pytest.mark.parametrize("basal", ["local", "standard"])
 @pytest.mark.parametrize("warm", [False, True])
 @pytest.mark.parametrize("name", list(PROBLEMS))
-def test_diva_assembly_and_velocity_match_oracle(require_gpu, name, warm):
+def test_diva_assembly_and_velocity_match_oracle(require_gpu, name, warm, basal):
     p = PROBLEMS[name]
     g = p.grid
+    p.assemble_beta = 0 if basal == "standard" else 1
+    try:
+        _diva_assembly_case(p, g, warm)
+    finally:
+        p.assemble_beta = 1
+
+
+def _diva_assembly_case(p, g, warm):
     if warm:   # the state and velocity one DIVA Picard iteration leaves
         r1 = po.picard(p, two_d=True, approx="diva", precond=1, linear_tol=1e-10, linear_maxiters=3000, maxiter=1)
         state, u, v = _copy_state(r1["state"]), r1["uvel"].copy(order="F"), r1["vvel"].copy(order="F")
@@ -104,7 +113,8 @@ def test_diva_picard_matches_oracle(require_gpu, name, accelerate):
 
 
 def test_diva_guards(require_gpu):
-    """DIVA needs its set-up (and the cell thickness); its basal term is local only."""
+    """DIVA needs its set-up (and the cell thickness); its standard basal
+    option needs the staggered lower surface."""
     import ctypes
     from cism_amd import _lib
     from cism_amd.velo import AssemblyInfo
@@ -116,5 +126,6 @@ def test_diva_guards(require_gpu):
     assert st == 3
     d = DIVASolver(p)
     d._ain.assemble_beta_standard = 1
+    d._ain.stag_lsrf = None
     st = d._L.cismgpu_system_assemble_2d(d.sys.handle, ctypes.byref(d._ain), 4, ctypes.byref(AssemblyInfo()))
-    assert st == 11
+    assert st == 3 and "stag_lsrf" in _lib.last_error()

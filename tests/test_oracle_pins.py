@@ -1298,3 +1298,27 @@ def test_accelerated_diva_picard_same_fixed_point(make):
     num = np.linalg.norm(np.stack([a["uvel"] - b["uvel"], a["vvel"] - b["vvel"]]))
     assert num / np.linalg.norm(np.stack([a["uvel"], a["vvel"]])) < 1e-5
     assert max(b["alpha"]) > 1.0
+
+
+def test_standard_diva_basal_term_on_planar_bed():
+    """basal_sliding_bc_2d_diva, standard option, on the periodic slab (bed
+    inclined in x only): the consistent rows of Auu sum to the local
+    beta_eff_x dx dy / (vol0 cos theta_x); those of Avv carry the face's area
+    factor 1/cos theta where the local option divides by cos theta_y = 1."""
+    p = synthetic.slab(n_cells=8, nz=6)
+    g = p.grid
+    out = {}
+    for opt in (1, 0):
+        p.assemble_beta = opt
+        out[opt] = po.assemble_2d(p, approx="diva")
+    p.assemble_beta = 1
+    loc, std = out[1], out[0]
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    th = loc["geo"]["diva"]["theta_basal_x"][own]
+    assert np.ptp(th) < 1e-12 and th.mean() > 1e-3
+    au_l, au_s = loc["Auu"][own[0], own[1], :], std["Auu"][own[0], own[1], :]
+    av_l, av_s = loc["Avv"][own[0], own[1], :], std["Avv"][own[0], own[1], :]
+    assert np.abs(au_s.sum(axis=-1) - au_l.sum(axis=-1)).max() <= 1e-11 * np.abs(au_l[..., 4]).max()
+    # the stiffness rows sum to ~0 on the uniform slab, so the row sums are the basal terms
+    ratio = av_s.sum(axis=-1) / av_l.sum(axis=-1)
+    assert np.allclose(ratio, 1.0 / np.cos(th.mean()), rtol=1e-9)
