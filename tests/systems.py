@@ -23,7 +23,7 @@ def stream_system(n=40, nz=11, seed=3):
 
 
 @functools.lru_cache(maxsize=None)
-def random_spd_system(n=12, nz=6, bc=BC_PERIODIC, seed=0):
+def random_spd_system(n=12, nz=6, bc=BC_PERIODIC, seed=0, ny=None):
     """Random symmetric positive definite 27-point 2x2-block system.
 
     Built from random SPD 16x16 "element" matrices on 2x2x2 node cubes, as
@@ -32,30 +32,31 @@ def random_spd_system(n=12, nz=6, bc=BC_PERIODIC, seed=0):
     rows with the periodic halo rule (as glissade_velo_higher.F90:2893-2898
     does); other grids keep every element two vertices inside the owned
     block, so that the last owned row (which the no_ice halo rule zeroes)
-    stays inactive."""
+    stays inactive.  ny (default n) makes the owned block n x ny."""
     from cism_amd.grid import indxA_3d
-    g = Grid.from_owned(n, n, nz, global_bc=bc)
+    g = Grid.from_owned(n, n if ny is None else ny, nz, global_bc=bc)
     rng = np.random.default_rng(seed)
     A = [g.zeros_mat() for _ in range(4)]
     periodic = bc in (0, 2)
-    no = g.ihi - g.ilo + 1
-    wrap = lambda v, lo: lo + (v - lo) % no
+    nox, noy = g.ihi - g.ilo + 1, g.jhi - g.jlo + 1
+    wrapi = lambda v: g.ilo + (v - g.ilo) % nox
+    wrapj = lambda v: g.jlo + (v - g.jlo) % noy
     act = np.zeros((g.nvx, g.nvy), np.int32, order="F")
     if periodic:
         act[:, :] = 1
-        cells = range(g.ilo, g.ihi + 1)
+        icells, jcells = range(g.ilo, g.ihi + 1), range(g.jlo, g.jhi + 1)
     else:
-        cells = range(g.ilo, g.ihi - 1)
+        icells, jcells = range(g.ilo, g.ihi - 1), range(g.jlo, g.jhi - 1)
         act[g.ilo - 1:g.ihi - 1, g.jlo - 1:g.jhi - 1] = 1
     offs = [(di, dj, dk) for dk in (0, 1) for dj in (0, 1) for di in (0, 1)]
-    for j in cells:
-        for i in cells:
+    for j in jcells:
+        for i in icells:
             for k in range(1, nz):
                 B = rng.standard_normal((16, 16)) * 0.3
                 K = B.T @ B
                 nodes = [(k + dk, i + di, j + dj) for (di, dj, dk) in offs]
                 for r, (kr, ir, jr) in enumerate(nodes):
-                    irw, jrw = (wrap(ir, g.ilo), wrap(jr, g.jlo)) if periodic else (ir, jr)
+                    irw, jrw = (wrapi(ir), wrapj(jr)) if periodic else (ir, jr)
                     for c, (kc, ic, jc) in enumerate(nodes):
                         m = indxA_3d(ic - ir, jc - jr, kc - kr)
                         A[0][m - 1, kr - 1, irw - 1, jrw - 1] += K[r, c]
