@@ -76,3 +76,23 @@ def test_nz_limit(require_gpu):
     with pytest.raises(_lib.CismGpuError) as e:
         pcg.System(g)
     assert e.value.status == 3 and "2..64" in str(e.value)
+
+
+@pytest.mark.parametrize("nz", [2, 33, 64])
+def test_assembly_deep_and_shallow_columns(require_gpu, nz):
+    """The BP assembly (stiffness, basal term, load, Dirichlet rows) at two
+    levels and at 33 and 64 levels against the restatement (rows 1e-12)."""
+    from cism_amd import synthetic
+    from cism_amd.velo import BPSolver
+    from test_gpu_assembly import _rowwise_close
+    p = synthetic.ice_stream(20, nz=nz, seed=nz)
+    ref = po.assemble(p)
+    s = BPSolver(p)
+    s.assemble(p.uvel, p.vvel)
+    (Auu, Auv, Avu, Avv), (bu, bv) = s.sys.get_matrix()
+    for a, key in ((Auu, "Auu"), (Auv, "Auv"), (Avu, "Avu"), (Avv, "Avv")):
+        e = _rowwise_close(a, ref[key], p.grid, ref["active_vertex"], 1e-12)
+        assert e < 1e-12, (key, e)
+    bscale = max(np.abs(ref["bu"]).max(), np.abs(ref["bv"]).max(), 1e-300)
+    assert np.abs(bu - ref["bu"]).max() / bscale < 1e-13
+    assert np.abs(bv - ref["bv"]).max() / bscale < 1e-13
