@@ -24,7 +24,9 @@ def test_header_declares_entry_points():
     for n in ("cismgpu_pcg_solver_standard_3d", "cismgpu_pcg_solver_chrongear_3d",
               "cismgpu_matvec_multiply_structured_3d", "cismgpu_system_assemble_3d",
               "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d",
-              "cismgpu_matvec_multiply_structured_2d", "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d"):
+              "cismgpu_matvec_multiply_structured_2d", "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d",
+              "cismgpu_system_assemble_2d", "cismgpu_system_diva_setup", "cismgpu_system_diva_velocity",
+              "cismgpu_system_l1l2_velocity"):
         assert n in names
 
 
