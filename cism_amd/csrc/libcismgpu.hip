@@ -1222,11 +1222,13 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     double* db = (double*)S->asm_buf;
     int32_t* ib = (int32_t*)(db + nd);
     double* p = db;
+    hipError_t herr = hipSuccess;   // first failure of the copies below
+    auto keep = [&](hipError_t e) { if (e != hipSuccess && herr == hipSuccess) herr = e; };
     auto putd = [&](const double* h, long long n, bool zero_if_null) -> const double* {
         double* dst = p;
         p += n;
-        if (h) hipMemcpyAsync(dst, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
-        else if (zero_if_null) hipMemsetAsync(dst, 0, n * sizeof(double), S->s);
+        if (h) keep(hipMemcpyAsync(dst, h, n * sizeof(double), hipMemcpyHostToDevice, S->s));
+        else if (zero_if_null) keep(hipMemsetAsync(dst, 0, n * sizeof(double), S->s));
         return dst;
     };
     a.sigma = putd(in->sigma, d.nz, true);
@@ -1256,8 +1258,8 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     auto puti = [&](const int32_t* h, long long n) -> const int32_t* {
         int32_t* dst = q;
         q += n;
-        if (h) hipMemcpyAsync(dst, h, n * sizeof(int32_t), hipMemcpyHostToDevice, S->s);
-        else hipMemsetAsync(dst, 0, n * sizeof(int32_t), S->s);
+        if (h) keep(hipMemcpyAsync(dst, h, n * sizeof(int32_t), hipMemcpyHostToDevice, S->s));
+        else keep(hipMemsetAsync(dst, 0, n * sizeof(int32_t), S->s));
         return dst;
     };
     a.acell = puti(in->active_cell, nc);
@@ -1268,6 +1270,7 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     a.efvs = S->efvs;
     a.dx = in->dx; a.dy = in->dy; a.efvs_constant = in->efvs_constant; a.effstrain_min = in->effstrain_min;
     a.n_glen = in->n_glen; a.whichefvs = in->whichefvs; a.nhalo = 2;
+    CG_CHECK(herr);
     // active mask (int8) for the solver
     {
         std::vector<int8_t> act(n2);
@@ -1564,10 +1567,13 @@ extern "C" int32_t cismgpu_system_diva_setup(cismgpu_system* S, const cismgpu_di
     double* p = S->diva_buf;
     DivaDev& dv = S->dv;
     auto take = [&](long long n) { double* q = p; p += n; return q; };
+    hipError_t herr = hipSuccess;
     auto up = [&](const double* h, long long n, bool zero) -> double* {
         double* q = take(n);
-        if (h) hipMemcpyAsync(q, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
-        else if (zero) hipMemsetAsync(q, 0, n * sizeof(double), S->s);
+        hipError_t e = hipSuccess;
+        if (h) e = hipMemcpyAsync(q, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
+        else if (zero) e = hipMemsetAsync(q, 0, n * sizeof(double), S->s);
+        if (e != hipSuccess && herr == hipSuccess) herr = e;
         return q;
     };
     dv.sfx = up(in->slope_factor_x, nc, true);
@@ -1588,6 +1594,7 @@ extern "C" int32_t cismgpu_system_diva_setup(cismgpu_system* S, const cismgpu_di
     dv.omega = up(nullptr, nc, true);
     dv.efva = take(4 * nc);
     int32_t* m = (int32_t*)p;
+    CG_CHECK(herr);
     CG_CHECK(hipMemcpyAsync(m, in->ice_plus_land_mask, nc * sizeof(int32_t), hipMemcpyHostToDevice, S->s));
     dv.ipl = m;
     dv.no_slip = in->no_slip ? 1 : 0;
