@@ -1943,7 +1943,7 @@ __global__ void k_ens_init(long long nvec, double* __restrict__ x, int nmem, Ens
     if (t < 8) sync[t] = 0u;
 }
 
-static void ens_tiles(cismgpu_ensemble* E) {
+static int ens_tiles(cismgpu_ensemble* E) {
     std::vector<int> tm, tc;
     int tmat = 0, tcol = 0;
     for (int m = 0; m < E->nmem; ++m) {
@@ -1964,15 +1964,18 @@ static void ens_tiles(cismgpu_ensemble* E) {
     if (E->d_tile_mem_col) hipFree(E->d_tile_mem_col);
     if (E->part_mat) hipFree(E->part_mat);
     if (E->part_col) hipFree(E->part_col);
-    hipMalloc(&E->d_tile_mem_mat, tm.size() * sizeof(int));
-    hipMalloc(&E->d_tile_mem_col, tc.size() * sizeof(int));
-    hipMemcpy(E->d_tile_mem_mat, tm.data(), tm.size() * sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(E->d_tile_mem_col, tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice);
-    hipMalloc(&E->part_mat, 2 * (size_t)tmat * sizeof(double));
-    hipMalloc(&E->part_col, 2 * 4 * (size_t)tcol * sizeof(double));
-    hipMemcpy(E->d_mem, E->mem.data(), E->nmem * sizeof(EnsMember), hipMemcpyHostToDevice);
+    E->d_tile_mem_mat = E->d_tile_mem_col = nullptr;
+    E->part_mat = E->part_col = nullptr;
     if (E->gexec) { hipGraphExecDestroy(E->gexec); E->gexec = nullptr; }
     if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+    CG_CHECK(hipMalloc(&E->d_tile_mem_mat, tm.size() * sizeof(int)));
+    CG_CHECK(hipMalloc(&E->d_tile_mem_col, tc.size() * sizeof(int)));
+    CG_CHECK(hipMemcpy(E->d_tile_mem_mat, tm.data(), tm.size() * sizeof(int), hipMemcpyHostToDevice));
+    CG_CHECK(hipMemcpy(E->d_tile_mem_col, tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice));
+    CG_CHECK(hipMalloc(&E->part_mat, 2 * (size_t)tmat * sizeof(double)));
+    CG_CHECK(hipMalloc(&E->part_col, 2 * 4 * (size_t)tcol * sizeof(double)));
+    CG_CHECK(hipMemcpy(E->d_mem, E->mem.data(), E->nmem * sizeof(EnsMember), hipMemcpyHostToDevice));
+    return CISMGPU_OK;
 }
 
 extern "C" int32_t cismgpu_ensemble_destroy(cismgpu_ensemble* E) {
@@ -2066,7 +2069,7 @@ extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* mem
         hipMemcpyAsync(E->active + M.c_off, S->active, (size_t)d.nvx * d.nvy, hipMemcpyDeviceToDevice, E->s);
     }
     if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "build");
-    ens_tiles(E);
+    if (ens_tiles(E) != CISMGPU_OK) return fail(CISMGPU_ERR_HIP, "tiles");
     *out = E;
     return CISMGPU_OK;
 }
@@ -2085,7 +2088,7 @@ extern "C" int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* E, int32_t* split
     hipEventCreate(&e1);
     for (int si = 0; si < 3; ++si) {
         for (auto& M : E->mem) M.split = splits[si];
-        ens_tiles(E);
+        { const int st_ = ens_tiles(E); if (st_) return st_; }
         for (int b = 0; b < 3; ++b) {
             std::vector<int> tl;
             for (int m = 0; m < E->nmem; ++m)
@@ -2115,7 +2118,7 @@ extern "C" int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* E, int32_t* split
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     for (int m = 0; m < E->nmem; ++m) E->mem[m].split = best[ens_bucket(E->side[m])];
-    ens_tiles(E);
+    { const int st_ = ens_tiles(E); if (st_) return st_; }
     CG_CHECK(hipMemsetAsync(Az, 0, E->vtotal * sizeof(float), E->s));
     CG_CHECK(hipStreamSynchronize(E->s));
     if (split_out) for (int b = 0; b < 3; ++b) split_out[b] = best[b];
