@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
                                                    const float* __restrict__ xv, float* __restrict__ yu,
                                                    float* __restrict__ yv, const float* __restrict__ bu,
                                                    const float* __restrict__ bv, double* __restrict__ partials) {
-    __shared__ float xs[2][3][6][WAVE + 2];
+    __shared__ float xs[2][4][6][WAVE + 2];
     // ex[q][c][w][lane]: q 0..2 slots 11, 12, 13 (read by tile row w + 1),
     // q 3..5 slots 5, 6, 7 (read by tile row w - 1), of the previous level
     __shared__ float ex[6][4][4][WAVE];
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
     const bool live = io < d.nxo && jo < d.nyo;
     const int i0 = d.ilo + bx * WAVE - 1, j0 = d.jlo + by * 4 - 1;
     auto stage = [&](int kk) {
-        const int slot = (kk + 1) % 3;
+        const int slot = (kk + 1) % 4;
         for (int e = threadIdx.x; e < 6 * (WAVE + 2); e += 256) {
             const int r = e / (WAVE + 2), c = e % (WAVE + 2);
             const int vi = i0 + c, vj = j0 + r;
@@ -274,6 +274,9 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
     float p8[4] = {0.f, 0.f, 0.f, 0.f}, p9[4] = {0.f, 0.f, 0.f, 0.f}, p10[4] = {0.f, 0.f, 0.f, 0.f};
     stage(-1);
     stage(0);
+    // x of level kk sits in ring slot (kk + 1) % 4.  stage(k + 1) overwrites
+    // level k - 3, last read in step k - 2; every thread is past the mid
+    // barrier of step k - 1 by then, so two barriers per level suffice.
     for (int k = 0; k < d.nz; ++k) {
         stage(k + 1);
         __syncthreads();   // x of levels k-1..k+1 staged; ex holds level k-1
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
                     ld_slot(Sb + po[m] + (13 - m) * (4 * WAVE), a, b, c, e);
                 }
                 const float cuu = a, cuv = c, cvu = b, cvv = e;     // transposed 2x2 block
-                const int slot = k % 3;                              // level k-1
+                const int slot = k % 4;                              // level k-1
                 const int r = w + 1 + ((m / 3) % 3 - 1), cc = lane + 1 + (m % 3 - 1);
                 const float xun = xs[0][slot][r][cc], xvn = xs[1][slot][r][cc];
                 su = su + cuu * xun + cuv * xvn;
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
                     cuu = own[m - 13][0]; cuv = own[m - 13][1]; cvu = own[m - 13][2]; cvv = own[m - 13][3];
                 }
                 if (have) {
-                    const int slot = (k + m / 9) % 3;
+                    const int slot = (k + m / 9) % 4;
                     const int r = w + 1 + ((m / 3) % 3 - 1), c = lane + 1 + (m % 3 - 1);
                     const float xun = xs[0][slot][r][c], xvn = xs[1][slot][r][c];
                     su = su + cuu * xun + cuv * xvn;
@@ -375,13 +378,12 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
                 yu[ck] = su;
                 yv[ck] = sv;
                 if (MODE == 1) {
-                    const int slot = (k + 1) % 3;
+                    const int slot = (k + 1) % 4;
                     acc += (double)su * (double)xs[0][slot][w + 1][lane + 1] +
                            (double)sv * (double)xs[1][slot][w + 1][lane + 1];
                 }
             }
         }
-        __syncthreads();   // ex holds level k; the x slot of level k-1 is restaged next
     }
     if (MODE != 0) {
         dd v[1] = {{acc, 0.0}};
