@@ -416,6 +416,8 @@ def run_ensemble(args):
         out = dict(members=args.ensemble, owned_nodes=int(nodes), solve_ms=round(info["solve_ms"], 2),
                    steps=info["steps"], converged=info["converged"], max_outer=info["max_outer"],
                    max_inner=info["max_inner"], total_inner=int(info["total_inner"]), grid_blocks=info["grid_blocks"],
+                   matrix_storage={1: "fp32 quad rows", 2: "bf16 quad rows", 3: "fp32 half (symmetric)"}.get(
+                       info["matrix_storage"], info["matrix_storage"]),
                    split_per_bucket=split, autotune_ms=np.round(tune_ms, 3).tolist(),
                    sequential_mixed_ms=round(seq_ms, 2), speedup_vs_sequential=round(seq_ms / info["solve_ms"], 3),
                    max_rel_err_vs_member_fp64=max_err)

@@ -71,6 +71,8 @@ struct EnsPtrs {
     const double* A64;               // compact rows, row-blocked (108 coefs x 64)
     const float* A32;
     const __hip_bfloat16* A16;
+    const float* S32;                // half (symmetric) storage of A32: centre + 13 upper couplings
+    int half;                        // 1: the FP32 products read S32, 0: A32
     const float* M32;                // SIA coefficients per member vector layout (6 fields)
     const int8_t* active;
     const double* b64;               // 2 components per member block
@@ -204,6 +206,90 @@ __device__ __forceinline__ dd matrix_tile(const EnsMember& M, int lt, const TA* 
     return acc;
 }
 
+// FP32 product with the half (symmetric) storage: row R holds its centre
+// (uu, uv, vv, 0) and 13 upper couplings as quads at
+//     S[((R/64)*14 + s)*256 + (R%64)*4 + c];
+// the coupling to a lower neighbour (m < 13) is the transposed slot 13 - m
+// of the neighbour's row.  Members keep an ice-free outer ring, so a
+// neighbour outside the owned columns couples with zero.  The products are
+// summed in the order and with the values of matrix_tile, so the results are
+// identical when the FP32 matrix is exactly symmetric.
+__device__ __forceinline__ dd matrix_tile_half(const EnsMember& M, int lt, const float* __restrict__ S,
+                                               const float* __restrict__ xu, const float* __restrict__ xv,
+                                               float* __restrict__ yu, float* __restrict__ yv) {
+    const Dims& d = M.d;
+    const int lane = threadIdx.x % WAVE;
+    const int cgi = lt % M.ncg, lg = lt / M.ncg;
+    const int c = cgi * WAVE + lane;
+    dd acc = {0.0, 0.0};
+    if (c >= M.ncol) return acc;
+    const int io = c % d.nxo, jo = c / d.nxo;
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const int kper = (d.nz + M.split - 1) / M.split;
+    const int k0 = lg * kper, k1 = min(d.nz, k0 + kper);
+    const long long pl = d.plane;
+    // bit q: horizontal neighbour q (0..8) lies inside the owned columns
+    unsigned inb = 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int pi = io + q % 3 - 1, pj = jo + q / 3 - 1;
+        inb |= (pi >= 0 && pi < d.nxo && pj >= 0 && pj < d.nyo) ? (1u << q) : 0u;
+    }
+    // raw buffer over this member's rows: 32-bit member-local byte offsets
+    // (the host keeps every member's part below 2 GiB), loads past it read zero
+    const float* Sm = S + (size_t)(M.a_off / WAVE) * (14 * 4 * WAVE);
+    const int mbytes = d.nz * M.ncolp * 56 * (int)sizeof(float);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)Sm, (short)0, mbytes, 0x00020000);
+    auto quad = [&](unsigned row, int sl, float& a, float& b, float& e, float& f) {
+        const unsigned off = (((row / WAVE) * 14u + (unsigned)sl) * (4u * WAVE) + (row % WAVE) * 4u) * 4u;
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        a = __uint_as_float(v[0]); b = __uint_as_float(v[1]); e = __uint_as_float(v[2]); f = __uint_as_float(v[3]);
+    };
+    for (int k = k0; k < k1; ++k) {
+        const int R = k * M.ncolp + c;     // member-local row
+        const long long c0 = vidx(d, k, i, j);
+        float su = 0.f, sv = 0.f;
+        // one row of three neighbours (fixed kA, jA) in flight at a time
+#pragma unroll 1
+        for (int rw = 0; rw < 9; ++rw) {
+            const int kA = rw / 3, jq = rw % 3;
+            float xa[3], xb[3], cuu[3], cuv[3], cvu[3], cvv[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int q = jq * 3 + t, m = kA * 9 + q;
+                const long long n = c0 + (kA - 1) * pl + (jq - 1) * d.px + (t - 1);
+                xa[t] = xu[n];
+                xb[t] = xv[n];
+                if (m < 13) {
+                    cuu[t] = cuv[t] = cvu[t] = cvv[t] = 0.f;
+                    if (((inb >> q) & 1u) && k + kA - 1 >= 0) {
+                        const int P = R + (kA - 1) * M.ncolp + (jq - 1) * d.nxo + (t - 1);
+                        float a, b, e, f;
+                        quad((unsigned)P, 13 - m, a, b, e, f);
+                        cuu[t] = a; cuv[t] = e; cvu[t] = b; cvv[t] = f;     // transposed
+                    }
+                } else {
+                    const int sl = m == 13 ? 0 : m - 13;
+                    float a, b, e, f;
+                    quad((unsigned)R, sl, a, b, e, f);
+                    if (m == 13) { cuu[t] = a; cuv[t] = b; cvu[t] = b; cvv[t] = e; }
+                    else { cuu[t] = a; cuv[t] = b; cvu[t] = e; cvv[t] = f; }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                su = su + cuu[t] * xa[t] + cuv[t] * xb[t];
+                sv = sv + cvu[t] * xa[t] + cvv[t] * xb[t];
+            }
+        }
+        yu[c0] = su;
+        yv[c0] = sv;
+        acc = dd_add_d(acc, (double)su * (double)xu[c0]);
+        acc = dd_add_d(acc, (double)sv * (double)xv[c0]);
+    }
+    return acc;
+}
+
 // ---------------------------------------------------------------- column phase work
 // One vertex column, one velocity component: the C-G vector update (START:
 // load r from the FP64 residual and clear e, d, q) followed by the SIA
@@ -310,6 +396,8 @@ __global__ __launch_bounds__(ENS_BS) void k_ensemble(EnsParams P, EnsPtrs Q) {
             else if (Q.use_bf16)
                 acc = matrix_tile<__hip_bfloat16, float, false>(M, lt, Q.A16, Q.z32 + vo, Q.z32 + vo + vl, nullptr,
                                                                 nullptr, Q.Az32 + vo, Q.Az32 + vo + vl);
+            else if (Q.half)
+                acc = matrix_tile_half(M, lt, Q.S32, Q.z32 + vo, Q.z32 + vo + vl, Q.Az32 + vo, Q.Az32 + vo + vl);
             else
                 acc = matrix_tile<float, float, false>(M, lt, Q.A32, Q.z32 + vo, Q.z32 + vo + vl, nullptr, nullptr,
                                                        Q.Az32 + vo, Q.Az32 + vo + vl);
@@ -436,19 +524,46 @@ __global__ void k_ens_pack(Dims d, long long a_off, int ncolp, const double* __r
     dst[((R / WAVE) * NCOEF + cf) * WAVE + (R % WAVE)] = ens_cvt<TA>(v);
 }
 
+// Half-storage rows of one member (compact rows, as k_ens_pack) from its
+// FP64 row-blocked matrix, rounded to FP32.
+__global__ void k_ens_pack_half(Dims d, long long a_off, int ncolp, const double* __restrict__ src,
+                                float* __restrict__ dst) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long nrow = (long long)d.nz * ncolp;
+    if (t >= nrow * 56) return;
+    const int cf = (int)(t % 56), sl = cf / 4, c4 = cf % 4;
+    const long long rr = t / 56;
+    const int k = (int)(rr / ncolp), c = (int)(rr % ncolp);
+    double v = 0.0;
+    if (c < d.nxo * d.nyo) {
+        const int io = c % d.nxo, jo = c / d.nxo;
+        const long long r = ((long long)k * d.nyo + jo) * d.nxp + io;
+        const double* row = src + (r / WAVE) * NCOEF * WAVE + (r % WAVE);
+        if (sl == 0) {
+            const int blk = c4 == 0 ? 0 : (c4 == 1 ? 1 : 3);
+            v = c4 == 3 ? 0.0 : row[(blk * 27 + 13) * WAVE];
+        } else {
+            v = row[(c4 * 27 + 13 + sl) * WAVE];
+        }
+    }
+    const long long R = rr + a_off;
+    dst[((R / WAVE) * 14 + sl) * (4 * WAVE) + (R % WAVE) * 4 + c4] = (float)v;
+}
+
 // Stand-alone FP32 products over a list of matrix tiles (autotuner).
 __global__ __launch_bounds__(ENS_BS) void k_ens_spmv_sweep(const EnsMember* __restrict__ mem,
                                                            const int* __restrict__ tiles, int ntiles,
                                                            const int* __restrict__ tile_mem, const float* __restrict__ A,
-                                                           const float* __restrict__ z, float* __restrict__ Az,
-                                                           double* __restrict__ part) {
+                                                           const float* __restrict__ S32, int half, const float* __restrict__ z,
+                                                           float* __restrict__ Az, double* __restrict__ part) {
     const int u = blockIdx.x * ENS_WPB + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     if (u >= ntiles) return;
     const int tile = tiles[u];
     const EnsMember& M = mem[tile_mem[tile]];
     const long long vo = M.v_off, vl = M.d.vlen;
-    dd acc = matrix_tile<float, float, false>(M, tile - M.t0_mat, A, z + vo, z + vo + vl, nullptr, nullptr, Az + vo,
-                                              Az + vo + vl);
+    dd acc = half ? matrix_tile_half(M, tile - M.t0_mat, S32, z + vo, z + vo + vl, Az + vo, Az + vo + vl)
+                     : matrix_tile<float, float, false>(M, tile - M.t0_mat, A, z + vo, z + vo + vl, nullptr, nullptr,
+                                                        Az + vo, Az + vo + vl);
     acc = wave_reduce(acc);
     if (threadIdx.x % WAVE == 0) { part[2 * (size_t)tile] = acc.hi; part[2 * (size_t)tile + 1] = acc.lo; }
 }

@@ -1916,6 +1916,8 @@ struct cismgpu_ensemble {
     double* A64 = nullptr;
     float* A32 = nullptr;
     __hip_bfloat16* A16 = nullptr;
+    float* S32 = nullptr;            // half (symmetric) storage of A32, when every member is exactly symmetric
+    int half = 0;                    // 1: the FP32 products use S32
     float* M32 = nullptr;
     int8_t* active = nullptr;
     double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr;
@@ -1983,7 +1985,7 @@ extern "C" int32_t cismgpu_ensemble_destroy(cismgpu_ensemble* E) {
     if (E->s) hipStreamSynchronize(E->s);
     if (E->gexec) hipGraphExecDestroy(E->gexec);
     if (E->graph) hipGraphDestroy(E->graph);
-    for (void* p : {(void*)E->d_mem, (void*)E->d_tile_mem_mat, (void*)E->d_tile_mem_col, (void*)E->A64, (void*)E->A32,
+    for (void* p : {(void*)E->d_mem, (void*)E->d_tile_mem_mat, (void*)E->d_tile_mem_col, (void*)E->A64, (void*)E->A32, (void*)E->S32,
                     (void*)E->A16, (void*)E->M32, (void*)E->active, (void*)E->b64, (void*)E->x64, (void*)E->r64,
                     (void*)E->f32, (void*)E->part_mat, (void*)E->part_col, (void*)E->st, (void*)E->sync})
         if (p) hipFree(p);
@@ -2069,6 +2071,38 @@ extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* mem
         hipMemcpyAsync(E->active + M.c_off, S->active, (size_t)d.nvx * d.nvy, hipMemcpyDeviceToDevice, E->s);
     }
     if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "build");
+    // half (symmetric) storage for the FP32 products when every member's FP32
+    // matrix is exactly symmetric (k_check_sym) and each member's part stays
+    // under 2 GiB, unless CISMGPU_SPMV_FULL=1
+    {
+        const char* env = getenv("CISMGPU_SPMV_FULL");
+        const size_t s_bytes = (size_t)rows * 56 * sizeof(float);
+        bool sym = !(env && env[0] == '1');
+        for (const EnsMember& M : E->mem)   // member-local 32-bit byte offsets
+            if ((size_t)M.d.nz * M.ncolp * 56 * sizeof(float) >= ((size_t)1 << 31)) sym = false;
+        unsigned long long* cnt = nullptr;
+        if (sym && hipMalloc(&cnt, sizeof(unsigned long long)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc sym");
+        for (int m = 0; sym && m < n; ++m) {
+            cismgpu_system* S = members[m];
+            unsigned long long h = 1;
+            if (hipMemsetAsync(cnt, 0, sizeof(h), S->s) != hipSuccess) { hipFree(cnt); return fail(CISMGPU_ERR_HIP, "sym"); }
+            k_check_sym<float><<<nblocks(S->d.nrows, 256), 256, 0, S->s>>>(S->d, S->A64, cnt);
+            if (hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
+                hipStreamSynchronize(S->s) != hipSuccess) { hipFree(cnt); return fail(CISMGPU_ERR_HIP, "sym"); }
+            sym = h == 0;
+        }
+        if (cnt) hipFree(cnt);
+        if (sym) {
+            if ((e = hipMalloc(&E->S32, s_bytes)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc S32");
+            for (int m = 0; m < n; ++m) {
+                const EnsMember& M = E->mem[m];
+                const long long nel = (long long)M.d.nz * M.ncolp * 56;
+                k_ens_pack_half<<<nblocks(nel, 256), 256, 0, E->s>>>(M.d, M.a_off, M.ncolp, members[m]->A64, E->S32);
+            }
+            if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "pack half");
+            E->half = 1;
+        }
+    }
     if (ens_tiles(E) != CISMGPU_OK) return fail(CISMGPU_ERR_HIP, "tiles");
     *out = E;
     return CISMGPU_OK;
@@ -2100,10 +2134,10 @@ extern "C" int32_t cismgpu_ensemble_autotune(cismgpu_ensemble* E, int32_t* split
                 CG_CHECK(hipMalloc(&dtl, tl.size() * sizeof(int)));
                 CG_CHECK(hipMemcpy(dtl, tl.data(), tl.size() * sizeof(int), hipMemcpyHostToDevice));
                 const unsigned g = nblocks((long long)tl.size(), ENS_WPB);
-                k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, z, Az, E->part_mat);
+                k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, E->S32, E->half, z, Az, E->part_mat);
                 hipEventRecord(e0, E->s);
                 for (int r = 0; r < 5; ++r)
-                    k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, z, Az, E->part_mat);
+                    k_ens_spmv_sweep<<<g, ENS_BS, 0, E->s>>>(E->d_mem, dtl, (int)tl.size(), E->d_tile_mem_mat, E->A32, E->S32, E->half, z, Az, E->part_mat);
                 hipEventRecord(e1, E->s);
                 hipEventSynchronize(e1);
                 float f = 0.f;
@@ -2158,6 +2192,8 @@ extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ens
     Q.part_mat = E->part_mat; Q.part_col = E->part_col; Q.st = E->st;
     Q.bar = E->sync; Q.err_flag = (int*)(E->sync + 5); Q.steps_out = (int*)(E->sync + 6);
     Q.use_bf16 = o->bf16 ? 1 : 0;
+    Q.S32 = E->S32;
+    Q.half = o->bf16 ? 0 : E->half;
     // (re)capture the graph: init kernel + persistent solve
     const bool same = E->gexec && E->ggrid == grid && E->gbf16 == Q.use_bf16 && !memcmp(&E->gparams, &P, sizeof(P));
     if (!same) {
@@ -2185,6 +2221,7 @@ extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ens
     hipEventDestroy(e1);
     info->solve_ms = ms;
     info->grid_blocks = grid;
+    info->matrix_storage = o->bf16 ? 2 : (Q.half ? 3 : 1);
     unsigned sy[8];
     CG_CHECK(hipMemcpy(sy, E->sync, sizeof(sy), hipMemcpyDeviceToHost));
     info->barrier_error = ((int*)sy)[5] ? 1 : 0;

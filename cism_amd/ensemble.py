@@ -25,7 +25,8 @@ class EnsOpts(ctypes.Structure):
 class EnsInfo(ctypes.Structure):
     _fields_ = [("solve_ms", ctypes.c_double), ("steps", ctypes.c_int32), ("converged", ctypes.c_int32),
                 ("max_outer", ctypes.c_int32), ("max_inner", ctypes.c_int32), ("total_inner", ctypes.c_int64),
-                ("max_err", ctypes.c_double), ("barrier_error", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
+                ("max_err", ctypes.c_double), ("barrier_error", ctypes.c_int32), ("grid_blocks", ctypes.c_int32),
+                ("matrix_storage", ctypes.c_int32)]
 
 
 def _declare(L):

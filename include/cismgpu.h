@@ -387,6 +387,8 @@ typedef struct cismgpu_ensemble_info {
     double  max_err;          /* largest final outer relative residual */
     int32_t barrier_error;    /* 1 if a grid barrier timed out (result invalid) */
     int32_t grid_blocks;      /* persistent workgroups launched */
+    int32_t matrix_storage;   /* inner products: 1 FP32 quad rows, 2 BF16 quad rows, 3 FP32 half
+                                 (symmetric) storage, used when every member is exactly symmetric */
 } cismgpu_ensemble_info;
 
 int32_t cismgpu_ensemble_create(int32_t nmembers, cismgpu_system* const* members, cismgpu_ensemble** out);

@@ -85,6 +85,35 @@ def test_ensemble_autotune(ens):
         assert _rel(E, refs, m) < TOL
 
 
+def test_ensemble_half_storage_identical(require_gpu):
+    """Every member's FP32 matrix is exactly symmetric, so the products keep
+    each row's centre and 13 upper couplings (224 B per node instead of 432)
+    and read the lower couplings transposed from the partner rows; they sum
+    the same products in the same order, so the solve is bit-identical to the
+    one on full quad rows (CISMGPU_SPMV_FULL=1)."""
+    import os
+    out = []
+    for flag in ("1", "0"):
+        os.environ["CISMGPU_SPMV_FULL"] = flag
+        try:
+            E = Ensemble(_members())
+        finally:
+            os.environ.pop("CISMGPU_SPMV_FULL", None)
+        try:
+            info = E.solve(tolerance=1e-10, inner_tol=1e-3, inner_maxiters=400, max_outer=30)
+            xs = [np.stack(E.get_x(m)) for m in range(len(E.problems))]
+            ib = E.solve(tolerance=1e-10, inner_tol=1e-3, inner_maxiters=400, max_outer=30, bf16=True)
+            out.append((info, xs, ib["matrix_storage"]))
+        finally:
+            E.close()
+    (i_full, x_full, b_full), (i_half, x_half, b_half) = out
+    assert i_full["matrix_storage"] == 1 and i_half["matrix_storage"] == 3 and b_full == b_half == 2
+    assert i_half["converged"] == i_full["converged"] == len(x_full)
+    assert i_half["steps"] == i_full["steps"] and i_half["total_inner"] == i_full["total_inner"]
+    for a, b in zip(x_half, x_full):
+        assert np.array_equal(a, b)
+
+
 def test_ensemble_rejects_deep_columns(require_gpu):
     p = synthetic.ice_stream(20, nz=17, seed=1)
     with pytest.raises(CismGpuError):
