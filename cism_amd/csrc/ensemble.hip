@@ -249,8 +249,9 @@ __device__ __forceinline__ dd matrix_tile_half(const EnsMember& M, int lt, const
         const int R = k * M.ncolp + c;     // member-local row
         const long long c0 = vidx(d, k, i, j);
         float su = 0.f, sv = 0.f;
-        // one row of three neighbours (fixed kA, jA) in flight at a time
-#pragma unroll 1
+        // rows of three neighbours (fixed kA, jA), unrolled by three: more
+        // loads in flight would spill past the kernel's 128 VGPRs
+#pragma unroll 3
         for (int rw = 0; rw < 9; ++rw) {
             const int kA = rw / 3, jq = rw % 3;
             float xa[3], xb[3], cuu[3], cuv[3], cvu[3], cvv[3];
