@@ -2044,33 +2044,6 @@ extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* mem
     E->ctotal = coff;
     const size_t na = (size_t)rows * NCOEF;
     hipError_t e = hipSuccess;
-    if ((e = hipMalloc(&E->d_mem, n * sizeof(EnsMember))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc");
-    if ((e = hipMalloc(&E->A64, na * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A64");
-    if ((e = hipMalloc(&E->A32, na * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A32");
-    if ((e = hipMalloc(&E->M32, 3 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc M");
-    if ((e = hipMalloc(&E->active, coff)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc active");
-    if ((e = hipMalloc(&E->b64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc b");
-    if ((e = hipMalloc(&E->x64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc x");
-    if ((e = hipMalloc(&E->r64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc r");
-    if ((e = hipMalloc(&E->f32, 6 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc f32");
-    if ((e = hipMalloc(&E->st, 2 * n * sizeof(EnsState))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc st");
-    if ((e = hipMalloc(&E->sync, 8 * sizeof(unsigned))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc sync");
-    hipMemsetAsync(E->f32, 0, 6 * (size_t)voff * sizeof(float), E->s);
-    hipMemsetAsync(E->r64, 0, voff * sizeof(double), E->s);
-    hipMemsetAsync(E->x64, 0, voff * sizeof(double), E->s);
-    for (int m = 0; m < n; ++m) {
-        cismgpu_system* S = members[m];
-        const EnsMember& M = E->mem[m];
-        const Dims& d = M.d;
-        const long long nel = (long long)d.nz * M.ncolp * NCOEF;
-        hipStreamSynchronize(S->s);
-        k_ens_pack<double><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A64);
-        k_ens_pack<float><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A32);
-        k_cast<double, float><<<nblocks(6 * d.vlen, 256), 256, 0, E->s>>>(6 * d.vlen, S->M64, E->M32 + 3 * M.v_off);
-        hipMemcpyAsync(E->b64 + M.v_off, S->V(2), 2 * d.vlen * sizeof(double), hipMemcpyDeviceToDevice, E->s);
-        hipMemcpyAsync(E->active + M.c_off, S->active, (size_t)d.nvx * d.nvy, hipMemcpyDeviceToDevice, E->s);
-    }
-    if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "build");
     // half (symmetric) storage for the FP32 products when every member's FP32
     // matrix is exactly symmetric (k_check_sym) and each member's part stays
     // under 2 GiB, unless CISMGPU_SPMV_FULL=1
@@ -2103,6 +2076,33 @@ extern "C" int32_t cismgpu_ensemble_create(int32_t n, cismgpu_system* const* mem
             E->half = 1;
         }
     }
+    if ((e = hipMalloc(&E->d_mem, n * sizeof(EnsMember))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc");
+    if ((e = hipMalloc(&E->A64, na * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A64");
+    if (!E->half && (e = hipMalloc(&E->A32, na * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc A32");
+    if ((e = hipMalloc(&E->M32, 3 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc M");
+    if ((e = hipMalloc(&E->active, coff)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc active");
+    if ((e = hipMalloc(&E->b64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc b");
+    if ((e = hipMalloc(&E->x64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc x");
+    if ((e = hipMalloc(&E->r64, voff * sizeof(double))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc r");
+    if ((e = hipMalloc(&E->f32, 6 * (size_t)voff * sizeof(float))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc f32");
+    if ((e = hipMalloc(&E->st, 2 * n * sizeof(EnsState))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc st");
+    if ((e = hipMalloc(&E->sync, 8 * sizeof(unsigned))) != hipSuccess) return fail(CISMGPU_ERR_HIP, "alloc sync");
+    hipMemsetAsync(E->f32, 0, 6 * (size_t)voff * sizeof(float), E->s);
+    hipMemsetAsync(E->r64, 0, voff * sizeof(double), E->s);
+    hipMemsetAsync(E->x64, 0, voff * sizeof(double), E->s);
+    for (int m = 0; m < n; ++m) {
+        cismgpu_system* S = members[m];
+        const EnsMember& M = E->mem[m];
+        const Dims& d = M.d;
+        const long long nel = (long long)d.nz * M.ncolp * NCOEF;
+        hipStreamSynchronize(S->s);
+        k_ens_pack<double><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A64);
+        if (!E->half) k_ens_pack<float><<<nblocks(nel, 256), 256, 0, E->s>>>(d, M.a_off, M.ncolp, S->A64, E->A32);
+        k_cast<double, float><<<nblocks(6 * d.vlen, 256), 256, 0, E->s>>>(6 * d.vlen, S->M64, E->M32 + 3 * M.v_off);
+        hipMemcpyAsync(E->b64 + M.v_off, S->V(2), 2 * d.vlen * sizeof(double), hipMemcpyDeviceToDevice, E->s);
+        hipMemcpyAsync(E->active + M.c_off, S->active, (size_t)d.nvx * d.nvy, hipMemcpyDeviceToDevice, E->s);
+    }
+    if ((e = hipStreamSynchronize(E->s)) != hipSuccess) return fail(CISMGPU_ERR_HIP, "build");
     if (ens_tiles(E) != CISMGPU_OK) return fail(CISMGPU_ERR_HIP, "tiles");
     *out = E;
     return CISMGPU_OK;
