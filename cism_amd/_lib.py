@@ -38,7 +38,7 @@ EXPORTS = (
     "cismgpu_system_work_vectors", "cismgpu_system_vector_op", "cismgpu_system_residual_velocity",
     "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d", "cismgpu_matvec_multiply_structured_2d",
     "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d", "cismgpu_system_assemble_2d", "cismgpu_system_diva_setup",
-    "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity",
+    "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity", "cismgpu_half_storage_supported",
 )
 
 
@@ -99,6 +99,8 @@ def _declare(L):
     L.cismgpu_system_matvec.argtypes = [vp, _D, _D]
     L.cismgpu_system_bench_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _D, _D]
     L.cismgpu_system_zero_x.argtypes = [vp]
+    L.cismgpu_half_storage_supported.argtypes = [_G]
+    L.cismgpu_half_storage_supported.restype = ctypes.c_int32
     L.cismgpu_bench_stream.argtypes = [ctypes.c_int64, ctypes.c_int32, _D, _D, _D]
     L.cismgpu_bench_stream.restype = ctypes.c_int32
     for n in ("cismgpu_system_zero_x", "cismgpu_system_create", "cismgpu_system_destroy", "cismgpu_system_set_matrix",

@@ -566,7 +566,7 @@ int ensure_half64(cismgpu_system* S) {
     S->a64_sym = false;
     const Dims& d = S->d;
     const char* env = getenv("CISMGPU_SPMV_FULL");
-    if (S->two_d || (env && env[0] == '1')) return CISMGPU_OK;
+    if (S->two_d || (env && env[0] == '1') || !half_storage_fits(d)) return CISMGPU_OK;
     unsigned long long* cnt = nullptr;
     unsigned long long h = 1;
     CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
@@ -597,7 +597,7 @@ int ensure_lowp(cismgpu_system* S, int precision) {
     auto sym_ok = [&](bool& ok) -> int {
         ok = false;
         const char* env = getenv("CISMGPU_SPMV_FULL");
-        if (S->two_d || (env && env[0] == '1')) return CISMGPU_OK;
+        if (S->two_d || (env && env[0] == '1') || !half_storage_fits(d)) return CISMGPU_OK;
         unsigned long long* cnt = nullptr;
         unsigned long long h = 1;
         CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
@@ -630,7 +630,9 @@ int ensure_lowp(cismgpu_system* S, int precision) {
 // Mixed precision: FP64 iterative refinement around an FP32 (or BF16-matrix)
 // Chronopoulos-Gear inner solve.  The outer residual r = b - A x uses the
 // FP64 matrix and compensated FP64 sums; the correction e solves A e = r in
-// FP32 to inner_tol relative (recurrence residual), then x += e.
+// FP32 to inner_tol relative (recurrence residual), then x += e.  The
+// refinement stops at the tolerance, after max_outer corrections, or when
+// the inner iterations of all corrections reach maxiters.
 int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_info* info) {
     const Dims& d = S->d;
     int st = ensure_lowp(S, o->precision);
@@ -664,7 +666,7 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
         if (st) return st;
         err = std::sqrt(S->scal_h[S_DOT]) / l2rhs;
         if (!(err == err)) { cismgpu_set_error("refinement residual is NaN"); return CISMGPU_ERR_NAN; }
-        if (err < o->tolerance || outer == std::max(o->max_outer, 1)) break;
+        if (err < o->tolerance || outer == std::max(o->max_outer, 1) || inner_total >= o->maxiters) break;
         // inner solve A e = r in FP32
         k_cast<double, float><<<nblocks(d.vlen, 256), 256, 0, S->s>>>(d.vlen, ru, w[2]);
         k_cast<double, float><<<nblocks(d.vlen, 256), 256, 0, S->s>>>(d.vlen, rv, w[3]);
@@ -672,12 +674,16 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
         CG_CHECK(hipMemsetAsync(w[1], 0, nvf, S->s));
         double ierr = 0.0;
         int nit = 0;
+        // maxiters bounds the inner iterations of the whole solve, as it
+        // bounds the iterations of the FP64 solve: each inner solve may use
+        // what the earlier ones left
+        const int budget = o->maxiters - inner_total;
         if (o->precision == CISMGPU_MIXED_BF16)
             st = cg_solve<__hip_bfloat16, float>(S, S->A16, w, nullptr, nullptr, o->precond, o->ncheck, inner_tol,
-                                                 o->maxiters, true, &ierr, &nit, true);
+                                                 budget, true, &ierr, &nit, true);
         else
             st = cg_solve<float, float>(S, S->A32, w, nullptr, nullptr, o->precond, o->ncheck, inner_tol,
-                                        o->maxiters, true, &ierr, &nit, true);
+                                        budget, true, &ierr, &nit, true);
         if (st) return st;
         inner_total += nit;
         k_ir_update<<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, xu, xv, w[0], w[1]);
@@ -1060,6 +1066,12 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     const double ncoef = S->two_d ? 36.0 : (half ? 4.0 * NSLOT : (double)NCOEF);
     if (bytes_per_launch) *bytes_per_launch = (double)nodes * (ncoef * esz + 4.0 * vsz);
     return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_half_storage_supported(const cismgpu_grid* grid) {
+    if (check_grid(grid)) return 0;
+    const Dims d = make_dims(grid);
+    return half_storage_fits(d) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------- HBM probes (STREAM-style)
@@ -1733,7 +1745,8 @@ extern "C" int32_t cismgpu_system_residual(cismgpu_system* S, double* ru, double
     int st = sync_scal(S);
     if (st) return st;
     if (L2) *L2 = std::sqrt(S->scal_h[S_DOT]);
-    if (L2rel) *L2rel = std::sqrt(S->scal_h[S_DOT]) / S->scal_h[S_L2RHS];
+    // |Ax - b| / |b|, and 0 when b = 0 (glissade_velo_higher.F90:9881-9885)
+    if (L2rel) *L2rel = S->scal_h[S_L2RHS] > 0.0 ? std::sqrt(S->scal_h[S_DOT]) / S->scal_h[S_L2RHS] : 0.0;
     if (ru && rv) {
         const long long nvec = (long long)d.nz * d.nvx * d.nvy;
         std::vector<double> h(nvec);

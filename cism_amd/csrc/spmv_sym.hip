@@ -20,15 +20,25 @@
 // products are summed in the same order (m = 0..26) with the same values as
 // the full-storage kernel, so when the FP32 matrix is exactly symmetric
 // (k_check_sym finds no mismatch) all three kernels give identical results;
-// the solver uses this layout only then.  x is read per neighbour (L1/L2 hits)
-// instead of being kept in a 3-level register window: the window cost 54
+// the solver uses this layout only then.  x is staged in LDS per 64 x 4
+// tile with a one-vertex ring: a 3-level ring of slots in k_spmv_sym and
+// k_spmv_sym64, a 4-level ring in k_spmv_symx (so that one barrier per
+// level less is needed).  A 3-level x window in registers instead cost 54
 // VGPRs and limited the kernel to 3 waves per SIMD (0.87 ms against 0.73 ms
 // at 1024^2 x 11).
+//
+// Partner offsets within a level are 32-bit (po[] below): the half layouts
+// are used only while one level of slots, nyo * nxp * 56 elements, stays
+// below 2^31 (half_storage_fits); larger grids keep the full-row kernels.
 #include "cismgpu_internal.h"
 
 namespace cg {
 
 constexpr int NSLOT = 14;
+
+inline bool half_storage_fits(const Dims& d) {
+    return (long long)d.nyo * d.nxp * NSLOT * 4 < (1LL << 31);
+}
 
 // partner column of row (i, j) at horizontal offset (iA, jA): its owned
 // image (periodic rules) as jo*nxp + io, or -1 if it has no stored row
@@ -136,7 +146,7 @@ __global__ __launch_bounds__(256) void k_spmv_sym(Dims d, const TA* __restrict__
     };
     double acc = 0.0;
     int po[9];
-    const int LS = d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);   // elements per level
+    const long long LS = (long long)d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);   // elements per level
     if (live) {
         const int i = d.ilo + io, j = d.jlo + jo;
         // element offsets of the partner rows' slot 0 within a level, -1 when
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(256) void k_spmv_symx(Dims d, const float* __restri
     double acc = 0.0;
     int po[9];
     unsigned inl = 0;   // bit h: partner h is the tile's own neighbouring thread
-    const int LS = d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);
+    const long long LS = (long long)d.nyo * d.nxp / WAVE * (NSLOT * 4 * WAVE);
     if (live) {
         const int i = d.ilo + io, j = d.jlo + jo;
 #pragma unroll

@@ -92,10 +92,14 @@ class PicardResult:
     L2_norm: list = field(default_factory=list)
     linear_iters: list = field(default_factory=list)
     linear_err: list = field(default_factory=list)
+    linear_status: list = field(default_factory=list)   # 0, or CISMGPU_ERR_NOT_CONVERGED per linear solve
+    linear_outer: list = field(default_factory=list)    # mixed solves: refinement steps per linear solve
     assemble_ms: list = field(default_factory=list)
     solve_ms: list = field(default_factory=list)
     alpha: list = field(default_factory=list)      # accelerated Picard: alpha after each evaluation
 
+
+NOT_CONVERGED = 12      # CISMGPU_ERR_NOT_CONVERGED: mixed refinement stopped before its tolerance
 
 # accelerated Picard constants (glissade_velo_higher.F90:831, 1049-1056)
 L2_NORM_LARGE = 1.0e20
@@ -299,7 +303,14 @@ class BPSolver:
                     done = True
             si = self.sys.solve(algo=algo, precond=precond, ncheck=ncheck, maxiters=linear_maxiters,
                                 tolerance=linear_tol, precision=precision, inner_tol=inner_tol, max_outer=max_outer,
-                                raise_on_error=precision == FP64)
+                                raise_on_error=False)
+            # A linear solve that stops short of its tolerance is not an error
+            # (the reference's solvers return niters = maxiters silently,
+            # glissade_velo_higher_pcg.F90:1196, :3047); every other status is.
+            if si["status"] not in (0, NOT_CONVERGED):
+                raise _lib.CismGpuError(si["status"], f"Picard iteration {counter}: linear solve")
+            res.linear_status.append(si["status"])
+            res.linear_outer.append(si["outer_iters"])
             res.linear_iters.append(si["niters"])
             res.linear_err.append(si["err"])
             res.solve_ms.append(si["solve_ms"])

@@ -137,7 +137,7 @@ typedef struct cismgpu_solve_opts {
     int32_t algo;            /* CISMGPU_ALGO_STANDARD or CISMGPU_ALGO_CHRONGEAR */
     int32_t precond;         /* 0 none, 1 diag, 2 SIA column, 3 tridiag local, 4 tridiag global (one task) */
     int32_t ncheck;          /* linear_solve_ncheck */
-    int32_t maxiters;        /* linear_maxiters (per inner solve in mixed mode) */
+    int32_t maxiters;        /* linear_maxiters (mixed mode: inner iterations summed over all corrections) */
     double  tolerance;       /* relative residual ||b-Ax||/||b|| target */
     int32_t precision;       /* CISMGPU_FP64 / CISMGPU_MIXED_FP32 / CISMGPU_MIXED_BF16 */
     double  inner_tol;       /* mixed: inner relative residual target per refinement step */
@@ -196,6 +196,12 @@ int32_t cismgpu_system_matvec(cismgpu_system* sys, double* yu, double* yv);
 int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nrep,
                                   double* avg_ms, double* bytes_per_launch);
 
+
+/* 1 when the half (symmetric) storage layouts can hold this grid's owned
+ * rows, 0 when the system keeps full rows whatever the symmetry (one level
+ * of half-storage slots, nyo * nxp * 56 elements, must stay below 2^31).
+ * Host-only query, no device work; no reference counterpart. */
+int32_t cismgpu_half_storage_supported(const cismgpu_grid* grid);
 
 /* HBM bandwidth probes (no reference counterpart; measurement only):
  * STREAM-style copy (2 x nbytes moved), triad (3 x nbytes) and read

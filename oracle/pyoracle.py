@@ -665,7 +665,8 @@ def residual_L2(grid, active_vertex, A, bu, bv, uvel, vvel):
     L2 = np.sqrt(global_sum(grid, wu, wv))
     bu2 = np.zeros(grid.vec_shape, order="F"); bv2 = np.zeros(grid.vec_shape, order="F")
     bu2[own] = np.where(act, bu[own] * bu[own], 0.0); bv2[own] = np.where(act, bv[own] * bv[own], 0.0)
-    return L2, L2 / np.sqrt(global_sum(grid, bu2, bv2))
+    rhs = np.sqrt(global_sum(grid, bu2, bv2))
+    return L2, (L2 / rhs if rhs > 0.0 else 0.0)      # (:9881-9885)
 
 
 def residual_L2_2d(grid, active_vertex, A, bu, bv, uvel, vvel):

@@ -74,3 +74,22 @@ def test_2d_wrappers_check_shapes_before_any_device_call():
     A2 = [np.zeros((g.nvx, g.nvy, 9), order="F") for _ in range(4)]
     with pytest.raises(ValueError):
         pcg.matvec_multiply_structured_2d(g, act, *A2, v3, v3)
+
+
+def test_half_storage_size_guard():
+    """The half (symmetric) storage SpMVs index one level of slots with
+    32-bit offsets, so they are chosen only while nyo * nxp * 56 < 2^31;
+    larger grids keep the full-row kernels (host-only query, no device)."""
+    L = _lib.load()
+
+    def ok(n, m, nz=11):
+        return L.cismgpu_half_storage_supported(ctypes.byref(Grid.from_owned(n, m, nz).cstruct()))
+
+    assert ok(1024, 1024) == 1 and ok(6000, 6000) == 1
+    # 6208 x 6208: nxp = 6208, 6208 * 6208 * 56 = 2.158e9 >= 2^31
+    assert ok(6208, 6208) == 0
+    # the owned row width is padded to 64: 6145 owned columns count as 6208
+    assert ok(6145, 6208) == 0 and ok(6144, 6208) == 1
+    g = Grid.from_owned(8, 8, 5)
+    g.ilo = 1
+    assert L.cismgpu_half_storage_supported(ctypes.byref(g.cstruct())) == 0

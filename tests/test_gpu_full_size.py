@@ -98,6 +98,33 @@ def test_mixed_within_1e6_and_reproducible(stream1024):
     assert r2["niters"] == r1["niters"] and np.array_equal(x2, x1)
 
 
+def test_fp64_and_mixed_against_the_oracle_full_size(stream1024):
+    """The oracle's all-cores FP64 Chronopoulos-Gear solve of the exported
+    1024^2 x 11 system to 1e-10 (about 20 s on 16 host threads): the GPU FP64
+    drop-in takes the same number of iterations (within one convergence
+    check) and both GPU solutions sit within the bar of the oracle's."""
+    from oracle import pyoracle as po
+    prob, bp, _ = stream1024
+    s, g = bp.sys, prob.grid
+    (Auu, Auv, Avu, Avv), (bu, bv) = s.get_matrix()
+    ou, ov = g.zeros_vec(), g.zeros_vec()
+    nt = min(os.cpu_count() or 1, 16)
+    st, oerr, onit = po.pcg_solve(g, bp.active_vertex, Auu, Auv, Avu, Avv, bu, bv, ou, ov, precond=2,
+                                  tolerance=1e-10, maxiters=5000, nthreads=nt)
+    del Auu, Auv, Avu, Avv
+    assert st == 0 and oerr <= 1e-10
+    xo = np.stack([ou, ov])
+    s.zero_x()
+    r = s.solve(precond=2, tolerance=1e-10, maxiters=50000)
+    x64 = np.stack(s.get_x())
+    rm, xm = _mixed(s)
+    e64, em = _rel(x64, xo), _rel(xm, xo)
+    print(f"oracle ({nt} threads): {onit} it, err {oerr:.3e}; GPU fp64 {r['niters']} it, vs oracle {e64:.3e}; "
+          f"mixed vs oracle {em:.3e}")
+    assert abs(r["niters"] - onit) <= 5
+    assert e64 < 1e-8 and em <= 1e-6
+
+
 def test_spmv_variants_identical_full_size(stream1024):
     prob, bp, _ = stream1024
     r_x, x_x = _mixed(bp.sys)

@@ -1,0 +1,312 @@
+"""Pins of the CPU restatement's auxiliary functions (oracle/) that the
+main pin file leaves to product-vs-oracle comparisons: the Picard
+convergence measures, the nonzero count, the all-cores C-G baseline, the
+thickness ramp of the surface gradient, and the host-geometry operators
+(lower surface, masks, slope angles, cell halo).
+
+Each expectation comes from a source independent of the restated code:
+numpy reductions written from the definition, Archimedes' floatation
+condition, planes whose slopes are known in closed form, the serial
+solver's iterates, and halo index arithmetic.
+"""
+import numpy as np
+import pytest
+
+from cism_amd import synthetic
+from cism_amd.grid import BC_NO_ICE, BC_OUTFLOW, BC_PERIODIC, Grid
+from oracle import pyoracle as po
+from systems import random_spd_system, stream_system
+
+RHOI, RHOO = 910.0, 1028.0
+
+
+# ----------------------------------------------------------------- Picard convergence measures
+def _speeds(u, v):
+    return np.sqrt(u * u + v * v)
+
+
+def _relative_speed_change(u0, v0, u1, v1):
+    """|(|u_old| - |u|) / |u|| where |u| != 0, NaN where |u| = 0."""
+    s1 = _speeds(u1, v1)
+    s0 = _speeds(u0, v0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.abs((s0 - s1) / s1)
+    return np.where(s1 != 0.0, d, np.nan)
+
+
+@pytest.mark.parametrize("whichresid", [0, 1, 2, 3, 4])
+def test_residual_velocity_3d_is_max_or_mean_relative_speed_change(whichresid):
+    """compute_residual_velocity_3d: maximum (0, 1, 3, 4) or mean (2) over
+    owned vertices of the relative speed change; 1 and 2 leave out the bed
+    level; zero speeds are skipped.  Planted: a huge change in a halo column
+    (never counted) and at the bed (counted by 0/3/4 only)."""
+    g = Grid.from_owned(9, 7, 6, global_bc=BC_NO_ICE)
+    rng = np.random.default_rng(11 + whichresid)
+    u0, v0, u1, v1 = (np.asfortranarray(rng.standard_normal(g.vec_shape)) for _ in range(4))
+    u1[:, 4, 4] = 0.0
+    v1[:, 4, 4] = 0.0                    # zero speed: skipped, not a division by zero
+    u1[2, 0, 3] = 1e-9                   # halo column, outside the owned range
+    ib, jb = g.ilo + 1, g.jlo + 2
+    u1[g.nz - 1, ib - 1, jb - 1] = 1e-7  # bed node of an owned column
+    v1[g.nz - 1, ib - 1, jb - 1] = 0.0
+    got = po.lib().orc_residual_velocity_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                            po.dp(u0), po.dp(v0), po.dp(u1), po.dp(v1))
+    d = _relative_speed_change(u0, v0, u1, v1)[g.owned_slice()]
+    if whichresid in (1, 2):
+        d = d[:-1]
+    d = d[~np.isnan(d)]
+    if whichresid == 2:
+        assert got == pytest.approx(float(d.mean()), rel=1e-14)
+    else:
+        assert got == float(d.max())
+        if whichresid == 1:
+            assert got < 1e6
+        else:
+            assert got > 1e6            # the planted bed change dominates
+
+
+@pytest.mark.parametrize("whichresid", [0, 1, 2])
+def test_residual_velocity_2d_is_max_or_mean_relative_speed_change(whichresid):
+    g = Grid.from_owned(10, 8, 1, global_bc=BC_OUTFLOW)
+    rng = np.random.default_rng(7)
+    u0, v0, u1, v1 = (np.asfortranarray(rng.standard_normal((g.nvx, g.nvy))) for _ in range(4))
+    u1[3, 3] = v1[3, 3] = 0.0
+    u1[0, 2] = 1e-9                      # halo vertex
+    got = po.lib().orc_residual_velocity_2d(g.nx, g.ny, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
+                                            po.dp(u0), po.dp(v0), po.dp(u1), po.dp(v1))
+    d = _relative_speed_change(u0, v0, u1, v1)[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi]
+    d = d[~np.isnan(d)]
+    if whichresid == 2:
+        assert got == pytest.approx(float(d.mean()), rel=1e-14)
+    else:
+        assert got == float(d.max()) and got < 1e6
+
+
+# ----------------------------------------------------------------- nonzero count
+def test_count_nonzeros_3d_and_2d_equal_numpy():
+    p, s = stream_system()
+    g = p.grid
+    own = g.owned_slice()
+    act = np.asarray(s["active_vertex"], bool)[own[1], own[2]]
+    want = sum(int(np.count_nonzero(s[k][:, :, own[1], own[2]][:, :, act])) for k in ("Auu", "Auv", "Avu", "Avv"))
+    assert want > 0 and s["nnz"] == want
+    # a planted extra entry in an inactive row is not counted, one in an active row is
+    A = [s[k].copy(order="F") for k in ("Auu", "Auv", "Avu", "Avv")]
+    inact = np.argwhere(~act)
+    ii, jj = inact[0] + [g.ilo - 1, g.jlo - 1]
+    A[1][5, 2, ii, jj] = 1.0
+    L = po.lib()
+    n1 = L.orc_count_nonzeros_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, po.ip(po.as_mask(s["active_vertex"])),
+                                 *[po.dp(a) for a in A])
+    assert n1 == want
+    ia, ja = np.argwhere(act)[0] + [g.ilo - 1, g.jlo - 1]
+    z = np.argwhere(A[2][:, :, ia, ja] == 0.0)[0]
+    A[2][z[0], z[1], ia, ja] = 2.0
+    n2 = L.orc_count_nonzeros_3d(g.nx, g.ny, g.nz, g.ilo, g.ihi, g.jlo, g.jhi, po.ip(po.as_mask(s["active_vertex"])),
+                                 *[po.dp(a) for a in A])
+    assert n2 == want + 1
+    # 2-D stencils (nx-1, ny-1, 9)
+    rng = np.random.default_rng(3)
+    g2 = Grid.from_owned(8, 9, 1, global_bc=BC_NO_ICE)
+    B = [np.asfortranarray(rng.standard_normal((g2.nvx, g2.nvy, 9)) * (rng.random((g2.nvx, g2.nvy, 9)) < 0.6))
+         for _ in range(4)]
+    a2 = np.asfortranarray((rng.random((g2.nvx, g2.nvy)) < 0.7).astype(np.int32))
+    il, jl = slice(g2.ilo - 1, g2.ihi), slice(g2.jlo - 1, g2.jhi)
+    m = a2[il, jl].astype(bool)
+    want2 = sum(int(np.count_nonzero(b[il, jl][m])) for b in B)
+    got2 = L.orc_count_nonzeros_2d(g2.nx, g2.ny, g2.ilo, g2.ihi, g2.jlo, g2.jhi, po.ip(a2), *[po.dp(b) for b in B])
+    assert got2 == want2
+
+
+# ----------------------------------------------------------------- all-cores C-G baseline
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_omp_chrongear_iterates_equal_serial(precond):
+    """The all-cores C-G (the CPU baseline's multi-core leg) follows the serial
+    solver iterate by iterate: same exit iteration and err, iterates equal to
+    rounding (its sums are per-row partials combined in order), and the same
+    result for any thread count."""
+    p, s = stream_system()
+    g = p.grid
+    A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
+    for n in (1, 2, 5, 6, 13):
+        xs = [g.zeros_vec(), g.zeros_vec()]
+        st, es, ns = po.pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], *xs, precond=precond,
+                                  tolerance=0.0, maxiters=n)
+        outs = []
+        for nt in (1, 3, 8):
+            xo = [g.zeros_vec(), g.zeros_vec()]
+            st2, eo, no = po.pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], *xo, precond=precond,
+                                       tolerance=0.0, maxiters=n, nthreads=nt)
+            assert st == st2 == 0 and ns == no == n
+            if n >= 5:
+                assert eo == pytest.approx(es, rel=1e-10)
+            outs.append(np.stack(xo))
+        ref = np.stack(xs)
+        rel = np.linalg.norm(outs[0] - ref) / np.linalg.norm(ref)
+        assert rel < 1e-12, (n, rel)
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    # converged solutions agree to the solver tolerance
+    xs = [g.zeros_vec(), g.zeros_vec()]
+    _, es, ns = po.pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], *xs, precond=precond,
+                             tolerance=1e-10, maxiters=5000)
+    xo = [g.zeros_vec(), g.zeros_vec()]
+    _, eo, no = po.pcg_solve(g, s["active_vertex"], *A, s["bu"], s["bv"], *xo, precond=precond,
+                             tolerance=1e-10, maxiters=5000, nthreads=4)
+    assert abs(ns - no) <= 5 and eo < 1e-10
+    assert np.linalg.norm(np.stack(xo) - np.stack(xs)) / np.linalg.norm(np.stack(xs)) < 1e-9
+
+
+# ----------------------------------------------------------------- surface gradient: thickness ramp
+def _gradient(nx, ny, dx, dy, ice, land, usrf, thck, thklim, ramp, margin=1):
+    gx = np.zeros((nx - 1, ny - 1), order="F")
+    gy = np.zeros((nx - 1, ny - 1), order="F")
+    po.lib().orc_surface_elevation_gradient(nx, ny, dx, dy, po.ip(ice), po.ip(land), po.dp(usrf), po.dp(thck),
+                                            thklim, ramp, margin, 0.0, po.dp(gx), po.dp(gy))
+    return gx, gy
+
+
+@pytest.mark.parametrize("slope", [2.0e-3, -3.0e-3])
+def test_surface_gradient_thickness_ramp_closed_form(slope):
+    """Hybrid margin with thck_gradient_ramp > 0: on a plane s = a x, an edge
+    whose upper cell is land-based has gradient clamp((H_up - thklim)/ramp,
+    0, 1) * a; the vertex gradient averages the two edges it sits between.
+    Without land (marine-based upper cells) the ramp does nothing."""
+    nx, ny, dx, dy = 14, 11, 1000.0, 1500.0
+    thklim, ramp = 10.0, 40.0
+    x = (np.arange(nx) + 0.5) * dx
+    usrf = np.asfortranarray(np.repeat((500.0 + slope * x)[:, None], ny, axis=1))
+    rng = np.random.default_rng(2)
+    thck = np.asfortranarray(rng.uniform(thklim - 15.0, thklim + 2.0 * ramp, (nx, ny)))
+    ice = np.ones((nx, ny), np.int32, order="F")
+    land = np.ones((nx, ny), np.int32, order="F")
+    gx, gy = _gradient(nx, ny, dx, dy, ice, land, usrf, thck, thklim, ramp)
+    up = 1 if slope > 0 else 0          # the upper cell of edge (i, i+1)
+    fac = np.clip((thck - thklim) / ramp, 0.0, 1.0)
+    edge = fac[up:nx - 1 + up, :] * slope                        # (nx-1, ny)
+    want = 0.5 * (edge[:, :-1] + edge[:, 1:])
+    assert np.allclose(gx, want, rtol=1e-12, atol=1e-18)
+    assert np.abs(gy).max() < 1e-15
+    assert (fac == 0).any() and ((fac > 0) & (fac < 1)).any() and (fac == 1).any()
+    # marine-based everywhere: the plain plane slope
+    gx0, _ = _gradient(nx, ny, dx, dy, ice, np.zeros_like(land), usrf, thck, thklim, ramp)
+    assert np.allclose(gx0, slope, rtol=1e-12)
+    # ramp = 0 disables the factor on land too
+    gx1, _ = _gradient(nx, ny, dx, dy, ice, land, usrf, thck, thklim, 0.0)
+    assert np.allclose(gx1, slope, rtol=1e-12)
+
+
+# ----------------------------------------------------------------- host geometry operators
+@pytest.mark.parametrize("eus", [0.0, 3.0])
+def test_lower_surface_is_archimedes_floatation(eus):
+    """glide_calclsrf: grounded ice sits on the bed; floating ice displaces
+    its own weight of sea water, rho_i H = rho_o (0 - lsrf).  The floatation
+    test compares the bed with sea level eus, but the floating base is
+    measured from 0 whatever eus is (glide_thck.F90:1070-1074), so with
+    eus != 0 the base stays at -rho_i/rho_o H."""
+    nx, ny = 7, 5
+    rng = np.random.default_rng(4)
+    thck = np.asfortranarray(rng.uniform(0.0, 800.0, (nx, ny)))
+    topg = np.asfortranarray(rng.uniform(-900.0, 300.0, (nx, ny)))
+    lsrf = np.zeros((nx, ny), order="F")
+    po.lib().orc_calclsrf(nx, ny, po.dp(thck), po.dp(topg), eus, po.dp(lsrf))
+    floating = RHOO * (eus - topg) > RHOI * thck
+    assert floating.any() and (~floating).any()
+    assert np.allclose(lsrf[floating], -RHOI * thck[floating] / RHOO, rtol=0, atol=1e-9)
+    assert np.array_equal(lsrf[~floating], topg[~floating])
+
+
+def test_masks_follow_their_definitions():
+    """ice: H > thklim; land: bed at or above sea level; ocean: ice-free
+    below sea level; floating: ice whose floatation base is above the bed."""
+    nx, ny, eus, thklim = 9, 6, -2.0, 1.0
+    rng = np.random.default_rng(6)
+    thck = np.asfortranarray(rng.choice([0.0, 0.5, 1.0, 1.5, 200.0, 900.0], (nx, ny)))
+    topg = np.asfortranarray(rng.choice([-2.0, -700.0, -50.0, 0.0, 40.0, -2.5], (nx, ny)))
+    out = [np.zeros((nx, ny), np.int32, order="F") for _ in range(4)]
+    po.lib().orc_get_masks(nx, ny, po.dp(thck), po.dp(topg), eus, thklim, *[po.ip(o) for o in out])
+    ice, flt, ocn, land = out
+    assert np.array_equal(ice, (thck > thklim).astype(np.int32))
+    assert np.array_equal(land, (topg >= eus).astype(np.int32))
+    assert np.array_equal(ocn, ((topg < eus) & (thck <= thklim)).astype(np.int32))
+    assert np.array_equal(flt, ((RHOI * thck < RHOO * (eus - topg)) & (thck > thklim)).astype(np.int32))
+    for m in out:
+        assert m.any() and not m.all()
+
+
+def test_slope_angles_of_a_plane():
+    """On z = a x + b y (cell centres) every slope-angle form gives the angle
+    of the plane: atan(|grad z|), atan|a|, atan|b|; the cell-centred forms
+    leave the outer ring at zero."""
+    nx, ny, dx, dy, a, b = 10, 8, 1000.0, 2000.0, 3.0e-3, -1.5e-3
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    z = np.asfortranarray(a * x[:, None] + b * y[None, :] + 40.0)
+    L = po.lib()
+    th = np.zeros((nx, ny), order="F")
+    L.orc_slope_angle(nx, ny, dx, dy, po.dp(z), po.dp(th))
+    t3 = [np.zeros((nx, ny), order="F") for _ in range(3)]
+    L.orc_slope_angle_xy(nx, ny, dx, dy, po.dp(z), *[po.dp(t) for t in t3])
+    s3 = [np.zeros((nx - 1, ny - 1), order="F") for _ in range(3)]
+    L.orc_slope_angle_staggered(nx, ny, dx, dy, po.dp(z), *[po.dp(t) for t in s3])
+    inner = (slice(1, nx - 1), slice(1, ny - 1))
+    want = np.arctan(np.hypot(a, b))
+    assert np.allclose(th[inner], want, rtol=1e-11)
+    assert np.allclose(t3[0][inner], want, rtol=1e-11)
+    assert np.allclose(t3[1][inner], np.arctan(abs(a)), rtol=1e-11)
+    assert np.allclose(t3[2][inner], np.arctan(abs(b)), rtol=1e-11)
+    ring = np.ones((nx, ny), bool)
+    ring[inner] = False
+    for t in (th, *t3):
+        assert not t[ring].any()
+    assert np.allclose(s3[0], want, rtol=1e-11)
+    assert np.allclose(s3[1], np.arctan(abs(a)), rtol=1e-11)
+    assert np.allclose(s3[2], np.arctan(abs(b)), rtol=1e-11)
+
+
+@pytest.mark.parametrize("bc", [BC_PERIODIC, 1, 2, BC_NO_ICE])
+def test_cell_halo_index_arithmetic(bc):
+    """One-task cell halo, 2 halo cells: periodic rules copy the owned image
+    (i -> 3 + (i - 3) mod (nx - 4)); outflow zeroes the 2 halo cells; no_ice
+    also zeroes the first and last owned cell."""
+    nx, ny = 11, 9
+    rng = np.random.default_rng(bc)
+    a = np.asfortranarray(rng.standard_normal((nx, ny)))
+    b = a.copy(order="F")
+    po.lib().orc_cell_halo(nx, ny, bc, po.dp(b))
+    want = np.empty_like(a)
+    for j in range(1, ny + 1):
+        for i in range(1, nx + 1):
+            if bc in (0, 2):
+                si = 3 + (i - 3) % (nx - 4)
+                sj = 3 + (j - 3) % (ny - 4)
+                want[i - 1, j - 1] = a[si - 1, sj - 1]
+            else:
+                w = 2 if bc == 1 else 3
+                edge = i <= w or i >= nx - w + 1 or j <= w or j >= ny - w + 1
+                want[i - 1, j - 1] = 0.0 if edge else a[i - 1, j - 1]
+    assert np.array_equal(b, want)
+
+
+# ----------------------------------------------------------------- residual vector
+def test_residual_vector_zero_rhs_and_planted_residual():
+    """compute_residual_vector_3d: |A u - b| over owned active rows against a
+    dense (scipy) product; relative to |b|, and 0 when b = 0."""
+    from test_oracle_pins import owned_operator, to_owned
+    g, act, A, bu, bv = random_spd_system(bc=BC_NO_ICE)
+    rng = np.random.default_rng(9)
+    u, v = (np.asfortranarray(rng.standard_normal(g.vec_shape)) for _ in range(2))
+    L2, L2r = po.residual_L2(g, act, A, bu, bv, u, v)
+    M, _ = owned_operator(g, act, A)
+    own = g.owned_slice()
+    uo, vo = u.copy(order="F"), v.copy(order="F")
+    actm = np.asarray(act, bool)
+    # inactive owned rows contribute nothing; their x still couples
+    r = M @ to_owned(g, uo, vo) - to_owned(g, bu, bv)
+    rowmask = to_owned(g, *(np.broadcast_to(actm[None], g.vec_shape).astype(float),) * 2) > 0
+    want = np.linalg.norm(r[rowmask])
+    bmask = to_owned(g, bu, bv)[rowmask]
+    assert L2 == pytest.approx(want, rel=1e-12)
+    assert L2r == pytest.approx(want / np.linalg.norm(bmask), rel=1e-12)
+    z = g.zeros_vec()
+    L2z, L2rz = po.residual_L2(g, act, A, z, z.copy(order="F"), u, v)
+    assert L2rz == 0.0 and L2z > 0.0
