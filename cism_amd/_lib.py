@@ -39,6 +39,7 @@ EXPORTS = (
     "cismgpu_pcg_solver_standard_2d", "cismgpu_pcg_solver_chrongear_2d", "cismgpu_matvec_multiply_structured_2d",
     "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d", "cismgpu_system_assemble_2d", "cismgpu_system_diva_setup",
     "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity", "cismgpu_half_storage_supported",
+    "cismgpu_residual_vector_3d_noalloc", "cismgpu_reprosum_calc",
 )
 
 
@@ -51,7 +52,7 @@ class SolveOpts(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("err", ctypes.c_double), ("niters", ctypes.c_int32), ("outer_iters", ctypes.c_int32),
                 ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double), ("spmv_count", ctypes.c_int32),
-                ("spmv_kind", ctypes.c_int32)]
+                ("spmv_kind", ctypes.c_int32), ("setup_ms", ctypes.c_double)]
 
 
 class CismGpuError(RuntimeError):
@@ -99,6 +100,10 @@ def _declare(L):
     L.cismgpu_system_matvec.argtypes = [vp, _D, _D]
     L.cismgpu_system_bench_spmv.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _D, _D]
     L.cismgpu_system_zero_x.argtypes = [vp]
+    L.cismgpu_residual_vector_3d_noalloc.argtypes = [_G] + [ctypes.c_void_p] * 13
+    L.cismgpu_residual_vector_3d_noalloc.restype = ctypes.c_int32
+    L.cismgpu_reprosum_calc.argtypes = [_D, ctypes.c_int64, ctypes.c_int32, _D]
+    L.cismgpu_reprosum_calc.restype = ctypes.c_int32
     L.cismgpu_half_storage_supported.argtypes = [_G]
     L.cismgpu_half_storage_supported.restype = ctypes.c_int32
     L.cismgpu_bench_stream.argtypes = [ctypes.c_int64, ctypes.c_int32, _D, _D, _D]
@@ -117,6 +122,19 @@ def bench_stream(nbytes: int = 2 << 30, nrep: int = 10):
     check(L.cismgpu_bench_stream(int(nbytes), int(nrep), ctypes.byref(c), ctypes.byref(t), ctypes.byref(r)),
           "bench_stream")
     return dict(copy=c.value, triad=t.value, read=r.value)
+
+
+def reprosum_calc(arr) -> "list[float]":
+    """cism_reprosum_calc on the device: exact, order-independent sums of the
+    columns of arr (nsummands, nflds) (a 1-D array is one field)."""
+    import numpy as np
+    a = np.asfortranarray(np.asarray(arr, dtype=np.float64))
+    if a.ndim == 1:
+        a = a.reshape(-1, 1, order="F")
+    out = np.zeros(a.shape[1])
+    check(load().cismgpu_reprosum_calc(a.ctypes.data_as(_D), a.shape[0], a.shape[1], out.ctypes.data_as(_D)),
+          "reprosum_calc")
+    return out
 
 
 def last_error() -> str:

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -18,6 +19,8 @@
 #include "assembly2d.hip"
 #include "spmv_sym.hip"
 #include "ensemble.hip"
+#include "noalloc.hip"
+#include "reprosum.hip"
 
 using namespace cg;
 
@@ -95,6 +98,11 @@ struct cismgpu_system {
     int ev_used = 0;
     bool timing = false;
     int spmv_kind_timed = 0;
+
+    bool repro = false;          // grid->reprosum: exact (integer-vector) global sums in the FP64 solves
+    unsigned long long* rs_part = nullptr;   // per-workgroup level sums of one exact sum
+    int* rs_bad = nullptr;       // a summand was not finite
+    double* rs_out = nullptr;    // exact sums as (value, 0) pairs, 4 slots
 
     double* V(int q) { return v64 + (size_t)q * d.vlen; }
     float* W(int q) { return v32 + (size_t)q * d.vlen; }
@@ -227,6 +235,17 @@ void launch_precond(cismgpu_system* S, int kind, const TV* ru, const TV* rv, TV*
 void scalars(cismgpu_system* S, int stage, const double* pa, int na, int sa, const double* pb, int nb, int sb,
              double tol = 0.0) {
     k_scalars<<<1, RED_THREADS, 0, S->s>>>(stage, pa, na, sa, pb, nb, sb, S->scal, tol);
+}
+
+// Exact (reproducible) sum over owned vertices of the summands a1*b1 + a2*b2
+// into the pair S->rs_out[2 * slot] (value, 0), for the scalar stage.
+template <typename TV>
+void rs_dot(cismgpu_system* S, int slot, const TV* a1, const TV* b1, const TV* a2, const TV* b2) {
+    const Dims& d = S->d;
+    const int nb = std::max(1, std::min(RS_MAX_BLOCKS, (int)nblocks((long long)d.nxo * d.nyo, RS_THREADS)));
+    k_fill_int<<<1, 64, 0, S->s>>>(S->rs_bad, 1, 0);
+    k_rs_levels<TV><<<nb, RS_THREADS, 0, S->s>>>(d, a1, b1, a2, b2, S->rs_part, S->rs_bad);
+    k_rs_final<<<1, RS_THREADS, 0, S->s>>>(S->rs_part, nb, S->rs_bad, S->rs_out + 2 * slot);
 }
 
 // dot over owned vertices of (a1,b1)+(a2,b2), compensated, into partB (stride 2)
@@ -446,14 +465,22 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
     const TV* M = std::is_same<TV, double>::value ? (const TV*)S->M64 : (const TV*)S->M32;
     const int nsp = spmv_blocks(d), ncb = col_blocks(d);
     const long long nv = d.vlen * sizeof(TV);
+    // reprosum: the global sums of the FP64 solve are exact integer-vector
+    // sums (the fused compensated partials are then not used)
+    const bool rs = S->repro && std::is_same<TV, double>::value && !given_r;
     int st = reset_scal(S);
     if (st) return st;
     for (TV* p : {du, dv, zu, zv, qu, qv, Azu, Azv}) CG_CHECK(hipMemsetAsync(p, 0, nv, S->s));
     *niters = maxiters;
     // L2 norm of the rhs (or of r0 for the inner solve)
-    k_dot<TV><<<dot_blocks<TV>(d), 256, 0, S->s>>>(d, given_r ? ru : bu, given_r ? ru : bu, given_r ? rv : bv,
-                                                   given_r ? rv : bv, S->partB);
-    scalars(S, ST_BB, S->partB, dot_blocks<TV>(d), 2, nullptr, 0, 0);
+    if (rs) {
+        rs_dot<TV>(S, 0, bu, bu, bv, bv);
+        scalars(S, ST_BB, S->rs_out, 1, 2, nullptr, 0, 0);
+    } else {
+        k_dot<TV><<<dot_blocks<TV>(d), 256, 0, S->s>>>(d, given_r ? ru : bu, given_r ? ru : bu, given_r ? rv : bv,
+                                                       given_r ? rv : bv, S->partB);
+        scalars(S, ST_BB, S->partB, dot_blocks<TV>(d), 2, nullptr, 0, 0);
+    }
     if (!given_r) {
         CG_CHECK(hipMemsetAsync(ru, 0, nv, S->s));
         CG_CHECK(hipMemsetAsync(rv, 0, nv, S->s));
@@ -465,7 +492,13 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
     apply_precond<TV>(S, precond, ru, rv, zu, zv);
     k_axpy_ops<TV><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, 1, S->scal, zu, zv, du, dv, nullptr, nullptr);
     spmv<TA, TV, 1>(S, A, du, dv, qu, qv, nullptr, nullptr, S->partB);
-    scalars(S, ST_CG_FIRST, S->partA, ncb, 4, S->partB, nsp, 2);
+    if (rs) {
+        rs_dot<TV>(S, 1, ru, zu, rv, zv);
+        rs_dot<TV>(S, 2, du, qu, dv, qv);
+        scalars(S, ST_CG_FIRST, S->rs_out + 2, 1, 2, S->rs_out + 4, 1, 2);
+    } else {
+        scalars(S, ST_CG_FIRST, S->partA, ncb, 4, S->partB, nsp, 2);
+    }
     halo<TV>(S, qu, qv, d.nz);
     k_first_update<TV><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, S->scal, xu, xv, ru, rv, du, dv, qu, qv);
     TV* vv[12] = {xu, xv, ru, rv, du, dv, qu, qv, zu, zv, Azu, Azv};
@@ -473,7 +506,13 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
     for (int iter = 2; iter <= maxiters; ++iter) {
         if (!z_ready) apply_precond<TV>(S, precond, ru, rv, zu, zv);
         spmv<TA, TV, 1>(S, A, zu, zv, Azu, Azv, nullptr, nullptr, S->partB);
-        scalars(S, ST_CG_ITER, S->partA, ncb, 4, S->partB, nsp, 2);
+        if (rs) {
+            rs_dot<TV>(S, 1, ru, zu, rv, zv);
+            rs_dot<TV>(S, 2, Azu, zu, Azv, zv);
+            scalars(S, ST_CG_ITER, S->rs_out + 2, 1, 2, S->rs_out + 4, 1, 2);
+        } else {
+            scalars(S, ST_CG_ITER, S->partA, ncb, 4, S->partB, nsp, 2);
+        }
         halo<TV>(S, Azu, Azv, d.nz);
         const bool check = (iter % ncheck == 0) || iter == 5;
         const bool fuse = fuse_ok && precond == 2 && !(check && !given_r);
@@ -491,7 +530,12 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
             } else {
                 spmv<TA, TV, 2>(S, A, xu, xv, ru, rv, bu, bv, S->partB);
                 k_ring<TV><<<ncb, 256, 0, S->s>>>(d, ru, rv, bu, bv);
-                scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+                if (rs) {
+                    rs_dot<TV>(S, 3, ru, ru, rv, rv);
+                    scalars(S, ST_RR, S->rs_out + 6, 1, 2, nullptr, 0, 0, tol);
+                } else {
+                    scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+                }
             }
             st = sync_scal(S);
             if (st) return st;
@@ -523,18 +567,34 @@ int std_solve(cismgpu_system* S, int precond, int ncheck, double tol, int maxite
     *niters = maxiters;
     st = set_scal(S, S_ETA0, 1.0);
     if (st) return st;
-    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
-    scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    const bool rs = S->repro;
+    if (rs) {
+        rs_dot<double>(S, 0, bu, bu, bv, bv);
+        scalars(S, ST_BB, S->rs_out, 1, 2, nullptr, 0, 0);
+    } else {
+        k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
+        scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    }
     double* vv[12] = {xu, xv, ru, rv, du, dv, qu, qv, zu, zv, nullptr, nullptr};
     bool z_ready = false;
     for (int iter = 1; iter <= maxiters; ++iter) {
         if (!z_ready) apply_precond<double>(S, precond, ru, rv, zu, zv);
-        scalars(S, ST_STD_ETA1, S->partA, ncb, 4, nullptr, 0, 0);
+        if (rs) {
+            rs_dot<double>(S, 1, ru, zu, rv, zv);
+            scalars(S, ST_STD_ETA1, S->rs_out + 2, 1, 2, nullptr, 0, 0);
+        } else {
+            scalars(S, ST_STD_ETA1, S->partA, ncb, 4, nullptr, 0, 0);
+        }
         k_axpy_ops<double><<<nblocks(d.nz * d.plane, 256), 256, 0, S->s>>>(d, 0, S->scal, zu, zv, du, dv, nullptr, nullptr);
         halo<double>(S, du, dv, d.nz);
         spmv<double, double, 1>(S, S->A64, du, dv, qu, qv, nullptr, nullptr, S->partB);
         k_ring<double><<<ncb, 256, 0, S->s>>>(d, qu, qv, nullptr, nullptr);
-        scalars(S, ST_STD_ETA2, S->partB, nsp, 2, nullptr, 0, 0);
+        if (rs) {
+            rs_dot<double>(S, 2, du, qu, dv, qv);
+            scalars(S, ST_STD_ETA2, S->rs_out + 4, 1, 2, nullptr, 0, 0);
+        } else {
+            scalars(S, ST_STD_ETA2, S->partB, nsp, 2, nullptr, 0, 0);
+        }
         const bool check = (iter % ncheck == 0) || iter == 5;
         const bool fuse = precond == 2 && !check;
         if (fuse) launch_update<double, 1, 1>(S, vv, S->M64);
@@ -544,7 +604,12 @@ int std_solve(cismgpu_system* S, int precond, int ncheck, double tol, int maxite
             halo<double>(S, xu, xv, d.nz);
             spmv<double, double, 2>(S, S->A64, xu, xv, ru, rv, bu, bv, S->partB);
             k_ring<double><<<ncb, 256, 0, S->s>>>(d, ru, rv, bu, bv);
-            scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+            if (rs) {
+                rs_dot<double>(S, 3, ru, ru, rv, rv);
+                scalars(S, ST_RR, S->rs_out + 6, 1, 2, nullptr, 0, 0, tol);
+            } else {
+                scalars(S, ST_RR, S->partB, nsp, 2, nullptr, 0, 0, tol);
+            }
             st = sync_scal(S);
             if (st) return st;
             if (S->scal_h[S_NAN] != 0.0) { cismgpu_set_error("PCG solver has failed, alpha = NaN"); return CISMGPU_ERR_NAN; }
@@ -644,8 +709,13 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
     // ||b|| once
     st = reset_scal(S);
     if (st) return st;
-    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
-    scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    if (S->repro) {
+        rs_dot<double>(S, 0, bu, bu, bv, bv);
+        scalars(S, ST_BB, S->rs_out, 1, 2, nullptr, 0, 0);
+    } else {
+        k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, bu, bu, bv, bv, S->partB);
+        scalars(S, ST_BB, S->partB, dot_blocks<double>(d), 2, nullptr, 0, 0);
+    }
     st = sync_scal(S);
     if (st) return st;
     const double l2rhs = S->scal_h[S_L2RHS];
@@ -661,7 +731,12 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
         spmv<double, double, 2>(S, S->A64, xu, xv, ru, rv, bu, bv, S->partB);
         S->timing = tspmv;
         k_ring<double><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, bu, bv);
-        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, nsp, 2, nullptr, 0, 0, S->scal, 0.0);
+        if (S->repro) {
+            rs_dot<double>(S, 3, ru, ru, rv, rv);
+            k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+        } else {
+            k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, nsp, 2, nullptr, 0, 0, S->scal, 0.0);
+        }
         st = sync_scal(S);
         if (st) return st;
         err = std::sqrt(S->scal_h[S_DOT]) / l2rhs;
@@ -754,6 +829,13 @@ static int32_t system_create(const cismgpu_grid* grid, bool two_d, cismgpu_syste
     if ((e = hipMalloc(&S->partB, 4 * nparts * sizeof(double))) != hipSuccess) return fail(e, "partB");
     if ((e = hipMalloc(&S->scal, S_NSLOTS * sizeof(double))) != hipSuccess) return fail(e, "scal");
     if ((e = hipHostMalloc(&S->scal_h, S_NSLOTS * sizeof(double), hipHostMallocDefault)) != hipSuccess) return fail(e, "scal_h");
+    S->repro = grid->reprosum != 0;
+    if (S->repro) {
+        if ((e = hipMalloc(&S->rs_part, (size_t)RS_MAX_BLOCKS * RS_L * sizeof(unsigned long long))) != hipSuccess)
+            return fail(e, "rs_part");
+        if ((e = hipMalloc(&S->rs_bad, sizeof(int))) != hipSuccess) return fail(e, "rs_bad");
+        if ((e = hipMalloc(&S->rs_out, 8 * sizeof(double))) != hipSuccess) return fail(e, "rs_out");
+    }
     hipMemsetAsync(S->v64, 0, 14 * d.vlen * sizeof(double), S->s);
     hipMemsetAsync(S->v32, 0, 14 * d.vlen * sizeof(float), S->s);
     hipMemsetAsync(S->M64, 0, 8 * d.vlen * sizeof(double), S->s);
@@ -782,7 +864,8 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     if (S->s) hipStreamSynchronize(S->s);
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
-                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf, (void*)S->efva2})
+                    (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf, (void*)S->efva2, (void*)S->rs_part, (void*)S->rs_bad,
+                    (void*)S->rs_out})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -949,12 +1032,30 @@ int32_t cismgpu_system_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cis
     o = &eff;
     memset(info, 0, sizeof(*info));
     info->err = NAN;
-    if ((o->precond == 3 || o->precond == 4) && S->tri.mode != o->precond) {
-        st = setup_tri(S, o->precond);
+    // per-assembly set-up, done by the first solve after an assembly or a
+    // new matrix and cached until the next: the line-solve factors, the
+    // exact-symmetry checks and the half-storage / low-precision copies
+    {
+        const bool fresh = ((o->precond == 3 || o->precond == 4) && S->tri.mode != o->precond) || !S->a64h_ready ||
+                           (o->precision == CISMGPU_MIXED_FP32 && !S->a32_ready) ||
+                           (o->precision == CISMGPU_MIXED_BF16 && !S->a16_ready) ||
+                           (o->precision != CISMGPU_FP64 && !S->m32_ready);
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((o->precond == 3 || o->precond == 4) && S->tri.mode != o->precond) {
+            st = setup_tri(S, o->precond);
+            if (st) return st;
+        }
+        st = ensure_half64(S);
         if (st) return st;
+        if (o->precision != CISMGPU_FP64) {
+            st = ensure_lowp(S, o->precision);
+            if (st) return st;
+        }
+        if (fresh) {
+            CG_CHECK(hipStreamSynchronize(S->s));
+            info->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
     }
-    st = ensure_half64(S);
-    if (st) return st;
     S->timing = o->time_spmv != 0;
     if (S->timing && S->ev.empty()) {
         S->ev.resize(8192);
@@ -1066,6 +1167,57 @@ int32_t cismgpu_system_bench_spmv(cismgpu_system* S, int32_t kind, int32_t nrep,
     const double ncoef = S->two_d ? 36.0 : (half ? 4.0 * NSLOT : (double)NCOEF);
     if (bytes_per_launch) *bytes_per_launch = (double)nodes * (ncoef * esz + 4.0 * vsz);
     return CISMGPU_OK;
+}
+
+// cism_reprosum_calc for one task (cism_reprosum_mod.F90:237-1057): the
+// exact, order-independent sum of each field of arr(nsummands, nflds),
+// correctly rounded.  Host arrays; work on the device.
+extern "C" int32_t cismgpu_reprosum_calc(const double* arr, int64_t nsummands, int32_t nflds, double* arr_gsum) {
+    if (!arr || !arr_gsum || nsummands < 0 || nflds < 1) {
+        cismgpu_set_error("reprosum_calc: bad arguments");
+        return CISMGPU_ERR_ARG;
+    }
+    if (nsummands > (1LL << 32)) {
+        cismgpu_set_error("reprosum_calc: %lld summands exceed the 2^32 the digit headroom allows", (long long)nsummands);
+        return CISMGPU_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        cismgpu_set_error("no HIP device available");
+        return CISMGPU_ERR_HIP;
+    }
+    hipStream_t s = nullptr;
+    double *a = nullptr, *out = nullptr;
+    unsigned long long* part = nullptr;
+    int* bad = nullptr;
+    int st = CISMGPU_OK;
+    auto run = [&]() -> int {
+        CG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        CG_CHECK(hipMalloc(&a, std::max<int64_t>(nsummands, 1) * sizeof(double)));
+        CG_CHECK(hipMalloc(&out, 2 * (size_t)nflds * sizeof(double)));
+        CG_CHECK(hipMalloc(&part, (size_t)RS_MAX_BLOCKS * RS_L * sizeof(unsigned long long)));
+        CG_CHECK(hipMalloc(&bad, sizeof(int)));
+        const int nb = std::max(1, std::min(RS_MAX_BLOCKS, (int)nblocks(nsummands, RS_THREADS)));
+        for (int f = 0; f < nflds; ++f) {
+            if (nsummands)
+                CG_CHECK(hipMemcpyAsync(a, arr + (size_t)f * nsummands, nsummands * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+            k_fill_int<<<1, 64, 0, s>>>(bad, 1, 0);
+            k_rs_levels_flat<<<nb, RS_THREADS, 0, s>>>(nsummands, a, part, bad);
+            k_rs_final<<<1, RS_THREADS, 0, s>>>(part, nb, bad, out + 2 * f);
+        }
+        std::vector<double> h(2 * (size_t)nflds);
+        CG_CHECK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        CG_CHECK(hipStreamSynchronize(s));
+        for (int f = 0; f < nflds; ++f) arr_gsum[f] = h[2 * f];
+        return CISMGPU_OK;
+    };
+    st = run();
+    if (s) hipStreamSynchronize(s);
+    for (void* p : {(void*)a, (void*)out, (void*)part, (void*)bad})
+        if (p) hipFree(p);
+    if (s) hipStreamDestroy(s);
+    return st;
 }
 
 extern "C" int32_t cismgpu_half_storage_supported(const cismgpu_grid* grid) {
@@ -1739,9 +1891,16 @@ extern "C" int32_t cismgpu_system_residual(cismgpu_system* S, double* ru, double
     { const int st_ = reset_scal(S); if (st_) return st_; }
     // r = b - A x over owned rows with partial (r,r); resid = -r on owned active rows
     k_spmv<double, double, 2><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), r0, r1, S->V(2), S->V(3), S->partB);
-    k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, spmv_blocks(d), 2, nullptr, 0, 0, S->scal, 0.0);
-    k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, S->V(2), S->V(2), S->V(3), S->V(3), S->partA);
-    k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->partA, dot_blocks<double>(d), 2, nullptr, 0, 0, S->scal, 0.0);
+    if (S->repro) {
+        rs_dot<double>(S, 3, r0, r0, r1, r1);
+        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+        rs_dot<double>(S, 0, S->V(2), S->V(2), S->V(3), S->V(3));
+        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->rs_out, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+    } else {
+        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, spmv_blocks(d), 2, nullptr, 0, 0, S->scal, 0.0);
+        k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, S->V(2), S->V(2), S->V(3), S->V(3), S->partA);
+        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->partA, dot_blocks<double>(d), 2, nullptr, 0, 0, S->scal, 0.0);
+    }
     int st = sync_scal(S);
     if (st) return st;
     if (L2) *L2 = std::sqrt(S->scal_h[S_DOT]);
@@ -2252,7 +2411,12 @@ extern "C" int32_t cismgpu_ensemble_solve(cismgpu_ensemble* E, const cismgpu_ens
         info->max_err = std::max(info->max_err, a.err);
     }
     if (info->barrier_error) { cismgpu_set_error("ensemble grid barrier timed out"); return CISMGPU_ERR_HIP; }
-    return info->converged == E->nmem ? CISMGPU_OK : CISMGPU_ERR_NOT_CONVERGED;
+    if (info->converged != E->nmem) {
+        cismgpu_set_error("%d of %d ensemble members reached the tolerance (largest residual %.3e after %d steps)",
+                          info->converged, E->nmem, info->max_err, info->steps);
+        return CISMGPU_ERR_NOT_CONVERGED;
+    }
+    return CISMGPU_OK;
 }
 
 extern "C" int32_t cismgpu_ensemble_get_x(cismgpu_ensemble* E, int32_t m, double* xu, double* xv) {

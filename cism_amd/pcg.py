@@ -218,7 +218,7 @@ class System:
         st = self._L.cismgpu_system_solve(self._h, ctypes.byref(o), ctypes.byref(info))
         res = dict(status=st, err=info.err, niters=info.niters, outer_iters=info.outer_iters,
                    solve_ms=info.solve_ms, spmv_ms=info.spmv_ms, spmv_count=info.spmv_count,
-                   spmv_kind=info.spmv_kind)
+                   spmv_kind=info.spmv_kind, setup_ms=info.setup_ms)
         if raise_on_error and st != 0:
             raise _lib.CismGpuError(st, "system_solve")
         return res

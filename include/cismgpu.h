@@ -46,7 +46,11 @@ typedef struct cismgpu_grid {
     int32_t nx, ny, nz;          /* cells incl. 2-cell halos; sigma levels */
     int32_t ilo, ihi, jlo, jhi;  /* staggered_ilo/ihi/jlo/jhi */
     int32_t global_bc;           /* 0 periodic, 1 outflow, 2 no_penetration, 3 no_ice */
-    int32_t reprosum;            /* accepted for interface parity; device sums are always compensated */
+    int32_t reprosum;            /* parallel%reprosum.  0: global sums from per-workgroup compensated
+                                    (double-double) partials in a fixed order; 1: the FP64 solves'
+                                    global sums (and the residual's) are exact integer-vector sums
+                                    of the packed summands a1*b1 + a2*b2, correctly rounded and
+                                    independent of summation order (cism_reprosum_calc) */
 } cismgpu_grid;
 
 const char* cismgpu_last_error(void);
@@ -153,6 +157,9 @@ typedef struct cismgpu_solve_info {
     double  spmv_ms;         /* summed duration of the timed SpMV launches */
     int32_t spmv_count;      /* number of timed SpMV launches */
     int32_t spmv_kind;       /* storage of the dominant SpMV: 0 fp64, 1 fp32, 2 bf16, 3 fp32 half (symmetric), 4 fp64 half */
+    double  setup_ms;        /* host wall time of the per-assembly set-up this call did before solve_ms began
+                                (symmetry checks, half-storage and low-precision copies, line-solve
+                                factors); 0 when a previous solve of the same matrix did it */
 } cismgpu_solve_info;
 
 int32_t cismgpu_system_create(const cismgpu_grid* grid, cismgpu_system** out);
@@ -196,6 +203,15 @@ int32_t cismgpu_system_matvec(cismgpu_system* sys, double* yu, double* yv);
 int32_t cismgpu_system_bench_spmv(cismgpu_system* sys, int32_t kind, int32_t nrep,
                                   double* avg_ms, double* bytes_per_launch);
 
+
+/* Replaces cism_reprosum_calc (cism_reprosum_mod.F90:237-1057) for one
+ * task: arr_gsum[f] = exact sum of arr(1:nsummands, f), f = 1..nflds
+ * (Fortran layout, host arrays), rounded once to the nearest double.  The
+ * summands are split into 30-bit integer digits on a fixed ladder covering
+ * every finite double and added as 64-bit integers, so the result does not
+ * depend on their order; NaN if a summand is not finite.  At most 2^32
+ * summands per field. */
+int32_t cismgpu_reprosum_calc(const double* arr, int64_t nsummands, int32_t nflds, double* arr_gsum);
 
 /* 1 when the half (symmetric) storage layouts can hold this grid's owned
  * rows, 0 when the system keeps full rows whatever the symmetry (one level
@@ -361,6 +377,22 @@ int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
  * |b| (compute_residual_vector_3d, glissade_velo_higher.F90:9693-9894). */
 int32_t cismgpu_system_residual(cismgpu_system* sys, double* resid_u, double* resid_v,
                                 double* L2_norm, double* L2_norm_relative);
+/* The same residual without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops); no counterpart in
+ * the reference.  Every pointer is a DEVICE pointer to the reference layout:
+ * active_vertex (nx-1, ny-1) int32, A (27, nz, nx-1, ny-1), b, u and resid
+ * (nz, nx-1, ny-1); resid_u / resid_v may be NULL; norms[0] = L2 norm,
+ * norms[1] = L2 / |b| (0 when b = 0) are written on the device.  Work is
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) and not
+ * waited for.  Partial sums use a library-held 2 MiB device array per GPU:
+ * at most 16,777,216 nodes (nz * (nx-1) * (ny-1)), larger grids are
+ * rejected; calls that overlap on one GPU race on that array. */
+int32_t cismgpu_residual_vector_3d_noalloc(const cismgpu_grid* grid, const int32_t* active_vertex,
+                                           const double* Auu, const double* Auv,
+                                           const double* Avu, const double* Avv,
+                                           const double* bu, const double* bv,
+                                           const double* uvel, const double* vvel,
+                                           double* resid_u, double* resid_v, double* norms, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched ensemble of independent systems (no reference counterpart:   */

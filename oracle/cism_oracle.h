@@ -47,6 +47,7 @@ double orc_global_sum(const orc_grid* g, const double* a1, const double* a2);
 void   orc_global_sum2(const orc_grid* g, const double* a1f1, const double* a2f1,
                        const double* a1f2, const double* a2f2, double out[2]);
 void   orc_ddpdd_sum(int64_t n, const double* arr, double* hi, double* lo);
+double orc_reprosum_calc(int64_t n, const double* arr);
 
 /* ---------------- solver path ---------------- */
 void orc_matvec_3d(const orc_grid* g, const int32_t* active_vertex,

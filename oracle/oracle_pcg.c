@@ -126,22 +126,114 @@ void orc_ddpdd_sum(int64_t n, const double* arr, double* hi, double* lo)
 }
 
 /* ------------------------------------------------------------------ */
+/* cism_reprosum_calc, integer-vector algorithm (cism_reprosum_mod.F90: */
+/* 237-1057 and :1063-1818), for one task and one field.  As in the     */
+/* reference, the extreme exponents of the nonzero summands fix the     */
+/* levels: level 0 holds the lowest bit any summand has, each level     */
+/* takes RV_D bits of every summand's mantissa, and the levels are      */
+/* 128-bit integers, so no level can overflow.  The integer vector is   */
+/* then carried into digits of one sign and rounded once to the nearest */
+/* double (ties to even): the exact sum, correctly rounded.            */
+/* ------------------------------------------------------------------ */
+#define RV_D 56
+typedef __int128 i128;
+
+double orc_reprosum_calc(int64_t n, const double* arr)
+{
+    int emin = 1 << 30, emax = -(1 << 30), bad = 0;
+    for (int64_t s = 0; s < n; ++s) {
+        const double x = arr[s];
+        if (!isfinite(x)) { bad = 1; continue; }
+        if (x == 0.0) continue;
+        int e;
+        frexp(x, &e);                       /* |x| = f 2^e, f in [0.5, 1) */
+        int lsb = e - 53;                   /* weight of the last mantissa bit */
+        if (lsb < -1074) lsb = -1074;       /* subnormals */
+        if (lsb < emin) emin = lsb;
+        if (e > emax) emax = e;
+    }
+    if (bad) return NAN;
+    if (emin > emax) return 0.0;
+    const int nlev = (emax - emin) / RV_D + 2;
+    i128* lev = (i128*)calloc((size_t)nlev, sizeof(i128));
+    for (int64_t s = 0; s < n; ++s) {
+        const double x = arr[s];
+        if (x == 0.0) continue;
+        int e;
+        const double f = frexp(fabs(x), &e);
+        int lsb = e - 53;
+        int64_t m;
+        if (lsb < -1074) { m = (int64_t)ldexp(f, e + 1074); lsb = -1074; }
+        else m = (int64_t)ldexp(f, 53);     /* exact: 53-bit integer mantissa */
+        const int off = lsb - emin;         /* bit offset of m above level 0 */
+        const int l = off / RV_D, sh = off % RV_D;
+        const i128 v = (i128)m << sh;       /* < 2^(53 + 55) */
+        const i128 mask = (((i128)1) << RV_D) - 1;
+        const i128 sg = x < 0 ? -1 : 1;
+        lev[l] += sg * (v & mask);
+        lev[l + 1] += sg * (v >> RV_D);
+    }
+    /* sign of the total, then digits of that sign */
+    i128 carry = 0;
+    for (int l = 0; l < nlev; ++l) carry = (lev[l] + carry) >> RV_D;
+    const int neg = carry < 0;
+    if (neg)
+        for (int l = 0; l < nlev; ++l) lev[l] = -lev[l];
+    carry = 0;
+    for (int l = 0; l < nlev; ++l) {
+        const i128 t = lev[l] + carry;
+        lev[l] = t & ((((i128)1) << RV_D) - 1);
+        carry = t >> RV_D;
+    }
+    int h = nlev - 1;
+    while (h >= 0 && lev[h] == 0) --h;
+    double r = 0.0;
+    if (carry > 0) {
+        r = INFINITY;
+    } else if (h >= 0) {
+        /* the top two digits hold at least 57 bits once the top one is nonzero */
+        const int lo = h >= 1 ? h - 1 : 0;
+        unsigned __int128 T = 0;
+        for (int l = h; l >= lo; --l) T = (T << RV_D) | (unsigned __int128)lev[l];
+        int sticky = 0;
+        for (int l = 0; l < lo; ++l) sticky |= lev[l] != 0;
+        int nb = 0;
+        for (unsigned __int128 u = T; u; u >>= 1) ++nb;
+        const int ebase = emin + RV_D * lo;
+        if (nb <= 53) {
+            r = ldexp((double)(uint64_t)T, ebase);
+        } else {
+            const int sh = nb - 53;
+            uint64_t q = (uint64_t)(T >> sh);
+            const unsigned __int128 rem = T - ((unsigned __int128)q << sh);
+            const unsigned __int128 half = ((unsigned __int128)1) << (sh - 1);
+            if (rem > half || (rem == half && (sticky || (q & 1)))) q += 1;
+            r = ldexp((double)q, ebase + sh);
+        }
+    }
+    free(lev);
+    return neg ? -r : r;
+}
+
+/* ------------------------------------------------------------------ */
 /* parallel_global_sum_stagger_real8_3d (parallel_mpi.F90:6663-6794):  */
 /* non-reproducible order local = (local + a1) + a2, loops j, i, k over */
 /* owned vertices; reproducible variant packs a1+a2 (:6718-6731) and   */
-/* sums the packed array (here with the DDPDD algorithm).             */
+/* sums the packed array with cism_reprosum_calc (integer vectors).    */
 /* ------------------------------------------------------------------ */
 double orc_global_sum(const orc_grid* g, const double* a1, const double* a2)
 {
     if (g->reprosum) {
-        double hi = 0.0, lo = 0.0;
+        const int64_t n = (int64_t)(g->ihi - g->ilo + 1) * (g->jhi - g->jlo + 1) * NZ;
+        double* arr = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+        int64_t c = 0;
         for (int j = g->jlo; j <= g->jhi; ++j)
             for (int i = g->ilo; i <= g->ihi; ++i)
-                for (int k = 1; k <= NZ; ++k) {
-                    double v = a2 ? a1[V3(k, i, j)] + a2[V3(k, i, j)] : a1[V3(k, i, j)];
-                    ddpdd_add(v, &hi, &lo);
-                }
-        return hi;
+                for (int k = 1; k <= NZ; ++k)
+                    arr[c++] = a2 ? a1[V3(k, i, j)] + a2[V3(k, i, j)] : a1[V3(k, i, j)];
+        const double r = orc_reprosum_calc(n, arr);
+        free(arr);
+        return r;
     }
     double s = 0.0;
     for (int j = g->jlo; j <= g->jhi; ++j)

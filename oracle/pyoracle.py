@@ -55,6 +55,8 @@ def _declare(L):
     L.orc_global_sum.argtypes = [G, _D, _D]
     L.orc_global_sum.restype = ctypes.c_double
     L.orc_ddpdd_sum.argtypes = [ctypes.c_int64, _D, _D, _D]
+    L.orc_reprosum_calc.argtypes = [ctypes.c_int64, _D]
+    L.orc_reprosum_calc.restype = ctypes.c_double
     L.orc_tridiag_solver.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D]
     L.orc_set_trace.argtypes = [_D, ctypes.c_int32]
     L.orc_trace_len.restype = ctypes.c_int32
@@ -235,6 +237,12 @@ def ddpdd_sum(arr: np.ndarray):
     a = np.ascontiguousarray(arr, dtype=np.float64)
     L.orc_ddpdd_sum(a.size, dp(a), ctypes.byref(hi), ctypes.byref(lo))
     return hi.value, lo.value
+
+
+def reprosum(arr: np.ndarray) -> float:
+    """cism_reprosum_calc (integer vectors) of a 1-D array of summands."""
+    a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+    return lib().orc_reprosum_calc(a.size, dp(a))
 
 
 def tridiag(sb, d, sp, rhs):

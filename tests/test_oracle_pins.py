@@ -317,14 +317,17 @@ def test_global_sums_exact_and_ordered():
     # non-reproducible order: local = (local + a1) + a2, loops j, i, k
     expect = reduce(lambda s, t: (s + t[0]) + t[1], seq, 0.0)
     assert po.global_sum(g, a1, a2) == expect
-    # reproducible (DDPDD) sum of the packed a1 + a2 is within 1 ulp of the exact sum
+    # reproducible sum (integer vectors) of the packed a1 + a2: the exact sum, correctly rounded;
+    # the DDPDD alternative is within 1 ulp of it
     exact = sum(Fraction(x + y) for x, y in seq)
     g.reprosum = 1
     try:
         r = po.global_sum(g, a1, a2)
     finally:
         g.reprosum = 0
-    assert abs(Fraction(r) - exact) <= abs(np.spacing(float(exact)))
+    assert r == float(exact)
+    hi, lo = po.ddpdd_sum(np.array([x + y for x, y in seq]))
+    assert abs(Fraction(hi) - exact) <= abs(np.spacing(float(exact)))
 
 
 def test_halo_rules():

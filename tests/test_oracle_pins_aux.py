@@ -310,3 +310,40 @@ def test_residual_vector_zero_rhs_and_planted_residual():
     z = g.zeros_vec()
     L2z, L2rz = po.residual_L2(g, act, A, z, z.copy(order="F"), u, v)
     assert L2rz == 0.0 and L2z > 0.0
+
+
+# ----------------------------------------------------------------- exact reproducible sums
+def _exact(a):
+    from fractions import Fraction
+    return float(sum((Fraction(float(x)) for x in a), Fraction(0)))
+
+
+def test_reprosum_is_the_correctly_rounded_exact_sum():
+    """cism_reprosum_calc (integer vectors): the exact sum of the summands,
+    rounded once to nearest-even, whatever their order -- against exact
+    rational arithmetic, over wide exponent ranges, cancellations,
+    subnormals and rounding ties."""
+    rng = np.random.default_rng(12)
+    for t in range(400):
+        n = int(rng.integers(0, 80))
+        a = rng.standard_normal(n) * 10.0 ** rng.integers(-300, 300, n)
+        if t % 3 == 0:
+            a = np.concatenate([a, -a[: n // 2]])
+        if t % 5 == 0:
+            a = np.concatenate([a, [5e-324, 3 * 2.0 ** -1074, -1e-310, 2.0 ** -1022]])
+        want = _exact(a)
+        got = po.reprosum(a)
+        assert got == want, (t, got, want)
+        assert po.reprosum(a[rng.permutation(a.size)]) == got
+    cases = {
+        (1e300, 1.0, -1e300): 1.0,                          # catastrophic cancellation
+        (1.0, 2.0 ** -53): 1.0,                             # tie: to even
+        (1.0, 2.0 ** -53, 2.0 ** -100): 1.0 + 2.0 ** -52,   # just above the tie
+        (1.0 + 2.0 ** -52, 2.0 ** -53): 1.0 + 2.0 ** -51,   # tie from an odd last bit: up
+        (): 0.0,
+        (5e-324, 5e-324): 1e-323,                           # subnormal sum, exact
+        (1e308, 1e308): float("inf"),
+    }
+    for a, want in cases.items():
+        assert po.reprosum(np.array(a, dtype=np.float64)) == want, a
+    assert np.isnan(po.reprosum(np.array([1.0, np.nan])))
