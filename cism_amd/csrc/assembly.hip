@@ -206,6 +206,198 @@ __global__ __launch_bounds__(64) void k_assemble(Dims d, AsmDev in, const double
         for (int m = 0; m < 27; ++m) a[(b * 27 + m) * WAVE] = acc[b][m];
 }
 
+// ---------------------------------------------------------------------------
+// Element-once assembly.  Pass 1 (k_elem): one thread per element computes,
+// at each of its 8 quadrature points, the inverse Jacobian and the factor
+// f = efvs * w * detJ / vol0 (the Jacobian, its cofactor inverse, the
+// strain rate and the Glen-law pow are done once per element instead of
+// once per element row), and the element viscosity efvs(k,i,j) as the mean
+// over the points (:5604-5610).  Pass 2 (k_rows): one thread per owned node
+// builds its row from the stored point data of the (up to) 8 elements that
+// contain it: the basis-function derivatives from the inverse Jacobian and
+// the row of the element matrix, added point by point into the row.  The
+// node's elements are visited layer by layer (the layer above the node,
+// then the layer below) and, within a layer, in the reference's cell order
+// (j-major, then i); this is one global order of the elements, and every
+// product is formed as in compute_element_matrix, so entry (P, Q) and
+// entry (Q, P) are summed from equal terms in the same order: the
+// assembled matrix is exactly symmetric.  Keeping the couplings to the level
+// above until the first layer is done, and those to the level below from the
+// second layer on, holds 72 of the row's 108 coefficients in registers.
+//
+// Point data at E[(p * EQ + q) * ne + e], q = 0..8 the inverse Jacobian
+// (row-major), q = 9 the factor f; e = (ci-1) + nx ((cj-1) + ny (kk-1)),
+// so neighbouring lanes (neighbouring i) read neighbouring words.
+// ---------------------------------------------------------------------------
+constexpr int EQ = 10;
+
+__device__ __forceinline__ long long elem_id(const Dims& d, int ci, int cj, int kk) {
+    return (long long)(ci - 1) + (long long)d.nx * ((cj - 1) + (long long)d.ny * (kk - 1));
+}
+
+// Inverse Jacobian at point p (get_basis_function_derivatives_3d,
+// :6576-6617); false if singular.
+__device__ __forceinline__ bool jac_inverse(const Elem& e, int p, double (&Ji)[9], double& detJ) {
+    double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double a = c_dxr[n][p], b = c_dyr[n][p], c = c_dzr[n][p];
+        J00 = J00 + a * e.x[n]; J01 = J01 + a * e.y[n]; J02 = J02 + a * e.z[n];
+        J10 = J10 + b * e.x[n]; J11 = J11 + b * e.y[n]; J12 = J12 + b * e.z[n];
+        J20 = J20 + c * e.x[n]; J21 = J21 + c * e.y[n]; J22 = J22 + c * e.z[n];
+    }
+    const double c00 = J11 * J22 - J12 * J21;
+    const double c01 = -(J10 * J22 - J12 * J20);
+    const double c02 = J10 * J21 - J11 * J20;
+    const double c10 = -(J01 * J22 - J02 * J21);
+    const double c11 = J00 * J22 - J02 * J20;
+    const double c12 = -(J00 * J21 - J01 * J20);
+    const double c20 = J01 * J12 - J02 * J11;
+    const double c21 = -(J00 * J12 - J02 * J10);
+    const double c22 = J00 * J11 - J01 * J10;
+    detJ = J00 * c00 + J01 * c01 + J02 * c02;
+    if (!(fabs(detJ) > 0.0)) return false;
+    Ji[0] = c00 / detJ; Ji[1] = c10 / detJ; Ji[2] = c20 / detJ;
+    Ji[3] = c01 / detJ; Ji[4] = c11 / detJ; Ji[5] = c21 / detJ;
+    Ji[6] = c02 / detJ; Ji[7] = c12 / detJ; Ji[8] = c22 / detJ;
+    return true;
+}
+
+__global__ __launch_bounds__(128) void k_elem(Dims d, AsmDev in, const double* __restrict__ uv,
+                                              const double* __restrict__ vv, double* __restrict__ E,
+                                              double* __restrict__ efvs, double* __restrict__ flag) {
+    const long long ne = (long long)(d.nz - 1) * d.nx * d.ny;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= ne) return;
+    const int ci = 1 + (int)(t % d.nx);
+    const long long r = t / d.nx;
+    const int cj = 1 + (int)(r % d.ny), kk = 1 + (int)(r / d.ny);
+    const long long cell = (long long)(ci - 1) + (long long)d.nx * (cj - 1);
+    double out = 0.0;
+    if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[cell]) {
+        Elem e;
+        load_elem(d, in, uv, vv, ci, cj, kk, e);
+        const double fl = in.flwafact[(long long)(kk - 1) + (long long)(d.nz - 1) * cell];
+        double s = 0.0;
+        for (int p = 0; p < 8; ++p) {
+            double Ji[9], detJ;
+            if (!jac_inverse(e, p, Ji, detJ)) {
+                flag[0] = 1.0;
+                s = NAN;
+                break;
+            }
+            double dx[8], dy[8], dz[8];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const double a = c_dxr[n][p], b = c_dyr[n][p], c = c_dzr[n][p];
+                dx[n] = Ji[0] * a + Ji[1] * b + Ji[2] * c;
+                dy[n] = Ji[3] * a + Ji[4] * b + Ji[5] * c;
+                dz[n] = Ji[6] * a + Ji[7] * b + Ji[8] * c;
+            }
+            const double ef = efvs_at(in, e, dx, dy, dz, fl);
+            s = s + ef;
+            double* q = E + (long long)p * EQ * ne + t;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) q[k * ne] = Ji[k];
+            q[9 * ne] = ef * 1.0 * detJ / 1.0e9;
+        }
+        out = s / 8.0;
+    }
+    efvs[(long long)(kk - 1) + (long long)(d.nz - 1) * cell] = out;
+}
+
+// Row NR of the element matrix of element e, from its point data, added
+// into the row: couplings to the element's upper-face nodes (kshift 0) go
+// to acc_up, those to its lower-face nodes to acc_lo, at index
+// (iA + 1) + 3 (jA + 1) of the 9 horizontal neighbours.
+template <int NR>
+__device__ __forceinline__ void add_row(const double* __restrict__ E, long long ne, long long e,
+                                        double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
+#pragma unroll 1
+    for (int p = 0; p < 8; ++p) {
+        const double* q = E + (long long)p * EQ * ne + e;
+        double Ji[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ji[k] = q[k * ne];
+        const double f = q[9 * ne];
+        const double ar = Ji[0] * c_dxr[NR][p] + Ji[1] * c_dyr[NR][p] + Ji[2] * c_dzr[NR][p];
+        const double br = Ji[3] * c_dxr[NR][p] + Ji[4] * c_dyr[NR][p] + Ji[5] * c_dzr[NR][p];
+        const double cr = Ji[6] * c_dxr[NR][p] + Ji[7] * c_dyr[NR][p] + Ji[8] * c_dzr[NR][p];
+#pragma unroll
+        for (int nc = 0; nc < 8; ++nc) {
+            const double a = c_dxr[nc][p], b = c_dyr[nc][p], c = c_dzr[nc][p];
+            const double dxn = Ji[0] * a + Ji[1] * b + Ji[2] * c;
+            const double dyn = Ji[3] * a + Ji[4] * b + Ji[5] * c;
+            const double dzn = Ji[6] * a + Ji[7] * b + Ji[8] * c;
+            const int h = (nd_i(nc) - nd_i(NR) + 1) + 3 * (nd_j(nc) - nd_j(NR) + 1);
+            double(&acc)[4][9] = nd_k(nc) == 0 ? acc_up : acc_lo;
+            acc[0][h] = acc[0][h] + f * (4.0 * ar * dxn + br * dyn + cr * dzn);
+            acc[1][h] = acc[1][h] + f * (2.0 * ar * dyn + br * dxn);
+            acc[2][h] = acc[2][h] + f * (2.0 * br * dxn + ar * dyn);
+            acc[3][h] = acc[3][h] + f * (4.0 * br * dyn + ar * dxn + cr * dzn);
+        }
+    }
+}
+
+// One thread per owned node (io fastest): the node's full row (108
+// coefficients) into the row-blocked matrix.
+__global__ __launch_bounds__(128) void k_rows(Dims d, AsmDev in, const double* __restrict__ E,
+                                              double* __restrict__ A) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k = (int)(kj / d.nyo);   // k 0-based level
+    const long long ne = (long long)(d.nz - 1) * d.nx * d.ny;
+    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    const int i = d.ilo + io, j = d.jlo + jo;
+    const int kf = k + 1;
+    const bool on = io < d.nxo && in.avert[v2(d, i, j)];
+    auto cell_on = [&](int ci, int cj) { return cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]; };
+    double mid[4][9];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int h = 0; h < 9; ++h) mid[b][h] = 0.0;
+    {
+        // layer kf - 1: the node is on the elements' lower face
+        double above[4][9];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int h = 0; h < 9; ++h) above[b][h] = 0.0;
+        if (on && kf - 1 >= 1) {
+            if (cell_on(i, j)) add_row<2>(E, ne, elem_id(d, i, j, kf - 1), above, mid);
+            if (cell_on(i + 1, j)) add_row<3>(E, ne, elem_id(d, i + 1, j, kf - 1), above, mid);
+            if (cell_on(i, j + 1)) add_row<1>(E, ne, elem_id(d, i, j + 1, kf - 1), above, mid);
+            if (cell_on(i + 1, j + 1)) add_row<0>(E, ne, elem_id(d, i + 1, j + 1, kf - 1), above, mid);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int h = 0; h < 9; ++h) a[(b * 27 + h) * WAVE] = above[b][h];
+    }
+    double below[4][9];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int h = 0; h < 9; ++h) below[b][h] = 0.0;
+    if (on && kf <= d.nz - 1) {
+        // layer kf: the node is on the elements' upper face
+        if (cell_on(i, j)) add_row<6>(E, ne, elem_id(d, i, j, kf), mid, below);
+        if (cell_on(i + 1, j)) add_row<7>(E, ne, elem_id(d, i + 1, j, kf), mid, below);
+        if (cell_on(i, j + 1)) add_row<5>(E, ne, elem_id(d, i, j + 1, kf), mid, below);
+        if (cell_on(i + 1, j + 1)) add_row<4>(E, ne, elem_id(d, i + 1, j + 1, kf), mid, below);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int h = 0; h < 9; ++h) {
+            a[(b * 27 + 9 + h) * WAVE] = mid[b][h];
+            a[(b * 27 + 18 + h) * WAVE] = below[b][h];
+        }
+}
+
 // efvs(k,i,j) = mean over the quadrature points (:5604-5610), one thread
 // per element in the assembly range.
 __global__ void k_efvs(Dims d, AsmDev in, const double* __restrict__ uv, const double* __restrict__ vv,
@@ -555,6 +747,47 @@ __device__ __forceinline__ bool row_image(const Dims& d, int& i, int& j) {
 __device__ __forceinline__ const double* row_ptr(const Dims& d, const double* A, int k0, int i, int j) {
     const long long r = ((long long)k0 * d.nyo + (j - d.jlo)) * d.nxp + (i - d.ilo);
     return A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
+}
+
+// check_symmetry_assembled_matrix_3d, read-only: the number of couplings
+// whose transposed partner (the partner row's entry, or zero where the
+// partner has no stored row) differs.  Zero means the symmetry fix would
+// change nothing and the matrix is exactly symmetric.  The same pass counts
+// the nonzero entries of the owned active rows into nnz.
+__global__ void k_symmetry_count(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
+                                 unsigned long long* __restrict__ count, unsigned long long* __restrict__ nnz) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long c = 0, z = 0;
+    if (t < d.nrows) {
+        const int io = (int)(t % d.nxp);
+        const long long kj = t / d.nxp;
+        const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+        const int i = d.ilo + io, j = d.jlo + jo;
+        if (io < d.nxo && avert[(long long)(i - 1) + (long long)d.nvx * (j - 1)]) {
+            const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+            for (int m = 0; m < 27; ++m) {
+                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+                if (k0 + kA < 0 || k0 + kA >= d.nz) continue;
+                int qi = i + iA, qj = j + jA;
+                const double* q = row_image(d, qi, qj) ? row_ptr(d, A, k0 + kA, qi, qj) : nullptr;
+                const int mm = 26 - m;
+                const int ob[4] = {0, 1, 3, 2}, pb[4] = {0, 2, 3, 1};
+                for (int s = 0; s < 4; ++s) {
+                    const double v1 = a[(ob[s] * 27 + m) * WAVE];
+                    const double v2_ = q ? q[(pb[s] * 27 + mm) * WAVE] : 0.0;
+                    c += (v1 != v2_) ? 1ull : 0ull;
+                }
+            }
+            // count_nonzeros_3d of the same rows (:10513-10579)
+            for (int cf = 0; cf < NCOEF; ++cf) z += (a[cf * WAVE] != 0.0) ? 1ull : 0ull;
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        c += __shfl_down(c, off, WAVE);
+        z += __shfl_down(z, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0 && c) atomicAdd(count, c);
+    if (threadIdx.x % WAVE == 0 && z) atomicAdd(nnz, z);
 }
 
 // check_symmetry_assembled_matrix_3d restated per entry: an entry and its

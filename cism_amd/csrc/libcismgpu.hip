@@ -74,7 +74,11 @@ struct cismgpu_system {
     void* asm_buf = nullptr;
     size_t asm_bytes = 0;
     double* efvs = nullptr;
-    double* Atmp = nullptr;      // second matrix buffer for the symmetry fix
+    double* Atmp = nullptr;      // second matrix buffer for the symmetry fix (only when a pair differs)
+    double* elemq = nullptr;     // element-once assembly: inverse Jacobian and f per element and point
+    bool raw_ready = false;      // stage holds the raw Auu/Avv of the current matrix
+    bool saved_raw_ready = false;
+    bool sym_exact = false;      // the assembled FP64 matrix is known to be exactly symmetric
     double* work = nullptr;      // Picard work vectors: slot q = (u, v) at work + 2*q*vlen
     int nwork = 0;
     double* saved = nullptr;     // saved assembly (A, b, preconditioner, raw diagonal blocks)
@@ -368,6 +372,15 @@ int setup_tri(cismgpu_system* S, int mode) {
         for (int q = 0; q < 13; ++q) { *f[q] = T.a + q * n2; *g[q] = T.a32 + q * n2; }
     }
     CG_CHECK(hipMemsetAsync(T.a, 0, 13 * n2 * sizeof(double), S->s));
+    // raw Auu/Avv of an assembled matrix: built from the owned rows and the
+    // halo rule the first time a line-solve preconditioner needs them
+    if (!S->raw_ready) {
+        const long long nraw27 = 27LL * d.nz * n2;
+        if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw27 * sizeof(double)));
+        k_rows_to_raw<<<nblocks(nraw27, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
+        k_rows_to_raw<<<nblocks(nraw27, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw27);
+        S->raw_ready = true;
+    }
     // scratch for sums: use rsu.. as sub/diag/sup holders
     double *su_s = T.rsu, *su_d = T.rsv, *su_p = T.zsu, *sv_s = T.zsv, *sv_d = T.gam, *sv_p = T.Asv;
     // stage[0] holds Auu, stage[1] Avv (raw layout)
@@ -632,15 +645,17 @@ int ensure_half64(cismgpu_system* S) {
     const Dims& d = S->d;
     const char* env = getenv("CISMGPU_SPMV_FULL");
     if (S->two_d || (env && env[0] == '1') || !half_storage_fits(d)) return CISMGPU_OK;
-    unsigned long long* cnt = nullptr;
-    unsigned long long h = 1;
-    CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
-    CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
-    k_check_sym<double><<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, cnt);
-    CG_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, S->s));
-    CG_CHECK(hipStreamSynchronize(S->s));
-    hipFree(cnt);
-    if (h != 0) return CISMGPU_OK;
+    if (!S->sym_exact) {   // the assembly's symmetry count already showed it otherwise
+        unsigned long long* cnt = nullptr;
+        unsigned long long h = 1;
+        CG_CHECK(hipMalloc(&cnt, sizeof(unsigned long long)));
+        CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
+        k_check_sym<double><<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, cnt);
+        CG_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, S->s));
+        CG_CHECK(hipStreamSynchronize(S->s));
+        hipFree(cnt);
+        if (h != 0) return CISMGPU_OK;
+    }
     const long long ns = d.nrows * NSLOT * 4;
     if (!S->A64h) CG_CHECK(hipMalloc(&S->A64h, ns * sizeof(double)));
     k_convert_sym64<<<nblocks(ns, 256), 256, 0, S->s>>>(d.nrows, S->A64, S->A64h);
@@ -865,7 +880,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
                     (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf, (void*)S->efva2, (void*)S->rs_part, (void*)S->rs_bad,
-                    (void*)S->rs_out})
+                    (void*)S->rs_out, (void*)S->elemq})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -917,6 +932,8 @@ static int set_matrix_impl(cismgpu_system* S, const int32_t* active_vertex, int 
     CG_CHECK(hipMemsetAsync(tmp, 0, d.vlen * sizeof(double), S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_matrix = true;
+    S->raw_ready = true;
+    S->sym_exact = false;
     S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
     S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
@@ -1472,10 +1489,21 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     { const int st_ = reset_scal(S); if (st_) return st_; }
     const double* uv = S->V(0);
     const double* vv = S->V(1);
+    // element-once assembly (CISMGPU_ASSEMBLY=gather: the per-row gather
+    // that evaluates every element once per row)
+    const char* aenv = getenv("CISMGPU_ASSEMBLY");
+    const bool gather = aenv && !strcmp(aenv, "gather");
+    if (!gather && !S->elemq) CG_CHECK(hipMalloc(&S->elemq, (size_t)8 * EQ * nfl * sizeof(double)));
     hipEventRecord(e0, S->s);
-    k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
-    hipEventRecord(e1, S->s);
-    k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
+    if (gather) {
+        k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
+        hipEventRecord(e1, S->s);
+        k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
+    } else {
+        k_elem<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->elemq, S->efvs, S->scal + S_NAN);
+        k_rows<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, S->elemq, S->A64);
+        hipEventRecord(e1, S->s);
+    }
     if (in->beta_internal) {
         if (a.beta_std)
             k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, d.nz - 1, 1, a.beta, a.beta, S->A64,
@@ -1488,27 +1516,39 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     else
         k_load_gravity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, a, S->V(2), S->V(3));
     if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d, a, lat, S->V(2), S->V(3));
-    k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
+    // Dirichlet rows and columns (no prescribed nodes: nothing to do)
+    if (in->umask_dirichlet || in->vmask_dirichlet)
+        k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
     halo<double>(S, S->V(2), S->V(3), d.nz);
     unsigned long long* cnt = nullptr;
-    CG_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
-    CG_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), S->s));
-    if (in->check_symmetry) {
-        if (!S->Atmp) CG_CHECK(hipMalloc(&S->Atmp, d.nrows * NCOEF * sizeof(double)));
-        k_symmetry<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a.avert, S->A64, S->Atmp, cnt + 1);
-        std::swap(S->A64, S->Atmp);
+    CG_CHECK(hipMalloc(&cnt, 3 * sizeof(unsigned long long)));
+    CG_CHECK(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), S->s));
+    // symmetry fix: count the differing pairs first; the copy-and-average
+    // pass runs only when one exists
+    S->sym_exact = false;
+    {
+        unsigned long long ndiff = 1;
+        k_symmetry_count<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt + 2, cnt);
+        CG_CHECK(hipMemcpyAsync(&ndiff, cnt + 2, sizeof(ndiff), hipMemcpyDeviceToHost, S->s));
+        CG_CHECK(hipStreamSynchronize(S->s));
+        if (ndiff == 0) {
+            S->sym_exact = true;
+        } else if (in->check_symmetry) {
+            if (!S->Atmp) CG_CHECK(hipMalloc(&S->Atmp, d.nrows * NCOEF * sizeof(double)));
+            k_symmetry<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a.avert, S->A64, S->Atmp, cnt + 1);
+            std::swap(S->A64, S->Atmp);
+            // the averaged matrix is counted again
+            CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
+            k_count_nnz<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt);
+        }
     }
-    k_count_nnz<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt);
     // preconditioner coefficients: owned rows, then the halo rule on the ring
     CG_CHECK(hipMemsetAsync(S->M64, 0, 8 * d.vlen * sizeof(double), S->s));
     k_precond_from_rows<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64, S->M64, S->M64 + 6 * d.vlen);
     for (int f = 0; f < 8; f += 2) halo<double>(S, S->M64 + f * d.vlen, S->M64 + (f + 1) * d.vlen, d.nz);
-    // raw Auu/Avv for the horizontal tridiagonal preconditioners
-    const long long nraw = 27LL * d.nz * n2;
-    if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
-    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
-    k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw);
+    // the raw Auu/Avv of the line-solve preconditioners are built when first needed
+    S->raw_ready = false;
     hipEventRecord(e2, S->s);
     unsigned long long hc[2] = {0, 0};
     CG_CHECK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, S->s));
@@ -1646,6 +1686,8 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     if (!S->stage) CG_CHECK(hipMalloc(&S->stage, 2 * nraw * sizeof(double)));
     k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 0, S->A64, S->stage);
     k_rows_to_raw<<<nblocks(nraw, 256), 256, 0, S->s>>>(d, 3, S->A64, S->stage + nraw);
+    S->raw_ready = true;
+    S->sym_exact = false;
     hipEventRecord(e2, S->s);
     unsigned long long hc[2] = {0, 0};
     CG_CHECK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, S->s));
@@ -1852,6 +1894,7 @@ extern "C" int32_t cismgpu_system_save_assembly(cismgpu_system* S) {
         if (sizes[q]) CG_CHECK(hipMemcpyAsync(parts[q], src[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_saved = true;
+    S->saved_raw_ready = S->raw_ready;
     return CISMGPU_OK;
 }
 
@@ -1868,6 +1911,8 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
         if (sizes[q]) CG_CHECK(hipMemcpyAsync(dst[q], parts[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->have_matrix = true;
+    S->raw_ready = S->saved_raw_ready;
+    S->sym_exact = false;
     S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
     S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
