@@ -69,7 +69,11 @@ def main():
         sys.exit(2)
 
     if args.only_ensemble:
-        print(json.dumps({"ensemble": run_ensemble(args)}), flush=True)
+        # the STREAM probe runs too: its copy kernel of known size calibrates
+        # the hardware counters of a profiled run of this leg
+        from cism_amd import _lib
+        hbm = {k: round(v, 1) for k, v in _lib.bench_stream(2 << 30, 3).items()}
+        print(json.dumps({"ensemble": run_ensemble(args), "hbm_measured_gbps": hbm}), flush=True)
         return
 
     import numpy as np

@@ -6,12 +6,14 @@
 // residual (:9693-9894).
 //
 // Owner-computes gather: one thread per owned node builds that node's whole
-// matrix row from the (up to) 8 elements that contain it, visiting them in
+// matrix row from the (up to) 8 elements that contain it.  No two threads
+// write the same row, so the scatter needs neither atomics nor colouring
+// passes, every coefficient is written exactly once, and the result is
+// deterministic.  The default path (k_elem, k_rows) evaluates each element's
+// quadrature-point data once and gathers the rows from it; k_assemble
+// evaluates every element once per row it contributes to, visiting them in
 // the reference's element order (cells j-major then i, layer above before
-// layer below) so every coefficient is summed in the same order as
-// element_to_global_matrix_3d.  No two threads write the same row, so the
-// scatter needs neither atomics nor colouring passes, every coefficient is
-// written exactly once, and the result is deterministic.
+// layer below), the order of element_to_global_matrix_3d.
 #include "cismgpu_internal.h"
 
 namespace cg {
@@ -749,10 +751,11 @@ __device__ __forceinline__ const double* row_ptr(const Dims& d, const double* A,
     return A + ((r / WAVE) * NCOEF) * WAVE + (r % WAVE);
 }
 
-// check_symmetry_assembled_matrix_3d, read-only: the number of couplings
-// whose transposed partner (the partner row's entry, or zero where the
-// partner has no stored row) differs.  Zero means the symmetry fix would
-// change nothing and the matrix is exactly symmetric.  The same pass counts
+// check_symmetry_assembled_matrix_3d, read-only: the number of coupling
+// pairs that differ (an entry and its transposed partner: the partner
+// row's entry, or zero where the partner has no stored row).  Zero means
+// the symmetry fix would change nothing and the matrix is exactly
+// symmetric.  The same pass counts
 // the nonzero entries of the owned active rows into nnz.
 __global__ void k_symmetry_count(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
                                  unsigned long long* __restrict__ count, unsigned long long* __restrict__ nnz) {
@@ -765,11 +768,18 @@ __global__ void k_symmetry_count(Dims d, const int32_t* __restrict__ avert, cons
         const int i = d.ilo + io, j = d.jlo + jo;
         if (io < d.nxo && avert[(long long)(i - 1) + (long long)d.nvx * (j - 1)]) {
             const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+            // each pair is looked at from one side: a lower coupling (m < 13)
+            // against its stored partner, the centre's uv against its vu, and
+            // any coupling to a node without a stored row against zero
+            c += (a[(1 * 27 + 13) * WAVE] != a[(2 * 27 + 13) * WAVE]) ? 1ull : 0ull;
             for (int m = 0; m < 27; ++m) {
+                if (m == 13) continue;
                 const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
                 if (k0 + kA < 0 || k0 + kA >= d.nz) continue;
                 int qi = i + iA, qj = j + jA;
                 const double* q = row_image(d, qi, qj) ? row_ptr(d, A, k0 + kA, qi, qj) : nullptr;
+                // an active stored partner checks the pair from its side (its m < 13)
+                if (q && m > 13 && avert[(long long)(qi - 1) + (long long)d.nvx * (qj - 1)]) continue;
                 const int mm = 26 - m;
                 const int ob[4] = {0, 1, 3, 2}, pb[4] = {0, 2, 3, 1};
                 for (int s = 0; s < 4; ++s) {
