@@ -1,21 +1,24 @@
 !-----------------------------------------------------------------------------
-! glissade_velo_higher_pcg_gpu.F90
+! glissade_velo_higher_pcg.F90 (drop-in replacement)
 !
 ! Drop-in Fortran entry points for the structured 3-D and 2-D velocity
 ! solvers of CISM's Glissade dycore, backed by the MI355X library
-! libcismgpu.so through its C ABI (include/cismgpu.h).  The public
-! subroutines keep the names and argument lists of module
-! glissade_velo_higher_pcg (glissade_velo_higher_pcg.F90:67-78, :827-838,
-! :1541-1552, :2632-2643, :5528-5540, :5656-5663) and the
+! libcismgpu.so through its C ABI (include/cismgpu.h).  The module keeps
+! the reference module's name, glissade_velo_higher_pcg, and its public
+! subroutines keep the names and argument lists of the reference's
+! (glissade_velo_higher_pcg.F90:56-58, :67-78, :827-838, :1541-1552,
+! :2632-2643, :5528-5540, :5656-5663) and the
 ! Auu/Auv/Avu/Avv(27,nz,nx-1,ny-1) and (nx-1,ny-1,9) block layouts, so a
-! caller switches by changing the module named in its use statement.
+! build switches by compiling this file in place of
+! libglissade/glissade_velo_higher_pcg.F90; the caller's use statement
+! (glissade_velo_higher.F90:77-80) stays as it is.
 !
 ! Arrays are passed by address (no copies on the host side).  A nonzero
 ! library status is reported through write_log(..., GM_FATAL), as the
 ! reference does for a NaN in the PCG scalars; non-convergence is silent
 ! (niters = maxiters), as in the reference.
 !-----------------------------------------------------------------------------
-module glissade_velo_higher_pcg_gpu
+module glissade_velo_higher_pcg
 
   use iso_c_binding, only: c_int32_t, c_double, c_char, c_ptr, c_null_char, c_associated, c_f_pointer
   use ieee_arithmetic, only: ieee_value, ieee_quiet_nan
@@ -433,4 +436,4 @@ contains
     if (status /= 0) call report(status, 'matvec_multiply_structured_2d')
   end subroutine matvec_multiply_structured_2d
 
-end module glissade_velo_higher_pcg_gpu
+end module glissade_velo_higher_pcg

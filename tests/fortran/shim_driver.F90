@@ -5,7 +5,7 @@
 program shim_driver
   use glimmer_global, only: dp
   use cism_parallel, only: parallel_type
-  use glissade_velo_higher_pcg_gpu, only: pcg_solver_standard_3d, pcg_solver_chrongear_3d, &
+  use glissade_velo_higher_pcg, only: pcg_solver_standard_3d, pcg_solver_chrongear_3d, &
                                           pcg_solver_standard_2d, pcg_solver_chrongear_2d
   implicit none
   integer :: hdr(12), nx, ny, nz, bc, algo, precond, ncheck, maxiters, niters, i, j, k, m, u

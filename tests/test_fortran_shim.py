@@ -1,4 +1,4 @@
-"""The Fortran drop-in (cism_amd/fortran/glissade_velo_higher_pcg_gpu.F90)
+"""The Fortran drop-in (cism_amd/fortran/glissade_velo_higher_pcg.F90)
 keeps the reference subroutine signatures and reaches the HIP library
 through iso_c_binding.  CPU: it compiles and links with amdflang against
 test doubles of the CISM modules it uses.  GPU: a Fortran driver calls
@@ -27,7 +27,7 @@ def build_driver():
 def test_shim_compiles_and_links():
     build_driver()
     assert os.path.exists(DRIVER)
-    syms = subprocess.run(["nm", os.path.join(FDIR, "_build", "glissade_velo_higher_pcg_gpu.o")],
+    syms = subprocess.run(["nm", os.path.join(FDIR, "_build", "glissade_velo_higher_pcg.o")],
                           capture_output=True, text=True, check=True).stdout
     for name in ("pcg_solver_standard_3d", "pcg_solver_chrongear_3d", "matvec_multiply_structured_3d",
                  "pcg_solver_standard_2d", "pcg_solver_chrongear_2d", "matvec_multiply_structured_2d"):
