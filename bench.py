@@ -557,6 +557,13 @@ def run_ensemble(args):
         log(f"ensemble solve: {solve_ms:.1f} ms, {info['steps']} steps, converged {info['converged']}")
         by, inner_b, outer_b = ensemble_bytes(probs, E, info["matrix_storage"])
         gbps = by / (solve_ms * 1e-3) / 1e9
+        # HBM bytes per launch from the committed counter runs of this leg
+        traffic = None
+        try:
+            with open(os.path.join(ROOT, "profiles", "ensemble_pmc_256.json")) as f:
+                traffic = json.load(f)["kernels"]["cg::k_ensemble<16>"]["traffic_bytes"]
+        except (OSError, KeyError, ValueError):
+            pass
         out = dict(members=args.ensemble, owned_nodes=int(nodes), solve_ms=round(solve_ms, 2),
                    solve_ms_runs=[round(r["solve_ms"], 2) for r in runs],
                    member_solves_per_s=round(args.ensemble / (solve_ms * 1e-3), 1),
@@ -569,7 +576,8 @@ def run_ensemble(args):
                              "algorithmic_bytes_per_solve": int(by), "achieved": round(gbps, 1),
                              "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
                              "bytes_per_node_inner_iteration": inner_b, "bytes_per_node_refinement_step": outer_b,
-                             "traffic_source": "profiles/ensemble_pmc_256.json"})
+                             "traffic": traffic,
+                             "traffic_source": "profiles/ensemble_pmc_256.json" if traffic else None})
         if not args.only_ensemble:
             seq_ms, max_err = 0.0, 0.0
             for m, sv in enumerate(E.solvers):
