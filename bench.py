@@ -226,7 +226,14 @@ def main():
     twod = run_twod(args, g, bp.active_vertex, twod_sys) if twod_sys is not None else None
     del s, bp, twod_sys
     cfg2 = run_config2(args) if args.config2_n > 0 else None
-    ens = run_ensemble(args) if args.ensemble > 0 else None
+    ens_sample = [] if not args.no_cpu else None
+    ens = run_ensemble(args, ens_sample) if args.ensemble > 0 else None
+    if ens_sample and cpu is not None:
+        try:
+            cpu["ensemble_members"] = ensemble_oracle_check(ens_sample)
+        except Exception as e:
+            log(f"ensemble oracle check failed: {e!r}")
+    del ens_sample
     pic = run_marine_picard(args) if args.picard_n > 0 else None
 
     errs = oracle_cmp or {}
@@ -352,6 +359,36 @@ def run_cpu_legs(args, g, act, s, x64, xm, xb, it64):
                   matrix_bytes_uploaded=int(4 * Auu.nbytes))
     log(f"host-array drop-in call: {dropin}")
     return cpu, cmp, dropin
+
+
+def ensemble_oracle_check(sample):
+    """Part of the CPU-baseline leg: the oracle's FP64 Chronopoulos-Gear
+    solve (all host cores) of a few ensemble members' exported systems, each
+    member's batched mixed-precision solution compared with it, and the
+    oracle's time per member beside the batched launch."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    ncores = min(os.cpu_count() or 1, 16)
+    rows, worst = [], 0.0
+    for m, g, act, A, b, x in sample:
+        ou, ov = g.zeros_vec(), g.zeros_vec()
+        tc = time.perf_counter()
+        st, oerr, onit = po.pcg_solve(g, act, *A, *b, ou, ov, precond=2, tolerance=1e-12, maxiters=50000,
+                                      nthreads=ncores)
+        t = time.perf_counter() - tc
+        own = g.owned_slice()
+        xo = np.stack([ou[own], ov[own]])
+        err = float(np.linalg.norm(np.stack([x[0][own], x[1][own]]) - xo) / np.linalg.norm(xo))
+        worst = max(worst, err)
+        rows.append(dict(member=m, nx=g.nx, ny=g.ny, nz=g.nz, owned_nodes=int(g.n_owned_nodes),
+                         oracle_iterations=int(onit), oracle_status=int(st), oracle_s=round(t, 3),
+                         rel_err_vs_oracle=err))
+    out = dict(members=rows, max_rel_err_vs_oracle=worst, meets_1e6=bool(worst <= 1e-6), cores=ncores,
+               note="ensemble members' batched FP64-refined FP32 solutions (tolerance 1e-10) against the oracle's "
+                    "FP64 solve of the same member system to 1e-12")
+    log(f"ensemble members against the oracle: {out}")
+    return out
 
 
 def oracle_float(v):
@@ -531,7 +568,7 @@ def ensemble_bytes(probs, E, storage):
     return total, inner, outer
 
 
-def run_ensemble(args):
+def run_ensemble(args, sample=None):
     """Batched solve of a seeded ensemble of uneven ice streams (side 32..192
     owned vertices, nz in {5,7,11,15}) in one persistent launch, against the
     same members solved one after another by the single-system mixed solver;
@@ -580,6 +617,11 @@ def run_ensemble(args):
                              "traffic_source": "profiles/ensemble_pmc_256.json" if traffic else None})
         if not args.only_ensemble:
             seq_ms, max_err = 0.0, 0.0
+            # members checked against the oracle by the CPU leg: the largest
+            # of each quarter of the ensemble
+            sizes = [p.grid.n_owned_nodes for p in probs]
+            q = max(1, len(probs) // 4)
+            picks = {max(range(a, min(a + q, len(probs))), key=lambda k: sizes[k]) for a in range(0, len(probs), q)}
             for m, sv in enumerate(E.solvers):
                 sv.sys.zero_x()
                 r = sv.sys.solve(precond=2, tolerance=1e-10, maxiters=20000, precision=pcg.MIXED_FP32,
@@ -594,6 +636,9 @@ def run_ensemble(args):
                 den = np.sqrt((ru[own] ** 2).sum() + (rv[own] ** 2).sum())
                 num = np.sqrt(((xu - ru)[own] ** 2).sum() + ((xv - rv)[own] ** 2).sum())
                 max_err = max(max_err, float(num / den) if den > 0 else 0.0)
+                if sample is not None and m in picks:
+                    (Auu, Auv, Avu, Avv), (bu, bv) = sv.sys.get_matrix()
+                    sample.append((m, g, sv.active_vertex, (Auu, Auv, Avu, Avv), (bu, bv), (xu, xv)))
             out.update(sequential_mixed_ms=round(seq_ms, 2), speedup_vs_sequential=round(seq_ms / solve_ms, 3),
                        max_rel_err_vs_member_fp64=max_err, meets_1e6=bool(max_err <= 1e-6))
         E.close()

@@ -219,13 +219,14 @@ __global__ __launch_bounds__(64) void k_assemble(Dims d, AsmDev in, const double
 // contain it: the basis-function derivatives from the inverse Jacobian and
 // the row of the element matrix, added point by point into the row.  The
 // node's elements are visited layer by layer (the layer above the node,
-// then the layer below) and, within a layer, in the reference's cell order
-// (j-major, then i); this is one global order of the elements, and every
-// product is formed as in compute_element_matrix, so entry (P, Q) and
-// entry (Q, P) are summed from equal terms in the same order: the
-// assembled matrix is exactly symmetric.  Keeping the couplings to the level
-// above until the first layer is done, and those to the level below from the
-// second layer on, holds 72 of the row's 108 coefficients in registers.
+// then the layer below); within a layer, point by point, and for each point
+// the cells in the reference's cell order (j-major, then i).  This is one
+// global order of the terms, and every product is formed as in
+// compute_element_matrix, so entry (P, Q) and entry (Q, P) are summed from
+// equal terms in the same order: the assembled matrix is exactly symmetric.
+// Keeping the couplings to the level above until the first layer is done,
+// and those to the level below from the second layer on, holds 72 of the
+// row's 108 coefficients in registers.
 //
 // Point data at E[(p * EQ + q) * ne + e], q = 0..8 the inverse Jacobian
 // (row-major), q = 9 the factor f; e = (ci-1) + nx ((cj-1) + ny (kk-1)),
@@ -308,96 +309,174 @@ __global__ __launch_bounds__(128) void k_elem(Dims d, AsmDev in, const double* _
     efvs[(long long)(kk - 1) + (long long)(d.nz - 1) * cell] = out;
 }
 
-// Row NR of the element matrix of element e, from its point data, added
-// into the row: couplings to the element's upper-face nodes (kshift 0) go
-// to acc_up, those to its lower-face nodes to acc_lo, at index
-// (iA + 1) + 3 (jA + 1) of the 9 horizontal neighbours.
+// Point p of row NR of the element matrix of element e, from its point
+// data, added into the row: couplings to the element's upper-face nodes
+// (kshift 0) go to acc_up, those to its lower-face nodes to acc_lo, at
+// index (iA + 1) + 3 (jA + 1) of the 9 horizontal neighbours.
 template <int NR>
-__device__ __forceinline__ void add_row(const double* __restrict__ E, long long ne, long long e,
-                                        double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
-#pragma unroll 1
-    for (int p = 0; p < 8; ++p) {
-        const double* q = E + (long long)p * EQ * ne + e;
-        double Ji[9];
+__device__ __forceinline__ void add_point(const double* __restrict__ E, long long ne, long long e, int p,
+                                          double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
+    const double* q = E + (long long)p * EQ * ne + e;
+    double Ji[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Ji[k] = q[k * ne];
-        const double f = q[9 * ne];
-        const double ar = Ji[0] * c_dxr[NR][p] + Ji[1] * c_dyr[NR][p] + Ji[2] * c_dzr[NR][p];
-        const double br = Ji[3] * c_dxr[NR][p] + Ji[4] * c_dyr[NR][p] + Ji[5] * c_dzr[NR][p];
-        const double cr = Ji[6] * c_dxr[NR][p] + Ji[7] * c_dyr[NR][p] + Ji[8] * c_dzr[NR][p];
+    for (int k = 0; k < 9; ++k) Ji[k] = q[k * ne];
+    const double f = q[9 * ne];
+    const double ar = Ji[0] * c_dxr[NR][p] + Ji[1] * c_dyr[NR][p] + Ji[2] * c_dzr[NR][p];
+    const double br = Ji[3] * c_dxr[NR][p] + Ji[4] * c_dyr[NR][p] + Ji[5] * c_dzr[NR][p];
+    const double cr = Ji[6] * c_dxr[NR][p] + Ji[7] * c_dyr[NR][p] + Ji[8] * c_dzr[NR][p];
 #pragma unroll
-        for (int nc = 0; nc < 8; ++nc) {
-            const double a = c_dxr[nc][p], b = c_dyr[nc][p], c = c_dzr[nc][p];
-            const double dxn = Ji[0] * a + Ji[1] * b + Ji[2] * c;
-            const double dyn = Ji[3] * a + Ji[4] * b + Ji[5] * c;
-            const double dzn = Ji[6] * a + Ji[7] * b + Ji[8] * c;
-            const int h = (nd_i(nc) - nd_i(NR) + 1) + 3 * (nd_j(nc) - nd_j(NR) + 1);
-            double(&acc)[4][9] = nd_k(nc) == 0 ? acc_up : acc_lo;
-            acc[0][h] = acc[0][h] + f * (4.0 * ar * dxn + br * dyn + cr * dzn);
-            acc[1][h] = acc[1][h] + f * (2.0 * ar * dyn + br * dxn);
-            acc[2][h] = acc[2][h] + f * (2.0 * br * dxn + ar * dyn);
-            acc[3][h] = acc[3][h] + f * (4.0 * br * dyn + ar * dxn + cr * dzn);
-        }
+    for (int nc = 0; nc < 8; ++nc) {
+        const double a = c_dxr[nc][p], b = c_dyr[nc][p], c = c_dzr[nc][p];
+        const double dxn = Ji[0] * a + Ji[1] * b + Ji[2] * c;
+        const double dyn = Ji[3] * a + Ji[4] * b + Ji[5] * c;
+        const double dzn = Ji[6] * a + Ji[7] * b + Ji[8] * c;
+        const int h = (nd_i(nc) - nd_i(NR) + 1) + 3 * (nd_j(nc) - nd_j(NR) + 1);
+        double(&acc)[4][9] = nd_k(nc) == 0 ? acc_up : acc_lo;
+        acc[0][h] = acc[0][h] + f * (4.0 * ar * dxn + br * dyn + cr * dzn);
+        acc[1][h] = acc[1][h] + f * (2.0 * ar * dyn + br * dxn);
+        acc[2][h] = acc[2][h] + f * (2.0 * br * dxn + ar * dyn);
+        acc[3][h] = acc[3][h] + f * (4.0 * br * dyn + ar * dxn + cr * dzn);
     }
 }
 
-// One thread per owned node (io fastest): the node's full row (108
-// coefficients) into the row-blocked matrix.
-__global__ __launch_bounds__(128) void k_rows(Dims d, AsmDev in, const double* __restrict__ E,
-                                              double* __restrict__ A) {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= d.nrows) return;
-    const int io = (int)(t % d.nxp);
-    const long long kj = t / d.nxp;
-    const int jo = (int)(kj % d.nyo), k = (int)(kj / d.nyo);   // k 0-based level
+// The four cells of one element layer around a node, point by point: for
+// each quadrature point the cells (i,j), (i+1,j), (i,j+1), (i+1,j+1) in
+// turn, as nodes N0..N3 of their elements.  Reading a point's data for the
+// four cells together keeps the reuse distance short: the lanes of a wave
+// read cells (i,j) and (i+1,j) from the same cache lines, and the other
+// wave of the workgroup (the next vertex row) reads cells (., j+1) as its
+// (., j).
+template <int N0, int N1, int N2, int N3>
+__device__ __forceinline__ void add_layer(const double* __restrict__ E, long long ne, long long e0, int nx,
+                                          unsigned on, double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
+#pragma unroll 1
+    for (int p = 0; p < 8; ++p) {
+        if (on & 1u) add_point<N0>(E, ne, e0, p, acc_up, acc_lo);
+        if (on & 2u) add_point<N1>(E, ne, e0 + 1, p, acc_up, acc_lo);
+        if (on & 4u) add_point<N2>(E, ne, e0 + nx, p, acc_up, acc_lo);
+        if (on & 8u) add_point<N3>(E, ne, e0 + nx + 1, p, acc_up, acc_lo);
+    }
+}
+
+// Rows on the rim of the owned region (first or last owned vertex row or
+// column, all levels): their partners may lie outside it.
+__device__ __forceinline__ bool rim_row(const Dims& d, int io, int jo) {
+    return io == 0 || io == d.nxo - 1 || jo == 0 || jo == d.nyo - 1;
+}
+
+// The rim rows per level: the first and the last owned vertex rows, then
+// the first and last columns in between; rim_row_at maps t < nz * that
+// count to the row-blocked row index.
+__host__ __device__ inline long long rim_rows_per_level(const Dims& d) {
+    return (long long)d.nxo * (d.nyo > 1 ? 2 : 1) + (long long)(d.nyo > 2 ? d.nyo - 2 : 0) * (d.nxo > 1 ? 2 : 1);
+}
+
+__device__ __forceinline__ long long rim_row_at(const Dims& d, long long t) {
+    const long long nr = rim_rows_per_level(d);
+    const int k = (int)(t / nr);
+    const long long r = t % nr;
+    int io, jo;
+    if (r < d.nxo) {
+        io = (int)r; jo = 0;
+    } else if (d.nyo > 1 && r < 2LL * d.nxo) {
+        io = (int)(r - d.nxo); jo = d.nyo - 1;
+    } else {
+        const long long q = r - (long long)d.nxo * (d.nyo > 1 ? 2 : 1);
+        const int w = d.nxo > 1 ? 2 : 1;
+        jo = 1 + (int)(q / w);
+        io = (q % w == 0) ? 0 : d.nxo - 1;
+    }
+    return ((long long)k * d.nyo + jo) * d.nxp + io;
+}
+
+// One thread per owned node: the node's full row (108 coefficients) into
+// the row-blocked matrix.  A workgroup is ROWS_J waves: 64 consecutive
+// vertices (io) of ROWS_J consecutive vertex rows (jo) of one level.  Each
+// coefficient sums its terms point by point and, within a point, cell by
+// cell in one global element order (layer, then j, then i), so entry (P, Q)
+// and entry (Q, P) are summed from equal terms in the same order: the
+// assembled matrix is exactly symmetric.  The layer below the node first
+// (couplings to the level above complete after it), then the layer above
+// (the level below): 72 of the 108 coefficients live at a time.  With nnz
+// given, the nonzero coefficients of the active rows off the rim are
+// counted into nnz[0] (count_nonzeros_3d, :10513-10579; the rim rows are
+// counted after the boundary passes by k_symmetry_rim).
+constexpr int ROWS_J = 2;
+__host__ __device__ inline long long rows_blocks(const Dims& d) {
+    return (long long)(d.nxp / WAVE) * ((d.nyo + ROWS_J - 1) / ROWS_J) * d.nz;
+}
+
+__global__ __launch_bounds__(WAVE * ROWS_J) __attribute__((amdgpu_waves_per_eu(2))) void k_rows(Dims d, AsmDev in, const double* __restrict__ E,
+                                                        double* __restrict__ A, unsigned long long* __restrict__ nnz) {
+    const int nbx = d.nxp / WAVE, nbj = (d.nyo + ROWS_J - 1) / ROWS_J;
+    const long long b = blockIdx.x;
+    const int bx = (int)(b % nbx);
+    const long long rb = b / nbx;
+    const int jo = (int)(rb % nbj) * ROWS_J + (int)(threadIdx.x / WAVE);
+    const int k = (int)(rb / nbj);                              // 0-based level
+    if (jo >= d.nyo) return;                                    // whole wave
+    const int io = bx * WAVE + (int)(threadIdx.x % WAVE);
+    const long long t = ((long long)k * d.nyo + jo) * d.nxp + io;
     const long long ne = (long long)(d.nz - 1) * d.nx * d.ny;
     double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
     const int i = d.ilo + io, j = d.jlo + jo;
     const int kf = k + 1;
     const bool on = io < d.nxo && in.avert[v2(d, i, j)];
     auto cell_on = [&](int ci, int cj) { return cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]; };
+    const unsigned cells = on ? ((cell_on(i, j) ? 1u : 0u) | (cell_on(i + 1, j) ? 2u : 0u) |
+                                 (cell_on(i, j + 1) ? 4u : 0u) | (cell_on(i + 1, j + 1) ? 8u : 0u))
+                              : 0u;
+    const bool count_row = nnz && on && !rim_row(d, io, jo);
+    unsigned z = 0;
     double mid[4][9];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-        for (int h = 0; h < 9; ++h) mid[b][h] = 0.0;
+        for (int h = 0; h < 9; ++h) mid[bb][h] = 0.0;
     {
         // layer kf - 1: the node is on the elements' lower face
         double above[4][9];
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-            for (int h = 0; h < 9; ++h) above[b][h] = 0.0;
-        if (on && kf - 1 >= 1) {
-            if (cell_on(i, j)) add_row<2>(E, ne, elem_id(d, i, j, kf - 1), above, mid);
-            if (cell_on(i + 1, j)) add_row<3>(E, ne, elem_id(d, i + 1, j, kf - 1), above, mid);
-            if (cell_on(i, j + 1)) add_row<1>(E, ne, elem_id(d, i, j + 1, kf - 1), above, mid);
-            if (cell_on(i + 1, j + 1)) add_row<0>(E, ne, elem_id(d, i + 1, j + 1, kf - 1), above, mid);
+            for (int h = 0; h < 9; ++h) above[bb][h] = 0.0;
+        if (cells && kf - 1 >= 1) add_layer<2, 3, 1, 0>(E, ne, elem_id(d, i, j, kf - 1), d.nx, cells, above, mid);
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+            for (int h = 0; h < 9; ++h) a[(bb * 27 + h) * WAVE] = above[bb][h];
+        if (count_row) {
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                for (int h = 0; h < 9; ++h) z += above[bb][h] != 0.0 ? 1u : 0u;
         }
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int h = 0; h < 9; ++h) a[(b * 27 + h) * WAVE] = above[b][h];
     }
     double below[4][9];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-        for (int h = 0; h < 9; ++h) below[b][h] = 0.0;
-    if (on && kf <= d.nz - 1) {
-        // layer kf: the node is on the elements' upper face
-        if (cell_on(i, j)) add_row<6>(E, ne, elem_id(d, i, j, kf), mid, below);
-        if (cell_on(i + 1, j)) add_row<7>(E, ne, elem_id(d, i + 1, j, kf), mid, below);
-        if (cell_on(i, j + 1)) add_row<5>(E, ne, elem_id(d, i, j + 1, kf), mid, below);
-        if (cell_on(i + 1, j + 1)) add_row<4>(E, ne, elem_id(d, i + 1, j + 1, kf), mid, below);
-    }
+        for (int h = 0; h < 9; ++h) below[bb][h] = 0.0;
+    // layer kf: the node is on the elements' upper face
+    if (cells && kf <= d.nz - 1) add_layer<6, 7, 5, 4>(E, ne, elem_id(d, i, j, kf), d.nx, cells, mid, below);
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
         for (int h = 0; h < 9; ++h) {
-            a[(b * 27 + 9 + h) * WAVE] = mid[b][h];
-            a[(b * 27 + 18 + h) * WAVE] = below[b][h];
+            a[(bb * 27 + 9 + h) * WAVE] = mid[bb][h];
+            a[(bb * 27 + 18 + h) * WAVE] = below[bb][h];
         }
+    if (nnz) {
+        if (count_row) {
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                for (int h = 0; h < 9; ++h) z += (mid[bb][h] != 0.0 ? 1u : 0u) + (below[bb][h] != 0.0 ? 1u : 0u);
+        }
+        unsigned long long zz = z;
+        for (int off = WAVE / 2; off > 0; off >>= 1) zz += __shfl_down(zz, off, WAVE);
+        if (threadIdx.x % WAVE == 0 && zz) atomicAdd(nnz, zz);
+    }
 }
 
 // efvs(k,i,j) = mean over the quadrature points (:5604-5610), one thread
@@ -716,9 +795,11 @@ __global__ void k_dirichlet(Dims d, AsmDev in, const double* __restrict__ uv, co
 // Halo of the assembled matrix for the no_ice rule: rows of the last owned
 // vertex row/column are zeroed (parallel_mpi.F90:10912-10921 applied by
 // staggered_parallel_halo_real8_4d to A, glissade_velo_higher.F90:2893-2898).
+// Those rows are rim rows: one thread per rim row.
 __global__ void k_matrix_noice_edge(Dims d, double* __restrict__ A) {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (t >= d.nrows) return;
+    const long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (u >= rim_rows_per_level(d) * d.nz) return;
+    const long long t = rim_row_at(d, u);
     const int io = (int)(t % d.nxp);
     const long long kj = t / d.nxp;
     const int jo = (int)(kj % d.nyo);
@@ -755,49 +836,90 @@ __device__ __forceinline__ const double* row_ptr(const Dims& d, const double* A,
 // pairs that differ (an entry and its transposed partner: the partner
 // row's entry, or zero where the partner has no stored row).  Zero means
 // the symmetry fix would change nothing and the matrix is exactly
-// symmetric.  The same pass counts
-// the nonzero entries of the owned active rows into nnz.
-__global__ void k_symmetry_count(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
-                                 unsigned long long* __restrict__ count, unsigned long long* __restrict__ nnz) {
-    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    unsigned long long c = 0, z = 0;
-    if (t < d.nrows) {
-        const int io = (int)(t % d.nxp);
-        const long long kj = t / d.nxp;
-        const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
-        const int i = d.ilo + io, j = d.jlo + jo;
-        if (io < d.nxo && avert[(long long)(i - 1) + (long long)d.nvx * (j - 1)]) {
-            const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
-            // each pair is looked at from one side: a lower coupling (m < 13)
-            // against its stored partner, the centre's uv against its vu, and
-            // any coupling to a node without a stored row against zero
-            c += (a[(1 * 27 + 13) * WAVE] != a[(2 * 27 + 13) * WAVE]) ? 1ull : 0ull;
-            for (int m = 0; m < 27; ++m) {
-                if (m == 13) continue;
-                const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
-                if (k0 + kA < 0 || k0 + kA >= d.nz) continue;
-                int qi = i + iA, qj = j + jA;
-                const double* q = row_image(d, qi, qj) ? row_ptr(d, A, k0 + kA, qi, qj) : nullptr;
-                // an active stored partner checks the pair from its side (its m < 13)
-                if (q && m > 13 && avert[(long long)(qi - 1) + (long long)d.nvx * (qj - 1)]) continue;
-                const int mm = 26 - m;
-                const int ob[4] = {0, 1, 3, 2}, pb[4] = {0, 2, 3, 1};
-                for (int s = 0; s < 4; ++s) {
-                    const double v1 = a[(ob[s] * 27 + m) * WAVE];
-                    const double v2_ = q ? q[(pb[s] * 27 + mm) * WAVE] : 0.0;
-                    c += (v1 != v2_) ? 1ull : 0ull;
-                }
-            }
-            // count_nonzeros_3d of the same rows (:10513-10579)
-            for (int cf = 0; cf < NCOEF; ++cf) z += (a[cf * WAVE] != 0.0) ? 1ull : 0ull;
+// symmetric.  The same pass counts the nonzero entries of the owned active
+// rows into nnz.
+//
+// rim_only: only the rim rows are checked (every pair from their side) and
+// counted.  For the element-once rows with no pass after them that changes
+// couplings (no Dirichlet nodes, local basal friction), a pair of owned
+// nodes is summed in both rows from the same stored point data, with equal
+// products, in one global element order, so it is equal bit for bit; only
+// a pair with a node outside the owned region (a halo vertex or a periodic
+// image, whose elements are other copies) or a rim row zeroed by the no-ice
+// rule can differ, and each of those has a rim row.
+// The checks and the nonzero count of row t (rows-blocked index); all
+// pairs from this row's side when all_pairs, else each pair from one side.
+__device__ __forceinline__ void sym_row(const Dims& d, const int32_t* __restrict__ avert,
+                                        const double* __restrict__ A, long long t, bool all_pairs,
+                                        unsigned long long& c, unsigned long long& z) {
+    const int io = (int)(t % d.nxp);
+    const long long kj = t / d.nxp;
+    const int jo = (int)(kj % d.nyo), k0 = (int)(kj / d.nyo);
+    const int i = d.ilo + io, j = d.jlo + jo;
+    if (io >= d.nxo || !avert[(long long)(i - 1) + (long long)d.nvx * (j - 1)]) return;
+    const double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
+    // a lower coupling (m < 13) against its stored partner, the centre's uv
+    // against its vu, and any coupling to a node without a stored row
+    // against zero
+    c += (a[(1 * 27 + 13) * WAVE] != a[(2 * 27 + 13) * WAVE]) ? 1ull : 0ull;
+    for (int m = 0; m < 27; ++m) {
+        if (m == 13) continue;
+        const int iA = m % 3 - 1, jA = (m / 3) % 3 - 1, kA = m / 9 - 1;
+        if (k0 + kA < 0 || k0 + kA >= d.nz) continue;
+        int qi = i + iA, qj = j + jA;
+        const double* q = row_image(d, qi, qj) ? row_ptr(d, A, k0 + kA, qi, qj) : nullptr;
+        // an active stored partner checks the pair from its side (its m < 13)
+        if (!all_pairs && q && m > 13 && avert[(long long)(qi - 1) + (long long)d.nvx * (qj - 1)]) continue;
+        const int mm = 26 - m;
+        const int ob[4] = {0, 1, 3, 2}, pb[4] = {0, 2, 3, 1};
+        for (int s = 0; s < 4; ++s) {
+            const double v1 = a[(ob[s] * 27 + m) * WAVE];
+            const double v2_ = q ? q[(pb[s] * 27 + mm) * WAVE] : 0.0;
+            c += (v1 != v2_) ? 1ull : 0ull;
         }
     }
+    // count_nonzeros_3d of the same row (:10513-10579)
+    for (int cf = 0; cf < NCOEF; ++cf) z += (a[cf * WAVE] != 0.0) ? 1ull : 0ull;
+}
+
+__device__ __forceinline__ void sym_flush(unsigned long long c, unsigned long long z, unsigned long long* count,
+                                          unsigned long long* nnz) {
     for (int off = WAVE / 2; off > 0; off >>= 1) {
         c += __shfl_down(c, off, WAVE);
         z += __shfl_down(z, off, WAVE);
     }
     if (threadIdx.x % WAVE == 0 && c) atomicAdd(count, c);
     if (threadIdx.x % WAVE == 0 && z) atomicAdd(nnz, z);
+}
+
+// check_symmetry_assembled_matrix_3d, read-only: the number of coupling
+// pairs that differ (an entry and its transposed partner: the partner
+// row's entry, or zero where the partner has no stored row).  Zero means
+// the symmetry fix would change nothing and the matrix is exactly
+// symmetric.  The same pass counts the nonzero entries of the owned active
+// rows into nnz.
+__global__ void k_symmetry_count(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
+                                 unsigned long long* __restrict__ count, unsigned long long* __restrict__ nnz) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long c = 0, z = 0;
+    if (t < d.nrows) sym_row(d, avert, A, t, false, c, z);
+    sym_flush(c, z, count, nnz);
+}
+
+// The same check over the rim rows only, every pair from their side, one
+// thread per rim row.  For the element-once rows with no pass after them
+// that changes couplings (no Dirichlet nodes, local basal friction), a pair
+// of owned nodes is summed in both rows from the same stored point data,
+// with equal products, in one global element order, so it is equal bit for
+// bit; only a pair with a node outside the owned region (a halo vertex or a
+// periodic image, whose elements are other copies) or a rim row zeroed by
+// the no-ice rule can differ, and each of those has a rim row.
+__global__ void k_symmetry_rim(Dims d, const int32_t* __restrict__ avert, const double* __restrict__ A,
+                               unsigned long long* __restrict__ count, unsigned long long* __restrict__ nnz) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long c = 0, z = 0;
+    if (t < rim_rows_per_level(d) * d.nz) sym_row(d, avert, A, rim_row_at(d, t), true, c, z);
+    sym_flush(c, z, count, nnz);
 }
 
 // check_symmetry_assembled_matrix_3d restated per entry: an entry and its

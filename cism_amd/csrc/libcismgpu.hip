@@ -526,7 +526,11 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
         } else {
             scalars(S, ST_CG_ITER, S->partA, ncb, 4, S->partB, nsp, 2);
         }
-        halo<TV>(S, Azu, Azv, d.nz);
+        // no-ice rule with every zeroed halo column outside the owned range:
+        // the product writes owned rows only, so A z's halo stays as zeroed
+        // at the start and the halo pass is skipped
+        if (!(d.bc == 3 && d.ilo >= 3 && d.ihi <= d.nvx - 2 && d.jlo >= 3 && d.jhi <= d.nvy - 2))
+            halo<TV>(S, Azu, Azv, d.nz);
         const bool check = (iter % ncheck == 0) || iter == 5;
         const bool fuse = fuse_ok && precond == 2 && !(check && !given_r);
         if (fuse) launch_update<TV, 1, 0>(S, vv, M);
@@ -1494,6 +1498,19 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     const char* aenv = getenv("CISMGPU_ASSEMBLY");
     const bool gather = aenv && !strcmp(aenv, "gather");
     if (!gather && !S->elemq) CG_CHECK(hipMalloc(&S->elemq, (size_t)8 * EQ * nfl * sizeof(double)));
+    // counters: nnz, pairs left asymmetric by the fix, differing pairs
+    unsigned long long* cnt = nullptr;
+    CG_CHECK(hipMalloc(&cnt, 3 * sizeof(unsigned long long)));
+    CG_CHECK(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), S->s));
+    // The element-once rows are symmetric by construction between owned
+    // nodes; with no pass after them that changes couplings inside the
+    // owned region (Dirichlet rows and columns, standard basal quadrature)
+    // the symmetry check and the nonzero count visit only the rim rows, and
+    // the other rows are counted as they are written (CISMGPU_SYMCHECK=full:
+    // every row checked and counted).
+    const char* senv = getenv("CISMGPU_SYMCHECK");
+    const bool rim_only = !gather && !(senv && !strcmp(senv, "full")) && !in->umask_dirichlet &&
+                          !in->vmask_dirichlet && !(in->beta_internal && a.beta_std);
     hipEventRecord(e0, S->s);
     if (gather) {
         k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
@@ -1501,7 +1518,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         k_efvs<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->efvs);
     } else {
         k_elem<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, a, uv, vv, S->elemq, S->efvs, S->scal + S_NAN);
-        k_rows<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, S->elemq, S->A64);
+        k_rows<<<(unsigned)rows_blocks(d), WAVE * ROWS_J, 0, S->s>>>(d, a, S->elemq, S->A64, rim_only ? cnt : nullptr);
         hipEventRecord(e1, S->s);
     }
     if (in->beta_internal) {
@@ -1519,17 +1536,18 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     // Dirichlet rows and columns (no prescribed nodes: nothing to do)
     if (in->umask_dirichlet || in->vmask_dirichlet)
         k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
-    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
+    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, S->A64);
     halo<double>(S, S->V(2), S->V(3), d.nz);
-    unsigned long long* cnt = nullptr;
-    CG_CHECK(hipMalloc(&cnt, 3 * sizeof(unsigned long long)));
-    CG_CHECK(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), S->s));
     // symmetry fix: count the differing pairs first; the copy-and-average
     // pass runs only when one exists
     S->sym_exact = false;
     {
         unsigned long long ndiff = 1;
-        k_symmetry_count<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt + 2, cnt);
+        if (rim_only)
+            k_symmetry_rim<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt + 2,
+                                                                                      cnt);
+        else
+            k_symmetry_count<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt + 2, cnt);
         CG_CHECK(hipMemcpyAsync(&ndiff, cnt + 2, sizeof(ndiff), hipMemcpyDeviceToHost, S->s));
         CG_CHECK(hipStreamSynchronize(S->s));
         if (ndiff == 0) {
@@ -1672,7 +1690,7 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     if (lateral) k_lateral_bc<<<nblocks(n2, 128), 128, 0, S->s>>>(d3, a, lat, lu, lv);
     k_sum_levels<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, lu, lv, d, S->V(2), S->V(3));
     k_dirichlet_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->V(2), S->V(3));
-    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, S->A64);
+    if (d.bc == 3) k_matrix_noice_edge<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, S->A64);
     halo<double>(S, S->V(2), S->V(3), 1);
     unsigned long long* cnt = nullptr;
     CG_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));

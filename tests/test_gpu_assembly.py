@@ -79,6 +79,37 @@ def test_assembly_matches_oracle(require_gpu, name):
     assert info["asym_pairs"] == 0 and ref["nwarn_asym"] == 0
 
 
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_rim_symmetry_check_matches_full_check(require_gpu, name):
+    """The default symmetry check and nonzero count (rim rows only, the
+    other rows counted as they are written) against the check of every
+    row (CISMGPU_SYMCHECK=full): the same matrix bit for bit, the same
+    nnz and asymmetric-pair count, and the same storage choice and FP64
+    solve (half storage is taken only for an exactly symmetric matrix)."""
+    import os
+    p = PROBLEMS[name]
+    out = []
+    for mode in (None, "full"):
+        if mode:
+            os.environ["CISMGPU_SYMCHECK"] = mode
+        try:
+            s = BPSolver(p)
+            info = s.assemble(p.uvel, p.vvel)
+            (Auu, Auv, Avu, Avv), (bu, bv) = s.sys.get_matrix()
+            s.sys.zero_x()
+            r = s.sys.solve(precond=2, tolerance=1e-9, maxiters=2000)
+            x = np.stack(s.sys.get_x())
+            out.append((info, np.stack([Auu, Auv, Avu, Avv]), r, x))
+            s.sys.close()
+        finally:
+            os.environ.pop("CISMGPU_SYMCHECK", None)
+    (i0, A0, r0, x0), (i1, A1, r1, x1) = out
+    assert np.array_equal(A0, A1)
+    assert i0["nnz"] == i1["nnz"] and i0["asym_pairs"] == i1["asym_pairs"]
+    assert r0["spmv_kind"] == r1["spmv_kind"] and r0["niters"] == r1["niters"]
+    assert np.array_equal(x0, x1)
+
+
 @pytest.mark.parametrize("name", ["dome", "stream", "marine"])
 def test_assembled_solve_matches_oracle(require_gpu, name):
     p = PROBLEMS[name]
