@@ -309,53 +309,55 @@ __global__ __launch_bounds__(128) void k_elem(Dims d, AsmDev in, const double* _
     efvs[(long long)(kk - 1) + (long long)(d.nz - 1) * cell] = out;
 }
 
-// Point p of row NR of the element matrix of element e, from its point
-// data, added into the row: couplings to the element's upper-face nodes
-// (kshift 0) go to acc_up, those to its lower-face nodes to acc_lo, at
-// index (iA + 1) + 3 (jA + 1) of the 9 horizontal neighbours.
+// Tile of k_rows: 64 consecutive vertices (io) of ROWS_J consecutive vertex
+// rows (jo) of one level, one wavefront per vertex row.  Its rows touch the
+// cells (i0 .. i0+64) x (j0 .. j0+ROWS_J) of an element layer.
+constexpr int ROWS_J = 4;
+constexpr int RC_I = WAVE + 1;
+constexpr int RC = RC_I * (ROWS_J + 1);
+constexpr int RQ = 25;          // per cell and point: 8 x 3 basis derivatives, f
+
+// Point p's contribution of one cell (tile cell c) to row NR of the element
+// matrix, from the staged derivatives: couplings to the element's
+// upper-face nodes (kshift 0) go to acc_up, those to its lower-face nodes to
+// acc_lo, at index (iA + 1) + 3 (jA + 1) of the 9 horizontal neighbours.
+// Every rounding is spelled out (products, fused multiply-adds), so that the
+// (row, column) term and the (column, row) term of each block pair are
+// evaluated by the same operations on the same values whatever the
+// compiler's contraction choices: uu(P,Q) = uu(Q,P), vv(P,Q) = vv(Q,P) and
+// uv(P,Q) = vu(Q,P) bit for bit.
 template <int NR>
-__device__ __forceinline__ void add_point(const double* __restrict__ E, long long ne, long long e, int p,
-                                          double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
-    const double* q = E + (long long)p * EQ * ne + e;
-    double Ji[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Ji[k] = q[k * ne];
-    const double f = q[9 * ne];
-    const double ar = Ji[0] * c_dxr[NR][p] + Ji[1] * c_dyr[NR][p] + Ji[2] * c_dzr[NR][p];
-    const double br = Ji[3] * c_dxr[NR][p] + Ji[4] * c_dyr[NR][p] + Ji[5] * c_dzr[NR][p];
-    const double cr = Ji[6] * c_dxr[NR][p] + Ji[7] * c_dyr[NR][p] + Ji[8] * c_dzr[NR][p];
+__device__ __forceinline__ void add_point(const double (*__restrict__ sd)[RC], int c, double (&acc_up)[4][9],
+                                          double (&acc_lo)[4][9]) {
+    const double f = sd[24][c];
+    const double ar = sd[3 * NR][c], br = sd[3 * NR + 1][c], cr = sd[3 * NR + 2][c];
+    const double ar2 = 2.0 * ar, br2 = 2.0 * br, ar4 = 4.0 * ar, br4 = 4.0 * br;   // exact
 #pragma unroll
     for (int nc = 0; nc < 8; ++nc) {
-        const double a = c_dxr[nc][p], b = c_dyr[nc][p], c = c_dzr[nc][p];
-        const double dxn = Ji[0] * a + Ji[1] * b + Ji[2] * c;
-        const double dyn = Ji[3] * a + Ji[4] * b + Ji[5] * c;
-        const double dzn = Ji[6] * a + Ji[7] * b + Ji[8] * c;
+        const double dxn = sd[3 * nc][c], dyn = sd[3 * nc + 1][c], dzn = sd[3 * nc + 2][c];
         const int h = (nd_i(nc) - nd_i(NR) + 1) + 3 * (nd_j(nc) - nd_j(NR) + 1);
         double(&acc)[4][9] = nd_k(nc) == 0 ? acc_up : acc_lo;
-        acc[0][h] = acc[0][h] + f * (4.0 * ar * dxn + br * dyn + cr * dzn);
-        acc[1][h] = acc[1][h] + f * (2.0 * ar * dyn + br * dxn);
-        acc[2][h] = acc[2][h] + f * (2.0 * br * dxn + ar * dyn);
-        acc[3][h] = acc[3][h] + f * (4.0 * br * dyn + ar * dxn + cr * dzn);
+        const double kuu = __fma_rn(cr, dzn, __fma_rn(br, dyn, __dmul_rn(ar4, dxn)));
+        const double kuv = __fma_rn(ar2, dyn, __dmul_rn(br, dxn));
+        const double kvu = __fma_rn(br2, dxn, __dmul_rn(ar, dyn));
+        const double kvv = __fma_rn(cr, dzn, __fma_rn(ar, dxn, __dmul_rn(br4, dyn)));
+        acc[0][h] = __fma_rn(f, kuu, acc[0][h]);
+        acc[1][h] = __fma_rn(f, kuv, acc[1][h]);
+        acc[2][h] = __fma_rn(f, kvu, acc[2][h]);
+        acc[3][h] = __fma_rn(f, kvv, acc[3][h]);
     }
 }
 
-// The four cells of one element layer around a node, point by point: for
-// each quadrature point the cells (i,j), (i+1,j), (i,j+1), (i+1,j+1) in
-// turn, as nodes N0..N3 of their elements.  Reading a point's data for the
-// four cells together keeps the reuse distance short: the lanes of a wave
-// read cells (i,j) and (i+1,j) from the same cache lines, and the other
-// wave of the workgroup (the next vertex row) reads cells (., j+1) as its
-// (., j).
+// The four cells of one element layer around a node at one point, in the
+// reference's cell order: (i,j), (i+1,j), (i,j+1), (i+1,j+1) as nodes
+// N0..N3 of their elements.
 template <int N0, int N1, int N2, int N3>
-__device__ __forceinline__ void add_layer(const double* __restrict__ E, long long ne, long long e0, int nx,
-                                          unsigned on, double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
-#pragma unroll 1
-    for (int p = 0; p < 8; ++p) {
-        if (on & 1u) add_point<N0>(E, ne, e0, p, acc_up, acc_lo);
-        if (on & 2u) add_point<N1>(E, ne, e0 + 1, p, acc_up, acc_lo);
-        if (on & 4u) add_point<N2>(E, ne, e0 + nx, p, acc_up, acc_lo);
-        if (on & 8u) add_point<N3>(E, ne, e0 + nx + 1, p, acc_up, acc_lo);
-    }
+__device__ __forceinline__ void add_cells(const double (*__restrict__ sd)[RC], int c0, unsigned on,
+                                          double (&acc_up)[4][9], double (&acc_lo)[4][9]) {
+    if (on & 1u) add_point<N0>(sd, c0, acc_up, acc_lo);
+    if (on & 2u) add_point<N1>(sd, c0 + 1, acc_up, acc_lo);
+    if (on & 4u) add_point<N2>(sd, c0 + RC_I, acc_up, acc_lo);
+    if (on & 8u) add_point<N3>(sd, c0 + RC_I + 1, acc_up, acc_lo);
 }
 
 // Rows on the rim of the owned region (first or last owned vertex row or
@@ -390,44 +392,71 @@ __device__ __forceinline__ long long rim_row_at(const Dims& d, long long t) {
 }
 
 // One thread per owned node: the node's full row (108 coefficients) into
-// the row-blocked matrix.  A workgroup is ROWS_J waves: 64 consecutive
-// vertices (io) of ROWS_J consecutive vertex rows (jo) of one level.  Each
-// coefficient sums its terms point by point and, within a point, cell by
-// cell in one global element order (layer, then j, then i), so entry (P, Q)
-// and entry (Q, P) are summed from equal terms in the same order: the
-// assembled matrix is exactly symmetric.  The layer below the node first
-// (couplings to the level above complete after it), then the layer above
-// (the level below): 72 of the 108 coefficients live at a time.  With nnz
-// given, the nonzero coefficients of the active rows off the rim are
-// counted into nnz[0] (count_nonzeros_3d, :10513-10579; the rim rows are
-// counted after the boundary passes by k_symmetry_rim).
-constexpr int ROWS_J = 2;
+// the row-blocked matrix; one workgroup per tile.  For each element layer
+// and quadrature point, the workgroup first computes the basis-function
+// derivatives of every active cell of the tile from its stored inverse
+// Jacobian (once per cell, not once per row) into LDS, then each thread adds
+// the point's terms of its (up to) 4 cells.  Each coefficient sums its
+// terms point by point and, within a point, cell by cell in one global
+// element order (layer, then j, then i), and entry (P, Q) and entry (Q, P)
+// are formed from the same staged values, so the assembled matrix is
+// exactly symmetric.  The layer below the node first (couplings to the
+// level above complete after it), then the layer above (the level below):
+// 72 of the 108 coefficients live at a time.  With nnz given, the nonzero
+// coefficients of the active rows off the rim are counted into nnz[0]
+// (count_nonzeros_3d, :10513-10579; the rim rows are counted after the
+// boundary passes by k_symmetry_rim).
 __host__ __device__ inline long long rows_blocks(const Dims& d) {
     return (long long)(d.nxp / WAVE) * ((d.nyo + ROWS_J - 1) / ROWS_J) * d.nz;
 }
 
-__global__ __launch_bounds__(WAVE * ROWS_J) __attribute__((amdgpu_waves_per_eu(2))) void k_rows(Dims d, AsmDev in, const double* __restrict__ E,
+__global__ __launch_bounds__(WAVE * ROWS_J) void k_rows(Dims d, AsmDev in, const double* __restrict__ E,
                                                         double* __restrict__ A, unsigned long long* __restrict__ nnz) {
+    constexpr int NT = WAVE * ROWS_J;
+    __shared__ double sd[RQ][RC];
     const int nbx = d.nxp / WAVE, nbj = (d.nyo + ROWS_J - 1) / ROWS_J;
     const long long b = blockIdx.x;
     const int bx = (int)(b % nbx);
     const long long rb = b / nbx;
-    const int jo = (int)(rb % nbj) * ROWS_J + (int)(threadIdx.x / WAVE);
-    const int k = (int)(rb / nbj);                              // 0-based level
-    if (jo >= d.nyo) return;                                    // whole wave
-    const int io = bx * WAVE + (int)(threadIdx.x % WAVE);
-    const long long t = ((long long)k * d.nyo + jo) * d.nxp + io;
+    const int jt = (int)(rb % nbj) * ROWS_J;                    // first vertex row of the tile
+    const int k = (int)(rb / nbj);                              // 0-based level (uniform)
+    const int w = (int)(threadIdx.x / WAVE), lane = (int)(threadIdx.x % WAVE);
+    const int jo = jt + w, io = bx * WAVE + lane;
+    const bool rowok = jo < d.nyo;
     const long long ne = (long long)(d.nz - 1) * d.nx * d.ny;
-    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
     const int i = d.ilo + io, j = d.jlo + jo;
+    const int ci0 = d.ilo + bx * WAVE, cj0 = d.jlo + jt;        // tile cell (0, 0)
     const int kf = k + 1;
-    const bool on = io < d.nxo && in.avert[v2(d, i, j)];
+    const bool on = rowok && io < d.nxo && in.avert[v2(d, i, j)];
     auto cell_on = [&](int ci, int cj) { return cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]; };
     const unsigned cells = on ? ((cell_on(i, j) ? 1u : 0u) | (cell_on(i + 1, j) ? 2u : 0u) |
                                  (cell_on(i, j + 1) ? 4u : 0u) | (cell_on(i + 1, j + 1) ? 8u : 0u))
                               : 0u;
+    const int c0 = w * RC_I + lane;
+    // derivatives of the tile's active cells of layer kk at point p
+    auto stage = [&](int kk, int p) {
+        for (int c = threadIdx.x; c < RC; c += NT) {
+            const int ci = ci0 + c % RC_I, cj = cj0 + c / RC_I;
+            if (!cell_on(ci, cj)) continue;
+            const double* q = E + (long long)p * EQ * ne + elem_id(d, ci, cj, kk);
+            double Ji[9];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) Ji[m] = q[m * ne];
+            sd[24][c] = q[9 * ne];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const double a = c_dxr[n][p], bb = c_dyr[n][p], cc = c_dzr[n][p];
+                sd[3 * n][c] = Ji[0] * a + Ji[1] * bb + Ji[2] * cc;
+                sd[3 * n + 1][c] = Ji[3] * a + Ji[4] * bb + Ji[5] * cc;
+                sd[3 * n + 2][c] = Ji[6] * a + Ji[7] * bb + Ji[8] * cc;
+            }
+        }
+    };
+    const bool have0 = kf - 1 >= 1, have1 = kf <= d.nz - 1;     // uniform over the workgroup
     const bool count_row = nnz && on && !rim_row(d, io, jo);
     unsigned z = 0;
+    const long long t = ((long long)k * d.nyo + (rowok ? jo : 0)) * d.nxp + io;
+    double* a = A + ((t / WAVE) * NCOEF) * WAVE + (t % WAVE);
     double mid[4][9];
 #pragma unroll
     for (int bb = 0; bb < 4; ++bb)
@@ -440,11 +469,21 @@ __global__ __launch_bounds__(WAVE * ROWS_J) __attribute__((amdgpu_waves_per_eu(2
         for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
             for (int h = 0; h < 9; ++h) above[bb][h] = 0.0;
-        if (cells && kf - 1 >= 1) add_layer<2, 3, 1, 0>(E, ne, elem_id(d, i, j, kf - 1), d.nx, cells, above, mid);
+        if (have0) {
+#pragma unroll 1
+            for (int p = 0; p < 8; ++p) {
+                stage(kf - 1, p);
+                __syncthreads();
+                if (cells) add_cells<2, 3, 1, 0>(sd, c0, cells, above, mid);
+                __syncthreads();
+            }
+        }
+        if (rowok) {
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb)
+            for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
-            for (int h = 0; h < 9; ++h) a[(bb * 27 + h) * WAVE] = above[bb][h];
+                for (int h = 0; h < 9; ++h) a[(bb * 27 + h) * WAVE] = above[bb][h];
+        }
         if (count_row) {
 #pragma unroll
             for (int bb = 0; bb < 4; ++bb)
@@ -457,15 +496,25 @@ __global__ __launch_bounds__(WAVE * ROWS_J) __attribute__((amdgpu_waves_per_eu(2
     for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
         for (int h = 0; h < 9; ++h) below[bb][h] = 0.0;
-    // layer kf: the node is on the elements' upper face
-    if (cells && kf <= d.nz - 1) add_layer<6, 7, 5, 4>(E, ne, elem_id(d, i, j, kf), d.nx, cells, mid, below);
-#pragma unroll
-    for (int bb = 0; bb < 4; ++bb)
-#pragma unroll
-        for (int h = 0; h < 9; ++h) {
-            a[(bb * 27 + 9 + h) * WAVE] = mid[bb][h];
-            a[(bb * 27 + 18 + h) * WAVE] = below[bb][h];
+    if (have1) {
+        // layer kf: the node is on the elements' upper face
+#pragma unroll 1
+        for (int p = 0; p < 8; ++p) {
+            stage(kf, p);
+            __syncthreads();
+            if (cells) add_cells<6, 7, 5, 4>(sd, c0, cells, mid, below);
+            __syncthreads();
         }
+    }
+    if (rowok) {
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+            for (int h = 0; h < 9; ++h) {
+                a[(bb * 27 + 9 + h) * WAVE] = mid[bb][h];
+                a[(bb * 27 + 18 + h) * WAVE] = below[bb][h];
+            }
+    }
     if (nnz) {
         if (count_row) {
 #pragma unroll
@@ -475,7 +524,7 @@ __global__ __launch_bounds__(WAVE * ROWS_J) __attribute__((amdgpu_waves_per_eu(2
         }
         unsigned long long zz = z;
         for (int off = WAVE / 2; off > 0; off >>= 1) zz += __shfl_down(zz, off, WAVE);
-        if (threadIdx.x % WAVE == 0 && zz) atomicAdd(nnz, zz);
+        if (lane == 0 && zz) atomicAdd(nnz, zz);
     }
 }
 
