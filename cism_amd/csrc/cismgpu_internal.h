@@ -38,6 +38,7 @@ namespace cg {
 constexpr int WAVE = 64;
 constexpr int NCOEF = 108;          // 4 blocks x 27 neighbours
 constexpr int RED_THREADS = 256;    // block size of the reduction kernels
+constexpr int SCAL_THREADS = 1024;  // block size of the one-workgroup scalar stage (k_scalars)
 
 struct Dims {
     int nx, ny, nz;                 // cells incl. halos; levels

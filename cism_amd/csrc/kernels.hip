@@ -703,15 +703,33 @@ __global__ void k_first_update(Dims d, const double* __restrict__ scal, TV* __re
 // Scalar stage: one block sums the per-block compensated partials in a fixed
 // order and advances the Krylov scalars on the device.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(RED_THREADS) void k_scalars(int stage, const double* __restrict__ pa, int na, int sa,
-                                                         const double* __restrict__ pb, int nb, int sb,
-                                                         double* __restrict__ scal, double tolerance) {
-    dd va = {0.0, 0.0}, vb = {0.0, 0.0};
-    for (int b = threadIdx.x; b < na; b += RED_THREADS) va = dd_add(va, dd{pa[sa * b], pa[sa * b + 1]});
-    if (pb)
-        for (int b = threadIdx.x; b < nb; b += RED_THREADS) vb = dd_add(vb, dd{pb[sb * b], pb[sb * b + 1]});
+// Sum of n compensated partials (stride s) by one thread of a
+// SCAL_THREADS-thread workgroup: four independent loads in flight, then
+// their adds, so the loop waits for memory once per four partials.
+__device__ __forceinline__ dd sum_partials(const double* __restrict__ p, int n, int s) {
+    dd v = {0.0, 0.0};
+    for (int b0 = threadIdx.x; b0 < n; b0 += 4 * SCAL_THREADS) {
+        double h[4], l[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = b0 + u * SCAL_THREADS;
+            h[u] = b < n ? p[(long long)s * b] : 0.0;
+            l[u] = b < n ? p[(long long)s * b + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b0 + u * SCAL_THREADS < n) v = dd_add(v, dd{h[u], l[u]});
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(SCAL_THREADS) void k_scalars(int stage, const double* __restrict__ pa, int na, int sa,
+                                                          const double* __restrict__ pb, int nb, int sb,
+                                                          double* __restrict__ scal, double tolerance) {
+    const dd va = sum_partials(pa, na, sa);
+    const dd vb = pb ? sum_partials(pb, nb, sb) : dd{0.0, 0.0};
     dd v[2] = {va, vb};
-    block_reduce_dd<2, RED_THREADS>(v);
+    block_reduce_dd<2, SCAL_THREADS>(v);
     if (threadIdx.x != 0) return;
     const double A = v[0].hi + v[0].lo, B = v[1].hi + v[1].lo;
     if (scal[S_STOP] != 0.0 && stage != ST_BB && stage != ST_RR) return;

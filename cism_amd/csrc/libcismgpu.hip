@@ -79,6 +79,7 @@ struct cismgpu_system {
     bool raw_ready = false;      // stage holds the raw Auu/Avv of the current matrix
     bool saved_raw_ready = false;
     bool sym_exact = false;      // the assembled FP64 matrix is known to be exactly symmetric
+    bool edge_zero = false;      // no-ice rule: the rows of the last owned vertex row/column are zero
     double* work = nullptr;      // Picard work vectors: slot q = (u, v) at work + 2*q*vlen
     int nwork = 0;
     double* saved = nullptr;     // saved assembly (A, b, preconditioner, raw diagonal blocks)
@@ -238,7 +239,7 @@ void launch_precond(cismgpu_system* S, int kind, const TV* ru, const TV* rv, TV*
 
 void scalars(cismgpu_system* S, int stage, const double* pa, int na, int sa, const double* pb, int nb, int sb,
              double tol = 0.0) {
-    k_scalars<<<1, RED_THREADS, 0, S->s>>>(stage, pa, na, sa, pb, nb, sb, S->scal, tol);
+    k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(stage, pa, na, sa, pb, nb, sb, S->scal, tol);
 }
 
 // Exact (reproducible) sum over owned vertices of the summands a1*b1 + a2*b2
@@ -526,10 +527,14 @@ int cg_solve(cismgpu_system* S, const TA* A, TV* const* v, const TV* bu, const T
         } else {
             scalars(S, ST_CG_ITER, S->partA, ncb, 4, S->partB, nsp, 2);
         }
-        // no-ice rule with every zeroed halo column outside the owned range:
-        // the product writes owned rows only, so A z's halo stays as zeroed
-        // at the start and the halo pass is skipped
-        if (!(d.bc == 3 && d.ilo >= 3 && d.ihi <= d.nvx - 2 && d.jlo >= 3 && d.jhi <= d.nvy - 2))
+        // The no-ice halo rule zeroes vertex columns 1, 2, nvx-1 and nvx
+        // (rows likewise).  The product writes owned rows only, so columns
+        // outside the owned range keep the zero they got at the start, and
+        // an owned last column holds zero rows after the assembly's edge
+        // pass, where A z is +0 already: then the halo pass of A z changes
+        // nothing and is skipped.
+        if (!(d.bc == 3 && d.ilo >= 3 && d.jlo >= 3 &&
+              (S->edge_zero || (d.ihi <= d.nvx - 2 && d.jhi <= d.nvy - 2))))
             halo<TV>(S, Azu, Azv, d.nz);
         const bool check = (iter % ncheck == 0) || iter == 5;
         const bool fuse = fuse_ok && precond == 2 && !(check && !given_r);
@@ -752,9 +757,9 @@ int mixed_solve(cismgpu_system* S, const cismgpu_solve_opts* o, cismgpu_solve_in
         k_ring<double><<<col_blocks(d), 256, 0, S->s>>>(d, ru, rv, bu, bv);
         if (S->repro) {
             rs_dot<double>(S, 3, ru, ru, rv, rv);
-            k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+            k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
         } else {
-            k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, nsp, 2, nullptr, 0, 0, S->scal, 0.0);
+            k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_DOT, S->partB, nsp, 2, nullptr, 0, 0, S->scal, 0.0);
         }
         st = sync_scal(S);
         if (st) return st;
@@ -938,6 +943,7 @@ static int set_matrix_impl(cismgpu_system* S, const int32_t* active_vertex, int 
     S->have_matrix = true;
     S->raw_ready = true;
     S->sym_exact = false;
+    S->edge_zero = false;
     S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
     S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
@@ -1537,6 +1543,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     if (in->umask_dirichlet || in->vmask_dirichlet)
         k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, S->A64);
+    S->edge_zero = d.bc == 3;
     halo<double>(S, S->V(2), S->V(3), d.nz);
     // symmetry fix: count the differing pairs first; the copy-and-average
     // pass runs only when one exists
@@ -1556,6 +1563,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
             if (!S->Atmp) CG_CHECK(hipMalloc(&S->Atmp, d.nrows * NCOEF * sizeof(double)));
             k_symmetry<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a.avert, S->A64, S->Atmp, cnt + 1);
             std::swap(S->A64, S->Atmp);
+            S->edge_zero = false;   // averaging may fill entries of the zeroed rows
             // the averaged matrix is counted again
             CG_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), S->s));
             k_count_nnz<<<nblocks(d.nrows, 256), 256, 0, S->s>>>(d, a.avert, S->A64, cnt);
@@ -1691,6 +1699,7 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
     k_sum_levels<<<nblocks(n2, 256), 256, 0, S->s>>>(d3, lu, lv, d, S->V(2), S->V(3));
     k_dirichlet_2d<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, S->A64);
+    S->edge_zero = d.bc == 3;
     halo<double>(S, S->V(2), S->V(3), 1);
     unsigned long long* cnt = nullptr;
     CG_CHECK(hipMalloc(&cnt, 2 * sizeof(unsigned long long)));
@@ -1931,6 +1940,7 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
     S->have_matrix = true;
     S->raw_ready = S->saved_raw_ready;
     S->sym_exact = false;
+    S->edge_zero = false;
     S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
     S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
@@ -1956,13 +1966,13 @@ extern "C" int32_t cismgpu_system_residual(cismgpu_system* S, double* ru, double
     k_spmv<double, double, 2><<<spmv_blocks(d), 256, 0, S->s>>>(d, S->A64, S->V(0), S->V(1), r0, r1, S->V(2), S->V(3), S->partB);
     if (S->repro) {
         rs_dot<double>(S, 3, r0, r0, r1, r1);
-        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+        k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_DOT, S->rs_out + 6, 1, 2, nullptr, 0, 0, S->scal, 0.0);
         rs_dot<double>(S, 0, S->V(2), S->V(2), S->V(3), S->V(3));
-        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->rs_out, 1, 2, nullptr, 0, 0, S->scal, 0.0);
+        k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_BB, S->rs_out, 1, 2, nullptr, 0, 0, S->scal, 0.0);
     } else {
-        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_DOT, S->partB, spmv_blocks(d), 2, nullptr, 0, 0, S->scal, 0.0);
+        k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_DOT, S->partB, spmv_blocks(d), 2, nullptr, 0, 0, S->scal, 0.0);
         k_dot<double><<<dot_blocks<double>(d), 256, 0, S->s>>>(d, S->V(2), S->V(2), S->V(3), S->V(3), S->partA);
-        k_scalars<<<1, RED_THREADS, 0, S->s>>>(ST_BB, S->partA, dot_blocks<double>(d), 2, nullptr, 0, 0, S->scal, 0.0);
+        k_scalars<<<1, SCAL_THREADS, 0, S->s>>>(ST_BB, S->partA, dot_blocks<double>(d), 2, nullptr, 0, 0, S->scal, 0.0);
     }
     int st = sync_scal(S);
     if (st) return st;
