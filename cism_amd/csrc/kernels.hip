@@ -339,11 +339,15 @@ __global__ void k_halo_zero(Dims d, TV* __restrict__ a, TV* __restrict__ b, int 
 // SIA: easy_sia_solver -- Thomas solve of the vertical tridiagonal part of
 // Auu/Avv on active columns; inactive columns are left untouched.
 // M holds 6 fields (sub, diag, sup for u; then for v) in vector layout.
+// With sym set the matrix is known to be exactly symmetric, so the sub
+// diagonal of level k equals the super diagonal of level k-1, which the
+// elimination has just read: the sub-diagonal field is not read at all
+// (the same values, 4 or 8 bytes per node less).
 // ---------------------------------------------------------------------------
 template <typename TV, int NZMAX>
 __device__ __forceinline__ void thomas_column(const Dims& d, const TV* __restrict__ Ms, const TV* __restrict__ Md,
                                               const TV* __restrict__ Mp, const TV (&rhs)[NZMAX], TV (&z)[NZMAX],
-                                              long long c0) {
+                                              long long c0, bool sym) {
     TV gam[NZMAX];
     TV beta = Md[c0];
     z[0] = rhs[0] / beta;
@@ -351,8 +355,9 @@ __device__ __forceinline__ void thomas_column(const Dims& d, const TV* __restric
     for (int k = 1; k < NZMAX; ++k) {
         if (k < d.nz) {
             const long long c = c0 + (long long)k * d.plane;
-            gam[k] = Mp[c - d.plane] / beta;
-            const TV sb = Ms[c];
+            const TV sp = Mp[c - d.plane];
+            gam[k] = sp / beta;
+            const TV sb = sym ? sp : Ms[c];
             beta = Md[c] - sb * gam[k];
             z[k] = (rhs[k] - sb * z[k - 1]) / beta;
         }
@@ -367,7 +372,7 @@ __device__ __forceinline__ void thomas_column(const Dims& d, const TV* __restric
 // residual (the next iteration's first step, :3301-3381) and the partial
 // (r, z) over owned columns.  With PRE = 0 only the update is done.
 // scal[S_ALPHA], scal[S_BETA] are read on the device.
-template <typename TV, int NZMAX, int PRE, int STD>
+template <typename TV, int NZMAX, int PRE, int STD, int SYM>
 __global__ __launch_bounds__(256) void k_update(Dims d, const double* __restrict__ scal,
                                                 TV* __restrict__ xu, TV* __restrict__ xv,
                                                 TV* __restrict__ ru, TV* __restrict__ rv,
@@ -410,8 +415,8 @@ __global__ __launch_bounds__(256) void k_update(Dims d, const double* __restrict
             if (active[t]) {
                 TV zU[NZMAX], zV[NZMAX];
                 const long long vl = d.vlen;
-                thomas_column<TV, NZMAX>(d, M + 0 * vl, M + 1 * vl, M + 2 * vl, rU, zU, c0);
-                thomas_column<TV, NZMAX>(d, M + 3 * vl, M + 4 * vl, M + 5 * vl, rV, zV, c0);
+                thomas_column<TV, NZMAX>(d, M + 0 * vl, M + 1 * vl, M + 2 * vl, rU, zU, c0, SYM != 0);
+                thomas_column<TV, NZMAX>(d, M + 3 * vl, M + 4 * vl, M + 5 * vl, rV, zV, c0, SYM != 0);
 #pragma unroll
                 for (int k = 0; k < NZMAX; ++k)
                     if (k < d.nz) {
@@ -461,7 +466,7 @@ __global__ __launch_bounds__(256) void k_precond(Dims d, int kind, const double*
                                                  const TV* __restrict__ M, const TV* __restrict__ Dg,
                                                  const int8_t* __restrict__ active,
                                                  const TV* __restrict__ zvs_u, const TV* __restrict__ zvs_v,
-                                                 double* __restrict__ partials) {
+                                                 double* __restrict__ partials, int msym) {
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long ncol = (long long)d.nvx * d.nvy;
     dd acc = {0.0, 0.0}, acc2 = {0.0, 0.0};
@@ -497,8 +502,8 @@ __global__ __launch_bounds__(256) void k_precond(Dims d, int kind, const double*
         } else {
             if (active[t]) {
                 const long long vl = d.vlen;
-                thomas_column<TV, NZMAX>(d, M + 0 * vl, M + 1 * vl, M + 2 * vl, rU, zU, c0);
-                thomas_column<TV, NZMAX>(d, M + 3 * vl, M + 4 * vl, M + 5 * vl, rV, zV, c0);
+                thomas_column<TV, NZMAX>(d, M + 0 * vl, M + 1 * vl, M + 2 * vl, rU, zU, c0, msym != 0);
+                thomas_column<TV, NZMAX>(d, M + 3 * vl, M + 4 * vl, M + 5 * vl, rV, zV, c0, msym != 0);
             } else {
 #pragma unroll
                 for (int k = 0; k < NZMAX; ++k)

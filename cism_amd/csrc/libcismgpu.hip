@@ -215,14 +215,31 @@ void spmv(cismgpu_system* S, const TA* A, const TV* xu, const TV* xv, TV* yu, TV
     if (t) { hipEventRecord(S->ev[S->ev_used + 1], S->s); S->ev_used += 2; }
 }
 
+// The preconditioner fields of an exactly symmetric assembled matrix: the
+// SIA solve takes each sub diagonal from the super diagonal of the level
+// below.  Only after an assembly that found the rows exactly symmetric:
+// there the fields of the halo columns come from the owned ones by the halo
+// rule (or are zero), so they are symmetric too; a matrix given by the
+// caller may carry other halo values.
+template <typename TV>
+int precond_sym(const cismgpu_system* S) {
+    return S->sym_exact ? 1 : 0;
+}
+
 template <typename TV, int PRE, int STD>
 void launch_update(cismgpu_system* S, TV* const* v, const TV* M) {
     // v: xu xv ru rv du dv qu qv zu zv Azu Azv
     const Dims& d = S->d;
-#define UPD(NZM) k_update<TV, NZM, PRE, STD><<<col_blocks(d), 256, 0, S->s>>>(d, S->scal, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11], M, S->active, S->partA)
-    if (d.nz <= 16) UPD(16);
-    else if (d.nz <= 32) UPD(32);
-    else UPD(64);
+#define UPD(NZM, SY) k_update<TV, NZM, PRE, STD, SY><<<col_blocks(d), 256, 0, S->s>>>(d, S->scal, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11], M, S->active, S->partA)
+    if (precond_sym<TV>(S)) {
+        if (d.nz <= 16) UPD(16, 1);
+        else if (d.nz <= 32) UPD(32, 1);
+        else UPD(64, 1);
+    } else {
+        if (d.nz <= 16) UPD(16, 0);
+        else if (d.nz <= 32) UPD(32, 0);
+        else UPD(64, 0);
+    }
 #undef UPD
 }
 
@@ -230,7 +247,8 @@ template <typename TV>
 void launch_precond(cismgpu_system* S, int kind, const TV* ru, const TV* rv, TV* zu, TV* zv, const TV* M,
                     const TV* Dg, const TV* zsu, const TV* zsv) {
     const Dims& d = S->d;
-#define PRC(NZM) k_precond<TV, NZM><<<col_blocks(d), 256, 0, S->s>>>(d, kind, S->scal, ru, rv, zu, zv, M, Dg, S->active, zsu, zsv, S->partA)
+    const int msym = precond_sym<TV>(S);
+#define PRC(NZM) k_precond<TV, NZM><<<col_blocks(d), 256, 0, S->s>>>(d, kind, S->scal, ru, rv, zu, zv, M, Dg, S->active, zsu, zsv, S->partA, msym)
     if (d.nz <= 16) PRC(16);
     else if (d.nz <= 32) PRC(32);
     else PRC(64);
