@@ -331,11 +331,8 @@ __device__ __forceinline__ void column_component(const Dims& d, const EnsPtrs& Q
                     bet = Md[cl];
                     zt = rk / bet;
                 } else {
-                    // half storage: every member is exactly symmetric, so the
-                    // sub diagonal is the super diagonal of the level below
-                    const float sp = Mp[cl - pl];
-                    gam[k] = sp / bet;
-                    const float sb = Q.half ? sp : Ms[cl];
+                    gam[k] = Mp[cl - pl] / bet;
+                    const float sb = Ms[cl];
                     bet = Md[cl] - sb * gam[k];
                     zt = (rk - sb * zprev) / bet;
                 }
