@@ -586,7 +586,8 @@ def run_ensemble(args, sample=None):
         nodes = sum(p.grid.n_owned_nodes for p in probs)
         split, tune_ms = E.autotune()
         log(f"ensemble autotune: split per bucket {split}")
-        opts = dict(tolerance=1e-10, inner_tol=1e-3, inner_maxiters=400, max_outer=30)
+        # inner tolerance 1e-4: the fewest lock-step steps of the sweep 1e-3 .. 1e-6
+        opts = dict(tolerance=1e-10, inner_tol=1e-4, inner_maxiters=600, max_outer=30)
         E.solve(**opts)                                   # warm-up (graph capture)
         runs = [E.solve(**opts) for _ in range(3)]
         info = runs[-1]
@@ -625,7 +626,7 @@ def run_ensemble(args, sample=None):
             for m, sv in enumerate(E.solvers):
                 sv.sys.zero_x()
                 r = sv.sys.solve(precond=2, tolerance=1e-10, maxiters=20000, precision=pcg.MIXED_FP32,
-                                 inner_tol=1e-3, max_outer=30)
+                                 inner_tol=1e-4, max_outer=30)
                 seq_ms += r["solve_ms"]
                 sv.sys.zero_x()
                 sv.sys.solve(precond=2, tolerance=1e-12, maxiters=20000)
