@@ -20,7 +20,8 @@ _lib = None
 
 STATUS = {
     0: "ok", 1: "NaN in PCG scalars", 2: "zero pivot in tridiagonal preconditioner", 3: "bad argument",
-    4: "singular element Jacobian", 5: "asymmetric element matrix", 10: "HIP runtime failure",
+    4: "singular element Jacobian", 5: "asymmetric element matrix", 6: "negative or NaN basal traction",
+    10: "HIP runtime failure",
     11: "unsupported configuration", 12: "iterative refinement did not converge",
 }
 
@@ -40,6 +41,7 @@ EXPORTS = (
     "cismgpu_system_create_2d", "cismgpu_system_set_matrix_2d", "cismgpu_system_assemble_2d", "cismgpu_system_diva_setup",
     "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity", "cismgpu_half_storage_supported",
     "cismgpu_residual_vector_3d_noalloc", "cismgpu_reprosum_calc",
+    "cismgpu_system_calcbeta_setup", "cismgpu_system_calcbeta",
 )
 
 

@@ -21,6 +21,7 @@
 #include "ensemble.hip"
 #include "noalloc.hip"
 #include "reprosum.hip"
+#include "calcbeta.hip"
 
 using namespace cg;
 
@@ -108,6 +109,16 @@ struct cismgpu_system {
     unsigned long long* rs_part = nullptr;   // per-workgroup level sums of one exact sum
     int* rs_bad = nullptr;       // a summand was not finite
     double* rs_out = nullptr;    // exact sums as (value, 0) pairs, 4 slots
+
+    // basal traction (glissade_calcbeta): uploaded basal physics, beta and
+    // its halo partner, a host-velocity staging pair, the invalid-value slot
+    void* cb_buf = nullptr;
+    CBDev cb{};
+    double* cb_beta = nullptr;   // (nvx, nvy), then a scratch field of the same size
+    double* cb_vel = nullptr;    // 2 (nvx, nvy) fields
+    unsigned long long* cb_bad = nullptr;
+    bool cb_setup = false;       // calcbeta_setup done
+    bool cb_valid = false;       // cb_beta holds a calcbeta result
 
     double* V(int q) { return v64 + (size_t)q * d.vlen; }
     float* W(int q) { return v32 + (size_t)q * d.vlen; }
@@ -907,7 +918,7 @@ int32_t cismgpu_system_destroy(cismgpu_system* S) {
     for (void* p : {(void*)S->active, (void*)S->A64, (void*)S->A32, (void*)S->A16, (void*)S->stage, (void*)S->M64,
                     (void*)S->M32, (void*)S->v64, (void*)S->v32, (void*)S->partA, (void*)S->partB, (void*)S->scal,
                     (void*)S->tri.a, (void*)S->tri.a32, S->asm_buf, (void*)S->efvs, (void*)S->Atmp, (void*)S->saved, (void*)S->work, (void*)S->load3, (void*)S->A64h, (void*)S->diva_buf, (void*)S->efva2, (void*)S->rs_part, (void*)S->rs_bad,
-                    (void*)S->rs_out, (void*)S->elemq})
+                    (void*)S->rs_out, (void*)S->elemq, S->cb_buf})
         if (p) hipFree(p);
     if (S->scal_h) hipHostFree(S->scal_h);
     for (auto& x : S->ev) hipEventDestroy(x);
@@ -1403,6 +1414,9 @@ static int upload_fe_tables() {
     return CISMGPU_OK;
 }
 
+// the assembly has a basal friction term (host field or the device calcbeta result)
+static inline bool has_beta(const cismgpu_assembly_in* in) { return in->beta_internal || in->beta_device; }
+
 // Inputs of an assembly into the system's device block.  d describes the
 // input fields: the system's grid for a 3-D system, the grid with the
 // input's sigma levels for a 2-D one.
@@ -1412,6 +1426,10 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     const long long nvec = (long long)d.nz * n2, nfl = (long long)(d.nz - 1) * nc;
     // device input block: doubles first, then int32s
     lateral = in->ocean_mask != nullptr;
+    if (in->beta_device && !S->cb_valid) {
+        cismgpu_set_error("assembly: beta_device needs a cismgpu_system_calcbeta result on this system");
+        return CISMGPU_ERR_ARG;
+    }
     if (lateral && (!in->usrf || !in->thck ||
                     (in->which_assemble_lateral == 0 && (!in->stagusrf_lateral || !in->stagthck_lateral)))) {
         cismgpu_set_error("assembly: lateral BC needs usrf, thck and (standard option) the staggered fields");
@@ -1448,7 +1466,8 @@ static int upload_assembly_inputs(cismgpu_system* S, const Dims& d, const cismgp
     a.sthck = putd(in->stagthck, n2, true);
     a.dsdx = putd(in->dusrf_dx, n2, true);
     a.dsdy = putd(in->dusrf_dy, n2, true);
-    a.beta = putd(in->beta_internal, n2, true);
+    a.beta = putd(in->beta_device ? nullptr : in->beta_internal, n2, !in->beta_device);
+    if (in->beta_device) a.beta = S->cb_beta;
     a.theta = putd(in->theta_basal, n2, true);
     a.flwafact = putd(in->flwafact, nfl, true);
     a.thck_cell = in->thck ? putd(in->thck, nc, true) : nullptr;
@@ -1498,7 +1517,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         return CISMGPU_ERR_ARG;
     }
     if (S->two_d) { cismgpu_set_error("assemble_3d needs a 3-D system"); return CISMGPU_ERR_ARG; }
-    if (in->assemble_beta_standard && in->beta_internal && !in->stag_lsrf) {
+    if (in->assemble_beta_standard && has_beta(in) && !in->stag_lsrf) {
         cismgpu_set_error("assemble_3d: the standard basal option needs stag_lsrf");
         return CISMGPU_ERR_ARG;
     }
@@ -1534,7 +1553,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     // every row checked and counted).
     const char* senv = getenv("CISMGPU_SYMCHECK");
     const bool rim_only = !gather && !(senv && !strcmp(senv, "full")) && !in->umask_dirichlet &&
-                          !in->vmask_dirichlet && !(in->beta_internal && a.beta_std);
+                          !in->vmask_dirichlet && !(has_beta(in) && a.beta_std);
     hipEventRecord(e0, S->s);
     if (gather) {
         k_assemble<<<nblocks(d.nrows, 64), 64, 0, S->s>>>(d, a, uv, vv, S->A64, S->scal + S_NAN);
@@ -1545,7 +1564,7 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
         k_rows<<<(unsigned)rows_blocks(d), WAVE * ROWS_J, 0, S->s>>>(d, a, S->elemq, S->A64, rim_only ? cnt : nullptr);
         hipEventRecord(e1, S->s);
     }
-    if (in->beta_internal) {
+    if (has_beta(in)) {
         if (a.beta_std)
             k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, d.nz - 1, 1, a.beta, a.beta, S->A64,
                                                                                S->scal + S_NAN);
@@ -1698,7 +1717,7 @@ extern "C" int32_t cismgpu_system_assemble_2d(cismgpu_system* S, const cismgpu_a
             k_basal_diva_2d<<<nblocks((long long)d.nxo * d.nyo, 256), 256, 0, S->s>>>(d, a, S->dv, S->A64);
     } else {
         if (!l1l2) k_efvs_2d<<<nblocks(nc, 128), 128, 0, S->s>>>(d, a, L, uv, vv, S->efvs);
-        if (in->beta_internal) {
+        if (has_beta(in)) {
             if (a.beta_std)
                 k_basal_std<<<nblocks((long long)d.nxo * d.nyo, 128), 128, 0, S->s>>>(d, a, 0, 0, a.beta, a.beta, S->A64,
                                                                                    S->scal + S_NAN);
@@ -1962,6 +1981,146 @@ extern "C" int32_t cismgpu_system_restore_assembly(cismgpu_system* S) {
     S->m32_ready = S->a32_ready = S->a16_ready = S->a64h_ready = false;
     S->a32_sym = S->a64_sym = false;
     S->tri.mode = 0;
+    return CISMGPU_OK;
+}
+
+// glissade_calcbeta: the fields each sliding law reads
+// (glissade_basal_traction.F90:200-644)
+static const char* calcbeta_missing(const cismgpu_calcbeta_in* in) {
+    const int b = in->whichbabc;
+    if (!in->f_ground) return "f_ground";
+    if (b == 1 && !in->bpmp_mask) return "bpmp_mask";
+    if ((b == 2 || b == 3 || b == 7 || (b >= 10 && b <= 14)) && !in->effecpress_stag) return "effecpress_stag";
+    if ((b == 2 || b == 7 || (b >= 10 && b <= 13)) && !in->coulomb_c) return "coulomb_c";
+    if ((b == 9 || (b >= 11 && b <= 13)) && !in->powerlaw_c) return "powerlaw_c";
+    if (b == 3 && !in->topg) return "topg";
+    if (b == 5 && !in->beta_external) return "beta_external";
+    if (b == 10 && (!in->thck || !in->flwa_basal)) return "thck and flwa_basal";
+    if (b == 15 && !in->mintauf) return "mintauf";
+    return nullptr;
+}
+
+extern "C" int32_t cismgpu_system_calcbeta_setup(cismgpu_system* S, const cismgpu_calcbeta_in* in) {
+    if (!S || !in) { cismgpu_set_error("calcbeta_setup: null argument"); return CISMGPU_ERR_ARG; }
+    const int b = in->whichbabc;
+    if (b < 0 || b > 15 || b == 6 || b == 8) {
+        cismgpu_set_error("calcbeta_setup: which_ho_babc %d is not evaluated on the device (6 no-slip is a "
+                          "Dirichlet condition, 8 ISMIP-HOM C a prescribed field)", b);
+        return CISMGPU_ERR_ARG;
+    }
+    if (in->which_beta_limit != 0 && in->which_beta_limit != 1) {
+        cismgpu_set_error("calcbeta_setup: bad which_ho_beta_limit %d", in->which_beta_limit);
+        return CISMGPU_ERR_ARG;
+    }
+    if (const char* miss = calcbeta_missing(in)) {
+        cismgpu_set_error("calcbeta_setup: which_ho_babc %d needs %s", b, miss);
+        return CISMGPU_ERR_ARG;
+    }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
+    if (b == 5) {   // beta could not be read (:405-411)
+        double mx = -INFINITY;
+        for (long long t = 0; t < n2; ++t) mx = std::max(mx, in->beta_external[t]);
+        if (mx < 0.0) {
+            cismgpu_set_error("calcbeta_setup: every external beta value is < 0 (beta not read from the input)");
+            return CISMGPU_ERR_BETA;
+        }
+    }
+    if (!S->cb_buf) {
+        // doubles: beta + scratch, velocity pair, 7 vertex fields, 2 cell fields; then ints, the bad slot
+        const size_t bytes = (size_t)(11 * n2 + 2 * nc) * sizeof(double) + (size_t)(n2 + nc) * sizeof(int32_t) + 64;
+        CG_CHECK(hipMalloc(&S->cb_buf, bytes));
+    }
+    double* p = (double*)S->cb_buf;
+    S->cb_beta = p; p += 2 * n2;
+    S->cb_vel = p; p += 2 * n2;
+    CBDev c{};
+    c.babc = b;
+    c.limit = in->which_beta_limit;
+    const double prm[CB_NPARAM] = {in->ho_beta_const, in->ho_beta_small, in->ho_beta_large, in->pseudo_plastic_q,
+                                   in->pseudo_plastic_u0, in->pseudo_plastic_phimin, in->pseudo_plastic_phimax,
+                                   in->pseudo_plastic_bedmin, in->pseudo_plastic_bedmax, in->eus, in->powerlaw_m,
+                                   in->zoet_iverson_ut, in->zoet_iverson_nmax, in->coulomb_bump_max_slope,
+                                   in->coulomb_bump_wavelength, in->friction_powerlaw_k, in->beta_grounded_min,
+                                   in->beta_powerlaw_umax};
+    for (int q = 0; q < CB_NPARAM; ++q) c.p[q] = prm[q];
+    hipError_t herr = hipSuccess;
+    auto put = [&](const double* h, long long n) -> const double* {
+        double* dst = p;
+        p += n;
+        if (!h) return nullptr;
+        const hipError_t e = hipMemcpyAsync(dst, h, n * sizeof(double), hipMemcpyHostToDevice, S->s);
+        if (e != hipSuccess && herr == hipSuccess) herr = e;
+        return dst;
+    };
+    c.fg = put(in->f_ground, n2);
+    c.N = put(in->effecpress_stag, n2);
+    c.cc = put(in->coulomb_c, n2);
+    c.pc = put(in->powerlaw_c, n2);
+    c.mintauf = put(in->mintauf, n2);
+    c.csf = put(in->c_space_factor_stag, n2);
+    c.bext = put(in->beta_external, n2);
+    c.topg = put(in->topg, nc);
+    c.flwa = put(in->flwa_basal, nc);
+    int32_t* ip = (int32_t*)p;
+    if (in->bpmp_mask) {
+        const hipError_t e = hipMemcpyAsync(ip, in->bpmp_mask, n2 * sizeof(int32_t), hipMemcpyHostToDevice, S->s);
+        if (e != hipSuccess && herr == hipSuccess) herr = e;
+        c.bpmp = ip;
+    }
+    ip += n2;
+    std::vector<int32_t> imask;
+    if (in->thck) {   // thck > 0 (:479-483)
+        imask.resize(nc);
+        for (long long t = 0; t < nc; ++t) imask[t] = in->thck[t] > 0.0 ? 1 : 0;
+        const hipError_t e = hipMemcpyAsync(ip, imask.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, S->s);
+        if (e != hipSuccess && herr == hipSuccess) herr = e;
+        c.imask = ip;
+    }
+    ip += nc;
+    S->cb_bad = (unsigned long long*)(((uintptr_t)ip + 7) & ~(uintptr_t)7);
+    CG_CHECK(herr);
+    CG_CHECK(hipStreamSynchronize(S->s));   // the host mask buffer goes out of scope
+    S->cb = c;
+    S->cb_setup = true;
+    S->cb_valid = false;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_calcbeta(cismgpu_system* S, const double* ubas, const double* vbas, double* beta) {
+    if (!S || !S->cb_setup) { cismgpu_set_error("calcbeta: no calcbeta_setup on this system"); return CISMGPU_ERR_ARG; }
+    if ((ubas == nullptr) != (vbas == nullptr)) {
+        cismgpu_set_error("calcbeta: pass both basal velocity components or neither");
+        return CISMGPU_ERR_ARG;
+    }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    const double *u = S->V(0), *v = S->V(1);
+    int lev = d.nz - 1;   // basal level of x (the only level of a 2-D system)
+    if (ubas) {
+        CG_CHECK(hipMemcpyAsync(S->cb_vel, ubas, n2 * sizeof(double), hipMemcpyHostToDevice, S->s));
+        CG_CHECK(hipMemcpyAsync(S->cb_vel + n2, vbas, n2 * sizeof(double), hipMemcpyHostToDevice, S->s));
+        u = S->cb_vel;
+        v = S->cb_vel + n2;
+        lev = -1;
+    }
+    S->cb_valid = false;
+    CG_CHECK(hipMemsetAsync(S->cb_bad, 0xff, sizeof(unsigned long long), S->s));
+    k_calcbeta<<<nblocks(n2, 256), 256, 0, S->s>>>(d, S->cb, u, v, lev, S->cb_beta, S->cb_bad);
+    CG_CHECK(hipGetLastError());
+    unsigned long long bad = 0;
+    CG_CHECK(hipMemcpyAsync(&bad, S->cb_bad, sizeof(bad), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    if (bad != ~0ULL) {
+        double bv = 0.0;
+        CG_CHECK(hipMemcpy(&bv, S->cb_beta + bad, sizeof(double), hipMemcpyDeviceToHost));
+        cismgpu_set_error("calcbeta: invalid beta %g at vertex i=%lld j=%lld", bv, bad % d.nvx + 1, bad / d.nvx + 1);
+        return CISMGPU_ERR_BETA;
+    }
+    halo2d<double>(S, S->cb_beta, S->cb_beta + n2);   // staggered halo (:703)
+    if (beta) CG_CHECK(hipMemcpyAsync(beta, S->cb_beta, n2 * sizeof(double), hipMemcpyDeviceToHost, S->s));
+    CG_CHECK(hipStreamSynchronize(S->s));
+    S->cb_valid = true;
     return CISMGPU_OK;
 }
 

@@ -50,6 +50,8 @@ class Problem:
     assemble_lateral: int = 0    # which_ho_assemble_lateral (glide_types.F90:1067): 0 standard, 1 local
     assemble_beta: int = 1       # which_ho_assemble_beta (glide_types.F90:359-360, 1050): 0 standard, 1 local
     assemble_taud: int = 1       # which_ho_assemble_taud (glide_types.F90:362-363, 1042): 0 standard, 1 local
+    friction: object = None      # basal.Friction: beta from the basal velocity in every Picard iteration
+                                 # (glissade_calcbeta); None: the fixed beta of whichbabc 0, 4, 5 or no-slip
     meta: dict = field(default_factory=dict)
 
     def cell_centres(self):

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib, geometry as ge
+from . import _lib, basal, geometry as ge
 from .grid import stagwbndsigma
 from .pcg import ALGO_CHRONGEAR, FP64, HO_PRECOND_SIA, System
 
@@ -34,7 +34,8 @@ class AssemblyIn(ctypes.Structure):
                 ("check_symmetry", ctypes.c_int32),
                 ("ocean_mask", _P), ("usrf", _P), ("thck", _P), ("stagusrf_lateral", _P), ("stagthck_lateral", _P),
                 ("which_assemble_lateral", ctypes.c_int32), ("flwa", _P), ("stag_lsrf", _P),
-                ("assemble_beta_standard", ctypes.c_int32), ("assemble_taud_standard", ctypes.c_int32)]
+                ("assemble_beta_standard", ctypes.c_int32), ("assemble_taud_standard", ctypes.c_int32),
+                ("beta_device", ctypes.c_int32)]
 
 
 class DivaIn(ctypes.Structure):
@@ -137,10 +138,21 @@ class BPSolver:
         self.flwafact = ge.flwafact(np.asfortranarray(p.flwa), p.n_glen)
         self.theta_basal = ge.slope_angle(p.dx, p.dy, ge.stagger(self.lsrf))
         self.beta_internal = ge.beta_internal(p, self.ice_mask, self.floating_mask, self.land_mask)
+        # velocity-dependent basal traction: glissade_calcbeta on the device before
+        # every assembly (glissade_velo_higher.F90:2316-2332)
+        self.friction = getattr(p, "friction", None)
+        if self.friction is not None:
+            if p.whichbabc == 6:
+                raise ValueError("friction: a no-slip problem (which_ho_babc 6) has no basal friction term")
+            if self.friction.whichbabc not in basal.DEVICE_OPTIONS:
+                raise ValueError(f"friction: which_ho_babc {self.friction.whichbabc} is not a device sliding law")
+            fground = ge.grounded_fraction(self.ice_mask, self.floating_mask, self.land_mask)
+            basal.device_setup(self.sys, self.friction, f_ground=fground, topg=p.topg, thck=p.thck, eus=p.eus,
+                               flwa_basal=np.asarray(p.flwa)[g.nz - 2])
         self.xV, self.yV = p.vertex_coords()
         self.sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
         self.swbs = np.ascontiguousarray(stagwbndsigma(p.sigma))
-        no_slip = p.whichbabc == 6
+        no_slip = p.whichbabc == 6 and self.friction is None
         self.umask = np.zeros(g.vec_shape, np.int32, order="F")
         if no_slip:
             self.umask[g.nz - 1] = 1
@@ -168,6 +180,9 @@ class BPSolver:
         self._ain.stag_lsrf = self.stag_lsrf.ctypes.data
         self._ain.assemble_beta_standard = 1 if getattr(p, "assemble_beta", 1) == 0 else 0
         self._ain.assemble_taud_standard = 1 if getattr(p, "assemble_taud", 1) == 0 else 0
+        if self.friction is not None:
+            self._ain.beta_internal = None
+            self._ain.beta_device = 1
 
     def _initial_velocity(self):
         u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
@@ -194,6 +209,14 @@ class BPSolver:
             a[own] = o
 
     # ---------------------------------------------------------- device steps
+    def calcbeta(self, want: bool = False):
+        """glissade_calcbeta from the basal velocity of the resident x (a
+        problem with a friction law; the assemblies call it); ``want``:
+        return beta (nx-1, ny-1)."""
+        if self.friction is None:
+            raise ValueError("calcbeta: the problem has no friction law")
+        return basal.device_calcbeta(self.sys, want=want)
+
     def assemble(self, uvel=None, vvel=None):
         """Assemble A, b for the viscosity of (uvel, vvel); None keeps the resident x."""
         if uvel is not None:
@@ -201,6 +224,8 @@ class BPSolver:
             v = np.asfortranarray(np.array(vvel, dtype=np.float64))
             self.zero_inactive(u, v)
             self.sys.set_x(u, v)
+        if self.friction is not None:
+            basal.device_calcbeta(self.sys)
         info = AssemblyInfo()
         _lib.check(self._L.cismgpu_system_assemble_3d(self.sys.handle, ctypes.byref(self._ain), ctypes.byref(info)),
                    "assemble_3d")
@@ -378,6 +403,8 @@ class SSASolver(BPSolver):
             v = np.asfortranarray(np.array(vvel, dtype=np.float64))
             self.zero_inactive(u, v)
             self.sys.set_x(u, v)
+        if self.friction is not None:
+            basal.device_calcbeta(self.sys)
         info = AssemblyInfo()
         _lib.check(self._L.cismgpu_system_assemble_2d(self.sys.handle, ctypes.byref(self._ain), self.APPROX,
                                                       ctypes.byref(info)), "assemble_2d")
@@ -444,6 +471,9 @@ class DIVASolver(SSASolver):
     APPROX = 4
 
     def __init__(self, problem, system: System | None = None, state: dict | None = None):
+        if getattr(problem, "friction", None) is not None:
+            raise NotImplementedError("DIVA with a velocity-dependent friction law: its basal velocity is the "
+                                      "3-D reconstruction (pass it to cismgpu_system_calcbeta as ubas/vbas)")
         super().__init__(problem, system)
         p = self.p
         self.diva = ge.diva_geometry(self.grid, p.dx, p.dy, self.usrf, self.lsrf, self.ice_mask, self.land_mask)

@@ -26,6 +26,7 @@ extern "C" {
 #define CISMGPU_ERR_ARG           3   /* bad argument (sizes, bounds, options) */
 #define CISMGPU_ERR_SINGULAR      4   /* singular element Jacobian (glissade_velo_higher.F90:6611-6626) */
 #define CISMGPU_ERR_ASYM          5   /* element matrix not symmetric (glissade_velo_higher.F90:10583-10644) */
+#define CISMGPU_ERR_BETA          6   /* negative or NaN basal traction (glissade_basal_traction.F90:687-699) */
 #define CISMGPU_ERR_HIP          10   /* HIP runtime failure (message in cismgpu_last_error) */
 #define CISMGPU_ERR_UNSUPPORTED  11   /* e.g. more than one task per row/column */
 #define CISMGPU_ERR_NOT_CONVERGED 12  /* mixed-precision refinement stalled before its tolerance */
@@ -279,6 +280,9 @@ typedef struct cismgpu_assembly_in {
                                         (glissade_velo_higher.F90:8955-9111; 2-D :8386-8527); 0: local (default) */
     int32_t assemble_taud_standard;  /* 1: which_ho_assemble_taud = 0, quadrature of the gravity load
                                         (:4350-4446); 0: local (default) */
+    int32_t beta_device;             /* 1: the basal friction field is the one the last
+                                        cismgpu_system_calcbeta left on the device (beta_internal is
+                                        not read); 0: beta_internal (host) */
 } cismgpu_assembly_in;
 
 typedef struct cismgpu_assembly_info {
@@ -353,6 +357,58 @@ int32_t cismgpu_system_diva_velocity(cismgpu_system* sys, double* uvel, double* 
  * the levels above are set on owned active vertices and zero elsewhere;
  * the caller applies the halo (staggered_parallel_halo, :3723-3724). */
 int32_t cismgpu_system_l1l2_velocity(cismgpu_system* sys, const int32_t* ice_mask, double* uvel, double* vvel);
+/* ------------------------------------------------------------------ */
+/* glissade_calcbeta (glissade_basal_traction.F90:69-719) on the device: */
+/* the basal traction coefficient at every vertex from the basal        */
+/* velocity, recomputed in each Picard iteration (called at             */
+/* glissade_velo_higher.F90:2316-2332).  The fields of the basal        */
+/* physics are uploaded once (calcbeta_setup); each calcbeta call       */
+/* evaluates the sliding law, the f_ground weighting with               */
+/* which_ho_beta_limit (:650-685), the beta >= 0 check (:687-699) and   */
+/* the staggered halo (:703) into a device field that an assembly with  */
+/* beta_device = 1 reads.  Units: Pa, m/yr, Pa yr/m, Pa^-n yr^-1.       */
+/* ------------------------------------------------------------------ */
+typedef struct cismgpu_calcbeta_in {
+    int32_t whichbabc;               /* which_ho_babc (glide_types.F90:253-268): 0 constant, 1 bpmp,
+                                        2 pseudo-plastic, 3 old pseudo-plastic, 4 large, 5 external,
+                                        7 Zoet-Iverson, 9 power law, 10 Coulomb friction, 11 Schoof,
+                                        12 modified Schoof, 13 Tsai, 14 power law with N, 15 yield
+                                        stress; 6 (no-slip) and 8 (ISMIP-HOM C) are refused */
+    int32_t which_beta_limit;        /* which_ho_beta_limit (:270-271): 0 multiply by f_ground, then
+                                        floor at beta_grounded_min; 1 floor, then multiply */
+    double ho_beta_const, ho_beta_small, ho_beta_large;                  /* Pa yr/m */
+    double pseudo_plastic_q, pseudo_plastic_u0;                         /* -, m/yr */
+    double pseudo_plastic_phimin, pseudo_plastic_phimax;                /* degrees */
+    double pseudo_plastic_bedmin, pseudo_plastic_bedmax;                /* m */
+    double eus;                                                         /* m */
+    double powerlaw_m;
+    double zoet_iverson_ut, zoet_iverson_nmax;                          /* m/yr, Pa */
+    double coulomb_bump_max_slope, coulomb_bump_wavelength;             /* -, m */
+    double friction_powerlaw_k;                                         /* m yr^-1 Pa^-2 */
+    double beta_grounded_min;                                           /* Pa yr/m */
+    double beta_powerlaw_umax;                                          /* m/yr; 0: no speed cap */
+    const double*  f_ground;             /* (nx-1, ny-1) grounded fraction, always read */
+    const double*  effecpress_stag;      /* (nx-1, ny-1) effective pressure N: options 2, 3, 7, 10-14 */
+    const double*  coulomb_c;            /* (nx-1, ny-1): options 2, 7, 10-13 */
+    const double*  powerlaw_c;           /* (nx-1, ny-1): options 9, 11-13 */
+    const double*  mintauf;              /* (nx-1, ny-1) yield stress: option 15 */
+    const double*  c_space_factor_stag;  /* (nx-1, ny-1) factor of options 10, 11, 13; NULL: 1 */
+    const int32_t* bpmp_mask;            /* (nx-1, ny-1): option 1 */
+    const double*  beta_external;        /* (nx-1, ny-1) Pa yr/m: option 5 */
+    const double*  topg;                 /* (nx, ny): option 3 (read at the cell with the vertex's
+                                            indices, as the reference does, :264-266) */
+    const double*  thck;                 /* (nx, ny): option 10 (cells with thck > 0) */
+    const double*  flwa_basal;           /* (nx, ny) basal-layer rate factor: option 10 */
+} cismgpu_calcbeta_in;
+
+int32_t cismgpu_system_calcbeta_setup(cismgpu_system* sys, const cismgpu_calcbeta_in* in);
+/* beta from the basal velocity: ubas/vbas (nx-1, ny-1) host arrays, or
+ * both NULL for the resident x (its basal level on a 3-D system, x itself
+ * on a 2-D one; a DIVA caller passes the basal level of its 3-D velocity).
+ * beta (nx-1, ny-1, halo applied) is an optional host copy.  A negative or
+ * NaN value returns CISMGPU_ERR_BETA naming the vertex. */
+int32_t cismgpu_system_calcbeta(cismgpu_system* sys, const double* ubas, const double* vbas, double* beta);
+
 /* Snapshot of the assembled system (A, b, preconditioner) kept on the
  * device, and its restoration: the matrix copies of the accelerated Picard
  * iteration (evaluate_accelerated_picard_3d, glissade_velo_higher.F90:10052-10184). */

@@ -131,6 +131,18 @@ int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, 
                                   const double* staglsrf, const double* beta, const double* beta_y, double* Auu,
                                   double* Avv);
 
+/* glissade_calcbeta (oracle_basal.c): scalar parameters p[ORC_CB_*] */
+enum {
+    ORC_CB_BETA_CONST, ORC_CB_BETA_SMALL, ORC_CB_BETA_LARGE, ORC_CB_PP_Q, ORC_CB_PP_U0, ORC_CB_PHIMIN,
+    ORC_CB_PHIMAX, ORC_CB_BEDMIN, ORC_CB_BEDMAX, ORC_CB_EUS, ORC_CB_M, ORC_CB_ZI_UT, ORC_CB_ZI_NMAX,
+    ORC_CB_BUMP_SLOPE, ORC_CB_BUMP_WAVELENGTH, ORC_CB_POWERLAW_K, ORC_CB_BETA_GMIN, ORC_CB_UMAX, ORC_CB_NPARAM
+};
+int orc_calcbeta(int nx, int ny, int whichbabc, int which_beta_limit, const double* p,
+                 const double* uvel, const double* vvel, const double* effecpress_stag, const double* coulomb_c,
+                 const double* powerlaw_c, const double* mintauf, const double* c_space_factor_stag,
+                 const int32_t* bpmp_mask, const double* beta_external, const double* f_ground, const double* topg,
+                 const double* thck, const double* flwa_basal, double* beta);
+
 #ifdef __cplusplus
 }
 #endif

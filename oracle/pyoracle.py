@@ -104,6 +104,8 @@ def _declare(L):
     L.orc_slope_angle_xy.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_slope_angle_staggered.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_cell_halo.argtypes = [i, i, i, _D]
+    L.orc_calcbeta.argtypes = [i, i, i, i, _D] + [_D] * 7 + [_I] + [_D] * 5
+    L.orc_calcbeta.restype = ctypes.c_int
     L.orc_assemble_stiffness_matrix_2d_diva.argtypes = [i, i, i, _D, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D,
                                                         _D, _D, i, d, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
     L.orc_assemble_stiffness_matrix_2d_diva.restype = ctypes.c_int
@@ -321,12 +323,46 @@ def geometry(problem):
                 f_ground=fground, stagthck_lateral=stagthck_lat, stagusrf_lateral=stagusrf_lat)
 
 
-def beta_internal(problem, geo):
+def calcbeta(problem, geo, ubas, vbas):
+    """glissade_calcbeta (glissade_basal_traction.F90:69-719) for the
+    problem's friction law (cism_amd.basal.Friction) from the basal velocity
+    (nx-1, ny-1): orc_calcbeta, then the staggered halo (:703).  A negative
+    or NaN beta raises ValueError naming the vertex (the reference's
+    GM_FATAL, :687-699)."""
+    fr = problem.friction
+    g = problem.grid
+    f64 = lambda a: None if a is None else np.asfortranarray(np.array(a, dtype=np.float64))
+    prm = np.ascontiguousarray(fr.params(problem.eus))
+    flwa_basal = f64(fr.flwa_basal if fr.flwa_basal is not None else np.asarray(problem.flwa)[g.nz - 2])
+    bpmp = None if fr.bpmp_mask is None else np.asfortranarray(np.array(fr.bpmp_mask, dtype=np.int32))
+    flds = [f64(getattr(fr, k)) for k in ("effecpress_stag", "coulomb_c", "powerlaw_c", "mintauf",
+                                           "c_space_factor_stag")]
+    bext, fg, topg, thck = f64(fr.beta_external), f64(geo["f_ground"]), f64(problem.topg), f64(problem.thck)
+    u, v = f64(ubas), f64(vbas)
+    b = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    opt = lambda a: None if a is None else dp(a)
+    st = lib().orc_calcbeta(g.nx, g.ny, fr.whichbabc, fr.which_beta_limit, dp(prm), dp(u), dp(v),
+                            *[opt(a) for a in flds], ip(bpmp) if bpmp is not None else None, opt(bext), dp(fg),
+                            opt(topg), opt(thck), opt(flwa_basal), dp(b))
+    if st < 0:
+        raise ValueError(f"which_ho_babc {fr.whichbabc} is not covered")
+    if st > 0:
+        t = st - 1
+        raise ValueError(f"invalid beta {b.ravel(order='F')[t]} at vertex i={t % (g.nx - 1) + 1} "
+                         f"j={t // (g.nx - 1) + 1}")
+    halo(g, b)
+    return b
+
+
+def beta_internal(problem, geo, ubas=None, vbas=None):
     """glissade_calcbeta for the constant/large/external cases
     (glissade_basal_traction.F90:202, 304, 399), times f_ground with
-    beta_grounded_min = 0 (:650-672), then the staggered halo (:697)."""
+    beta_grounded_min = 0 (:650-672), then the staggered halo (:697); a
+    problem with a friction law: :func:`calcbeta` of the basal velocity."""
     p = problem
     g = p.grid
+    if getattr(p, "friction", None) is not None:
+        return calcbeta(p, geo, ubas, vbas)
     if p.whichbabc == 0:
         b = np.full((g.nx - 1, g.ny - 1), p.beta_const, order="F")
     elif p.whichbabc == 4:
@@ -418,7 +454,7 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
                                             p.effstrain_min, p.n_glen, dp(efvs), *[dp(a) for a in A])
     if st:
         raise RuntimeError(f"oracle assembly failed with status {st}")
-    binternal = beta_internal(p, geo)
+    binternal = beta_internal(p, geo, uvel[nz - 1], vvel[nz - 1])
     if p.whichbabc != 6:
         if getattr(p, "assemble_beta", 1) == 0:
             st = L.orc_basal_sliding_bc_standard(nx, ny, nz, 2, 0, 1, ip(geo["active_cell"]), dp(xV), dp(yV),
@@ -541,7 +577,9 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     A = [np.zeros((nx - 1, ny - 1, 9), order="F") for _ in range(4)]
     efvs = np.zeros((nz - 1, nx, ny), order="F")
     extra = {}
-    binternal = beta_internal(p, geo)
+    if approx == "diva" and getattr(p, "friction", None) is not None:
+        raise NotImplementedError("DIVA with a velocity-dependent friction law")
+    binternal = beta_internal(p, geo, u2, v2)
     if approx == "diva":
         dg = geo.get("diva") or geometry_diva(p, geo)
         geo["diva"] = dg

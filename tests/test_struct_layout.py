@@ -8,7 +8,7 @@ import subprocess
 
 import pytest
 
-from cism_amd import _lib, ensemble, grid, velo
+from cism_amd import _lib, basal, ensemble, grid, velo
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -21,6 +21,7 @@ MIRRORS = [
     ("cismgpu_diva_in", velo.DivaIn),
     ("cismgpu_ensemble_opts", ensemble.EnsOpts),
     ("cismgpu_ensemble_info", ensemble.EnsInfo),
+    ("cismgpu_calcbeta_in", basal.CalcbetaIn),
 ]
 
 
