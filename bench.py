@@ -200,6 +200,13 @@ def main():
         spmv_iso[name] = dict(avg_ms=round(ms, 4), algorithmic_bytes=int(by), gbps=round(by / (ms * 1e-3) / 1e9, 1),
                               frac_of_peak=round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3))
     log(f"isolated SpMV: {spmv_iso}")
+    # velocity-dependent basal traction on the same grid (glissade_calcbeta, Schoof law)
+    cbeta, cb_sample = None, None
+    try:
+        cbeta, cb_sample = run_calcbeta(args, bp, s, xm)
+        log(f"calcbeta: {cbeta}")
+    except Exception as e:
+        log(f"calcbeta leg failed: {e!r}")
     # measured HBM ceilings on this box (STREAM-style copy / triad / read, 2 GiB arrays)
     from cism_amd import _lib
     hbm = {k: round(v, 1) for k, v in _lib.bench_stream(2 << 30, 10).items()}
@@ -223,6 +230,12 @@ def main():
         except Exception as e:   # the CPU leg must not hide the GPU result
             log(f"CPU baseline failed: {e!r}")
 
+    if cpu is not None and cb_sample is not None:
+        try:
+            cpu["calcbeta"] = calcbeta_oracle_check(cb_sample)
+        except Exception as e:
+            log(f"calcbeta oracle check failed: {e!r}")
+    del cb_sample
     twod = run_twod(args, g, bp.active_vertex, twod_sys) if twod_sys is not None else None
     del s, bp, twod_sys
     cfg2 = run_config2(args) if args.config2_n > 0 else None
@@ -278,6 +291,7 @@ def main():
         "dropin_host_call": dropin,
         "hbm_measured_gbps": hbm,
         "assembly": {"element_kernel_ms": round(ainfo["assemble_ms"], 2), "total_ms": round(ainfo["total_ms"], 2)},
+        "calcbeta": cbeta,
         "cpu_baseline": cpu,
         "fp64_config2": cfg2,
         "ensemble": ens,
@@ -290,6 +304,72 @@ def main():
     if args.out:
         with open(args.out, "w") as f:
             f.write(line + "\n")
+
+
+# algorithmic bytes per vertex of k_calcbeta for the Schoof law with a
+# c_space_factor field: u, v (16) + f_ground, N, coulomb_c, powerlaw_c,
+# c_space_factor (40) read, beta (8) written
+CALCBETA_BYTES_PER_VERTEX = 64
+
+
+def run_calcbeta(args, bp, s, xm, ncall=20):
+    """glissade_calcbeta (which_ho_babc 11, Schoof, with c_space_factor) from
+    the resident basal velocity of the headline grid: wall time per call
+    (kernel, invalid-value check and stream synchronisation) and the sample
+    the CPU leg checks against the oracle."""
+    import numpy as np
+
+    from cism_amd import basal, geometry as ge
+    p = bp.p
+    g = p.grid
+    rng = np.random.default_rng(11)
+    shp = (g.nvx, g.nvy)
+    stag = 0.25 * (p.thck[:-1, 1:] + p.thck[1:, 1:] + p.thck[:-1, :-1] + p.thck[1:, :-1])
+    fr = basal.Friction(11, powerlaw_m=3.0,
+                        effecpress_stag=np.asfortranarray(910.0 * 9.81 * stag * rng.uniform(0.05, 0.3, shp)),
+                        coulomb_c=np.asfortranarray(rng.uniform(0.3, 0.6, shp)),
+                        powerlaw_c=np.asfortranarray(rng.uniform(1.0e4, 3.0e4, shp)),
+                        c_space_factor_stag=np.asfortranarray(rng.uniform(0.8, 1.0, shp)))
+    fg = ge.grounded_fraction(bp.ice_mask, bp.floating_mask, bp.land_mask)
+    basal.device_setup(s, fr, f_ground=fg, topg=p.topg, thck=p.thck, eus=p.eus,
+                       flwa_basal=np.asarray(p.flwa)[g.nz - 2])
+    s.set_x(np.asfortranarray(xm[0]), np.asfortranarray(xm[1]))
+    for _ in range(3):
+        basal.device_calcbeta(s)
+    tw = time.perf_counter()
+    for _ in range(ncall):
+        basal.device_calcbeta(s)
+    ms = 1e3 * (time.perf_counter() - tw) / ncall
+    beta = basal.device_calcbeta(s, want=True)
+    nv = g.nvx * g.nvy
+    out = dict(law="which_ho_babc 11 (Schoof) with c_space_factor", vertices=nv, ms_per_call=round(ms, 4),
+               algorithmic_bytes=nv * CALCBETA_BYTES_PER_VERTEX,
+               note="wall time per cismgpu_system_calcbeta call: k_calcbeta, the invalid-value readback and the "
+                    "stream synchronisation; the kernel alone is in profiles/kernel_stats_calcbeta1024.md")
+    sample = dict(problem=p, friction=fr, f_ground=fg, ubas=np.asfortranarray(xm[0][g.nz - 1]),
+                  vbas=np.asfortranarray(xm[1][g.nz - 1]), beta=beta)
+    return out, sample
+
+
+def calcbeta_oracle_check(sample):
+    """The oracle's calcbeta of the same basal velocity (1 thread): error of
+    the device field against it and the CPU time."""
+    import types
+
+    import numpy as np
+
+    from oracle import pyoracle as po
+    p = sample["problem"]
+    q = types.SimpleNamespace(grid=p.grid, friction=sample["friction"], eus=p.eus, topg=p.topg, thck=p.thck,
+                              flwa=p.flwa)
+    tw = time.perf_counter()
+    ref = po.calcbeta(q, {"f_ground": sample["f_ground"]}, sample["ubas"], sample["vbas"])
+    cpu_s = time.perf_counter() - tw
+    b = sample["beta"]
+    scale = np.where(ref != 0, np.abs(ref), 1.0)
+    return dict(cpu_s=round(cpu_s, 3), cores=1, max_rel_err_vs_oracle=float(np.max(np.abs(b - ref) / scale)),
+                kind="port", sample=f"the whole {p.grid.nvx}x{p.grid.nvy} vertex field, one call of orc_calcbeta "
+                                    f"on 1 thread")
 
 
 def run_cpu_legs(args, g, act, s, x64, xm, xb, it64):
