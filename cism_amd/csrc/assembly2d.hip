@@ -533,8 +533,11 @@ __global__ void k_basal_diva_2d(Dims d, AsmDev in, DivaDev dv, double* __restric
 
 // compute_3d_velocity_diva: basal traction at every vertex; the 3-D velocity
 // (reference layout (nz, nvx, nvy), optional) on active vertices
+// ubo/vbo (optional, (nvx, nvy)): the basal level at every vertex, zero at
+// inactive ones (the basal velocity the next glissade_calcbeta reads)
 __global__ void k_diva_velocity(Dims d, AsmDev in, Lev L, DivaDev dv, const double* __restrict__ uv,
-                                const double* __restrict__ vv, double* __restrict__ u3, double* __restrict__ v3) {
+                                const double* __restrict__ vv, double* __restrict__ u3, double* __restrict__ v3,
+                                double* __restrict__ ubo, double* __restrict__ vbo) {
 #pragma clang fp contract(off)
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (t >= (long long)d.nvx * d.nvy) return;
@@ -544,16 +547,22 @@ __global__ void k_diva_velocity(Dims d, AsmDev in, Lev L, DivaDev dv, const doub
     const double btx = dv.bex[t] * u, bty = dv.bey[t] * v;
     dv.btx[t] = btx;
     dv.bty[t] = bty;
-    if (!u3 || !in.avert[t]) return;
-    double ub, vb;
-    const double b = in.beta[t];
-    if (b > 0.0) {
-        ub = u * dv.bex[t] / b;
-        vb = v * dv.bey[t] / b;
-    } else {
-        ub = u;
-        vb = v;
+    double ub = 0.0, vb = 0.0;
+    if (in.avert[t]) {
+        const double b = in.beta[t];
+        if (b > 0.0) {
+            ub = u * dv.bex[t] / b;
+            vb = v * dv.bey[t] / b;
+        } else {
+            ub = u;
+            vb = v;
+        }
     }
+    if (ubo) {
+        ubo[t] = ub;
+        vbo[t] = vb;
+    }
+    if (!u3 || !in.avert[t]) return;
     const int nz = L.nz;
     u3[(long long)(nz - 1) + (long long)nz * t] = ub;
     v3[(long long)(nz - 1) + (long long)nz * t] = vb;

@@ -116,6 +116,8 @@ struct cismgpu_system {
     CBDev cb{};
     double* cb_beta = nullptr;   // (nvx, nvy), then a scratch field of the same size
     double* cb_vel = nullptr;    // 2 (nvx, nvy) fields
+    double* cb_dvel = nullptr;   // DIVA: basal level of the last compute_3d_velocity_diva, 2 (nvx, nvy) fields
+    bool cb_dvel_valid = false;
     unsigned long long* cb_bad = nullptr;
     bool cb_setup = false;       // calcbeta_setup done
     bool cb_valid = false;       // cb_beta holds a calcbeta result
@@ -1870,6 +1872,7 @@ extern "C" int32_t cismgpu_system_diva_setup(cismgpu_system* S, const cismgpu_di
     dv.no_slip = in->no_slip ? 1 : 0;
     CG_CHECK(hipStreamSynchronize(S->s));
     S->diva_ready = true;
+    S->cb_dvel_valid = false;
     return CISMGPU_OK;
 }
 
@@ -1887,8 +1890,11 @@ extern "C" int32_t cismgpu_system_diva_velocity(cismgpu_system* S, double* uvel,
         CG_CHECK(hipMalloc(&u3, 2 * nz * n2 * sizeof(double)));
         CG_CHECK(hipMemsetAsync(u3, 0, 2 * nz * n2 * sizeof(double), S->s));   // inactive vertices stay zero
     }
+    // the basal level goes to the calcbeta basal pair when the system has a friction law
+    double* ub = S->cb_setup ? S->cb_dvel : nullptr;
     k_diva_velocity<<<nblocks(n2, 256), 256, 0, S->s>>>(d, S->asm_in, Lev{(int)nz}, S->dv, S->V(0), S->V(1), u3,
-                                                        u3 ? u3 + nz * n2 : nullptr);
+                                                        u3 ? u3 + nz * n2 : nullptr, ub, ub ? ub + n2 : nullptr);
+    if (ub) S->cb_dvel_valid = true;
     int st = CISMGPU_OK;
     if (u3) {
         if (hipMemcpyAsync(uvel, u3, nz * n2 * sizeof(double), hipMemcpyDeviceToHost, S->s) != hipSuccess ||
@@ -2027,13 +2033,16 @@ extern "C" int32_t cismgpu_system_calcbeta_setup(cismgpu_system* S, const cismgp
         }
     }
     if (!S->cb_buf) {
-        // doubles: beta + scratch, velocity pair, 7 vertex fields, 2 cell fields; then ints, the bad slot
-        const size_t bytes = (size_t)(11 * n2 + 2 * nc) * sizeof(double) + (size_t)(n2 + nc) * sizeof(int32_t) + 64;
+        // doubles: beta + scratch, host and DIVA velocity pairs, 7 vertex fields, 2 cell fields;
+        // then ints, the bad slot
+        const size_t bytes = (size_t)(13 * n2 + 2 * nc) * sizeof(double) + (size_t)(n2 + nc) * sizeof(int32_t) + 64;
         CG_CHECK(hipMalloc(&S->cb_buf, bytes));
     }
     double* p = (double*)S->cb_buf;
     S->cb_beta = p; p += 2 * n2;
     S->cb_vel = p; p += 2 * n2;
+    S->cb_dvel = p; p += 2 * n2;
+    S->cb_dvel_valid = false;
     CBDev c{};
     c.babc = b;
     c.limit = in->which_beta_limit;
@@ -2097,6 +2106,13 @@ extern "C" int32_t cismgpu_system_calcbeta(cismgpu_system* S, const double* ubas
     const long long n2 = (long long)d.nvx * d.nvy;
     const double *u = S->V(0), *v = S->V(1);
     int lev = d.nz - 1;   // basal level of x (the only level of a 2-D system)
+    if (!ubas && S->two_d && S->diva_ready && S->cb_dvel_valid) {
+        // DIVA: the basal level of the last compute_3d_velocity_diva (the
+        // reference's uvel(nz,:,:), glissade_velo_higher.F90:2293-2296)
+        u = S->cb_dvel;
+        v = S->cb_dvel + n2;
+        lev = -1;
+    }
     if (ubas) {
         CG_CHECK(hipMemcpyAsync(S->cb_vel, ubas, n2 * sizeof(double), hipMemcpyHostToDevice, S->s));
         CG_CHECK(hipMemcpyAsync(S->cb_vel + n2, vbas, n2 * sizeof(double), hipMemcpyHostToDevice, S->s));

@@ -466,19 +466,19 @@ class DIVASolver(SSASolver):
     where beta_eff and omega_k are saved and restored with the matrix
     (:9898-10048).  No Dirichlet rows: no-slip enters through beta_eff
     (:1423-1428).  The state (efvs_qp_3d, btractx/y) stays on the device and
-    starts at zero unless given."""
+    starts at zero unless given.  A friction law reads the basal level of
+    the last 3-D velocity (the 2-D guess before the first), as calcbeta
+    reads uvel(nz,:,:) (:2293-2296)."""
 
     APPROX = 4
 
     def __init__(self, problem, system: System | None = None, state: dict | None = None):
-        if getattr(problem, "friction", None) is not None:
-            raise NotImplementedError("DIVA with a velocity-dependent friction law: its basal velocity is the "
-                                      "3-D reconstruction (pass it to cismgpu_system_calcbeta as ubas/vbas)")
         super().__init__(problem, system)
         p = self.p
         self.diva = ge.diva_geometry(self.grid, p.dx, p.dy, self.usrf, self.lsrf, self.ice_mask, self.land_mask)
         self.no_slip = p.whichbabc == 6
-        self._ain.beta_internal = self.beta_internal.ctypes.data
+        if self.friction is None:
+            self._ain.beta_internal = self.beta_internal.ctypes.data
         self._ain.umask_dirichlet = None
         self._ain.vmask_dirichlet = None
         self.set_state(state)

@@ -403,8 +403,9 @@ typedef struct cismgpu_calcbeta_in {
 
 int32_t cismgpu_system_calcbeta_setup(cismgpu_system* sys, const cismgpu_calcbeta_in* in);
 /* beta from the basal velocity: ubas/vbas (nx-1, ny-1) host arrays, or
- * both NULL for the resident x (its basal level on a 3-D system, x itself
- * on a 2-D one; a DIVA caller passes the basal level of its 3-D velocity).
+ * both NULL for the system's own: the basal level of x on a 3-D system, x
+ * itself on a 2-D one, and on a DIVA system the basal level of the last
+ * cismgpu_system_diva_velocity since diva_setup (x before the first).
  * beta (nx-1, ny-1, halo applied) is an optional host copy.  A negative or
  * NaN value returns CISMGPU_ERR_BETA naming the vertex. */
 int32_t cismgpu_system_calcbeta(cismgpu_system* sys, const double* ubas, const double* vbas, double* beta);

@@ -543,7 +543,7 @@ def diva_state(problem):
                 btracty=np.zeros((g.nx - 1, g.ny - 1), order="F"))
 
 
-def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", state=None):
+def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", state=None, ubas=None, vbas=None):
     """One assembly of the 2-D system in the reference's order
     (glissade_velo_higher.F90:2353-2725): stiffness (assemble_stiffness_matrix_2d),
     basal BC (basal_sliding_bc_2d; for DIVA beta_eff from omega and
@@ -551,7 +551,9 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     :2636), Dirichlet rows with the basal-level masks (none for DIVA),
     halo of A and b, symmetry report and nonzero count.  The default 2-D
     velocity is the basal level of the 3-D guess (:1362).  approx "diva"
-    updates state (efvs_qp_3d; btractx/y are read) in place."""
+    updates state (efvs_qp_3d; btractx/y are read) in place.  ubas/vbas:
+    the basal velocity a friction law reads (default the 2-D velocity; for
+    DIVA the basal level of the last 3-D velocity, :2293-2296)."""
     L = lib()
     p = problem
     g = p.grid
@@ -577,9 +579,7 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     A = [np.zeros((nx - 1, ny - 1, 9), order="F") for _ in range(4)]
     efvs = np.zeros((nz - 1, nx, ny), order="F")
     extra = {}
-    if approx == "diva" and getattr(p, "friction", None) is not None:
-        raise NotImplementedError("DIVA with a velocity-dependent friction law")
-    binternal = beta_internal(p, geo, u2, v2)
+    binternal = beta_internal(p, geo, u2 if ubas is None else ubas, v2 if vbas is None else vbas)
     if approx == "diva":
         dg = geo.get("diva") or geometry_diva(p, geo)
         geo["diva"] = dg
@@ -652,9 +652,11 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
                 nnz=nnz, geo=geo, **extra)
 
 
-def diva_velocity(problem, s, u2, v2):
+def diva_velocity(problem, s, u2, v2, beta=None):
     """compute_3d_velocity_diva after a 2-D DIVA solve: updates the basal
-    traction in s["state"] and returns the 3-D velocity (nz, nx-1, ny-1)."""
+    traction in s["state"] and returns the 3-D velocity (nz, nx-1, ny-1).
+    beta: the current beta_internal (default the one of assembly s; after a
+    restored assembly the reference keeps the last calcbeta result)."""
     L = lib()
     p = problem
     g = p.grid
@@ -662,7 +664,8 @@ def diva_velocity(problem, s, u2, v2):
     u3 = g.zeros_vec(); v3 = g.zeros_vec()
     st = s["state"]
     L.orc_compute_3d_velocity_diva(g.nx, g.ny, g.nz, ip(s["active_vertex"]), ip(dg["ice_plus_land_mask"]),
-                                   dp(s["omega_k"]), dp(s["beta_internal"]), dp(s["beta_eff_x"]), dp(s["beta_eff_y"]),
+                                   dp(s["omega_k"]), dp(s["beta_internal"] if beta is None else beta),
+                                   dp(s["beta_eff_x"]), dp(s["beta_eff_y"]),
                                    dp(dg["stag_theta_slope_x"]), dp(dg["stag_theta_slope_y"]),
                                    dp(dg["stag_slope_factor_x"]), dp(dg["stag_slope_factor_y"]), dp(u2), dp(v2),
                                    dp(st["btractx"]), dp(st["btracty"]), dp(u3), dp(v3))
@@ -779,6 +782,8 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
     criterion = np.inf
     counter = 0
     s = None
+    ub = vb = None          # DIVA: basal level of the last 3-D velocity (None: the 2-D guess)
+    beta_cur = None         # DIVA: beta_internal of the last assembly (kept across a restore)
     while criterion >= resid_target and counter < maxiter:
         counter += 1
         done = False
@@ -788,9 +793,12 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
                 v = np.asfortranarray(v_old + alpha * dv)
                 halo(g, u); halo(g, v)
                 if diva:     # basal traction of the extended velocity (:2188-2210)
-                    diva_velocity(p, s, u, v)
+                    e3u, e3v = diva_velocity(p, s, u, v, beta=beta_cur)
+                    ub, vb = e3u[g.nz - 1].copy(order="F"), e3v[g.nz - 1].copy(order="F")
             usav, vsav = u.copy(order="F"), v.copy(order="F")
-            s = (assemble_2d(p, u, v, geo, approx=approx, state=dstate) if two_d else assemble(p, u, v, geo))
+            s = (assemble_2d(p, u, v, geo, approx=approx, state=dstate, ubas=ub, vbas=vb) if two_d
+                 else assemble(p, u, v, geo))
+            beta_cur = s.get("beta_internal")
             A = [s["Auu"], s["Auv"], s["Avu"], s["Avv"]]
             L2, L2r = (residual_L2_2d if two_d else residual_L2)(g, s["active_vertex"], A, s["bu"], s["bv"], u, v)
             hist["L2"].append(L2)
@@ -818,8 +826,9 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
         hist["linear_iters"].append(nit); hist["linear_err"].append(err)
         halo(g, u); halo(g, v)
         if diva:
-            u3, v3 = diva_velocity(p, s, u, v)
+            u3, v3 = diva_velocity(p, s, u, v, beta=beta_cur)
             halo(g, u3); halo(g, v3)
+            ub, vb = u3[g.nz - 1].copy(order="F"), v3[g.nz - 1].copy(order="F")
         if two_d:
             rv = L.orc_residual_velocity_2d(g.nx, g.ny, g.ilo, g.ihi, g.jlo, g.jhi, whichresid,
                                             dp(usav), dp(vsav), dp(u), dp(v))

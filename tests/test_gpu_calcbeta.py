@@ -6,9 +6,9 @@ Bar: beta within 1e-13 relative of the oracle at every vertex, halo
 included (the device pow and tan are within an ulp or two of the host's
 correctly rounded ones, the rest is rounded in the same order); the host
 and resident basal-velocity paths give the same field bitwise; a NaN beta
-is CISMGPU_ERR_BETA naming the oracle's vertex; Picard (BP, plain and
-accelerated, and SSA) with a velocity-dependent law reaches the oracle's
-velocities (1e-7) in the same number of iterations.
+is CISMGPU_ERR_BETA naming the oracle's vertex; Picard (BP and DIVA, plain
+and accelerated; SSA; L1L2) with a velocity-dependent law reaches the
+oracle's velocities (1e-7) in the same number of iterations.
 """
 from types import SimpleNamespace
 
@@ -19,7 +19,7 @@ from cism_amd import _lib, basal, synthetic
 from cism_amd.basal import Friction
 from cism_amd.grid import BC_NO_ICE, BC_PERIODIC, Grid
 from cism_amd.pcg import System
-from cism_amd.velo import AssemblyInfo, BPSolver, SSASolver
+from cism_amd.velo import AssemblyInfo, BPSolver, DIVASolver, L1L2Solver, SSASolver
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -192,6 +192,31 @@ def test_ssa_picard_with_coulomb_law_matches_oracle(require_gpu):
     kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=80)
     ref = po.picard(p, two_d=True, precond=1, **kw)
     r = SSASolver(p).picard(precond=1, **kw)
+    _compare(r, ref)
+
+
+@pytest.mark.parametrize("law", ["powerlaw", "schoof"])
+@pytest.mark.parametrize("accelerate", [False, True])
+def test_diva_picard_with_sliding_law_matches_oracle(require_gpu, law, accelerate):
+    """DIVA: calcbeta reads the basal level of the last 3-D velocity
+    (compute_3d_velocity_diva after each solve and each extension), and the
+    3-D velocity after a restored assembly uses the latest beta."""
+    p = _stream_with(_powerlaw if law == "powerlaw" else _schoof)
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=80, accelerate=accelerate)
+    ref = po.picard(p, two_d=True, approx="diva", precond=1, **kw)
+    r = DIVASolver(p).picard(precond=1, **kw)
+    _compare(r, ref)
+    if accelerate:
+        assert r.alpha == pytest.approx(ref["alpha"])
+    num = np.linalg.norm(np.stack([r.uvel3d - ref["uvel3d"], r.vvel3d - ref["vvel3d"]]))
+    assert num / np.linalg.norm(np.stack([ref["uvel3d"], ref["vvel3d"]])) < 1e-7
+
+
+def test_l1l2_picard_with_sliding_law_matches_oracle(require_gpu):
+    p = _stream_with(_powerlaw)
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=80)
+    ref = po.picard(p, two_d=True, approx="l1l2", precond=1, **kw)
+    r = L1L2Solver(p).picard(precond=1, **kw)
     _compare(r, ref)
 
 
