@@ -50,6 +50,7 @@ class Problem:
     assemble_lateral: int = 0    # which_ho_assemble_lateral (glide_types.F90:1067): 0 standard, 1 local
     assemble_beta: int = 1       # which_ho_assemble_beta (glide_types.F90:359-360, 1050): 0 standard, 1 local
     assemble_taud: int = 1       # which_ho_assemble_taud (glide_types.F90:362-363, 1042): 0 standard, 1 local
+    kinbcmask: np.ndarray | None = None   # (nx-1, ny-1) int: 1 = u, v prescribed (from uvel/vvel) in the column
     friction: object = None      # basal.Friction: beta from the basal velocity in every Picard iteration
                                  # (glissade_calcbeta); None: the fixed beta of whichbabc 0, 4, 5 or no-slip
     meta: dict = field(default_factory=dict)
@@ -276,6 +277,109 @@ def marine_sheet(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0, f
                   h_shelf=h_shelf, bed_front=bed_front, bed_interior=bed_interior, gl_frac=gl_frac,
                   front_frac=front_frac, margin_frac=margin_frac),
     )
+    return p
+
+
+# The reference's ice-stream test (tests/stream/runStream.py): a uniform slab
+# (H = 1000 m, bed slope 1e-3 along x) over till whose yield stress varies
+# across the flow only, which_ho_babc = 15 (beta = tauf / |u|), doubly
+# periodic, with one across-flow column of prescribed velocity.
+STREAM_HALF_WIDTH = 25000.0
+STREAM_H = 1000.0
+STREAM_DSDX = -1.0e-3
+STREAM_A = 1.0e-16
+STREAM_SCHOOF_M = 1.55
+
+
+def stream_driving_stress():
+    return RHOI_SYN * 9.81 * STREAM_H * (-STREAM_DSDX)
+
+
+def stream_yield_stress(kind: str, y: np.ndarray) -> np.ndarray:
+    """Till yield stress across the stream: Raymond (2000), 5.2 kPa inside
+    and 70 kPa outside; Schoof (2006), tau_d |y / L|^m with
+    L = W / (m + 1)^(1/m) (runStream.py:150-175)."""
+    W = STREAM_HALF_WIDTH
+    if kind == "raymond":
+        t = np.full(np.shape(y), 5.2e3)
+        t[np.abs(y) >= W] = 0.7e5
+        return t
+    m = STREAM_SCHOOF_M
+    L = W / (m + 1.0) ** (1.0 / m)
+    return stream_driving_stress() * np.abs(y / L) ** m
+
+
+def stream_velocity(kind: str, y: np.ndarray) -> np.ndarray:
+    """Analytic along-flow speed across the stream (m/yr): Raymond (2000)
+    u = 2A/(n+1) ((tau_d - tau_0)/H)^n (W^(n+1) - |y|^(n+1)); Schoof (2006)
+    eq. 4.3 with n = 3 (runStream.py:160-190)."""
+    W, H, A, n = STREAM_HALF_WIDTH, STREAM_H, STREAM_A, 3.0
+    taud = stream_driving_stress()
+    y = np.asarray(y, dtype=np.float64)
+    if kind == "raymond":
+        t0 = np.minimum(stream_yield_stress(kind, y), taud)
+        u = 2.0 * A / (n + 1.0) * ((taud - t0) / H) ** n * (W ** (n + 1) - np.abs(y) ** (n + 1))
+        return np.maximum(u, 0.0)
+    m = STREAM_SCHOOF_M
+    L = W / (m + 1.0) ** (1.0 / m)
+    B = A ** (-1.0 / n)
+    yl = np.abs(y / L)
+    u = -2.0 * taud ** 3 * L ** 4 / (B ** 3 * H ** 3) * (
+        ((y / L) ** 4 - (m + 1.0) ** (4.0 / m)) / 4.0
+        - 3.0 * (yl ** (m + 4.0) - (m + 1.0) ** (1.0 + 4.0 / m)) / ((m + 1.0) * (m + 4.0))
+        + 3.0 * (yl ** (2.0 * m + 4.0) - (m + 1.0) ** (2.0 + 4.0 / m)) / ((m + 1.0) ** 2 * (2.0 * m + 4.0))
+        - (yl ** (3.0 * m + 4.0) - (m + 1.0) ** (3.0 + 4.0 / m)) / ((m + 1.0) ** 3 * (3.0 * m + 4.0)))
+    u[np.abs(y) >= W] = 0.0
+    return u
+
+
+def yield_stress_stream(kind: str = "raymond", n_stream: int = 25, nz: int = 2, along: float = 30000.0,
+                        kin_column: int = 2) -> Problem:
+    """The ice-stream test of the reference (tests/stream/runStream.py,
+    stream.config): n_stream cells across the stream (dx = dy = 2W/n_stream),
+    strips of 5H (Raymond) or 3L - W (Schoof) of strong till on each side,
+    ``along`` metres along the flow, doubly periodic; the bed continues its
+    slope through the halos (the periodic_offset_ew of the reference).  The
+    yield stress is the problem's friction (which_ho_babc 15,
+    beta_grounded_min 0); owned vertex column ``kin_column`` carries the
+    analytic speed as a kinematic (Dirichlet) BC with v = 0, the rest of the
+    velocity starts at zero.  Vertex rows sit at y = (jo - (ny-1)/2 + 1/2) dy
+    so that the stream edges |y| = W fall on vertices; meta["y_vertex"] holds
+    every vertex row's y (wrapped into one period)."""
+    from .basal import Friction
+    if kind not in ("raymond", "schoof"):
+        raise ValueError("kind must be 'raymond' or 'schoof'")
+    W, H = STREAM_HALF_WIDTH, STREAM_H
+    dy = 2.0 * W / n_stream
+    dx = dy
+    if kind == "raymond":
+        strong = 5.0 * H
+    else:
+        strong = 3.0 * (W / (STREAM_SCHOOF_M + 1.0) ** (1.0 / STREAM_SCHOOF_M)) - W
+    ny_own = n_stream + 2 * int(round(strong / dy + 1.0e-10))
+    nx_own = int(round(along / dx))
+    g = Grid.from_owned(nx_own, ny_own, nz, global_bc=BC_PERIODIC)
+    period = ny_own * dy
+    yv = (np.arange(1, g.ny) - g.jlo) * dy - dy * (ny_own - 1) / 2.0 + dy / 2.0
+    yv = (yv + period / 2.0) % period - period / 2.0
+    xc = (np.arange(1, g.nx + 1) - 2.5) * dx
+    topg = np.asfortranarray(np.repeat((1000.0 + STREAM_DSDX * xc)[:, None], g.ny, axis=1))
+    tauf = np.asfortranarray(np.repeat(stream_yield_stress(kind, yv)[None, :], g.nvx, axis=0))
+    uvel, vvel = g.zeros_vec(), g.zeros_vec()
+    kin = np.zeros((g.nvx, g.nvy), np.int32, order="F")
+    ic = g.ilo - 1 + kin_column
+    kin[ic, :] = 1
+    uvel[:, ic, :] = stream_velocity(kind, yv)[None, :]
+    p = Problem(
+        grid=g, dx=dx, dy=dy, sigma=glide_sigma(nz),
+        thck=np.full((g.nx, g.ny), H, order="F"), topg=topg,
+        flwa=np.full((nz - 1, g.nx, g.ny), STREAM_A, order="F"),
+        beta=np.zeros((g.nvx, g.nvy), order="F"),
+        uvel=np.asfortranarray(uvel), vvel=np.asfortranarray(vvel), whichbabc=HO_BABC_BETA_EXTERNAL,
+        gradient_margin=0, kinbcmask=kin,
+        meta=dict(workload="yield_stress_stream", kind=kind, n_stream=n_stream, nz=nz, y_vertex=yv),
+    )
+    p.friction = Friction(15, mintauf=tauf, beta_grounded_min=0.0)
     return p
 
 

@@ -153,10 +153,17 @@ class BPSolver:
         self.sigma = np.ascontiguousarray(p.sigma, dtype=np.float64)
         self.swbs = np.ascontiguousarray(stagwbndsigma(p.sigma))
         no_slip = p.whichbabc == 6 and self.friction is None
+        # Dirichlet masks (glissade_velo_higher.F90:1418-1474): the bed for
+        # no-slip, whole columns where kinbcmask = 1 (u and v prescribed from
+        # the input velocity)
         self.umask = np.zeros(g.vec_shape, np.int32, order="F")
         if no_slip:
             self.umask[g.nz - 1] = 1
+        self.kinbc = self._kinbc_columns()
+        if self.kinbc is not None:
+            self.umask[:, self.kinbc] = 1
         self.vmask = self.umask.copy(order="F")
+        has_mask = bool(self.umask.any())
         # lateral BC at calving fronts: staggered thickness/surface over ice-covered or
         # land cells (glissade_velo_higher.F90:1510-1515, 2006-2016)
         self.thck = np.asfortranarray(np.array(p.thck, dtype=np.float64))
@@ -169,8 +176,8 @@ class BPSolver:
             self.active_vertex.ctypes.data, self.xV.ctypes.data, self.yV.ctypes.data, self.stagusrf.ctypes.data,
             self.stagthck.ctypes.data, self.dusrf_dx.ctypes.data, self.dusrf_dy.ctypes.data,
             self.flwafact.ctypes.data, None if no_slip else self.beta_internal.ctypes.data,
-            self.theta_basal.ctypes.data, self.umask.ctypes.data if no_slip else None,
-            self.vmask.ctypes.data if no_slip else None, p.whichefvs, p.efvs_constant, p.effstrain_min,
+            self.theta_basal.ctypes.data, self.umask.ctypes.data if has_mask else None,
+            self.vmask.ctypes.data if has_mask else None, p.whichefvs, p.efvs_constant, p.effstrain_min,
             p.n_glen, 1,
             self.ocean_mask.ctypes.data if has_front else None, self.usrf.ctypes.data, self.thck.ctypes.data,
             self.stagusrf_lat.ctypes.data, self.stagthck_lat.ctypes.data, int(p.assemble_lateral))
@@ -183,6 +190,16 @@ class BPSolver:
         if self.friction is not None:
             self._ain.beta_internal = None
             self._ain.beta_device = 1
+
+    def _kinbc_columns(self):
+        """Boolean (nx-1, ny-1) of the kinematic-BC columns (kinbcmask = 1), or None."""
+        kin = getattr(self.p, "kinbcmask", None)
+        if kin is None:
+            return None
+        k = np.asarray(kin)
+        if k.shape != (self.grid.nvx, self.grid.nvy):
+            raise ValueError("kinbcmask must be (nx-1, ny-1)")
+        return (k == 1) if (k == 1).any() else None
 
     def _initial_velocity(self):
         u = np.asfortranarray(np.array(self.p.uvel, dtype=np.float64))
@@ -479,8 +496,16 @@ class DIVASolver(SSASolver):
         self.no_slip = p.whichbabc == 6
         if self.friction is None:
             self._ain.beta_internal = self.beta_internal.ctypes.data
-        self._ain.umask_dirichlet = None
-        self._ain.vmask_dirichlet = None
+        # no Dirichlet rows for no-slip (beta_eff carries it, :1423-1428); kinematic-BC columns stay
+        if self.kinbc is None:
+            self._ain.umask_dirichlet = None
+            self._ain.vmask_dirichlet = None
+        else:
+            self.umask[:] = 0
+            self.umask[:, self.kinbc] = 1
+            self.vmask = self.umask.copy(order="F")
+            self._ain.umask_dirichlet = self.umask.ctypes.data
+            self._ain.vmask_dirichlet = self.vmask.ctypes.data
         self.set_state(state)
 
     def set_state(self, state: dict | None = None):

@@ -468,6 +468,9 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     umask = np.zeros(g.vec_shape, np.int32, order="F")
     if p.whichbabc == 6:
         umask[nz - 1, :, :] = 1
+    kin = getattr(p, "kinbcmask", None)       # kinematic-BC columns (:1444-1455)
+    if kin is not None:
+        umask[:, np.asarray(kin) == 1] = 1
     vmask = umask.copy(order="F")
     L.orc_dirichlet_bc_3d(nx, ny, nz, 2, ip(av), ip(umask), ip(vmask), dp(uvel), dp(vvel),
                           *[dp(a) for a in A], dp(bu), dp(bv))
@@ -636,6 +639,9 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     umask = np.zeros((nx - 1, ny - 1), np.int32, order="F")
     if p.whichbabc == 6 and approx != "diva":     # DIVA: no-slip through beta_eff (:1423-1428)
         umask[:, :] = 1
+    kin = getattr(p, "kinbcmask", None)       # kinematic-BC columns (:1444-1455)
+    if kin is not None:
+        umask[np.asarray(kin) == 1] = 1
     vmask = umask.copy(order="F")
     L.orc_dirichlet_bc_2d(nx, ny, 2, ip(av), ip(umask), ip(vmask), dp(u2), dp(v2), *[dp(a) for a in A],
                           dp(bu), dp(bv))

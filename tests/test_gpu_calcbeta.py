@@ -230,3 +230,41 @@ def test_device_beta_follows_the_velocity(require_gpu):
     geo = po.geometry(p)
     ref = po.calcbeta(p, geo, r.uvel[p.grid.nz - 1], r.vvel[p.grid.nz - 1])
     np.testing.assert_allclose(got, ref, rtol=1e-13, atol=0.0)
+
+
+@pytest.mark.parametrize("approx", ["bp", "ssa", "l1l2", "diva"])
+@pytest.mark.parametrize("kind", ["raymond", "schoof"])
+def test_yield_stress_stream_matches_oracle_and_analytic(require_gpu, kind, approx):
+    """The reference's ice-stream test (tests/stream): yield-stress till
+    (which_ho_babc 15) recomputed on the device every iteration, a kinematic
+    column, doubly periodic.  Device Picard = oracle Picard (iteration count,
+    velocities 1e-7) and within 1.5 % of the Raymond / Schoof closed form."""
+    p = synthetic.yield_stress_stream(kind)
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=300)
+    if approx == "bp":
+        ref = po.picard(p, algo="chrongear", precond=1, **kw)
+        r = BPSolver(p).picard(algo=3, precond=1, **kw)
+    else:
+        ref = po.picard(p, two_d=True, approx=approx, precond=1, **kw)
+        r = {"ssa": SSASolver, "l1l2": L1L2Solver, "diva": DIVASolver}[approx](p).picard(precond=1, **kw)
+    _compare(r, ref)
+    yv = p.meta["y_vertex"]
+    ua = synthetic.stream_velocity(kind, yv)
+    g = p.grid
+    u = (r.uvel3d[0] if getattr(r, "uvel3d", None) is not None else (r.uvel[0] if approx == "bp" else r.uvel))
+    u = u[g.ilo - 1 + 9]
+    inside = np.abs(yv) < synthetic.STREAM_HALF_WIDTH
+    assert np.max(np.abs(u[inside] - ua[inside])) / ua.max() < 1.5e-2
+
+
+def test_yield_stress_stream_mixed_inner_solves(require_gpu):
+    """The BP stream with FP32 inner solves inside FP64 refinement reaches the
+    oracle's FP64 Picard velocities (1e-6) within one Picard iteration."""
+    from cism_amd.pcg import MIXED_FP32
+    p = synthetic.yield_stress_stream("raymond")
+    kw = dict(linear_tol=1e-10, linear_maxiters=3000, resid_target=1e-4, maxiter=300)
+    ref = po.picard(p, algo="chrongear", precond=1, **kw)
+    r = BPSolver(p).picard(algo=3, precond=1, precision=MIXED_FP32, inner_tol=1e-6, **kw)
+    assert r.converged and abs(r.iterations - ref["iterations"]) <= 1
+    num = np.linalg.norm(np.stack([r.uvel - ref["uvel"], r.vvel - ref["vvel"]]))
+    assert num / np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]])) < 1e-6
