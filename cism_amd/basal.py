@@ -29,7 +29,7 @@ HO_BABC_BETA_LARGE = 4
 HO_BABC_BETA_EXTERNAL = 5
 HO_BABC_NO_SLIP = 6
 HO_BABC_ZOET_IVERSON = 7
-HO_BABC_ISHOMC = 8
+HO_BABC_ISHOMC = 8            # ISMIP-HOM experiment C's prescribed field
 HO_BABC_POWERLAW = 9
 HO_BABC_COULOMB_FRICTION = 10
 HO_BABC_SCHOOF = 11
@@ -37,7 +37,7 @@ HO_BABC_MODIFIED_SCHOOF = 12
 HO_BABC_TSAI = 13
 HO_BABC_POWERLAW_EFFECPRESS = 14
 HO_BABC_YIELD_PICARD = 15
-DEVICE_OPTIONS = (0, 1, 2, 3, 4, 5, 7, 9, 10, 11, 12, 13, 14, 15)
+DEVICE_OPTIONS = (0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15)
 
 CISMGPU_ERR_BETA = 6
 
@@ -45,7 +45,8 @@ CISMGPU_ERR_BETA = 6
 PARAMS = ("ho_beta_const", "ho_beta_small", "ho_beta_large", "pseudo_plastic_q", "pseudo_plastic_u0",
           "pseudo_plastic_phimin", "pseudo_plastic_phimax", "pseudo_plastic_bedmin", "pseudo_plastic_bedmax",
           "eus", "powerlaw_m", "zoet_iverson_ut", "zoet_iverson_nmax", "coulomb_bump_max_slope",
-          "coulomb_bump_wavelength", "friction_powerlaw_k", "beta_grounded_min", "beta_powerlaw_umax")
+          "coulomb_bump_wavelength", "friction_powerlaw_k", "beta_grounded_min", "beta_powerlaw_umax",
+          "dew", "dns")
 VERTEX_FIELDS = ("effecpress_stag", "coulomb_c", "powerlaw_c", "mintauf", "c_space_factor_stag")
 
 
@@ -82,9 +83,10 @@ class Friction:
     beta_external: np.ndarray | None = None
     flwa_basal: np.ndarray | None = None
 
-    def params(self, eus: float) -> np.ndarray:
-        """The scalar parameters in the C-ABI order (eus from the problem)."""
-        return np.array([eus if k == "eus" else getattr(self, k) for k in PARAMS], dtype=np.float64)
+    def params(self, eus: float, dew: float = 0.0, dns: float = 0.0) -> np.ndarray:
+        """The scalar parameters in the C-ABI order (eus and the grid spacing from the problem)."""
+        grid = {"eus": eus, "dew": dew, "dns": dns}
+        return np.array([grid[k] if k in grid else getattr(self, k) for k in PARAMS], dtype=np.float64)
 
 
 _P = ctypes.c_void_p
@@ -112,8 +114,9 @@ def _f64(a):
     return None if a is None else np.asfortranarray(np.array(a, dtype=np.float64))
 
 
-def inputs(friction: Friction, *, f_ground, topg, thck, eus, flwa_basal):
-    """(CalcbetaIn, arrays kept alive for it) for one friction and geometry."""
+def inputs(friction: Friction, *, f_ground, topg, thck, eus, flwa_basal, dew=0.0, dns=0.0):
+    """(CalcbetaIn, arrays kept alive for it) for one friction and geometry
+    (dew, dns: the grid spacing, read by ISMIP-HOM C)."""
     keep = dict(f_ground=_f64(f_ground), topg=_f64(topg), thck=_f64(thck),
                 flwa_basal=_f64(friction.flwa_basal if friction.flwa_basal is not None else flwa_basal))
     for k in VERTEX_FIELDS + ("beta_external",):
@@ -122,7 +125,7 @@ def inputs(friction: Friction, *, f_ground, topg, thck, eus, flwa_basal):
                          else np.asfortranarray(np.array(friction.bpmp_mask, dtype=np.int32)))
     ptr = lambda a: None if a is None else a.ctypes.data
     cin = CalcbetaIn(friction.whichbabc, friction.which_beta_limit,
-                     *[float(x) for x in friction.params(eus)],
+                     *[float(x) for x in friction.params(eus, dew, dns)],
                      ptr(keep["f_ground"]), *[ptr(keep[k]) for k in VERTEX_FIELDS],
                      ptr(keep["bpmp_mask"]), ptr(keep["beta_external"]), ptr(keep["topg"]), ptr(keep["thck"]),
                      ptr(keep["flwa_basal"]))
@@ -131,7 +134,7 @@ def inputs(friction: Friction, *, f_ground, topg, thck, eus, flwa_basal):
 
 def device_setup(system, friction: Friction, **geometry):
     """Upload the basal physics of ``friction`` to ``system`` (cismgpu_system_calcbeta_setup);
-    geometry: f_ground, topg, thck, eus, flwa_basal (see :func:`inputs`)."""
+    geometry: f_ground, topg, thck, eus, flwa_basal, dew, dns (see :func:`inputs`)."""
     L = _lib.load()
     _declare(L)
     cin, keep = inputs(friction, **geometry)

@@ -1,10 +1,11 @@
 // calcbeta.hip -- basal traction coefficient beta at the vertices from the
 // basal velocity: glissade_calcbeta (glissade_basal_traction.F90:69-719)
-// for every which_ho_babc option but no-slip (6) and ISMIP-HOM C (8), the
+// for every which_ho_babc option but no-slip (6), the
 // f_ground weighting with which_ho_beta_limit (:650-685) and the beta >= 0
-// check (:687-699).  One thread per vertex; the field is read once and
-// written once (HBM-bound, a few hundred microseconds at 1024 x 1024), so
-// the Picard loop recomputes beta without a host round trip.
+// check (:687-699).  One thread per vertex; the fields are read once and
+// beta written once (about 40 us at 1027 x 1027 for the Schoof law, bound
+// by its FP64 pow calls), so the Picard loop recomputes beta without a host
+// round trip.
 //
 // Roundings: contraction is off and every expression is evaluated in the
 // reference's left-to-right order; pow and tan are the device library's
@@ -17,7 +18,7 @@ namespace cg {
 enum {
     CB_BETA_CONST, CB_BETA_SMALL, CB_BETA_LARGE, CB_PP_Q, CB_PP_U0, CB_PHIMIN, CB_PHIMAX, CB_BEDMIN, CB_BEDMAX,
     CB_EUS, CB_M, CB_ZI_UT, CB_ZI_NMAX, CB_BUMP_SLOPE, CB_BUMP_WAVELENGTH, CB_POWERLAW_K, CB_BETA_GMIN, CB_UMAX,
-    CB_NPARAM
+    CB_DEW, CB_DNS, CB_NPARAM
 };
 
 struct CBDev {
@@ -30,6 +31,27 @@ struct CBDev {
 
 // glimmer_physcon.F90:57
 constexpr double CB_PI = 3.14159265358979;
+
+// ISMIP-HOM C (:343-397) on one task (two halo cells): the global field
+// 1000 + 1000 sin(w dew ew) sin(w dns ns), w = 2 pi / (global_ewn dew), on
+// the owned cells (ew = i - 2, ns = j - 2), parallel_halo's cell rule
+// outside them (periodic: wrap; outflow, no-ice: zero); vertex (i, j) takes
+// cell (i, j).
+__device__ __forceinline__ double ishomc_at(const Dims& d, double dew, double dns, int i, int j) {
+#pragma clang fp contract(off)
+    const int n = d.nx - 4, m = d.ny - 4;
+    if (d.bc == 0 || d.bc == 2) {
+        if (i <= 2) i += n; else if (i >= d.nx - 1) i -= n;
+        if (j <= 2) j += m; else if (j >= d.ny - 1) j -= m;
+    } else {
+        const int w = d.bc == 1 ? 2 : 3;
+        if (i <= w || i >= d.nx - w + 1 || j <= w || j >= d.ny - w + 1) return 0.0;
+    }
+    const double ldomain = n * dew;
+    const double omega = 2.0 * CB_PI / ldomain;
+    const double x = dew * (i - 2), y = dns * (j - 2);
+    return 1000.0 + 1000.0 * sin(omega * x) * sin(omega * y);
+}
 
 // beta at every vertex t = (i-1) + nvx (j-1).  Velocity: level lev of a
 // system vector (vidx), or a vertex array when lev < 0.  bad: smallest
@@ -87,6 +109,9 @@ __global__ void k_calcbeta(Dims d, CBDev c, const double* __restrict__ u, const 
         b = c.cc[t] * nc * pow(speed, 1.0 / m - 1.0) / pow(speed + c.p[CB_ZI_UT], 1.0 / m);
         break;
     }
+    case 8:   // ISMIP-HOM C (:343-397)
+        b = ishomc_at(d, c.p[CB_DEW], c.p[CB_DNS], i, j);
+        break;
     case 9:   // power law (:415-433)
         b = c.pc[t] * pow(speed, 1.0 / m - 1.0);
         break;

@@ -2009,9 +2009,13 @@ static const char* calcbeta_missing(const cismgpu_calcbeta_in* in) {
 extern "C" int32_t cismgpu_system_calcbeta_setup(cismgpu_system* S, const cismgpu_calcbeta_in* in) {
     if (!S || !in) { cismgpu_set_error("calcbeta_setup: null argument"); return CISMGPU_ERR_ARG; }
     const int b = in->whichbabc;
-    if (b < 0 || b > 15 || b == 6 || b == 8) {
-        cismgpu_set_error("calcbeta_setup: which_ho_babc %d is not evaluated on the device (6 no-slip is a "
-                          "Dirichlet condition, 8 ISMIP-HOM C a prescribed field)", b);
+    if (b < 0 || b > 15 || b == 6) {
+        cismgpu_set_error("calcbeta_setup: which_ho_babc %d is not a sliding law (6 no-slip is a Dirichlet "
+                          "condition)", b);
+        return CISMGPU_ERR_ARG;
+    }
+    if (b == 8 && (!(in->dew > 0.0) || !(in->dns > 0.0))) {
+        cismgpu_set_error("calcbeta_setup: ISMIP-HOM C (8) needs the grid spacing dew, dns");
         return CISMGPU_ERR_ARG;
     }
     if (in->which_beta_limit != 0 && in->which_beta_limit != 1) {
@@ -2051,7 +2055,7 @@ extern "C" int32_t cismgpu_system_calcbeta_setup(cismgpu_system* S, const cismgp
                                    in->pseudo_plastic_bedmin, in->pseudo_plastic_bedmax, in->eus, in->powerlaw_m,
                                    in->zoet_iverson_ut, in->zoet_iverson_nmax, in->coulomb_bump_max_slope,
                                    in->coulomb_bump_wavelength, in->friction_powerlaw_k, in->beta_grounded_min,
-                                   in->beta_powerlaw_umax};
+                                   in->beta_powerlaw_umax, in->dew, in->dns};
     for (int q = 0; q < CB_NPARAM; ++q) c.p[q] = prm[q];
     hipError_t herr = hipSuccess;
     auto put = [&](const double* h, long long n) -> const double* {

@@ -383,6 +383,51 @@ def yield_stress_stream(kind: str = "raymond", n_stream: int = 25, nz: int = 2, 
     return p
 
 
+def ismip_hom(experiment: str = "c", size_km: float = 80.0, n: int = 40, nz: int = 11,
+              flwa: float = 1.0e-16) -> Problem:
+    """ISMIP-HOM experiment A or C (Pattyn et al. 2008) as the reference sets
+    it up (tests/ismip-hom/runISMIP_HOM.py, ismip-hom.config): n x n cells
+    of size_km / n, doubly periodic, the surface sloping along x (0.5 deg
+    for A, 0.1 deg for C) and continuing through the halos (the reference's
+    periodic_offset_ew).  A: bed bumps, thickness 1000 - 500 sin(wx) sin(wy)
+    under the surface 4000 - x tan(a), frozen bed (which_ho_babc 4);
+    C: thickness 1000 on the bed 1000 - x tan(a), basal friction
+    1000 + 1000 sin(wx) sin(wy) built on the velocity grid by calcbeta
+    (which_ho_babc 8).  beta_grounded_min 0, velocity guess zero."""
+    from .basal import Friction
+    if experiment not in ("a", "c"):
+        raise ValueError("experiment must be 'a' or 'c'")
+    size = size_km * 1000.0
+    dx = size / n
+    g = Grid.from_owned(n, n, nz, global_bc=BC_PERIODIC)
+    x = (np.arange(1, g.nx + 1) - 3 + 0.5) * dx          # cell centres, owned cell 3 at dx / 2
+    y = (np.arange(1, g.ny + 1) - 3 + 0.5) * dx
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    w = 2.0 * np.pi / size
+    if experiment == "a":
+        alpha = 0.5 * np.pi / 180.0
+        zz = 4000.0 - X * np.tan(alpha)
+        thck = 1000.0 - 500.0 * np.sin(w * X) * np.sin(w * Y)
+        topg = zz - thck
+        whichbabc = HO_BABC_BETA_LARGE
+    else:
+        alpha = 0.1 * np.pi / 180.0
+        thck = np.full((g.nx, g.ny), 1000.0)
+        topg = 1000.0 - X * np.tan(alpha)
+        whichbabc = HO_BABC_BETA_EXTERNAL
+    p = Problem(
+        grid=g, dx=dx, dy=dx, sigma=glide_sigma(nz),
+        thck=np.asfortranarray(thck), topg=np.asfortranarray(topg),
+        flwa=np.full((nz - 1, g.nx, g.ny), flwa, order="F"),
+        beta=np.zeros((g.nvx, g.nvy), order="F"),
+        uvel=g.zeros_vec(), vvel=g.zeros_vec(), whichbabc=whichbabc,
+        meta=dict(workload="ismip_hom", experiment=experiment, size_km=size_km, n=n, nz=nz),
+    )
+    if experiment == "c":
+        p.friction = Friction(8, beta_grounded_min=0.0)
+    return p
+
+
 def depth_integrated_system(Auu, Auv, Avu, Avv, bu, bv):
     """2-D (9-point) system P^T A P of an assembled 3-D system, with P x(i,j) =
     x on every level of column (i,j): block stencil entry (i,j; m2) = the sum

@@ -372,8 +372,8 @@ typedef struct cismgpu_calcbeta_in {
     int32_t whichbabc;               /* which_ho_babc (glide_types.F90:253-268): 0 constant, 1 bpmp,
                                         2 pseudo-plastic, 3 old pseudo-plastic, 4 large, 5 external,
                                         7 Zoet-Iverson, 9 power law, 10 Coulomb friction, 11 Schoof,
-                                        12 modified Schoof, 13 Tsai, 14 power law with N, 15 yield
-                                        stress; 6 (no-slip) and 8 (ISMIP-HOM C) are refused */
+                                        8 ISMIP-HOM C, 12 modified Schoof, 13 Tsai, 14 power law
+                                        with N, 15 yield stress; 6 (no-slip) is refused */
     int32_t which_beta_limit;        /* which_ho_beta_limit (:270-271): 0 multiply by f_ground, then
                                         floor at beta_grounded_min; 1 floor, then multiply */
     double ho_beta_const, ho_beta_small, ho_beta_large;                  /* Pa yr/m */
@@ -387,6 +387,7 @@ typedef struct cismgpu_calcbeta_in {
     double friction_powerlaw_k;                                         /* m yr^-1 Pa^-2 */
     double beta_grounded_min;                                           /* Pa yr/m */
     double beta_powerlaw_umax;                                          /* m/yr; 0: no speed cap */
+    double dew, dns;                                                    /* m: grid spacing, option 8 */
     const double*  f_ground;             /* (nx-1, ny-1) grounded fraction, always read */
     const double*  effecpress_stag;      /* (nx-1, ny-1) effective pressure N: options 2, 3, 7, 10-14 */
     const double*  coulomb_c;            /* (nx-1, ny-1): options 2, 7, 10-13 */

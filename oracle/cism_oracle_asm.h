@@ -135,9 +135,10 @@ int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, 
 enum {
     ORC_CB_BETA_CONST, ORC_CB_BETA_SMALL, ORC_CB_BETA_LARGE, ORC_CB_PP_Q, ORC_CB_PP_U0, ORC_CB_PHIMIN,
     ORC_CB_PHIMAX, ORC_CB_BEDMIN, ORC_CB_BEDMAX, ORC_CB_EUS, ORC_CB_M, ORC_CB_ZI_UT, ORC_CB_ZI_NMAX,
-    ORC_CB_BUMP_SLOPE, ORC_CB_BUMP_WAVELENGTH, ORC_CB_POWERLAW_K, ORC_CB_BETA_GMIN, ORC_CB_UMAX, ORC_CB_NPARAM
+    ORC_CB_BUMP_SLOPE, ORC_CB_BUMP_WAVELENGTH, ORC_CB_POWERLAW_K, ORC_CB_BETA_GMIN, ORC_CB_UMAX, ORC_CB_DEW,
+    ORC_CB_DNS, ORC_CB_NPARAM
 };
-int orc_calcbeta(int nx, int ny, int whichbabc, int which_beta_limit, const double* p,
+int orc_calcbeta(int nx, int ny, int global_bc, int whichbabc, int which_beta_limit, const double* p,
                  const double* uvel, const double* vvel, const double* effecpress_stag, const double* coulomb_c,
                  const double* powerlaw_c, const double* mintauf, const double* c_space_factor_stag,
                  const int32_t* bpmp_mask, const double* beta_external, const double* f_ground, const double* topg,

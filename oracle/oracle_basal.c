@@ -2,8 +2,7 @@
  * oracle_basal.c -- CPU restatement of glissade_calcbeta
  * (libglissade/glissade_basal_traction.F90:69-719): the basal traction
  * coefficient beta at the vertices from the basal velocity, for every
- * which_ho_babc option except ISMIP-HOM C (:343-397, which scatters a
- * global field) and no-slip (:644, beta untouched), then the f_ground
+ * which_ho_babc option but no-slip (:644, beta untouched), then the f_ground
  * weighting with which_ho_beta_limit (:650-685), the beta >= 0 check
  * (:687-699); the staggered halo that follows (:703) is orc_halo3d with
  * one level.
@@ -46,7 +45,28 @@ static double speed_at(double u, double v, double umax)
  * negative or NaN (the reference stops there with GM_FATAL, :690-697),
  * or -1 for an option it does not cover.
  */
-int orc_calcbeta(int nx, int ny, int whichbabc, int which_beta_limit, const double* p,
+/* ISMIP-HOM C (:343-397) on one task (two halo cells): the global field
+ * 1000 + 1000 sin(w dew ew) sin(w dns ns), w = 2 pi / (global_ewn dew),
+ * placed on the owned cells (ew = i - 2, ns = j - 2), the cell halo rule
+ * of parallel_halo (periodic: wrap; outflow / no-ice: zero), and each
+ * vertex (i, j) takes cell (i, j). */
+static double ishomc_at(int nx, int ny, int global_bc, double dew, double dns, int i, int j)
+{
+    const int n = nx - 4, m = ny - 4;
+    if (global_bc == 0 || global_bc == 2) {
+        if (i <= 2) i += n; else if (i >= nx - 1) i -= n;
+        if (j <= 2) j += m; else if (j >= ny - 1) j -= m;
+    } else {
+        const int w = global_bc == 1 ? 2 : 3;
+        if (i <= w || i >= nx - w + 1 || j <= w || j >= ny - w + 1) return 0.0;
+    }
+    const double ldomain = n * dew;
+    const double omega = 2.0 * ORC_PI / ldomain;
+    const double x = dew * (i - 2), y = dns * (j - 2);
+    return 1000.0 + 1000.0 * sin(omega * x) * sin(omega * y);
+}
+
+int orc_calcbeta(int nx, int ny, int global_bc, int whichbabc, int which_beta_limit, const double* p,
                  const double* uvel, const double* vvel, const double* effecpress_stag, const double* coulomb_c,
                  const double* powerlaw_c, const double* mintauf, const double* c_space_factor_stag,
                  const int32_t* bpmp_mask, const double* beta_external, const double* f_ground, const double* topg,
@@ -100,6 +120,9 @@ int orc_calcbeta(int nx, int ny, int whichbabc, int which_beta_limit, const doub
                 b = tau_c * pow(speed, 1.0 / m - 1.0) / pow(speed + p[ORC_CB_ZI_UT], 1.0 / m);
                 break;
             }
+            case 8:   /* ISMIP-HOM C (:343-397) */
+                b = ishomc_at(nx, ny, global_bc, p[ORC_CB_DEW], p[ORC_CB_DNS], i, j);
+                break;
             case 9:   /* power law (:415-433) */
                 b = powerlaw_c[v] * pow(speed, 1.0 / m - 1.0);
                 break;

@@ -104,7 +104,7 @@ def _declare(L):
     L.orc_slope_angle_xy.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_slope_angle_staggered.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_cell_halo.argtypes = [i, i, i, _D]
-    L.orc_calcbeta.argtypes = [i, i, i, i, _D] + [_D] * 7 + [_I] + [_D] * 5
+    L.orc_calcbeta.argtypes = [i, i, i, i, i, _D] + [_D] * 7 + [_I] + [_D] * 5
     L.orc_calcbeta.restype = ctypes.c_int
     L.orc_assemble_stiffness_matrix_2d_diva.argtypes = [i, i, i, _D, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D,
                                                         _D, _D, i, d, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
@@ -332,7 +332,7 @@ def calcbeta(problem, geo, ubas, vbas):
     fr = problem.friction
     g = problem.grid
     f64 = lambda a: None if a is None else np.asfortranarray(np.array(a, dtype=np.float64))
-    prm = np.ascontiguousarray(fr.params(problem.eus))
+    prm = np.ascontiguousarray(fr.params(problem.eus, getattr(problem, "dx", 0.0), getattr(problem, "dy", 0.0)))
     flwa_basal = f64(fr.flwa_basal if fr.flwa_basal is not None else np.asarray(problem.flwa)[g.nz - 2])
     bpmp = None if fr.bpmp_mask is None else np.asfortranarray(np.array(fr.bpmp_mask, dtype=np.int32))
     flds = [f64(getattr(fr, k)) for k in ("effecpress_stag", "coulomb_c", "powerlaw_c", "mintauf",
@@ -341,7 +341,7 @@ def calcbeta(problem, geo, ubas, vbas):
     u, v = f64(ubas), f64(vbas)
     b = np.zeros((g.nx - 1, g.ny - 1), order="F")
     opt = lambda a: None if a is None else dp(a)
-    st = lib().orc_calcbeta(g.nx, g.ny, fr.whichbabc, fr.which_beta_limit, dp(prm), dp(u), dp(v),
+    st = lib().orc_calcbeta(g.nx, g.ny, g.global_bc, fr.whichbabc, fr.which_beta_limit, dp(prm), dp(u), dp(v),
                             *[opt(a) for a in flds], ip(bpmp) if bpmp is not None else None, opt(bext), dp(fg),
                             opt(topg), opt(thck), opt(flwa_basal), dp(b))
     if st < 0:

@@ -57,16 +57,17 @@ def _velocity(g, seed):
 
 
 def _oracle(g, fr, f, eus, ub, vb):
-    p = SimpleNamespace(grid=g, friction=fr, eus=eus, topg=f["topg"], thck=f["thck"], flwa=f["flwa"])
+    p = SimpleNamespace(grid=g, friction=fr, eus=eus, topg=f["topg"], thck=f["thck"], flwa=f["flwa"], dx=DX, dy=DY)
     return po.calcbeta(p, {"f_ground": f["fg"]}, ub, vb)
 
 
 def _device(sys3, g, fr, f, eus, **kw):
     basal.device_setup(sys3, fr, f_ground=f["fg"], topg=f["topg"], thck=f["thck"], eus=eus,
-                       flwa_basal=f["flwa"][g.nz - 2])
+                       flwa_basal=f["flwa"][g.nz - 2], dew=DX, dns=DY)
     return basal.device_calcbeta(sys3, want=True, **kw)
 
 
+DX, DY = 1500.0, 1500.0          # grid spacing (read by ISMIP-HOM C)
 GRIDS = {"noice": Grid.from_owned(40, 36, 5, global_bc=BC_NO_ICE),
          "periodic": Grid.from_owned(33, 30, 4, global_bc=BC_PERIODIC)}
 CASES = [(opt, limit, gname) for opt in basal.DEVICE_OPTIONS for limit in (0, 1) for gname in GRIDS
@@ -268,3 +269,28 @@ def test_yield_stress_stream_mixed_inner_solves(require_gpu):
     assert r.converged and abs(r.iterations - ref["iterations"]) <= 1
     num = np.linalg.norm(np.stack([r.uvel - ref["uvel"], r.vvel - ref["vvel"]]))
     assert num / np.linalg.norm(np.stack([ref["uvel"], ref["vvel"]])) < 1e-6
+
+
+@pytest.mark.parametrize("experiment,size_km", [("a", 80.0), ("c", 80.0), ("c", 20.0)])
+def test_ismip_hom_matches_oracle(require_gpu, experiment, size_km):
+    """ISMIP-HOM A and C as the reference runs them (tests/ismip-hom; C's
+    friction built on the velocity grid by calcbeta option 8 on the device):
+    device Picard = oracle Picard (iteration count, velocities 1e-7), and the
+    device solution's basal drag balances the driving stress (the mean of
+    beta u_b / cos(theta_b) = rho g mean(H) tan(alpha), 1e-6)."""
+    p = synthetic.ismip_hom(experiment, size_km, n=20, nz=7)
+    kw = dict(linear_tol=1e-10, linear_maxiters=2000, resid_target=1e-5, maxiter=100)
+    ref = po.picard(p, algo="chrongear", precond=2, **kw)
+    s = BPSolver(p)
+    r = s.picard(algo=3, precond=2, **kw)
+    _compare(r, ref)
+    g = p.grid
+    geo = po.geometry(p)
+    beta = s.calcbeta(want=True) if experiment == "c" else s.beta_internal
+    if experiment == "c":      # the device field of the returned velocity = the oracle's
+        np.testing.assert_allclose(beta, po.calcbeta(p, geo, r.uvel[-1], r.vvel[-1]), rtol=1e-13, atol=0.0)
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    alpha = (0.5 if experiment == "a" else 0.1) * np.pi / 180.0
+    taud = 910.0 * 9.81 * geo["stagthck"][own].mean() * np.tan(alpha)
+    c = np.cos(geo["theta_basal"])
+    assert (beta * r.uvel[-1] / c)[own].mean() == pytest.approx(taud, rel=1e-6)

@@ -220,3 +220,50 @@ def test_staggered_halo_is_applied():
     g0 = Grid.from_owned(8, 6, 3, global_bc=BC_NO_ICE)
     _, b = _case(Friction(9, powerlaw_c=pc, beta_grounded_min=0.0), u, v, grid=g0)
     assert (b[:2] == 0).all() and (b[:, :2] == 0).all()
+
+
+def test_ismip_hom_c_field():
+    """Option 8 on a periodic 8 x 8-cell task: the field of ISMIP-HOM C at
+    the vertices, 1000 + 1000 sin(w x) sin(w y) with x, y the vertex
+    positions: mean 1000 over a period, 2000 and 0 at the quarter points,
+    odd about 1000 under a half-period shift, periodic halo."""
+    g = Grid.from_owned(8, 8, 3, global_bc=BC_PERIODIC)
+    u, v = _speeds(g, 20)
+    p = SimpleNamespace(grid=g, friction=Friction(8, beta_grounded_min=0.0), eus=0.0, dx=2500.0, dy=2500.0,
+                        topg=np.zeros((g.nx, g.ny)), thck=np.full((g.nx, g.ny), 1000.0),
+                        flwa=np.full((g.nz - 1, g.nx, g.ny), 1e-16))
+    b = po.calcbeta(p, {"f_ground": np.ones((g.nvx, g.nvy))}, u, v)
+    own = b[g.ilo - 1:g.ihi, g.jlo - 1:g.jhi]            # vertex (i, j) at x = (i - 2) dx: quarter points at 2, 6
+    assert own.mean() == pytest.approx(1000.0, rel=1e-14)
+    assert own[1, 1] == pytest.approx(2000.0, rel=1e-14) and own[5, 5] == pytest.approx(2000.0, rel=1e-14)
+    assert abs(own[1, 5]) < 1e-10 and abs(own[5, 1]) < 1e-10
+    np.testing.assert_allclose(np.roll(own, 4, axis=0) - 1000.0, -(own - 1000.0), atol=1e-10)
+    w = np.asfortranarray(b.copy())
+    po.halo(g, w)
+    assert np.array_equal(b, w)
+
+
+@pytest.mark.parametrize("experiment", ["a", "c"])
+def test_ismip_hom_force_balance(experiment):
+    """ISMIP-HOM A and C (tests/ismip-hom): on a doubly periodic domain the
+    stiffness of a uniform velocity is zero, so at a converged Picard step
+    the basal drag balances the driving stress exactly: the mean over the
+    owned vertices of beta u_b / cos(theta_b) (the local basal term is
+    taken over the sloping bed's area) equals rho g mean(H) tan(alpha) to
+    the Picard tolerance, and the mean of beta v_b / cos(theta_b)
+    vanishes."""
+    from cism_amd import synthetic as sy
+    p = sy.ismip_hom(experiment, 80.0, n=20, nz=7)
+    r = po.picard(p, algo="chrongear", precond=2, linear_tol=1e-10, linear_maxiters=2000, resid_target=1e-6,
+                  maxiter=100)
+    assert r["converged"]
+    g = p.grid
+    geo = po.geometry(p)
+    ub, vb = r["uvel"][-1], r["vvel"][-1]
+    beta = po.beta_internal(p, geo, ub, vb)
+    own = (slice(g.ilo - 1, g.ihi), slice(g.jlo - 1, g.jhi))
+    alpha = (0.5 if experiment == "a" else 0.1) * np.pi / 180.0
+    taud = 910.0 * 9.81 * geo["stagthck"][own].mean() * np.tan(alpha)
+    c = np.cos(geo["theta_basal"])
+    assert (beta * ub / c)[own].mean() == pytest.approx(taud, rel=1e-8)
+    assert abs((beta * vb / c)[own].mean()) < 1e-9 * taud
