@@ -161,13 +161,18 @@ def ice_stream(n_vert: int, nz: int = 11, dx: float = 1000.0, seed: int = 0,
 
 
 def slab(n_cells: int = 8, nz: int = 21, dx: float = 1000.0, thck: float = 1000.0,
-         theta_deg: float = 0.5, beta: float = 2000.0, flwa: float = 1.0e-16) -> Problem:
+         theta_deg: float = 0.5, beta: float = 2000.0, flwa: float = 1.0e-16,
+         global_bc: int = BC_PERIODIC) -> Problem:
     """Doubly periodic inclined slab (tests/slab): uniform thickness on a bed
     tilted by ``theta`` in x, linear sliding with uniform beta.  Periodic BCs
     wrap the vertex grid, so the surface is shifted by the slab drop across
     one period through the cell elevations only (the solver sees the slope
-    through the surface gradient and the node z coordinates)."""
-    g = Grid.from_owned(n_cells, n_cells, nz, global_bc=BC_PERIODIC)
+    through the surface gradient and the node z coordinates).  global_bc 2
+    (no-penetration) keeps the periodic halos and holds u = 0 on the
+    east/west boundary column and v = 0 on the north/south boundary row."""
+    if global_bc not in (0, 2):
+        raise ValueError("slab: global_bc 0 (periodic) or 2 (no-penetration)")
+    g = Grid.from_owned(n_cells, n_cells, nz, global_bc=global_bc)
     xc = (np.arange(1, g.nx + 1) - 2.5) * dx
     slope = np.tan(np.deg2rad(theta_deg))
     topg = np.asfortranarray(np.repeat((-slope * xc)[:, None], g.ny, axis=1) + 5000.0)

@@ -163,7 +163,11 @@ class BPSolver:
         if self.kinbc is not None:
             self.umask[:, self.kinbc] = 1
         self.vmask = self.umask.copy(order="F")
-        has_mask = bool(self.umask.any())
+        self.nopen_u, self.nopen_v = self._no_penetration()
+        if self.nopen_u is not None:
+            self.umask[:, self.nopen_u] = 1
+            self.vmask[:, self.nopen_v] = 1
+        has_mask = bool(self.umask.any() or self.vmask.any())
         # lateral BC at calving fronts: staggered thickness/surface over ice-covered or
         # land cells (glissade_velo_higher.F90:1510-1515, 2006-2016)
         self.thck = np.asfortranarray(np.array(p.thck, dtype=np.float64))
@@ -190,6 +194,33 @@ class BPSolver:
         if self.friction is not None:
             self._ain.beta_internal = None
             self._ain.beta_device = 1
+
+    def _no_penetration(self):
+        """No-penetration global BC (global_bc 2): u = 0 on the vertex column
+        of the east/west global boundary and v = 0 on the row of the
+        north/south one, i.e. the last owned column/row and its periodic
+        image in the halo (staggered_no_penetration_mask, parallel_mpi.F90:
+        10187-10231, applied at glissade_velo_higher.F90:1461-1471).
+        Boolean (nx-1, ny-1) masks for u and v, or (None, None)."""
+        g = self.grid
+        if g.global_bc != 2:
+            return None, None
+        mu = np.zeros((g.nvx, g.nvy), bool, order="F")
+        mv = np.zeros((g.nvx, g.nvy), bool, order="F")
+        mu[[1, g.nvx - 2], :] = True
+        mv[:, [1, g.nvy - 2]] = True
+        return mu, mv
+
+    def _apply_no_penetration(self, u, v):
+        """Zero u and v where the no-penetration BC prescribes them (3-D or 2-D arrays)."""
+        if self.nopen_u is None:
+            return
+        if u.ndim == 3:
+            u[:, self.nopen_u] = 0.0
+            v[:, self.nopen_v] = 0.0
+        else:
+            u[self.nopen_u] = 0.0
+            v[self.nopen_v] = 0.0
 
     def _kinbc_columns(self):
         """Boolean (nx-1, ny-1) of the kinematic-BC columns (kinbcmask = 1), or None."""
@@ -240,6 +271,7 @@ class BPSolver:
             u = np.asfortranarray(np.array(uvel, dtype=np.float64))
             v = np.asfortranarray(np.array(vvel, dtype=np.float64))
             self.zero_inactive(u, v)
+            self._apply_no_penetration(u, v)
             self.sys.set_x(u, v)
         if self.friction is not None:
             basal.device_calcbeta(self.sys)
@@ -297,6 +329,7 @@ class BPSolver:
         (:3652-3661)."""
         u, v = self._initial_velocity()
         self.zero_inactive(u, v)
+        self._apply_no_penetration(u, v)
         self.sys.set_x(u, v)
         _lib.check(self._L.cismgpu_system_work_vectors(self.sys.handle, 4), "work_vectors")
         X, USAV, UOLD, DU, UBEST = self.X, self.USAV, self.UOLD, self.DU, self.UBEST
@@ -419,6 +452,7 @@ class SSASolver(BPSolver):
             u = np.asfortranarray(np.array(uvel, dtype=np.float64))
             v = np.asfortranarray(np.array(vvel, dtype=np.float64))
             self.zero_inactive(u, v)
+            self._apply_no_penetration(u, v)
             self.sys.set_x(u, v)
         if self.friction is not None:
             basal.device_calcbeta(self.sys)
@@ -497,13 +531,17 @@ class DIVASolver(SSASolver):
         if self.friction is None:
             self._ain.beta_internal = self.beta_internal.ctypes.data
         # no Dirichlet rows for no-slip (beta_eff carries it, :1423-1428); kinematic-BC columns stay
-        if self.kinbc is None:
+        if self.kinbc is None and self.nopen_u is None:
             self._ain.umask_dirichlet = None
             self._ain.vmask_dirichlet = None
         else:
             self.umask[:] = 0
-            self.umask[:, self.kinbc] = 1
+            if self.kinbc is not None:
+                self.umask[:, self.kinbc] = 1
             self.vmask = self.umask.copy(order="F")
+            if self.nopen_u is not None:
+                self.umask[:, self.nopen_u] = 1
+                self.vmask[:, self.nopen_v] = 1
             self._ain.umask_dirichlet = self.umask.ctypes.data
             self._ain.vmask_dirichlet = self.vmask.ctypes.data
         self.set_state(state)

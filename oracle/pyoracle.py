@@ -323,6 +323,24 @@ def geometry(problem):
                 f_ground=fground, stagthck_lateral=stagthck_lat, stagusrf_lateral=stagusrf_lat)
 
 
+def no_penetration_masks(grid):
+    """staggered_no_penetration_mask (parallel_mpi.F90:10187-10231) on one
+    task: u mask 1 on vertex columns local_ewn-uhalo.. and ..lhalo, v mask
+    on the matching rows, then the staggered halo of both; (u, v) int32
+    (nx-1, ny-1) masks, zero unless global_bc is 2."""
+    g = grid
+    mu = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    mv = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    if g.global_bc == 2:
+        mu[g.nx - 2 - 1:, :] = 1.0        # umask(local_ewn-uhalo:, :) with uhalo = 2
+        mu[:2, :] = 1.0                   # umask(:lhalo, :)
+        mv[:, g.ny - 2 - 1:] = 1.0
+        mv[:, :2] = 1.0
+        halo(g, mu)
+        halo(g, mv)
+    return mu.astype(np.int32), mv.astype(np.int32)
+
+
 def calcbeta(problem, geo, ubas, vbas):
     """glissade_calcbeta (glissade_basal_traction.F90:69-719) for the
     problem's friction law (cism_amd.basal.Friction) from the basal velocity
@@ -472,6 +490,9 @@ def assemble(problem, uvel=None, vvel=None, geo=None):
     if kin is not None:
         umask[:, np.asarray(kin) == 1] = 1
     vmask = umask.copy(order="F")
+    npu, npv = no_penetration_masks(g)        # (:1461-1471)
+    umask[:, npu == 1] = 1
+    vmask[:, npv == 1] = 1
     L.orc_dirichlet_bc_3d(nx, ny, nz, 2, ip(av), ip(umask), ip(vmask), dp(uvel), dp(vvel),
                           *[dp(a) for a in A], dp(bu), dp(bv))
     for a in A:
@@ -643,6 +664,9 @@ def assemble_2d(problem, uvel_2d=None, vvel_2d=None, geo=None, approx="ssa", sta
     if kin is not None:
         umask[np.asarray(kin) == 1] = 1
     vmask = umask.copy(order="F")
+    npu, npv = no_penetration_masks(g)        # (:1461-1471)
+    umask[npu == 1] = 1
+    vmask[npv == 1] = 1
     L.orc_dirichlet_bc_2d(nx, ny, 2, ip(av), ip(umask), ip(vmask), dp(u2), dp(v2), *[dp(a) for a in A],
                           dp(bu), dp(bv))
     for a in A:
@@ -778,6 +802,9 @@ def picard(problem, *, maxiter=100, resid_target=1e-4, whichresid=0, algo="chron
         inact = geo["active_vertex"][own[1], own[2]] == 0
         for a in (u, v):
             o = a[own]; o[:, inact] = 0.0; a[own] = o
+    npu, npv = no_penetration_masks(g)         # u, v = 0 on the no-penetration walls (:1461-1471)
+    u[..., npu == 1] = 0.0
+    v[..., npv == 1] = 0.0
     hist = dict(L2=[], resid_velo=[], linear_iters=[], linear_err=[], alpha=[])
     L2_large, gamma, alpha_max, thresh = 1.0e20, 0.40, 3.0, 0.85        # (:831, :1049-1056)
     if accelerate:
