@@ -12,7 +12,6 @@ compute_residual_velocity_3d (:10314-10446) and compute_residual_vector_3d
 from __future__ import annotations
 
 import ctypes
-import time
 from dataclasses import dataclass, field
 
 import numpy as np
