@@ -238,6 +238,158 @@ def grounded_fraction(ice_mask, floating_mask, land_mask):
     return np.asfortranarray(np.where(vmask & anyg, 1.0, 0.0))
 
 
+RHOO = 1028.0   # ocean density (glimmer_physcon.F90:68)
+
+
+def flotation_function(thck, topg, eus, ice_mask, land_mask, which=2, global_bc=3):
+    """Flotation function at the cells (glissade_grounding_line.F90:237-392):
+    0 Pattyn (-rho_o (b - eus) / (rho_i H) - 1), 1 inverse Pattyn
+    (1 - rho_i H / (-rho_o (b - eus)), at least -10), both -10 on land and 0
+    in ice-free ocean, which then takes the smallest value of its ice or land
+    edge neighbours (else corner neighbours) and the cell halo; 2 linear
+    (-(b - eus) - rho_i/rho_o H, at least 1e-4 in magnitude; on land
+    -max(b - eus, 50)).  Positive where the ice floats."""
+    H = np.asarray(thck, dtype=np.float64)
+    d = np.asarray(topg, dtype=np.float64) - eus
+    ice, land = ice_mask == 1, land_mask == 1
+    if which in (0, 1):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if which == 0:
+                fi = (-RHOO * d) / (RHOI * H) - 1.0
+            else:
+                fi = np.maximum(1.0 - (RHOI * H) / (-RHOO * d), -10.0)
+        f = np.where(land, -10.0, np.where(ice, fi, 0.0))
+        # ice-free ocean cells off the outermost ring: the smallest value of
+        # their ice or land edge neighbours, else of their corner neighbours
+        has = ice | land
+        fin = np.where(has, f, np.inf)
+        inner = (slice(1, -1), slice(1, -1))
+        sh = lambda a, b: fin[1 + a:fin.shape[0] - 1 + a, 1 + b:fin.shape[1] - 1 + b]
+        emin = np.minimum.reduce([sh(0, -1), sh(-1, 0), sh(1, 0), sh(0, 1)])
+        cmin = np.minimum.reduce([sh(-1, -1), sh(1, -1), sh(-1, 1), sh(1, 1)])
+        fill = np.where(np.isfinite(emin), emin, cmin)
+        ex = f.copy()
+        ex[inner] = np.where(~has[inner] & np.isfinite(fill), fill, f[inner])
+        _cell_halo(ex, global_bc)
+        return np.asfortranarray(ex)
+    fl = -d - (RHOI / RHOO) * H
+    fl = np.where(np.abs(fl) < 1.0e-4, np.where(fl < 0.0, -1.0e-4, 1.0e-4), fl)
+    return np.asfortranarray(np.where(land, -np.maximum(d, 50.0), fl))
+
+
+def _cell_halo(a, global_bc):
+    """parallel_halo of a cell field on one task with two halo cells:
+    periodic and no-penetration wrap, outflow and no-ice zero."""
+    nx, ny = a.shape
+    if global_bc in (0, 2):
+        a[0, 2:ny - 2] = a[nx - 4, 2:ny - 2]; a[1, 2:ny - 2] = a[nx - 3, 2:ny - 2]
+        a[nx - 2, 2:ny - 2] = a[2, 2:ny - 2]; a[nx - 1, 2:ny - 2] = a[3, 2:ny - 2]
+        a[:, 0] = a[:, ny - 4]; a[:, 1] = a[:, ny - 3]
+        a[:, ny - 2] = a[:, 2]; a[:, ny - 1] = a[:, 3]
+    else:
+        w = 2 if global_bc == 1 else 3
+        a[:w, :] = 0.0; a[nx - w:, :] = 0.0; a[:, :w] = 0.0; a[:, ny - w:] = 0.0
+
+
+def grounded_area(f1, f2, f3, f4):
+    """compute_grounded_fraction (glissade_grounding_line.F90:678-1080),
+    vectorised: the grounded (f <= 0) area of the unit square under the
+    bilinear interpolant of corner values f1 (SW), f2 (SE), f3 (NE), f4 (NW),
+    by the reference's closed forms for one, two adjacent, two opposite and
+    three floating corners.  NaN where the reference stops (an opposite-
+    corner pattern with bc - ad < 0 or (b + d)(c + d) < 0)."""
+    F = np.stack([np.asarray(x, dtype=np.float64) for x in (f1, f2, f3, f4)])
+    cf = F > 0.0
+    nf = cf.sum(axis=0)
+    out = np.zeros(F.shape[1:])
+    out[nf == 0] = 1.0
+    zero = ((nf == 4) | (F.min(axis=0) == 0.0)) & (nf != 0)     # no floating corner: grounded first
+
+    def roll(r):
+        return F[r % 4], F[(r + 1) % 4], F[(r + 2) % 4], F[(r + 3) % 4]
+
+    def pick(rot):
+        g = [np.zeros_like(F[0]) for _ in range(4)]
+        for r in range(4):
+            m = rot == r
+            for q, v in enumerate(roll(r)):
+                g[q] = np.where(m, v, g[q])
+        return g
+
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # one or three floating corners: the lone corner first
+        lone = np.where(nf == 1, cf, ~cf)
+        rot1 = np.where(lone[0], 0, np.where(lone[1], 1, np.where(lone[2], 2, 3)))
+        a, f2r, f3r, f4r = pick(rot1)
+        b, c, d = f2r - a, f4r - a, a + f3r - f2r - f4r
+        t = (a * d) / (b * c)
+        corner = np.where(np.abs(t) > 1.0e-6, ((b * c - a * d) * np.log(np.abs(1.0 - t)) + a * d) / (d * d),
+                          (a * a) / (2.0 * b * c))
+        m13 = ((nf == 1) | (nf == 3)) & ~zero
+        out = np.where(m13, np.where(nf == 1, 1.0 - corner, corner), out)
+        # two floating corners
+        adj = (cf[0] & cf[1]) | (cf[1] & cf[2]) | (cf[2] & cf[3]) | (cf[3] & cf[0])
+        rot2 = np.where(cf[0] & cf[1], 0, np.where(cf[1] & cf[2], 1, np.where(cf[2] & cf[3], 2, 3)))
+        var = F[1] * F[3] - F[0] * F[2]
+        rotated = ~adj & (var < 0.0)
+        rot2 = np.where(adj, rot2, np.where(rotated, 1, 0))
+        a, f2r, f3r, f4r = pick(rot2)
+        b, c, d = f2r - a, f4r - a, a + f3r - f2r - f4r
+        trap = np.where(np.abs(d / c) > 1.0e-6, ((b * c - a * d) * np.log(1.0 + d / c) - b * d) / (d * d),
+                        -(2.0 * a + b) / (2.0 * c))
+        bcad = b * c - a * d
+        usual = np.abs(bcad) > 1.0e-6
+        c1 = np.where(usual, (bcad * np.log(1.0 - (a * d) / (b * c)) + a * d) / (d * d), a * a / (b * c))
+        c2 = np.where(usual, (bcad * np.log(bcad / ((b + d) * (c + d))) + d * (a + b + c + d)) / (d * d),
+                      (a + b) * (a + c) / (b * c))
+        grounded_corners = np.where(F[0] > 0.0, rotated, ~rotated)
+        opp = np.where(grounded_corners, c1 + c2, 1.0 - (c1 + c2))
+        bad = ~adj & ((bcad < 0.0) | ((b + d) * (c + d) < 0.0))
+        opp = np.where(bad, np.nan, opp)
+        m2 = (nf == 2) & ~zero
+        out = np.where(m2, np.where(adj, 1.0 - trap, opp), out)
+    out[zero] = 0.0
+    return out
+
+
+def grounded_fraction_options(thck, topg, eus, ice_mask, floating_mask, land_mask, which_ground=0,
+                              which_flotation=2, which_no_glp=0, global_bc=3):
+    """f_ground at the vertices for which_ho_ground 0 (no GLP: the
+    grounded-neighbour rule, or with which_ho_fground_no_glp = 1 the sign of
+    the staggered flotation function) and 1 (GLP for basal friction: the
+    grounded area of the bilinear flotation function over the staggered
+    cell) (glissade_grounding_line.F90:404-520).  Returns (f_ground,
+    f_flotation)."""
+    if which_ground not in (0, 1):
+        raise ValueError("which_ho_ground 0 or 1")
+    ff = flotation_function(thck, topg, eus, ice_mask, land_mask, which_flotation, global_bc)
+    ice = ice_mask == 1
+    land = land_mask == 1
+    vmask = (ice[:-1, 1:] | ice[1:, 1:] | ice[:-1, :-1] | ice[1:, :-1] |
+             land[:-1, 1:] | land[1:, 1:] | land[:-1, :-1])
+    if which_ground == 0 and which_no_glp == 0:
+        return grounded_fraction(ice_mask, floating_mask, land_mask), ff
+    if which_ground == 0:
+        stag = (ff[:-1, 1:] + ff[1:, 1:] + ff[:-1, :-1] + ff[1:, :-1]) / 4.0
+        fg = np.where(vmask & (stag <= 0.0), 1.0, 0.0)
+    else:
+        fg = np.where(vmask, grounded_area(ff[:-1, :-1], ff[1:, :-1], ff[1:, 1:], ff[:-1, 1:]), 0.0)
+        if np.isnan(fg).any():
+            raise ValueError("grounded fraction: a flotation pattern the reference stops on")
+    return np.asfortranarray(fg), ff
+
+
+def problem_f_ground(problem, ice_mask, floating_mask, land_mask):
+    """f_ground for a problem's which_ho_ground, which_ho_flotation_function
+    and which_ho_fground_no_glp (defaults 0, 2, 0: the grounded-neighbour
+    rule)."""
+    p = problem
+    fg, _ = grounded_fraction_options(p.thck, p.topg, p.eus, ice_mask, floating_mask, land_mask,
+                                      getattr(p, "which_ho_ground", 0), getattr(p, "which_ho_flotation_function", 2),
+                                      getattr(p, "which_ho_fground_no_glp", 0), p.grid.global_bc)
+    return fg
+
+
 def stagger_masked(var, mask):
     """glissade_stagger with stagger_margin = 1: mean over the cells whose mask
     is 1, zero where none is (glissade_grid_operators.F90:135-146)."""
@@ -251,8 +403,8 @@ def stagger_masked(var, mask):
 
 def beta_internal(problem, ice_mask, floating_mask, land_mask):
     """glissade_calcbeta for constant / large / external beta (glissade_basal_traction.F90:
-    202, 304, 399) times f_ground (:650-672, beta_grounded_min = 0), then its
-    staggered halo."""
+    202, 304, 399) times f_ground (:650-672, beta_grounded_min = 0; f_ground
+    by the problem's grounding-line options), then its staggered halo."""
     g = problem.grid
     if problem.whichbabc == 0:
         b = np.full((g.nvx, g.nvy), problem.beta_const)
@@ -260,6 +412,6 @@ def beta_internal(problem, ice_mask, floating_mask, land_mask):
         b = np.full((g.nvx, g.nvy), 1.0e10)
     else:
         b = np.array(problem.beta, dtype=np.float64)
-    b = np.asfortranarray(b * grounded_fraction(ice_mask, floating_mask, land_mask))
+    b = np.asfortranarray(b * problem_f_ground(problem, ice_mask, floating_mask, land_mask))
     staggered_halo_2d(g, b)                    # (:697)
     return b

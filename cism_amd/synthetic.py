@@ -51,6 +51,9 @@ class Problem:
     assemble_beta: int = 1       # which_ho_assemble_beta (glide_types.F90:359-360, 1050): 0 standard, 1 local
     assemble_taud: int = 1       # which_ho_assemble_taud (glide_types.F90:362-363, 1042): 0 standard, 1 local
     kinbcmask: np.ndarray | None = None   # (nx-1, ny-1) int: 1 = u, v prescribed (from uvel/vvel) in the column
+    which_ho_ground: int = 0     # glide_types.F90:378-380, 1090: 0 no GLP, 1 GLP for basal friction
+    which_ho_flotation_function: int = 2   # :389-392, 1114: 0 Pattyn, 1 inverse Pattyn, 2 linear
+    which_ho_fground_no_glp: int = 0       # :382-383, 1099: 0 grounded neighbour, 1 flotation function
     friction: object = None      # basal.Friction: beta from the basal velocity in every Picard iteration
                                  # (glissade_calcbeta); None: the fixed beta of whichbabc 0, 4, 5 or no-slip
     meta: dict = field(default_factory=dict)

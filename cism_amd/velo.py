@@ -145,7 +145,7 @@ class BPSolver:
                 raise ValueError("friction: a no-slip problem (which_ho_babc 6) has no basal friction term")
             if self.friction.whichbabc not in basal.DEVICE_OPTIONS:
                 raise ValueError(f"friction: which_ho_babc {self.friction.whichbabc} is not a device sliding law")
-            fground = ge.grounded_fraction(self.ice_mask, self.floating_mask, self.land_mask)
+            fground = ge.problem_f_ground(p, self.ice_mask, self.floating_mask, self.land_mask)
             basal.device_setup(self.sys, self.friction, f_ground=fground, topg=p.topg, thck=p.thck, eus=p.eus,
                                flwa_basal=np.asarray(p.flwa)[g.nz - 2], dew=p.dx, dns=p.dy)
         self.xV, self.yV = p.vertex_coords()

@@ -144,6 +144,15 @@ int orc_calcbeta(int nx, int ny, int global_bc, int whichbabc, int which_beta_li
                  const int32_t* bpmp_mask, const double* beta_external, const double* f_ground, const double* topg,
                  const double* thck, const double* flwa_basal, double* beta);
 
+/* glissade_grounded_fraction (oracle_basal.c) */
+int orc_compute_grounded_fraction(const double* f_flotation4, double* f_ground);
+void orc_flotation_function(int nx, int ny, int global_bc, int which, const double* thck, const double* topg,
+                            double eus, const int32_t* ice_mask, const int32_t* land_mask, double* f_flotation);
+int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int which_flotation, int which_no_glp,
+                          const double* thck, const double* topg, double eus, const int32_t* ice_mask,
+                          const int32_t* floating_mask, const int32_t* land_mask, double* f_flotation,
+                          double* f_ground);
+
 #ifdef __cplusplus
 }
 #endif

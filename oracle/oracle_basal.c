@@ -17,6 +17,7 @@
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "cism_oracle_asm.h"
 
@@ -24,8 +25,9 @@
 #define C2(i, j) ((size_t)((i) - 1) + (size_t)nx * (size_t)((j) - 1))
 #define V2(i, j) ((size_t)((i) - 1) + (size_t)NVX * (size_t)((j) - 1))
 
-/* glimmer_physcon.F90:57 */
+/* glimmer_physcon.F90:57, 67-68 */
 static const double ORC_PI = 3.14159265358979;
+static const double RHOI_B = 910.0, RHOO_B = 1028.0;
 
 /* speed with the floor smallnum = 1e-2 m/yr (:180), capped at
  * beta_powerlaw_umax when that is positive (:187-189) */
@@ -194,5 +196,169 @@ int orc_calcbeta(int nx, int ny, int global_bc, int whichbabc, int which_beta_li
         }
     for (size_t v = 0; v < nv; ++v)
         if (!(beta[v] >= 0.0)) return (int)v + 1;
+    return 0;
+}
+
+/*
+ * Grounded fraction at the vertices (glissade_grounded_fraction,
+ * glissade_grounding_line.F90:59-674) for which_ho_ground 0 (no GLP: the
+ * grounded-neighbour rule or the sign of the staggered flotation function,
+ * which_ho_fground_no_glp 0/1) and 1 (GLP for basal friction: the grounded
+ * area of the bilinear interpolant of the flotation function over the
+ * staggered cell, compute_grounded_fraction :678-1080), with the flotation
+ * functions 0 (Pattyn), 1 (inverse Pattyn) and 2 (linear) and, for 0 and 1,
+ * their extrapolation to ice-free ocean cells (:237-392).
+ */
+
+/* the grounded area of the unit square where the bilinear interpolant of
+ * the corner values f[0..3] (SW, SE, NE, NW) is <= 0 (:678-1080); returns
+ * 0, or -1 for the two patterns the reference stops on */
+int orc_compute_grounded_fraction(const double* fq, double* f_ground)
+{
+    const double eps06 = 1.0e-6;
+    int cfloat[4], nfloat = 0;
+    double fmin = fq[0];
+    for (int n = 0; n < 4; ++n) {
+        cfloat[n] = fq[n] > 0.0;
+        nfloat += cfloat[n];
+        if (fq[n] < fmin) fmin = fq[n];
+    }
+    if (nfloat == 0) { *f_ground = 1.0; return 0; }
+    if (nfloat == 4 || fmin == 0.0) { *f_ground = 0.0; return 0; }
+    double f1, f2, f3, f4;
+    /* the corner values rolled so that corner r (0-based) comes first */
+#define ROLL(r) do { f1 = fq[(r) % 4]; f2 = fq[((r) + 1) % 4]; f3 = fq[((r) + 2) % 4]; f4 = fq[((r) + 3) % 4]; } while (0)
+    if (nfloat == 1 || nfloat == 3) {
+        int lv[4];
+        for (int n = 0; n < 4; ++n) lv[n] = nfloat == 1 ? cfloat[n] : !cfloat[n];
+        if (lv[0]) ROLL(0); else if (lv[1]) ROLL(1); else if (lv[2]) ROLL(2); else ROLL(3);
+        const double a = f1, b = f2 - f1, c = f4 - f1, d = f1 + f3 - f2 - f4;
+        double f_corner;
+        if (fabs((a * d) / (b * c)) > eps06) f_corner = ((b * c - a * d) * log(fabs(1.0 - (a * d) / (b * c))) + a * d) / (d * d);
+        else f_corner = (a * a) / (2.0 * b * c);
+        *f_ground = nfloat == 1 ? 1.0 - f_corner : f_corner;
+        return 0;
+    }
+    /* nfloat == 2 */
+    int adjacent = 1, rotated = 0;
+    if (cfloat[0] && cfloat[1]) ROLL(0);
+    else if (cfloat[1] && cfloat[2]) ROLL(1);
+    else if (cfloat[2] && cfloat[3]) ROLL(2);
+    else if (cfloat[3] && cfloat[0]) ROLL(3);
+    else {
+        adjacent = 0;
+        const double var = fq[1] * fq[3] - fq[0] * fq[2];
+        if (var >= 0.0) ROLL(0);
+        else { ROLL(1); rotated = 1; }
+    }
+#undef ROLL
+    const double a = f1, b = f2 - f1, c = f4 - f1, d = f1 + f3 - f2 - f4;
+    if (adjacent) {
+        double f_trap;
+        if (fabs(d / c) > eps06) f_trap = ((b * c - a * d) * log(1.0 + d / c) - b * d) / (d * d);
+        else f_trap = -(2.0 * a + b) / (2.0 * c);
+        *f_ground = 1.0 - f_trap;
+        return 0;
+    }
+    if (b * c - a * d < 0.0 || (b + d) * (c + d) < 0.0) return -1;
+    double c1, c2;
+    if (fabs(b * c - a * d) > eps06) {
+        c1 = ((b * c - a * d) * log(1.0 - (a * d) / (b * c)) + a * d) / (d * d);
+        c2 = ((b * c - a * d) * log((b * c - a * d) / ((b + d) * (c + d))) + d * (a + b + c + d)) / (d * d);
+    } else {
+        c1 = a * a / (b * c);
+        c2 = (a + b) * (a + c) / (b * c);
+    }
+    const int corners_grounded = (fq[0] > 0.0) ? rotated : !rotated;
+    *f_ground = corners_grounded ? c1 + c2 : 1.0 - (c1 + c2);
+    return 0;
+}
+
+/* flotation function at the cells (:237-392); ice-free ocean cells of
+ * options 0 and 1 take the smallest value of their ice or land edge
+ * neighbours (else corner neighbours), then the cell halo */
+void orc_flotation_function(int nx, int ny, int global_bc, int which, const double* thck, const double* topg,
+                            double eus, const int32_t* ice_mask, const int32_t* land_mask, double* ff)
+{
+    const double land_topg_min = 50.0, marine_min = 1.0e-4, land_pattyn = -10.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const size_t c = C2(i, j);
+            double f;
+            if (which == 0 || which == 1) {
+                if (land_mask[c] == 1) f = land_pattyn;
+                else if (ice_mask[c] == 1) {
+                    if (which == 0) f = -RHOO_B * (topg[c] - eus) / (RHOI_B * thck[c]) - 1.0;
+                    else {
+                        f = 1.0 - RHOI_B * thck[c] / (-RHOO_B * (topg[c] - eus));
+                        if (f < land_pattyn) f = land_pattyn;
+                    }
+                } else f = 0.0;
+            } else {
+                if (land_mask[c] == 1) {
+                    const double dd = (topg[c] - eus) > land_topg_min ? (topg[c] - eus) : land_topg_min;
+                    f = -dd;
+                } else {
+                    f = -(topg[c] - eus) - (RHOI_B / RHOO_B) * thck[c];
+                    if (fabs(f) < marine_min) f = f < 0.0 ? -marine_min : marine_min;
+                }
+            }
+            ff[c] = f;
+        }
+    if (which != 0 && which != 1) return;
+    double* ex = (double*)malloc(sizeof(double) * (size_t)nx * (size_t)ny);
+    for (size_t c = 0; c < (size_t)nx * (size_t)ny; ++c) ex[c] = ff[c];
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            if (ice_mask[C2(i, j)] != 0 || land_mask[C2(i, j)] != 0) continue;
+            int filled = 0;
+            for (int pass = 0; pass < 2 && !filled; ++pass)        /* edges, then corners */
+                for (int jj = j - 1; jj <= j + 1; ++jj)
+                    for (int ii = i - 1; ii <= i + 1; ++ii) {
+                        const int edge = (ii == i || jj == j), corner = (abs(ii - i) == 1 && abs(jj - j) == 1);
+                        if (!(pass == 0 ? edge : corner)) continue;
+                        if (!(ice_mask[C2(ii, jj)] == 1 || land_mask[C2(ii, jj)] == 1)) continue;
+                        const double v = ff[C2(ii, jj)];
+                        if (!filled) { filled = 1; ex[C2(i, j)] = v; }
+                        else if (v < ex[C2(i, j)]) ex[C2(i, j)] = v;
+                    }
+        }
+    orc_cell_halo(nx, ny, global_bc, ex);
+    for (size_t c = 0; c < (size_t)nx * (size_t)ny; ++c) ff[c] = ex[c];
+    free(ex);
+}
+
+/* f_ground at the vertices (:404-674, which_ho_ground 0 and 1); returns 0,
+ * -1 for an unsupported option, or 1 + the vertex index of a pattern the
+ * reference stops on */
+int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int which_flotation, int which_no_glp,
+                          const double* thck, const double* topg, double eus, const int32_t* ice_mask,
+                          const int32_t* floating_mask, const int32_t* land_mask, double* ff, double* f_ground)
+{
+    if (which_ground < 0 || which_ground > 1 || which_flotation < 0 || which_flotation > 2) return -1;
+    orc_flotation_function(nx, ny, global_bc, which_flotation, thck, topg, eus, ice_mask, land_mask, ff);
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            /* vertex mask: the reference tests land cell (i+1, j+1) twice (:218-229) */
+            const int vmask = ice_mask[C2(i, j + 1)] == 1 || ice_mask[C2(i + 1, j + 1)] == 1 ||
+                              ice_mask[C2(i, j)] == 1 || ice_mask[C2(i + 1, j)] == 1 ||
+                              land_mask[C2(i, j + 1)] == 1 || land_mask[C2(i + 1, j + 1)] == 1 ||
+                              land_mask[C2(i, j)] == 1 || land_mask[C2(i + 1, j + 1)] == 1;
+            double f = 0.0;
+            if (vmask) {
+                if (which_ground == 0 && which_no_glp == 1) {
+                    const double s = (ff[C2(i, j + 1)] + ff[C2(i + 1, j + 1)] + ff[C2(i, j)] + ff[C2(i + 1, j)]) / 4.0;
+                    f = s <= 0.0 ? 1.0 : 0.0;
+                } else if (which_ground == 0) {
+#define CGR(a, b) ((ice_mask[C2(a, b)] == 1 && floating_mask[C2(a, b)] == 0) || land_mask[C2(a, b)] == 1)
+                    f = (CGR(i, j + 1) || CGR(i + 1, j + 1) || CGR(i, j) || CGR(i + 1, j)) ? 1.0 : 0.0;
+#undef CGR
+                } else {
+                    const double fq[4] = {ff[C2(i, j)], ff[C2(i + 1, j)], ff[C2(i + 1, j + 1)], ff[C2(i, j + 1)]};
+                    if (orc_compute_grounded_fraction(fq, &f)) return 1 + (int)V2(i, j);
+                }
+            }
+            f_ground[V2(i, j)] = f;
+        }
     return 0;
 }

@@ -105,6 +105,10 @@ def _declare(L):
     L.orc_slope_angle_staggered.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_cell_halo.argtypes = [i, i, i, _D]
     L.orc_calcbeta.argtypes = [i, i, i, i, i, _D] + [_D] * 7 + [_I] + [_D] * 5
+    L.orc_compute_grounded_fraction.argtypes = [_D, _D]
+    L.orc_compute_grounded_fraction.restype = ctypes.c_int
+    L.orc_grounded_fraction.argtypes = [i, i, i, i, i, i, _D, _D, ctypes.c_double, _I, _I, _I, _D, _D]
+    L.orc_grounded_fraction.restype = ctypes.c_int
     L.orc_calcbeta.restype = ctypes.c_int
     L.orc_assemble_stiffness_matrix_2d_diva.argtypes = [i, i, i, _D, _D, i, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D,
                                                         _D, _D, i, d, d, d, _D, _D, _D, _D, _D, _D, _D, _D]
@@ -308,8 +312,7 @@ def geometry(problem):
     L.orc_stagger(nx, ny, dp(lsrf), dp(staglsrf))
     theta = np.zeros((nx - 1, ny - 1), order="F")
     L.orc_slope_angle(nx - 1, ny - 1, p.dx, p.dy, dp(staglsrf), dp(theta))
-    fground = np.zeros((nx - 1, ny - 1), order="F")
-    L.orc_grounded_fraction_no_glp(nx, ny, ip(ice), ip(flt), ip(land), dp(fground))
+    fground = grounded_fraction(p, ice, flt, land)
     # thickness and surface at vertices for the standard lateral BC: mean over
     # ice-covered or land cells (glissade_velo_higher.F90:1510-1515, 2006-2016)
     ipl = np.asfortranarray(((ice == 1) | (land == 1)).astype(np.int32))
@@ -321,6 +324,40 @@ def geometry(problem):
                 stagthck=stagthck, stagusrf=stagusrf, dusrf_dx=dsdx, dusrf_dy=dsdy,
                 active_cell=acell, active_vertex=avert, flwafact=flwafact, theta_basal=theta,
                 f_ground=fground, stagthck_lateral=stagthck_lat, stagusrf_lateral=stagusrf_lat)
+
+
+def grounded_fraction(problem, ice, flt, land):
+    """f_ground at the vertices for the problem's which_ho_ground (0, 1),
+    which_ho_flotation_function (0-2) and which_ho_fground_no_glp (0, 1)
+    (glissade_grounded_fraction, glissade_grounding_line.F90:59-674); the
+    defaults (0, 2, 0) are orc_grounded_fraction_no_glp."""
+    p = problem
+    g = p.grid
+    wg = getattr(p, "which_ho_ground", 0)
+    wf = getattr(p, "which_ho_flotation_function", 2)
+    wn = getattr(p, "which_ho_fground_no_glp", 0)
+    fground = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    if wg == 0 and wn == 0:
+        lib().orc_grounded_fraction_no_glp(g.nx, g.ny, ip(ice), ip(flt), ip(land), dp(fground))
+        return fground
+    ff = np.zeros((g.nx, g.ny), order="F")
+    f64 = lambda a: np.asfortranarray(np.array(a, dtype=np.float64))
+    st = lib().orc_grounded_fraction(g.nx, g.ny, g.global_bc, wg, wf, wn, dp(f64(p.thck)), dp(f64(p.topg)), p.eus,
+                                     ip(ice), ip(flt), ip(land), dp(ff), dp(fground))
+    if st:
+        raise ValueError(f"grounded fraction: status {st}")
+    return fground
+
+
+def compute_grounded_fraction(f4):
+    """compute_grounded_fraction (glissade_grounding_line.F90:678-1080) of the
+    corner values (SW, SE, NE, NW) of one staggered cell."""
+    f = np.ascontiguousarray(f4, dtype=np.float64)
+    out = np.zeros(1)
+    st = lib().orc_compute_grounded_fraction(dp(f), dp(out))
+    if st:
+        raise ValueError("compute_grounded_fraction: a pattern the reference stops on")
+    return float(out[0])
 
 
 def no_penetration_masks(grid):

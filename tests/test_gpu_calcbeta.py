@@ -294,3 +294,32 @@ def test_ismip_hom_matches_oracle(require_gpu, experiment, size_km):
     taud = 910.0 * 9.81 * geo["stagthck"][own].mean() * np.tan(alpha)
     c = np.cos(geo["theta_basal"])
     assert (beta * r.uvel[-1] / c)[own].mean() == pytest.approx(taud, rel=1e-6)
+
+
+def _marine_glp(flotation, friction=False):
+    p = synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3, h_max=1000.0, h_shelf=200.0)
+    p.which_ho_ground = 1                       # GLP for basal friction (glissade_grounding_line.F90:487-519)
+    p.which_ho_flotation_function = flotation
+    if friction:
+        g = p.grid
+        N = np.asfortranarray(910.0 * 9.81 * synthetic_stag(p.thck) * 0.2)
+        p.friction = Friction(11, powerlaw_c=np.full((g.nvx, g.nvy), 2.0e4), coulomb_c=np.full((g.nvx, g.nvy), 0.5),
+                              effecpress_stag=N)
+    return p
+
+
+@pytest.mark.parametrize("flotation,friction", [(2, False), (0, False), (2, True)])
+def test_marine_picard_with_grounding_line_parameterisation(require_gpu, flotation, friction):
+    """Marine sheet with the GLP (fractional f_ground at the grounding line,
+    host geometry = oracle to 1e-13): accelerated BP Picard (fixed beta, or
+    the Schoof law recomputed on the device) = the oracle's (iteration count,
+    alpha ladder, velocities 1e-7)."""
+    p = _marine_glp(flotation, friction)
+    kw = dict(linear_tol=1e-10, resid_target=1e-4, maxiter=80, accelerate=True)
+    ref = po.picard(p, algo="chrongear", precond=2, linear_maxiters=2000, **kw)
+    s = BPSolver(p)
+    r = s.picard(algo=3, precond=2, linear_maxiters=2000, **kw)
+    _compare(r, ref)
+    assert r.alpha == ref["alpha"]
+    fg = po.geometry(p)["f_ground"]
+    assert ((fg > 0) & (fg < 1)).sum() > 20     # the parameterisation is active

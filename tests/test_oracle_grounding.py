@@ -1,0 +1,101 @@
+"""Grounding-line parameterisation (glissade_grounded_fraction,
+glissade_grounding_line.F90:59-674).
+
+The oracle's compute_grounded_fraction (the reference's closed forms for
+the grounded area of the bilinear flotation function over a staggered cell)
+is pinned by direct integration: for each x the grounded length in y of
+f = (a + b x) + (c + d x) y is exact, and a 200,000-point midpoint rule in
+x gives the area to better than 1e-7.  Every pattern class is drawn (one,
+two adjacent, two opposite in both orientations, three floating corners).
+The flotation functions are pinned at the flotation thickness, on land and
+in the ice-free ocean, and the product's numpy version of the whole
+f_ground equals the oracle's on marine sheets for every option.  CPU only."""
+import numpy as np
+import pytest
+
+from cism_amd import geometry as ge, synthetic as sy
+from oracle import pyoracle as po
+
+
+def _area(f):
+    """grounded (f <= 0) area of the unit square under the bilinear interpolant
+    of corner values SW, SE, NE, NW, integrated in x with the exact length in y"""
+    a, b, c, d = f[0], f[1] - f[0], f[3] - f[0], f[0] + f[2] - f[1] - f[3]
+    n = 200_000
+    x = (np.arange(n) + 0.5) / n
+    A, B = a + b * x, c + d * x
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ys = np.clip(-A / B, 0.0, 1.0)
+    length = np.where(B > 0, ys, np.where(B < 0, 1.0 - ys, (A <= 0).astype(float)))
+    return length.mean()
+
+
+def _samples(kind, n, rng):
+    out = []
+    while len(out) < n:
+        f = rng.uniform(-2.0, 2.0, 4)
+        fl = f > 0
+        k = fl.sum()
+        adj = any(fl[q] and fl[(q + 1) % 4] for q in range(4))
+        if kind == "one" and k == 1 or kind == "three" and k == 3 or \
+           kind == "adjacent" and k == 2 and adj or kind == "opposite" and k == 2 and not adj:
+            out.append(f)
+    return out
+
+
+@pytest.mark.parametrize("kind", ["one", "adjacent", "opposite", "three"])
+def test_closed_forms_match_direct_integration(kind):
+    rng = np.random.default_rng({"one": 1, "adjacent": 2, "opposite": 3, "three": 4}[kind])
+    orient = set()
+    for f in _samples(kind, 60, rng):
+        got = po.compute_grounded_fraction(f)
+        assert got == pytest.approx(_area(f), abs=1e-7), f
+        if kind == "opposite":
+            orient.add(f[1] * f[3] - f[0] * f[2] >= 0)
+    if kind == "opposite":
+        assert orient == {True, False}          # both of the reference's orientations drawn
+
+
+def test_simple_patterns():
+    assert po.compute_grounded_fraction([-1, -1, -1, -1]) == 1.0
+    assert po.compute_grounded_fraction([1, 1, 1, 1]) == 0.0
+    assert po.compute_grounded_fraction([-1, 1, 1, -1]) == pytest.approx(0.5)   # grounding line at x = 1/2
+    assert po.compute_grounded_fraction([-3, 1, 1, -3]) == pytest.approx(0.75)
+    assert po.compute_grounded_fraction([0.0, 1, 1, 1]) == 0.0                # a zero corner counts as floating
+
+
+def test_flotation_functions():
+    nx = ny = 8
+    thck = np.full((nx, ny), 500.0)
+    topg = np.full((nx, ny), -500.0 * 910.0 / 1028.0)          # exactly at flotation
+    topg[2, :] = 30.0                                         # a land row inside the halo ring
+    ice = np.ones((nx, ny), np.int32)
+    ice[5, 5] = 0                                             # one ice-free ocean cell
+    thck[5, 5] = 0.0
+    land = (topg >= 0).astype(np.int32)
+    f0 = ge.flotation_function(thck, topg, 0.0, ice, land, 0, global_bc=1)
+    f1 = ge.flotation_function(thck, topg, 0.0, ice, land, 1, global_bc=1)
+    f2 = ge.flotation_function(thck, topg, 0.0, ice, land, 2, global_bc=1)
+    assert np.abs(f0[3, 3]) < 1e-15 and np.abs(f1[3, 3]) < 1e-15
+    assert f2[3, 3] in (1e-4, -1e-4)                          # the linear function keeps 1e-4 from zero
+    assert f0[2, 4] == -10.0 and f2[2, 4] == -50.0            # land: -10, or -max(b - eus, 50)
+    assert f0[0, 4] == 0.0                                    # the outflow cell halo is zero
+    assert f0[5, 5] == pytest.approx(0.0, abs=1e-15)          # extrapolated from its ice neighbours
+    assert f2[5, 5] == pytest.approx(500.0 * 910.0 / 1028.0)  # linear: the ocean depth
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_product_matches_oracle_for_every_option(seed):
+    p = sy.marine_sheet(40, nz=5, dx=2000.0, seed=seed, h_max=1000.0, h_shelf=200.0)
+    g = p.grid
+    ice, flt, ocn, land = ge.get_masks(p.thck, p.topg, p.eus, p.thklim)
+    I = lambda a: np.asfortranarray(a.astype(np.int32))
+    partial = 0
+    for wf in (0, 1, 2):
+        for wg, wn in ((0, 0), (0, 1), (1, 0)):
+            p.which_ho_ground, p.which_ho_flotation_function, p.which_ho_fground_no_glp = wg, wf, wn
+            fo = po.grounded_fraction(p, I(ice), I(flt), I(land))
+            fp = ge.problem_f_ground(p, ice, flt, land)
+            np.testing.assert_allclose(fp, fo, rtol=0, atol=1e-13)
+            partial += int(((fp > 0) & (fp < 1)).sum())
+    assert partial > 300                                      # the GLP gives fractional vertices
