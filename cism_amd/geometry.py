@@ -356,12 +356,13 @@ def grounded_fraction_options(thck, topg, eus, ice_mask, floating_mask, land_mas
                               which_flotation=2, which_no_glp=0, global_bc=3):
     """f_ground at the vertices for which_ho_ground 0 (no GLP: the
     grounded-neighbour rule, or with which_ho_fground_no_glp = 1 the sign of
-    the staggered flotation function) and 1 (GLP for basal friction: the
+    the staggered flotation function), 1 (GLP for basal friction: the
     grounded area of the bilinear flotation function over the staggered
-    cell) (glissade_grounding_line.F90:404-520).  Returns (f_ground,
-    f_flotation)."""
-    if which_ground not in (0, 1):
-        raise ValueError("which_ho_ground 0 or 1")
+    cell) and 2 (the mean over the cell's four quadrants, each bilinear
+    between a cell value, two edge midpoints and the vertex)
+    (glissade_grounding_line.F90:404-618).  Returns (f_ground, f_flotation)."""
+    if which_ground not in (0, 1, 2):
+        raise ValueError("which_ho_ground 0, 1 or 2")
     ff = flotation_function(thck, topg, eus, ice_mask, land_mask, which_flotation, global_bc)
     ice = ice_mask == 1
     land = land_mask == 1
@@ -373,8 +374,20 @@ def grounded_fraction_options(thck, topg, eus, ice_mask, floating_mask, land_mas
         stag = (ff[:-1, 1:] + ff[1:, 1:] + ff[:-1, :-1] + ff[1:, :-1]) / 4.0
         fg = np.where(vmask & (stag <= 0.0), 1.0, 0.0)
     else:
-        fg = np.where(vmask, grounded_area(ff[:-1, :-1], ff[1:, :-1], ff[1:, 1:], ff[:-1, 1:]), 0.0)
-        if np.isnan(fg).any():
+        sw, se, ne, nw = ff[:-1, :-1], ff[1:, :-1], ff[1:, 1:], ff[:-1, 1:]
+        if which_ground == 1:
+            area = grounded_area(sw, se, ne, nw)
+        else:
+            # the four quadrants of the staggered cell, each bilinear between a
+            # cell value, two edge midpoints and the vertex (:521-618)
+            ctr = 0.25 * (nw + ne + sw + se)
+            q1 = grounded_area(sw, 0.5 * (sw + se), ctr, 0.5 * (sw + nw))
+            q2 = grounded_area(0.5 * (se + sw), se, 0.5 * (se + ne), ctr)
+            q3 = grounded_area(ctr, 0.5 * (ne + se), ne, 0.5 * (ne + nw))
+            q4 = grounded_area(0.5 * (nw + sw), ctr, 0.5 * (nw + ne), nw)
+            area = 0.25 * (q1 + q2 + q3 + q4)
+        fg = np.where(vmask, area, 0.0)
+        if np.isnan(fg[vmask]).any():
             raise ValueError("grounded fraction: a flotation pattern the reference stops on")
     return np.asfortranarray(fg), ff
 

@@ -203,9 +203,10 @@ int orc_calcbeta(int nx, int ny, int global_bc, int whichbabc, int which_beta_li
  * Grounded fraction at the vertices (glissade_grounded_fraction,
  * glissade_grounding_line.F90:59-674) for which_ho_ground 0 (no GLP: the
  * grounded-neighbour rule or the sign of the staggered flotation function,
- * which_ho_fground_no_glp 0/1) and 1 (GLP for basal friction: the grounded
+ * which_ho_fground_no_glp 0/1), 1 (GLP for basal friction: the grounded
  * area of the bilinear interpolant of the flotation function over the
- * staggered cell, compute_grounded_fraction :678-1080), with the flotation
+ * staggered cell, compute_grounded_fraction :678-1080) and 2 (the same
+ * over the cell's four quadrants, :521-618), with the flotation
  * functions 0 (Pattyn), 1 (inverse Pattyn) and 2 (linear) and, for 0 and 1,
  * their extrapolation to ice-free ocean cells (:237-392).
  */
@@ -328,14 +329,14 @@ void orc_flotation_function(int nx, int ny, int global_bc, int which, const doub
     free(ex);
 }
 
-/* f_ground at the vertices (:404-674, which_ho_ground 0 and 1); returns 0,
+/* f_ground at the vertices (:404-674, which_ho_ground 0, 1, 2); returns 0,
  * -1 for an unsupported option, or 1 + the vertex index of a pattern the
  * reference stops on */
 int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int which_flotation, int which_no_glp,
                           const double* thck, const double* topg, double eus, const int32_t* ice_mask,
                           const int32_t* floating_mask, const int32_t* land_mask, double* ff, double* f_ground)
 {
-    if (which_ground < 0 || which_ground > 1 || which_flotation < 0 || which_flotation > 2) return -1;
+    if (which_ground < 0 || which_ground > 2 || which_flotation < 0 || which_flotation > 2) return -1;
     orc_flotation_function(nx, ny, global_bc, which_flotation, thck, topg, eus, ice_mask, land_mask, ff);
     for (int j = 1; j <= ny - 1; ++j)
         for (int i = 1; i <= nx - 1; ++i) {
@@ -353,9 +354,22 @@ int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int w
 #define CGR(a, b) ((ice_mask[C2(a, b)] == 1 && floating_mask[C2(a, b)] == 0) || land_mask[C2(a, b)] == 1)
                     f = (CGR(i, j + 1) || CGR(i + 1, j + 1) || CGR(i, j) || CGR(i + 1, j)) ? 1.0 : 0.0;
 #undef CGR
-                } else {
+                } else if (which_ground == 1) {
                     const double fq[4] = {ff[C2(i, j)], ff[C2(i + 1, j)], ff[C2(i + 1, j + 1)], ff[C2(i, j + 1)]};
                     if (orc_compute_grounded_fraction(fq, &f)) return 1 + (int)V2(i, j);
+                } else {
+                    /* GLP deluxe (:521-618): the staggered cell's four quadrants, each
+                     * bilinear between a cell value, two edge midpoints and the vertex */
+                    const double sw = ff[C2(i, j)], se = ff[C2(i + 1, j)], ne = ff[C2(i + 1, j + 1)], nw = ff[C2(i, j + 1)];
+                    const double ctr = 0.25 * (nw + ne + sw + se);
+                    const double q[4][4] = {{sw, 0.5 * (sw + se), ctr, 0.5 * (sw + nw)},
+                                            {0.5 * (se + sw), se, 0.5 * (se + ne), ctr},
+                                            {ctr, 0.5 * (ne + se), ne, 0.5 * (ne + nw)},
+                                            {0.5 * (nw + sw), ctr, 0.5 * (nw + ne), nw}};
+                    double fgq[4];
+                    for (int r = 0; r < 4; ++r)
+                        if (orc_compute_grounded_fraction(q[r], &fgq[r])) return 1 + (int)V2(i, j);
+                    f = 0.25 * (fgq[0] + fgq[1] + fgq[2] + fgq[3]);
                 }
             }
             f_ground[V2(i, j)] = f;

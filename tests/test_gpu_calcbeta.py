@@ -296,9 +296,9 @@ def test_ismip_hom_matches_oracle(require_gpu, experiment, size_km):
     assert (beta * r.uvel[-1] / c)[own].mean() == pytest.approx(taud, rel=1e-6)
 
 
-def _marine_glp(flotation, friction=False):
+def _marine_glp(flotation, friction=False, ground=1):
     p = synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3, h_max=1000.0, h_shelf=200.0)
-    p.which_ho_ground = 1                       # GLP for basal friction (glissade_grounding_line.F90:487-519)
+    p.which_ho_ground = ground                  # GLP over the staggered cell (1) or its quadrants (2)
     p.which_ho_flotation_function = flotation
     if friction:
         g = p.grid
@@ -308,13 +308,13 @@ def _marine_glp(flotation, friction=False):
     return p
 
 
-@pytest.mark.parametrize("flotation,friction", [(2, False), (0, False), (2, True)])
-def test_marine_picard_with_grounding_line_parameterisation(require_gpu, flotation, friction):
+@pytest.mark.parametrize("flotation,friction,ground", [(2, False, 1), (0, False, 1), (2, True, 1), (2, False, 2)])
+def test_marine_picard_with_grounding_line_parameterisation(require_gpu, flotation, friction, ground):
     """Marine sheet with the GLP (fractional f_ground at the grounding line,
     host geometry = oracle to 1e-13): accelerated BP Picard (fixed beta, or
     the Schoof law recomputed on the device) = the oracle's (iteration count,
     alpha ladder, velocities 1e-7)."""
-    p = _marine_glp(flotation, friction)
+    p = _marine_glp(flotation, friction, ground)
     kw = dict(linear_tol=1e-10, resid_target=1e-4, maxiter=80, accelerate=True)
     ref = po.picard(p, algo="chrongear", precond=2, linear_maxiters=2000, **kw)
     s = BPSolver(p)

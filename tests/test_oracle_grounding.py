@@ -92,10 +92,53 @@ def test_product_matches_oracle_for_every_option(seed):
     I = lambda a: np.asfortranarray(a.astype(np.int32))
     partial = 0
     for wf in (0, 1, 2):
-        for wg, wn in ((0, 0), (0, 1), (1, 0)):
+        for wg, wn in ((0, 0), (0, 1), (1, 0), (2, 0)):
             p.which_ho_ground, p.which_ho_flotation_function, p.which_ho_fground_no_glp = wg, wf, wn
             fo = po.grounded_fraction(p, I(ice), I(flt), I(land))
             fp = ge.problem_f_ground(p, ice, flt, land)
             np.testing.assert_allclose(fp, fo, rtol=0, atol=1e-13)
             partial += int(((fp > 0) & (fp < 1)).sum())
     assert partial > 300                                      # the GLP gives fractional vertices
+
+
+def _cells_from_vertex_values(rng, linear):
+    """a small ice-covered marine block whose flotation function (linear
+    option) is either a linear field or a random one"""
+    from types import SimpleNamespace
+    from cism_amd.grid import Grid
+    g = Grid.from_owned(6, 6, 3)
+    X, Y = np.meshgrid(np.arange(g.nx, dtype=float), np.arange(g.ny, dtype=float), indexing="ij")
+    target = (0.7 * (X - 4.3) - 0.4 * (Y - 3.1)) if linear else rng.uniform(-1.0, 1.0, (g.nx, g.ny))
+    thck = np.full((g.nx, g.ny), 400.0)
+    topg = -(target + (910.0 / 1028.0) * thck)            # the linear flotation function equals target
+    return SimpleNamespace(grid=g, thck=thck, topg=topg, eus=0.0), g
+
+
+def test_quadrant_option_reduces_to_the_cell_option_for_a_linear_field():
+    """which_ho_ground 2 interpolates over four quadrants from midpoint
+    averages; for a linear flotation field those are exact, so the grounded
+    area of the staggered cell is the same as option 1's."""
+    p, g = _cells_from_vertex_values(None, True)
+    ice = np.ones((g.nx, g.ny), np.int32)
+    flt = np.ones((g.nx, g.ny), np.int32)
+    land = np.zeros((g.nx, g.ny), np.int32)
+    f1, ff = ge.grounded_fraction_options(p.thck, p.topg, 0.0, ice, flt, land, 1, 2, 0, g.global_bc)
+    f2, _ = ge.grounded_fraction_options(p.thck, p.topg, 0.0, ice, flt, land, 2, 2, 0, g.global_bc)
+    assert ((f1 > 0) & (f1 < 1)).sum() > 5
+    np.testing.assert_allclose(f2, f1, atol=1e-12)
+
+
+def test_quadrant_option_is_the_mean_of_the_quadrant_areas():
+    rng = np.random.default_rng(9)
+    p, g = _cells_from_vertex_values(rng, False)
+    ice = np.ones((g.nx, g.ny), np.int32)
+    flt = np.ones((g.nx, g.ny), np.int32)
+    land = np.zeros((g.nx, g.ny), np.int32)
+    f2, ff = ge.grounded_fraction_options(p.thck, p.topg, 0.0, ice, flt, land, 2, 2, 0, g.global_bc)
+    for (i, j) in ((3, 3), (4, 2), (2, 5)):
+        sw, se, ne, nw = ff[i, j], ff[i + 1, j], ff[i + 1, j + 1], ff[i, j + 1]
+        ctr = (sw + se + ne + nw) / 4.0
+        quads = [[sw, (sw + se) / 2, ctr, (sw + nw) / 2], [(sw + se) / 2, se, (se + ne) / 2, ctr],
+                 [ctr, (ne + se) / 2, ne, (ne + nw) / 2], [(sw + nw) / 2, ctr, (nw + ne) / 2, nw]]
+        assert f2[i, j] == pytest.approx(np.mean([_area(q) for q in quads]), abs=1e-7)
+
