@@ -432,11 +432,19 @@ def run_cpu_legs(args, g, act, s, x64, xm, xb, it64):
         cpu = one
     cpu["host"] = host_cpu()
     cpu["dome_picard"] = dome_picard_legs()
-    xu, xv = g.zeros_vec(), g.zeros_vec()
-    td = time.perf_counter()
-    derr, dnit = pcg.pcg_solver_chrongear_3d(g, act, *A, bu, bv, xu, xv, 2, 5, args.tol, 50000)
-    dropin = dict(seconds=round(time.perf_counter() - td, 3), iterations=int(dnit), err=derr,
+    # the host-array drop-in, as the Fortran shim calls it: the first call
+    # makes the library's device system for this grid, a second call (the
+    # next Picard iteration) reuses it
+    calls = []
+    for _ in range(2):
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        td = time.perf_counter()
+        derr, dnit = pcg.pcg_solver_chrongear_3d(g, act, *A, bu, bv, xu, xv, 2, 5, args.tol, 50000)
+        calls.append(round(time.perf_counter() - td, 3))
+    dropin = dict(seconds=calls[0], seconds_next_call=calls[1], iterations=int(dnit), err=derr,
                   matrix_bytes_uploaded=int(4 * Auu.nbytes))
+    from cism_amd import _lib
+    _lib.load().cismgpu_dropin_release()       # give the drop-in's device memory back
     log(f"host-array drop-in call: {dropin}")
     return cpu, cmp, dropin
 

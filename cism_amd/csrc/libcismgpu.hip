@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cismgpu_internal.h"
@@ -2721,6 +2722,61 @@ extern "C" int32_t cismgpu_ensemble_member_info(cismgpu_ensemble* E, int32_t m, 
 }
 
 // ---------------------------------------------------------------- drop-in entry points
+// The host-array drop-ins keep one device system per (device, grid, 2-D)
+// between calls: a Picard loop calls the solver with one grid every
+// iteration, so its device buffers are allocated once (the reference's
+// solvers allocate their work arrays on every call).  Nothing else carries
+// over: the matrix, rhs and guess are uploaded by every call and every
+// coefficient row is rewritten.  cismgpu_dropin_release frees the system.
+struct DropinCache {
+    int dev = -1;
+    cismgpu_grid g{};
+    bool two_d = false;
+    cismgpu_system* S = nullptr;
+};
+static std::mutex g_dropin_mu;
+static DropinCache g_dropin;
+
+// the cached system for this grid (made if absent); g_dropin_mu held
+static int dropin_system(const cismgpu_grid* grid, bool two_d, cismgpu_system** out) {
+    if (!grid) { cismgpu_set_error("null grid"); return CISMGPU_ERR_ARG; }
+    int dev = 0;
+    CG_CHECK(hipGetDevice(&dev));
+    if (g_dropin.S && g_dropin.dev == dev && g_dropin.two_d == two_d &&
+        !memcmp(&g_dropin.g, grid, sizeof(cismgpu_grid))) {
+        *out = g_dropin.S;
+        return CISMGPU_OK;
+    }
+    if (g_dropin.S) {
+        cismgpu_system_destroy(g_dropin.S);
+        g_dropin.S = nullptr;
+    }
+    cismgpu_system* S = nullptr;
+    const int st = two_d ? cismgpu_system_create_2d(grid, &S) : cismgpu_system_create(grid, &S);
+    if (st) return st;
+    g_dropin.dev = dev;
+    g_dropin.g = *grid;
+    g_dropin.two_d = two_d;
+    g_dropin.S = S;
+    *out = S;
+    return CISMGPU_OK;
+}
+
+// after a HIP failure the cached system is not trusted again
+static void dropin_drop_on(int st) {
+    if (st == CISMGPU_ERR_HIP && g_dropin.S) {
+        cismgpu_system_destroy(g_dropin.S);
+        g_dropin.S = nullptr;
+    }
+}
+
+int32_t cismgpu_dropin_release(void) {
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
+    if (g_dropin.S) cismgpu_system_destroy(g_dropin.S);
+    g_dropin.S = nullptr;
+    return CISMGPU_OK;
+}
+
 static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, const double* Auu, const double* Auv,
                   const double* Avu, const double* Avv, const double* bu, const double* bv, double* xu, double* xv,
                   int32_t precond, int32_t ncheck, double tol, int32_t maxiters, double* err, int32_t* niters,
@@ -2729,8 +2785,9 @@ static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, con
         cismgpu_set_error("null argument");
         return CISMGPU_ERR_ARG;
     }
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
     cismgpu_system* S = nullptr;
-    int st = two_d ? cismgpu_system_create_2d(grid, &S) : cismgpu_system_create(grid, &S);
+    int st = dropin_system(grid, two_d, &S);
     if (st) return st;
     st = two_d ? cismgpu_system_set_matrix_2d(S, active, Auu, Auv, Avu, Avv, bu, bv, 0)
                : cismgpu_system_set_matrix(S, active, Auu, Auv, Avu, Avv, bu, bv, 0);
@@ -2748,7 +2805,7 @@ static int dropin(int algo, const cismgpu_grid* grid, const int32_t* active, con
         if (info.err == info.err) *err = info.err;   // untouched when no check ran
         *niters = info.niters;
     }
-    cismgpu_system_destroy(S);
+    dropin_drop_on(st);
     return st;
 }
 
@@ -2774,15 +2831,16 @@ int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const in
                                               const double* Auu, const double* Auv, const double* Avu,
                                               const double* Avv, const double* xu, const double* xv, double* yu,
                                               double* yv) {
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
     cismgpu_system* S = nullptr;
-    int st = cismgpu_system_create(grid, &S);
+    int st = dropin_system(grid, false, &S);
     if (st) return st;
     const Dims& d = S->d;
     std::vector<double> zero((size_t)d.nz * d.nvx * d.nvy, 0.0);
     st = cismgpu_system_set_matrix(S, active_vertex, Auu, Auv, Avu, Avv, zero.data(), zero.data(), 0);
     if (!st) st = cismgpu_system_set_x(S, xu, xv);
     if (!st) st = cismgpu_system_matvec(S, yu, yv);
-    cismgpu_system_destroy(S);
+    dropin_drop_on(st);
     return st;
 }
 
@@ -2808,15 +2866,16 @@ int32_t cismgpu_matvec_multiply_structured_2d(const cismgpu_grid* grid, const in
                                               const double* Auu, const double* Auv, const double* Avu,
                                               const double* Avv, const double* xu, const double* xv, double* yu,
                                               double* yv) {
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
     cismgpu_system* S = nullptr;
-    int st = cismgpu_system_create_2d(grid, &S);
+    int st = dropin_system(grid, true, &S);
     if (st) return st;
     const Dims& d = S->d;
     std::vector<double> zero((size_t)d.nvx * d.nvy, 0.0);
     st = cismgpu_system_set_matrix_2d(S, active_vertex, Auu, Auv, Avu, Avv, zero.data(), zero.data(), 0);
     if (!st) st = cismgpu_system_set_x(S, xu, xv);
     if (!st) st = cismgpu_system_matvec(S, yu, yv);
-    cismgpu_system_destroy(S);
+    dropin_drop_on(st);
     return st;
 }
 

@@ -94,6 +94,14 @@ int32_t cismgpu_matvec_multiply_structured_3d(const cismgpu_grid* grid, const in
                                               const double* xu, const double* xv,
                                               double* yu, double* yv);
 
+/* The host-array drop-ins above and below keep one device system between
+ * calls, for the last (device, grid, 2-D) they were called with, so a
+ * Picard loop allocates device memory once; every call still uploads the
+ * matrix, rhs and guess.  No reference counterpart (the reference's
+ * solvers hold no state): frees that system, for callers that want the
+ * memory back. */
+int32_t cismgpu_dropin_release(void);
+
 /* 2-D (depth-integrated: SSA, L1L2, DIVA) solves.  Matrices are the
  * reference's (nx-1, ny-1, 9) stencils (m last, m = 1 + (iA+1) + 3(jA+1),
  * indxA_2d), vectors (nx-1, ny-1); grid->nz is ignored.  precond 0 none,

@@ -7,6 +7,8 @@ status 1); and the nz limit.
 Bar: FP64 solutions 1e-10 relative with the same iteration count; the
 mixed solve 1e-6 of the FP64 solve.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -96,3 +98,30 @@ def test_assembly_deep_and_shallow_columns(require_gpu, nz):
     bscale = max(np.abs(ref["bu"]).max(), np.abs(ref["bv"]).max(), 1e-300)
     assert np.abs(bu - ref["bu"]).max() / bscale < 1e-13
     assert np.abs(bv - ref["bv"]).max() / bscale < 1e-13
+
+
+def test_dropin_device_system_is_reused_without_carrying_state(require_gpu):
+    """The host-array drop-in keeps its device system for the last grid: a
+    solve of another matrix on the same grid (other active mask, other rhs),
+    a solve on another grid and a return to the first give bit for bit the
+    results of fresh systems (cismgpu_dropin_release before each)."""
+    L = _lib.load()
+    L.cismgpu_dropin_release.restype = ctypes.c_int32
+
+    def solve(sysdef, fresh):
+        g, act, A, bu, bv = sysdef
+        if fresh:
+            assert L.cismgpu_dropin_release() == 0
+        xu, xv = g.zeros_vec(), g.zeros_vec()
+        err, nit = pcg.pcg_solver_chrongear_3d(g, act, *A, bu, bv, xu, xv, 2, 5, 1e-10, 500)
+        return np.stack([xu, xv]), err, nit
+
+    s1 = random_spd_system(n=12, nz=6, bc=BC_NO_ICE, seed=21)
+    s2 = random_spd_system(n=12, nz=6, bc=BC_NO_ICE, seed=22)
+    s3 = random_spd_system(n=10, nz=5, bc=BC_PERIODIC, seed=23)
+    ref = [solve(s, True) for s in (s1, s2, s3, s1)]
+    got = [solve(s1, True)] + [solve(s, False) for s in (s2, s3, s1)]
+    for (x, e, n), (xr, er, nr) in zip(got, ref):
+        assert np.array_equal(x, xr) and e == er and n == nr
+    assert not np.array_equal(got[0][0], got[1][0])
+    assert L.cismgpu_dropin_release() == 0
