@@ -200,6 +200,21 @@ def main():
         spmv_iso[name] = dict(avg_ms=round(ms, 4), algorithmic_bytes=int(by), gbps=round(by / (ms * 1e-3) / 1e9, 1),
                               frac_of_peak=round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3))
     log(f"isolated SpMV: {spmv_iso}")
+    # internal stress of the mixed solution (compute_internal_stress): one kernel over
+    # the elements and the six (nz-1, nx, ny) fields copied to the host
+    stress = None
+    try:
+        s.set_x(np.asfortranarray(xm[0]), np.asfortranarray(xm[1]))
+        bp.internal_stress()
+        tw = time.perf_counter()
+        tau = bp.internal_stress()
+        stress = dict(ms_with_host_copy=round(1e3 * (time.perf_counter() - tw), 2),
+                      max_tau_eff_kpa=round(float(tau["tau_eff"].max()) / 1e3, 2),
+                      note="kernel alone: profiles/kernel_stats_icestream1024.md (k_internal_stress)")
+        del tau
+        log(f"internal stress: {stress}")
+    except Exception as e:
+        log(f"internal stress leg failed: {e!r}")
     # velocity-dependent basal traction on the same grid (glissade_calcbeta, Schoof law)
     cbeta, cb_sample = None, None
     try:
@@ -292,6 +307,7 @@ def main():
         "hbm_measured_gbps": hbm,
         "assembly": {"element_kernel_ms": round(ainfo["assemble_ms"], 2), "total_ms": round(ainfo["total_ms"], 2)},
         "calcbeta": cbeta,
+        "internal_stress": stress,
         "cpu_baseline": cpu,
         "fp64_config2": cfg2,
         "ensemble": ens,

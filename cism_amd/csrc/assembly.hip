@@ -95,9 +95,9 @@ __device__ __forceinline__ bool basis_derivs(const Elem& e, int p, double* dx, d
     return true;
 }
 
-// compute_effective_viscosity, BP branch
-__device__ __forceinline__ double efvs_at(const AsmDev& in, const Elem& e, const double* dx, const double* dy,
-                                          const double* dz, double flwafact) {
+// compute_effective_viscosity, BP branch, from the point's velocity gradients
+__device__ __forceinline__ double efvs_from_strain(const AsmDev& in, double ux, double uy, double uz, double vx,
+                                                   double vy, double vz, double flwafact) {
     const double p_effstr = (1.0 - in.n_glen) / in.n_glen;
     if (in.whichefvs == 0) return in.efvs_constant;
     if (in.whichefvs == 1) {
@@ -105,16 +105,23 @@ __device__ __forceinline__ double efvs_at(const AsmDev& in, const Elem& e, const
         const double effstrain = (500.0 / scyr) / 200.0e3 * scyr;
         return flwafact * pow(effstrain, p_effstr);
     }
+    const double t1 = ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
+    const double t2 = 0.25 * (uz * uz + vz * vz);
+    const double esq = in.effstrain_min * in.effstrain_min + t1 + t2;
+    return flwafact * pow(esq, p_effstr / 2.0);
+}
+
+// compute_effective_viscosity, BP branch
+__device__ __forceinline__ double efvs_at(const AsmDev& in, const Elem& e, const double* dx, const double* dy,
+                                          const double* dz, double flwafact) {
+    if (in.whichefvs == 0 || in.whichefvs == 1) return efvs_from_strain(in, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, flwafact);
     double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         ux = ux + dx[n] * e.u[n]; uy = uy + dy[n] * e.u[n]; uz = uz + dz[n] * e.u[n];
         vx = vx + dx[n] * e.v[n]; vy = vy + dy[n] * e.v[n]; vz = vz + dz[n] * e.v[n];
     }
-    const double t1 = ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy));
-    const double t2 = 0.25 * (uz * uz + vz * vz);
-    const double esq = in.effstrain_min * in.effstrain_min + t1 + t2;
-    return flwafact * pow(esq, p_effstr / 2.0);
+    return efvs_from_strain(in, ux, uy, uz, vx, vy, vz, flwafact);
 }
 
 __device__ __forceinline__ bool cell_in_range(const Dims& d, int nhalo, int ci, int cj) {
@@ -526,6 +533,54 @@ __global__ __launch_bounds__(WAVE * ROWS_J) void k_rows(Dims d, AsmDev in, const
         for (int off = WAVE / 2; off > 0; off >>= 1) zz += __shfl_down(zz, off, WAVE);
         if (lane == 0 && zz) atomicAdd(nnz, zz);
     }
+}
+
+// compute_internal_stress (glissade_velo_higher.F90:7046-7281), BP branch:
+// per active element in the assembly range, the mean over the 8 quadrature
+// points of tau_xz = efvs du/dz, tau_yz = efvs dv/dz, tau_xx = 2 efvs du/dx,
+// tau_yy = 2 efvs dv/dy, tau_xy = efvs (dv/dx + du/dy), efvs from the
+// velocity at the point, and tau_eff = sqrt(xx^2 + yy^2 + xx yy + xy^2 +
+// xz^2 + yz^2); zero elsewhere.  Six fields of (nz-1, nx, ny) at tau + q nfl;
+// one thread per element.
+__global__ void k_internal_stress(Dims d, AsmDev in, const double* __restrict__ uv, const double* __restrict__ vv,
+                                  double* __restrict__ tau) {
+#pragma clang fp contract(off)
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n = (long long)(d.nz - 1) * d.nx * d.ny;
+    if (t >= n) return;
+    const int kk = 1 + (int)(t % (d.nz - 1));
+    const long long cell = t / (d.nz - 1);
+    const int ci = 1 + (int)(cell % d.nx), cj = 1 + (int)(cell / d.nx);
+    double txz = 0.0, tyz = 0.0, txx = 0.0, tyy = 0.0, txy = 0.0, teff = 0.0;
+    if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]) {
+        Elem e;
+        load_elem(d, in, uv, vv, ci, cj, kk, e);
+        const double fl = in.flwafact[t];
+        for (int p = 0; p < 8; ++p) {
+            double dx[8], dy[8], dz[8], detJ;
+            if (!basis_derivs(e, p, dx, dy, dz, detJ)) { txz = tyz = txx = tyy = txy = NAN; break; }
+            double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                ux = ux + dx[q] * e.u[q]; uy = uy + dy[q] * e.u[q]; uz = uz + dz[q] * e.u[q];
+                vx = vx + dx[q] * e.v[q]; vy = vy + dy[q] * e.v[q]; vz = vz + dz[q] * e.v[q];
+            }
+            const double ef = efvs_from_strain(in, ux, uy, uz, vx, vy, vz, fl);   // the same sums
+            txz = txz + ef * uz;
+            tyz = tyz + ef * vz;
+            txx = txx + 2.0 * ef * ux;
+            tyy = tyy + 2.0 * ef * vy;
+            txy = txy + ef * (vx + uy);
+        }
+        txz = txz / 8.0; tyz = tyz / 8.0; txx = txx / 8.0; tyy = tyy / 8.0; txy = txy / 8.0;
+        teff = sqrt(txx * txx + tyy * tyy + txx * tyy + txy * txy + txz * txz + tyz * tyz);
+    }
+    tau[t] = txz;
+    tau[n + t] = tyz;
+    tau[2 * n + t] = txx;
+    tau[3 * n + t] = tyy;
+    tau[4 * n + t] = txy;
+    tau[5 * n + t] = teff;
 }
 
 // efvs(k,i,j) = mean over the quadrature points (:5604-5610), one thread

@@ -93,7 +93,8 @@ struct cismgpu_system {
     double* load3 = nullptr;     // 2-D system: nz_levels-level load work pair
     double* diva_buf = nullptr;  // 2-D system: DIVA inputs and state
     DivaDev dv{};
-    AsmDev asm_in{};             // device inputs of the last 2-D assembly
+    AsmDev asm_in{};             // device inputs of the last assembly (2-D, or 3-D when asm3_ready)
+    bool asm3_ready = false;     // asm_in holds the inputs of a 3-D assembly
     double* efva2 = nullptr;     // 2-D system: (4, nx, ny) vertical-mean viscosities (L1L2)
     int last_approx = 0;         // which_ho_approx of the last 2-D assembly
     bool diva_ready = false;
@@ -1583,6 +1584,8 @@ extern "C" int32_t cismgpu_system_assemble_3d(cismgpu_system* S, const cismgpu_a
     if (in->umask_dirichlet || in->vmask_dirichlet)
         k_dirichlet<<<nblocks(d.nrows, 128), 128, 0, S->s>>>(d, a, uv, vv, S->A64, S->V(2), S->V(3));
     if (d.bc == 3) k_matrix_noice_edge<<<nblocks(rim_rows_per_level(d) * d.nz, 256), 256, 0, S->s>>>(d, S->A64);
+    S->asm_in = a;
+    S->asm3_ready = true;
     S->edge_zero = d.bc == 3;
     halo<double>(S, S->V(2), S->V(3), d.nz);
     // symmetry fix: count the differing pairs first; the copy-and-average
@@ -2142,6 +2145,30 @@ extern "C" int32_t cismgpu_system_calcbeta(cismgpu_system* S, const double* ubas
     if (beta) CG_CHECK(hipMemcpyAsync(beta, S->cb_beta, n2 * sizeof(double), hipMemcpyDeviceToHost, S->s));
     CG_CHECK(hipStreamSynchronize(S->s));
     S->cb_valid = true;
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_internal_stress(cismgpu_system* S, double* tau_xz, double* tau_yz, double* tau_xx,
+                                                  double* tau_yy, double* tau_xy, double* tau_eff) {
+    if (!S || S->two_d || !S->asm3_ready) {
+        cismgpu_set_error("internal_stress: needs a 3-D system with an assembly");
+        return CISMGPU_ERR_ARG;
+    }
+    const Dims& d = S->d;
+    const long long nfl = (long long)(d.nz - 1) * d.nx * d.ny;
+    double* tau = nullptr;
+    CG_CHECK(hipMalloc(&tau, 6 * nfl * sizeof(double)));
+    k_internal_stress<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, S->asm_in, S->V(0), S->V(1), tau);
+    double* out[6] = {tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff};
+    hipError_t e = hipGetLastError();
+    for (int q = 0; q < 6 && e == hipSuccess; ++q)
+        if (out[q]) e = hipMemcpyAsync(out[q], tau + q * nfl, nfl * sizeof(double), hipMemcpyDeviceToHost, S->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->s);
+    hipFree(tau);
+    if (e != hipSuccess) {
+        cismgpu_set_error("internal_stress: %s", hipGetErrorString(e));
+        return CISMGPU_ERR_HIP;
+    }
     return CISMGPU_OK;
 }
 

@@ -279,6 +279,20 @@ class BPSolver:
                    "assemble_3d")
         return dict(nnz=info.nnz, asym_pairs=info.asym_pairs, assemble_ms=info.assemble_ms, total_ms=info.total_ms)
 
+    def internal_stress(self):
+        """compute_internal_stress (glissade_velo_higher.F90:7046-7281) of the
+        resident x with the last assembly's geometry: dict of tau_xz, tau_yz,
+        tau_xx, tau_yy, tau_xy, tau_eff, each (nz-1, nx, ny) in Pa."""
+        g = self.grid
+        names = ("tau_xz", "tau_yz", "tau_xx", "tau_yy", "tau_xy", "tau_eff")
+        out = {k: np.zeros((g.nz - 1, g.nx, g.ny), order="F") for k in names}
+        D = ctypes.POINTER(ctypes.c_double)
+        f = self._L.cismgpu_system_internal_stress
+        f.argtypes = [ctypes.c_void_p] + [D] * 6
+        f.restype = ctypes.c_int32
+        _lib.check(f(self.sys.handle, *[out[k].ctypes.data_as(D) for k in names]), "internal_stress")
+        return out
+
     def efvs(self):
         g = self.grid
         out = np.zeros((g.nz - 1, g.nx, g.ny), order="F")

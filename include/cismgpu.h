@@ -437,6 +437,13 @@ int32_t cismgpu_system_vector_op(cismgpu_system* sys, int32_t op, int32_t dst, i
  * mean (2) relative speed change over owned vertices; 1 and 2 skip the bed. */
 int32_t cismgpu_system_residual_velocity(cismgpu_system* sys, int32_t whichresid, int32_t old_slot,
                                          double* resid);
+/* compute_internal_stress (glissade_velo_higher.F90:7046-7281, BP branch)
+ * from the resident x and the inputs of the last 3-D assembly: tau_xz,
+ * tau_yz, tau_xx, tau_yy, tau_xy (Pa; means over the quadrature points of
+ * 2 efvs times the strain rates) and tau_eff, each (nz-1, nx, ny), zero
+ * outside the active cells; any output may be NULL. */
+int32_t cismgpu_system_internal_stress(cismgpu_system* sys, double* tau_xz, double* tau_yz, double* tau_xx,
+                                       double* tau_yy, double* tau_xy, double* tau_eff);
 /* efvs(nz-1, nx, ny) of the last assembly (mean over quadrature points) */
 int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
 /* resid = A x - b on owned active vertices, L2 norm and norm relative to

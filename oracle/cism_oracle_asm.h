@@ -153,6 +153,12 @@ int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int w
                           const int32_t* floating_mask, const int32_t* land_mask, double* f_flotation,
                           double* f_ground);
 
+int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int nhalo, const int32_t* active_cell,
+                                const double* xVertex, const double* yVertex, const double* stagusrf,
+                                const double* stagthck, const double* flwafact, int whichefvs, double efvs_constant,
+                                double effstrain_min, double n_glen, const double* uvel, const double* vvel,
+                                double* tau);
+
 #ifdef __cplusplus
 }
 #endif

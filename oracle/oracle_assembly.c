@@ -358,6 +358,57 @@ int orc_element_matrix(const double* x, const double* y, const double* z, const 
     return ORC_OK;
 }
 
+/* compute_internal_stress (glissade_velo_higher.F90:7046-7281), BP branch:
+ * tau (6 fields of (nz-1, nx, ny): xz, yz, xx, yy, xy, eff), zero outside
+ * the active cells of the assembly range */
+int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int nhalo, const int32_t* active_cell,
+                                const double* xVertex, const double* yVertex, const double* stagusrf,
+                                const double* stagthck, const double* flwafact, int whichefvs, double efvs_constant,
+                                double effstrain_min, double n_glen, const double* uvel, const double* vvel,
+                                double* tau)
+{
+    init_tables();
+    const size_t nfl = (size_t)(nz - 1) * nx * ny;
+    memset(tau, 0, 6 * nfl * sizeof(double));
+    double x[8], y[8], z[8], u[8], v[8], dx[8], dy[8], dz[8], detJ;
+    for (int j = nhalo + 1; j <= ny - nhalo + 1; ++j)
+        for (int i = nhalo + 1; i <= nx - nhalo + 1; ++i) {
+            if (!active_cell[C2(i, j)]) continue;
+            for (int k = 1; k <= nz - 1; ++k) {
+                for (int n = 0; n < 8; ++n) {
+                    const int in = i + node_di[n], jn = j + node_dj[n], kn = k + node_dk[n];
+                    x[n] = xVertex[V2(in, jn)];
+                    y[n] = yVertex[V2(in, jn)];
+                    z[n] = stagusrf[V2(in, jn)] - sigma[kn - 1] * stagthck[V2(in, jn)];
+                    u[n] = uvel[V3(kn, in, jn)];
+                    v[n] = vvel[V3(kn, in, jn)];
+                }
+                double txz = 0.0, tyz = 0.0, txx = 0.0, tyy = 0.0, txy = 0.0;
+                for (int p = 0; p < 8; ++p) {
+                    if (basis_derivs(x, y, z, p, dx, dy, dz, &detJ)) return ORC_ERR_SINGULAR;
+                    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
+                    for (int n = 0; n < 8; ++n) {
+                        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n]; uz = uz + dz[n] * u[n];
+                        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n]; vz = vz + dz[n] * v[n];
+                    }
+                    const double ef = effective_viscosity(whichefvs, efvs_constant, effstrain_min, n_glen, dx, dy, dz,
+                                                          u, v, flwafact[E3(k, i, j)]);
+                    txz = txz + ef * uz;
+                    tyz = tyz + ef * vz;
+                    txx = txx + 2.0 * ef * ux;
+                    tyy = tyy + 2.0 * ef * vy;
+                    txy = txy + ef * (vx + uy);
+                }
+                txz = txz / 8.0; tyz = tyz / 8.0; txx = txx / 8.0; tyy = tyy / 8.0; txy = txy / 8.0;
+                const size_t e = E3(k, i, j);
+                tau[e] = txz; tau[nfl + e] = tyz; tau[2 * nfl + e] = txx; tau[3 * nfl + e] = tyy;
+                tau[4 * nfl + e] = txy;
+                tau[5 * nfl + e] = sqrt(txx * txx + tyy * tyy + txx * tyy + txy * txy + txz * txz + tyz * tyz);
+            }
+        }
+    return ORC_OK;
+}
+
 /* assemble_stiffness_matrix_3d (glissade_velo_higher.F90:5382-5643) with
  * element_to_global_matrix_3d (:8200-8268). */
 int orc_assemble_stiffness_matrix_3d(int nx, int ny, int nz, const double* sigma, int nhalo,

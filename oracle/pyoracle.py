@@ -105,6 +105,9 @@ def _declare(L):
     L.orc_slope_angle_staggered.argtypes = [i, i, d, d, _D, _D, _D, _D]
     L.orc_cell_halo.argtypes = [i, i, i, _D]
     L.orc_calcbeta.argtypes = [i, i, i, i, i, _D] + [_D] * 7 + [_I] + [_D] * 5
+    L.orc_compute_internal_stress.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, i, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, _D, _D, _D]
+    L.orc_compute_internal_stress.restype = ctypes.c_int
     L.orc_compute_grounded_fraction.argtypes = [_D, _D]
     L.orc_compute_grounded_fraction.restype = ctypes.c_int
     L.orc_grounded_fraction.argtypes = [i, i, i, i, i, i, _D, _D, ctypes.c_double, _I, _I, _I, _D, _D]
@@ -347,6 +350,28 @@ def grounded_fraction(problem, ice, flt, land):
     if st:
         raise ValueError(f"grounded fraction: status {st}")
     return fground
+
+
+def internal_stress(problem, uvel, vvel, geo=None):
+    """compute_internal_stress (glissade_velo_higher.F90:7046-7281, BP
+    branch) of a 3-D velocity: dict of tau_xz, tau_yz, tau_xx, tau_yy,
+    tau_xy, tau_eff, each (nz-1, nx, ny)."""
+    p = problem
+    g = p.grid
+    geo = geo or geometry(p)
+    xV, yV = p.vertex_coords()
+    nfl = (g.nz - 1) * g.nx * g.ny
+    tau = np.zeros(6 * nfl)
+    st = lib().orc_compute_internal_stress(g.nx, g.ny, g.nz, dp(np.ascontiguousarray(p.sigma, dtype=np.float64)), 2,
+                                           ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["stagusrf"]),
+                                           dp(geo["stagthck"]), dp(geo["flwafact"]), p.whichefvs, p.efvs_constant,
+                                           p.effstrain_min, p.n_glen, dp(np.asfortranarray(uvel, dtype=np.float64)),
+                                           dp(np.asfortranarray(vvel, dtype=np.float64)), dp(tau))
+    if st:
+        raise RuntimeError(f"oracle internal stress failed with status {st}")
+    names = ("tau_xz", "tau_yz", "tau_xx", "tau_yy", "tau_xy", "tau_eff")
+    return {k: np.asfortranarray(tau[q * nfl:(q + 1) * nfl].reshape((g.nz - 1, g.nx, g.ny), order="F"))
+            for q, k in enumerate(names)}
 
 
 def compute_grounded_fraction(f4):

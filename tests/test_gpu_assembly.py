@@ -200,3 +200,19 @@ def test_device_residual_velocity_and_vector_ops(require_gpu, whichresid):
     s._vop(3, -1)
     xu, xv = s.sys.get_x()
     assert not xu.any() and not xv.any()
+
+
+@pytest.mark.parametrize("name", ["dome", "stream", "marine"])
+def test_internal_stress_matches_oracle(require_gpu, name):
+    """compute_internal_stress (glissade_velo_higher.F90:7046-7281) on the
+    device after a Picard solve = the oracle's of the same velocity, each
+    component within 1e-12 of its largest magnitude."""
+    p = PROBLEMS[name]
+    s = BPSolver(p)
+    r = s.picard(algo=3, precond=2, linear_tol=1e-10, linear_maxiters=2000, resid_target=1e-3, maxiter=8)
+    got = s.internal_stress()
+    ref = po.internal_stress(p, r.uvel, r.vvel)
+    for k, v in ref.items():
+        scale = np.abs(v).max()
+        assert scale > 0, k
+        assert np.abs(got[k] - v).max() <= 1e-12 * scale, k
