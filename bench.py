@@ -215,6 +215,21 @@ def main():
         log(f"internal stress: {stress}")
     except Exception as e:
         log(f"internal stress leg failed: {e!r}")
+    # basal friction heat flux and traction of the same solution
+    # (compute_basal_friction_heatflx, both options; one kernel and the cell halo)
+    bfric = None
+    try:
+        bfric = {}
+        for which, name in ((1, "local"), (0, "standard")):
+            bp.basal_friction(which)
+            tw = time.perf_counter()
+            bf, _, _ = bp.basal_friction(which)
+            bfric[name] = dict(ms_with_host_copy=round(1e3 * (time.perf_counter() - tw), 2),
+                               max_w_per_m2=round(float(bf.max()), 4))
+        bfric["note"] = "kernel alone: profiles/kernel_stats_icestream1024.md (k_basal_friction)"
+        log(f"basal friction: {bfric}")
+    except Exception as e:
+        log(f"basal friction leg failed: {e!r}")
     # velocity-dependent basal traction on the same grid (glissade_calcbeta, Schoof law)
     cbeta, cb_sample = None, None
     try:
@@ -308,6 +323,7 @@ def main():
         "assembly": {"element_kernel_ms": round(ainfo["assemble_ms"], 2), "total_ms": round(ainfo["total_ms"], 2)},
         "calcbeta": cbeta,
         "internal_stress": stress,
+        "basal_friction": bfric,
         "cpu_baseline": cpu,
         "fp64_config2": cfg2,
         "ensemble": ens,

@@ -43,6 +43,7 @@ EXPORTS = (
     "cismgpu_residual_vector_3d_noalloc", "cismgpu_reprosum_calc",
     "cismgpu_system_calcbeta_setup", "cismgpu_system_calcbeta", "cismgpu_dropin_release",
     "cismgpu_system_internal_stress",
+    "cismgpu_system_basal_friction",
 )
 
 

@@ -175,4 +175,69 @@ __global__ void k_calcbeta(Dims d, CBDev c, const double* __restrict__ u, const 
     if (!(b >= 0.0)) atomicMin(bad, (unsigned long long)t);
 }
 
+// compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042) before
+// its cell halo, and the basal traction beta u_b (:3817-3818).  One thread
+// per cell c = (i-1) + nx (j-1); thread c < nvx nvy also writes the traction
+// of vertex c.  which 0 (standard): owned active cells, beta u.u at the four
+// Gauss points from the bilinear interpolants (weights 1), over 4 scyr;
+// which 1 (local): glissade_unstagger's mean, over cells 2..nx-1 and
+// 2..ny-1, of beta (u^2 + v^2) / scyr at active vertices (0 elsewhere).
+// Velocity: level lev of a system vector, or vertex arrays when lev < 0.
+__global__ void k_basal_friction(Dims d, int which, int nhalo, const int32_t* __restrict__ acell,
+                                 const int32_t* __restrict__ avert, const double* __restrict__ u,
+                                 const double* __restrict__ v, int lev, const double* __restrict__ beta,
+                                 double* __restrict__ bfric, double* __restrict__ btrac) {
+#pragma clang fp contract(off)
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n2 = (long long)d.nvx * d.nvy;
+    if (c >= (long long)d.nx * d.ny) return;
+    auto vel = [&](const double* a, int i, int j) { return a[lev >= 0 ? vidx(d, lev, i, j) : v2(d, i, j)]; };
+    if (btrac && c < n2) {
+        const int i = 1 + (int)(c % d.nvx), j = 1 + (int)(c / d.nvx);
+        btrac[c] = beta[c] * vel(u, i, j);
+        btrac[n2 + c] = beta[c] * vel(v, i, j);
+    }
+    if (!bfric) return;
+    const double scyr = 31536000.0;   // glimmer_physcon.F90:90
+    const int i = 1 + (int)(c % d.nx), j = 1 + (int)(c / d.nx);
+    double f = 0.0;
+    if (which == 0) {
+        if (i >= nhalo + 1 && i <= d.nx - nhalo && j >= nhalo + 1 && j <= d.ny - nhalo && acell[c]) {
+            // nodes SW, SE, NE, NW: vertices (i-1, j-1), (i, j-1), (i, j), (i-1, j)
+            const int vi[4] = {i - 1, i, i, i - 1}, vj[4] = {j - 1, j - 1, j, j};
+            double un[4], vn[4], bn[4];
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                un[n] = vel(u, vi[n], vj[n]);
+                vn[n] = vel(v, vi[n], vj[n]);
+                bn[n] = beta[v2(d, vi[n], vj[n])];
+            }
+            double sum_w = 0.0;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                double uq = 0.0, vq = 0.0, bq = 0.0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    uq = uq + c_phi2[n][p] * un[n];
+                    vq = vq + c_phi2[n][p] * vn[n];
+                    bq = bq + c_phi2[n][p] * bn[n];
+                }
+                f = f + 1.0 * bq * (uq * uq + vq * vq);
+                sum_w = sum_w + 1.0;
+            }
+            f = f / (sum_w * scyr);
+        }
+    } else if (i >= 2 && i <= d.nx - 1 && j >= 2 && j <= d.ny - 1) {
+        // stagbfricflx at the four vertices, summed in glissade_unstagger's order
+        auto st = [&](int a, int b) {
+            if (!avert[v2(d, a, b)]) return 0.0;
+            const double ua = vel(u, a, b), va = vel(v, a, b);
+            const double s = beta[v2(d, a, b)] * (ua * ua + va * va);
+            return s / scyr;
+        };
+        f = (st(i, j) + st(i - 1, j) + st(i, j - 1) + st(i - 1, j - 1)) / 4.0;
+    }
+    bfric[c] = f;
+}
+
 }  // namespace cg

@@ -2172,6 +2172,85 @@ extern "C" int32_t cismgpu_system_internal_stress(cismgpu_system* S, double* tau
     return CISMGPU_OK;
 }
 
+extern "C" int32_t cismgpu_system_basal_friction(cismgpu_system* S, int32_t which_bfric, const double* ubas,
+                                                 const double* vbas, const double* beta, double* bfricflx,
+                                                 double* btractx, double* btracty) {
+    if (!S || !S->asm_in.acell) {
+        cismgpu_set_error("basal_friction: needs a system with an assembly");
+        return CISMGPU_ERR_ARG;
+    }
+    if (which_bfric != 0 && which_bfric != 1) {
+        cismgpu_set_error("basal_friction: bad which_ho_assemble_bfric %d (0 standard, 1 local)", which_bfric);
+        return CISMGPU_ERR_ARG;
+    }
+    if ((ubas == nullptr) != (vbas == nullptr)) {
+        cismgpu_set_error("basal_friction: pass both basal velocity components or neither");
+        return CISMGPU_ERR_ARG;
+    }
+    const Dims& d = S->d;
+    const long long n2 = (long long)d.nvx * d.nvy, nc = (long long)d.nx * d.ny;
+    const bool diva = S->two_d && S->last_approx == CISMGPU_APPROX_DIVA;
+    const double *u = S->V(0), *v = S->V(1);
+    int lev = d.nz - 1;   // basal level of x (the only level of a 2-D system: SSA's and L1L2's basal velocity)
+    if (!ubas && diva) {
+        // DIVA's basal velocity is the bed level of compute_3d_velocity_diva
+        if (!S->cb_dvel_valid) {
+            cismgpu_set_error("basal_friction: a DIVA system keeps its basal velocity only with a friction law "
+                              "and after diva_velocity; pass ubas, vbas");
+            return CISMGPU_ERR_ARG;
+        }
+        u = S->cb_dvel;
+        v = S->cb_dvel + n2;
+        lev = -1;
+    }
+    double* tmp = nullptr;
+    CG_CHECK(hipMalloc(&tmp, (nc + 5 * n2) * sizeof(double)));
+    double *bf = tmp, *bt = tmp + nc, *hv = tmp + nc + 2 * n2, *hb = tmp + nc + 4 * n2;
+    hipError_t e = hipSuccess;
+    auto up = [&](double* dst, const double* h) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, h, n2 * sizeof(double), hipMemcpyHostToDevice, S->s);
+    };
+    if (ubas) {
+        up(hv, ubas);
+        up(hv + n2, vbas);
+        u = hv;
+        v = hv + n2;
+        lev = -1;
+    }
+    const double* b = S->asm_in.beta;   // the last assembly's beta (calcbeta's on the device)
+    if (beta) {
+        up(hb, beta);
+        b = hb;
+    }
+    if (e == hipSuccess) {
+        k_basal_friction<<<nblocks(nc, 256), 256, 0, S->s>>>(d, which_bfric, 2, S->asm_in.acell, S->asm_in.avert, u, v,
+                                                             lev, b, bfricflx ? bf : nullptr,
+                                                             (btractx || btracty) ? bt : nullptr);
+        if (bfricflx) {   // parallel_halo of the cell field (:3807)
+            if (d.bc == 0 || d.bc == 2) {
+                k_cell_halo<<<nblocks(d.ny, 256), 256, 0, S->s>>>(d, 0, bf);
+                k_cell_halo<<<nblocks(d.nx, 256), 256, 0, S->s>>>(d, 1, bf);
+            } else {
+                k_cell_halo<<<nblocks(nc, 256), 256, 0, S->s>>>(d, 0, bf);
+            }
+        }
+        e = hipGetLastError();
+    }
+    auto down = [&](double* h, const double* src, long long n) {
+        if (h && e == hipSuccess) e = hipMemcpyAsync(h, src, n * sizeof(double), hipMemcpyDeviceToHost, S->s);
+    };
+    down(bfricflx, bf, nc);
+    down(btractx, bt, n2);
+    down(btracty, bt + n2, n2);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->s);
+    hipFree(tmp);
+    if (e != hipSuccess) {
+        cismgpu_set_error("basal_friction: %s", hipGetErrorString(e));
+        return CISMGPU_ERR_HIP;
+    }
+    return CISMGPU_OK;
+}
+
 extern "C" int32_t cismgpu_system_get_efvs(cismgpu_system* S, double* efvs) {
     if (!S || !S->efvs || !efvs) return CISMGPU_ERR_ARG;
     const int nzl = S->two_d ? S->nz_levels : S->d.nz;

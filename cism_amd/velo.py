@@ -293,6 +293,35 @@ class BPSolver:
         _lib.check(f(self.sys.handle, *[out[k].ctypes.data_as(D) for k in names]), "internal_stress")
         return out
 
+    def basal_friction(self, which_bfric=None, ubas=None, vbas=None, beta=None):
+        """compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042)
+        with its cell halo, and the basal traction beta u_b (:3817-3818):
+        (bfricflx (nx, ny) W/m^2, btractx, btracty (nx-1, ny-1) Pa).
+        which_bfric: which_ho_assemble_bfric (default: the problem's, else 1,
+        local).  ubas/vbas/beta: None for the system's basal velocity and the
+        last assembly's beta."""
+        g = self.grid
+        if which_bfric is None:
+            which_bfric = getattr(self.p, "assemble_bfric", 1)
+        if (ubas is None) != (vbas is None):
+            raise ValueError("basal_friction: pass both basal velocity components or neither")
+        D = ctypes.POINTER(ctypes.c_double)
+        f64 = lambda a: None if a is None else np.asfortranarray(np.array(a, dtype=np.float64))
+        ub, vb, b = f64(ubas), f64(vbas), f64(beta)
+        for a in (ub, vb, b):
+            if a is not None and a.shape != (g.nx - 1, g.ny - 1):
+                raise ValueError(f"basal_friction: vertex fields are ({g.nx - 1}, {g.ny - 1}), got {a.shape}")
+        bf = np.zeros((g.nx, g.ny), order="F")
+        bx = np.zeros((g.nx - 1, g.ny - 1), order="F")
+        by = np.zeros((g.nx - 1, g.ny - 1), order="F")
+        f = self._L.cismgpu_system_basal_friction
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int32] + [D] * 6
+        f.restype = ctypes.c_int32
+        ptr = lambda a: None if a is None else a.ctypes.data_as(D)
+        _lib.check(f(self.sys.handle, int(which_bfric), ptr(ub), ptr(vb), ptr(b), ptr(bf), ptr(bx), ptr(by)),
+                   "basal_friction")
+        return bf, bx, by
+
     def efvs(self):
         g = self.grid
         out = np.zeros((g.nz - 1, g.nx, g.ny), order="F")

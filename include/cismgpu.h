@@ -444,6 +444,20 @@ int32_t cismgpu_system_residual_velocity(cismgpu_system* sys, int32_t whichresid
  * outside the active cells; any output may be NULL. */
 int32_t cismgpu_system_internal_stress(cismgpu_system* sys, double* tau_xz, double* tau_yz, double* tau_xx,
                                        double* tau_yy, double* tau_xy, double* tau_eff);
+/* compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042, called
+ * at :3795) with its cell halo (:3807), and the basal traction
+ * btractx/btracty = beta u_b (:3817-3818).  which_bfric: 0 standard (2x2
+ * Gauss points of the bilinear interpolants over owned active cells), 1 local
+ * (active vertices, unstaggered).  ubas, vbas (nx-1, ny-1): the basal
+ * velocity, or both NULL for the system's own (3-D: bed level of x; SSA and
+ * L1L2: x; DIVA: the bed level of the last diva_velocity, kept with a
+ * friction law).  beta (nx-1, ny-1): NULL for the last assembly's beta (the
+ * calcbeta result when it read the device's).  Masks: the last assembly's.
+ * Outputs bfricflx (nx, ny) in W/m^2, btractx, btracty (nx-1, ny-1) in Pa;
+ * any may be NULL. */
+int32_t cismgpu_system_basal_friction(cismgpu_system* sys, int32_t which_bfric, const double* ubas,
+                                      const double* vbas, const double* beta, double* bfricflx, double* btractx,
+                                      double* btracty);
 /* efvs(nz-1, nx, ny) of the last assembly (mean over quadrature points) */
 int32_t cismgpu_system_get_efvs(cismgpu_system* sys, double* efvs);
 /* resid = A x - b on owned active vertices, L2 norm and norm relative to

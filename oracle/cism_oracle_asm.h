@@ -159,6 +159,10 @@ int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int
                                 double effstrain_min, double n_glen, const double* uvel, const double* vvel,
                                 double* tau);
 
+void orc_basal_friction(int nx, int ny, int nhalo, int global_bc, int which, const int32_t* active_cell,
+                        const int32_t* active_vertex, const double* ubas, const double* vbas, const double* beta,
+                        double* bfricflx, double* btractx, double* btracty);
+
 #ifdef __cplusplus
 }
 #endif

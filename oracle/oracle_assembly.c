@@ -1802,3 +1802,70 @@ int orc_basal_sliding_bc_standard(int nx, int ny, int nz, int nhalo, int two_d, 
         }
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------ */
+
+/* compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042, called
+ * :3795-3807) and the basal traction beside it (:3817-3818), on one task.
+ * bfricflx (nx, ny), W/m^2:
+ *   which 0 (standard): on owned active cells, beta u.u at the 2x2 Gauss
+ *     points from the bilinear interpolants of the four vertex values,
+ *     weights 1, divided by (sum of weights) scyr (:6917-6960);
+ *   which 1 (local): beta (u^2 + v^2) / scyr at active vertices, 0 elsewhere,
+ *     then glissade_unstagger's plain mean of the four vertices onto cells
+ *     2..nx-1, 2..ny-1 (:6962-6978; glissade_grid_operators.F90:277-285);
+ * then the cell halo of global_bc (:3807).  btractx/btracty (nx-1, ny-1) =
+ * beta times the basal velocity at every vertex.  Any output may be NULL. */
+void orc_basal_friction(int nx, int ny, int nhalo, int global_bc, int which, const int32_t* active_cell,
+                        const int32_t* active_vertex, const double* ubas, const double* vbas, const double* beta,
+                        double* bfricflx, double* btractx, double* btracty)
+{
+    const double scyr = 31536000.0;   /* glimmer_physcon.F90:90 */
+    init_tables_2d();
+    if (bfricflx) {
+        for (size_t c = 0; c < (size_t)nx * ny; ++c) bfricflx[c] = 0.0;
+        if (which == 0) {
+            for (int j = nhalo + 1; j <= ny - nhalo; ++j)
+                for (int i = nhalo + 1; i <= nx - nhalo; ++i) {
+                    if (!active_cell[C2(i, j)]) continue;
+                    double u[4], v[4], b[4];
+                    for (int n = 0; n < 4; ++n) {
+                        const size_t q = V2(i + di2[n], j + dj2[n]);
+                        u[n] = ubas[q];
+                        v[n] = vbas[q];
+                        b[n] = beta[q];
+                    }
+                    double f = 0.0, sum_w = 0.0;
+                    for (int p = 0; p < 4; ++p) {
+                        double uq = 0.0, vq = 0.0, bq = 0.0;
+                        for (int n = 0; n < 4; ++n) {
+                            uq = uq + phi2[n][p] * u[n];
+                            vq = vq + phi2[n][p] * v[n];
+                            bq = bq + phi2[n][p] * b[n];
+                        }
+                        f = f + 1.0 * bq * (uq * uq + vq * vq);
+                        sum_w = sum_w + 1.0;
+                    }
+                    bfricflx[C2(i, j)] = f / (sum_w * scyr);
+                }
+        } else {
+            double* st = calloc((size_t)NVX * NVY, sizeof(double));
+            for (int j = 1; j <= NVY; ++j)
+                for (int i = 1; i <= NVX; ++i)
+                    if (active_vertex[V2(i, j)]) {
+                        const size_t q = V2(i, j);
+                        st[q] = beta[q] * (ubas[q] * ubas[q] + vbas[q] * vbas[q]);
+                        st[q] = st[q] / scyr;
+                    }
+            for (int j = 2; j <= ny - 1; ++j)
+                for (int i = 2; i <= nx - 1; ++i)
+                    bfricflx[C2(i, j)] = (st[V2(i, j)] + st[V2(i - 1, j)] + st[V2(i, j - 1)] + st[V2(i - 1, j - 1)]) / 4.0;
+            free(st);
+        }
+        orc_cell_halo(nx, ny, global_bc, bfricflx);
+    }
+    for (size_t q = 0; q < (size_t)NVX * NVY; ++q) {
+        if (btractx) btractx[q] = beta[q] * ubas[q];
+        if (btracty) btracty[q] = beta[q] * vbas[q];
+    }
+}

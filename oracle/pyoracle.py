@@ -108,6 +108,7 @@ def _declare(L):
     L.orc_compute_internal_stress.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, i, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double, _D, _D, _D]
     L.orc_compute_internal_stress.restype = ctypes.c_int
+    L.orc_basal_friction.argtypes = [i, i, i, i, i, _I, _I, _D, _D, _D, _D, _D, _D]
     L.orc_compute_grounded_fraction.argtypes = [_D, _D]
     L.orc_compute_grounded_fraction.restype = ctypes.c_int
     L.orc_grounded_fraction.argtypes = [i, i, i, i, i, i, _D, _D, ctypes.c_double, _I, _I, _I, _D, _D]
@@ -372,6 +373,23 @@ def internal_stress(problem, uvel, vvel, geo=None):
     names = ("tau_xz", "tau_yz", "tau_xx", "tau_yy", "tau_xy", "tau_eff")
     return {k: np.asfortranarray(tau[q * nfl:(q + 1) * nfl].reshape((g.nz - 1, g.nx, g.ny), order="F"))
             for q, k in enumerate(names)}
+
+
+def basal_friction(grid, which, active_cell, active_vertex, ubas, vbas, beta):
+    """compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042)
+    with its cell halo (:3807), and btractx/btracty = beta u_b (:3817-3818):
+    (bfricflx (nx, ny), btractx, btracty (nx-1, ny-1))."""
+    g = grid
+    f64 = lambda a: np.asfortranarray(np.array(a, dtype=np.float64))
+    i32 = lambda a: np.asfortranarray(np.array(a, dtype=np.int32))
+    bf = np.zeros((g.nx, g.ny), order="F")
+    bx = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    by = np.zeros((g.nx - 1, g.ny - 1), order="F")
+    ub, vb, b = f64(ubas), f64(vbas), f64(beta)
+    ac, av = i32(active_cell), i32(active_vertex)
+    lib().orc_basal_friction(g.nx, g.ny, 2, g.global_bc, int(which), ip(ac), ip(av), dp(ub), dp(vb), dp(b), dp(bf),
+                             dp(bx), dp(by))
+    return bf, bx, by
 
 
 def compute_grounded_fraction(f4):
