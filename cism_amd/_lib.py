@@ -42,7 +42,7 @@ EXPORTS = (
     "cismgpu_system_diva_velocity", "cismgpu_system_l1l2_velocity", "cismgpu_half_storage_supported",
     "cismgpu_residual_vector_3d_noalloc", "cismgpu_reprosum_calc",
     "cismgpu_system_calcbeta_setup", "cismgpu_system_calcbeta", "cismgpu_dropin_release",
-    "cismgpu_system_internal_stress",
+    "cismgpu_system_internal_stress", "cismgpu_system_internal_stress_2d",
     "cismgpu_system_basal_friction",
 )
 

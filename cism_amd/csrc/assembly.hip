@@ -535,14 +535,20 @@ __global__ __launch_bounds__(WAVE * ROWS_J) void k_rows(Dims d, AsmDev in, const
     }
 }
 
-// compute_internal_stress (glissade_velo_higher.F90:7046-7281), BP branch:
-// per active element in the assembly range, the mean over the 8 quadrature
-// points of tau_xz = efvs du/dz, tau_yz = efvs dv/dz, tau_xx = 2 efvs du/dx,
-// tau_yy = 2 efvs dv/dy, tau_xy = efvs (dv/dx + du/dy), efvs from the
-// velocity at the point, and tau_eff = sqrt(xx^2 + yy^2 + xx yy + xy^2 +
-// xz^2 + yz^2); zero elsewhere.  Six fields of (nz-1, nx, ny) at tau + q nfl;
-// one thread per element.
-__global__ void k_internal_stress(Dims d, AsmDev in, const double* __restrict__ uv, const double* __restrict__ vv,
+// compute_internal_stress (glissade_velo_higher.F90:7046-7281): per active
+// element in the assembly range, the mean over the 8 quadrature points of
+// tau_xz = efvs du/dz, tau_yz = efvs dv/dz, tau_xx = 2 efvs du/dx,
+// tau_yy = 2 efvs dv/dy, tau_xy = efvs (dv/dx + du/dy), and tau_eff =
+// sqrt(xx^2 + yy^2 + xx yy + xy^2 + xz^2 + yz^2); zero elsewhere.  Six fields
+// of (nz-1, nx, ny) at tau + q nfl; one thread per element.  approx is
+// whichapprox: SIA (0) takes only the vertical shear, SSA (1) only the
+// horizontal strain rates (the viscosity of those alone: the strain-rate
+// sums with the dropped terms zero round the same), BP (2) and DIVA (4)
+// every one; L1L2 (3) every one with the viscosity efvs_l1l2(k,i,j) of its
+// assembly.  Velocity (vmode): 0 level k of a 3-D system vector, 1 the one
+// level of a 2-D system at every k (SSA), 2 plain (nz, nx-1, ny-1) arrays.
+__global__ void k_internal_stress(Dims d, AsmDev in, int approx, int vmode, const double* __restrict__ uv,
+                                  const double* __restrict__ vv, const double* __restrict__ efvs_l1l2,
                                   double* __restrict__ tau) {
 #pragma clang fp contract(off)
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -554,18 +560,33 @@ __global__ void k_internal_stress(Dims d, AsmDev in, const double* __restrict__ 
     double txz = 0.0, tyz = 0.0, txx = 0.0, tyy = 0.0, txy = 0.0, teff = 0.0;
     if (cell_in_range(d, in.nhalo, ci, cj) && in.acell[c2(d, ci, cj)]) {
         Elem e;
-        load_elem(d, in, uv, vv, ci, cj, kk, e);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int in_ = ci + nd_i(q), jn = cj + nd_j(q), kn = kk + nd_k(q);
+            const long long pv = v2(d, in_, jn);
+            e.x[q] = in.xV[pv];
+            e.y[q] = in.yV[pv];
+            e.z[q] = in.susrf[pv] - in.sigma[kn - 1] * in.sthck[pv];
+            const long long c = vmode == 0 ? vidx(d, kn - 1, in_, jn)
+                              : vmode == 1 ? vidx(d, 0, in_, jn) : (long long)(kn - 1) + (long long)d.nz * pv;
+            e.u[q] = uv[c];
+            e.v[q] = vv[c];
+        }
         const double fl = in.flwafact[t];
+        const bool vert = approx != 1, horiz = approx != 0;
         for (int p = 0; p < 8; ++p) {
             double dx[8], dy[8], dz[8], detJ;
             if (!basis_derivs(e, p, dx, dy, dz, detJ)) { txz = tyz = txx = tyy = txy = NAN; break; }
             double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                ux = ux + dx[q] * e.u[q]; uy = uy + dy[q] * e.u[q]; uz = uz + dz[q] * e.u[q];
-                vx = vx + dx[q] * e.v[q]; vy = vy + dy[q] * e.v[q]; vz = vz + dz[q] * e.v[q];
+                if (vert) { uz = uz + dz[q] * e.u[q]; vz = vz + dz[q] * e.v[q]; }
+                if (horiz) {
+                    ux = ux + dx[q] * e.u[q]; uy = uy + dy[q] * e.u[q];
+                    vx = vx + dx[q] * e.v[q]; vy = vy + dy[q] * e.v[q];
+                }
             }
-            const double ef = efvs_from_strain(in, ux, uy, uz, vx, vy, vz, fl);   // the same sums
+            const double ef = approx == 3 ? efvs_l1l2[t] : efvs_from_strain(in, ux, uy, uz, vx, vy, vz, fl);
             txz = txz + ef * uz;
             tyz = tyz + ef * vz;
             txx = txx + 2.0 * ef * ux;

@@ -2158,7 +2158,7 @@ extern "C" int32_t cismgpu_system_internal_stress(cismgpu_system* S, double* tau
     const long long nfl = (long long)(d.nz - 1) * d.nx * d.ny;
     double* tau = nullptr;
     CG_CHECK(hipMalloc(&tau, 6 * nfl * sizeof(double)));
-    k_internal_stress<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, S->asm_in, S->V(0), S->V(1), tau);
+    k_internal_stress<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, S->asm_in, 2, 0, S->V(0), S->V(1), nullptr, tau);
     double* out[6] = {tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff};
     hipError_t e = hipGetLastError();
     for (int q = 0; q < 6 && e == hipSuccess; ++q)
@@ -2167,6 +2167,58 @@ extern "C" int32_t cismgpu_system_internal_stress(cismgpu_system* S, double* tau
     hipFree(tau);
     if (e != hipSuccess) {
         cismgpu_set_error("internal_stress: %s", hipGetErrorString(e));
+        return CISMGPU_ERR_HIP;
+    }
+    return CISMGPU_OK;
+}
+
+extern "C" int32_t cismgpu_system_internal_stress_2d(cismgpu_system* S, const double* uvel, const double* vvel,
+                                                     double* tau_xz, double* tau_yz, double* tau_xx, double* tau_yy,
+                                                     double* tau_xy, double* tau_eff) {
+    if (!S || !S->two_d || !S->asm_in.acell || S->last_approx == 0) {
+        cismgpu_set_error("internal_stress_2d: needs a 2-D system with an SSA, L1L2 or DIVA assembly");
+        return CISMGPU_ERR_ARG;
+    }
+    if ((uvel == nullptr) != (vvel == nullptr)) {
+        cismgpu_set_error("internal_stress_2d: pass both velocity components or neither");
+        return CISMGPU_ERR_ARG;
+    }
+    const int approx = S->last_approx;
+    if (!uvel && approx != CISMGPU_APPROX_SSA) {
+        cismgpu_set_error("internal_stress_2d: L1L2 and DIVA need the 3-D velocity (of l1l2_velocity or diva_velocity)");
+        return CISMGPU_ERR_ARG;
+    }
+    Dims d = S->d;   // the element loop runs over the nz levels of the inputs
+    d.nz = S->nz_levels;
+    const long long n2 = (long long)d.nvx * d.nvy, nfl = (long long)(d.nz - 1) * d.nx * d.ny;
+    const long long nv = uvel ? 2LL * d.nz * n2 : 0;
+    double* buf = nullptr;
+    CG_CHECK(hipMalloc(&buf, (6 * nfl + nv) * sizeof(double)));
+    double *tau = buf, *u3 = buf + 6 * nfl;
+    hipError_t e = hipSuccess;
+    const double *u = S->V(0), *v = S->V(1);
+    int vmode = 1;   // SSA: the 2-D velocity at every level
+    if (uvel) {
+        e = hipMemcpyAsync(u3, uvel, d.nz * n2 * sizeof(double), hipMemcpyHostToDevice, S->s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(u3 + d.nz * n2, vvel, d.nz * n2 * sizeof(double), hipMemcpyHostToDevice, S->s);
+        u = u3;
+        v = u3 + d.nz * n2;
+        vmode = 2;
+    }
+    if (e == hipSuccess) {
+        // DIVA takes the 3-D branch of compute_internal_stress (every strain rate)
+        const int branch = approx == CISMGPU_APPROX_DIVA ? 2 : approx;
+        k_internal_stress<<<nblocks(nfl, 128), 128, 0, S->s>>>(d, S->asm_in, branch, vmode, u, v, S->efvs, tau);
+        e = hipGetLastError();
+    }
+    double* out[6] = {tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff};
+    for (int q = 0; q < 6 && e == hipSuccess; ++q)
+        if (out[q]) e = hipMemcpyAsync(out[q], tau + q * nfl, nfl * sizeof(double), hipMemcpyDeviceToHost, S->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->s);
+    hipFree(buf);
+    if (e != hipSuccess) {
+        cismgpu_set_error("internal_stress_2d: %s", hipGetErrorString(e));
         return CISMGPU_ERR_HIP;
     }
     return CISMGPU_OK;

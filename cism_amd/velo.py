@@ -487,6 +487,32 @@ class SSASolver(BPSolver):
             o[inact] = 0.0
             a[il, jl] = o
 
+    def internal_stress(self, uvel3d=None, vvel3d=None):
+        """compute_internal_stress (glissade_velo_higher.F90:7046-7281) for
+        the last assembly's approximation, of the 3-D velocity (nz, nx-1,
+        ny-1) with that assembly's geometry: SSA reads the resident x at
+        every level when no velocity is given; L1L2 and DIVA take
+        :meth:`velocity_3d` of the resident x by default.  The same dict as
+        :meth:`BPSolver.internal_stress`."""
+        g = self.grid
+        if uvel3d is None and self.APPROX != 1:
+            uvel3d, vvel3d = self.velocity_3d()[:2]
+        f64 = lambda a: None if a is None else np.asfortranarray(np.array(a, dtype=np.float64))
+        u3, v3 = f64(uvel3d), f64(vvel3d)
+        for a in (u3, v3):
+            if a is not None and a.shape != (g.nz, g.nvx, g.nvy):
+                raise ValueError(f"internal_stress: the 3-D velocity is ({g.nz}, {g.nvx}, {g.nvy}), got {a.shape}")
+        names = ("tau_xz", "tau_yz", "tau_xx", "tau_yy", "tau_xy", "tau_eff")
+        out = {k: np.zeros((g.nz - 1, g.nx, g.ny), order="F") for k in names}
+        D = ctypes.POINTER(ctypes.c_double)
+        f = self._L.cismgpu_system_internal_stress_2d
+        f.argtypes = [ctypes.c_void_p] + [D] * 8
+        f.restype = ctypes.c_int32
+        ptr = lambda a: None if a is None else a.ctypes.data_as(D)
+        _lib.check(f(self.sys.handle, ptr(u3), ptr(v3), *[out[k].ctypes.data_as(D) for k in names]),
+                   "internal_stress_2d")
+        return out
+
     def assemble(self, uvel=None, vvel=None):
         """Assemble the 2-D SSA A, b for the viscosity of (uvel, vvel) (nx-1, ny-1);
         None keeps the resident x."""

@@ -444,6 +444,17 @@ int32_t cismgpu_system_residual_velocity(cismgpu_system* sys, int32_t whichresid
  * outside the active cells; any output may be NULL. */
 int32_t cismgpu_system_internal_stress(cismgpu_system* sys, double* tau_xz, double* tau_yz, double* tau_xx,
                                        double* tau_yy, double* tau_xy, double* tau_eff);
+/* compute_internal_stress (glissade_velo_higher.F90:7046-7281, called at
+ * :3779) on a 2-D system, for its last assembly's approximation: SSA
+ * (horizontal strain rates and their viscosity), L1L2 (every strain rate,
+ * the layer viscosity of the last L1L2 assembly), DIVA (the 3-D branch).
+ * uvel, vvel (nz, nx-1, ny-1): the 3-D velocity (of
+ * cismgpu_system_l1l2_velocity or cismgpu_system_diva_velocity); both NULL
+ * on an SSA system for the resident 2-D x at every level.  Outputs as
+ * cismgpu_system_internal_stress; any may be NULL. */
+int32_t cismgpu_system_internal_stress_2d(cismgpu_system* sys, const double* uvel, const double* vvel,
+                                          double* tau_xz, double* tau_yz, double* tau_xx, double* tau_yy,
+                                          double* tau_xy, double* tau_eff);
 /* compute_basal_friction_heatflx (glissade_velo_higher.F90:6831-7042, called
  * at :3795) with its cell halo (:3807), and the basal traction
  * btractx/btracty = beta u_b (:3817-3818).  which_bfric: 0 standard (2x2

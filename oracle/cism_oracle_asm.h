@@ -158,6 +158,12 @@ int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int
                                 const double* stagthck, const double* flwafact, int whichefvs, double efvs_constant,
                                 double effstrain_min, double n_glen, const double* uvel, const double* vvel,
                                 double* tau);
+int orc_compute_internal_stress_approx(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                       const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                       const double* stagusrf, const double* stagthck, int whichapprox,
+                                       const double* flwafact, int whichefvs, const double* efvs,
+                                       double efvs_constant, double effstrain_min, double n_glen, const double* uvel,
+                                       const double* vvel, double* tau);
 
 void orc_basal_friction(int nx, int ny, int nhalo, int global_bc, int which, const int32_t* active_cell,
                         const int32_t* active_vertex, const double* ubas, const double* vbas, const double* beta,

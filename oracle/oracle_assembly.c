@@ -358,14 +358,20 @@ int orc_element_matrix(const double* x, const double* y, const double* z, const 
     return ORC_OK;
 }
 
-/* compute_internal_stress (glissade_velo_higher.F90:7046-7281), BP branch:
- * tau (6 fields of (nz-1, nx, ny): xz, yz, xx, yy, xy, eff), zero outside
- * the active cells of the assembly range */
-int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int nhalo, const int32_t* active_cell,
-                                const double* xVertex, const double* yVertex, const double* stagusrf,
-                                const double* stagthck, const double* flwafact, int whichefvs, double efvs_constant,
-                                double effstrain_min, double n_glen, const double* uvel, const double* vvel,
-                                double* tau)
+/* compute_internal_stress (glissade_velo_higher.F90:7046-7281) for
+ * whichapprox (HO_APPROX_SIA 0: vertical shear only; SSA 1: horizontal
+ * strain rates only, viscosity of the SSA effective strain; BP 2 and DIVA 4:
+ * every strain rate and the 3-D viscosity; L1L2 3: every strain rate, the
+ * viscosity efvs(k,i,j) given): tau (6 fields of (nz-1, nx, ny): xz, yz,
+ * xx, yy, xy, eff), zero outside the active cells of the assembly range.
+ * The viscosity is compute_effective_viscosity's (:7285-7470) branch for
+ * whichapprox, from the eight nodes. */
+int orc_compute_internal_stress_approx(int nx, int ny, int nz, const double* sigma, int nhalo,
+                                       const int32_t* active_cell, const double* xVertex, const double* yVertex,
+                                       const double* stagusrf, const double* stagthck, int whichapprox,
+                                       const double* flwafact, int whichefvs, const double* efvs,
+                                       double efvs_constant, double effstrain_min, double n_glen, const double* uvel,
+                                       const double* vvel, double* tau)
 {
     init_tables();
     const size_t nfl = (size_t)(nz - 1) * nx * ny;
@@ -388,11 +394,26 @@ int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int
                     if (basis_derivs(x, y, z, p, dx, dy, dz, &detJ)) return ORC_ERR_SINGULAR;
                     double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0;
                     for (int n = 0; n < 8; ++n) {
-                        ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n]; uz = uz + dz[n] * u[n];
-                        vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n]; vz = vz + dz[n] * v[n];
+                        if (whichapprox != 1) { uz = uz + dz[n] * u[n]; vz = vz + dz[n] * v[n]; }
+                        if (whichapprox != 0) {
+                            ux = ux + dx[n] * u[n]; uy = uy + dy[n] * u[n];
+                            vx = vx + dx[n] * v[n]; vy = vy + dy[n] * v[n];
+                        }
                     }
-                    const double ef = effective_viscosity(whichefvs, efvs_constant, effstrain_min, n_glen, dx, dy, dz,
-                                                          u, v, flwafact[E3(k, i, j)]);
+                    double ef;
+                    const double fl = flwafact[E3(k, i, j)];
+                    if (whichapprox == 3) {
+                        ef = efvs[E3(k, i, j)];
+                    } else if (whichefvs == 2 && whichapprox == 0) {   /* :7406-7415 */
+                        const double esq = effstrain_min * effstrain_min + 0.25 * (uz * uz + vz * vz);
+                        ef = fl * pow(esq, (1.0 - n_glen) / n_glen / 2.0);
+                    } else if (whichefvs == 2 && whichapprox == 1) {   /* :7416-7429 */
+                        const double esq = effstrain_min * effstrain_min +
+                                           (ux * ux + vy * vy + ux * vy + 0.25 * ((vx + uy) * (vx + uy)));
+                        ef = fl * pow(esq, (1.0 - n_glen) / n_glen / 2.0);
+                    } else {
+                        ef = effective_viscosity(whichefvs, efvs_constant, effstrain_min, n_glen, dx, dy, dz, u, v, fl);
+                    }
                     txz = txz + ef * uz;
                     tyz = tyz + ef * vz;
                     txx = txx + 2.0 * ef * ux;
@@ -407,6 +428,18 @@ int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int
             }
         }
     return ORC_OK;
+}
+
+/* the BP branch of compute_internal_stress */
+int orc_compute_internal_stress(int nx, int ny, int nz, const double* sigma, int nhalo, const int32_t* active_cell,
+                                const double* xVertex, const double* yVertex, const double* stagusrf,
+                                const double* stagthck, const double* flwafact, int whichefvs, double efvs_constant,
+                                double effstrain_min, double n_glen, const double* uvel, const double* vvel,
+                                double* tau)
+{
+    return orc_compute_internal_stress_approx(nx, ny, nz, sigma, nhalo, active_cell, xVertex, yVertex, stagusrf,
+                                              stagthck, 2, flwafact, whichefvs, NULL, efvs_constant, effstrain_min,
+                                              n_glen, uvel, vvel, tau);
 }
 
 /* assemble_stiffness_matrix_3d (glissade_velo_higher.F90:5382-5643) with

@@ -108,6 +108,9 @@ def _declare(L):
     L.orc_compute_internal_stress.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, _D, i, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double, _D, _D, _D]
     L.orc_compute_internal_stress.restype = ctypes.c_int
+    L.orc_compute_internal_stress_approx.argtypes = [i, i, i, _D, i, _I, _D, _D, _D, _D, i, _D, i, _D,
+                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D, _D]
+    L.orc_compute_internal_stress_approx.restype = ctypes.c_int
     L.orc_basal_friction.argtypes = [i, i, i, i, i, _I, _I, _D, _D, _D, _D, _D, _D]
     L.orc_compute_grounded_fraction.argtypes = [_D, _D]
     L.orc_compute_grounded_fraction.restype = ctypes.c_int
@@ -353,21 +356,26 @@ def grounded_fraction(problem, ice, flt, land):
     return fground
 
 
-def internal_stress(problem, uvel, vvel, geo=None):
-    """compute_internal_stress (glissade_velo_higher.F90:7046-7281, BP
-    branch) of a 3-D velocity: dict of tau_xz, tau_yz, tau_xx, tau_yy,
-    tau_xy, tau_eff, each (nz-1, nx, ny)."""
+def internal_stress(problem, uvel, vvel, geo=None, approx=2, efvs=None):
+    """compute_internal_stress (glissade_velo_higher.F90:7046-7281) of a
+    3-D velocity (nz, nx-1, ny-1) for whichapprox ``approx`` (0 SIA, 1 SSA,
+    2 BP, 3 L1L2 with the layer viscosity ``efvs`` (nz-1, nx, ny), 4 DIVA):
+    dict of tau_xz, tau_yz, tau_xx, tau_yy, tau_xy, tau_eff, each
+    (nz-1, nx, ny)."""
     p = problem
     g = p.grid
     geo = geo or geometry(p)
     xV, yV = p.vertex_coords()
     nfl = (g.nz - 1) * g.nx * g.ny
     tau = np.zeros(6 * nfl)
-    st = lib().orc_compute_internal_stress(g.nx, g.ny, g.nz, dp(np.ascontiguousarray(p.sigma, dtype=np.float64)), 2,
-                                           ip(geo["active_cell"]), dp(xV), dp(yV), dp(geo["stagusrf"]),
-                                           dp(geo["stagthck"]), dp(geo["flwafact"]), p.whichefvs, p.efvs_constant,
-                                           p.effstrain_min, p.n_glen, dp(np.asfortranarray(uvel, dtype=np.float64)),
-                                           dp(np.asfortranarray(vvel, dtype=np.float64)), dp(tau))
+    if approx == 3 and efvs is None:
+        raise ValueError("internal_stress: L1L2 needs the layer viscosity efvs")
+    ef = None if efvs is None else np.asfortranarray(efvs, dtype=np.float64)
+    st = lib().orc_compute_internal_stress_approx(
+        g.nx, g.ny, g.nz, dp(np.ascontiguousarray(p.sigma, dtype=np.float64)), 2, ip(geo["active_cell"]), dp(xV),
+        dp(yV), dp(geo["stagusrf"]), dp(geo["stagthck"]), int(approx), dp(geo["flwafact"]), p.whichefvs,
+        None if ef is None else dp(ef), p.efvs_constant, p.effstrain_min, p.n_glen,
+        dp(np.asfortranarray(uvel, dtype=np.float64)), dp(np.asfortranarray(vvel, dtype=np.float64)), dp(tau))
     if st:
         raise RuntimeError(f"oracle internal stress failed with status {st}")
     names = ("tau_xz", "tau_yz", "tau_xx", "tau_yy", "tau_xy", "tau_eff")

@@ -61,3 +61,39 @@ def test_pure_strain():
     np.testing.assert_allclose(tau["tau_eff"][m], two_eta_e[m], rtol=1e-12)
     for k in ("tau_xz", "tau_yz", "tau_xy"):
         assert np.abs(tau[k][m]).max() < 1e-10 * two_eta_e[m].max()
+
+
+def _shear_plus_strain(p, geo, a, e):
+    x, y, z = _nodes(p, geo)
+    return np.asfortranarray(a * z + e * x), np.asfortranarray(-e * y)
+
+
+def test_approximation_branches():
+    """u = a z + e x, v = -e y on the slab, whose strain rates are exact:
+    BP (and DIVA) keep both, with eta of e^2 + a^2/4; SSA drops the vertical
+    shear from the strain rates and the viscosity; SIA keeps only it; L1L2
+    uses the viscosity it is given."""
+    p = sy.slab(n_cells=6, nz=5)
+    geo = po.geometry(p)
+    a, e = 0.02, 1.0e-3
+    u, v = _shear_plus_strain(p, geo, a, e)
+    m = _active(p, geo)
+    for approx, esq, has_xz, has_xx in ((2, e * e + 0.25 * a * a, True, True), (4, e * e + 0.25 * a * a, True, True),
+                                        (1, e * e, False, True), (0, 0.25 * a * a, True, False)):
+        tau = po.internal_stress(p, u, v, geo, approx=approx)
+        eta = _eta(p, geo, esq)
+        want_xz = eta * a if has_xz else 0.0 * eta
+        want_xx = 2.0 * eta * e if has_xx else 0.0 * eta
+        np.testing.assert_allclose(tau["tau_xz"][m], want_xz[m], rtol=1e-12, atol=1e-10 * np.abs(eta * a).max())
+        np.testing.assert_allclose(tau["tau_xx"][m], want_xx[m], rtol=1e-12, atol=1e-10 * np.abs(eta * e).max())
+        np.testing.assert_allclose(tau["tau_yy"][m], -want_xx[m], rtol=1e-12, atol=1e-10 * np.abs(eta * e).max())
+        np.testing.assert_allclose(tau["tau_eff"][m], np.sqrt(want_xx ** 2 + want_xz ** 2)[m], rtol=1e-12)
+    # L1L2: the viscosity is the layer field passed in, read per element
+    g = p.grid
+    rng = np.random.default_rng(2)
+    efvs = np.asfortranarray(rng.uniform(1e5, 1e7, (g.nz - 1, g.nx, g.ny)))
+    tau = po.internal_stress(p, u, v, geo, approx=3, efvs=efvs)
+    np.testing.assert_allclose(tau["tau_xz"][m], (efvs * a)[m], rtol=1e-12)
+    np.testing.assert_allclose(tau["tau_xx"][m], (2.0 * efvs * e)[m], rtol=1e-12)
+    with pytest.raises(ValueError, match="efvs"):
+        po.internal_stress(p, u, v, geo, approx=3)
