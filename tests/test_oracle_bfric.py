@@ -146,3 +146,25 @@ def test_traction_is_beta_times_velocity():
     _, bx, by = po.basal_friction(g, 1, ac, av, u, v, b)
     np.testing.assert_array_equal(bx, b * u)
     np.testing.assert_array_equal(by, b * v)
+
+
+def test_no_ice_and_no_penetration_halos():
+    """global_bc 3 (no_ice) zeroes three rings of cells after the friction
+    is computed; global_bc 2 (no_penetration) wraps like periodic cells."""
+    rng = np.random.default_rng(13)
+    for bc in (2, 3):
+        g = _grid(bc=bc)
+        ac, av = _all_active(g)
+        f = lambda: np.asfortranarray(rng.uniform(0.5, 2.0, (g.nx - 1, g.ny - 1)))
+        u, v, b = f(), f(), f()
+        for which in (0, 1):
+            bf, _, _ = po.basal_friction(g, which, ac, av, u, v, b)
+            if bc == 3:
+                m = np.zeros((g.nx, g.ny), bool)
+                m[3:g.nx - 3, 3:g.ny - 3] = True
+                assert (bf[~m] == 0).all() and (bf[m] > 0).all()
+                ref, _, _ = po.basal_friction(_grid(bc=1), which, ac, av, u, v, b)
+                np.testing.assert_array_equal(bf[m], ref[m])
+            else:
+                ref, _, _ = po.basal_friction(_grid(bc=0), which, ac, av, u, v, b)
+                np.testing.assert_array_equal(bf, ref)
