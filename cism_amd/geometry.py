@@ -241,14 +241,22 @@ def grounded_fraction(ice_mask, floating_mask, land_mask):
 RHOO = 1028.0   # ocean density (glimmer_physcon.F90:68)
 
 
-def flotation_function(thck, topg, eus, ice_mask, land_mask, which=2, global_bc=3):
+def flotation_function(thck, topg, eus, ice_mask, land_mask, which=2, global_bc=3, topg_raised=None):
     """Flotation function at the cells (glissade_grounding_line.F90:237-392):
     0 Pattyn (-rho_o (b - eus) / (rho_i H) - 1), 1 inverse Pattyn
     (1 - rho_i H / (-rho_o (b - eus)), at least -10), both -10 on land and 0
     in ice-free ocean, which then takes the smallest value of its ice or land
     edge neighbours (else corner neighbours) and the cell halo; 2 linear
     (-(b - eus) - rho_i/rho_o H, at least 1e-4 in magnitude; on land
-    -max(b - eus, 50)).  Positive where the ice floats."""
+    -max(b - eus, 50)); 3 the same with the raised bed topg_raised in place
+    of b (:299-326; the reference stops without it).  Positive where the ice
+    floats."""
+    if which not in (0, 1, 2, 3):
+        raise ValueError(f"which_ho_flotation_function {which}: 0, 1, 2 or 3")
+    if which == 3:
+        if topg_raised is None:
+            raise ValueError("which_ho_flotation_function 3 needs topg_raised")
+        topg = topg_raised
     H = np.asarray(thck, dtype=np.float64)
     d = np.asarray(topg, dtype=np.float64) - eus
     ice, land = ice_mask == 1, land_mask == 1
@@ -353,7 +361,7 @@ def grounded_area(f1, f2, f3, f4):
 
 
 def grounded_fraction_options(thck, topg, eus, ice_mask, floating_mask, land_mask, which_ground=0,
-                              which_flotation=2, which_no_glp=0, global_bc=3):
+                              which_flotation=2, which_no_glp=0, global_bc=3, topg_raised=None):
     """f_ground at the vertices for which_ho_ground 0 (no GLP: the
     grounded-neighbour rule, or with which_ho_fground_no_glp = 1 the sign of
     the staggered flotation function), 1 (GLP for basal friction: the
@@ -363,7 +371,7 @@ def grounded_fraction_options(thck, topg, eus, ice_mask, floating_mask, land_mas
     (glissade_grounding_line.F90:404-618).  Returns (f_ground, f_flotation)."""
     if which_ground not in (0, 1, 2):
         raise ValueError("which_ho_ground 0, 1 or 2")
-    ff = flotation_function(thck, topg, eus, ice_mask, land_mask, which_flotation, global_bc)
+    ff = flotation_function(thck, topg, eus, ice_mask, land_mask, which_flotation, global_bc, topg_raised)
     ice = ice_mask == 1
     land = land_mask == 1
     vmask = (ice[:-1, 1:] | ice[1:, 1:] | ice[:-1, :-1] | ice[1:, :-1] |
@@ -399,7 +407,8 @@ def problem_f_ground(problem, ice_mask, floating_mask, land_mask):
     p = problem
     fg, _ = grounded_fraction_options(p.thck, p.topg, p.eus, ice_mask, floating_mask, land_mask,
                                       getattr(p, "which_ho_ground", 0), getattr(p, "which_ho_flotation_function", 2),
-                                      getattr(p, "which_ho_fground_no_glp", 0), p.grid.global_bc)
+                                      getattr(p, "which_ho_fground_no_glp", 0), p.grid.global_bc,
+                                      getattr(p, "topg_raised", None))
     return fg
 
 

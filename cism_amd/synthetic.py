@@ -54,6 +54,7 @@ class Problem:
     which_ho_ground: int = 0     # glide_types.F90:378-380, 1090: 0 no GLP, 1 GLP for basal friction
     which_ho_flotation_function: int = 2   # :389-392, 1114: 0 Pattyn, 1 inverse Pattyn, 2 linear
     which_ho_fground_no_glp: int = 0       # :382-383, 1099: 0 grounded neighbour, 1 flotation function
+    topg_raised: np.ndarray | None = None  # (nx, ny) m: the bed of flotation function 3 (glissade.F90:3330)
     friction: object = None      # basal.Friction: beta from the basal velocity in every Picard iteration
                                  # (glissade_calcbeta); None: the fixed beta of whichbabc 0, 4, 5 or no-slip
     meta: dict = field(default_factory=dict)

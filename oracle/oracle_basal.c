@@ -277,7 +277,9 @@ int orc_compute_grounded_fraction(const double* fq, double* f_ground)
 
 /* flotation function at the cells (:237-392); ice-free ocean cells of
  * options 0 and 1 take the smallest value of their ice or land edge
- * neighbours (else corner neighbours), then the cell halo */
+ * neighbours (else corner neighbours), then the cell halo.  Option 3
+ * (:299-326) is option 2's formula on the raised bed, which the caller
+ * passes as topg. */
 void orc_flotation_function(int nx, int ny, int global_bc, int which, const double* thck, const double* topg,
                             double eus, const int32_t* ice_mask, const int32_t* land_mask, double* ff)
 {
@@ -336,7 +338,7 @@ int orc_grounded_fraction(int nx, int ny, int global_bc, int which_ground, int w
                           const double* thck, const double* topg, double eus, const int32_t* ice_mask,
                           const int32_t* floating_mask, const int32_t* land_mask, double* ff, double* f_ground)
 {
-    if (which_ground < 0 || which_ground > 2 || which_flotation < 0 || which_flotation > 2) return -1;
+    if (which_ground < 0 || which_ground > 2 || which_flotation < 0 || which_flotation > 3) return -1;
     orc_flotation_function(nx, ny, global_bc, which_flotation, thck, topg, eus, ice_mask, land_mask, ff);
     for (int j = 1; j <= ny - 1; ++j)
         for (int i = 1; i <= nx - 1; ++i) {

@@ -335,7 +335,7 @@ def geometry(problem):
 
 def grounded_fraction(problem, ice, flt, land):
     """f_ground at the vertices for the problem's which_ho_ground (0, 1),
-    which_ho_flotation_function (0-2) and which_ho_fground_no_glp (0, 1)
+    which_ho_flotation_function (0-3) and which_ho_fground_no_glp (0, 1)
     (glissade_grounded_fraction, glissade_grounding_line.F90:59-674); the
     defaults (0, 2, 0) are orc_grounded_fraction_no_glp."""
     p = problem
@@ -343,13 +343,17 @@ def grounded_fraction(problem, ice, flt, land):
     wg = getattr(p, "which_ho_ground", 0)
     wf = getattr(p, "which_ho_flotation_function", 2)
     wn = getattr(p, "which_ho_fground_no_glp", 0)
+    if wf == 3 and getattr(p, "topg_raised", None) is None:
+        # the reference computes the flotation function for every option first
+        raise ValueError("which_ho_flotation_function 3 needs topg_raised")
     fground = np.zeros((g.nx - 1, g.ny - 1), order="F")
     if wg == 0 and wn == 0:
         lib().orc_grounded_fraction_no_glp(g.nx, g.ny, ip(ice), ip(flt), ip(land), dp(fground))
         return fground
     ff = np.zeros((g.nx, g.ny), order="F")
     f64 = lambda a: np.asfortranarray(np.array(a, dtype=np.float64))
-    st = lib().orc_grounded_fraction(g.nx, g.ny, g.global_bc, wg, wf, wn, dp(f64(p.thck)), dp(f64(p.topg)), p.eus,
+    bed = p.topg_raised if wf == 3 else p.topg   # option 3: option 2 on the raised bed (:299-326)
+    st = lib().orc_grounded_fraction(g.nx, g.ny, g.global_bc, wg, wf, wn, dp(f64(p.thck)), dp(f64(bed)), p.eus,
                                      ip(ice), ip(flt), ip(land), dp(ff), dp(fground))
     if st:
         raise ValueError(f"grounded fraction: status {st}")

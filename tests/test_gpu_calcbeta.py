@@ -300,6 +300,10 @@ def _marine_glp(flotation, friction=False, ground=1):
     p = synthetic.marine_sheet(40, nz=7, dx=2000.0, seed=3, h_max=1000.0, h_shelf=200.0)
     p.which_ho_ground = ground                  # GLP over the staggered cell (1) or its quadrants (2)
     p.which_ho_flotation_function = flotation
+    if flotation == 3:   # a raised bed (pinning points) under every fourth cell
+        raised = np.array(p.topg, dtype=np.float64)
+        raised[::4, ::4] += 60.0
+        p.topg_raised = np.asfortranarray(raised)
     if friction:
         g = p.grid
         N = np.asfortranarray(910.0 * 9.81 * synthetic_stag(p.thck) * 0.2)
@@ -308,7 +312,8 @@ def _marine_glp(flotation, friction=False, ground=1):
     return p
 
 
-@pytest.mark.parametrize("flotation,friction,ground", [(2, False, 1), (0, False, 1), (2, True, 1), (2, False, 2)])
+@pytest.mark.parametrize("flotation,friction,ground", [(2, False, 1), (0, False, 1), (2, True, 1), (2, False, 2),
+                                                      (3, False, 1)])
 def test_marine_picard_with_grounding_line_parameterisation(require_gpu, flotation, friction, ground):
     """Marine sheet with the GLP (fractional f_ground at the grounding line,
     host geometry = oracle to 1e-13): accelerated BP Picard (fixed beta, or

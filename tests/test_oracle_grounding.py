@@ -142,3 +142,43 @@ def test_quadrant_option_is_the_mean_of_the_quadrant_areas():
                  [ctr, (ne + se) / 2, ne, (ne + nw) / 2], [(sw + nw) / 2, ctr, (nw + ne) / 2, nw]]
         assert f2[i, j] == pytest.approx(np.mean([_area(q) for q in quads]), abs=1e-7)
 
+
+
+def test_raised_topography_option():
+    """which_ho_flotation_function 3 is the linear function of the raised bed
+    topg_raised (glissade_grounding_line.F90:299-326): raising the bed of an
+    afloat cell by its flotation margin plus 10 m grounds it by 10 m; the
+    product follows the oracle for every which_ho_ground; without the field
+    both stop."""
+    p = sy.marine_sheet(40, nz=5, dx=2000.0, seed=3, h_max=1000.0, h_shelf=200.0)
+    ice, flt, ocn, land = ge.get_masks(p.thck, p.topg, p.eus, p.thklim)
+    f2 = ge.flotation_function(p.thck, p.topg, p.eus, ice, land, 2, p.grid.global_bc)
+    afloat = np.argwhere((ice == 1) & (f2 > 1.0))[0]
+    raised = np.array(p.topg, dtype=np.float64)
+    raised[tuple(afloat)] += f2[tuple(afloat)] + 10.0
+    f3 = ge.flotation_function(p.thck, p.topg, p.eus, ice, land, 3, p.grid.global_bc, topg_raised=raised)
+    assert f3[tuple(afloat)] == pytest.approx(-10.0, abs=1e-9)
+    other = np.ones(raised.shape, bool)
+    other[tuple(afloat)] = False
+    np.testing.assert_array_equal(f3[other], f2[other])
+    I = lambda a: np.asfortranarray(a.astype(np.int32))
+    p.which_ho_flotation_function = 3
+    with pytest.raises(ValueError, match="topg_raised"):
+        ge.problem_f_ground(p, ice, flt, land)
+    with pytest.raises(ValueError, match="topg_raised"):
+        po.grounded_fraction(p, I(ice), I(flt), I(land))
+    p.topg_raised = raised
+    for wg, wn in ((0, 1), (1, 0), (2, 0)):
+        p.which_ho_ground, p.which_ho_fground_no_glp = wg, wn
+        fo = po.grounded_fraction(p, I(ice), I(flt), I(land))
+        fp = ge.problem_f_ground(p, ice, flt, land)
+        np.testing.assert_allclose(fp, fo, rtol=0, atol=1e-13)
+    # the raised cell's four vertices gain grounded area under the GLP
+    p.which_ho_ground, p.which_ho_fground_no_glp = 1, 0
+    p.topg_raised = None
+    p.which_ho_flotation_function = 2
+    base = ge.problem_f_ground(p, ice, flt, land)
+    p.which_ho_flotation_function, p.topg_raised = 3, raised
+    up = ge.problem_f_ground(p, ice, flt, land)
+    i, j = afloat
+    assert (up[i - 1:i + 1, j - 1:j + 1] > base[i - 1:i + 1, j - 1:j + 1]).all()
